@@ -1,0 +1,252 @@
+"""ctypes binding of the CPU oracle (oracle/wc_oracle.c) and of oracle/_ref.
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only as the checker.  The product
+(wacommplusplus_amd) never imports it.
+
+Structures mirror the reference's config_data (Config.hpp:36-72) and
+particle_data (Particle.hpp:31-41) field for field.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class CalibrationRegion(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "west", "south", "east", "north", "windErrorStdDev", "windErrorComponentCorrelation",
+        "windErrorSpatialScale", "windErrorTemporalScale")]
+
+
+class Config(C.Structure):
+    """config_data (Config.hpp:36-72)."""
+    _fields_ = [
+        ("random", C.c_bool), ("randomSources", C.c_bool),
+        ("deltat", C.c_double), ("dti", C.c_double), ("survprob", C.c_double), ("tau0", C.c_double),
+        ("sv", C.c_double), ("crid", C.c_double), ("sigma", C.c_double), ("shoreLimit", C.c_double),
+        ("upperClosure", C.c_int), ("lowerClosure", C.c_int), ("horizontalClosure", C.c_int),
+        ("randomSeed", C.c_uint64), ("trackingDirection", C.c_int), ("backwardDiffusion", C.c_int),
+        ("integrator", C.c_int), ("restartCheckpoint", C.c_double), ("driftModel", C.c_int),
+        ("leewayCoefficientEnsemble", C.c_bool), ("leewayResidualCorrelation", C.c_double),
+        ("leewayRandomSide", C.c_bool), ("leewayRightSideProbability", C.c_double),
+        ("leewayJibeProbabilityHourly", C.c_double), ("windErrorStdDev", C.c_double),
+        ("windErrorComponentCorrelation", C.c_double), ("windErrorSpatialScale", C.c_double),
+        ("windErrorTemporalScale", C.c_double), ("observationalCalibrationRegionCount", C.c_uint32),
+        ("observationalCalibrationRegions", CalibrationRegion * 16),
+        ("driftObjectType", C.c_uint16), ("driftSide", C.c_int8), ("hasWind", C.c_bool),
+        ("windU10", C.c_double), ("windV10", C.c_double),
+    ]
+
+
+CONSTRAINT, KILL, REFLECTION = 1, 2, 3     # Config.hpp:201-203
+EULER, MIDPOINT = 1, 2                     # Config.hpp:206-207
+
+
+def default_config(**kw) -> Config:
+    """Config::setDefault values that the move reads (Config.cpp:46-202)."""
+    c = Config()
+    c.random = True
+    c.randomSources = True
+    c.deltat = 3600.0
+    c.dti = 30.0
+    c.survprob = 1.0e-4
+    c.tau0 = 86400.0
+    c.sv = 0.0
+    c.crid = 1.0
+    c.sigma = 3.46
+    c.shoreLimit = 0.25
+    c.upperClosure = CONSTRAINT
+    c.lowerClosure = KILL
+    c.horizontalClosure = REFLECTION
+    c.randomSeed = 5489
+    c.trackingDirection = 1
+    c.backwardDiffusion = 0
+    c.integrator = MIDPOINT
+    c.restartCheckpoint = math.nan
+    c.driftModel = 0
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+# numpy view of particle_data (64 bytes, Particle.hpp:31-41)
+PARTICLE_DTYPE = np.dtype({
+    "names": ["id", "k", "j", "i", "health", "age", "time", "driftObjectType", "driftSide"],
+    "formats": [np.uint64, np.float64, np.float64, np.float64, np.float64, np.float64, np.float64,
+                np.uint16, np.int8],
+    "offsets": [0, 8, 16, 24, 32, 40, 48, 56, 58],
+    "itemsize": 64,
+})
+
+
+def make_particles(n, k=0.0, j=0.0, i=0.0, time=0.0, health=1.0, age=0.0, first_id=0):
+    p = np.zeros(n, dtype=PARTICLE_DTYPE)
+    p["id"] = np.arange(first_id, first_id + n, dtype=np.uint64)
+    p["k"], p["j"], p["i"] = k, j, i
+    p["health"], p["age"], p["time"] = health, age, time
+    return p
+
+
+class _Grid(C.Structure):
+    _fields_ = [("s_w", C.c_int), ("s_rho", C.c_int), ("eta", C.c_int), ("xi", C.c_int),
+                ("lonRad", C.c_void_p), ("latRad", C.c_void_p), ("h", C.c_void_p),
+                ("depthIntervals", C.c_void_p)]
+
+
+class _Record(C.Structure):
+    _fields_ = [("zeta", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("akt", C.c_void_p)]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _load(path):
+    lib = C.CDLL(path)
+    lib.wco_mix.restype = C.c_uint64
+    lib.wco_mix.argtypes = [C.c_uint64]
+    for name in ("wco_uniform", "wco_normal"):
+        f = getattr(lib, name)
+        f.restype = C.c_double
+        f.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_uint64, C.c_uint64]
+    lib.wco_restart_elapsed.restype = C.c_double
+    lib.wco_restart_elapsed.argtypes = [C.c_double] * 3
+    lib.wco_reflect_cell.restype = C.c_double
+    lib.wco_reflect_cell.argtypes = [C.c_double, C.c_int, C.c_double, C.c_int]
+    lib.wco_reflect_domain.restype = C.c_double
+    lib.wco_reflect_domain.argtypes = [C.c_double, C.c_double]
+    lib.wco_move.argtypes = [C.POINTER(Config), C.POINTER(_Grid), C.c_double, C.c_double,
+                             C.POINTER(_Record), C.POINTER(_Record), C.c_void_p]
+    lib.wco_move_all.argtypes = [C.POINTER(Config), C.POINTER(_Grid), C.c_double, C.c_double,
+                                 C.POINTER(_Record), C.POINTER(_Record), C.c_void_p, C.c_long, C.c_int]
+    lib.wco_grid_count.argtypes = [C.c_void_p, C.c_long, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.wco_emission_times.restype = C.c_long
+    lib.wco_emission_times.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_void_p, C.c_long]
+    lib.wco_cell_metric.argtypes = [C.POINTER(_Grid), C.c_int, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+    return lib
+
+
+_LIBS = {}
+
+
+def lib(plant: int = 0):
+    """The oracle library; plant>0 selects a deliberately wrong build."""
+    if plant not in _LIBS:
+        path = os.path.join(HERE, "liboracle.so") if plant == 0 else \
+            os.path.join(HERE, "_plant", f"liboracle_plant{plant}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        _LIBS[plant] = _load(path)
+    return _LIBS[plant]
+
+
+def ref_lib():
+    """oracle/_ref/libwcref.so (reference headers compiled in place) or None."""
+    path = os.path.join(HERE, "_ref", "libwcref.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ref_mix.restype = C.c_uint64
+    L.ref_mix.argtypes = [C.c_uint64]
+    for name in ("ref_uniform", "ref_normal"):
+        f = getattr(L, name)
+        f.restype = C.c_double
+        f.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_uint64, C.c_uint64]
+    L.ref_restart_elapsed.restype = C.c_double
+    L.ref_restart_elapsed.argtypes = [C.c_double] * 3
+    L.ref_reflect_cell.restype = C.c_double
+    L.ref_reflect_cell.argtypes = [C.c_double, C.c_int, C.c_double, C.c_int]
+    L.ref_reflect_domain.restype = C.c_double
+    L.ref_reflect_domain.argtypes = [C.c_double, C.c_double]
+    L.ref_scheme.restype = C.c_char_p
+    L.ref_emission_times.restype = C.c_long
+    L.ref_emission_times.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_void_p, C.c_long]
+    L.ref_source_position_offset.restype = C.c_double
+    L.ref_source_position_offset.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    return L
+
+
+class Window:
+    """A forcing window held on the host: static grid + T records (FP32 fields).
+
+    Layout (NumericalHelpers.hpp:16-18, cuda/kernel.cu:63-69):
+      zeta[T][eta][xi], u,v[T][s_rho][eta][xi], w,akt[T][s_w][eta][xi];
+      h, lonRad, latRad[eta][xi]; depthIntervals[s_w-1] (entry kI+s_w-2).
+    """
+
+    def __init__(self, ocean_time, lonRad, latRad, h, depthIntervals, zeta, u, v, w, akt, mask=None):
+        self.ocean_time = np.ascontiguousarray(ocean_time, dtype=np.float64)
+        self.lonRad = np.ascontiguousarray(lonRad, dtype=np.float64)
+        self.latRad = np.ascontiguousarray(latRad, dtype=np.float64)
+        self.h = np.ascontiguousarray(h, dtype=np.float64)
+        self.depthIntervals = np.ascontiguousarray(depthIntervals, dtype=np.float64)
+        self.zeta = np.ascontiguousarray(zeta, dtype=np.float32)
+        self.u = np.ascontiguousarray(u, dtype=np.float32)
+        self.v = np.ascontiguousarray(v, dtype=np.float32)
+        self.w = np.ascontiguousarray(w, dtype=np.float32)
+        self.akt = np.ascontiguousarray(akt, dtype=np.float32)
+        self.mask = None if mask is None else np.ascontiguousarray(mask, dtype=np.float64)
+        self.T = self.zeta.shape[0]
+        self.eta, self.xi = self.h.shape
+        self.s_rho = self.u.shape[1]
+        self.s_w = self.w.shape[1]
+        assert self.depthIntervals.shape[0] >= self.s_w - 1
+        assert self.u.shape == (self.T, self.s_rho, self.eta, self.xi) == self.v.shape
+        assert self.w.shape == (self.T, self.s_w, self.eta, self.xi) == self.akt.shape
+        self._grid = _Grid(self.s_w, self.s_rho, self.eta, self.xi, _ptr(self.lonRad).value,
+                           _ptr(self.latRad).value, _ptr(self.h).value, _ptr(self.depthIntervals).value)
+
+    def record(self, t):
+        return _Record(_ptr(self.zeta[t]).value, _ptr(self.u[t]).value, _ptr(self.v[t]).value,
+                       _ptr(self.w[t]).value, _ptr(self.akt[t]).value)
+
+    def interval(self, ocean_time_idx, direction, deltat=3600.0):
+        """(start, end, record index pair) as Particle.cpp:110-115,183-184."""
+        nxt = ocean_time_idx + direction
+        start = float(self.ocean_time[ocean_time_idx])
+        end = float(self.ocean_time[nxt]) if 0 <= nxt < self.T else start + direction * deltat
+        if nxt < 0 or nxt >= self.T:
+            nxt = ocean_time_idx
+        return start, end, ocean_time_idx, nxt
+
+    def cell_metric(self, j, i, plant=0):
+        dx, dy = C.c_double(), C.c_double()
+        lib(plant).wco_cell_metric(C.byref(self._grid), j, i, C.byref(dx), C.byref(dy))
+        return dx.value, dy.value
+
+
+def move(window: Window, cfg: Config, ocean_time_idx: int, particles: np.ndarray, nthreads: int = 0,
+         plant: int = 0):
+    """One forcing interval of Particle::move over every particle, in place."""
+    assert particles.dtype == PARTICLE_DTYPE and particles.flags.c_contiguous
+    s, e, t0, t1 = window.interval(ocean_time_idx, cfg.trackingDirection, cfg.deltat)
+    r0, r1 = window.record(t0), window.record(t1)
+    lib(plant).wco_move_all(C.byref(cfg), C.byref(window._grid), s, e, C.byref(r0), C.byref(r1),
+                            _ptr(particles), len(particles), nthreads)
+    return particles
+
+
+def grid_count(particles: np.ndarray, s_rho: int, eta: int, xi: int, plant: int = 0) -> np.ndarray:
+    counts = np.zeros((s_rho, eta, xi), dtype=np.uint32)
+    lib(plant).wco_grid_count(_ptr(particles), len(particles), s_rho, eta, xi, _ptr(counts))
+    return counts
+
+
+def emission_times(mode, value, source_start, source_end, interval_start, interval_end, cap=1 << 20,
+                   use_ref=False):
+    buf = np.zeros(cap, dtype=np.float64)
+    L = ref_lib() if use_ref else lib()
+    f = L.ref_emission_times if use_ref else L.wco_emission_times
+    n = f(mode, value, source_start, source_end, interval_start, interval_end, _ptr(buf), cap)
+    if n < 0:
+        return n
+    return buf[:n].copy()

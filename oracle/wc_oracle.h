@@ -1,0 +1,168 @@
+/*
+ * wc_oracle.h -- CPU restatement of WaComM++'s particle move and concentration
+ * gridding, used ONLY as the checker for the HIP implementation.
+ *
+ * TEST INFRASTRUCTURE.  Nothing in the product (the wacommplusplus_amd package
+ * and its libwacomm_hip.so) may include, link or call this code.  Only the test
+ * suite, the smoke check and the CPU-baseline leg of the benchmark use it.
+ *
+ * Every function restates the reference algorithm in plain C99 (FP64, IEEE,
+ * compiled with -ffp-contract=off so no fused multiply-add changes a rounding
+ * point) and cites the file:line of WaComM++ that it follows.
+ *
+ * Pinning (what proves this restatement is the reference algorithm):
+ *   - wco_uniform / wco_normal / wco_mix: golden values of
+ *     tests/NumericalHelpersTest.cpp:87-89, and bit-equality with
+ *     NumericalHelpers.hpp compiled from the reference sources (oracle/_ref).
+ *   - wco_reflect_cell / wco_reflect_domain / wco_restart_elapsed: the
+ *     assertions of tests/NumericalHelpersTest.cpp:25-55 and bit-equality with
+ *     oracle/_ref.
+ *   - wco_emission_times: tests/SourceEmissionTest.cpp cases and bit-equality
+ *     with SourceEmission.hpp compiled from the reference sources (oracle/_ref).
+ *   - wco_move (Particle::move): Particle.cpp needs NetCDF and log4cplus
+ *     headers that this image lacks, so it cannot be built here.  It is pinned by
+ *     the closed forms and invariants of tests/ParticleMoveTest.cpp (exact
+ *     time-linear integration, late emission, backward return, vertical w,
+ *     Euler/midpoint convergence orders against i0*exp(bt/dx), restart
+ *     equivalence, Euler fallback identity), the analytic uniform-current
+ *     trajectory of tests/ApplicationEndToEndTest.cpp:252-257, the diffusion
+ *     variance sigma_k^2 per step of docs/model.md:204-210, and closure
+ *     invariants.  Tests also plant mistakes in this oracle (WCO_PLANT) to
+ *     show that each pin catches a plausible error.
+ *   - wco_grid_count (Concentration::evaluate): tests/ConcentrationTest.cpp
+ *     and brute force on tiny inputs.
+ */
+#ifndef WC_ORACLE_H
+#define WC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field-for-field the reference config_data (Config.hpp:25-72), so the same
+ * bytes can be handed to the oracle and to the HIP library. */
+typedef struct {
+    double west, south, east, north;
+    double windErrorStdDev, windErrorComponentCorrelation;
+    double windErrorSpatialScale, windErrorTemporalScale;
+} wco_calibration_region;
+
+typedef struct {
+    _Bool random;
+    _Bool randomSources;
+    double deltat;
+    double dti;
+    double survprob;
+    double tau0;
+    double sv;
+    double crid;
+    double sigma;
+    double shoreLimit;
+    int upperClosure;
+    int lowerClosure;
+    int horizontalClosure;
+    uint64_t randomSeed;
+    int trackingDirection;
+    int backwardDiffusion;
+    int integrator;
+    double restartCheckpoint;
+    int driftModel;
+    _Bool leewayCoefficientEnsemble;
+    double leewayResidualCorrelation;
+    _Bool leewayRandomSide;
+    double leewayRightSideProbability;
+    double leewayJibeProbabilityHourly;
+    double windErrorStdDev;
+    double windErrorComponentCorrelation;
+    double windErrorSpatialScale;
+    double windErrorTemporalScale;
+    uint32_t observationalCalibrationRegionCount;
+    wco_calibration_region observationalCalibrationRegions[16];
+    uint16_t driftObjectType;
+    int8_t driftSide;
+    _Bool hasWind;
+    double windU10;
+    double windV10;
+} wco_config;
+
+/* Field-for-field the reference particle_data (Particle.hpp:31-41). */
+typedef struct {
+    uint64_t id;
+    double k, j, i;
+    double health, age, time;
+    uint16_t driftObjectType;
+    int8_t driftSide;
+} wco_particle;
+
+enum { WCO_CONSTRAINT = 1, WCO_KILL = 2, WCO_REFLECTION = 3 };   /* Config.hpp:201-203 */
+enum { WCO_EULER = 1, WCO_MIDPOINT = 2 };                        /* Config.hpp:206-207 */
+
+/* Keyed RNG, scheme ordered-splitmix64-v2 (NumericalHelpers.hpp:84-120). */
+uint64_t wco_mix(uint64_t value);
+double wco_uniform(uint64_t seed, uint64_t particle, int64_t interval, uint64_t substep, uint64_t component);
+double wco_normal(uint64_t seed, uint64_t particle, int64_t interval, uint64_t substep, uint64_t component);
+
+/* Numerical helpers (NumericalHelpers.hpp:45-74). */
+double wco_restart_elapsed(double intervalStart, double intervalEnd, double checkpoint);
+double wco_reflect_cell(double oldCoordinate, int oldCell, double candidate, int candidateCell);
+double wco_reflect_domain(double coordinate, double maximum);
+
+/* Static grid of one forcing window (OceanModelAdapter.hpp:20-34).
+ * lonRad/latRad/h: [eta][xi].  depthIntervals: s_w-1 values, entry kI+s_w-2
+ * holds depthIntervals(kI) of the reference (ROMSAdapter.cpp:233-235). */
+typedef struct {
+    int s_w, s_rho, eta, xi;
+    const double *lonRad, *latRad, *h, *depthIntervals;
+} wco_grid;
+
+/* One interval's pair of forcing records.  zeta: [eta][xi]; u, v: [s_rho][eta][xi];
+ * w, akt: [s_w][eta][xi]; logical level k is stored at k+levels-1
+ * (NumericalHelpers.hpp:16-18).  rec0 is the record at ocean_time_idx, rec1 the
+ * record at ocean_time_idx+trackingDirection (or rec0 again when that index is
+ * outside the window, Particle.cpp:183-184). */
+typedef struct {
+    const float *zeta, *u, *v, *w, *akt;
+} wco_record;
+
+/* Particle::move for one particle over one forcing interval
+ * (Particle.cpp:71-824).  intervalStart/intervalEnd are the physical times of
+ * the interval (Particle.cpp:110-115). */
+void wco_move(const wco_config *cfg, const wco_grid *grid, double intervalStart, double intervalEnd,
+              const wco_record *rec0, const wco_record *rec1, wco_particle *p);
+
+/* The OpenMP particle loop of Wacomm::run (Wacomm.cpp:501-525):
+ * schedule(dynamic,64) over wco_move.  nthreads<=0 keeps the OpenMP default. */
+void wco_move_all(const wco_config *cfg, const wco_grid *grid, double intervalStart, double intervalEnd,
+                  const wco_record *rec0, const wco_record *rec1, wco_particle *ps, long n, int nthreads);
+
+/* Reference-signature convenience: the window holds T records back to back
+ * (zeta [T][eta][xi], u/v [T][s_rho][eta][xi], w/akt [T][s_w][eta][xi]) and
+ * oceanTime[T]; resolves the interval and record pair exactly as
+ * Particle.cpp:110-115,183-184 do, then calls wco_move_all. */
+void wco_move_window(const wco_config *cfg, const wco_grid *grid, int ocean_time_idx, int T,
+                     const double *oceanTime, const float *zeta, const float *u, const float *v,
+                     const float *w, const float *akt, wco_particle *ps, long n, int nthreads);
+
+/* Concentration::evaluate (Concentration.hpp:16-44) for one frame:
+ * counts[s_rho][eta][xi] (storage level k+s_rho-1) += 1 for every alive
+ * particle whose rounded indices lie in the grid. */
+void wco_grid_count(const wco_particle *ps, long n, int s_rho, int eta, int xi, uint32_t *counts);
+
+/* sourceEmissionTimes (SourceEmission.hpp:38-79).  mode: 0 uniform_rate,
+ * 1 forcing_interval_batch, 2 single_pulse.  Writes at most cap times; returns
+ * the number of emission times, or -1 for an invalid schedule (the reference
+ * throws), -2 when cap is too small. */
+long wco_emission_times(int mode, double value, double sourceStart, double sourceEnd,
+                        double intervalStart, double intervalEnd, double *times, long cap);
+
+/* Haversine cell sizes of Particle.cpp:548-574 for cell (j,i): dx toward
+ * (j,i+1), dy toward (j+1,i). */
+void wco_cell_metric(const wco_grid *grid, int j, int i, double *dx, double *dy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
