@@ -1,0 +1,231 @@
+"""Benchmark of the particle-move + gridding path on one MI355X.
+
+Workload (the full-size plain-method configuration): 1024(eta) x 768(xi) x 11
+sigma-level grid at 150 m with ~30 % land, u/v from 64 random Fourier modes
+(5-40 km, rms 0.3 m/s), w from continuity, 25 hourly records; 4,000,000
+particles released at t0 from 50 random ocean point sources at 1 m depth;
+Kh = 1 m2/s (sigma = sqrt(2 Kh dti)), AKt = 1e-4 m2/s, dti = 30 s, explicit
+midpoint, default closures.  Synthetic data generated in memory.
+
+A step is one forcing interval of the hot path for the whole population:
+move (120 substeps) -> remove dead -> grid, i.e. one pass of the interval
+loop of Wacomm::run.  All forcing records are resident in HBM before the
+timed region.  The headline value is the FP64-state parity path (FP32-stored
+forcing, as the reference stores it); the mixed-precision path with BF16,
+FP16 and FP32 storage is reported alongside.
+
+    python bench.py [--gpus 1] [--steps 24] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_step(store_bytes: int, static_bytes: int, midpoint: bool, random: bool) -> int:
+    """Unique forcing values the reference stencil reads in one live substep
+    (Particle.cpp:271-290, 352-500, 585-641, 709-739), both records counted:
+      dynamic: zeta(X) 8, u 8, v 8, w 16, zeta(X') 8 [+ midpoint: zeta(X*) 8,
+               u/v/w(X*) 32] [+ diffusion: AKt 16]
+      static:  h(X) 4, dx+dy 2, depthIntervals 1, h(X') 4 [+ midpoint: h(X*) 4,
+               dx+dy(X*) 2, depthIntervals 1]."""
+    n_dyn = 48 + (40 if midpoint else 0) + (16 if random else 0)
+    n_stat = 11 + (7 if midpoint else 0)
+    return n_dyn * store_bytes + n_stat * static_bytes
+
+
+STATE_BYTES = 96   # FP64 SoA: read id,k,j,i,health,age,time (56 B) + write k,j,i,health,age (40 B)
+
+
+def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
+    """Warmup intervals, then `steps` timed intervals from t0 (records resident)."""
+    import torch
+    for w in range(warmup):
+        ctx.set_particles(p0)
+        ctx.move_interval(cfg, float(W.times[w]), float(W.times[w + 1]), w, w + 1, path=path)
+        ctx.remove_dead()
+        ctx.grid(counts=False, conc=False)
+        ctx.sync()
+    ctx.set_particles(p0)
+    ctx.sync()
+    ctx.reset_kernel_time()
+    ctx.reset_step_count()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ctx.move_interval(cfg, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
+        ctx.remove_dead()                 # synchronous (returns the survivor count)
+        ctx.grid(counts=False, conc=False)
+        if s % 6 == 5:
+            log(f"  {progress} interval {s + 1}/{steps}")
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.set_timing(False)
+    move_ms, move_n = ctx.kernel_time(0)
+    grid_ms, grid_n = ctx.kernel_time(1)
+    return dict(wall_s=wall, steps=ctx.step_count(), move_ms=move_ms, move_launches=move_n,
+                grid_ms=grid_ms, grid_launches=grid_n, alive=ctx.count())
+
+
+def cpu_baseline(W, cfg, p0, n_sample=100_000, n_intervals=4):
+    """The oracle (FP64 C restatement of Particle::move, OpenMP dynamic,64 as
+    Wacomm.cpp:501) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from runner import oracle_pair, cfg_to_oracle
+    import pyoracle as po
+    stride = max(1, len(p0) // n_sample)
+    p = p0[::stride].copy()
+    ocfg = cfg_to_oracle(cfg)
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    windows = [oracle_pair(W, t, t + 1) for t in range(n_intervals)]
+    steps = 0
+    t0 = time.perf_counter()
+    for t in range(n_intervals):
+        alive_before = p["health"] > 0
+        po.move(windows[t], ocfg, 0, p, nthreads=cores)
+        steps += int(alive_before.sum())
+    wall = time.perf_counter() - t0
+    # every alive particle takes the 120 substeps of the interval (single pulse at t0)
+    psteps = steps * int(round(3600.0 / cfg.dti))
+    return dict(value=psteps / wall, unit="particle-steps/s", cores=cores, kind="port",
+                sample=f"{len(p)} of the 4M particles (every {stride}th), first {n_intervals} intervals "
+                       f"({psteps:.3g} particle-steps, {wall:.1f} s wall on {cores} threads)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=4_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is described "
+                                   "in DESIGN.md and not run", "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+    assert 1 <= args.steps <= 24, "the window holds 24 intervals"
+    import torch
+    import wacommplusplus_amd as wc
+    from wacommplusplus_amd.synthetic import full_window, release
+
+    log("generating the 1024x768x11 window (25 records) and the 4M release")
+    W = full_window()
+    p0 = release(W, args.particles, 50)
+    cfg = wc.default_config(random=True, sigma=wc.sigma_from_kh(1.0, 30.0), crid=1.0)
+    n_slots = W.T
+
+    results = {}
+    variants = [("parity", "f32")] + ([] if args.no_variants else [("fast", "bf16"), ("fast", "f16"),
+                                                                   ("fast", "f32")])
+    ctx = wc.Context(0)
+    for path, store in variants:
+        key = "f64" if path == "parity" else store
+        log(f"[{key}] uploading {n_slots} records ({store} storage)")
+        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=n_slots,
+                     store=store)
+        for t in range(n_slots):
+            ctx.upload_record(t, *W.record(t))
+        log(f"[{key}] timing {args.steps} intervals ({args.warmup} warmup)")
+        results[key] = run_timed(wc, ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key)
+        log(f"[{key}] {results[key]}")
+
+    # PCIe-inclusive rate: two resident slots, one record uploaded per interval
+    pcie = None
+    if not args.no_variants:
+        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=2, store="f32")
+        ctx.set_particles(p0)
+        ctx.upload_record(0, *W.record(0))
+        ctx.reset_step_count()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            ctx.upload_record((s + 1) % 2, *W.record(s + 1))
+            ctx.move_interval(cfg, float(W.times[s]), float(W.times[s + 1]), s % 2, (s + 1) % 2, path="parity")
+            ctx.remove_dead()
+            ctx.grid(counts=False)
+        ctx.sync()
+        pcie = ctx.step_count() / (time.perf_counter() - t0)
+        log(f"[pcie] {pcie:.4g} particle-steps/s")
+    ctx.close()
+
+    main_r = results["f64"]
+    value = main_r["steps"] / main_r["wall_s"]
+    bps = algorithmic_bytes_per_step(4, 8, True, True)
+    avg_launch_s = main_r["move_ms"] / 1e3 / main_r["move_launches"]
+    steps_per_launch = main_r["steps"] / main_r["move_launches"]
+    bytes_per_launch = steps_per_launch * bps + args.particles * STATE_BYTES
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "move_parity_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None if args.no_cpu else cpu_baseline(W, cfg, p0)
+
+    variants_out = {}
+    for key, r in results.items():
+        if key == "f64":
+            continue
+        b = algorithmic_bytes_per_step(2 if key in ("bf16", "f16") else 4, 4, True, True)
+        a_s = r["move_ms"] / 1e3 / r["move_launches"]
+        spl = r["steps"] / r["move_launches"]
+        variants_out[key] = dict(value=r["steps"] / r["wall_s"], ms_per_step=1e3 * r["wall_s"] / args.steps,
+                                 move_kernel_ms=r["move_ms"] / r["move_launches"],
+                                 achieved_gbs=(spl * b + args.particles * STATE_BYTES) / a_s / 1e9)
+    line = {
+        "metric": "particle-steps/s at 4M (BF16 vs FP64 forcing); conc. L2 err at equal GPU-ms",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * main_r["wall_s"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "full-size plain method: 1024x768x11 sigma grid at 150 m, 64-mode forcing, ~30% land, "
+                        "4M particles from 50 sources, Kh=1 m2/s, dt=30 s, midpoint, 24 h; FP64 state, "
+                        "FP32-stored forcing (the reference's storage), records resident in HBM",
+            "particles": args.particles, "grid": [11, 1024, 768], "substeps_per_step": 120,
+            "path": "parity (FP64)",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "move_parity_kernel<SoAAccess>",
+            "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_particle_step": bps,
+        },
+        "cpu_baseline": cpu,
+        "variants": variants_out,
+        "pcie_inclusive_value": pcie,
+        "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),
+        "alive_after": main_r["alive"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,244 @@
+/*
+ * wacomm_hip.h -- C ABI of libwacomm_hip.so, the MI355X (gfx950) particle-move
+ * and concentration-gridding backend for WaComM++.
+ *
+ * Two layers:
+ *   1. Drop-in entry points with the reference's own argument lists
+ *      (wc_hip_move_particles replaces cudaMoveParticle; wc_hip_concentration
+ *      replaces Concentration::evaluate).  All pointers are device pointers, as
+ *      in the reference's CUDA call.
+ *   2. The native context API: the library owns the particle state (SoA) and a
+ *      ring of forcing records on the device (two slots = the reference's
+ *      bounded current/adjacent pair), and runs the interval loop of
+ *      Wacomm::run (move -> remove dead -> grid) without host round trips.
+ *
+ * Conventions: every function returns a wc_status (never exits the process);
+ * wc_last_error() returns the message of the last failure on a context.  All
+ * work is asynchronous on the context's stream unless stated; wc_sync() waits.
+ * Calls on one context must be serialised by the caller; one context per
+ * device.  Plain C types only (no HIP or torch types in the signatures).
+ */
+#ifndef WACOMM_HIP_H
+#define WACOMM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define WC_BOOL bool
+extern "C" {
+#else
+#define WC_BOOL _Bool
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Shared data structures (byte-compatible with the reference)
+ * ------------------------------------------------------------------------- */
+
+/* observational_calibration_region, Config.hpp:25-34 */
+typedef struct wc_calibration_region {
+    double west, south, east, north;
+    double windErrorStdDev, windErrorComponentCorrelation;
+    double windErrorSpatialScale, windErrorTemporalScale;
+} wc_calibration_region;
+
+/* config_data, Config.hpp:36-72 (same field order, types and padding) */
+typedef struct wc_config_data {
+    WC_BOOL random;
+    WC_BOOL randomSources;
+    double deltat;
+    double dti;
+    double survprob;
+    double tau0;
+    double sv;
+    double crid;
+    double sigma;
+    double shoreLimit;
+    int upperClosure;
+    int lowerClosure;
+    int horizontalClosure;
+    uint64_t randomSeed;
+    int trackingDirection;
+    int backwardDiffusion;
+    int integrator;
+    double restartCheckpoint;
+    int driftModel;
+    WC_BOOL leewayCoefficientEnsemble;
+    double leewayResidualCorrelation;
+    WC_BOOL leewayRandomSide;
+    double leewayRightSideProbability;
+    double leewayJibeProbabilityHourly;
+    double windErrorStdDev;
+    double windErrorComponentCorrelation;
+    double windErrorSpatialScale;
+    double windErrorTemporalScale;
+    uint32_t observationalCalibrationRegionCount;
+    wc_calibration_region observationalCalibrationRegions[16];
+    uint16_t driftObjectType;
+    int8_t driftSide;
+    WC_BOOL hasWind;
+    double windU10;
+    double windV10;
+} wc_config_data;
+
+/* particle_data, Particle.hpp:31-41 (64 bytes) */
+typedef struct wc_particle_data {
+    uint64_t id;
+    double k;
+    double j;
+    double i;
+    double health;
+    double age;
+    double time;
+    uint16_t driftObjectType;
+    int8_t driftSide;
+} wc_particle_data;
+
+typedef enum wc_status {
+    WC_OK = 0,
+    WC_ERR_HIP = 1,          /* a HIP runtime call or kernel launch failed */
+    WC_ERR_INVALID = 2,      /* bad argument (sizes, slots, null pointers) */
+    WC_ERR_UNSUPPORTED = 3,  /* configuration the device path does not implement
+                                (driftModel==1, jibing): use the CPU backend */
+    WC_ERR_NOMEM = 4,        /* device allocation failed */
+    WC_ERR_NO_DEVICE = 5     /* no HIP device, or an invalid device index */
+} wc_status;
+
+/* Closure and integrator codes, Config.hpp:201-207 */
+enum { WC_CLOSURE_CONSTRAINT = 1, WC_CLOSURE_KILL = 2, WC_CLOSURE_REFLECTION = 3 };
+enum { WC_INTEGRATOR_EULER = 1, WC_INTEGRATOR_MIDPOINT = 2 };
+
+/* Forcing storage on the device (the move always reads FP32 values from the
+ * host; the context converts once per record upload). */
+enum { WC_STORE_F32 = 0, WC_STORE_F16 = 1, WC_STORE_BF16 = 2 };
+
+/* Move paths of wc_move_interval:
+ *   WC_PATH_PARITY: FP64 state and the reference's rounding points
+ *                   (positions match Particle::move to ~1e-12 cells); reads
+ *                   FP32 forcing.
+ *   WC_PATH_FAST:   integer cell + FP32 offset state, FP32 interpolation of
+ *                   F32/F16/BF16 forcing, Philox4x32-10 diffusion draws keyed
+ *                   on (seed, id, interval, substep). */
+enum { WC_PATH_PARITY = 0, WC_PATH_FAST = 1 };
+
+/* ---------------------------------------------------------------------------
+ * 1. Drop-in entry points
+ * ------------------------------------------------------------------------- */
+
+/* Replaces cudaMoveParticle (cuda/kernel.h:7, cuda/kernel.cu:370-382): moves
+ * particleCount particles in place over the forcing interval that starts at
+ * times[timeIndex], exactly as Particle::move (Particle.cpp:71-824) does.
+ * config, particles, times and all field arrays are DEVICE pointers laid out
+ * as the reference's (index4 = ((t*levels + k+levels-1)*eta + j)*xi + i,
+ * kernel.cu:63-69).  mask, numThread, numGPU are unused (as in the reference);
+ * windU10..stokesV must be NULL or are ignored for passive particles.
+ * stream is a hipStream_t (NULL = default stream).  Asynchronous: results are
+ * valid after the stream is synchronised.  Returns WC_ERR_UNSUPPORTED when
+ * config->driftModel==1 or leewayJibeProbabilityHourly>0. */
+int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, int timeIndex, int oceanTime,
+                          int sW, int sRho, int eta, int xi, double *times, double *mask, double *lonRad,
+                          double *latRad, double *depthIntervals, double *h, float *zeta, float *u, float *v,
+                          float *w, float *akt, float *windU10, float *windV10, float *stokesU, float *stokesV,
+                          int particleCount, int numThread, int numGPU, void *stream);
+
+/* Replaces Concentration::evaluate (Concentration.hpp:16-44) for one frame:
+ * conc[s_rho][eta][xi] (device, float, storage level k+s_rho-1) += 1 for every
+ * alive particle whose (int)round(k), (int)round(j), (int)round(i) lie in the
+ * grid.  Counts are accumulated exactly in uint32 and added once per cell,
+ * which equals the reference's repeated +1.0f below 2^24 per cell.
+ * particles is a device pointer. */
+int wc_hip_concentration(const wc_particle_data *particles, int particleCount, int s_rho, int eta, int xi,
+                         float *conc, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * 2. Native context API
+ * ------------------------------------------------------------------------- */
+
+typedef struct wc_ctx wc_ctx;
+
+/* Creates a context on HIP device `device` with its own stream (default
+ * priority).  Replaces the per-interval setup of Wacomm.cpp:386-482. */
+int wc_ctx_create(int device, wc_ctx **out);
+int wc_ctx_destroy(wc_ctx *ctx);
+/* Message of the last failure on ctx (empty when none); valid until the next call. */
+const char *wc_last_error(const wc_ctx *ctx);
+int wc_sync(wc_ctx *ctx);
+
+/* Static grid of the forcing window (OceanModelAdapter.hpp:20-34), host
+ * pointers copied to the device.  lonRad, latRad, h, mask: [eta][xi];
+ * depthIntervals: s_w-1 values (entry kI+s_w-2 = depthIntervals(kI) of the
+ * reference, ROMSAdapter.cpp:233-235).  Also precomputes each cell's
+ * haversine sizes (Particle.cpp:544-574) in FP64 on the host.
+ * n_slots = resident forcing records (>= 2; 2 = the current/adjacent pair of
+ * WacommPlusPlus.cpp:98-161).  store = WC_STORE_*. */
+int wc_set_grid(wc_ctx *ctx, int s_w, int s_rho, int eta, int xi, const double *lonRad, const double *latRad,
+                const double *h, const double *mask, const double *depthIntervals, int n_slots, int store);
+
+/* Uploads one forcing record (host FP32 arrays: zeta [eta][xi], u, v
+ * [s_rho][eta][xi], w, akt [s_w][eta][xi]) into ring slot `slot`, converting
+ * to the context's storage type.  Replaces the whole-window re-upload of
+ * Wacomm.cpp:417-477 by one record per interval. */
+int wc_upload_record(wc_ctx *ctx, int slot, const float *zeta, const float *u, const float *v, const float *w,
+                     const float *akt);
+
+/* Particle state: host AoS in and out (Particles.hpp:17 holds particle_data).
+ * wc_set_particles replaces the device population; wc_append_particles adds
+ * to it (emission). */
+int wc_set_particles(wc_ctx *ctx, const wc_particle_data *particles, size_t n);
+int wc_append_particles(wc_ctx *ctx, const wc_particle_data *particles, size_t n);
+int wc_get_particles(wc_ctx *ctx, wc_particle_data *particles, size_t n);
+int wc_particle_count(const wc_ctx *ctx, size_t *n);
+
+/* One forcing interval of Particle::move for every particle (the loop of
+ * Wacomm.cpp:501-525).  config is a HOST pointer.  slot0 holds the record at
+ * the interval start, slot1 the record at its end (the same slot when the
+ * window has no next record, Particle.cpp:183-184).  path = WC_PATH_*.
+ * Asynchronous on the context stream. */
+int wc_move_interval(wc_ctx *ctx, const wc_config_data *config, double intervalStart, double intervalEnd,
+                     int slot0, int slot1, int path);
+
+/* Order-preserving removal of particles with health <= 0 (Wacomm.cpp:710-713).
+ * Synchronous: *remaining receives the new count. */
+int wc_remove_dead(wc_ctx *ctx, size_t *remaining);
+
+/* Concentration::evaluate (Concentration.hpp:16-44) into the context's device
+ * count frame [s_rho][eta][xi] (uint32, zeroed first).  counts_out, when not
+ * NULL, receives a host copy (synchronous).  conc_out, when not NULL,
+ * receives the reference's float frame (Wacomm.cpp:722-739): counts as float,
+ * and 1e37 on land cells (mask != 1) when mask_output is nonzero. */
+int wc_grid(wc_ctx *ctx, uint32_t *counts_out, float *conc_out, int mask_output);
+
+/* Accumulated device time of the move kernels launched since the last reset,
+ * measured with HIP events on the context stream around each launch (enable
+ * with wc_set_timing).  kernel: 0 = move, 1 = grid. */
+int wc_set_timing(wc_ctx *ctx, int enabled);
+int wc_kernel_time(wc_ctx *ctx, int kernel, double *total_ms, long *launches);
+int wc_reset_kernel_time(wc_ctx *ctx);
+
+/* Live particle-substeps taken by the moves since the last reset (counted on
+ * the device; the numerator of particle-steps/s).  Synchronous. */
+int wc_step_count(wc_ctx *ctx, uint64_t *steps);
+int wc_reset_step_count(wc_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * 3. Host utilities (no device needed)
+ * ------------------------------------------------------------------------- */
+
+/* Library version string. */
+const char *wc_version(void);
+
+/* Haversine cell sizes of Particle.cpp:544-574 for every cell of an
+ * [eta][xi] lon/lat grid (radians): dx toward (j,i+1), dy toward (j+1,i);
+ * the last row and column hold 0.  The context computes its table with this. */
+void wc_cell_metric_table(int eta, int xi, const double *lonRad, const double *latRad, double *dx, double *dy);
+
+/* Philox4x32-10 block (the generator of WC_PATH_FAST's diffusion draws):
+ * out = philox(ctr[4], key[2]).  No reference counterpart; exposed so the
+ * generator can be checked against the published known-answer vectors. */
+void wc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WACOMM_HIP_H */

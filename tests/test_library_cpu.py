@@ -1,0 +1,92 @@
+"""CPU checks of libwacomm_hip.so: it loads, exports every symbol the C header
+declares, shares the reference's struct layouts with the oracle, and its host
+utilities agree with the oracle and with published vectors.  No kernel is
+launched here (this container has no GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import wacommplusplus_amd as wc
+from wacommplusplus_amd.synthetic import full_window, plain_window, depth_to_k
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "wacomm_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(wc_[a-z0-9_]+)\s*\(", text))
+
+
+def test_library_exports_every_declared_symbol():
+    L = wc.load_library()
+    declared = header_symbols()
+    assert declared == set(wc.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.wc_version().startswith(b"wacomm-hip")
+
+
+def test_struct_layouts_match_reference_and_oracle():
+    assert C.sizeof(wc.ConfigData) == C.sizeof(po.Config)
+    for (n1, t1), (n2, t2) in zip(wc.ConfigData._fields_, po.Config._fields_):
+        assert n1 == n2 and getattr(wc.ConfigData, n1).offset == getattr(po.Config, n2).offset
+    assert wc.PARTICLE_DTYPE == po.PARTICLE_DTYPE and wc.PARTICLE_DTYPE.itemsize == 64
+
+
+def test_context_without_device_fails_cleanly():
+    # no GPU in this container: creating a context must report, not crash
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(wc.WcError) as e:
+        wc.Context(0)
+    assert e.value.status == 5
+
+
+def test_metric_table_equals_oracle():
+    W = full_window(T=1, eta=64, xi=48)
+    dx, dy = wc.cell_metric_table(W.lonRad, W.latRad)
+    g = po.Window(W.times, W.lonRad, W.latRad, W.h, W.depthIntervals,
+                  *[a[None] for a in W.record(0)])
+    for j in range(0, 63, 3):
+        for i in range(0, 47, 3):
+            assert g.cell_metric(j, i) == (dx[j, i], dy[j, i])
+
+
+def test_philox_known_answers():
+    # Random123 known-answer vectors for philox4x32-10 (Salmon et al., SC'11)
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, want in kat:
+        assert wc.philox4x32_10(ctr, key).tolist() == want
+
+
+def test_depth_to_k():
+    # ApplicationEndToEnd: 10 m release at the grid of ParticleMoveTest-like sigma gives k=-0.2
+    assert abs(depth_to_k(1.0, 50.0, 11) - (-0.2)) < 1e-12
+    assert abs(depth_to_k(1.0, 40.0, 12) - (-0.275)) < 1e-12
+    assert depth_to_k(0.0, 40.0, 12) == 0.0
+    assert depth_to_k(100.0, 40.0, 12) == -10.0
+
+
+def test_synthetic_full_window_properties():
+    W = full_window(T=2, eta=256, xi=192)
+    z, u, v, w, a = W.record(0)
+    land = W.mask == 0
+    assert 0.25 < land.mean() < 0.35
+    assert (W.h[land] == 0).all() and (u[:, land] == 0).all() and (w[:, land] == 0).all()
+    speed = np.sqrt(u[-1] ** 2 + v[-1] ** 2)[~land]
+    assert 0.15 < np.sqrt((speed ** 2).mean()) < 0.45
+    assert (w[0] == 0).all()                      # no flow through the bottom

@@ -1,0 +1,17 @@
+"""wacommplusplus_amd -- MI355X (gfx950) particle move and concentration
+gridding for WaComM++, behind the reference's Particles/Wacomm roles.
+
+The compute path is libwacomm_hip.so (hand-written HIP kernels, C ABI in
+include/wacomm_hip.h); this package is its host-side binding plus synthetic
+workload generators.  There is no CPU fallback: importing the binding fails
+loudly when the library has not been built.
+"""
+from .types import (BACKWARD, CONSTRAINT, EULER, FORWARD, KILL, MIDPOINT, PARTICLE_DTYPE, REFLECTION,
+                    ConfigData, default_config, make_particles, sigma_from_kh)
+from ._lib import EXPORTS, LIB_PATH, Context, WcError, cell_metric_table, load_library, philox4x32_10
+
+__all__ = [
+    "BACKWARD", "CONSTRAINT", "EULER", "FORWARD", "KILL", "MIDPOINT", "PARTICLE_DTYPE", "REFLECTION",
+    "ConfigData", "default_config", "make_particles", "sigma_from_kh", "EXPORTS", "LIB_PATH", "Context",
+    "WcError", "cell_metric_table", "load_library", "philox4x32_10",
+]

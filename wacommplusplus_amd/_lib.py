@@ -1,0 +1,236 @@
+"""ctypes binding of libwacomm_hip.so (include/wacomm_hip.h).
+
+The library is built in-tree by wacommplusplus_amd.build; loading fails loudly
+when it is missing -- there is no CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .types import PARTICLE_DTYPE, PATH, STORE, ConfigData
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libwacomm_hip.so")
+
+WC_OK, WC_ERR_HIP, WC_ERR_INVALID, WC_ERR_UNSUPPORTED, WC_ERR_NOMEM, WC_ERR_NO_DEVICE = range(6)
+_STATUS = {1: "WC_ERR_HIP", 2: "WC_ERR_INVALID", 3: "WC_ERR_UNSUPPORTED", 4: "WC_ERR_NOMEM",
+           5: "WC_ERR_NO_DEVICE"}
+
+# every symbol include/wacomm_hip.h declares
+EXPORTS = (
+    "wc_hip_move_particles", "wc_hip_concentration", "wc_ctx_create", "wc_ctx_destroy", "wc_last_error",
+    "wc_sync", "wc_set_grid", "wc_upload_record", "wc_set_particles", "wc_append_particles",
+    "wc_get_particles", "wc_particle_count", "wc_move_interval", "wc_remove_dead", "wc_grid",
+    "wc_set_timing", "wc_kernel_time", "wc_reset_kernel_time", "wc_step_count", "wc_reset_step_count",
+    "wc_version", "wc_cell_metric_table", "wc_philox4x32_10",
+)
+
+
+class WcError(RuntimeError):
+    def __init__(self, status, message=""):
+        self.status = status
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+
+
+_lib = None
+
+vp = C.c_void_p
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libwacomm_hip.so (once) and declares the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m wacommplusplus_amd.build` "
+                          "(the HIP backend has no CPU fallback)")
+    L = C.CDLL(path)
+    i, d, sz = C.c_int, C.c_double, C.c_size_t
+    sig = {
+        "wc_hip_move_particles": [vp, vp, i, i, i, i, i, i] + [vp] * 6 + [vp] * 9 + [i, i, i, vp],
+        "wc_hip_concentration": [vp, i, i, i, i, vp, vp],
+        "wc_ctx_create": [i, C.POINTER(vp)],
+        "wc_ctx_destroy": [vp],
+        "wc_sync": [vp],
+        "wc_set_grid": [vp, i, i, i, i, vp, vp, vp, vp, vp, i, i],
+        "wc_upload_record": [vp, i, vp, vp, vp, vp, vp],
+        "wc_set_particles": [vp, vp, sz],
+        "wc_append_particles": [vp, vp, sz],
+        "wc_get_particles": [vp, vp, sz],
+        "wc_particle_count": [vp, C.POINTER(sz)],
+        "wc_move_interval": [vp, C.POINTER(ConfigData), d, d, i, i, i],
+        "wc_remove_dead": [vp, C.POINTER(sz)],
+        "wc_grid": [vp, vp, vp, i],
+        "wc_set_timing": [vp, i],
+        "wc_kernel_time": [vp, i, C.POINTER(d), C.POINTER(C.c_long)],
+        "wc_reset_kernel_time": [vp],
+        "wc_step_count": [vp, C.POINTER(C.c_uint64)],
+        "wc_reset_step_count": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.wc_last_error.argtypes = [vp]
+    L.wc_last_error.restype = C.c_char_p
+    L.wc_version.argtypes = []
+    L.wc_version.restype = C.c_char_p
+    L.wc_cell_metric_table.argtypes = [i, i, vp, vp, vp, vp]
+    L.wc_cell_metric_table.restype = None
+    L.wc_philox4x32_10.argtypes = [vp, vp, vp]
+    L.wc_philox4x32_10.restype = None
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(vp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Context:
+    """One device context: the particle population (SoA on the device), a
+    ring of forcing records and the count frame.  Mirrors the roles of
+    Wacomm::run's device state (Wacomm.cpp:386-610) without per-interval
+    re-allocation."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = vp()
+        rc = self.L.wc_ctx_create(device, C.byref(h))
+        if rc:
+            raise WcError(rc, f"wc_ctx_create(device={device})")
+        self.h = h
+        self.shape = None
+
+    def _chk(self, rc):
+        if rc:
+            raise WcError(rc, self.L.wc_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.wc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- grid and forcing ---------------------------------------------------
+    def set_grid(self, lonRad, latRad, h, depthIntervals, s_rho, mask=None, n_slots=2, store="f32"):
+        lon, lat, hh, dep = _f64(lonRad), _f64(latRad), _f64(h), _f64(depthIntervals)
+        eta, xi = hh.shape
+        m = None if mask is None else _f64(mask)
+        self._keep = (lon, lat, hh, dep, m)
+        self._chk(self.L.wc_set_grid(self.h, s_rho + 1, s_rho, eta, xi, _p(lon), _p(lat), _p(hh),
+                                     None if m is None else _p(m), _p(dep), n_slots, STORE[store]))
+        self.shape = (s_rho, eta, xi)
+
+    def upload_record(self, slot, zeta, u, v, w, akt):
+        arrs = [_f32(a) for a in (zeta, u, v, w, akt)]
+        s_rho, eta, xi = self.shape
+        assert arrs[0].shape == (eta, xi) and arrs[1].shape == (s_rho, eta, xi) == arrs[2].shape
+        assert arrs[3].shape == (s_rho + 1, eta, xi) == arrs[4].shape
+        self._chk(self.L.wc_upload_record(self.h, slot, *[_p(a) for a in arrs]))
+
+    # -- particles ----------------------------------------------------------
+    def set_particles(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=PARTICLE_DTYPE)
+        self._chk(self.L.wc_set_particles(self.h, _p(p), len(p)))
+
+    def append_particles(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=PARTICLE_DTYPE)
+        self._chk(self.L.wc_append_particles(self.h, _p(p), len(p)))
+
+    def count(self) -> int:
+        n = C.c_size_t()
+        self._chk(self.L.wc_particle_count(self.h, C.byref(n)))
+        return n.value
+
+    def get_particles(self) -> np.ndarray:
+        n = self.count()
+        out = np.zeros(n, dtype=PARTICLE_DTYPE)
+        self._chk(self.L.wc_get_particles(self.h, _p(out), n))
+        return out
+
+    # -- the hot path -------------------------------------------------------
+    def move_interval(self, config: ConfigData, start: float, end: float, slot0: int, slot1: int,
+                      path: str = "parity"):
+        self._chk(self.L.wc_move_interval(self.h, C.byref(config), float(start), float(end), slot0, slot1,
+                                          PATH[path]))
+
+    def remove_dead(self) -> int:
+        n = C.c_size_t()
+        self._chk(self.L.wc_remove_dead(self.h, C.byref(n)))
+        return n.value
+
+    def grid(self, counts: bool = True, conc: bool = False, mask_output: bool = False):
+        s_rho, eta, xi = self.shape
+        c = np.zeros((s_rho, eta, xi), np.uint32) if counts else None
+        f = np.zeros((s_rho, eta, xi), np.float32) if conc else None
+        self._chk(self.L.wc_grid(self.h, None if c is None else _p(c), None if f is None else _p(f),
+                                 int(mask_output)))
+        if counts and conc:
+            return c, f
+        return c if counts else f
+
+    def sync(self):
+        self._chk(self.L.wc_sync(self.h))
+
+    # -- measurement --------------------------------------------------------
+    def set_timing(self, on: bool = True):
+        self._chk(self.L.wc_set_timing(self.h, int(on)))
+
+    def kernel_time(self, kernel: int = 0):
+        ms, n = C.c_double(), C.c_long()
+        self._chk(self.L.wc_kernel_time(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def reset_kernel_time(self):
+        self._chk(self.L.wc_reset_kernel_time(self.h))
+
+    def step_count(self) -> int:
+        v = C.c_uint64()
+        self._chk(self.L.wc_step_count(self.h, C.byref(v)))
+        return v.value
+
+    def reset_step_count(self):
+        self._chk(self.L.wc_reset_step_count(self.h))
+
+
+def cell_metric_table(lonRad, latRad):
+    """Host haversine table of the library (Particle.cpp:544-574)."""
+    L = load_library()
+    lon, lat = _f64(lonRad), _f64(latRad)
+    eta, xi = lon.shape
+    dx, dy = np.zeros_like(lon), np.zeros_like(lon)
+    L.wc_cell_metric_table(eta, xi, _p(lon), _p(lat), _p(dx), _p(dy))
+    return dx, dy
+
+
+def philox4x32_10(ctr, key):
+    L = load_library()
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    L.wc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out

@@ -1,0 +1,666 @@
+// C ABI of libwacomm_hip.so: drop-in entry points and the native context.
+// Declarations and contracts: include/wacomm_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wacomm_hip.h"
+#include "wc_launch.h"
+#include "wc_philox.h"
+
+#pragma clang fp contract(off)
+
+using namespace wc;
+
+struct wc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // static grid
+    bool haveGrid = false;
+    int s_w = 0, s_rho = 0, eta = 0, xi = 0;
+    int store = WC_STORE_F32;
+    double *d_h = nullptr, *d_depth = nullptr, *d_dx = nullptr, *d_dy = nullptr, *d_mask = nullptr;
+    float *d_hf = nullptr, *d_idxf = nullptr, *d_idyf = nullptr, *d_depthf = nullptr;
+
+    // forcing record ring: per slot, FP32 fields (parity path) and, for 16-bit
+    // storage, converted copies (fast path)
+    struct Slot {
+        float *f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};      // zeta, u, v, w, akt
+        void *lp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        bool filled = false;
+    };
+    std::vector<Slot> slots;
+
+    // particles (SoA, double-buffered for compaction)
+    size_t n = 0, cap = 0;
+    SoA s{}, s2{};
+    uint32_t *d_blockCounts = nullptr;
+    unsigned long long *d_total = nullptr;
+    unsigned long long *d_steps = nullptr;
+    uint32_t *d_counts = nullptr;
+    float *d_conc = nullptr;
+
+    // timing: an event pair per timed launch, resolved when queried (no
+    // synchronisation between launches)
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
+    std::vector<hipEvent_t> spare;
+    hipEvent_t cur = nullptr;
+    double ms[2] = {0, 0};
+    long launches[2] = {0, 0};
+};
+
+namespace {
+
+int fail(wc_ctx *c, int code, const char *fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? WC_ERR_NOMEM : WC_ERR_HIP, "%s: %s (%s:%d)", #call, \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                               \
+    } while (0)
+
+size_t field_count(const wc_ctx *c, int f) {
+    const size_t plane = (size_t)c->eta * c->xi;
+    return f == 0 ? plane : (f <= 2 ? plane * c->s_rho : plane * c->s_w);
+}
+
+double std_min(double a, double b) { return (b < a) ? b : a; }
+double std_max(double a, double b) { return (a < b) ? b : a; }
+
+// NumericalHelpers::restartElapsed (NumericalHelpers.hpp:45-52)
+double restart_elapsed(double s, double e, double c) {
+    if (e > s) {
+        if (e <= c) return NAN;
+        return std_min(e - s, std_max(0.0, c - s));
+    }
+    if (e >= c) return NAN;
+    return std_min(s - e, std_max(0.0, s - c));
+}
+
+// Per-interval scalars of Particle::move (Particle.cpp:81-125, 507, 518-523).
+// Returns false when the interval is wholly before a restart checkpoint (the
+// reference returns before touching the particle, Particle.cpp:117-120).
+bool make_params(const wc_config_data *cfg, double start, double end, int s_w, int s_rho, int eta, int xi,
+                 MoveParams &P) {
+    P.dti = cfg->dti;
+    P.tau0 = cfg->tau0;
+    P.survprob = cfg->survprob;
+    P.crid = cfg->crid;
+    P.sv = cfg->sv;
+    P.sigma = cfg->sigma;
+    P.shoreLimit = cfg->shoreLimit;
+    P.intervalStart = start;
+    P.intervalLength = std::fabs(end - start);
+    P.elapsed0 = 0;
+    if (!std::isnan(cfg->restartCheckpoint)) {
+        double e = restart_elapsed(start, end, cfg->restartCheckpoint);
+        if (std::isnan(e)) return false;
+        P.elapsed0 = e;
+    }
+    P.kLowerLimit = -s_w + 2;
+    P.direction = cfg->trackingDirection;
+    P.ageKill = cfg->survprob > 0 ? -cfg->tau0 * std::log(cfg->survprob) : INFINITY;
+    P.seed = cfg->randomSeed;
+    P.intervalKey = (int64_t)std::llround(std_min(start, end));
+    P.upper = cfg->upperClosure;
+    P.lower = cfg->lowerClosure;
+    P.horizontal = cfg->horizontalClosure;
+    P.dirInt = cfg->trackingDirection;
+    P.randomOn = cfg->random && (P.direction > 0 || cfg->backwardDiffusion);
+    P.midpoint = cfg->integrator == WC_INTEGRATOR_MIDPOINT;
+    P.s_w = s_w;
+    P.s_rho = s_rho;
+    P.eta = eta;
+    P.xi = xi;
+    return true;
+}
+
+int check_supported(wc_ctx *c, const wc_config_data *cfg) {
+    if (cfg->driftModel == 1)
+        return fail(c, WC_ERR_UNSUPPORTED, "driftModel==1 (leeway) is not implemented on the device path");
+    if (cfg->leewayJibeProbabilityHourly > 0)
+        return fail(c, WC_ERR_UNSUPPORTED, "leeway jibing is not implemented on the device path");
+    if (!(cfg->dti > 0)) return fail(c, WC_ERR_INVALID, "dti must be > 0");
+    if (cfg->trackingDirection != 1 && cfg->trackingDirection != -1)
+        return fail(c, WC_ERR_INVALID, "trackingDirection must be +1 or -1");
+    return WC_OK;
+}
+
+void free_soa(SoA &s) {
+    hipFree(s.id); hipFree(s.k); hipFree(s.j); hipFree(s.i); hipFree(s.health); hipFree(s.age); hipFree(s.time);
+    s = SoA{};
+}
+
+int alloc_soa(wc_ctx *c, SoA &s, size_t cap) {
+    HIPCHK(c, hipMalloc(&s.id, cap * sizeof(uint64_t)));
+    HIPCHK(c, hipMalloc(&s.k, cap * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.j, cap * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.i, cap * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.health, cap * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.age, cap * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.time, cap * sizeof(double)));
+    return WC_OK;
+}
+
+int reserve(wc_ctx *c, size_t need) {
+    if (need <= c->cap) return WC_OK;
+    if (need > (size_t)INT32_MAX) return fail(c, WC_ERR_INVALID, "at most %d particles per context", INT32_MAX);
+    size_t cap = c->cap ? c->cap : 1024;
+    while (cap < need) cap *= 2;
+    if (cap > (size_t)INT32_MAX) cap = INT32_MAX;
+    SoA a{}, b{};
+    int rc = alloc_soa(c, a, cap);
+    if (rc) { free_soa(a); return rc; }
+    rc = alloc_soa(c, b, cap);
+    if (rc) { free_soa(a); free_soa(b); return rc; }
+    if (c->n) {
+        const size_t n = c->n;
+        HIPCHK(c, hipMemcpyAsync(a.id, c->s.id, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.k, c->s.k, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.j, c->s.j, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.i, c->s.i, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.health, c->s.health, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.age, c->s.age, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.time, c->s.time, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    free_soa(c->s);
+    free_soa(c->s2);
+    hipFree(c->d_blockCounts);
+    c->d_blockCounts = nullptr;
+    c->s = a;
+    c->s2 = b;
+    c->cap = cap;
+    HIPCHK(c, hipMalloc(&c->d_blockCounts, ((cap + 255) / 256) * sizeof(uint32_t)));
+    return WC_OK;
+}
+
+// Host AoS -> device SoA at offset `at`.
+int upload_particles(wc_ctx *c, const wc_particle_data *p, size_t n, size_t at) {
+    std::vector<uint64_t> id(n);
+    std::vector<double> f[6];
+    for (auto &v : f) v.resize(n);
+    for (size_t q = 0; q < n; ++q) {
+        id[q] = p[q].id;
+        f[0][q] = p[q].k; f[1][q] = p[q].j; f[2][q] = p[q].i;
+        f[3][q] = p[q].health; f[4][q] = p[q].age; f[5][q] = p[q].time;
+        if (p[q].driftObjectType != 0)
+            return fail(c, WC_ERR_UNSUPPORTED, "particle %zu is not passive (driftObjectType=%u)", q,
+                        (unsigned)p[q].driftObjectType);
+    }
+    double *dst[6] = {c->s.k, c->s.j, c->s.i, c->s.health, c->s.age, c->s.time};
+    HIPCHK(c, hipMemcpyAsync(c->s.id + at, id.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    for (int q = 0; q < 6; ++q)
+        HIPCHK(c, hipMemcpyAsync(dst[q] + at, f[q].data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+hipEvent_t take_event(wc_ctx *c) {
+    if (!c->spare.empty()) {
+        hipEvent_t e = c->spare.back();
+        c->spare.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void timing_begin(wc_ctx *c) {
+    c->cur = nullptr;
+    if (!c->timing) return;
+    c->cur = take_event(c);
+    if (c->cur && hipEventRecord(c->cur, c->stream) != hipSuccess) {
+        c->spare.push_back(c->cur);
+        c->cur = nullptr;
+    }
+}
+
+int timing_end(wc_ctx *c, int kernel) {
+    if (!c->timing || !c->cur) return WC_OK;
+    hipEvent_t e = take_event(c);
+    if (!e) return fail(c, WC_ERR_HIP, "hipEventCreate failed");
+    HIPCHK(c, hipEventRecord(e, c->stream));
+    c->pending[kernel].emplace_back(c->cur, e);
+    c->cur = nullptr;
+    return WC_OK;
+}
+
+// Resolves recorded event pairs into the accumulated totals.
+int timing_collect(wc_ctx *c) {
+    for (int k = 0; k < 2; ++k) {
+        for (auto &pr : c->pending[k]) {
+            HIPCHK(c, hipEventSynchronize(pr.second));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, pr.first, pr.second));
+            c->ms[k] += ms;
+            c->launches[k] += 1;
+            c->spare.push_back(pr.first);
+            c->spare.push_back(pr.second);
+        }
+        c->pending[k].clear();
+    }
+    return WC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *wc_version(void) { return "wacomm-hip 0.1 (gfx950)"; }
+
+// Haversine cell sizes, the same expression as Particle.cpp:548-574, for
+// every cell (j < eta-1, i < xi-1); edge rows/columns hold 0.
+void wc_cell_metric_table(int eta, int xi, const double *lonRad, const double *latRad, double *dx, double *dy) {
+    for (int j = 0; j < eta; ++j)
+        for (int i = 0; i < xi; ++i) {
+            const size_t c = (size_t)j * xi + i;
+            dx[c] = 0;
+            dy[c] = 0;
+            if (j + 1 >= eta || i + 1 >= xi) continue;
+            double dLat = latRad[c + xi] - latRad[c];
+            double dLon = 0;
+            double t = pow(sin(0.5 * dLat), 2) + pow(sin(0.5 * dLon), 2) * cos(latRad[c + xi]) * cos(latRad[c]);
+            dy[c] = 2.0 * atan2(pow(t, .5), pow(1.0 - t, .5)) * 6371000.0;
+            dLat = 0;
+            dLon = lonRad[c + 1] - lonRad[c];
+            t = pow(sin(0.5 * dLat), 2) + pow(sin(0.5 * dLon), 2) * cos(latRad[c + 1]) * cos(latRad[c]);
+            dx[c] = 2.0 * atan2(pow(t, .5), pow(1.0 - t, .5)) * 6371000.0;
+        }
+}
+
+void wc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    const Philox4 r = philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    for (int q = 0; q < 4; ++q) out[q] = r.v[q];
+}
+
+// ---- drop-in entry points --------------------------------------------------
+
+int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, int timeIndex, int oceanTime, int sW,
+                          int sRho, int eta, int xi, double *times, double *mask, double *lonRad, double *latRad,
+                          double *depthIntervals, double *h, float *zeta, float *u, float *v, float *w, float *akt,
+                          float *windU10, float *windV10, float *stokesU, float *stokesV, int particleCount,
+                          int numThread, int numGPU, void *stream) {
+    (void)mask; (void)numThread; (void)numGPU;
+    (void)windU10; (void)windV10; (void)stokesU; (void)stokesV;
+    if (!config || !times || !lonRad || !latRad || !depthIntervals || !h || !zeta || !u || !v || !w || !akt)
+        return WC_ERR_INVALID;
+    if (particleCount < 0 || oceanTime <= 0 || timeIndex < 0 || timeIndex >= oceanTime || sW != sRho + 1 ||
+        sW < 2 || eta < 2 || xi < 2)
+        return WC_ERR_INVALID;
+    if (particleCount == 0) return WC_OK;
+    if (!particles) return WC_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    // config and times live in device memory, as in the reference call
+    wc_config_data cfg;
+    if (hipMemcpyAsync(&cfg, config, sizeof cfg, hipMemcpyDeviceToHost, st) != hipSuccess) return WC_ERR_HIP;
+    std::vector<double> t(oceanTime);
+    if (hipMemcpyAsync(t.data(), times, oceanTime * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return WC_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return WC_ERR_HIP;
+    int rc = check_supported(nullptr, &cfg);
+    if (rc) return rc;
+    // interval and record pair: Particle.cpp:110-115, 183-184
+    const int dir = cfg.trackingDirection;
+    int next = timeIndex + dir;
+    const double start = t[timeIndex];
+    const double end = (next >= 0 && next < oceanTime) ? t[next] : start + dir * cfg.deltat;
+    if (next < 0 || next >= oceanTime) next = timeIndex;
+    MoveParams P;
+    if (!make_params(&cfg, start, end, sW, sRho, eta, xi, P)) return WC_OK;
+    const size_t plane = (size_t)eta * xi;
+    RecordPair<float> F{zeta + plane * timeIndex, zeta + plane * next,
+                        u + plane * sRho * timeIndex, u + plane * sRho * next,
+                        v + plane * sRho * timeIndex, v + plane * sRho * next,
+                        w + plane * sW * timeIndex, w + plane * sW * next,
+                        akt + plane * sW * timeIndex, akt + plane * sW * next};
+    GridPtrs G{h, depthIntervals, nullptr, nullptr, lonRad, latRad, nullptr, nullptr, nullptr, nullptr};
+    return launch_move_parity_aos(P, F, G, particles, particleCount, nullptr, st) == hipSuccess ? WC_OK
+                                                                                                : WC_ERR_HIP;
+}
+
+int wc_hip_concentration(const wc_particle_data *particles, int particleCount, int s_rho, int eta, int xi,
+                         float *conc, void *stream) {
+    if (particleCount < 0 || s_rho < 1 || eta < 1 || xi < 1 || !conc) return WC_ERR_INVALID;
+    if ((size_t)s_rho * eta * xi > 0xFFFFFFFFull) return WC_ERR_INVALID;
+    if (particleCount == 0) return WC_OK;
+    if (!particles) return WC_ERR_INVALID;
+    return launch_grid_count_aos_float(particles, particleCount, s_rho, eta, xi, conc, (hipStream_t)stream) ==
+                   hipSuccess
+               ? WC_OK
+               : WC_ERR_HIP;
+}
+
+// ---- native context ----------------------------------------------------------
+
+int wc_ctx_create(int device, wc_ctx **out) {
+    if (!out) return WC_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return WC_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return WC_ERR_NO_DEVICE;
+    wc_ctx *c = new wc_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc(&c->d_total, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_steps, sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(c->d_steps, 0, sizeof(unsigned long long)) != hipSuccess) {
+        wc_ctx_destroy(c);
+        return WC_ERR_HIP;
+    }
+    *out = c;
+    return WC_OK;
+}
+
+static void free_grid(wc_ctx *c) {
+    hipFree(c->d_h); hipFree(c->d_depth); hipFree(c->d_dx); hipFree(c->d_dy); hipFree(c->d_mask);
+    c->d_h = c->d_depth = c->d_dx = c->d_dy = c->d_mask = nullptr;
+    hipFree(c->d_hf); hipFree(c->d_idxf); hipFree(c->d_idyf); hipFree(c->d_depthf);
+    c->d_hf = c->d_idxf = c->d_idyf = c->d_depthf = nullptr;
+    for (auto &s : c->slots)
+        for (int f = 0; f < 5; ++f) { hipFree(s.f[f]); hipFree(s.lp[f]); }
+    c->slots.clear();
+    hipFree(c->d_counts);
+    hipFree(c->d_conc);
+    c->d_counts = nullptr;
+    c->d_conc = nullptr;
+    c->haveGrid = false;
+}
+
+int wc_ctx_destroy(wc_ctx *c) {
+    if (!c) return WC_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_grid(c);
+    free_soa(c->s);
+    free_soa(c->s2);
+    hipFree(c->d_blockCounts);
+    hipFree(c->d_total);
+    hipFree(c->d_steps);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    for (int k = 0; k < 2; ++k)
+        for (auto &pr : c->pending[k]) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+    for (auto e : c->spare) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return WC_OK;
+}
+
+const char *wc_last_error(const wc_ctx *c) { return c ? c->err.c_str() : ""; }
+
+int wc_sync(wc_ctx *c) {
+    if (!c) return WC_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lonRad, const double *latRad,
+                const double *h, const double *mask, const double *depthIntervals, int n_slots, int store) {
+    if (!c) return WC_ERR_INVALID;
+    if (s_w != s_rho + 1 || s_rho < 1 || eta < 2 || xi < 2 || n_slots < 2 || n_slots > 4096 || !lonRad ||
+        !latRad || !h || !depthIntervals)
+        return fail(c, WC_ERR_INVALID, "invalid grid: s_w=%d s_rho=%d eta=%d xi=%d slots=%d", s_w, s_rho, eta, xi,
+                    n_slots);
+    if (store != WC_STORE_F32 && store != WC_STORE_F16 && store != WC_STORE_BF16)
+        return fail(c, WC_ERR_INVALID, "unknown storage type %d", store);
+    if ((size_t)s_w * eta * xi > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "grid too large");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_grid(c);
+    c->s_w = s_w; c->s_rho = s_rho; c->eta = eta; c->xi = xi; c->store = store;
+    const size_t plane = (size_t)eta * xi;
+    std::vector<double> dx(plane), dy(plane), m(plane, 1.0);
+    wc_cell_metric_table(eta, xi, lonRad, latRad, dx.data(), dy.data());
+    if (mask) std::memcpy(m.data(), mask, plane * sizeof(double));
+    HIPCHK(c, hipMalloc(&c->d_h, plane * 8));
+    HIPCHK(c, hipMalloc(&c->d_dx, plane * 8));
+    HIPCHK(c, hipMalloc(&c->d_dy, plane * 8));
+    HIPCHK(c, hipMalloc(&c->d_mask, plane * 8));
+    HIPCHK(c, hipMalloc(&c->d_depth, (size_t)(s_w - 1) * 8));
+    HIPCHK(c, hipMemcpy(c->d_h, h, plane * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_dx, dx.data(), plane * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_dy, dy.data(), plane * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_mask, m.data(), plane * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_depth, depthIntervals, (size_t)(s_w - 1) * 8, hipMemcpyHostToDevice));
+    // FP32 tables of the mixed-precision path
+    std::vector<float> hf(plane), idxf(plane), idyf(plane), depthf(s_w - 1);
+    for (size_t q = 0; q < plane; ++q) {
+        hf[q] = (float)h[q];
+        idxf[q] = dx[q] > 0 ? (float)(1.0 / dx[q]) : 0.0f;
+        idyf[q] = dy[q] > 0 ? (float)(1.0 / dy[q]) : 0.0f;
+    }
+    for (int q = 0; q < s_w - 1; ++q) depthf[q] = (float)depthIntervals[q];
+    HIPCHK(c, hipMalloc(&c->d_hf, plane * 4));
+    HIPCHK(c, hipMalloc(&c->d_idxf, plane * 4));
+    HIPCHK(c, hipMalloc(&c->d_idyf, plane * 4));
+    HIPCHK(c, hipMalloc(&c->d_depthf, (size_t)(s_w - 1) * 4));
+    HIPCHK(c, hipMemcpy(c->d_hf, hf.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_idxf, idxf.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_idyf, idyf.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_depthf, depthf.data(), (size_t)(s_w - 1) * 4, hipMemcpyHostToDevice));
+    c->slots.resize(n_slots);
+    for (auto &s : c->slots) {
+        for (int f = 0; f < 5; ++f) {
+            HIPCHK(c, hipMalloc(&s.f[f], field_count(c, f) * sizeof(float)));
+            if (store != WC_STORE_F32) HIPCHK(c, hipMalloc(&s.lp[f], field_count(c, f) * 2));
+        }
+    }
+    HIPCHK(c, hipMalloc(&c->d_counts, plane * s_rho * sizeof(uint32_t)));
+    c->haveGrid = true;
+    return WC_OK;
+}
+
+int wc_upload_record(wc_ctx *c, int slot, const float *zeta, const float *u, const float *v, const float *w,
+                     const float *akt) {
+    if (!c) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    if (slot < 0 || slot >= (int)c->slots.size()) return fail(c, WC_ERR_INVALID, "slot %d out of range", slot);
+    const float *src[5] = {zeta, u, v, w, akt};
+    for (int f = 0; f < 5; ++f)
+        if (!src[f]) return fail(c, WC_ERR_INVALID, "null forcing field %d", f);
+    HIPCHK(c, hipSetDevice(c->device));
+    auto &s = c->slots[slot];
+    for (int f = 0; f < 5; ++f) {
+        HIPCHK(c, hipMemcpyAsync(s.f[f], src[f], field_count(c, f) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        if (c->store != WC_STORE_F32)
+            HIPCHK(c, launch_convert_store(c->store, s.f[f], s.lp[f], field_count(c, f), c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s.filled = true;
+    return WC_OK;
+}
+
+int wc_set_particles(wc_ctx *c, const wc_particle_data *p, size_t n) {
+    if (!c || (n && !p)) return WC_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = reserve(c, n);
+    if (rc) return rc;
+    c->n = 0;
+    if (n) {
+        rc = upload_particles(c, p, n, 0);
+        if (rc) return rc;
+    }
+    c->n = n;
+    return WC_OK;
+}
+
+int wc_append_particles(wc_ctx *c, const wc_particle_data *p, size_t n) {
+    if (!c || (n && !p)) return WC_ERR_INVALID;
+    if (!n) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = reserve(c, c->n + n);
+    if (rc) return rc;
+    rc = upload_particles(c, p, n, c->n);
+    if (rc) return rc;
+    c->n += n;
+    return WC_OK;
+}
+
+int wc_get_particles(wc_ctx *c, wc_particle_data *p, size_t n) {
+    if (!c || (n && !p)) return WC_ERR_INVALID;
+    if (n > c->n) return fail(c, WC_ERR_INVALID, "asked for %zu particles, context holds %zu", n, c->n);
+    if (!n) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<uint64_t> id(n);
+    std::vector<double> f[6];
+    for (auto &v : f) v.resize(n);
+    const double *src[6] = {c->s.k, c->s.j, c->s.i, c->s.health, c->s.age, c->s.time};
+    HIPCHK(c, hipMemcpyAsync(id.data(), c->s.id, n * 8, hipMemcpyDeviceToHost, c->stream));
+    for (int q = 0; q < 6; ++q)
+        HIPCHK(c, hipMemcpyAsync(f[q].data(), src[q], n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < n; ++q) {
+        wc_particle_data d{};
+        d.id = id[q];
+        d.k = f[0][q]; d.j = f[1][q]; d.i = f[2][q];
+        d.health = f[3][q]; d.age = f[4][q]; d.time = f[5][q];
+        d.driftObjectType = 0;
+        d.driftSide = 0;
+        p[q] = d;
+    }
+    return WC_OK;
+}
+
+int wc_particle_count(const wc_ctx *c, size_t *n) {
+    if (!c || !n) return WC_ERR_INVALID;
+    *n = c->n;
+    return WC_OK;
+}
+
+int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double end, int slot0, int slot1,
+                     int path) {
+    if (!c || !cfg) return WC_ERR_INVALID;
+    int rc = check_supported(c, cfg);
+    if (rc) return rc;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    const int ns = (int)c->slots.size();
+    if (slot0 < 0 || slot0 >= ns || slot1 < 0 || slot1 >= ns || !c->slots[slot0].filled || !c->slots[slot1].filled)
+        return fail(c, WC_ERR_INVALID, "forcing slots %d/%d not uploaded", slot0, slot1);
+    if (path != WC_PATH_PARITY && path != WC_PATH_FAST) return fail(c, WC_ERR_INVALID, "unknown path %d", path);
+    if (!std::isfinite(start) || !std::isfinite(end)) return fail(c, WC_ERR_INVALID, "non-finite interval");
+    MoveParams P;
+    if (!make_params(cfg, start, end, c->s_w, c->s_rho, c->eta, c->xi, P)) return WC_OK;
+    if (c->n == 0) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const auto &a = c->slots[slot0], &b = c->slots[slot1];
+    GridPtrs G{c->d_h, c->d_depth, c->d_dx, c->d_dy, nullptr, nullptr, c->d_hf, c->d_idxf, c->d_idyf, c->d_depthf};
+    timing_begin(c);
+    if (path == WC_PATH_PARITY) {
+        RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
+        HIPCHK(c, launch_move_parity_soa(P, F, G, c->s, (int)c->n, c->d_steps, c->stream));
+    } else {
+        const void *pair[10];
+        for (int f = 0; f < 5; ++f) {
+            pair[2 * f] = c->store == WC_STORE_F32 ? (const void *)a.f[f] : a.lp[f];
+            pair[2 * f + 1] = c->store == WC_STORE_F32 ? (const void *)b.f[f] : b.lp[f];
+        }
+        HIPCHK(c, launch_move_fast_soa(c->store, P, pair, G, c->s, (int)c->n, c->d_steps, c->stream));
+    }
+    return timing_end(c, 0);
+}
+
+int wc_remove_dead(wc_ctx *c, size_t *remaining) {
+    if (!c) return WC_ERR_INVALID;
+    if (c->n == 0) {
+        if (remaining) *remaining = 0;
+        return WC_OK;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_compact(c->s, c->s2, (int)c->n, c->d_blockCounts, c->d_total, c->stream));
+    unsigned long long total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, c->d_total, sizeof total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::swap(c->s, c->s2);
+    c->n = (size_t)total;
+    if (remaining) *remaining = c->n;
+    return WC_OK;
+}
+
+int wc_grid(wc_ctx *c, uint32_t *counts_out, float *conc_out, int mask_output) {
+    if (!c) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t plane = (size_t)c->eta * c->xi, total = plane * c->s_rho;
+    timing_begin(c);
+    HIPCHK(c, launch_fill_u32(c->d_counts, total, 0u, c->stream));
+    HIPCHK(c, launch_grid_count_soa(c->s, (int)c->n, c->s_rho, c->eta, c->xi, c->d_counts, c->stream));
+    int rc = timing_end(c, 1);
+    if (rc) return rc;
+    if (counts_out)
+        HIPCHK(c, hipMemcpyAsync(counts_out, c->d_counts, total * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if (conc_out) {
+        if (!c->d_conc) HIPCHK(c, hipMalloc(&c->d_conc, total * sizeof(float)));
+        HIPCHK(c, launch_counts_to_conc(c->d_counts, c->d_mask, mask_output, c->s_rho, plane, c->d_conc, c->stream));
+        HIPCHK(c, hipMemcpyAsync(conc_out, c->d_conc, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (counts_out || conc_out) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+int wc_set_timing(wc_ctx *c, int enabled) {
+    if (!c) return WC_ERR_INVALID;
+    c->timing = enabled != 0;
+    return WC_OK;
+}
+
+int wc_kernel_time(wc_ctx *c, int kernel, double *total_ms, long *launches) {
+    if (!c || kernel < 0 || kernel > 1) return WC_ERR_INVALID;
+    int rc = timing_collect(c);
+    if (rc) return rc;
+    if (total_ms) *total_ms = c->ms[kernel];
+    if (launches) *launches = c->launches[kernel];
+    return WC_OK;
+}
+
+int wc_reset_kernel_time(wc_ctx *c) {
+    if (!c) return WC_ERR_INVALID;
+    int rc = timing_collect(c);
+    if (rc) return rc;
+    c->ms[0] = c->ms[1] = 0;
+    c->launches[0] = c->launches[1] = 0;
+    return WC_OK;
+}
+
+int wc_step_count(wc_ctx *c, uint64_t *steps) {
+    if (!c || !steps) return WC_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned long long v = 0;
+    HIPCHK(c, hipMemcpyAsync(&v, c->d_steps, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *steps = v;
+    return WC_OK;
+}
+
+int wc_reset_step_count(wc_ctx *c) {
+    if (!c) return WC_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Device-side data structures shared by the move and gridding kernels of
+// libwacomm_hip.so (gfx950).  The parity and gridding sources are compiled with
+// contraction off (-ffp-contract=off plus a file pragma) so that every
+// rounding point of the reference's Particle::move is kept; the mixed-precision
+// source enables contraction explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wacomm_hip.h"
+
+namespace wc {
+
+// Scalars of one forcing interval, resolved on the host from config_data and
+// the interval bounds (Particle.cpp:81-125, 507, 523).  Passed by value.
+struct MoveParams {
+    double dti, tau0, survprob, crid, sv, sigma, shoreLimit;
+    double intervalStart;   // physical time of the interval start
+    double intervalLength;  // |end - start|
+    double elapsed0;        // restart-clipped start offset (Particle.cpp:116-120)
+    double kLowerLimit;     // -s_w + 2
+    double direction;       // trackingDirection as double
+    double ageKill;         // exp(-age/tau0) < survprob  <=>  age > ageKill (mixed-precision path)
+    uint64_t seed;
+    int64_t intervalKey;    // llround(min(start, end))
+    int upper, lower, horizontal;
+    int dirInt;
+    int randomOn;           // random && (direction > 0 || backwardDiffusion)
+    int midpoint;
+    int s_w, s_rho, eta, xi;
+};
+
+// The two forcing records of the interval (record at the interval start and
+// at its end), laid out per record as zeta [eta][xi], u/v [s_rho][eta][xi],
+// w/akt [s_w][eta][xi].
+template <class T>
+struct RecordPair {
+    const T *zeta0, *zeta1, *u0, *u1, *v0, *v1, *w0, *w1, *akt0, *akt1;
+};
+
+// Static grid: h [eta][xi] (FP64), depthIntervals [s_w-1], and either the
+// per-cell haversine sizes dx, dy [eta][xi] or lon/lat to compute them.
+// The mixed-precision path reads FP32 copies: h, 1/dx, 1/dy, depthIntervals.
+struct GridPtrs {
+    const double *h, *depth, *dx, *dy, *lonRad, *latRad;
+    const float *hf, *idxf, *idyf, *depthf;
+};
+
+// Particle state, structure of arrays (the context's canonical layout).
+struct SoA {
+    uint64_t *id;
+    double *k, *j, *i, *health, *age, *time;
+};
+
+struct State {
+    uint64_t id;
+    double k, j, i, health, age, time;
+};
+
+struct SoAAccess {
+    SoA s;
+    __device__ __forceinline__ void load(int idx, State &p) const {
+        p.id = s.id[idx]; p.k = s.k[idx]; p.j = s.j[idx]; p.i = s.i[idx];
+        p.health = s.health[idx]; p.age = s.age[idx]; p.time = s.time[idx];
+    }
+    __device__ __forceinline__ void store(int idx, const State &p) const {
+        s.k[idx] = p.k; s.j[idx] = p.j; s.i[idx] = p.i; s.health[idx] = p.health; s.age[idx] = p.age;
+    }
+};
+
+struct AoSAccess {
+    wc_particle_data *p;
+    __device__ __forceinline__ void load(int idx, State &q) const {
+        const wc_particle_data &d = p[idx];
+        q.id = d.id; q.k = d.k; q.j = d.j; q.i = d.i; q.health = d.health; q.age = d.age; q.time = d.time;
+    }
+    __device__ __forceinline__ void store(int idx, const State &q) const {
+        wc_particle_data &d = p[idx];
+        d.k = q.k; d.j = q.j; d.i = q.i; d.health = q.health; d.age = q.age;
+    }
+};
+
+// SplitMix64 finaliser and ordered key combination, scheme
+// ordered-splitmix64-v2 (NumericalHelpers.hpp:84-112).
+__device__ __forceinline__ uint64_t mix64(uint64_t v) {
+    v += 0x9e3779b97f4a7c15ULL;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+}
+
+// Sum of a 64-bit value over the wavefront, added once per wave to *counter.
+__device__ __forceinline__ void wave_add_u64(unsigned long long *counter, unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(counter, v);
+}
+
+}  // namespace wc

@@ -1,0 +1,209 @@
+// Concentration gridding and dead-particle compaction for gfx950.
+//
+// Gridding (Concentration.hpp:16-44): each alive particle's (int)round(k, j, i)
+// -- round half away from zero -- names a cell of the [s_rho][eta][xi] frame.
+// Instead of one float atomic per particle (the reference's omp atomic), each
+// 64-wide wavefront groups its lanes by cell with ballots (a segmented
+// reduction over equal keys) and one lane adds the group's size with a single
+// uint32 atomic.  Particles released together stay clustered, so a wave holds
+// few distinct cells and issues few atomics; integer adds make the result
+// exact and independent of scheduling.
+//
+// Compaction (Wacomm.cpp:710-713, erase(remove_if(!isAlive))): order-preserving
+// stream compaction of the SoA state: block counts -> exclusive scan -> scatter
+// with wave-ballot prefix sums inside each block.
+#include "wc_device.h"
+
+#pragma clang fp contract(off)
+
+namespace wc {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Number of active lanes below this lane in a 64-bit ballot mask.
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+    return __popcll(mask & ((1ULL << lane_id()) - 1ULL));
+}
+
+// Out is uint32_t (exact counts, the context path) or float (the drop-in path,
+// which adds into the caller's float frame like the reference does).
+template <class Acc, class Out>
+__global__ __launch_bounds__(kBlock) void grid_count_kernel(Acc acc, int n, int s_rho, int eta, int xi,
+                                                            Out *__restrict__ counts) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = false;
+    uint32_t key = 0;
+    if (idx < n) {
+        double k, j, i, health;
+        acc.position(idx, k, j, i, health);
+        if (health > 0) {                                        // Particle::isAlive
+            const double kr = round(k), jr = round(j), ir = round(i);   // Concentration.hpp:31-33
+            if (jr >= 0 && jr < eta && ir >= 0 && ir < xi && kr >= -s_rho + 1 && kr <= 0) {   // :36
+                key = (uint32_t)((((int)kr + s_rho - 1) * (size_t)eta + (int)jr) * xi + (int)ir);
+                active = true;
+            }
+        }
+    }
+    // wave-level segmented count over equal keys; the loop trip count is
+    // wave-uniform (it depends only on ballots)
+    for (;;) {
+        const unsigned long long m = __ballot(active);
+        if (m == 0) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const bool same = active && key == lk;
+        const unsigned long long sm = __ballot(same);
+        if (lane_id() == leader) atomicAdd(&counts[lk], (Out)__popcll(sm));
+        if (same) active = false;
+    }
+}
+
+struct SoAPos {
+    const double *k, *j, *i, *health;
+    __device__ __forceinline__ void position(int idx, double &K, double &J, double &I, double &H) const {
+        K = k[idx]; J = j[idx]; I = i[idx]; H = health[idx];
+    }
+};
+struct AoSPos {
+    const wc_particle_data *p;
+    __device__ __forceinline__ void position(int idx, double &K, double &J, double &I, double &H) const {
+        K = p[idx].k; J = p[idx].j; I = p[idx].i; H = p[idx].health;
+    }
+};
+
+__global__ void fill_u32_kernel(uint32_t *__restrict__ a, size_t n, uint32_t v) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        a[q] = v;
+}
+
+// counts -> the reference's float frame: +1.0f repeated `count` times from
+// zero saturates at 2^24; land cells (mask != 1) get 1e37 when masking
+// (Wacomm.cpp:724-739).
+__global__ void counts_to_conc_kernel(const uint32_t *__restrict__ counts, const double *__restrict__ mask,
+                                      int mask_output, int s_rho, size_t plane, float *__restrict__ conc) {
+    const size_t total = plane * s_rho;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t c = counts[q];
+        float v = c > 16777216u ? 16777216.0f : (float)c;
+        if (mask_output && mask[q % plane] != 1.0) v = 1e37f;
+        conc[q] = v;
+    }
+}
+
+// ---- compaction ----------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void alive_count_kernel(const double *__restrict__ health, int n,
+                                                             uint32_t *__restrict__ blockCounts) {
+    __shared__ uint32_t waveCount[kWaves];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool alive = idx < n && health[idx] > 0;
+    const unsigned long long m = __ballot(alive);
+    if (lane_id() == 0) waveCount[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < kWaves; ++w) s += waveCount[w];
+        blockCounts[blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of nb block counts in place by one block of 1024 threads;
+// total[0] receives the sum.
+__global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t *__restrict__ counts, int nb,
+                                                           unsigned long long *__restrict__ total) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per;
+    uint32_t s = 0;
+    for (int q = 0; q < per && b0 + q < nb; ++q) s += counts[b0 + q];
+    part[t] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t == 0 ? 0 : part[t - 1];
+    for (int q = 0; q < per && b0 + q < nb; ++q) {
+        const uint32_t c = counts[b0 + q];
+        counts[b0 + q] = run;
+        run += c;
+    }
+    if (t == 1023) total[0] = part[1023];
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_alive_kernel(SoA src, SoA dst, int n,
+                                                               const uint32_t *__restrict__ blockOffsets) {
+    __shared__ uint32_t waveBase[kWaves];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool alive = idx < n && src.health[idx] > 0;
+    const unsigned long long m = __ballot(alive);
+    if (lane_id() == 0) waveBase[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = blockOffsets[blockIdx.x];
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) base += waveBase[w];
+    if (alive) {
+        const uint32_t o = base + (uint32_t)lanes_below(m);
+        dst.id[o] = src.id[idx]; dst.k[o] = src.k[idx]; dst.j[o] = src.j[idx]; dst.i[o] = src.i[idx];
+        dst.health[o] = src.health[idx]; dst.age[o] = src.age[idx]; dst.time[o] = src.time[idx];
+    }
+}
+
+int grid_blocks(size_t n) {
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b > 65536) b = 65536;
+    return (int)(b ? b : 1);
+}
+
+}  // namespace
+
+hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    fill_u32_kernel<<<grid_blocks(n), kBlock, 0, stream>>>(a, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi, uint32_t *counts,
+                                 hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    grid_count_kernel<SoAPos, uint32_t><<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+        SoAPos{s.k, s.j, s.i, s.health}, n, s_rho, eta, xi, counts);
+    return hipGetLastError();
+}
+
+// Adds each cell's count of this call into a float frame: exact while every
+// cell stays at or below 2^24.
+hipError_t launch_grid_count_aos_float(const wc_particle_data *p, int n, int s_rho, int eta, int xi, float *conc,
+                                       hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    grid_count_kernel<AoSPos, float><<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(AoSPos{p}, n, s_rho, eta,
+                                                                                      xi, conc);
+    return hipGetLastError();
+}
+
+hipError_t launch_counts_to_conc(const uint32_t *counts, const double *mask, int mask_output, int s_rho,
+                                 size_t plane, float *conc, hipStream_t stream) {
+    counts_to_conc_kernel<<<grid_blocks(plane * s_rho), kBlock, 0, stream>>>(counts, mask, mask_output, s_rho,
+                                                                             plane, conc);
+    return hipGetLastError();
+}
+
+// blockCounts must hold ceil(n/256) entries; total a device u64.
+hipError_t launch_compact(const SoA &src, const SoA &dst, int n, uint32_t *blockCounts,
+                          unsigned long long *total, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + kBlock - 1) / kBlock;
+    alive_count_kernel<<<nb, kBlock, 0, stream>>>(src.health, n, blockCounts);
+    scan_blocks_kernel<<<1, 1024, 0, stream>>>(blockCounts, nb, total);
+    scatter_alive_kernel<<<nb, kBlock, 0, stream>>>(src, dst, n, blockCounts);
+    return hipGetLastError();
+}
+
+}  // namespace wc

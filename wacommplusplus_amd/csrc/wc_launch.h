@@ -1,0 +1,29 @@
+// Host-side launchers of the device kernels (internal to libwacomm_hip.so).
+#pragma once
+
+#include "wc_device.h"
+
+namespace wc {
+
+hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, const SoA &s,
+                                  int n, unsigned long long *stepCounter, hipStream_t stream);
+hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
+                                  wc_particle_data *p, int n, unsigned long long *stepCounter, hipStream_t stream);
+
+// Mixed-precision path; T is float, __half or __hip_bfloat16 storage (store code WC_STORE_*).
+hipError_t launch_move_fast_soa(int store, const MoveParams &P, const void *const *recordPair, const GridPtrs &G,
+                                const SoA &s, int n, unsigned long long *stepCounter, hipStream_t stream);
+// FP32 -> 16-bit storage conversion of one record array.
+hipError_t launch_convert_store(int store, const float *src, void *dst, size_t n, hipStream_t stream);
+
+hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream);
+hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi, uint32_t *counts,
+                                 hipStream_t stream);
+hipError_t launch_grid_count_aos_float(const wc_particle_data *p, int n, int s_rho, int eta, int xi, float *conc,
+                                       hipStream_t stream);
+hipError_t launch_counts_to_conc(const uint32_t *counts, const double *mask, int mask_output, int s_rho,
+                                 size_t plane, float *conc, hipStream_t stream);
+hipError_t launch_compact(const SoA &src, const SoA &dst, int n, uint32_t *blockCounts, unsigned long long *total,
+                          hipStream_t stream);
+
+}  // namespace wc
