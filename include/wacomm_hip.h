@@ -208,6 +208,52 @@ int wc_remove_dead(wc_ctx *ctx, size_t *remaining);
  * and 1e37 on land cells (mask != 1) when mask_output is nonzero. */
 int wc_grid(wc_ctx *ctx, uint32_t *counts_out, float *conc_out, int mask_output);
 
+/* A release point (Source.hpp) and its schedule (SourceEmission.hpp:12-16).
+ * start/end are seconds on the forcing time axis (end < 0: open-ended). */
+typedef struct wc_source {
+    double k, j, i;
+    int mode;          /* 0 uniform_rate, 1 forcing_interval_batch, 2 single_pulse */
+    double value;      /* particles per hour / per forcing record / in the pulse */
+    double start, end;
+} wc_source;
+
+/* Replaces Sources::emit (Sources.cpp:233-248), Source::emit (Source.cpp:25-44)
+ * and sourceEmissionTimes (SourceEmission.hpp:38-79): appends the particles
+ * released in [intervalStart, intervalEnd) in time order, ids continuing from
+ * the last particle's id (0 when empty); with config->randomSources each
+ * coordinate gets 0.25*normal(seed, id, SOURCE_POSITION_RANDOM_INTERVAL, 0, c)
+ * (SourceEmission.hpp:18-23).  *emitted receives the count.  Returns
+ * WC_ERR_INVALID for an invalid schedule (where the reference throws). */
+int wc_emit(wc_ctx *ctx, const wc_config_data *config, const wc_source *sources, int n_sources,
+            double intervalStart, double intervalEnd, size_t *emitted);
+
+/* A forcing window on the host (oceanmodel_data, OceanModelAdapter.hpp:20-34):
+ * T records, zeta [T][eta][xi], u/v [T][s_rho][eta][xi], w/akt [T][s_w][eta][xi],
+ * oceanTime [T].  The grid must match the context's wc_set_grid. */
+typedef struct wc_window {
+    int T;
+    const double *oceanTime;
+    const float *zeta, *u, *v, *w, *akt;
+} wc_window;
+
+typedef struct wc_run_stats {
+    double solver_s;          /* wall time of the interval loop */
+    double move_ms;           /* device time of the move kernels */
+    uint64_t particle_steps;  /* live substeps taken */
+    size_t particles;         /* particles alive at the end */
+    int intervals;            /* intervals processed */
+} wc_run_stats;
+
+/* The interval loop of Wacomm::run (Wacomm.cpp:200-770) for one window:
+ * for each interval in the tracking direction that is active under the
+ * restart checkpoint (NumericalHelpers.hpp:54-60): emit (forward tracking,
+ * emitAtIntervalStart), move, remove dead, grid into frame idx+direction.
+ * Records stream through a two-slot ring (one upload per interval).  conc,
+ * when not NULL, is the host [T][s_rho][eta][xi] float array of
+ * Wacomm.cpp:172-186 (zeroed here), with land cells = 1e37 when mask_output. */
+int wc_run_window(wc_ctx *ctx, const wc_config_data *config, const wc_window *window, const wc_source *sources,
+                  int n_sources, int path, float *conc, int mask_output, wc_run_stats *stats);
+
 /* Accumulated device time of the move kernels launched since the last reset,
  * measured with HIP events on the context stream around each launch (enable
  * with wc_set_timing).  kernel: 0 = move, 1 = grid. */

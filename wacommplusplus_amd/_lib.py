@@ -25,8 +25,27 @@ EXPORTS = (
     "wc_sync", "wc_set_grid", "wc_upload_record", "wc_set_particles", "wc_append_particles",
     "wc_get_particles", "wc_particle_count", "wc_move_interval", "wc_remove_dead", "wc_grid",
     "wc_set_timing", "wc_kernel_time", "wc_reset_kernel_time", "wc_step_count", "wc_reset_step_count",
-    "wc_version", "wc_cell_metric_table", "wc_philox4x32_10",
+    "wc_version", "wc_cell_metric_table", "wc_philox4x32_10", "wc_emit", "wc_run_window",
 )
+
+
+class Source(C.Structure):
+    """wc_source: release point and schedule (Source.hpp, SourceEmission.hpp:12-16)."""
+    _fields_ = [("k", C.c_double), ("j", C.c_double), ("i", C.c_double), ("mode", C.c_int),
+                ("value", C.c_double), ("start", C.c_double), ("end", C.c_double)]
+
+
+UNIFORM_RATE, FORCING_INTERVAL_BATCH, SINGLE_PULSE = 0, 1, 2
+
+
+class WindowDesc(C.Structure):
+    _fields_ = [("T", C.c_int), ("oceanTime", C.c_void_p), ("zeta", C.c_void_p), ("u", C.c_void_p),
+                ("v", C.c_void_p), ("w", C.c_void_p), ("akt", C.c_void_p)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("solver_s", C.c_double), ("move_ms", C.c_double), ("particle_steps", C.c_uint64),
+                ("particles", C.c_size_t), ("intervals", C.c_int)]
 
 
 class WcError(RuntimeError):
@@ -70,6 +89,8 @@ def load_library(path: str = LIB_PATH):
         "wc_reset_kernel_time": [vp],
         "wc_step_count": [vp, C.POINTER(C.c_uint64)],
         "wc_reset_step_count": [vp],
+        "wc_emit": [vp, C.POINTER(ConfigData), vp, i, d, d, C.POINTER(sz)],
+        "wc_run_window": [vp, C.POINTER(ConfigData), C.POINTER(WindowDesc), vp, i, i, vp, i, C.POINTER(RunStats)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -195,6 +216,28 @@ class Context:
 
     def sync(self):
         self._chk(self.L.wc_sync(self.h))
+
+    def emit(self, config: ConfigData, sources, start: float, end: float) -> int:
+        """Sources::emit for one interval (Sources.cpp:233-248)."""
+        arr = (Source * max(1, len(sources)))(*sources)
+        n = C.c_size_t()
+        self._chk(self.L.wc_emit(self.h, C.byref(config), arr, len(sources), float(start), float(end), C.byref(n)))
+        return n.value
+
+    def run_window(self, config: ConfigData, times, zeta, u, v, w, akt, sources=(), path="parity",
+                   conc=True, mask_output=False):
+        """The interval loop of Wacomm::run over a host window (Wacomm.cpp:200-770).
+        Returns (conc [T][s_rho][eta][xi] or None, RunStats)."""
+        t = _f64(times)
+        arrs = [_f32(a) for a in (zeta, u, v, w, akt)]
+        desc = WindowDesc(len(t), _p(t).value, *[_p(a).value for a in arrs])
+        src = (Source * max(1, len(sources)))(*sources)
+        s_rho, eta, xi = self.shape
+        out = np.zeros((len(t), s_rho, eta, xi), np.float32) if conc else None
+        st = RunStats()
+        self._chk(self.L.wc_run_window(self.h, C.byref(config), C.byref(desc), src, len(sources), PATH[path],
+                                       None if out is None else _p(out), int(mask_output), C.byref(st)))
+        return out, st
 
     # -- measurement --------------------------------------------------------
     def set_timing(self, on: bool = True):

@@ -2,6 +2,8 @@
 // Declarations and contracts: include/wacomm_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -35,6 +37,7 @@ struct wc_ctx {
         float *f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};      // zeta, u, v, w, akt
         void *lp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
         bool filled = false;
+        int record = -1;      // window record held by the slot (wc_run_window)
     };
     std::vector<Slot> slots;
 
@@ -491,6 +494,7 @@ int wc_upload_record(wc_ctx *c, int slot, const float *zeta, const float *u, con
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     s.filled = true;
+    s.record = -1;
     return WC_OK;
 }
 
@@ -618,6 +622,120 @@ int wc_grid(wc_ctx *c, uint32_t *counts_out, float *conc_out, int mask_output) {
         HIPCHK(c, hipMemcpyAsync(conc_out, c->d_conc, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     }
     if (counts_out || conc_out) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+int wc_emit(wc_ctx *c, const wc_config_data *cfg, const wc_source *sources, int n_sources, double start,
+            double end, size_t *emitted) {
+    if (emitted) *emitted = 0;
+    if (!c || !cfg || n_sources < 0 || (n_sources && !sources)) return WC_ERR_INVALID;
+    struct Emission {
+        double time;
+        int source;
+    };
+    std::vector<Emission> em;
+    std::vector<double> t;
+    for (int q = 0; q < n_sources; ++q) {
+        if (!emission_times(sources[q].mode, sources[q].value, sources[q].start, sources[q].end, start, end, t))
+            return fail(c, WC_ERR_INVALID, "invalid emission schedule of source %d", q);
+        for (double x : t) em.push_back({x, q});
+    }
+    std::stable_sort(em.begin(), em.end(), [](const Emission &a, const Emission &b) { return a.time < b.time; });
+    if (em.empty()) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    uint64_t firstId = 0;
+    if (c->n) {                                               // Source.cpp:27: last id + 1
+        HIPCHK(c, hipMemcpy(&firstId, c->s.id + (c->n - 1), 8, hipMemcpyDeviceToHost));
+        firstId += 1;
+    }
+    const size_t m = em.size();
+    int rc = reserve(c, c->n + m);
+    if (rc) return rc;
+    std::vector<double> times(m), pos(3 * (size_t)n_sources);
+    std::vector<int> src(m);
+    for (size_t q = 0; q < m; ++q) { times[q] = em[q].time; src[q] = em[q].source; }
+    for (int q = 0; q < n_sources; ++q) {
+        pos[3 * q] = sources[q].k; pos[3 * q + 1] = sources[q].j; pos[3 * q + 2] = sources[q].i;
+    }
+    double *d_times = nullptr, *d_pos = nullptr;
+    int *d_src = nullptr;
+    HIPCHK(c, hipMalloc(&d_times, m * 8));
+    HIPCHK(c, hipMalloc(&d_src, m * 4));
+    HIPCHK(c, hipMalloc(&d_pos, pos.size() * 8));
+    HIPCHK(c, hipMemcpy(d_times, times.data(), m * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d_src, src.data(), m * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d_pos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, launch_emit(c->s, c->n, (int)m, d_times, d_src, d_pos, firstId, cfg->randomSeed,
+                          cfg->randomSources ? 1 : 0, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipFree(d_times); hipFree(d_src); hipFree(d_pos);
+    c->n += m;
+    if (emitted) *emitted = m;
+    return WC_OK;
+}
+
+int wc_run_window(wc_ctx *c, const wc_config_data *cfg, const wc_window *win, const wc_source *sources,
+                  int n_sources, int path, float *conc, int mask_output, wc_run_stats *stats) {
+    if (!c || !cfg || !win || !win->oceanTime || win->T < 1) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    int rc = check_supported(c, cfg);
+    if (rc) return rc;
+    const int T = win->T, dir = cfg->trackingDirection;
+    const size_t plane = (size_t)c->eta * c->xi, frame = plane * c->s_rho;
+    const size_t nf[5] = {plane, plane * c->s_rho, plane * c->s_rho, plane * c->s_w, plane * c->s_w};
+    const float *fields[5] = {win->zeta, win->u, win->v, win->w, win->akt};
+    if (conc) std::memset(conc, 0, frame * T * sizeof(float));
+    wc_run_stats st{};
+    const bool wasTiming = c->timing;
+    c->timing = true;
+    rc = timing_collect(c);
+    if (rc) return rc;
+    const double ms0 = c->ms[0];
+    uint64_t steps0 = 0;
+    rc = wc_step_count(c, &steps0);
+    if (rc) return rc;
+    auto ensure = [&](int rec) -> int {                       // record rec -> slot rec % 2
+        auto &s = c->slots[rec % 2];
+        if (s.filled && s.record == rec) return WC_OK;
+        const float *src[5];
+        for (int f = 0; f < 5; ++f) src[f] = fields[f] + nf[f] * rec;
+        int r = wc_upload_record(c, rec % 2, src[0], src[1], src[2], src[3], src[4]);
+        if (r) return r;
+        s.record = rec;
+        return WC_OK;
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    const int first = dir < 0 ? T - 1 : 0;                   // Wacomm.cpp:200-206
+    for (int idx = first; idx >= 0 && idx < T && idx + dir >= 0 && idx + dir < T; idx += dir) {
+        const double start = win->oceanTime[idx], end = win->oceanTime[idx + dir];
+        const double ck = cfg->restartCheckpoint;
+        if (std::isfinite(ck) && std::isnan(restart_elapsed(start, end, ck))) continue;   // activeInterval
+        if (dir > 0 && (!std::isfinite(ck) || start >= ck) && n_sources > 0) {            // :264-269
+            rc = wc_emit(c, cfg, sources, n_sources, start, end, nullptr);
+            if (rc) break;
+        }
+        if ((rc = ensure(idx)) || (rc = ensure(idx + dir))) break;
+        rc = wc_move_interval(c, cfg, start, end, idx % 2, (idx + dir) % 2, path);
+        if (rc) break;
+        size_t remaining = 0;
+        if ((rc = wc_remove_dead(c, &remaining))) break;     // :710-713
+        if ((rc = wc_grid(c, nullptr, conc ? conc + frame * (idx + dir) : nullptr, mask_output))) break;   // :722-739
+        st.intervals += 1;
+    }
+    st.solver_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc) {
+        c->timing = wasTiming;
+        return rc;
+    }
+    rc = timing_collect(c);
+    c->timing = wasTiming;
+    if (rc) return rc;
+    uint64_t steps1 = 0;
+    if ((rc = wc_step_count(c, &steps1))) return rc;
+    st.move_ms = c->ms[0] - ms0;
+    st.particle_steps = steps1 - steps0;
+    st.particles = c->n;
+    if (stats) *stats = st;
     return WC_OK;
 }
 

@@ -1,6 +1,8 @@
 // Host-side launchers of the device kernels (internal to libwacomm_hip.so).
 #pragma once
 
+#include <vector>
+
 #include "wc_device.h"
 
 namespace wc {
@@ -25,5 +27,10 @@ hipError_t launch_counts_to_conc(const uint32_t *counts, const double *mask, int
                                  size_t plane, float *conc, hipStream_t stream);
 hipError_t launch_compact(const SoA &src, const SoA &dst, int n, uint32_t *blockCounts, unsigned long long *total,
                           hipStream_t stream);
+
+hipError_t launch_emit(const SoA &s, size_t at, int n, const double *times, const int *src, const double *pos,
+                       uint64_t firstId, uint64_t seed, int randomSources, hipStream_t stream);
+bool emission_times(int mode, double value, double sourceStart, double sourceEnd, double intervalStart,
+                    double intervalEnd, std::vector<double> &out);
 
 }  // namespace wc
