@@ -53,7 +53,6 @@ STATE_BYTES = 96   # FP64 SoA: read id,k,j,i,health,age,time (56 B) + write k,j,
 
 def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
     """Warmup intervals, then `steps` timed intervals from t0 (records resident)."""
-    import torch
     for w in range(warmup):
         ctx.set_particles(p0)
         ctx.move_interval(cfg, float(W.times[w]), float(W.times[w + 1]), w, w + 1, path=path)
@@ -65,7 +64,7 @@ def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
     ctx.reset_kernel_time()
     ctx.reset_step_count()
     ctx.set_timing(True)
-    torch.cuda.synchronize()
+    ctx.sync()
     t0 = time.perf_counter()
     for s in range(steps):
         ctx.move_interval(cfg, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
@@ -74,7 +73,6 @@ def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
         if s % 6 == 5:
             log(f"  {progress} interval {s + 1}/{steps}")
     ctx.sync()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ctx.set_timing(False)
     move_ms, move_n = ctx.kernel_time(0)
@@ -123,7 +121,6 @@ def main():
                                    "in DESIGN.md and not run", "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     assert 1 <= args.steps <= 24, "the window holds 24 intervals"
-    import torch
     import wacommplusplus_amd as wc
     from wacommplusplus_amd.synthetic import full_window, release
 
@@ -148,6 +145,32 @@ def main():
         results[key] = run_timed(wc, ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key)
         log(f"[{key}] {results[key]}")
 
+    # precision of the mixed-precision path against the FP64 path (diffusion
+    # off, same forcing and release, `steps` intervals, whole population)
+    precision = {}
+    if not args.no_variants:
+        det = cfg.copy(random=False)
+        finals = {}
+        for path, store in [("parity", "f32"), ("fast", "bf16"), ("fast", "f16"), ("fast", "f32")]:
+            key = "f64" if path == "parity" else store
+            ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=n_slots,
+                         store=store)
+            for t in range(n_slots):
+                ctx.upload_record(t, *W.record(t))
+            ctx.set_particles(p0)
+            for s in range(args.steps):
+                ctx.move_interval(det, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
+            finals[key] = ctx.get_particles()
+            log(f"[precision] {key} done")
+        ref = finals.pop("f64")
+        for key, a in finals.items():
+            both = (a["health"] > 0) & (ref["health"] > 0)
+            d = np.stack([np.abs(a[f][both] - ref[f][both]) for f in ("k", "j", "i")])
+            precision[key] = dict(max_cells=float(d.max()), rms_cells=float(np.sqrt((d ** 2).mean())),
+                                  p99_cells=float(np.quantile(d.max(axis=0), 0.99)),
+                                  both_alive=float(both.mean()))
+        log(f"[precision] {precision}")
+
     # PCIe-inclusive rate: two resident slots, one record uploaded per interval
     pcie = None
     if not args.no_variants:
@@ -155,7 +178,7 @@ def main():
         ctx.set_particles(p0)
         ctx.upload_record(0, *W.record(0))
         ctx.reset_step_count()
-        torch.cuda.synchronize()
+        ctx.sync()
         t0 = time.perf_counter()
         for s in range(args.steps):
             ctx.upload_record((s + 1) % 2, *W.record(s + 1))
@@ -220,6 +243,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "variants": variants_out,
+        "precision_vs_f64": {"hours": args.steps, "diffusion": "off", "particles": args.particles,
+                             **precision},
         "pcie_inclusive_value": pcie,
         "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),
         "alive_after": main_r["alive"],

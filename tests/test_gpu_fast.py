@@ -1,0 +1,126 @@
+"""GPU tests of the mixed-precision path (WC_PATH_FAST): integer cell + FP32
+offset state, FP32 interpolation of FP32/FP16/BF16 forcing, Philox4x32-10
+diffusion draws.  It is not draw-for-draw identical to the reference, so it is
+held to stated precision and statistical bars:
+
+  - diffusion off, against the FP64 parity path on the same forcing, after
+    24 h on a 4,000-particle sample of the full-size release:
+      FP32 storage  max |delta| < 1e-3 cells
+      FP16 storage  max |delta| < 0.05 cells (rms reported)
+      BF16 storage  max |delta| < 0.5 cells  (rms reported)
+    The bound is a worst case over particles; the per-particle error grows
+    like a random walk over the ~170 cells a particle crosses in 24 h.
+  - diffusion on, zero current: per-axis displacement variance after one
+    interval = 120 sigma_k^2 (docs/model.md:204-210) within 5 standard errors,
+    mean within 5 standard errors of zero, x/y draws uncorrelated.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import wacommplusplus_amd as wc
+from runner import run_device
+from wacommplusplus_amd.synthetic import ForcingWindow, full_window, plain_window, rectilinear, release
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = wc.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def full():
+    return full_window()
+
+
+@pytest.fixture(scope="module")
+def reference_24h(ctx, full):
+    p = release(full, 4_000_000, 50)[::1000].copy()
+    cfg = wc.default_config(random=False)
+    return p, cfg, run_device(ctx, full, cfg, p, 24, path="parity")
+
+
+def deviation(a, b):
+    both = (a["health"] > 0) & (b["health"] > 0)
+    d = np.stack([np.abs(a[f][both] - b[f][both]) for f in ("k", "j", "i")])
+    return float(d.max()), float(np.sqrt((d ** 2).mean())), float(both.mean())
+
+
+@pytest.mark.parametrize("store,bound", [("f32", 1e-3), ("f16", 0.05), ("bf16", 0.5)])
+def test_fast_path_deviation_vs_fp64(ctx, full, reference_24h, store, bound):
+    p, cfg, ref = reference_24h
+    dev = run_device(ctx, full, cfg, p, 24, path="fast", store=store)
+    mx, rms, alive = deviation(dev, ref)
+    print(f"fast/{store}: max |delta| {mx:.3e} cells, rms {rms:.3e} cells, both alive {alive:.4f}", flush=True)
+    assert mx < bound
+    assert alive > 0.97
+
+
+def still_window(T=2, eta=64, xi=64, s_rho=4):
+    """Zero current, AKt=0: only the horizontal random walk moves particles."""
+    lon, lat = rectilinear(eta, xi, 150.0)
+    s_w = s_rho + 1
+
+    def rec(t):
+        return (np.zeros((eta, xi), np.float32), np.zeros((s_rho, eta, xi), np.float32),
+                np.zeros((s_rho, eta, xi), np.float32), np.zeros((s_w, eta, xi), np.float32),
+                np.zeros((s_w, eta, xi), np.float32))
+    return ForcingWindow(s_rho, eta, xi, lon, lat, np.full((eta, xi), 40.0), np.ones((eta, xi)),
+                         np.full(s_w - 1, 1.0 / (s_w - 1)), np.arange(T) * 3600.0, rec)
+
+
+@pytest.mark.parametrize("store", ["f32", "bf16"])
+def test_fast_path_diffusion_statistics(ctx, store):
+    W = still_window()
+    n = 200_000
+    p = wc.make_particles(n, k=-1.5, j=32.25, i=32.75)
+    sigma = wc.sigma_from_kh(1.0, 30.0)
+    cfg = wc.default_config(random=True, sigma=sigma, integrator=wc.MIDPOINT)
+    out = run_device(ctx, W, cfg, p, 1, path="fast", store=store)
+    dx_tab, dy_tab = wc.cell_metric_table(W.lonRad, W.latRad)
+    dx_m = (out["i"] - 32.75) * dx_tab[32, 32]
+    dy_m = (out["j"] - 32.25) * dy_tab[32, 32]
+    kLL = -W.s_w + 2
+    sigma_k = sigma * (1 - (-1.5) / kLL)
+    expected = 120 * sigma_k ** 2
+    se = math.sqrt(2.0 / n)
+    assert abs(dx_m.var() / expected - 1) < 5 * se
+    assert abs(dy_m.var() / expected - 1) < 5 * se
+    assert abs(dx_m.mean()) < 5 * math.sqrt(expected / n)
+    assert abs(dy_m.mean()) < 5 * math.sqrt(expected / n)
+    assert abs(np.corrcoef(dx_m, dy_m)[0, 1]) < 5 / math.sqrt(n)
+    assert np.all(out["k"] == -1.5)                           # AKt = 0: no vertical random leap
+
+
+def test_fast_path_uniform_current_matches_parity(ctx):
+    """Plain configuration (uniform u=0.2, v=0.1 exactly representable as the
+    same float in every storage): FP32 fast path within 1e-4 cells of the
+    FP64 path after 12 h."""
+    W = plain_window(T=13)
+    p = release(W, 2000, 1, depth_m=1.0)
+    p["j"], p["i"] = 59.5, 49.5
+    cfg = wc.default_config(random=False)
+    ref = run_device(ctx, W, cfg, p, 12, path="parity")
+    dev = run_device(ctx, W, cfg, p, 12, path="fast", store="f32")
+    mx, rms, alive = deviation(dev, ref)
+    assert alive == 1.0 and mx < 1e-4, mx
+
+
+def test_fast_path_full_size_invariants(ctx, full):
+    p = release(full, 4_000_000, 50)
+    cfg = wc.default_config(random=True, sigma=wc.sigma_from_kh(1.0, 30.0))
+    a, frames = run_device(ctx, full, cfg, p, 2, path="fast", store="bf16", grid_frames=True)
+    alive = a[a["health"] > 0]
+    assert len(alive) > 0.99 * len(p)
+    kr, jr, ir = np.round(alive["k"]), np.round(alive["j"]), np.round(alive["i"])
+    inside = (jr >= 0) & (jr < full.eta) & (ir >= 0) & (ir < full.xi) & (kr >= -full.s_rho + 1) & (kr <= 0)
+    assert int(frames[-1].sum()) == int(inside.sum())
+    # keyed draws: a permuted population gives identical per-id results
+    perm = np.random.default_rng(3).permutation(len(p))
+    b = run_device(ctx, full, cfg, p[perm], 2, path="fast", store="bf16")
+    assert np.array_equal(b[np.argsort(b["id"])], a)

@@ -29,13 +29,12 @@ struct wc_ctx {
     int s_w = 0, s_rho = 0, eta = 0, xi = 0;
     int store = WC_STORE_F32;
     double *d_h = nullptr, *d_depth = nullptr, *d_dx = nullptr, *d_dy = nullptr, *d_mask = nullptr;
-    float *d_hf = nullptr, *d_idxf = nullptr, *d_idyf = nullptr, *d_depthf = nullptr;
+    float *d_hf = nullptr, *d_md = nullptr, *d_depthf = nullptr;
+    void *d_pack = nullptr;    // gather layout of the mixed-precision path
 
-    // forcing record ring: per slot, FP32 fields (parity path) and, for 16-bit
-    // storage, converted copies (fast path)
+    // forcing record ring: per slot, the FP32 fields as uploaded
     struct Slot {
         float *f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};      // zeta, u, v, w, akt
-        void *lp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
         bool filled = false;
         int record = -1;      // window record held by the slot (wc_run_window)
     };
@@ -339,7 +338,7 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
                         v + plane * sRho * timeIndex, v + plane * sRho * next,
                         w + plane * sW * timeIndex, w + plane * sW * next,
                         akt + plane * sW * timeIndex, akt + plane * sW * next};
-    GridPtrs G{h, depthIntervals, nullptr, nullptr, lonRad, latRad, nullptr, nullptr, nullptr, nullptr};
+    GridPtrs G{h, depthIntervals, nullptr, nullptr, lonRad, latRad, nullptr, nullptr, nullptr, nullptr};  // metric inline
     return launch_move_parity_aos(P, F, G, particles, particleCount, nullptr, st) == hipSuccess ? WC_OK
                                                                                                 : WC_ERR_HIP;
 }
@@ -381,10 +380,11 @@ int wc_ctx_create(int device, wc_ctx **out) {
 static void free_grid(wc_ctx *c) {
     hipFree(c->d_h); hipFree(c->d_depth); hipFree(c->d_dx); hipFree(c->d_dy); hipFree(c->d_mask);
     c->d_h = c->d_depth = c->d_dx = c->d_dy = c->d_mask = nullptr;
-    hipFree(c->d_hf); hipFree(c->d_idxf); hipFree(c->d_idyf); hipFree(c->d_depthf);
-    c->d_hf = c->d_idxf = c->d_idyf = c->d_depthf = nullptr;
+    hipFree(c->d_hf); hipFree(c->d_md); hipFree(c->d_depthf); hipFree(c->d_pack);
+    c->d_hf = c->d_md = c->d_depthf = nullptr;
+    c->d_pack = nullptr;
     for (auto &s : c->slots)
-        for (int f = 0; f < 5; ++f) { hipFree(s.f[f]); hipFree(s.lp[f]); }
+        for (int f = 0; f < 5; ++f) hipFree(s.f[f]);
     c->slots.clear();
     hipFree(c->d_counts);
     hipFree(c->d_conc);
@@ -450,27 +450,23 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
     HIPCHK(c, hipMemcpy(c->d_mask, m.data(), plane * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_depth, depthIntervals, (size_t)(s_w - 1) * 8, hipMemcpyHostToDevice));
     // FP32 tables of the mixed-precision path
-    std::vector<float> hf(plane), idxf(plane), idyf(plane), depthf(s_w - 1);
+    std::vector<float> hf(plane), md(2 * plane), depthf(s_w - 1);
     for (size_t q = 0; q < plane; ++q) {
         hf[q] = (float)h[q];
-        idxf[q] = dx[q] > 0 ? (float)(1.0 / dx[q]) : 0.0f;
-        idyf[q] = dy[q] > 0 ? (float)(1.0 / dy[q]) : 0.0f;
+        md[2 * q] = dx[q] > 0 ? (float)(1.0 / dx[q]) : 0.0f;
+        md[2 * q + 1] = dy[q] > 0 ? (float)(1.0 / dy[q]) : 0.0f;
     }
     for (int q = 0; q < s_w - 1; ++q) depthf[q] = (float)depthIntervals[q];
     HIPCHK(c, hipMalloc(&c->d_hf, plane * 4));
-    HIPCHK(c, hipMalloc(&c->d_idxf, plane * 4));
-    HIPCHK(c, hipMalloc(&c->d_idyf, plane * 4));
+    HIPCHK(c, hipMalloc(&c->d_md, plane * 8));
     HIPCHK(c, hipMalloc(&c->d_depthf, (size_t)(s_w - 1) * 4));
+    HIPCHK(c, hipMalloc(&c->d_pack, fast_pack_bytes(store, s_w, plane)));
     HIPCHK(c, hipMemcpy(c->d_hf, hf.data(), plane * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_idxf, idxf.data(), plane * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_idyf, idyf.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_md, md.data(), plane * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_depthf, depthf.data(), (size_t)(s_w - 1) * 4, hipMemcpyHostToDevice));
     c->slots.resize(n_slots);
     for (auto &s : c->slots) {
-        for (int f = 0; f < 5; ++f) {
-            HIPCHK(c, hipMalloc(&s.f[f], field_count(c, f) * sizeof(float)));
-            if (store != WC_STORE_F32) HIPCHK(c, hipMalloc(&s.lp[f], field_count(c, f) * 2));
-        }
+        for (int f = 0; f < 5; ++f) HIPCHK(c, hipMalloc(&s.f[f], field_count(c, f) * sizeof(float)));
     }
     HIPCHK(c, hipMalloc(&c->d_counts, plane * s_rho * sizeof(uint32_t)));
     c->haveGrid = true;
@@ -487,11 +483,8 @@ int wc_upload_record(wc_ctx *c, int slot, const float *zeta, const float *u, con
         if (!src[f]) return fail(c, WC_ERR_INVALID, "null forcing field %d", f);
     HIPCHK(c, hipSetDevice(c->device));
     auto &s = c->slots[slot];
-    for (int f = 0; f < 5; ++f) {
+    for (int f = 0; f < 5; ++f)
         HIPCHK(c, hipMemcpyAsync(s.f[f], src[f], field_count(c, f) * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        if (c->store != WC_STORE_F32)
-            HIPCHK(c, launch_convert_store(c->store, s.f[f], s.lp[f], field_count(c, f), c->stream));
-    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     s.filled = true;
     s.record = -1;
@@ -571,19 +564,13 @@ int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double 
     if (c->n == 0) return WC_OK;
     HIPCHK(c, hipSetDevice(c->device));
     const auto &a = c->slots[slot0], &b = c->slots[slot1];
-    GridPtrs G{c->d_h, c->d_depth, c->d_dx, c->d_dy, nullptr, nullptr, c->d_hf, c->d_idxf, c->d_idyf, c->d_depthf};
+    GridPtrs G{c->d_h, c->d_depth, c->d_dx, c->d_dy, nullptr, nullptr, c->d_hf, c->d_md, nullptr, c->d_depthf};
+    RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
     timing_begin(c);
-    if (path == WC_PATH_PARITY) {
-        RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
+    if (path == WC_PATH_PARITY)
         HIPCHK(c, launch_move_parity_soa(P, F, G, c->s, (int)c->n, c->d_steps, c->stream));
-    } else {
-        const void *pair[10];
-        for (int f = 0; f < 5; ++f) {
-            pair[2 * f] = c->store == WC_STORE_F32 ? (const void *)a.f[f] : a.lp[f];
-            pair[2 * f + 1] = c->store == WC_STORE_F32 ? (const void *)b.f[f] : b.lp[f];
-        }
-        HIPCHK(c, launch_move_fast_soa(c->store, P, pair, G, c->s, (int)c->n, c->d_steps, c->stream));
-    }
+    else
+        HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps, c->stream));
     return timing_end(c, 0);
 }
 

@@ -12,11 +12,13 @@ hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
                                   wc_particle_data *p, int n, unsigned long long *stepCounter, hipStream_t stream);
 
-// Mixed-precision path; T is float, __half or __hip_bfloat16 storage (store code WC_STORE_*).
-hipError_t launch_move_fast_soa(int store, const MoveParams &P, const void *const *recordPair, const GridPtrs &G,
-                                const SoA &s, int n, unsigned long long *stepCounter, hipStream_t stream);
-// FP32 -> 16-bit storage conversion of one record array.
-hipError_t launch_convert_store(int store, const float *src, void *dst, size_t n, hipStream_t stream);
+// Mixed-precision path (store = WC_STORE_*): packs the interval's record pair
+// into packBuf (fast_pack_bytes) and moves the particles.  G.idxf must point to
+// the float2 {1/dx, 1/dy} table.
+size_t fast_pack_bytes(int store, int s_w, size_t plane);
+hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
+                                const GridPtrs &G, const SoA &s, int n, unsigned long long *stepCounter,
+                                hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream);
 hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi, uint32_t *counts,

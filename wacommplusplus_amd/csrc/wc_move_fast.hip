@@ -4,12 +4,23 @@
 // keyed random displacement, surface/bottom/shore/open-boundary closures,
 // exponential decay), computed in FP32 from FP32, FP16 or BF16 forcing.
 //
+// Gather layout.  The move is bound by the number of scattered load
+// instructions, not by bytes, so once per interval a pack kernel interleaves
+// the interval's two forcing records into per-grid-point structs that one
+// naturally aligned load fetches:
+//   UV[k][j][i] = {u(t0), u(t1), v(t0), v(t1)}                 rho level k
+//   WA[k][j][i] = {w_k(t0), w_k(t1), w_k-1(t0), w_k-1(t1),
+//                  a_k(t0), a_k(t1), a_k-1(t0), a_k-1(t1)}     W levels k, k-1
+//   Z [j][i]    = {zeta(t0), zeta(t1), h, -}                   (FP32)
+//   MD[j][i]    = {1/dx, 1/dy}                                   (FP32, static)
+// A bilinear stage is then 4 UV + 4 WA + 4 Z loads instead of ~60.
+//
 // State: each work-item keeps its particle as an integer cell (jI, iI) plus
-// FP32 offsets in [0,1) and an FP32 level k, for a whole forcing interval in
-// registers; only elapsed time and age stay FP64.  The FP64 SoA state is read
-// once and written once per interval (j = jI + offset is exact in FP64).
-// Health is evaluated once per interval: exp(-age/tau0) < survprob is
-// equivalent to age > -tau0*log(survprob), tested every substep.
+// FP32 offsets in [0,1) and an FP32 level k for a whole forcing interval in
+// registers; elapsed time and age stay FP64.  The FP64 SoA state is read once
+// and written once per interval (j = jI + offset is exact in FP64).  Health is
+// evaluated once per interval: exp(-age/tau0) < survprob is equivalent to
+// age > -tau0*log(survprob), tested every substep.
 //
 // Diffusion draws: Philox4x32-10 with key = seed, counter = (substep,
 // interval key, id lo, id hi); one call gives the four uniforms of two
@@ -25,62 +36,100 @@
 namespace wc {
 namespace {
 
-// storage -> float
 template <class T>
-__device__ __forceinline__ float ld(const T *__restrict__ p, size_t i);
+struct alignas(4 * sizeof(T)) UV4 {
+    T v[4];
+};
+template <class T>
+struct alignas(8 * sizeof(T)) WA8 {
+    T v[8];
+};
+struct alignas(16) ZH {
+    float z0, z1, h, pad;
+};
+
+template <class T>
+__device__ __forceinline__ float tof(T x);
 template <>
-__device__ __forceinline__ float ld<float>(const float *__restrict__ p, size_t i) {
-    return p[i];
-}
+__device__ __forceinline__ float tof<float>(float x) { return x; }
 template <>
-__device__ __forceinline__ float ld<_Float16>(const _Float16 *__restrict__ p, size_t i) {
-    return (float)p[i];
-}
+__device__ __forceinline__ float tof<_Float16>(_Float16 x) { return (float)x; }
 template <>
-__device__ __forceinline__ float ld<uint16_t>(const uint16_t *__restrict__ p, size_t i) {
-    return __uint_as_float((uint32_t)p[i] << 16);   // bfloat16 bits
+__device__ __forceinline__ float tof<uint16_t>(uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
+
+template <class T>
+__device__ __forceinline__ T fromf(float x);
+template <>
+__device__ __forceinline__ float fromf<float>(float x) { return x; }
+template <>
+__device__ __forceinline__ _Float16 fromf<_Float16>(float x) { return (_Float16)x; }
+template <>
+__device__ __forceinline__ uint16_t fromf<uint16_t>(float x) {
+    const uint32_t b = __float_as_uint(x);   // round to nearest even (finite inputs)
+    return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
 }
 
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 
 template <class T>
-struct FastSampler {
-    const T *zeta0, *zeta1, *u0, *u1, *v0, *v1, *w0, *w1, *a0, *a1;
-    const float *hf, *idxf, *idyf, *depthf;
+struct Packed {
+    const UV4<T> *uv;     // [s_w-1 levels: k = kLL..0][eta][xi]   (rho levels)
+    const WA8<T> *wa;     // [s_w-1 levels: k = kLL..0][eta][xi]   (W levels k, k-1)
+    const ZH *z;          // [eta][xi]
+    const float2 *md;     // [eta][xi] {1/dx, 1/dy}
+    const float *depthf;
     int xi;
+    int nlev;             // s_w - 1
     size_t plane;
-    int s_rho, s_w;
 
-    // time-lerped bilinear sample of one level plane at base offset `b`
-    // (index of corner (jI, iI)); fi, fj offsets in [0,1)
-    __device__ __forceinline__ float bil(const T *__restrict__ f0, const T *__restrict__ f1, size_t b, float fi,
-                                         float fj, float al) const {
-        const float c00 = lerpf(ld(f0, b), ld(f1, b), al);
-        const float c01 = lerpf(ld(f0, b + 1), ld(f1, b + 1), al);
-        const float c10 = lerpf(ld(f0, b + xi), ld(f1, b + xi), al);
-        const float c11 = lerpf(ld(f0, b + xi + 1), ld(f1, b + xi + 1), al);
-        return lerpf(lerpf(c00, c01, fi), lerpf(c10, c11, fi), fj);
+    // bilinear weights of the four corners
+    struct W4 {
+        float w00, w01, w10, w11;
+    };
+    __device__ __forceinline__ static W4 weights(float fi, float fj) {
+        const float gi = 1.0f - fi, gj = 1.0f - fj;
+        return W4{gi * gj, fi * gj, gi * fj, fi * fj};
     }
-    __device__ __forceinline__ float h(size_t c, float fi, float fj) const {
-        return lerpf(lerpf(hf[c], hf[c + 1], fi), lerpf(hf[c + xi], hf[c + xi + 1], fi), fj);
+    // level index of logical level k (k = -(nlev-1) .. 0)
+    __device__ __forceinline__ size_t lev(int k) const { return (size_t)(k + nlev - 1) * plane; }
+
+    __device__ __forceinline__ float hc(size_t c, const W4 &w, float al) const {
+        const ZH a = z[c], b = z[c + 1], d = z[c + xi], e = z[c + xi + 1];
+        const float zz = w.w00 * lerpf(a.z0, a.z1, al) + w.w01 * lerpf(b.z0, b.z1, al) +
+                         w.w10 * lerpf(d.z0, d.z1, al) + w.w11 * lerpf(e.z0, e.z1, al);
+        const float hh = w.w00 * a.h + w.w01 * b.h + w.w10 * d.h + w.w11 * e.h;
+        return hh + zz;
     }
-    __device__ __forceinline__ float zeta(size_t c, float fi, float fj, float al) const {
-        return bil(zeta0, zeta1, c, fi, fj, al);
+    __device__ __forceinline__ void uv_at(int kI, size_t c, const W4 &w, float al, float &uu, float &vv) const {
+        const UV4<T> *p = uv + lev(kI) + c;
+        const UV4<T> a = p[0], b = p[1], d = p[xi], e = p[xi + 1];
+        uu = w.w00 * lerpf(tof(a.v[0]), tof(a.v[1]), al) + w.w01 * lerpf(tof(b.v[0]), tof(b.v[1]), al) +
+             w.w10 * lerpf(tof(d.v[0]), tof(d.v[1]), al) + w.w11 * lerpf(tof(e.v[0]), tof(e.v[1]), al);
+        vv = w.w00 * lerpf(tof(a.v[2]), tof(a.v[3]), al) + w.w01 * lerpf(tof(b.v[2]), tof(b.v[3]), al) +
+             w.w10 * lerpf(tof(d.v[2]), tof(d.v[3]), al) + w.w11 * lerpf(tof(e.v[2]), tof(e.v[3]), al);
     }
-    // velocity at rho level kI (u, v) and W levels kI, kI-1 (w), weight kF on kI-1
-    __device__ __forceinline__ void velocity(int kI, float kF, size_t c, float fi, float fj, float al, float &uu,
-                                             float &vv, float &ww) const {
-        const size_t br = (size_t)(kI + s_rho - 1) * plane + c;
-        uu = bil(u0, u1, br, fi, fj, al);
-        vv = bil(v0, v1, br, fi, fj, al);
-        const size_t bw = (size_t)(kI + s_w - 1) * plane + c;
-        const float wt = bil(w0, w1, bw, fi, fj, al);
-        const float wb = bil(w0, w1, bw - plane, fi, fj, al);
-        ww = lerpf(wt, wb, kF);
-    }
-    __device__ __forceinline__ float akt(int kI, float kF, size_t c, float fi, float fj, float al) const {
-        const size_t bw = (size_t)(kI + s_w - 1) * plane + c;
-        return lerpf(bil(a0, a1, bw, fi, fj, al), bil(a0, a1, bw - plane, fi, fj, al), kF);
+    // w (and AKt when wantA) at W levels kI (weight 1-kF) and kI-1 (weight kF)
+    __device__ __forceinline__ void wa_at(int kI, float kF, size_t c, const W4 &w, float alw, float ala, bool wantA,
+                                          float &ww, float &aa) const {
+        const WA8<T> *p = wa + lev(kI) + c;
+        const WA8<T> a = p[0], b = p[1], d = p[xi], e = p[xi + 1];
+        auto corner = [&](const WA8<T> &q, float &wc, float &ac) {
+            const float wt = lerpf(tof(q.v[0]), tof(q.v[1]), alw), wb = lerpf(tof(q.v[2]), tof(q.v[3]), alw);
+            wc = lerpf(wt, wb, kF);
+            if (wantA) {
+                const float at = lerpf(tof(q.v[4]), tof(q.v[5]), ala), ab = lerpf(tof(q.v[6]), tof(q.v[7]), ala);
+                ac = lerpf(at, ab, kF);
+            } else {
+                ac = 0.f;
+            }
+        };
+        float w0, w1, w2, w3, a0, a1, a2, a3;
+        corner(a, w0, a0);
+        corner(b, w1, a1);
+        corner(d, w2, a2);
+        corner(e, w3, a3);
+        ww = w.w00 * w0 + w.w01 * w1 + w.w10 * w2 + w.w11 * w3;
+        aa = w.w00 * a0 + w.w01 * a1 + w.w10 * a2 + w.w11 * a3;
     }
 };
 
@@ -98,8 +147,7 @@ __device__ double reflect_domain(double c, double m) {
     return r;
 }
 
-// Split a coordinate held as cell + offset after a displacement d (in cells):
-// returns the new cell, offset in [0,1).
+// Split a coordinate held as cell + offset after a displacement d (in cells).
 __device__ __forceinline__ void advance(int cell, float off, float d, int &ncell, float &noff) {
     const float x = off + d;
     const float fl = floorf(x);
@@ -112,13 +160,14 @@ __device__ __forceinline__ void advance(int cell, float off, float d, int &ncell
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, FastSampler<T> S, SoA st, int n,
+__global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> S, SoA st, int n,
                                                         unsigned long long *stepCounter) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned steps = 0;
     if (idx < n) {
         const uint64_t id = st.id[idx];
-        double kd = st.k[idx], jd = st.j[idx], id_ = st.i[idx];
+        double kd = st.k[idx];
+        const double jd = st.j[idx], id_ = st.i[idx];
         double health = st.health[idx], age = st.age[idx];
         const double ptime = st.time[idx];
         const float kLL = (float)P.kLowerLimit;
@@ -151,6 +200,7 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, FastSample
         const uint32_t seedLo = (uint32_t)P.seed, seedHi = (uint32_t)(P.seed >> 32);
         const uint32_t ik = (uint32_t)P.intervalKey ^ (uint32_t)((uint64_t)P.intervalKey >> 32);
         const double L = P.intervalLength;
+        const double invL = L == 0 ? 0.0 : 1.0 / L, invDti = 1.0 / P.dti;
         double elapsed = P.elapsed0;
 
         while (elapsed < L) {
@@ -174,24 +224,26 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, FastSample
             }
             ++steps;
             changed = true;
-            const float alpha = L == 0 ? 0.f : (float)((elapsed + 0.5 * stepDt) / L);
-            const float alphaStart = L == 0 ? 0.f : (float)(elapsed / L);
+            const float alpha = (float)((elapsed + 0.5 * stepDt) * invL);
+            const float alphaStart = (float)(elapsed * invL);
             const float dt = (float)stepDt;
             const int kI = (int)ceilf(k);
             const float kF = (float)kI - k;
             const size_t c = (size_t)jc * xi + ic;
+            const auto w0 = Packed<T>::weights(fi, fj);
 
-            const float hc = S.h(c, fi, fj) + S.zeta(c, fi, fj, alpha);
+            const float hc = S.hc(c, w0, alpha);
             bool kill = false;
             if (hc <= shore && P.horizontal == WC_CLOSURE_KILL) kill = true;
 
-            float uu, vv, ww;
-            S.velocity(kI, kF, c, fi, fj, P.midpoint ? alphaStart : alpha, uu, vv, ww);
+            const float a1 = P.midpoint ? alphaStart : alpha;
+            float uu, vv, ww, aa;
+            S.uv_at(kI, c, w0, a1, uu, vv);
+            S.wa_at(kI, kF, c, w0, a1, alpha, P.randomOn != 0, ww, aa);
             float rx = 0, ry = 0, rz = 0;
             if (P.randomOn) {
-                const float aa = S.akt(kI, kF, c, fi, fj, alpha);
                 const float sigmaDepth = (float)P.sigma * (1.0f - k / kLL);
-                const uint32_t substep = (uint32_t)floor(elapsed / P.dti);
+                const uint32_t substep = (uint32_t)floor(elapsed * invDti + 1e-9);
                 const Philox4 r = philox4x32_10(substep, ik, (uint32_t)id, (uint32_t)(id >> 32), seedLo, seedHi);
                 const float u1 = ((float)(r.v[0] >> 8) + 0.5f) * 0x1.0p-24f;
                 const float u2 = (float)(r.v[1] >> 8) * 0x1.0p-24f;
@@ -201,12 +253,13 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, FastSample
                 float s1, c1, s2, c2;
                 __sincosf(6.2831853071795865f * u2, &s1, &c1);
                 __sincosf(6.2831853071795865f * u4, &s2, &c2);
-                const float scale = sqrtf((float)(stepDt / P.dti)) * sigmaDepth;
+                const float scale = sqrtf((float)(stepDt * invDti)) * sigmaDepth;
                 rx = scale * ra * c1;
                 ry = scale * ra * s1;
                 rz = scale * rb * c2 * aa * (float)P.crid;
             }
-            const float idx_ = S.idxf[c], idy_ = S.idyf[c];
+            const float2 md = S.md[c];
+            const float idx_ = md.x, idy_ = md.y;
             const float zdist = hc * S.depthf[kI + P.s_w - 2];
             float dxl = dir * uu * dt, dyl = dir * vv * dt, dzl = dir * (sv + ww) * dt;
 
@@ -221,23 +274,28 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, FastSample
                 bool midValid = valid_cell(jm, im, eta, xi);
                 float hcMid = 0;
                 size_t cm = 0;
+                typename Packed<T>::W4 wm{};
                 if (midValid) {
                     cm = (size_t)jm * xi + im;
-                    hcMid = S.h(cm, fim, fjm) + S.zeta(cm, fim, fjm, alpha);
+                    wm = Packed<T>::weights(fim, fjm);
+                    hcMid = S.hc(cm, wm, alpha);
                     midValid = hcMid > shore && hcMid > 0;
                 }
                 if (midValid) {
                     const int kMI = (int)ceilf(kMid);
                     const float kMF = (float)kMI - kMid;
-                    float um, vm, wm;
-                    S.velocity(kMI, kMF, cm, fim, fjm, alpha, um, vm, wm);
+                    float um, vm, wmv, dummy;
+                    S.uv_at(kMI, cm, wm, alpha, um, vm);
+                    S.wa_at(kMI, kMF, cm, wm, alpha, alpha, false, wmv, dummy);
                     // leap in metres of the start cell: u* dt dx/dx*
-                    dxl = dir * um * dt * (S.idxf[cm] / idx_);
-                    dyl = dir * vm * dt * (S.idyf[cm] / idy_);
-                    dzl = dir * (sv + wm) * dt * (zdist / (hcMid * S.depthf[kMI + P.s_w - 2]));
+                    const float2 mdm = S.md[cm];
+                    dxl = dir * um * dt * (mdm.x / idx_);
+                    dyl = dir * vm * dt * (mdm.y / idy_);
+                    dzl = dir * (sv + wmv) * dt * (zdist / (hcMid * S.depthf[kMI + P.s_w - 2]));
                 } else {
-                    float ue, ve, we;
-                    S.velocity(kI, kF, c, fi, fj, alpha, ue, ve, we);
+                    float ue, ve, we, dummy;
+                    S.uv_at(kI, c, w0, alpha, ue, ve);
+                    S.wa_at(kI, kF, c, w0, alpha, alpha, false, we, dummy);
                     dxl = dir * ue * dt;
                     dyl = dir * ve * dt;
                     dzl = dir * (sv + we) * dt;
@@ -263,7 +321,7 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, FastSample
             }
             if (valid_cell(jn, in_, eta, xi)) {                    // :709-776
                 const size_t cn = (size_t)jn * xi + in_;
-                const float hcdet = S.h(cn, fin, fjn) + S.zeta(cn, fin, fjn, alpha);
+                const float hcdet = S.hc(cn, Packed<T>::weights(fin, fjn), alpha);
                 if (hcdet <= shore) {
                     if (P.horizontal == WC_CLOSURE_CONSTRAINT) {
                         jn = jc; fjn = fj; in_ = ic; fin = fi;
@@ -318,55 +376,77 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, FastSample
     if (stepCounter) wave_add_u64(stepCounter, steps);
 }
 
+// Interleaves the interval's records (FP32 ring slots) into the gather layout.
 template <class T>
-hipError_t launch_t(const MoveParams &P, const void *const *pr, const GridPtrs &G, const SoA &s, int n,
-                    unsigned long long *ctr, hipStream_t stream) {
-    FastSampler<T> S;
-    S.zeta0 = (const T *)pr[0]; S.zeta1 = (const T *)pr[1];
-    S.u0 = (const T *)pr[2]; S.u1 = (const T *)pr[3];
-    S.v0 = (const T *)pr[4]; S.v1 = (const T *)pr[5];
-    S.w0 = (const T *)pr[6]; S.w1 = (const T *)pr[7];
-    S.a0 = (const T *)pr[8]; S.a1 = (const T *)pr[9];
-    S.hf = G.hf; S.idxf = G.idxf; S.idyf = G.idyf; S.depthf = G.depthf;
+__global__ void pack_kernel(RecordPair<float> F, const float *__restrict__ hf, UV4<T> *__restrict__ uv,
+                            WA8<T> *__restrict__ wa, ZH *__restrict__ z, int s_rho, int s_w, size_t plane) {
+    const int nlev = s_w - 1;                 // particle levels k = -(nlev-1) .. 0
+    const size_t total = plane * nlev;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t lv = q / plane, c = q - lv * plane;     // lv = k + nlev - 1
+        const int k = (int)lv - (nlev - 1);
+        const size_t r = (size_t)(k + s_rho - 1) * plane + c;          // rho level k
+        const size_t wt = (size_t)(k + s_w - 1) * plane + c;           // W level k
+        const size_t wb = wt - plane;                                    // W level k-1
+        UV4<T> a;
+        a.v[0] = fromf<T>(F.u0[r]); a.v[1] = fromf<T>(F.u1[r]);
+        a.v[2] = fromf<T>(F.v0[r]); a.v[3] = fromf<T>(F.v1[r]);
+        uv[q] = a;
+        WA8<T> b;
+        b.v[0] = fromf<T>(F.w0[wt]); b.v[1] = fromf<T>(F.w1[wt]);
+        b.v[2] = fromf<T>(F.w0[wb]); b.v[3] = fromf<T>(F.w1[wb]);
+        b.v[4] = fromf<T>(F.akt0[wt]); b.v[5] = fromf<T>(F.akt1[wt]);
+        b.v[6] = fromf<T>(F.akt0[wb]); b.v[7] = fromf<T>(F.akt1[wb]);
+        wa[q] = b;
+        if (lv == 0) z[c] = ZH{tof<T>(fromf<T>(F.zeta0[c])), tof<T>(fromf<T>(F.zeta1[c])), hf[c], 0.0f};
+    }
+}
+
+int grid_blocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    return (int)(b ? b : 1);
+}
+
+template <class T>
+hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G, const SoA &s,
+                    int n, unsigned long long *ctr, hipStream_t stream) {
+    const size_t plane = (size_t)P.eta * P.xi;
+    const int nlev = P.s_w - 1;
+    Packed<T> S;
+    char *base = (char *)packBuf;
+    S.uv = (const UV4<T> *)base;
+    base += sizeof(UV4<T>) * plane * nlev;
+    S.wa = (const WA8<T> *)base;
+    base += sizeof(WA8<T>) * plane * nlev;
+    S.z = (const ZH *)base;
+    S.md = (const float2 *)G.idxf;
+    S.depthf = G.depthf;
     S.xi = P.xi;
-    S.plane = (size_t)P.eta * P.xi;
-    S.s_rho = P.s_rho;
-    S.s_w = P.s_w;
+    S.nlev = nlev;
+    S.plane = plane;
+    pack_kernel<T><<<grid_blocks(plane * nlev), 256, 0, stream>>>(F, G.hf, (UV4<T> *)S.uv, (WA8<T> *)S.wa,
+                                                                  (ZH *)S.z, P.s_rho, P.s_w, plane);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n <= 0) return e;
     const int block = 256;
     move_fast_kernel<T><<<(n + block - 1) / block, block, 0, stream>>>(P, S, s, n, ctr);
     return hipGetLastError();
 }
 
-__global__ void to_bf16_kernel(const float *__restrict__ src, uint16_t *__restrict__ dst, size_t n) {
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t b = __float_as_uint(src[q]);
-        // round to nearest even (finite inputs)
-        dst[q] = (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
-    }
-}
-
-__global__ void to_f16_kernel(const float *__restrict__ src, _Float16 *__restrict__ dst, size_t n) {
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-        dst[q] = (_Float16)src[q];
-}
-
 }  // namespace
 
-hipError_t launch_move_fast_soa(int store, const MoveParams &P, const void *const *pair, const GridPtrs &G,
-                                const SoA &s, int n, unsigned long long *ctr, hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    if (store == WC_STORE_F16) return launch_t<_Float16>(P, pair, G, s, n, ctr, stream);
-    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, pair, G, s, n, ctr, stream);
-    return launch_t<float>(P, pair, G, s, n, ctr, stream);
+size_t fast_pack_bytes(int store, int s_w, size_t plane) {
+    const size_t nlev = (size_t)(s_w - 1);
+    const size_t t = store == WC_STORE_F32 ? 4 : 2;
+    return plane * nlev * (4 * t + 8 * t) + plane * sizeof(ZH);
 }
 
-hipError_t launch_convert_store(int store, const float *src, void *dst, size_t n, hipStream_t stream) {
-    size_t blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    if (blocks == 0) return hipSuccess;
-    if (store == WC_STORE_BF16) to_bf16_kernel<<<(int)blocks, 256, 0, stream>>>(src, (uint16_t *)dst, n);
-    else if (store == WC_STORE_F16) to_f16_kernel<<<(int)blocks, 256, 0, stream>>>(src, (_Float16 *)dst, n);
-    return hipGetLastError();
+hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
+                                const GridPtrs &G, const SoA &s, int n, unsigned long long *ctr, hipStream_t stream) {
+    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, stream);
+    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, stream);
+    return launch_t<float>(P, F, packBuf, G, s, n, ctr, stream);
 }
 
 }  // namespace wc
