@@ -165,9 +165,9 @@ def main():
         ref = finals.pop("f64")
         for key, a in finals.items():
             both = (a["health"] > 0) & (ref["health"] > 0)
-            d = np.stack([np.abs(a[f][both] - ref[f][both]) for f in ("k", "j", "i")])
+            d = np.max(np.stack([np.abs(a[f][both] - ref[f][both]) for f in ("k", "j", "i")]), axis=0)
             precision[key] = dict(max_cells=float(d.max()), rms_cells=float(np.sqrt((d ** 2).mean())),
-                                  p99_cells=float(np.quantile(d.max(axis=0), 0.99)),
+                                  p99_cells=float(np.quantile(d, 0.99)), p50_cells=float(np.median(d)),
                                   both_alive=float(both.mean()))
         log(f"[precision] {precision}")
 

@@ -29,6 +29,7 @@ struct wc_ctx {
     int s_w = 0, s_rho = 0, eta = 0, xi = 0;
     int store = WC_STORE_F32;
     double *d_h = nullptr, *d_depth = nullptr, *d_dx = nullptr, *d_dy = nullptr, *d_mask = nullptr;
+    double2 *d_metric = nullptr;   // {dx, dy} per cell
     float *d_hf = nullptr, *d_md = nullptr, *d_depthf = nullptr;
     void *d_pack = nullptr;    // gather layout of the mixed-precision path
 
@@ -380,7 +381,8 @@ int wc_ctx_create(int device, wc_ctx **out) {
 static void free_grid(wc_ctx *c) {
     hipFree(c->d_h); hipFree(c->d_depth); hipFree(c->d_dx); hipFree(c->d_dy); hipFree(c->d_mask);
     c->d_h = c->d_depth = c->d_dx = c->d_dy = c->d_mask = nullptr;
-    hipFree(c->d_hf); hipFree(c->d_md); hipFree(c->d_depthf); hipFree(c->d_pack);
+    hipFree(c->d_hf); hipFree(c->d_md); hipFree(c->d_depthf); hipFree(c->d_pack); hipFree(c->d_metric);
+    c->d_metric = nullptr;
     c->d_hf = c->d_md = c->d_depthf = nullptr;
     c->d_pack = nullptr;
     for (auto &s : c->slots)
@@ -460,7 +462,14 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
     HIPCHK(c, hipMalloc(&c->d_hf, plane * 4));
     HIPCHK(c, hipMalloc(&c->d_md, plane * 8));
     HIPCHK(c, hipMalloc(&c->d_depthf, (size_t)(s_w - 1) * 4));
-    HIPCHK(c, hipMalloc(&c->d_pack, fast_pack_bytes(store, s_w, plane)));
+    {
+        const size_t a = fast_pack_bytes(store, s_w, plane), b = parity_pack_bytes(s_w, plane);
+        HIPCHK(c, hipMalloc(&c->d_pack, a > b ? a : b));
+    }
+    std::vector<double> metric(2 * plane);
+    for (size_t q = 0; q < plane; ++q) { metric[2 * q] = dx[q]; metric[2 * q + 1] = dy[q]; }
+    HIPCHK(c, hipMalloc(&c->d_metric, plane * 16));
+    HIPCHK(c, hipMemcpy(c->d_metric, metric.data(), plane * 16, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_hf, hf.data(), plane * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_md, md.data(), plane * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_depthf, depthf.data(), (size_t)(s_w - 1) * 4, hipMemcpyHostToDevice));
@@ -568,7 +577,7 @@ int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double 
     RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
     timing_begin(c);
     if (path == WC_PATH_PARITY)
-        HIPCHK(c, launch_move_parity_soa(P, F, G, c->s, (int)c->n, c->d_steps, c->stream));
+        HIPCHK(c, launch_move_parity_soa(P, F, G, c->d_pack, c->d_metric, c->s, (int)c->n, c->d_steps, c->stream));
     else
         HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps, c->stream));
     return timing_end(c, 0);

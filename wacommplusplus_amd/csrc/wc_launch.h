@@ -7,8 +7,12 @@
 
 namespace wc {
 
-hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, const SoA &s,
-                                  int n, unsigned long long *stepCounter, hipStream_t stream);
+// Context path: packs the interval's record pair into packBuf
+// (parity_pack_bytes) and moves the SoA population; metric = {dx, dy} table.
+size_t parity_pack_bytes(int s_w, size_t plane);
+hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
+                                  const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
+                                  hipStream_t stream);
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
                                   wc_particle_data *p, int n, unsigned long long *stepCounter, hipStream_t stream);
 

@@ -261,6 +261,11 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
             const float2 md = S.md[c];
             const float idx_ = md.x, idy_ = md.y;
             const float zdist = hc * S.depthf[kI + P.s_w - 2];
+            // A particle exactly on a land node line has hc = 0: the reference's
+            // k + zleap/zdist is then 0/0 (Particle.cpp:579, 657).  This path
+            // keeps k instead (no vertical displacement where the water column
+            // has no depth); the parity path keeps the reference arithmetic.
+            const float invZ = zdist > 0.f ? 1.0f / zdist : 0.f;
             float dxl = dir * uu * dt, dyl = dir * vv * dt, dzl = dir * (sv + ww) * dt;
 
             if (P.midpoint) {                                      // Particle.cpp:585-641
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
                 float fjm, fim;
                 advance(jc, fj, dir * vv * 0.5f * dt * idy_, jm, fjm);
                 advance(ic, fi, dir * uu * 0.5f * dt * idx_, im, fim);
-                const float kMid = fminf(0.0f, fmaxf(kLL, k + kHalf / zdist));
+                const float kMid = fminf(0.0f, fmaxf(kLL, k + kHalf * invZ));
                 bool midValid = valid_cell(jm, im, eta, xi);
                 float hcMid = 0;
                 size_t cm = 0;
@@ -308,7 +313,7 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
             float fjn, fin;
             advance(jc, fj, yleap * idy_, jn, fjn);
             advance(ic, fi, xleap * idx_, in_, fin);
-            float kdet = k + zleap / zdist;
+            float kdet = k + zleap * invZ;
             if (kdet > 0.f) {                                      // :665-704
                 if (P.upper == WC_CLOSURE_CONSTRAINT) kdet = 0.f;
                 else if (P.upper == WC_CLOSURE_KILL) kill = true;

@@ -22,6 +22,18 @@
 namespace wc {
 namespace {
 
+// packed gather layout of one interval (see PackedSampler)
+struct alignas(16) PackUV {
+    float v[4];      // u(t0), u(t1), v(t0), v(t1) at rho level k
+};
+struct alignas(32) PackWA {
+    float v[8];      // w_k(t0), w_k(t1), w_k-1(t0), w_k-1(t1), akt_k(t0), akt_k(t1), akt_k-1(t0), akt_k-1(t1)
+};
+struct alignas(16) PackZD {
+    float z0, z1;    // zeta(t0), zeta(t1)
+    double h;
+};
+
 __device__ __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double std_min(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double sign_of(double a, double b) {
@@ -48,82 +60,124 @@ __device__ __forceinline__ double reflect_domain(double c, double m) {
     return r;
 }
 
-// (float)interpolateTime(a[idx], b[idx], alpha)  (Particle.cpp:186-202)
-__device__ __forceinline__ float tl(const float *__restrict__ a, const float *__restrict__ b, size_t idx,
-                                    double al) {
-    return (float)((1.0 - al) * (double)a[idx] + al * (double)b[idx]);
+// (float)interpolateTime(a, b, alpha)  (Particle.cpp:186-202)
+__device__ __forceinline__ float tl(float a, float b, double al) {
+    return (float)((1.0 - al) * (double)a + al * (double)b);
 }
 
-struct Sampler {
-    const RecordPair<float> &F;
-    const GridPtrs &G;
+// Corner values of a bilinear stencil in the reference's order
+// (j,i), (j+1,i), (j+1,i+1), (j,i+1)  (Particle.cpp:271-274).
+struct C4f {
+    float a, b, c, d;
+};
+struct C4d {
+    double a, b, c, d;
+};
+
+// float partial products summed in float, exactly as Particle.cpp:271-277
+__device__ __forceinline__ float bil(const C4f &q, double iF, double jF) {
+    float t1 = q.a * (1.0 - iF) * (1.0 - jF);
+    float t2 = q.b * (1.0 - iF) * jF;
+    float t3 = q.c * iF * jF;
+    float t4 = q.d * iF * (1.0 - jF);
+    return t1 + t2 + t3 + t4;
+}
+// trilinear over two W levels (Particle.cpp:464-474, 490-500)
+__device__ __forceinline__ float tril(const C4f &t, const C4f &b, double iF, double jF, double kF) {
+    float t1 = t.a * (1.0 - iF) * (1.0 - jF) * (1.0 - kF);
+    float t2 = t.b * (1.0 - iF) * jF * (1.0 - kF);
+    float t3 = t.c * iF * jF * (1.0 - kF);
+    float t4 = t.d * iF * (1.0 - jF) * (1.0 - kF);
+    float t5 = b.a * (1.0 - iF) * (1.0 - jF) * kF;
+    float t6 = b.b * (1.0 - iF) * jF * kF;
+    float t7 = b.c * iF * jF * kF;
+    float t8 = b.d * iF * (1.0 - jF) * kF;
+    return t1 + t2 + t3 + t4 + t5 + t6 + t7 + t8;
+}
+
+// Sampler over the reference's own array layout (drop-in entry point):
+// index4 = ((t*levels + k+levels-1)*eta + j)*xi + i per record.
+struct RawSampler {
+    RecordPair<float> F;
+    GridPtrs G;
     int xi;
     size_t plane;
     int s_rho, s_w;
 
-    __device__ __forceinline__ size_t lvl(int k, int levels, int j, int i) const {
-        return (size_t)(k + levels - 1) * plane + (size_t)j * xi + i;
+    __device__ __forceinline__ size_t lvl(int k, int levels) const { return (size_t)(k + levels - 1) * plane; }
+    __device__ __forceinline__ C4f at4(const float *f0, const float *f1, size_t base, int j, int i, double al) const {
+        const size_t q = base + (size_t)j * xi + i;
+        return C4f{tl(f0[q], f1[q], al), tl(f0[q + xi], f1[q + xi], al), tl(f0[q + xi + 1], f1[q + xi + 1], al),
+                   tl(f0[q + 1], f1[q + 1], al)};
     }
-    __device__ __forceinline__ float zeta(int j, int i, double a) const {
-        return tl(F.zeta0, F.zeta1, (size_t)j * xi + i, a);
+    __device__ __forceinline__ void zh4(int j, int i, double al, C4f &z, C4d &h) const {
+        z = at4(F.zeta0, F.zeta1, 0, j, i, al);
+        const size_t q = (size_t)j * xi + i;
+        h = C4d{G.h[q], G.h[q + xi], G.h[q + xi + 1], G.h[q + 1]};
     }
-    __device__ __forceinline__ double h(int j, int i) const { return G.h[(size_t)j * xi + i]; }
-
-    // advectiveVelocity, passive branch (Particle.cpp:352-475)
-    __device__ __forceinline__ void velocity(int kI, double kF, int jI, double jF, int iI, double iF, double a,
-                                             float &uu, float &vv, float &ww) const {
-        const double wi0 = 1.0 - iF, wj0 = 1.0 - jF;
-        float u1 = tl(F.u0, F.u1, lvl(kI, s_rho, jI, iI), a) * wi0 * wj0;
-        float u2 = tl(F.u0, F.u1, lvl(kI, s_rho, jI + 1, iI), a) * wi0 * jF;
-        float u3 = tl(F.u0, F.u1, lvl(kI, s_rho, jI + 1, iI + 1), a) * iF * jF;
-        float u4 = tl(F.u0, F.u1, lvl(kI, s_rho, jI, iI + 1), a) * iF * wj0;
-        uu = u1 + u2 + u3 + u4;
-        float v1 = tl(F.v0, F.v1, lvl(kI, s_rho, jI, iI), a) * wi0 * wj0;
-        float v2 = tl(F.v0, F.v1, lvl(kI, s_rho, jI + 1, iI), a) * wi0 * jF;
-        float v3 = tl(F.v0, F.v1, lvl(kI, s_rho, jI + 1, iI + 1), a) * iF * jF;
-        float v4 = tl(F.v0, F.v1, lvl(kI, s_rho, jI, iI + 1), a) * iF * wj0;
-        vv = v1 + v2 + v3 + v4;
-        const double wk0 = 1.0 - kF;
-        float w1 = tl(F.w0, F.w1, lvl(kI, s_w, jI, iI), a) * wi0 * wj0 * wk0;
-        float w2 = tl(F.w0, F.w1, lvl(kI, s_w, jI + 1, iI), a) * wi0 * jF * wk0;
-        float w3 = tl(F.w0, F.w1, lvl(kI, s_w, jI + 1, iI + 1), a) * iF * jF * wk0;
-        float w4 = tl(F.w0, F.w1, lvl(kI, s_w, jI, iI + 1), a) * iF * wj0 * wk0;
-        float w5 = tl(F.w0, F.w1, lvl(kI - 1, s_w, jI, iI), a) * wi0 * wj0 * kF;
-        float w6 = tl(F.w0, F.w1, lvl(kI - 1, s_w, jI + 1, iI), a) * wi0 * jF * kF;
-        float w7 = tl(F.w0, F.w1, lvl(kI - 1, s_w, jI + 1, iI + 1), a) * iF * jF * kF;
-        float w8 = tl(F.w0, F.w1, lvl(kI - 1, s_w, jI, iI + 1), a) * iF * wj0 * kF;
-        ww = w1 + w2 + w3 + w4 + w5 + w6 + w7 + w8;
+    __device__ __forceinline__ void uvw4(int kI, int j, int i, double al, C4f &u, C4f &v, C4f &wt, C4f &wb) const {
+        u = at4(F.u0, F.u1, lvl(kI, s_rho), j, i, al);
+        v = at4(F.v0, F.v1, lvl(kI, s_rho), j, i, al);
+        wt = at4(F.w0, F.w1, lvl(kI, s_w), j, i, al);
+        wb = at4(F.w0, F.w1, lvl(kI - 1, s_w), j, i, al);
     }
-
-    // AKt at the particle (Particle.cpp:490-500)
-    __device__ __forceinline__ float akt(int kI, double kF, int jI, double jF, int iI, double iF, double a) const {
-        const double wi0 = 1.0 - iF, wj0 = 1.0 - jF, wk0 = 1.0 - kF;
-        float a1 = tl(F.akt0, F.akt1, lvl(kI, s_w, jI, iI), a) * wi0 * wj0 * wk0;
-        float a2 = tl(F.akt0, F.akt1, lvl(kI, s_w, jI + 1, iI), a) * wi0 * jF * wk0;
-        float a3 = tl(F.akt0, F.akt1, lvl(kI, s_w, jI + 1, iI + 1), a) * iF * jF * wk0;
-        float a4 = tl(F.akt0, F.akt1, lvl(kI, s_w, jI, iI + 1), a) * iF * wj0 * wk0;
-        float a5 = tl(F.akt0, F.akt1, lvl(kI - 1, s_w, jI, iI), a) * wi0 * wj0 * kF;
-        float a6 = tl(F.akt0, F.akt1, lvl(kI - 1, s_w, jI + 1, iI), a) * wi0 * jF * kF;
-        float a7 = tl(F.akt0, F.akt1, lvl(kI - 1, s_w, jI + 1, iI + 1), a) * iF * jF * kF;
-        float a8 = tl(F.akt0, F.akt1, lvl(kI - 1, s_w, jI, iI + 1), a) * iF * wj0 * kF;
-        return a1 + a2 + a3 + a4 + a5 + a6 + a7 + a8;
+    __device__ __forceinline__ void akt4(int kI, int j, int i, double al, C4f &at, C4f &ab) const {
+        at = at4(F.akt0, F.akt1, lvl(kI, s_w), j, i, al);
+        ab = at4(F.akt0, F.akt1, lvl(kI - 1, s_w), j, i, al);
     }
-
-    // Haversine cell sizes (Particle.cpp:548-574 / 616-621) from the table or
-    // recomputed from lon/lat with the same expression.
+    __device__ __forceinline__ double depth(int kI) const { return G.depth[kI + s_w - 2]; }
+    // haversine cell sizes recomputed with the expression of Particle.cpp:548-574
     __device__ __forceinline__ void metric(int j, int i, double &dx, double &dy) const {
         const size_t c = (size_t)j * xi + i;
-        if (G.dx) {
-            dx = G.dx[c];
-            dy = G.dy[c];
-            return;
-        }
         double dLat = G.latRad[c + xi] - G.latRad[c];
         double t = pow(sin(0.5 * dLat), 2);
         dy = 2.0 * atan2(pow(t, .5), pow(1.0 - t, .5)) * 6371000.0;
         double dLon = G.lonRad[c + 1] - G.lonRad[c];
         t = pow(sin(0.5 * dLon), 2) * cos(G.latRad[c + 1]) * cos(G.latRad[c]);
         dx = 2.0 * atan2(pow(t, .5), pow(1.0 - t, .5)) * 6371000.0;
+    }
+};
+
+// Sampler over the context's packed interval layout: one 16-byte load gives
+// u and v of both records at a corner, one 32-byte load w and AKt of both
+// records at both W levels, one 16-byte load zeta of both records and h.  The
+// values are the same FP32/FP64 numbers as the raw layout, so results are
+// bit-identical; only the number of load instructions changes.
+struct PackedSampler {
+    const PackUV *uv;
+    const PackWA *wa;
+    const PackZD *z;
+    const double2 *md;     // {dx, dy} haversine table
+    const double *depthTab;
+    int xi, nlev, s_w;
+    size_t plane;
+
+    __device__ __forceinline__ void zh4(int j, int i, double al, C4f &zz, C4d &h) const {
+        const size_t q = (size_t)j * xi + i;
+        const PackZD A = z[q], B = z[q + xi], C = z[q + xi + 1], D = z[q + 1];
+        zz = C4f{tl(A.z0, A.z1, al), tl(B.z0, B.z1, al), tl(C.z0, C.z1, al), tl(D.z0, D.z1, al)};
+        h = C4d{A.h, B.h, C.h, D.h};
+    }
+    __device__ __forceinline__ void uvw4(int kI, int j, int i, double al, C4f &u, C4f &v, C4f &wt, C4f &wb) const {
+        const size_t q = (size_t)(kI + nlev - 1) * plane + (size_t)j * xi + i;
+        const PackUV A = uv[q], B = uv[q + xi], C = uv[q + xi + 1], D = uv[q + 1];
+        u = C4f{tl(A.v[0], A.v[1], al), tl(B.v[0], B.v[1], al), tl(C.v[0], C.v[1], al), tl(D.v[0], D.v[1], al)};
+        v = C4f{tl(A.v[2], A.v[3], al), tl(B.v[2], B.v[3], al), tl(C.v[2], C.v[3], al), tl(D.v[2], D.v[3], al)};
+        const PackWA E = wa[q], G2 = wa[q + xi], H = wa[q + xi + 1], K = wa[q + 1];
+        wt = C4f{tl(E.v[0], E.v[1], al), tl(G2.v[0], G2.v[1], al), tl(H.v[0], H.v[1], al), tl(K.v[0], K.v[1], al)};
+        wb = C4f{tl(E.v[2], E.v[3], al), tl(G2.v[2], G2.v[3], al), tl(H.v[2], H.v[3], al), tl(K.v[2], K.v[3], al)};
+    }
+    __device__ __forceinline__ void akt4(int kI, int j, int i, double al, C4f &at, C4f &ab) const {
+        const size_t q = (size_t)(kI + nlev - 1) * plane + (size_t)j * xi + i;
+        const PackWA E = wa[q], G2 = wa[q + xi], H = wa[q + xi + 1], K = wa[q + 1];
+        at = C4f{tl(E.v[4], E.v[5], al), tl(G2.v[4], G2.v[5], al), tl(H.v[4], H.v[5], al), tl(K.v[4], K.v[5], al)};
+        ab = C4f{tl(E.v[6], E.v[7], al), tl(G2.v[6], G2.v[7], al), tl(H.v[6], H.v[7], al), tl(K.v[6], K.v[7], al)};
+    }
+    __device__ __forceinline__ double depth(int kI) const { return depthTab[kI + s_w - 2]; }
+    __device__ __forceinline__ void metric(int j, int i, double &dx, double &dy) const {
+        const double2 m = md[(size_t)j * xi + i];
+        dx = m.x;
+        dy = m.y;
     }
 };
 
@@ -136,9 +190,21 @@ __device__ __forceinline__ double keyed_normal(uint64_t substepKey, uint64_t c) 
     return sqrt(-2.0 * log(u1)) * cos(6.28318530717958647692 * u2);
 }
 
+// advectiveVelocity, passive branch (Particle.cpp:352-475)
+template <class S>
+__device__ __forceinline__ void velocity(const S &smp, int kI, double kF, int jI, double jF, int iI, double iF,
+                                         double a, float &uu, float &vv, float &ww) {
+    C4f u, v, wt, wb;
+    smp.uvw4(kI, jI, iI, a, u, v, wt, wb);
+    uu = bil(u, iF, jF);
+    vv = bil(v, iF, jF);
+    ww = tril(wt, wb, iF, jF, kF);
+}
+
 // Particle::move for one particle (Particle.cpp:71-824); returns the number of
 // live substeps taken.
-__device__ unsigned move_one(const MoveParams &P, const Sampler &S, State &p) {
+template <class Smp>
+__device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
     unsigned steps = 0;
     const double L = P.intervalLength;
     double elapsed = P.elapsed0;
@@ -182,28 +248,29 @@ __device__ unsigned move_one(const MoveParams &P, const Sampler &S, State &p) {
         const double jF = p.j - jI, iF = p.i - iI;
 
         // zeta, h at the particle (:271-290)
-        float z1 = S.zeta(jI, iI, alpha) * (1.0 - iF) * (1.0 - jF);
-        float z2 = S.zeta(jI + 1, iI, alpha) * (1.0 - iF) * jF;
-        float z3 = S.zeta(jI + 1, iI + 1, alpha) * iF * jF;
-        float z4 = S.zeta(jI, iI + 1, alpha) * iF * (1.0 - jF);
-        float zz = z1 + z2 + z3 + z4;
-        double h1 = S.h(jI, iI) * (1.0 - iF) * (1.0 - jF);
-        double h2 = S.h(jI + 1, iI) * (1.0 - iF) * jF;
-        double h3 = S.h(jI + 1, iI + 1) * iF * jF;
-        double h4 = S.h(jI, iI + 1) * iF * (1.0 - jF);
+        C4f zc;
+        C4d hcn;
+        S.zh4(jI, iI, alpha, zc, hcn);
+        const float zz = bil(zc, iF, jF);
+        double h1 = hcn.a * (1.0 - iF) * (1.0 - jF);
+        double h2 = hcn.b * (1.0 - iF) * jF;
+        double h3 = hcn.c * iF * jF;
+        double h4 = hcn.d * iF * (1.0 - jF);
         double hh = h1 + h2 + h3 + h4;
         double hc = hh + zz;
         if (hc <= P.shoreLimit && P.horizontal == WC_CLOSURE_KILL) p.health = -1;   // :311-338
 
         float uu, vv, ww;                                   // :479-481
-        S.velocity(kI, kF, jI, jF, iI, iF, P.midpoint ? alphaStart : alpha, uu, vv, ww);
+        velocity(S, kI, kF, jI, jF, iI, iF, P.midpoint ? alphaStart : alpha, uu, vv, ww);
 
         double dxleap = direction * uu * stepDt;            // :507-510
         double dyleap = direction * vv * stepDt;
         double dzleap = direction * (P.sv + ww) * stepDt;
         double rxleap = 0, ryleap = 0, rzleap = 0;
-        if (P.randomOn) {                                   // :518-533
-            const float aa = S.akt(kI, kF, jI, jF, iI, iF, alpha);
+        if (P.randomOn) {                                   // :490-500, 518-533
+            C4f at, ab;
+            S.akt4(kI, jI, iI, alpha, at, ab);
+            const float aa = tril(at, ab, iF, jF, kF);
             const double sigmaDepth = P.sigma * (1 - p.k / kLL);
             const uint64_t substep = (uint64_t)floor(elapsed / P.dti);
             const double stochasticScale = sqrt(stepDt / P.dti);
@@ -218,7 +285,7 @@ __device__ unsigned move_one(const MoveParams &P, const Sampler &S, State &p) {
 
         double xdist, ydist;                                // :544-579
         S.metric(jI, iI, xdist, ydist);
-        const double zdist = hc * S.G.depth[kI + P.s_w - 2];
+        const double zdist = hc * S.depth(kI);
 
         if (P.midpoint) {                                   // :585-641
             double kHalfLeap = direction * (P.sv + ww) * 0.5 * stepDt;
@@ -234,28 +301,29 @@ __device__ unsigned move_one(const MoveParams &P, const Sampler &S, State &p) {
             if (midValid) {
                 iMI = (int)iMd; jMI = (int)jMd;
                 iMF = iMid - iMI; jMF = jMid - jMI;
-                hcMid = S.h(jMI, iMI) * (1.0 - iMF) * (1.0 - jMF) + S.h(jMI + 1, iMI) * (1.0 - iMF) * jMF +
-                        S.h(jMI + 1, iMI + 1) * iMF * jMF + S.h(jMI, iMI + 1) * iMF * (1.0 - jMF) +
-                        S.zeta(jMI, iMI, alpha) * (1.0 - iMF) * (1.0 - jMF) +
-                        S.zeta(jMI + 1, iMI, alpha) * (1.0 - iMF) * jMF +
-                        S.zeta(jMI + 1, iMI + 1, alpha) * iMF * jMF +
-                        S.zeta(jMI, iMI + 1, alpha) * iMF * (1.0 - jMF);
+                C4f zm;
+                C4d hm;
+                S.zh4(jMI, iMI, alpha, zm, hm);
+                hcMid = hm.a * (1.0 - iMF) * (1.0 - jMF) + hm.b * (1.0 - iMF) * jMF +
+                        hm.c * iMF * jMF + hm.d * iMF * (1.0 - jMF) +
+                        zm.a * (1.0 - iMF) * (1.0 - jMF) + zm.b * (1.0 - iMF) * jMF +
+                        zm.c * iMF * jMF + zm.d * iMF * (1.0 - jMF);
                 midValid = hcMid > P.shoreLimit && hcMid > 0;
             }
             if (midValid) {
                 const int kMI = (int)ceil(kMid);
                 const double kMF = ceil(kMid) - kMid;
                 float um, vm, wm;
-                S.velocity(kMI, kMF, jMI, jMF, iMI, iMF, alpha, um, vm, wm);
+                velocity(S, kMI, kMF, jMI, jMF, iMI, iMF, alpha, um, vm, wm);
                 double xdistMid, ydistMid;
                 S.metric(jMI, iMI, xdistMid, ydistMid);
-                const double zdistMid = hcMid * S.G.depth[kMI + P.s_w - 2];
+                const double zdistMid = hcMid * S.depth(kMI);
                 dxleap = direction * um * stepDt * xdist / xdistMid;
                 dyleap = direction * vm * stepDt * ydist / ydistMid;
                 dzleap = direction * (P.sv + wm) * stepDt * zdist / zdistMid;
             } else {
                 float ue, ve, we;
-                S.velocity(kI, kF, jI, jF, iI, iF, alpha, ue, ve, we);
+                velocity(S, kI, kF, jI, jF, iI, iF, alpha, ue, ve, we);
                 dxleap = direction * ue * stepDt;
                 dyleap = direction * ve * stepDt;
                 dzleap = direction * (P.sv + we) * stepDt;
@@ -285,16 +353,15 @@ __device__ unsigned move_one(const MoveParams &P, const Sampler &S, State &p) {
         if (valid_cell(jdd, idd, eta, xi)) {
             const int jdetI = (int)jdd, idetI = (int)idd;
             const double idetF = idet - idetI, jdetF = jdet - jdetI;   // :716-739
-            z1 = S.zeta(jdetI, idetI, alpha) * (1.0 - idetF) * (1.0 - jdetF);
-            z2 = S.zeta(jdetI + 1, idetI, alpha) * (1.0 - idetF) * jdetF;
-            z3 = S.zeta(jdetI + 1, idetI + 1, alpha) * idetF * jdetF;
-            z4 = S.zeta(jdetI, idetI + 1, alpha) * idetF * (1.0 - jdetF);
-            float zzdet = z1 + z2 + z3 + z4;
-            h1 = S.h(jdetI, idetI) * (1.0 - idetF) * (1.0 - jdetF);
-            h2 = S.h(jdetI + 1, idetI) * (1.0 - idetF) * jdetF;
-            h3 = S.h(jdetI + 1, idetI + 1) * idetF * jdetF;
-            h4 = S.h(jdetI, idetI + 1) * idetF * (1.0 - jdetF);
-            double hcdet = (h1 + h2 + h3 + h4) + zzdet;
+            C4f zd;
+            C4d hd;
+            S.zh4(jdetI, idetI, alpha, zd, hd);
+            const float zzdet = bil(zd, idetF, jdetF);
+            h1 = hd.a * (1.0 - idetF) * (1.0 - jdetF);
+            h2 = hd.b * (1.0 - idetF) * jdetF;
+            h3 = hd.c * idetF * jdetF;
+            h4 = hd.d * idetF * (1.0 - jdetF);
+            const double hcdet = (h1 + h2 + h3 + h4) + zzdet;
             if (hcdet <= P.shoreLimit) {                    // :753-771
                 if (P.horizontal == WC_CLOSURE_CONSTRAINT) { idet = p.i; jdet = p.j; }
                 else if (P.horizontal == WC_CLOSURE_KILL) p.health = -1;
@@ -322,13 +389,12 @@ __device__ unsigned move_one(const MoveParams &P, const Sampler &S, State &p) {
     return steps;
 }
 
-template <class Acc>
-__global__ __launch_bounds__(256) void move_parity_kernel(MoveParams P, RecordPair<float> F, GridPtrs G, Acc acc,
-                                                          int n, unsigned long long *stepCounter) {
+template <class Smp, class Acc>
+__global__ __launch_bounds__(256) void move_parity_kernel(MoveParams P, Smp S, Acc acc, int n,
+                                                          unsigned long long *stepCounter) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned steps = 0;
     if (idx < n) {
-        Sampler S{F, G, P.xi, (size_t)P.eta * P.xi, P.s_rho, P.s_w};
         State p;
         acc.load(idx, p);
         steps = move_one(P, S, p);
@@ -337,14 +403,56 @@ __global__ __launch_bounds__(256) void move_parity_kernel(MoveParams P, RecordPa
     if (stepCounter) wave_add_u64(stepCounter, steps);
 }
 
+// Packs the interval's two FP32 records (and FP64 h) into the gather layout.
+__global__ void pack_parity_kernel(RecordPair<float> F, const double *__restrict__ h, PackUV *__restrict__ uv,
+                                   PackWA *__restrict__ wa, PackZD *__restrict__ z, int s_rho, int s_w,
+                                   size_t plane) {
+    const int nlev = s_w - 1;
+    const size_t total = plane * nlev;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t lv = q / plane, c = q - lv * plane;
+        const int k = (int)lv - (nlev - 1);
+        const size_t r = (size_t)(k + s_rho - 1) * plane + c;
+        const size_t wt = (size_t)(k + s_w - 1) * plane + c, wb = wt - plane;
+        uv[q] = PackUV{{F.u0[r], F.u1[r], F.v0[r], F.v1[r]}};
+        wa[q] = PackWA{{F.w0[wt], F.w1[wt], F.w0[wb], F.w1[wb], F.akt0[wt], F.akt1[wt], F.akt0[wb], F.akt1[wb]}};
+        if (lv == 0) z[c] = PackZD{F.zeta0[c], F.zeta1[c], h[c]};
+    }
+}
+
 }  // namespace
 
-hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, const SoA &s,
-                                  int n, unsigned long long *stepCounter, hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
+size_t parity_pack_bytes(int s_w, size_t plane) {
+    return plane * (size_t)(s_w - 1) * (sizeof(PackUV) + sizeof(PackWA)) + plane * sizeof(PackZD);
+}
+
+hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
+                                  const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
+                                  hipStream_t stream) {
+    const size_t plane = (size_t)P.eta * P.xi;
+    const int nlev = P.s_w - 1;
+    char *base = (char *)packBuf;
+    PackedSampler S;
+    S.uv = (const PackUV *)base;
+    base += sizeof(PackUV) * plane * nlev;
+    S.wa = (const PackWA *)base;
+    base += sizeof(PackWA) * plane * nlev;
+    S.z = (const PackZD *)base;
+    S.md = metric;
+    S.depthTab = G.depth;
+    S.xi = P.xi;
+    S.nlev = nlev;
+    S.s_w = P.s_w;
+    S.plane = plane;
+    size_t blocks = (plane * nlev + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    pack_parity_kernel<<<(int)blocks, 256, 0, stream>>>(F, G.h, (PackUV *)S.uv, (PackWA *)S.wa, (PackZD *)S.z,
+                                                        P.s_rho, P.s_w, plane);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n <= 0) return e;
     const int block = 256;
-    const int grid = (n + block - 1) / block;
-    move_parity_kernel<SoAAccess><<<grid, block, 0, stream>>>(P, F, G, SoAAccess{s}, n, stepCounter);
+    move_parity_kernel<PackedSampler, SoAAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, SoAAccess{s}, n,
+                                                                                                stepCounter);
     return hipGetLastError();
 }
 
@@ -352,8 +460,9 @@ hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &
                                   wc_particle_data *p, int n, unsigned long long *stepCounter, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int block = 256;
-    const int grid = (n + block - 1) / block;
-    move_parity_kernel<AoSAccess><<<grid, block, 0, stream>>>(P, F, G, AoSAccess{p}, n, stepCounter);
+    RawSampler S{F, G, P.xi, (size_t)P.eta * P.xi, P.s_rho, P.s_w};
+    move_parity_kernel<RawSampler, AoSAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, AoSAccess{p}, n,
+                                                                                             stepCounter);
     return hipGetLastError();
 }
 
