@@ -150,6 +150,7 @@ def main():
     precision = {}
     if not args.no_variants:
         det = cfg.copy(random=False)
+        pp = release(W, args.particles, 50, spread=0.25)      # distinct trajectories
         finals = {}
         for path, store in [("parity", "f32"), ("fast", "bf16"), ("fast", "f16"), ("fast", "f32")]:
             key = "f64" if path == "parity" else store
@@ -157,7 +158,7 @@ def main():
                          store=store)
             for t in range(n_slots):
                 ctx.upload_record(t, *W.record(t))
-            ctx.set_particles(p0)
+            ctx.set_particles(pp)
             for s in range(args.steps):
                 ctx.move_interval(det, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
             finals[key] = ctx.get_particles()
@@ -244,6 +245,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": variants_out,
         "precision_vs_f64": {"hours": args.steps, "diffusion": "off", "particles": args.particles,
+                             "release": "50 sources, per-particle N(0, 0.25^2)-cell offsets",
                              **precision},
         "pcie_inclusive_value": pcie,
         "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),

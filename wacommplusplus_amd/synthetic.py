@@ -187,9 +187,11 @@ def point_sources(window: ForcingWindow, n_sources: int, seed: int = 5489, margi
 
 
 def release(window: ForcingWindow, n_particles: int, n_sources: int, depth_m: float = 1.0, time: float = 0.0,
-            seed: int = 5489, dtype=None):
+            seed: int = 5489, spread: float = 0.0):
     """A single pulse of n_particles split evenly over n_sources point
-    sources (ids 0..n-1 in source order, Source.cpp:25-44 numbering)."""
+    sources (ids 0..n-1 in source order, Source.cpp:25-44 numbering).
+    spread > 0 adds seeded N(0, spread^2) offsets to j and i of each particle,
+    as randomSources does with 0.25 (SourceEmission.hpp:20-23)."""
     from .types import make_particles
     js, is_ = point_sources(window, n_sources, seed)
     per = np.full(n_sources, n_particles // n_sources)
@@ -199,4 +201,8 @@ def release(window: ForcingWindow, n_particles: int, n_sources: int, depth_m: fl
     p["k"] = depth_to_k(depth_m, h_src, window.s_w)
     p["j"] = np.repeat(js, per)
     p["i"] = np.repeat(is_, per)
+    if spread > 0:
+        rng = np.random.default_rng(seed + 2)
+        p["j"] += spread * rng.standard_normal(n_particles)
+        p["i"] += spread * rng.standard_normal(n_particles)
     return p
