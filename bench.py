@@ -145,31 +145,46 @@ def main():
         results[key] = run_timed(wc, ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key)
         log(f"[{key}] {results[key]}")
 
-    # precision of the mixed-precision path against the FP64 path (diffusion
-    # off, same forcing and release, `steps` intervals, whole population)
-    precision = {}
+    # precision of the mixed-precision path against the FP64 path:
+    #  (a) per particle, diffusion off, whole population, `steps` intervals;
+    #  (b) the count grid after `steps` intervals, diffusion on, against the
+    #      difference between two FP64 runs with different seeds
+    precision = {"per_particle": {}, "conc_l2": {}}
     if not args.no_variants:
         det = cfg.copy(random=False)
         pp = release(W, args.particles, 50, spread=0.25)      # distinct trajectories
-        finals = {}
+        finals, grids = {}, {}
         for path, store in [("parity", "f32"), ("fast", "bf16"), ("fast", "f16"), ("fast", "f32")]:
             key = "f64" if path == "parity" else store
             ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=n_slots,
                          store=store)
             for t in range(n_slots):
                 ctx.upload_record(t, *W.record(t))
-            ctx.set_particles(pp)
-            for s in range(args.steps):
-                ctx.move_interval(det, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
-            finals[key] = ctx.get_particles()
+            runs = [("det", det, pp), ("rnd", cfg, pp[::10])]
+            if key == "f64":
+                runs.append(("rnd2", cfg.copy(randomSeed=1234), pp[::10]))
+            for tag, c, parts in runs:
+                ctx.set_particles(parts)
+                for s in range(args.steps):
+                    ctx.move_interval(c, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
+                if tag == "det":
+                    finals[key] = ctx.get_particles()
+                else:
+                    grids[(key, tag)] = ctx.grid().astype(np.float64)
             log(f"[precision] {key} done")
         ref = finals.pop("f64")
         for key, a in finals.items():
             both = (a["health"] > 0) & (ref["health"] > 0)
             d = np.max(np.stack([np.abs(a[f][both] - ref[f][both]) for f in ("k", "j", "i")]), axis=0)
             q = np.quantile(d, [0.5, 0.9, 0.99])
-            precision[key] = dict(p50_cells=float(q[0]), p90_cells=float(q[1]), p99_cells=float(q[2]),
-                                  max_cells=float(d.max()), both_alive=float(both.mean()))
+            precision["per_particle"][key] = dict(p50_cells=float(q[0]), p90_cells=float(q[1]),
+                                                  p99_cells=float(q[2]), max_cells=float(d.max()),
+                                                  both_alive=float(both.mean()))
+        g0 = grids[("f64", "rnd")]
+        nrm = np.linalg.norm(g0)
+        precision["conc_l2"]["f64_seed_to_seed"] = float(np.linalg.norm(grids[("f64", "rnd2")] - g0) / nrm)
+        for key in ("bf16", "f16", "f32"):
+            precision["conc_l2"][key] = float(np.linalg.norm(grids[(key, "rnd")] - g0) / nrm)
         log(f"[precision] {precision}")
 
     # PCIe-inclusive rate: two resident slots, one record uploaded per interval
@@ -244,9 +259,13 @@ def main():
         },
         "cpu_baseline": cpu,
         "variants": variants_out,
-        "precision_vs_f64": {"hours": args.steps, "diffusion": "off", "particles": args.particles,
-                             "release": "50 sources, per-particle N(0, 0.25^2)-cell offsets",
-                             **precision},
+        "precision_vs_f64": {"hours": args.steps,
+                             "per_particle": {"diffusion": "off", "particles": args.particles,
+                                              "release": "50 sources, per-particle N(0, 0.25^2)-cell offsets",
+                                              **precision["per_particle"]},
+                             "conc_l2": {"diffusion": "on", "particles": args.particles // 10,
+                                         "what": "relative L2 of the count grid vs the FP64 path",
+                                         **precision["conc_l2"]}},
         "pcie_inclusive_value": pcie,
         "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),
         "alive_after": main_r["alive"],

@@ -261,22 +261,16 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
             const float2 md = S.md[c];
             const float idx_ = md.x, idy_ = md.y;
             const float zdist = hc * S.depthf[kI + P.s_w - 2];
-            // Vertical step in levels.  The reference caps |leap| at zdist and
-            // divides by zdist (Particle.cpp:644-657), so as the water column
-            // hc -> 0+ the step tends to one level in the direction of the
-            // uncapped leap.  A particle pinned on a coast line can sit there
-            // with hc ~ 1e-11 in FP64 and hc == 0 in FP32 (0/0 in the formula);
-            // this path takes the limit for zdist == 0.
-            auto levels = [&](float leap, float sgnHint) -> float {
-                if (zdist > 0.f) {
-                    if (fabsf(leap) > zdist) leap = copysignf(zdist, leap);
-                    return leap / zdist;
-                }
-                const float d = leap != 0.f ? leap : sgnHint;
-                return (float)((d > 0.f) - (d < 0.f));
+            // Vertical step in levels: the leap capped at one cell height and
+            // divided by it (Particle.cpp:585-594, 644-657).  Where the water
+            // column has no depth (hc == 0: a particle exactly on a land node
+            // line) the reference arithmetic gives 0/0; as in the FP64 path
+            // the non-finite level then ends the particle at its next substep.
+            auto levels = [&](float leap) -> float {
+                if (fabsf(leap) > zdist) leap = copysignf(fabsf(zdist), leap);
+                return leap / zdist;
             };
             float dxl = dir * uu * dt, dyl = dir * vv * dt, dzl = dir * (sv + ww) * dt;
-            float dzSign = dzl;           // sign of the deterministic vertical leap
 
             if (P.midpoint) {                                      // Particle.cpp:585-641
                 const float kHalf = dir * (sv + ww) * 0.5f * dt;
@@ -284,7 +278,7 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
                 float fjm, fim;
                 advance(jc, fj, dir * vv * 0.5f * dt * idy_, jm, fjm);
                 advance(ic, fi, dir * uu * 0.5f * dt * idx_, im, fim);
-                const float kMid = fminf(0.0f, fmaxf(kLL, k + levels(kHalf, 0.f)));
+                const float kMid = fminf(0.0f, fmaxf(kLL, k + levels(kHalf)));
                 bool midValid = valid_cell(jm, im, eta, xi);
                 float hcMid = 0;
                 size_t cm = 0;
@@ -306,7 +300,6 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
                     dxl = dir * um * dt * (mdm.x / idx_);
                     dyl = dir * vm * dt * (mdm.y / idy_);
                     dzl = dir * (sv + wmv) * dt * (zdist / (hcMid * S.depthf[kMI + P.s_w - 2]));
-                    dzSign = dir * (sv + wmv);
                 } else {
                     float ue, ve, we, dummy;
                     S.uv_at(kI, c, w0, alpha, ue, ve);
@@ -314,7 +307,6 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
                     dxl = dir * ue * dt;
                     dyl = dir * ve * dt;
                     dzl = dir * (sv + we) * dt;
-                    dzSign = dzl;
                 }
             }
             const float xleap = dxl + rx, yleap = dyl + ry;
@@ -323,7 +315,7 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
             float fjn, fin;
             advance(jc, fj, yleap * idy_, jn, fjn);
             advance(ic, fi, xleap * idx_, in_, fin);
-            float kdet = k + levels(zleap, dzSign);
+            float kdet = k + levels(zleap);
             if (kdet > 0.f) {                                      // :665-704
                 if (P.upper == WC_CLOSURE_CONSTRAINT) kdet = 0.f;
                 else if (P.upper == WC_CLOSURE_KILL) kill = true;
