@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--particles", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the precision comparison (its diffusion-off runs launch the move kernels "
+                         "on other populations, which a kernel-trace average would mix in)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is described "
@@ -150,7 +153,7 @@ def main():
     #  (b) the count grid after `steps` intervals, diffusion on, against the
     #      difference between two FP64 runs with different seeds
     precision = {"per_particle": {}, "conc_l2": {}}
-    if not args.no_variants:
+    if not args.no_variants and not args.no_precision:
         det = cfg.copy(random=False)
         pp = release(W, args.particles, 50, spread=0.25)      # distinct trajectories
         finals, grids = {}, {}
