@@ -81,6 +81,52 @@ def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
                 grid_ms=grid_ms, grid_launches=grid_n, alive=ctx.count())
 
 
+def roms_records(W, t, mask_u, mask_v):
+    """Record t of the window restated on ROMS's staggered grids: u at the
+    u faces (mean of the two rho points beside each face), v at the v faces,
+    fill values on land faces."""
+    zeta, u, v, w, akt = W.record(t)
+    uf = (0.5 * (u[:, :, :-1] + u[:, :, 1:])).astype(np.float32)
+    vf = (0.5 * (v[:, :-1, :] + v[:, 1:, :])).astype(np.float32)
+    uf[:, mask_u == 0] = np.float32(9.9692099683868690e+36)
+    vf[:, mask_v == 0] = np.float32(9.9692099683868690e+36)
+    return zeta, uf, vf, w, akt
+
+
+def roms_normalize_bench(wc, ctx, W, n_records):
+    """ROMS normalisation of full-size records (1024x768, 11 levels) into
+    forcing slots: device kernel time per record and the algorithmic bytes
+    it moves, plus the host-to-device upload of the raw record."""
+    m = W.mask
+    mask_u = m[:, :-1] * m[:, 1:]
+    mask_v = m[:-1, :] * m[1:, :]
+    lon, lat = W.lonRad / 0.0174533, W.latRad / 0.0174533
+    angle = np.full(m.shape, 0.3)
+    ctx.set_roms_grid(np.linspace(-1.0, 0.0, W.s_w), lon, lat, W.h, m, mask_u, mask_v, angle=angle, n_slots=2)
+    recs = [roms_records(W, t, mask_u, mask_v) for t in range(min(n_records, W.T))]
+    ctx.upload_roms_record(0, *recs[0])            # warm-up
+    ctx.sync()
+    ctx.reset_kernel_time()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for q, r in enumerate(recs):
+        ctx.upload_roms_record(q % 2, *r)
+    wall = time.perf_counter() - t0
+    ctx.set_timing(False)
+    ms, n = ctx.kernel_time(2)
+    plane, s_rho = W.eta * W.xi, W.s_rho
+    pu, pv = mask_u.size, mask_v.size
+    land = int((m == 0).sum())
+    # reads: raw u, v, zeta, the three masks, {cos, sin} at wet points;
+    # writes: U, V, and W, AKT, zeta at land points
+    nbytes = (4 * s_rho * (pu + pv) + 4 * plane + plane + pu + pv + 16 * (plane - land) +
+              8 * s_rho * plane + 4 * (2 * W.s_w + 1) * land)
+    kernel_ms = ms / max(1, n)
+    achieved = nbytes / (kernel_ms / 1e3) / 1e9
+    return dict(records=n, kernel_ms=kernel_ms, algorithmic_bytes=nbytes, achieved_gbs=achieved,
+                frac=achieved / HBM_PEAK_GBS, upload_ms_pcie_inclusive=1e3 * wall / len(recs))
+
+
 def cpu_baseline(W, cfg, p0, n_sample=100_000, n_intervals=4):
     """The oracle (FP64 C restatement of Particle::move, OpenMP dynamic,64 as
     Wacomm.cpp:501) on a bounded sample of the same workload."""
@@ -207,6 +253,8 @@ def main():
         ctx.sync()
         pcie = ctx.step_count() / (time.perf_counter() - t0)
         log(f"[pcie] {pcie:.4g} particle-steps/s")
+    roms = roms_normalize_bench(wc, ctx, W, args.steps)
+    log(f"[roms] {roms}")
     ctx.close()
 
     main_r = results["f64"]
@@ -270,6 +318,7 @@ def main():
                                          "what": "relative L2 of the count grid vs the FP64 path",
                                          **precision["conc_l2"]}},
         "pcie_inclusive_value": pcie,
+        "roms_normalize": roms,
         "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),
         "alive_after": main_r["alive"],
     }

@@ -175,11 +175,63 @@ int wc_set_grid(wc_ctx *ctx, int s_w, int s_rho, int eta, int xi, const double *
                 const double *h, const double *mask, const double *depthIntervals, int n_slots, int store);
 
 /* Uploads one forcing record (host FP32 arrays: zeta [eta][xi], u, v
- * [s_rho][eta][xi], w, akt [s_w][eta][xi]) into ring slot `slot`, converting
- * to the context's storage type.  Replaces the whole-window re-upload of
- * Wacomm.cpp:417-477 by one record per interval. */
+ * [s_rho][eta][xi], w, akt [s_w][eta][xi]) into ring slot `slot`; slots hold
+ * FP32 and each interval's pack converts the record pair to the context's
+ * storage type.  Replaces the whole-window re-upload of Wacomm.cpp:417-477
+ * by one record per interval. */
 int wc_upload_record(wc_ctx *ctx, int slot, const float *zeta, const float *u, const float *v, const float *w,
                      const float *akt);
+
+/* Copies ring slot `slot` back to the host as the move reads it (FP32, the
+ * layout of wc_upload_record); a NULL field is skipped.  Replaces reads of
+ * OceanModelAdapter::Zeta/U/V/W/AKT (OceanModelAdapter.hpp:20-34), e.g. the
+ * assertions of tests/ROMSAdapterTest.cpp:67-74. */
+int wc_get_record(wc_ctx *ctx, int slot, float *zeta, float *u, float *v, float *w, float *akt);
+
+/* The static grid of a ROMS file, as ROMSAdapter::process reads it
+ * (ROMSAdapter.cpp:56-166).  All arrays are host pointers, row-major:
+ * lon_rho, lat_rho (degrees), h, mask_rho, angle (radians): [eta_rho][xi_rho];
+ * mask_u: [eta_u][xi_u]; mask_v: [eta_v][xi_v]; s_w: the s_rho+1 sigma values
+ * of the W levels in file order (bottom, -1, first).  Masks hold 0 or 1.
+ * earth_relative: u and v carry standard_name eastward/northward_sea_water_
+ * velocity, so no rotation (angle may then be NULL).  missing_*: the
+ * variable's _FillValue / missing_value attributes (at most
+ * WC_ROMS_MAX_MISSING each); the NetCDF float and double fill values are
+ * always treated as missing too (ROMSAdapter.cpp:29-43).  The attribute and
+ * units checks on the file's metadata belong to the file reader. */
+#define WC_ROMS_MAX_MISSING 14
+typedef struct wc_roms_grid {
+    int s_rho, eta_rho, xi_rho, eta_u, xi_u, eta_v, xi_v;
+    const double *s_w;
+    const double *lon_rho, *lat_rho, *h;
+    const double *mask_rho, *mask_u, *mask_v;
+    const double *angle;
+    int earth_relative;
+    const double *missing_u, *missing_v, *missing_angle;
+    int n_missing_u, n_missing_v, n_missing_angle;
+} wc_roms_grid;
+
+/* Replaces the static part of ROMSAdapter::process (ROMSAdapter.cpp:
+ * 146-151, 228-275): checks the masks (0/1 only) and the angle at wet rho
+ * points (finite, not missing), then calls wc_set_grid with lonRad = latRad
+ * = 0.0174533 * degrees, h zeroed on land, depthIntervals(k) = s_w(k) -
+ * s_w(k-1), mask = mask_rho.  The u and v grids must not exceed the rho grid
+ * (ROMS staggering: eta_u = eta_rho, xi_u = xi_rho-1, eta_v = eta_rho-1,
+ * xi_v = xi_rho).  Returns WC_ERR_INVALID where the reference throws. */
+int wc_set_roms_grid(wc_ctx *ctx, const wc_roms_grid *grid, int n_slots, int store);
+
+/* Replaces the per-record part of ROMSAdapter::process (ROMSAdapter.cpp:
+ * 152-166, 254-300, 309-368, 448-466) on the device, writing ring slot
+ * `slot`: host FP32 zeta [eta_rho][xi_rho], u [s_rho][eta_u][xi_u],
+ * v [s_rho][eta_v][xi_v], w, akt [s_rho+1][eta_rho][xi_rho].  u and v are
+ * averaged over wet faces onto wet rho points (float), rotated by the grid
+ * angle into east/north (double, rounded to float) unless earth_relative;
+ * w, akt are zeroed on land; zeta is zeroed on land and where >= 1e36.
+ * Bit-identical to the reference's arithmetic.  Returns WC_ERR_INVALID (slot
+ * left unfilled) for a non-finite or missing u/v at a wet face or a rotated
+ * velocity outside float range, where the reference throws. */
+int wc_upload_roms_record(wc_ctx *ctx, int slot, const float *zeta, const float *u, const float *v,
+                          const float *w, const float *akt);
 
 /* Particle state: host AoS in and out (Particles.hpp:17 holds particle_data).
  * wc_set_particles replaces the device population; wc_append_particles adds
@@ -256,7 +308,7 @@ int wc_run_window(wc_ctx *ctx, const wc_config_data *config, const wc_window *wi
 
 /* Accumulated device time of the move kernels launched since the last reset,
  * measured with HIP events on the context stream around each launch (enable
- * with wc_set_timing).  kernel: 0 = move, 1 = grid. */
+ * with wc_set_timing).  kernel: 0 = move, 1 = grid, 2 = ROMS normalisation. */
 int wc_set_timing(wc_ctx *ctx, int enabled);
 int wc_kernel_time(wc_ctx *ctx, int kernel, double *total_ms, long *launches);
 int wc_reset_kernel_time(wc_ctx *ctx);

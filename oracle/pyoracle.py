@@ -103,6 +103,14 @@ class _Record(C.Structure):
                 ("akt", C.c_void_p)]
 
 
+class _RomsGrid(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("s_rho", "eta", "xi", "eta_u", "xi_u", "eta_v", "xi_v")] + \
+               [(n, C.c_void_p) for n in ("mask_rho", "mask_u", "mask_v", "angle")] + \
+               [("earth_relative", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("missing_u", "missing_v", "missing_angle")] + \
+               [(n, C.c_int) for n in ("n_missing_u", "n_missing_v", "n_missing_angle")]
+
+
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -131,6 +139,9 @@ def _load(path):
                                        C.c_double, C.c_void_p, C.c_long]
     lib.wco_cell_metric.argtypes = [C.POINTER(_Grid), C.c_int, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]
+    lib.wco_roms_normalize.restype = C.c_int
+    lib.wco_roms_normalize.argtypes = [C.POINTER(_RomsGrid)] + [C.c_void_p] * 10
+    lib.wco_roms_static.argtypes = [C.c_int] * 3 + [C.c_void_p] * 9
     return lib
 
 
@@ -250,3 +261,48 @@ def emission_times(mode, value, source_start, source_end, interval_start, interv
     if n < 0:
         return n
     return buf[:n].copy()
+
+
+class RomsGrid:
+    """The static ROMS grid of one file: rho/u/v masks, the grid angle and
+    the missing-value attributes (ROMSAdapter.cpp:63-166)."""
+
+    def __init__(self, s_rho, mask_rho, mask_u, mask_v, angle=None, earth_relative=False,
+                 missing_u=(), missing_v=(), missing_angle=()):
+        self.mask_rho = np.ascontiguousarray(mask_rho, dtype=np.float64)
+        self.mask_u = np.ascontiguousarray(mask_u, dtype=np.float64)
+        self.mask_v = np.ascontiguousarray(mask_v, dtype=np.float64)
+        eta, xi = self.mask_rho.shape
+        self.angle = np.ascontiguousarray(np.zeros((eta, xi)) if angle is None else angle, dtype=np.float64)
+        self.missing = [np.ascontiguousarray(np.r_[m, 0.0][: max(1, len(m))], dtype=np.float64)
+                        for m in (missing_u, missing_v, missing_angle)]
+        self.counts = [len(m) for m in (missing_u, missing_v, missing_angle)]
+        self.s_rho, self.eta, self.xi = s_rho, eta, xi
+        self._g = _RomsGrid(s_rho, eta, xi, *self.mask_u.shape, *self.mask_v.shape,
+                            _ptr(self.mask_rho).value, _ptr(self.mask_u).value, _ptr(self.mask_v).value,
+                            _ptr(self.angle).value, int(bool(earth_relative)),
+                            *[_ptr(m).value for m in self.missing], *self.counts)
+
+
+def roms_normalize(g: RomsGrid, zeta, u, v, w, akt):
+    """One ROMS record onto the rho grid: (rc, zeta, U, V, W, AKT)."""
+    ins = [np.ascontiguousarray(a, dtype=np.float32) for a in (zeta, u, v, w, akt)]
+    assert ins[1].shape == (g.s_rho,) + g.mask_u.shape and ins[2].shape == (g.s_rho,) + g.mask_v.shape
+    assert ins[0].shape == (g.eta, g.xi) and ins[3].shape == ins[4].shape == (g.s_rho + 1, g.eta, g.xi)
+    outs = [np.zeros((g.eta, g.xi), np.float32), np.zeros((g.s_rho, g.eta, g.xi), np.float32),
+            np.zeros((g.s_rho, g.eta, g.xi), np.float32), np.zeros((g.s_rho + 1, g.eta, g.xi), np.float32),
+            np.zeros((g.s_rho + 1, g.eta, g.xi), np.float32)]
+    rc = lib().wco_roms_normalize(C.byref(g._g), *[_ptr(a) for a in ins], *[_ptr(a) for a in outs])
+    return (rc, *outs)
+
+
+def roms_static(s_w_values, lon, lat, h, mask_rho):
+    """(lonRad, latRad, h, depthIntervals) of ROMSAdapter.cpp:232-275."""
+    sw = np.ascontiguousarray(s_w_values, dtype=np.float64)
+    lon, lat, h, m = (np.ascontiguousarray(a, dtype=np.float64) for a in (lon, lat, h, mask_rho))
+    eta, xi = m.shape
+    lr, la, ho = np.zeros((eta, xi)), np.zeros((eta, xi)), np.zeros((eta, xi))
+    di = np.zeros(len(sw) - 1)
+    lib().wco_roms_static(len(sw) - 1, eta, xi, _ptr(sw), _ptr(lon), _ptr(lat), _ptr(h), _ptr(m), _ptr(lr),
+                          _ptr(la), _ptr(ho), _ptr(di))
+    return lr, la, ho, di

@@ -11,6 +11,7 @@
  */
 #include "wc_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <string.h>
 
@@ -544,4 +545,120 @@ long wco_emission_times(int mode, double value, double sourceStart, double sourc
         }
     }
     return n;
+}
+
+/* ---- ROMS normalisation: ROMSAdapter.cpp:29-49, 56-300, 309-368, 448-466 */
+
+#define WCO_NC_FILL_FLOAT 9.9692099683868690e+36f    /* netcdf.h NC_FILL_FLOAT */
+#define WCO_NC_FILL_DOUBLE 9.9692099683868690e+36    /* netcdf.h NC_FILL_DOUBLE */
+
+/* validRomsValue (ROMSAdapter.cpp:45-49) over the attribute list plus the
+ * two NetCDF fill values romsMissingValues appends (:40-41). */
+static int roms_valid(double value, const double *missing, int n) {
+    if (!isfinite(value)) return 0;
+    for (int q = 0; q < n; q++)
+        if (value == missing[q]) return 0;
+    if (value == (double)WCO_NC_FILL_FLOAT || value == WCO_NC_FILL_DOUBLE) return 0;
+    return 1;
+}
+
+int wco_roms_normalize(const wco_roms_grid *g, const float *zeta, const float *u, const float *v, const float *w,
+                       const float *akt, float *ZETA, float *U, float *V, float *W, float *AKT) {
+    const int s_rho = g->s_rho, s_w = g->s_rho + 1, eta = g->eta, xi = g->xi;
+    const int eta_u = g->eta_u, xi_u = g->xi_u, eta_v = g->eta_v, xi_v = g->xi_v;
+    const size_t plane = (size_t)eta * xi;
+    const double *mr = g->mask_rho, *mu = g->mask_u, *mv = g->mask_v;
+    /* input checks of :146-166 */
+    for (size_t c = 0; c < plane; c++) {
+        if (mr[c] != 0.0 && mr[c] != 1.0) return -1;
+        if (!g->earth_relative && mr[c] > 0.0 && !roms_valid(g->angle[c], g->missing_angle, g->n_missing_angle))
+            return -2;
+    }
+    for (int j = 0; j < eta_u; j++)
+        for (int i = 0; i < xi_u; i++) {
+            const double m = mu[(size_t)j * xi_u + i];
+            if (m != 0.0 && m != 1.0) return -1;
+            if (m > 0.0)
+                for (int k = 0; k < s_rho; k++)
+                    if (!roms_valid(u[((size_t)k * eta_u + j) * xi_u + i], g->missing_u, g->n_missing_u)) return -3;
+        }
+    for (int j = 0; j < eta_v; j++)
+        for (int i = 0; i < xi_v; i++) {
+            const double m = mv[(size_t)j * xi_v + i];
+            if (m != 0.0 && m != 1.0) return -1;
+            if (m > 0.0)
+                for (int k = 0; k < s_rho; k++)
+                    if (!roms_valid(v[((size_t)k * eta_v + j) * xi_v + i], g->missing_v, g->n_missing_v)) return -4;
+        }
+    /* uv2rho :309-368: float sums over wet faces, float / int average */
+    for (int k = 0; k < s_rho; k++)
+        for (int j = 0; j < eta; j++)
+            for (int i = 0; i < xi; i++) {
+                const size_t o = (size_t)k * plane + (size_t)j * xi + i;
+                float usum = 0.0f, vsum = 0.0f;
+                int ucount = 0, vcount = 0;
+                if (mr[(size_t)j * xi + i] > 0.0) {
+                    if (j < eta_u && i < xi_u && mu[(size_t)j * xi_u + i] > 0.0) {
+                        usum += u[((size_t)k * eta_u + j) * xi_u + i]; ucount++;
+                    }
+                    if (j < eta_u && i > 0 && i - 1 < xi_u && mu[(size_t)j * xi_u + i - 1] > 0.0) {
+                        usum += u[((size_t)k * eta_u + j) * xi_u + i - 1]; ucount++;
+                    }
+                    if (j < eta_v && i < xi_v && mv[(size_t)j * xi_v + i] > 0.0) {
+                        vsum += v[((size_t)k * eta_v + j) * xi_v + i]; vcount++;
+                    }
+                    if (j > 0 && j - 1 < eta_v && i < xi_v && mv[(size_t)(j - 1) * xi_v + i] > 0.0) {
+                        vsum += v[((size_t)k * eta_v + j - 1) * xi_v + i]; vcount++;
+                    }
+                    U[o] = ucount > 0 ? usum / ucount : 0.0f;
+                    V[o] = vcount > 0 ? vsum / vcount : 0.0f;
+                } else {
+                    U[o] = 0.0f;
+                    V[o] = 0.0f;
+                }
+            }
+    /* rotation :276-300: cosine/sine are 1/0 on land, east/north in double */
+    int invalid = 0;
+    for (int k = 0; k < s_rho; k++)
+        for (size_t c = 0; c < plane; c++) {
+            if (mr[c] <= 0.0) continue;
+            const double cs = g->earth_relative ? 1.0 : cos(g->angle[c]);
+            const double sn = g->earth_relative ? 0.0 : sin(g->angle[c]);
+            const size_t o = (size_t)k * plane + c;
+            const double gu = U[o], gv = V[o];
+            const double east = g->earth_relative ? gu : gu * cs - gv * sn;
+            const double north = g->earth_relative ? gv : gu * sn + gv * cs;
+            if (!isfinite(east) || !isfinite(north) || fabs(east) > (double)FLT_MAX || fabs(north) > (double)FLT_MAX)
+                invalid = 1;
+            else {
+                U[o] = (float)east;
+                V[o] = (float)north;
+            }
+        }
+    /* wakt2wakt :448-466 */
+    for (int k = 0; k < s_w; k++)
+        for (size_t c = 0; c < plane; c++) {
+            const size_t o = (size_t)k * plane + c;
+            W[o] = mr[c] > 0.0 ? w[o] : 0.0f;
+            AKT[o] = mr[c] > 0.0 ? akt[o] : 0.0f;
+        }
+    /* zeta :254-264 */
+    for (size_t c = 0; c < plane; c++)
+        ZETA[c] = (mr[c] == 0 || zeta[c] >= 1e+36f) ? 0.0f : zeta[c];
+    return invalid ? -5 : 0;
+}
+
+void wco_roms_static(int s_rho, int eta, int xi, const double *s_w_values, const double *lon, const double *lat,
+                     const double *h, const double *mask_rho, double *lonRad, double *latRad, double *h_out,
+                     double *depthIntervals) {
+    const int s_w = s_rho + 1;
+    /* DepthIntervals(k) = sW(k) - sW(k-1), k = -s_w+2..0 (:232-235); sW
+       storage index k+s_w-1 */
+    for (int k = -s_w + 2; k <= 0; k++)
+        depthIntervals[k + s_w - 2] = s_w_values[k + s_w - 1] - s_w_values[k + s_w - 2];
+    for (size_t c = 0; c < (size_t)eta * xi; c++) {
+        h_out[c] = mask_rho[c] == 0 ? 0.0 : h[c];          /* :244-252 */
+        lonRad[c] = 0.0174533 * lon[c];                     /* :268-275 */
+        latRad[c] = 0.0174533 * lat[c];
+    }
 }

@@ -158,6 +158,38 @@ void wco_grid_count(const wco_particle *ps, long n, int s_rho, int eta, int xi, 
 long wco_emission_times(int mode, double value, double sourceStart, double sourceEnd,
                         double intervalStart, double intervalEnd, double *times, long cap);
 
+/* ROMS normalisation onto the rho grid (ROMSAdapter.cpp:56-300, 309-368,
+ * 448-466).  Grid: mask_rho/angle [eta][xi], mask_u [eta_u][xi_u], mask_v
+ * [eta_v][xi_v].  One record: zeta [eta][xi], u [s_rho][eta_u][xi_u], v
+ * [s_rho][eta_v][xi_v], w/akt [s_rho+1][eta][xi].  u and v are averaged over
+ * wet faces onto wet rho points in float, rotated by the grid angle into
+ * east/north in double (skipped when earth_relative) and rounded to float;
+ * w and akt are zeroed on land; zeta is zeroed on land or where >= 1e36.
+ * missing_*: the variable's _FillValue/missing_value attributes; the NetCDF
+ * float and double fill values are always rejected as well.
+ * Outputs U, V [s_rho][eta][xi], W, AKT [s_rho+1][eta][xi], ZETA [eta][xi].
+ * Returns 0, or where the reference throws: -1 a mask value other than 0 or
+ * 1, -2 a missing or non-finite angle at a wet rho point, -3 / -4 a missing
+ * or non-finite u / v at a wet face, -5 a rotated velocity that is
+ * non-finite or outside float range. */
+typedef struct {
+    int s_rho, eta, xi, eta_u, xi_u, eta_v, xi_v;
+    const double *mask_rho, *mask_u, *mask_v, *angle;
+    int earth_relative;
+    const double *missing_u, *missing_v, *missing_angle;
+    int n_missing_u, n_missing_v, n_missing_angle;
+} wco_roms_grid;
+
+int wco_roms_normalize(const wco_roms_grid *g, const float *zeta, const float *u, const float *v, const float *w,
+                       const float *akt, float *ZETA, float *U, float *V, float *W, float *AKT);
+
+/* Static-grid fields of ROMSAdapter.cpp:232-275: lonRad/latRad =
+ * 0.0174533 * degrees, h zeroed where mask_rho == 0, depthIntervals entry
+ * k+s_w-2 = s_w(k) - s_w(k-1) for the s_w values bottom first. */
+void wco_roms_static(int s_rho, int eta, int xi, const double *s_w_values, const double *lon, const double *lat,
+                     const double *h, const double *mask_rho, double *lonRad, double *latRad, double *h_out,
+                     double *depthIntervals);
+
 /* Haversine cell sizes of Particle.cpp:548-574 for cell (j,i): dx toward
  * (j,i+1), dy toward (j+1,i). */
 void wco_cell_metric(const wco_grid *grid, int j, int i, double *dx, double *dy);

@@ -26,6 +26,7 @@ EXPORTS = (
     "wc_get_particles", "wc_particle_count", "wc_move_interval", "wc_remove_dead", "wc_grid",
     "wc_set_timing", "wc_kernel_time", "wc_reset_kernel_time", "wc_step_count", "wc_reset_step_count",
     "wc_version", "wc_cell_metric_table", "wc_philox4x32_10", "wc_emit", "wc_run_window",
+    "wc_set_roms_grid", "wc_upload_roms_record", "wc_get_record",
 )
 
 
@@ -46,6 +47,19 @@ class WindowDesc(C.Structure):
 class RunStats(C.Structure):
     _fields_ = [("solver_s", C.c_double), ("move_ms", C.c_double), ("particle_steps", C.c_uint64),
                 ("particles", C.c_size_t), ("intervals", C.c_int)]
+
+
+ROMS_MAX_MISSING = 14      # WC_ROMS_MAX_MISSING
+
+
+class RomsGridDesc(C.Structure):
+    """wc_roms_grid: the static grid of a ROMS file (ROMSAdapter.cpp:56-166)."""
+    _fields_ = [(n, C.c_int) for n in ("s_rho", "eta_rho", "xi_rho", "eta_u", "xi_u", "eta_v", "xi_v")] + \
+               [(n, C.c_void_p) for n in ("s_w", "lon_rho", "lat_rho", "h", "mask_rho", "mask_u", "mask_v",
+                                          "angle")] + \
+               [("earth_relative", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("missing_u", "missing_v", "missing_angle")] + \
+               [(n, C.c_int) for n in ("n_missing_u", "n_missing_v", "n_missing_angle")]
 
 
 class WcError(RuntimeError):
@@ -91,6 +105,9 @@ def load_library(path: str = LIB_PATH):
         "wc_reset_step_count": [vp],
         "wc_emit": [vp, C.POINTER(ConfigData), vp, i, d, d, C.POINTER(sz)],
         "wc_run_window": [vp, C.POINTER(ConfigData), C.POINTER(WindowDesc), vp, i, i, vp, i, C.POINTER(RunStats)],
+        "wc_set_roms_grid": [vp, C.POINTER(RomsGridDesc), i, i],
+        "wc_upload_roms_record": [vp, i, vp, vp, vp, vp, vp],
+        "wc_get_record": [vp, i, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -172,6 +189,46 @@ class Context:
         assert arrs[0].shape == (eta, xi) and arrs[1].shape == (s_rho, eta, xi) == arrs[2].shape
         assert arrs[3].shape == (s_rho + 1, eta, xi) == arrs[4].shape
         self._chk(self.L.wc_upload_record(self.h, slot, *[_p(a) for a in arrs]))
+
+    def set_roms_grid(self, s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v, angle=None,
+                      earth_relative=False, missing_u=(), missing_v=(), missing_angle=(), n_slots=2,
+                      store="f32"):
+        """The static part of ROMSAdapter::process (ROMSAdapter.cpp:146-151,
+        228-275): lon/lat in degrees, s_w bottom first, masks 0/1, angle in
+        radians (unused when earth_relative).  Leaves a grid for which
+        upload_roms_record normalises raw ROMS records."""
+        sw, lon, lat, hh, mr, mu, mv = (_f64(a) for a in (s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v))
+        ang = None if angle is None else _f64(angle)
+        miss = [_f64(np.r_[m, 0.0]) for m in (missing_u, missing_v, missing_angle)]
+        eta, xi = mr.shape
+        s_rho = len(sw) - 1
+        desc = RomsGridDesc(s_rho, eta, xi, *mu.shape, *mv.shape, *[_p(a).value for a in (sw, lon, lat, hh, mr,
+                                                                                          mu, mv)],
+                            None if ang is None else _p(ang).value, int(bool(earth_relative)),
+                            *[_p(m).value for m in miss], len(missing_u), len(missing_v), len(missing_angle))
+        self._keep = (sw, lon, lat, hh, mr, mu, mv, ang, miss)
+        self._chk(self.L.wc_set_roms_grid(self.h, C.byref(desc), n_slots, STORE[store]))
+        self.shape = (s_rho, eta, xi)
+        self.roms_shape = (mu.shape, mv.shape)
+
+    def upload_roms_record(self, slot, zeta, u, v, w, akt):
+        """One raw ROMS record (u on the u grid, v on the v grid) normalised
+        into ring slot `slot` on the device."""
+        arrs = [_f32(a) for a in (zeta, u, v, w, akt)]
+        s_rho, eta, xi = self.shape
+        (ush, vsh) = self.roms_shape
+        assert arrs[0].shape == (eta, xi) and arrs[1].shape == (s_rho,) + ush and arrs[2].shape == (s_rho,) + vsh
+        assert arrs[3].shape == (s_rho + 1, eta, xi) == arrs[4].shape
+        self._chk(self.L.wc_upload_roms_record(self.h, slot, *[_p(a) for a in arrs]))
+
+    def get_record(self, slot):
+        """Ring slot `slot` as (zeta, u, v, w, akt) FP32 host arrays."""
+        s_rho, eta, xi = self.shape
+        out = (np.zeros((eta, xi), np.float32), np.zeros((s_rho, eta, xi), np.float32),
+               np.zeros((s_rho, eta, xi), np.float32), np.zeros((s_rho + 1, eta, xi), np.float32),
+               np.zeros((s_rho + 1, eta, xi), np.float32))
+        self._chk(self.L.wc_get_record(self.h, slot, *[_p(a) for a in out]))
+        return out
 
     # -- particles ----------------------------------------------------------
     def set_particles(self, p: np.ndarray):

@@ -19,7 +19,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libwacomm_hip.so")
-SOURCES = ["wc_api.hip", "wc_move_parity.hip", "wc_move_fast.hip", "wc_grid.hip", "wc_driver.hip"]
+SOURCES = ["wc_api.hip", "wc_move_parity.hip", "wc_move_fast.hip", "wc_grid.hip", "wc_driver.hip",
+           "wc_roms.hip"]
 ARCH = os.environ.get("WC_OFFLOAD_ARCH", "gfx950")
 
 

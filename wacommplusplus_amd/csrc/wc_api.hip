@@ -19,6 +19,9 @@
 
 using namespace wc;
 
+// timed kernel groups: 0 move, 1 grid, 2 ROMS normalisation
+constexpr int kTimedKernels = 3;
+
 struct wc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -50,15 +53,26 @@ struct wc_ctx {
     uint32_t *d_counts = nullptr;
     float *d_conc = nullptr;
 
+    // ROMS normalisation (wc_set_roms_grid): per rho point the wet/face bits,
+    // {cos, sin} of the grid angle, staging for one record's raw u and v,
+    // the error flag
+    bool haveRoms = false;
+    RomsDims roms{};
+    RomsMissing romsMissing{};
+    uint8_t *d_face = nullptr;
+    double2 *d_rot = nullptr;
+    float *d_u_raw = nullptr, *d_v_raw = nullptr;
+    uint32_t *d_flag = nullptr;
+
     // timing: an event pair per timed launch, resolved when queried (no
     // synchronisation between launches)
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[kTimedKernels];
     std::vector<hipEvent_t> spare;
     hipEvent_t cur = nullptr;
-    double ms[2] = {0, 0};
-    long launches[2] = {0, 0};
+    double ms[kTimedKernels] = {};
+    long launches[kTimedKernels] = {};
 };
 
 namespace {
@@ -253,7 +267,7 @@ int timing_end(wc_ctx *c, int kernel) {
 
 // Resolves recorded event pairs into the accumulated totals.
 int timing_collect(wc_ctx *c) {
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kTimedKernels; ++k) {
         for (auto &pr : c->pending[k]) {
             HIPCHK(c, hipEventSynchronize(pr.second));
             float ms = 0;
@@ -390,6 +404,13 @@ static void free_grid(wc_ctx *c) {
     c->slots.clear();
     hipFree(c->d_counts);
     hipFree(c->d_conc);
+    hipFree(c->d_face); hipFree(c->d_rot);
+    hipFree(c->d_u_raw); hipFree(c->d_v_raw); hipFree(c->d_flag);
+    c->d_face = nullptr;
+    c->d_rot = nullptr;
+    c->d_u_raw = c->d_v_raw = nullptr;
+    c->d_flag = nullptr;
+    c->haveRoms = false;
     c->d_counts = nullptr;
     c->d_conc = nullptr;
     c->haveGrid = false;
@@ -407,7 +428,7 @@ int wc_ctx_destroy(wc_ctx *c) {
     hipFree(c->d_steps);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < kTimedKernels; ++k)
         for (auto &pr : c->pending[k]) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
     for (auto e : c->spare) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -497,6 +518,143 @@ int wc_upload_record(wc_ctx *c, int slot, const float *zeta, const float *u, con
     HIPCHK(c, hipStreamSynchronize(c->stream));
     s.filled = true;
     s.record = -1;
+    return WC_OK;
+}
+
+// validRomsValue (ROMSAdapter.cpp:45-49) over the attribute list plus the two
+// NetCDF fill values romsMissingValues appends (:40-41)
+static bool roms_valid(double value, const double *extra, int n_extra) {
+    if (!std::isfinite(value)) return false;
+    for (int q = 0; q < n_extra; ++q)
+        if (value == extra[q]) return false;
+    return value != (double)kNcFillFloat && value != kNcFillDouble;
+}
+
+int wc_set_roms_grid(wc_ctx *c, const wc_roms_grid *g, int n_slots, int store) {
+    if (!c || !g) return WC_ERR_INVALID;
+    const int s_rho = g->s_rho, eta = g->eta_rho, xi = g->xi_rho;
+    if (s_rho < 1 || eta < 2 || xi < 2 || g->eta_u < 0 || g->xi_u < 0 || g->eta_v < 0 || g->xi_v < 0)
+        return fail(c, WC_ERR_INVALID, "invalid ROMS dimensions");
+    if (g->eta_u > eta || g->xi_u > xi || g->eta_v > eta || g->xi_v > xi)
+        return fail(c, WC_ERR_INVALID, "u/v grids (%dx%d, %dx%d) larger than the rho grid (%dx%d)", g->eta_u,
+                    g->xi_u, g->eta_v, g->xi_v, eta, xi);
+    if (!g->s_w || !g->lon_rho || !g->lat_rho || !g->h || !g->mask_rho || (g->eta_u * g->xi_u && !g->mask_u) ||
+        (g->eta_v * g->xi_v && !g->mask_v) || (!g->earth_relative && !g->angle))
+        return fail(c, WC_ERR_INVALID, "null ROMS grid field");
+    if (g->n_missing_u < 0 || g->n_missing_v < 0 || g->n_missing_angle < 0 ||
+        g->n_missing_u > WC_ROMS_MAX_MISSING || g->n_missing_v > WC_ROMS_MAX_MISSING ||
+        g->n_missing_angle > WC_ROMS_MAX_MISSING ||
+        (g->n_missing_u && !g->missing_u) || (g->n_missing_v && !g->missing_v) ||
+        (g->n_missing_angle && !g->missing_angle))
+        return fail(c, WC_ERR_INVALID, "at most %d missing values per variable", WC_ROMS_MAX_MISSING);
+    const size_t plane = (size_t)eta * xi, pu = (size_t)g->eta_u * g->xi_u, pv = (size_t)g->eta_v * g->xi_v;
+    // input checks of ROMSAdapter.cpp:146-166 that do not depend on a record
+    for (size_t q = 0; q < plane; ++q) {
+        const double m = g->mask_rho[q];
+        if (m != 0.0 && m != 1.0) return fail(c, WC_ERR_INVALID, "ROMS rho mask requires finite zero/one values");
+        if (!g->earth_relative && m > 0.0 &&
+            !roms_valid(g->angle[q], g->missing_angle, g->n_missing_angle))
+            return fail(c, WC_ERR_INVALID, "ROMS angle is missing or non-finite at a wet rho point (%zu, %zu)",
+                        q / xi, q % xi);
+    }
+    for (size_t q = 0; q < pu; ++q)
+        if (g->mask_u[q] != 0.0 && g->mask_u[q] != 1.0)
+            return fail(c, WC_ERR_INVALID, "ROMS u mask requires finite zero/one values");
+    for (size_t q = 0; q < pv; ++q)
+        if (g->mask_v[q] != 0.0 && g->mask_v[q] != 1.0)
+            return fail(c, WC_ERR_INVALID, "ROMS v mask requires finite zero/one values");
+    // static fields of ROMSAdapter.cpp:232-275
+    const int s_w = s_rho + 1;
+    std::vector<double> lonRad(plane), latRad(plane), h(plane), depth(s_w - 1);
+    for (int k = -s_w + 2; k <= 0; ++k) depth[k + s_w - 2] = g->s_w[k + s_w - 1] - g->s_w[k + s_w - 2];
+    for (size_t q = 0; q < plane; ++q) {
+        h[q] = g->mask_rho[q] == 0 ? 0.0 : g->h[q];
+        lonRad[q] = 0.0174533 * g->lon_rho[q];
+        latRad[q] = 0.0174533 * g->lat_rho[q];
+    }
+    int rc = wc_set_grid(c, s_w, s_rho, eta, xi, lonRad.data(), latRad.data(), h.data(), g->mask_rho, depth.data(),
+                         n_slots, store);
+    if (rc) return rc;
+    // device tables: wet/face bits per rho point, {cos, sin} of the angle (1, 0 on land)
+    const int eta_u = g->eta_u, xi_u = g->xi_u, eta_v = g->eta_v, xi_v = g->xi_v;
+    std::vector<uint8_t> face(plane);
+    std::vector<double2> rot(plane);
+    for (int j = 0; j < eta; ++j)
+        for (int i = 0; i < xi; ++i) {
+            const size_t q = (size_t)j * xi + i;
+            const bool wet = g->mask_rho[q] > 0.0;
+            uint32_t f = wet ? kFaceWet : 0;
+            if (j < eta_u && i < xi_u && g->mask_u[(size_t)j * xi_u + i] > 0.0) f |= kFaceU;
+            if (j < eta_u && i > 0 && i - 1 < xi_u && g->mask_u[(size_t)j * xi_u + i - 1] > 0.0) f |= kFaceUWest;
+            if (j < eta_v && i < xi_v && g->mask_v[(size_t)j * xi_v + i] > 0.0) f |= kFaceV;
+            if (j > 0 && j - 1 < eta_v && i < xi_v && g->mask_v[(size_t)(j - 1) * xi_v + i] > 0.0) f |= kFaceVSouth;
+            face[q] = (uint8_t)f;
+            const bool turn = wet && !g->earth_relative;
+            rot[q] = make_double2(turn ? std::cos(g->angle[q]) : 1.0, turn ? std::sin(g->angle[q]) : 0.0);
+        }
+    HIPCHK(c, hipMalloc(&c->d_face, plane));
+    HIPCHK(c, hipMalloc(&c->d_rot, plane * sizeof(double2)));
+    HIPCHK(c, hipMalloc(&c->d_u_raw, (pu ? pu : 1) * s_rho * sizeof(float)));
+    HIPCHK(c, hipMalloc(&c->d_v_raw, (pv ? pv : 1) * s_rho * sizeof(float)));
+    HIPCHK(c, hipMalloc(&c->d_flag, sizeof(uint32_t)));
+    HIPCHK(c, hipMemcpy(c->d_face, face.data(), plane, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_rot, rot.data(), plane * sizeof(double2), hipMemcpyHostToDevice));
+    c->roms = RomsDims{s_rho, eta, xi, g->eta_u, g->xi_u, g->eta_v, g->xi_v, g->earth_relative ? 1 : 0};
+    RomsMissing &m = c->romsMissing;
+    m = RomsMissing{};
+    for (int q = 0; q < g->n_missing_u; ++q) m.u[m.nu++] = g->missing_u[q];
+    for (int q = 0; q < g->n_missing_v; ++q) m.v[m.nv++] = g->missing_v[q];
+    m.u[m.nu++] = (double)kNcFillFloat; m.u[m.nu++] = kNcFillDouble;
+    m.v[m.nv++] = (double)kNcFillFloat; m.v[m.nv++] = kNcFillDouble;
+    c->haveRoms = true;
+    return WC_OK;
+}
+
+int wc_upload_roms_record(wc_ctx *c, int slot, const float *zeta, const float *u, const float *v, const float *w,
+                          const float *akt) {
+    if (!c) return WC_ERR_INVALID;
+    if (!c->haveRoms) return fail(c, WC_ERR_INVALID, "wc_set_roms_grid first");
+    if (slot < 0 || slot >= (int)c->slots.size()) return fail(c, WC_ERR_INVALID, "slot %d out of range", slot);
+    const RomsDims &d = c->roms;
+    const size_t pu = (size_t)d.eta_u * d.xi_u * d.s_rho, pv = (size_t)d.eta_v * d.xi_v * d.s_rho;
+    if (!zeta || !w || !akt || (pu && !u) || (pv && !v)) return fail(c, WC_ERR_INVALID, "null ROMS record field");
+    HIPCHK(c, hipSetDevice(c->device));
+    auto &s = c->slots[slot];
+    s.filled = false;
+    s.record = -1;
+    HIPCHK(c, hipMemcpyAsync(s.f[0], zeta, field_count(c, 0) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.f[3], w, field_count(c, 3) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.f[4], akt, field_count(c, 4) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (pu) HIPCHK(c, hipMemcpyAsync(c->d_u_raw, u, pu * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (pv) HIPCHK(c, hipMemcpyAsync(c->d_v_raw, v, pv * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_fill_u32(c->d_flag, 1, 0u, c->stream));
+    timing_begin(c);
+    HIPCHK(c, launch_roms_normalize(d, c->romsMissing, c->d_face, c->d_rot, c->d_u_raw, c->d_v_raw, s.f[1],
+                                    s.f[2], s.f[3], s.f[4], s.f[0], c->d_flag, c->stream));
+    int trc = timing_end(c, 2);
+    if (trc) return trc;
+    uint32_t flag = 0;
+    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (flag & 1u) return fail(c, WC_ERR_INVALID, "ROMS u is missing or non-finite at a wet face");
+    if (flag & 2u) return fail(c, WC_ERR_INVALID, "ROMS v is missing or non-finite at a wet face");
+    if (flag & 4u) return fail(c, WC_ERR_INVALID, "ROMS normalized velocity is outside finite float range");
+    s.filled = true;
+    return WC_OK;
+}
+
+int wc_get_record(wc_ctx *c, int slot, float *zeta, float *u, float *v, float *w, float *akt) {
+    if (!c) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    if (slot < 0 || slot >= (int)c->slots.size() || !c->slots[slot].filled)
+        return fail(c, WC_ERR_INVALID, "slot %d out of range or not uploaded", slot);
+    HIPCHK(c, hipSetDevice(c->device));
+    float *dst[5] = {zeta, u, v, w, akt};
+    for (int f = 0; f < 5; ++f)
+        if (dst[f])
+            HIPCHK(c, hipMemcpyAsync(dst[f], c->slots[slot].f[f], field_count(c, f) * sizeof(float),
+                                     hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return WC_OK;
 }
 
@@ -742,7 +900,7 @@ int wc_set_timing(wc_ctx *c, int enabled) {
 }
 
 int wc_kernel_time(wc_ctx *c, int kernel, double *total_ms, long *launches) {
-    if (!c || kernel < 0 || kernel > 1) return WC_ERR_INVALID;
+    if (!c || kernel < 0 || kernel >= kTimedKernels) return WC_ERR_INVALID;
     int rc = timing_collect(c);
     if (rc) return rc;
     if (total_ms) *total_ms = c->ms[kernel];
@@ -754,8 +912,10 @@ int wc_reset_kernel_time(wc_ctx *c) {
     if (!c) return WC_ERR_INVALID;
     int rc = timing_collect(c);
     if (rc) return rc;
-    c->ms[0] = c->ms[1] = 0;
-    c->launches[0] = c->launches[1] = 0;
+    for (int k = 0; k < kTimedKernels; ++k) {
+        c->ms[k] = 0;
+        c->launches[k] = 0;
+    }
     return WC_OK;
 }
 

@@ -39,4 +39,27 @@ hipError_t launch_emit(const SoA &s, size_t at, int n, const double *times, cons
 bool emission_times(int mode, double value, double sourceStart, double sourceEnd, double intervalStart,
                     double intervalEnd, std::vector<double> &out);
 
+// ROMS record normalisation (wc_roms.hip).  missing: the variable's
+// _FillValue / missing_value attributes plus the NetCDF float and double fill
+// values.  flag bits: 1 bad u at a wet face, 2 bad v at a wet face, 4 a
+// rotated velocity outside float range.
+constexpr int kRomsMissing = 16;
+constexpr float kNcFillFloat = 9.9692099683868690e+36f;     // netcdf.h NC_FILL_FLOAT
+constexpr double kNcFillDouble = 9.9692099683868690e+36;    // netcdf.h NC_FILL_DOUBLE
+struct RomsDims {
+    int s_rho, eta, xi, eta_u, xi_u, eta_v, xi_v;
+    int earth;
+};
+struct RomsMissing {
+    double u[kRomsMissing], v[kRomsMissing];
+    int nu, nv;
+};
+// face: per rho point, kFaceWet | kFaceU (u face (j,i) wet) | kFaceUWest (u
+// face (j,i-1) wet) | kFaceV (v face (j,i) wet) | kFaceVSouth (v face (j-1,i)
+// wet), faces outside their grid counted dry.
+constexpr uint32_t kFaceWet = 1, kFaceU = 2, kFaceUWest = 4, kFaceV = 8, kFaceVSouth = 16;
+hipError_t launch_roms_normalize(const RomsDims &d, const RomsMissing &miss, const uint8_t *face,
+                                 const double2 *rot, const float *u, const float *v, float *U, float *V, float *W,
+                                 float *AKT, float *zeta, uint32_t *flag, hipStream_t stream);
+
 }  // namespace wc
