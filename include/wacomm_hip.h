@@ -209,6 +209,8 @@ typedef struct wc_roms_grid {
     int earth_relative;
     const double *missing_u, *missing_v, *missing_angle;
     int n_missing_u, n_missing_v, n_missing_angle;
+    const double *s_rho_values;   /* optional (NULL): s_rho sigma values, bottom first, for
+                                     wc_check_roms_grid */
 } wc_roms_grid;
 
 /* Replaces the static part of ROMSAdapter::process (ROMSAdapter.cpp:
@@ -232,6 +234,22 @@ int wc_set_roms_grid(wc_ctx *ctx, const wc_roms_grid *grid, int n_slots, int sto
  * velocity outside float range, where the reference throws. */
 int wc_upload_roms_record(wc_ctx *ctx, int slot, const float *zeta, const float *u, const float *v,
                           const float *w, const float *akt);
+
+/* The grid checks of OceanModelAdapter::appendBoundaryRecord
+ * (OceanModelAdapter.cpp:199-214) for the next file of a run, against the
+ * grid of the last wc_set_roms_grid: same dimensions; s_rho values (when both
+ * were given) and s_w values within 1e-12; lon_rho, lat_rho within 1e-10;
+ * h (zeroed on land) within 1e-8; identical mask_rho.  Returns WC_OK, or
+ * WC_ERR_INVALID with the reference's message. */
+int wc_check_roms_grid(wc_ctx *ctx, const wc_roms_grid *next);
+
+/* The duplicate-boundary check of appendBoundaryRecord
+ * (OceanModelAdapter.cpp:216-233) on the device: compares slots a and b
+ * under float != (a NaN always conflicts).  *conflict = 0 when they agree,
+ * else the class the reference throws for, at the first conflicting point in
+ * its (j, i) loop order: 1 surface elevation, 2 horizontal velocity,
+ * 3 vertical fields (w, AKt).  *point (optional) = that point j*xi+i, or -1. */
+int wc_compare_records(wc_ctx *ctx, int slot_a, int slot_b, int *conflict, long long *point);
 
 /* Particle state: host AoS in and out (Particles.hpp:17 holds particle_data).
  * wc_set_particles replaces the device population; wc_append_particles adds
@@ -281,7 +299,9 @@ int wc_emit(wc_ctx *ctx, const wc_config_data *config, const wc_source *sources,
 
 /* A forcing window on the host (oceanmodel_data, OceanModelAdapter.hpp:20-34):
  * T records, zeta [T][eta][xi], u/v [T][s_rho][eta][xi], w/akt [T][s_w][eta][xi],
- * oceanTime [T].  The grid must match the context's wc_set_grid. */
+ * oceanTime [T] strictly increasing (else WC_ERR_INVALID, as
+ * ROMSAdapter.cpp:197-201 throws).  The grid must match the context's
+ * wc_set_grid. */
 typedef struct wc_window {
     int T;
     const double *oceanTime;

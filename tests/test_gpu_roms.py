@@ -198,3 +198,115 @@ def test_move_over_roms_window(ctx, random):
         ok &= d <= (1e-9 if random else 0.0)
     print(f"diffusion={'on' if random else 'off'}: {ok.mean():.5f} of particles match", flush=True)
     assert ok.mean() >= (0.995 if random else 1.0)
+
+
+# --------------------------------------------------------------------------
+# appendBoundaryRecord checks (OceanModelAdapter.cpp:199-233)
+# --------------------------------------------------------------------------
+
+def first_conflict(a, b):
+    """Brute force of the reference's loop order (OceanModelAdapter.cpp:
+    218-231): points in (j, i) order; per point zeta, then u/v over the
+    levels, then w/AKt; the first conflict under float !=."""
+    za, ua, va, wa, ka = a
+    zb, ub, vb, wb, kb = b
+    cls = [za != zb, np.any((ua != ub) | (va != vb), axis=0), np.any((wa != wb) | (ka != kb), axis=0)]
+    for p in range(za.size):
+        for q in range(3):
+            if cls[q].flat[p]:
+                return q + 1, p
+    return 0, -1
+
+
+def test_compare_records_duplicate_boundary(ctx):
+    c = random_roms(seed=31, T=2, s_rho=4, eta=24, xi=32)
+    load(ctx, c, n_slots=3)
+    upload(ctx, c, 0, slot=0)
+    upload(ctx, c, 0, slot=1)
+    assert ctx.compare_records(0, 1) == (0, -1)
+    base = ctx.get_record(0)
+    m, mu, mv = c["mask_rho"], c["mask_u"], c["mask_v"]
+    wet = np.argwhere(m > 0)
+    # a wet point whose u face (j, i) and v face (j, i) are wet too
+    cand = [(int(a), int(b)) for a, b in wet if b < mu.shape[1] and a < mv.shape[0] and mu[a, b] and mv[a, b]]
+    j, i = cand[len(cand) // 2]
+    # tests/NativeAdapterTest.cpp:77-90: each field + 1 at one point is a conflict of its class
+    for name, cls in (("zeta", 1), ("u", 2), ("v", 2), ("w", 3), ("akt", 3)):
+        d = {q: c[q].copy() for q in ("zeta", "u", "v", "w", "akt")}
+        if name == "zeta":
+            d["zeta"][0, j, i] += 1.0
+        elif name == "u":
+            d["u"][0, 1, j, i] += 1.0
+        elif name == "v":
+            d["v"][0, 2, j, i] += 1.0
+        else:
+            d[name][0, 3, j, i] += 1.0
+        ctx.upload_roms_record(2, d["zeta"][0], d["u"][0], d["v"][0], d["w"][0], d["akt"][0])
+        got = ctx.compare_records(0, 2)
+        assert got == first_conflict(base, ctx.get_record(2)), (name, got)
+        assert got[0] == cls, (name, got)
+    # a later-class conflict at an earlier point wins over an earlier class at a later point
+    d = {q: c[q].copy() for q in ("zeta", "u", "v", "w", "akt")}
+    d["zeta"][0, -1, -1] = 5.0
+    d["w"][0, 0, wet[0][0], wet[0][1]] += 1.0
+    ctx.upload_roms_record(2, d["zeta"][0], d["u"][0], d["v"][0], d["w"][0], d["akt"][0])
+    got = ctx.compare_records(0, 2)
+    assert got == first_conflict(base, ctx.get_record(2)) and got[0] == 3
+    # different records: the first point in loop order
+    upload(ctx, c, 1, slot=2)
+    assert ctx.compare_records(0, 2) == first_conflict(base, ctx.get_record(2))
+    # NaN never equals itself
+    d = {q: c[q].copy() for q in ("zeta", "u", "v", "w", "akt")}
+    d["akt"][0, :, j, i] = np.nan
+    ctx.upload_roms_record(1, d["zeta"][0], d["u"][0], d["v"][0], d["w"][0], d["akt"][0])
+    ctx.upload_roms_record(2, d["zeta"][0], d["u"][0], d["v"][0], d["w"][0], d["akt"][0])
+    assert ctx.compare_records(1, 2) == (3, j * c["mask_rho"].shape[1] + i)
+
+
+def test_check_roms_grid(ctx):
+    c = random_roms(seed=33, T=1, s_rho=4, eta=24, xi=32)
+    s_rho_values = np.linspace(-0.875, -0.125, 4)
+    ctx.set_roms_grid(c["s_w"], c["lon"], c["lat"], c["h"], c["mask_rho"], c["mask_u"], c["mask_v"],
+                      angle=c["angle"], s_rho_values=s_rho_values)
+
+    def check(**over):
+        a = dict(s_w=c["s_w"], lon_rho=c["lon"], lat_rho=c["lat"], h=c["h"], mask_rho=c["mask_rho"],
+                 mask_u=c["mask_u"], mask_v=c["mask_v"], s_rho_values=s_rho_values)
+        a.update(over)
+        ctx.check_roms_grid(**a)
+
+    check()
+    check(s_rho_values=None)
+    land = np.argwhere(c["mask_rho"] == 0)[0]
+    wet = np.argwhere(c["mask_rho"] > 0)[0]
+    ok = [dict(s_w=c["s_w"] + 1e-13), dict(s_rho_values=s_rho_values + 1e-13), dict(lon_rho=c["lon"] + 1e-11),
+          dict(lat_rho=c["lat"] - 1e-11), dict(h=c["h"] + 1e-9)]
+    h_land = c["h"].copy()
+    h_land[tuple(land)] += 5.0                       # land h is zeroed on both sides
+    ok.append(dict(h=h_land))
+    for o in ok:
+        check(**o)
+    h_wet = c["h"].copy()
+    h_wet[tuple(wet)] += 1e-7
+    mask = c["mask_rho"].copy()
+    mask[tuple(land)] = 1.0
+    bad = [(dict(s_w=c["s_w"] + 1e-11), "vertical coordinates"),
+           (dict(s_rho_values=s_rho_values - 1e-11), "vertical coordinates"),
+           (dict(lon_rho=c["lon"] + 1e-9), "horizontal grids"), (dict(lat_rho=c["lat"] + 1e-9), "horizontal grids"),
+           (dict(h=h_wet), "horizontal grids"), (dict(mask_rho=mask), "horizontal grids"),
+           (dict(lon_rho=c["lon"][:-1], lat_rho=c["lat"][:-1], h=c["h"][:-1], mask_rho=c["mask_rho"][:-1],
+                 mask_v=c["mask_v"][:-1]), "grid dimensions")]
+    for o, text in bad:
+        expect_rejected(lambda: check(**o), text)
+
+
+def test_run_window_rejects_non_chronological_time(ctx):
+    # ROMSAdapter.cpp:197-201 / OceanModelAdapter.cpp:264-265
+    c = random_roms(seed=35, T=3, s_rho=2, eta=8, xi=10)
+    g = oracle_grid(po, c)
+    W = roms_window(c, g)
+    ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask)
+    zeta, u, v, w, akt = W.stacked()
+    cfg = wc.default_config(random=False)
+    for times in ([0.0, 3600.0, 3600.0], [0.0, 7200.0, 3600.0], [0.0, np.nan, 7200.0]):
+        expect_rejected(lambda: ctx.run_window(cfg, np.array(times), zeta, u, v, w, akt), "chronological")

@@ -26,7 +26,7 @@ EXPORTS = (
     "wc_get_particles", "wc_particle_count", "wc_move_interval", "wc_remove_dead", "wc_grid",
     "wc_set_timing", "wc_kernel_time", "wc_reset_kernel_time", "wc_step_count", "wc_reset_step_count",
     "wc_version", "wc_cell_metric_table", "wc_philox4x32_10", "wc_emit", "wc_run_window",
-    "wc_set_roms_grid", "wc_upload_roms_record", "wc_get_record",
+    "wc_set_roms_grid", "wc_upload_roms_record", "wc_get_record", "wc_check_roms_grid", "wc_compare_records",
 )
 
 
@@ -59,7 +59,8 @@ class RomsGridDesc(C.Structure):
                                           "angle")] + \
                [("earth_relative", C.c_int)] + \
                [(n, C.c_void_p) for n in ("missing_u", "missing_v", "missing_angle")] + \
-               [(n, C.c_int) for n in ("n_missing_u", "n_missing_v", "n_missing_angle")]
+               [(n, C.c_int) for n in ("n_missing_u", "n_missing_v", "n_missing_angle")] + \
+               [("s_rho_values", C.c_void_p)]
 
 
 class WcError(RuntimeError):
@@ -108,6 +109,8 @@ def load_library(path: str = LIB_PATH):
         "wc_set_roms_grid": [vp, C.POINTER(RomsGridDesc), i, i],
         "wc_upload_roms_record": [vp, i, vp, vp, vp, vp, vp],
         "wc_get_record": [vp, i, vp, vp, vp, vp, vp],
+        "wc_check_roms_grid": [vp, C.POINTER(RomsGridDesc)],
+        "wc_compare_records": [vp, i, i, C.POINTER(i), C.POINTER(C.c_longlong)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -190,26 +193,49 @@ class Context:
         assert arrs[3].shape == (s_rho + 1, eta, xi) == arrs[4].shape
         self._chk(self.L.wc_upload_record(self.h, slot, *[_p(a) for a in arrs]))
 
+    @staticmethod
+    def _roms_desc(s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v, angle, earth_relative, missing_u,
+                   missing_v, missing_angle, s_rho_values):
+        sw, lon, lat, hh, mr, mu, mv = (_f64(a) for a in (s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v))
+        ang = None if angle is None else _f64(angle)
+        sr = None if s_rho_values is None else _f64(s_rho_values)
+        miss = [_f64(np.r_[m, 0.0]) for m in (missing_u, missing_v, missing_angle)]
+        eta, xi = mr.shape
+        desc = RomsGridDesc(len(sw) - 1, eta, xi, *mu.shape, *mv.shape,
+                            *[_p(a).value for a in (sw, lon, lat, hh, mr, mu, mv)],
+                            None if ang is None else _p(ang).value, int(bool(earth_relative)),
+                            *[_p(m).value for m in miss], len(missing_u), len(missing_v), len(missing_angle),
+                            None if sr is None else _p(sr).value)
+        return desc, (sw, lon, lat, hh, mr, mu, mv, ang, miss, sr)
+
     def set_roms_grid(self, s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v, angle=None,
-                      earth_relative=False, missing_u=(), missing_v=(), missing_angle=(), n_slots=2,
-                      store="f32"):
+                      earth_relative=False, missing_u=(), missing_v=(), missing_angle=(), s_rho_values=None,
+                      n_slots=2, store="f32"):
         """The static part of ROMSAdapter::process (ROMSAdapter.cpp:146-151,
         228-275): lon/lat in degrees, s_w bottom first, masks 0/1, angle in
         radians (unused when earth_relative).  Leaves a grid for which
         upload_roms_record normalises raw ROMS records."""
-        sw, lon, lat, hh, mr, mu, mv = (_f64(a) for a in (s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v))
-        ang = None if angle is None else _f64(angle)
-        miss = [_f64(np.r_[m, 0.0]) for m in (missing_u, missing_v, missing_angle)]
-        eta, xi = mr.shape
-        s_rho = len(sw) - 1
-        desc = RomsGridDesc(s_rho, eta, xi, *mu.shape, *mv.shape, *[_p(a).value for a in (sw, lon, lat, hh, mr,
-                                                                                          mu, mv)],
-                            None if ang is None else _p(ang).value, int(bool(earth_relative)),
-                            *[_p(m).value for m in miss], len(missing_u), len(missing_v), len(missing_angle))
-        self._keep = (sw, lon, lat, hh, mr, mu, mv, ang, miss)
+        desc, keep = self._roms_desc(s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v, angle, earth_relative,
+                                     missing_u, missing_v, missing_angle, s_rho_values)
+        self._keep = keep
         self._chk(self.L.wc_set_roms_grid(self.h, C.byref(desc), n_slots, STORE[store]))
-        self.shape = (s_rho, eta, xi)
+        mr, mu, mv = keep[4], keep[5], keep[6]
+        self.shape = (len(keep[0]) - 1,) + mr.shape
         self.roms_shape = (mu.shape, mv.shape)
+
+    def check_roms_grid(self, s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v, s_rho_values=None):
+        """appendBoundaryRecord's grid checks (OceanModelAdapter.cpp:199-214)
+        of the next file against the loaded ROMS grid; raises WcError."""
+        desc, keep = self._roms_desc(s_w, lon_rho, lat_rho, h, mask_rho, mask_u, mask_v, None, True, (), (), (),
+                                     s_rho_values)
+        self._chk(self.L.wc_check_roms_grid(self.h, C.byref(desc)))
+
+    def compare_records(self, slot_a, slot_b):
+        """Duplicate-boundary check (OceanModelAdapter.cpp:216-233):
+        (conflict, point) with conflict 0 none, 1 zeta, 2 u/v, 3 w/AKt."""
+        conflict, point = C.c_int(), C.c_longlong()
+        self._chk(self.L.wc_compare_records(self.h, slot_a, slot_b, C.byref(conflict), C.byref(point)))
+        return conflict.value, point.value
 
     def upload_roms_record(self, slot, zeta, u, v, w, akt):
         """One raw ROMS record (u on the u grid, v on the v grid) normalised

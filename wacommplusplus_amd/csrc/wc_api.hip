@@ -63,6 +63,10 @@ struct wc_ctx {
     double2 *d_rot = nullptr;
     float *d_u_raw = nullptr, *d_v_raw = nullptr;
     uint32_t *d_flag = nullptr;
+    // host copy of the ROMS static grid, for the adjacent-file check
+    std::vector<double> romsLon, romsLat, romsH, romsMask, romsSw, romsSrho;
+    // duplicate-record comparison key
+    unsigned long long *d_key = nullptr;
 
     // timing: an event pair per timed launch, resolved when queried (no
     // synchronisation between launches)
@@ -410,6 +414,8 @@ static void free_grid(wc_ctx *c) {
     c->d_rot = nullptr;
     c->d_u_raw = c->d_v_raw = nullptr;
     c->d_flag = nullptr;
+    hipFree(c->d_key);
+    c->d_key = nullptr;
     c->haveRoms = false;
     c->d_counts = nullptr;
     c->d_conc = nullptr;
@@ -600,6 +606,15 @@ int wc_set_roms_grid(wc_ctx *c, const wc_roms_grid *g, int n_slots, int store) {
     HIPCHK(c, hipMemcpy(c->d_face, face.data(), plane, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_rot, rot.data(), plane * sizeof(double2), hipMemcpyHostToDevice));
     c->roms = RomsDims{s_rho, eta, xi, g->eta_u, g->xi_u, g->eta_v, g->xi_v, g->earth_relative ? 1 : 0};
+    c->romsLon.assign(g->lon_rho, g->lon_rho + plane);
+    c->romsLat.assign(g->lat_rho, g->lat_rho + plane);
+    c->romsH = h;
+    c->romsMask.assign(g->mask_rho, g->mask_rho + plane);
+    c->romsSw.assign(g->s_w, g->s_w + s_w);
+    if (g->s_rho_values)
+        c->romsSrho.assign(g->s_rho_values, g->s_rho_values + s_rho);
+    else
+        c->romsSrho.clear();
     RomsMissing &m = c->romsMissing;
     m = RomsMissing{};
     for (int q = 0; q < g->n_missing_u; ++q) m.u[m.nu++] = g->missing_u[q];
@@ -640,6 +655,53 @@ int wc_upload_roms_record(wc_ctx *c, int slot, const float *zeta, const float *u
     if (flag & 2u) return fail(c, WC_ERR_INVALID, "ROMS v is missing or non-finite at a wet face");
     if (flag & 4u) return fail(c, WC_ERR_INVALID, "ROMS normalized velocity is outside finite float range");
     s.filled = true;
+    return WC_OK;
+}
+
+int wc_check_roms_grid(wc_ctx *c, const wc_roms_grid *g) {
+    if (!c || !g) return WC_ERR_INVALID;
+    if (!c->haveRoms) return fail(c, WC_ERR_INVALID, "wc_set_roms_grid first");
+    // OceanModelAdapter.cpp:199-214, in the reference's order and with its tolerances
+    const int s_rho = c->roms.s_rho, eta = c->roms.eta, xi = c->roms.xi;
+    if (g->s_rho != s_rho || g->eta_rho != eta || g->xi_rho != xi)
+        return fail(c, WC_ERR_INVALID, "Adjacent forcing files have incompatible grid dimensions");
+    if (!g->s_w || !g->lon_rho || !g->lat_rho || !g->h || !g->mask_rho)
+        return fail(c, WC_ERR_INVALID, "null ROMS grid field");
+    if (g->s_rho_values && !c->romsSrho.empty())
+        for (int k = 0; k < s_rho; ++k)
+            if (std::fabs(c->romsSrho[k] - g->s_rho_values[k]) > 1e-12)
+                return fail(c, WC_ERR_INVALID, "Adjacent forcing files have incompatible vertical coordinates");
+    for (int k = 0; k <= s_rho; ++k)
+        if (std::fabs(c->romsSw[k] - g->s_w[k]) > 1e-12)
+            return fail(c, WC_ERR_INVALID, "Adjacent forcing files have incompatible vertical coordinates");
+    const size_t plane = (size_t)eta * xi;
+    for (size_t q = 0; q < plane; ++q) {
+        const double h = g->mask_rho[q] == 0 ? 0.0 : g->h[q];      // both sides as loaded (:244-252)
+        if (std::fabs(c->romsLon[q] - g->lon_rho[q]) > 1e-10 || std::fabs(c->romsLat[q] - g->lat_rho[q]) > 1e-10 ||
+            std::fabs(c->romsH[q] - h) > 1e-8 || c->romsMask[q] != g->mask_rho[q])
+            return fail(c, WC_ERR_INVALID, "Adjacent forcing files have incompatible horizontal grids");
+    }
+    return WC_OK;
+}
+
+int wc_compare_records(wc_ctx *c, int slot_a, int slot_b, int *conflict, long long *point) {
+    if (!c || !conflict) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    const int ns = (int)c->slots.size();
+    if (slot_a < 0 || slot_a >= ns || slot_b < 0 || slot_b >= ns || !c->slots[slot_a].filled ||
+        !c->slots[slot_b].filled)
+        return fail(c, WC_ERR_INVALID, "forcing slots %d/%d not uploaded", slot_a, slot_b);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->d_key) HIPCHK(c, hipMalloc(&c->d_key, sizeof(unsigned long long)));
+    const auto &a = c->slots[slot_a], &b = c->slots[slot_b];
+    RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
+    HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t *>(c->d_key), 2, 0xFFFFFFFFu, c->stream));
+    HIPCHK(c, launch_compare_records(F, c->s_rho, (size_t)c->eta * c->xi, c->d_key, c->stream));
+    unsigned long long key = 0;
+    HIPCHK(c, hipMemcpyAsync(&key, c->d_key, sizeof key, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *conflict = key == ~0ull ? 0 : (int)(key % 3) + 1;
+    if (point) *point = key == ~0ull ? -1 : (long long)(key / 3);
     return WC_OK;
 }
 
@@ -835,6 +897,9 @@ int wc_run_window(wc_ctx *c, const wc_config_data *cfg, const wc_window *win, co
     int rc = check_supported(c, cfg);
     if (rc) return rc;
     const int T = win->T, dir = cfg->trackingDirection;
+    for (int t = 1; t < T; ++t)                               // ROMSAdapter.cpp:197-201
+        if (!(win->oceanTime[t] > win->oceanTime[t - 1]))
+            return fail(c, WC_ERR_INVALID, "ocean time must be strictly chronological (record %d)", t);
     const size_t plane = (size_t)c->eta * c->xi, frame = plane * c->s_rho;
     const size_t nf[5] = {plane, plane * c->s_rho, plane * c->s_rho, plane * c->s_w, plane * c->s_w};
     const float *fields[5] = {win->zeta, win->u, win->v, win->w, win->akt};

@@ -62,4 +62,9 @@ hipError_t launch_roms_normalize(const RomsDims &d, const RomsMissing &miss, con
                                  const double2 *rot, const float *u, const float *v, float *U, float *V, float *W,
                                  float *AKT, float *zeta, uint32_t *flag, hipStream_t stream);
 
+// Duplicate-boundary comparison of two records (wc_roms.hip): *key (set to
+// all ones beforehand) receives min(point*3 + class) over the conflicts.
+hipError_t launch_compare_records(const RecordPair<float> &F, int s_rho, size_t plane, unsigned long long *key,
+                                  hipStream_t stream);
+
 }  // namespace wc

@@ -128,7 +128,48 @@ __global__ __launch_bounds__(kBlock) void roms_normalize_kernel(RomsDims d, Roms
     if (k == 0 && (!wet || z >= 1e+36f)) zeta[c] = 0.0f;
 }
 
+// The duplicate-boundary check of OceanModelAdapter::appendBoundaryRecord
+// (OceanModelAdapter.cpp:216-233): two records must agree under float `!=`
+// (so a NaN always conflicts).  The reference loops over points (j, i) and,
+// per point, checks zeta, then u/v over the levels, then w/AKt, throwing at
+// the first conflict; the conflict it reports is therefore the one with the
+// smallest key point*3 + class (class 0 zeta, 1 horizontal, 2 vertical).
+// One work-item per (row, point), row 0 zeta, rows 1..s_rho u/v of a level,
+// the rest w/AKt of a level; each wave offers its lowest conflicting lane
+// (the smallest point) to one atomicMin.
+__global__ __launch_bounds__(kBlock) void compare_records_kernel(RecordPair<float> F, int s_rho, size_t plane,
+                                                                 unsigned long long *__restrict__ key) {
+    const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const int row = blockIdx.y;
+    bool conflict = false;
+    unsigned cls = 0;
+    if (c < plane) {
+        if (row == 0) {
+            conflict = F.zeta0[c] != F.zeta1[c];
+        } else if (row <= s_rho) {
+            const size_t o = (size_t)(row - 1) * plane + c;
+            conflict = F.u0[o] != F.u1[o] || F.v0[o] != F.v1[o];
+            cls = 1;
+        } else {
+            const size_t o = (size_t)(row - 1 - s_rho) * plane + c;
+            conflict = F.w0[o] != F.w1[o] || F.akt0[o] != F.akt1[o];
+            cls = 2;
+        }
+    }
+    const unsigned long long lanes = __ballot(conflict);
+    if (lanes && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)lanes) - 1))
+        atomicMin(key, (unsigned long long)c * 3ull + cls);
+}
+
 }  // namespace
+
+hipError_t launch_compare_records(const RecordPair<float> &F, int s_rho, size_t plane, unsigned long long *key,
+                                  hipStream_t stream) {
+    if (plane == 0) return hipSuccess;
+    const dim3 grid((unsigned)((plane + kBlock - 1) / kBlock), (unsigned)(2 * s_rho + 2));
+    compare_records_kernel<<<grid, kBlock, 0, stream>>>(F, s_rho, plane, key);
+    return hipGetLastError();
+}
 
 hipError_t launch_roms_normalize(const RomsDims &d, const RomsMissing &miss, const uint8_t *face,
                                  const double2 *rot, const float *u, const float *v, float *U, float *V, float *W,
