@@ -1,0 +1,32 @@
+"""The C++17 host layer (include/wacomm_hip.hpp) exercised by C++ programs
+built with g++ against it (tests/cpp/*.cpp, built by
+wacommplusplus_amd.build into tests/cpp/_bin/)."""
+import os
+import subprocess
+
+import pytest
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_bin")
+
+
+def run(name, tmp_path):
+    exe = os.path.join(BIN, name)
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} missing: build with `python -m wacommplusplus_amd.build`")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASS" in r.stdout
+
+
+def test_particles_text_restart(tmp_path):
+    """tests/RestartTest.cpp:104-122 restated, plus the Fortran-record and
+    malformed-record branches of Particles::loadFromTxt (CPU only)."""
+    run("particles_txt_test", tmp_path)
+
+
+@pytest.mark.gpu
+def test_wacomm_run_and_concentration(tmp_path):
+    """Wacomm::run and Concentration::evaluate of the C++ layer on the
+    device: uniform-current closed form and exact counts."""
+    run("wacomm_run_test", tmp_path)

@@ -58,7 +58,29 @@ def build(verbose: bool = False) -> str:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         os.replace(tmp_lib, LIB)
+    build_cpp_tests(verbose)
     return LIB
+
+
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+
+
+def build_cpp_tests(verbose: bool = False) -> None:
+    """The C++ host-layer tests (tests/cpp/*.cpp), built with g++ against
+    include/wacomm_hip.hpp and the library, into tests/cpp/_bin/."""
+    out = os.path.join(CPP_TESTS, "_bin")
+    os.makedirs(out, exist_ok=True)
+    for name in sorted(os.listdir(CPP_TESTS)):
+        if not name.endswith(".cpp"):
+            continue
+        exe = os.path.join(out, name[:-4])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(CPP_TESTS, name), "-L", PKG, "-lwacomm_hip",
+               "-Wl,-rpath,$ORIGIN/../../../wacommplusplus_amd", "-o", exe + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":
