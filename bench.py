@@ -127,6 +127,43 @@ def roms_normalize_bench(wc, ctx, W, n_records):
                 frac=achieved / HBM_PEAK_GBS, upload_ms_pcie_inclusive=1e3 * wall / len(recs))
 
 
+def ensemble_bench(wc, ctx, W, cfg, steps):
+    """The batched-ensemble workload: 256 release scenarios on the same grid
+    and forcing (lognormal counts, median 40k, clipped to 10k-500k, ~16M in
+    total; own source, Kh log-uniform in [0.1, 10] m2/s, release offset
+    0-6 h), FP32 state, BF16 forcing, per-scenario surface counts every
+    interval.  The launch shape is autotuned on the first interval; the
+    `steps` intervals then run once with direct launches and once as
+    replays of the captured step, each from the same initial population."""
+    from wacommplusplus_amd.synthetic import ensemble
+    log("[ensemble] generating 256 scenarios")
+    p, scen, sigma, counts = ensemble(W)
+    ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=W.T, store="bf16")
+    for t in range(W.T):
+        ctx.upload_record(t, *W.record(t))
+    times = [float(W.times[q]) for q in range(steps + 1)]
+    slots = list(range(steps + 1))
+    ctx.ens_set(p, scen, sigma)
+    best_ms = ctx.ens_autotune(cfg, times[0], times[1], 0, 1)
+    shape = ctx.ens_get_launch()
+    log(f"[ensemble] {len(p)} particles; autotuned block={shape[0]} ppt={shape[1]} occ={shape[2]} "
+        f"({best_ms:.2f} ms on the first interval)")
+    out = {}
+    for use_graph in (False, True):
+        ctx.ens_set(p, scen, sigma)
+        st = ctx.ens_run(cfg, times, slots, use_graph=use_graph, surface=True)
+        key = "graph" if use_graph else "direct"
+        out[key] = dict(value=st.particle_steps / st.wall_s, device_ms=st.device_ms, wall_s=st.wall_s,
+                        particle_steps=int(st.particle_steps))
+        log(f"[ensemble] {key}: {out[key]}")
+    surf = ctx.ens_surface()
+    return dict(metric="particle-steps/s", value=out["graph"]["value"], scenarios=len(sigma), particles=len(p),
+                intervals=steps, launch=dict(block=shape[0], ppt=shape[1], occupancy_hint=shape[2]),
+                autotune_first_interval_ms=best_ms, graph=out["graph"], direct=out["direct"],
+                surface_counts_last_interval=int(surf.sum()), dtype="f32 state, bf16 forcing",
+                counts=dict(min=int(counts.min()), median=float(np.median(counts)), max=int(counts.max())))
+
+
 def cpu_baseline(W, cfg, p0, n_sample=100_000, n_intervals=4):
     """The oracle (FP64 C restatement of Particle::move, OpenMP dynamic,64 as
     Wacomm.cpp:501) on a bounded sample of the same workload."""
@@ -161,6 +198,7 @@ def main():
     ap.add_argument("--particles", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the precision comparison (its diffusion-off runs launch the move kernels "
                          "on other populations, which a kernel-trace average would mix in)")
@@ -255,6 +293,7 @@ def main():
         log(f"[pcie] {pcie:.4g} particle-steps/s")
     roms = roms_normalize_bench(wc, ctx, W, args.steps)
     log(f"[roms] {roms}")
+    ens = None if args.no_ensemble else ensemble_bench(wc, ctx, W, cfg, args.steps)
     ctx.close()
 
     main_r = results["f64"]
@@ -319,6 +358,7 @@ def main():
                                          **precision["conc_l2"]}},
         "pcie_inclusive_value": pcie,
         "roms_normalize": roms,
+        "ensemble": ens,
         "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),
         "alive_after": main_r["alive"],
     }

@@ -355,6 +355,62 @@ void wc_cell_metric_table(int eta, int xi, const double *lonRad, const double *l
  * generator can be checked against the published known-answer vectors. */
 void wc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* ---------------------------------------------------------------------------
+ * 3. Batched ensemble (no counterpart in the reference: many release
+ *    scenarios, each with its own source, horizontal diffusion and release
+ *    offset, moved as one population on the mixed-precision path)
+ * ------------------------------------------------------------------------- */
+
+/* Loads the ensemble population on the context's grid (wc_set_grid first;
+ * the storage type of wc_set_grid is the forcing precision).  particles[q]
+ * belongs to scenario scenario[q] < n_scenarios <= 256; sigma[s] is scenario
+ * s's horizontal diffusion scale (sqrt(2 Kh dti)); a particle's emission
+ * time is its release (late activation, Particle.cpp:211-215).  State is
+ * compact: integer cell + FP32 offsets and level, FP32 health, FP64 age and
+ * emission time.  Per-scenario surface-layer count grids
+ * [n_scenarios][eta][xi] (uint32) are allocated here. */
+int wc_ens_set(wc_ctx *ctx, const wc_particle_data *particles, const uint8_t *scenario, size_t n,
+               const double *sigma, int n_scenarios);
+
+/* Launch shape of the ensemble move: block in {64, 128, 256} work-items,
+ * ppt particles per work-item (1..64), occupancy hint occ in {0 (none),
+ * 3, 4} waves per SIMD.  wc_ens_autotune picks them by timing. */
+int wc_ens_set_launch(wc_ctx *ctx, int block, int ppt, int occ);
+int wc_ens_get_launch(wc_ctx *ctx, int *block, int *ppt, int *occ);
+
+/* Times every offered launch shape on one interval [start, end] (records in
+ * slot0, slot1) over the current population, which is restored afterwards,
+ * and keeps the fastest.  *best_ms (optional) receives its time. */
+int wc_ens_autotune(wc_ctx *ctx, const wc_config_data *config, double start, double end, int slot0, int slot1,
+                    double *best_ms);
+
+typedef struct wc_ens_stats {
+    double device_ms;         /* device time of the whole run (events) */
+    double wall_s;            /* host wall time of the launches and the final wait */
+    uint64_t particle_steps;  /* live substeps taken */
+    int intervals;
+} wc_ens_stats;
+
+/* Runs n_intervals forcing intervals: interval q spans times[q] ->
+ * times[q+1] with records in slots[q], slots[q+1] (n_intervals+1 entries
+ * each).  One step = pack the record pair, move the population, and when
+ * `surface` is nonzero clear and fill the per-scenario surface counts, all
+ * on the device reading the interval from a device array.  With use_graph
+ * the step is captured once into a HIP graph (kept until the launch shape,
+ * the grid or the population changes) and replayed per interval; otherwise
+ * the same kernels are launched directly.  The two give identical results. */
+int wc_ens_run(wc_ctx *ctx, const wc_config_data *config, const double *times, const int *slots, int n_intervals,
+               int use_graph, int surface, wc_ens_stats *stats);
+
+/* Per-scenario surface counts of the last step run with surface != 0:
+ * counts[n_scenarios][eta][xi]. */
+int wc_ens_surface(wc_ctx *ctx, uint32_t *counts);
+
+/* The ensemble population as particle_data (j = cell row + offset, i = cell
+ * column + offset; NaN for a particle that started outside the grid) and,
+ * when scenario is not NULL, each particle's scenario. */
+int wc_ens_get(wc_ctx *ctx, wc_particle_data *particles, uint8_t *scenario, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,8 @@ EXPORTS = (
     "wc_set_timing", "wc_kernel_time", "wc_reset_kernel_time", "wc_step_count", "wc_reset_step_count",
     "wc_version", "wc_cell_metric_table", "wc_philox4x32_10", "wc_emit", "wc_run_window",
     "wc_set_roms_grid", "wc_upload_roms_record", "wc_get_record", "wc_check_roms_grid", "wc_compare_records",
+    "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
+    "wc_ens_get",
 )
 
 
@@ -61,6 +63,11 @@ class RomsGridDesc(C.Structure):
                [(n, C.c_void_p) for n in ("missing_u", "missing_v", "missing_angle")] + \
                [(n, C.c_int) for n in ("n_missing_u", "n_missing_v", "n_missing_angle")] + \
                [("s_rho_values", C.c_void_p)]
+
+
+class EnsStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("wall_s", C.c_double), ("particle_steps", C.c_uint64),
+                ("intervals", C.c_int)]
 
 
 class WcError(RuntimeError):
@@ -111,6 +118,13 @@ def load_library(path: str = LIB_PATH):
         "wc_get_record": [vp, i, vp, vp, vp, vp, vp],
         "wc_check_roms_grid": [vp, C.POINTER(RomsGridDesc)],
         "wc_compare_records": [vp, i, i, C.POINTER(i), C.POINTER(C.c_longlong)],
+        "wc_ens_set": [vp, vp, vp, sz, vp, i],
+        "wc_ens_set_launch": [vp, i, i, i],
+        "wc_ens_get_launch": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)],
+        "wc_ens_autotune": [vp, C.POINTER(ConfigData), d, d, i, i, C.POINTER(d)],
+        "wc_ens_run": [vp, C.POINTER(ConfigData), vp, vp, i, i, i, C.POINTER(EnsStats)],
+        "wc_ens_surface": [vp, vp],
+        "wc_ens_get": [vp, vp, vp, sz],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -321,6 +335,52 @@ class Context:
         self._chk(self.L.wc_run_window(self.h, C.byref(config), C.byref(desc), src, len(sources), PATH[path],
                                        None if out is None else _p(out), int(mask_output), C.byref(st)))
         return out, st
+
+    # -- batched ensemble ---------------------------------------------------
+    def ens_set(self, particles: np.ndarray, scenario, sigma):
+        """An ensemble population on the current grid: particles (PARTICLE_DTYPE),
+        scenario (uint8 per particle), sigma (per scenario)."""
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        sc = np.ascontiguousarray(scenario, dtype=np.uint8)
+        sg = _f64(sigma)
+        assert len(sc) == len(p)
+        self._chk(self.L.wc_ens_set(self.h, _p(p), _p(sc), len(p), _p(sg), len(sg)))
+        self.ens_shape = (len(sg),) + self.shape[1:]
+        self.ens_n = len(p)
+
+    def ens_set_launch(self, block=256, ppt=1, occ=0):
+        self._chk(self.L.wc_ens_set_launch(self.h, block, ppt, occ))
+
+    def ens_get_launch(self):
+        b, p, o = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.wc_ens_get_launch(self.h, C.byref(b), C.byref(p), C.byref(o)))
+        return b.value, p.value, o.value
+
+    def ens_autotune(self, config: ConfigData, start, end, slot0, slot1):
+        ms = C.c_double()
+        self._chk(self.L.wc_ens_autotune(self.h, C.byref(config), float(start), float(end), slot0, slot1,
+                                         C.byref(ms)))
+        return ms.value
+
+    def ens_run(self, config: ConfigData, times, slots, use_graph=True, surface=True) -> EnsStats:
+        t = _f64(times)
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        assert len(t) == len(sl) and len(t) >= 1
+        st = EnsStats()
+        self._chk(self.L.wc_ens_run(self.h, C.byref(config), _p(t), _p(sl), len(t) - 1, int(use_graph),
+                                    int(surface), C.byref(st)))
+        return st
+
+    def ens_surface(self):
+        out = np.zeros(self.ens_shape, np.uint32)
+        self._chk(self.L.wc_ens_surface(self.h, _p(out)))
+        return out
+
+    def ens_get(self):
+        out = np.zeros(self.ens_n, dtype=PARTICLE_DTYPE)
+        sc = np.zeros(self.ens_n, np.uint8)
+        self._chk(self.L.wc_ens_get(self.h, _p(out), _p(sc), self.ens_n))
+        return out, sc
 
     # -- measurement --------------------------------------------------------
     def set_timing(self, on: bool = True):

@@ -68,6 +68,26 @@ struct wc_ctx {
     // duplicate-record comparison key
     unsigned long long *d_key = nullptr;
 
+    // batched ensemble (wc_ens_*): compact state, a saved copy for the
+    // autotuner, scenario sigmas, per-scenario surface counts, the device
+    // step array and counter read by the captured step, the graph
+    struct Ens {
+        bool have = false;
+        size_t n = 0;
+        int nScen = 0;
+        EnsSoA s{}, save{};
+        float *d_sigma = nullptr;
+        uint32_t *d_surface = nullptr;
+        EnsStep *d_steps = nullptr;
+        int *d_stepIdx = nullptr;
+        unsigned long long *d_count = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int block = 256, ppt = 1, occ = 0;
+        bool tuned = false;
+        bool graphSurface = false;
+    } ens;
+
     // timing: an event pair per timed launch, resolved when queried (no
     // synchronisation between launches)
     bool timing = false;
@@ -396,7 +416,33 @@ int wc_ctx_create(int device, wc_ctx **out) {
     return WC_OK;
 }
 
+static void free_ens_graph(wc_ctx *c) {
+    if (c->ens.exec) hipGraphExecDestroy(c->ens.exec);
+    if (c->ens.graph) hipGraphDestroy(c->ens.graph);
+    c->ens.exec = nullptr;
+    c->ens.graph = nullptr;
+}
+
+static void free_ens_soa(EnsSoA &s) {
+    hipFree(s.id); hipFree(s.cell); hipFree(s.fj); hipFree(s.fi); hipFree(s.k); hipFree(s.health);
+    hipFree(s.age); hipFree(s.time); hipFree(s.scen);
+    s = EnsSoA{};
+}
+
+static void free_ens(wc_ctx *c) {
+    free_ens_graph(c);
+    free_ens_soa(c->ens.s);
+    free_ens_soa(c->ens.save);
+    hipFree(c->ens.d_sigma); hipFree(c->ens.d_surface); hipFree(c->ens.d_steps); hipFree(c->ens.d_stepIdx);
+    hipFree(c->ens.d_count);
+    const int block = c->ens.block, ppt = c->ens.ppt, occ = c->ens.occ;
+    const bool tuned = c->ens.tuned;
+    c->ens = wc_ctx::Ens{};
+    c->ens.block = block; c->ens.ppt = ppt; c->ens.occ = occ; c->ens.tuned = tuned;
+}
+
 static void free_grid(wc_ctx *c) {
+    free_ens_graph(c);      // the captured step holds the pack buffer
     hipFree(c->d_h); hipFree(c->d_depth); hipFree(c->d_dx); hipFree(c->d_dy); hipFree(c->d_mask);
     c->d_h = c->d_depth = c->d_dx = c->d_dy = c->d_mask = nullptr;
     hipFree(c->d_hf); hipFree(c->d_md); hipFree(c->d_depthf); hipFree(c->d_pack); hipFree(c->d_metric);
@@ -427,6 +473,7 @@ int wc_ctx_destroy(wc_ctx *c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     free_grid(c);
+    free_ens(c);
     free_soa(c->s);
     free_soa(c->s2);
     hipFree(c->d_blockCounts);
@@ -955,6 +1002,321 @@ int wc_run_window(wc_ctx *c, const wc_config_data *cfg, const wc_window *win, co
     st.particle_steps = steps1 - steps0;
     st.particles = c->n;
     if (stats) *stats = st;
+    return WC_OK;
+}
+
+// ---- batched ensemble ------------------------------------------------------
+
+namespace {
+
+constexpr int kEnsMaxSteps = 4096;
+
+int alloc_ens_soa(wc_ctx *c, EnsSoA &s, size_t n) {
+    HIPCHK(c, hipMalloc(&s.id, n * 8));
+    HIPCHK(c, hipMalloc(&s.cell, n * 4));
+    HIPCHK(c, hipMalloc(&s.fj, n * 4));
+    HIPCHK(c, hipMalloc(&s.fi, n * 4));
+    HIPCHK(c, hipMalloc(&s.k, n * 4));
+    HIPCHK(c, hipMalloc(&s.health, n * 4));
+    HIPCHK(c, hipMalloc(&s.age, n * 8));
+    HIPCHK(c, hipMalloc(&s.time, n * 8));
+    HIPCHK(c, hipMalloc(&s.scen, n));
+    return WC_OK;
+}
+
+int copy_ens_soa(wc_ctx *c, const EnsSoA &dst, const EnsSoA &src, size_t n) {
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    HIPCHK(c, hipMemcpyAsync(dst.id, src.id, n * 8, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.cell, src.cell, n * 4, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.fj, src.fj, n * 4, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.fi, src.fi, n * 4, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.k, src.k, n * 4, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.health, src.health, n * 4, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.age, src.age, n * 8, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.time, src.time, n * 8, k, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst.scen, src.scen, n, k, c->stream));
+    return WC_OK;
+}
+
+EnsLaunch ens_launch(wc_ctx *c, bool surface) {
+    EnsLaunch L{};
+    L.steps = c->ens.d_steps;
+    L.stepIdx = c->ens.d_stepIdx;
+    L.packBuf = c->d_pack;
+    L.G = GridPtrs{c->d_h, c->d_depth, c->d_dx, c->d_dy, nullptr, nullptr, c->d_hf, c->d_md, nullptr, c->d_depthf};
+    L.s = c->ens.s;
+    L.n = (int)c->ens.n;
+    L.sigma = c->ens.d_sigma;
+    L.nScen = c->ens.nScen;
+    L.surface = surface ? c->ens.d_surface : nullptr;
+    L.stepCounter = c->ens.d_count;
+    L.s_w = c->s_w; L.s_rho = c->s_rho; L.eta = c->eta; L.xi = c->xi;
+    L.block = c->ens.block; L.ppt = c->ens.ppt; L.occ = c->ens.occ;
+    return L;
+}
+
+// Device step array for intervals [times[q], times[q+1]] with records in
+// slots[q], slots[q+1]; inactive intervals (restart checkpoint) take no
+// substeps.  Resets the device step counter.
+int upload_ens_steps(wc_ctx *c, const wc_config_data *cfg, const double *times, const int *slots, int n) {
+    std::vector<EnsStep> st(n);
+    for (int q = 0; q < n; ++q) {
+        if (!std::isfinite(times[q]) || !std::isfinite(times[q + 1]))
+            return fail(c, WC_ERR_INVALID, "non-finite interval %d", q);
+        const int a = slots[q], b = slots[q + 1], ns = (int)c->slots.size();
+        if (a < 0 || a >= ns || b < 0 || b >= ns || !c->slots[a].filled || !c->slots[b].filled)
+            return fail(c, WC_ERR_INVALID, "forcing slots %d/%d not uploaded", a, b);
+        MoveParams &P = st[q].P;
+        if (!make_params(cfg, times[q], times[q + 1], c->s_w, c->s_rho, c->eta, c->xi, P))
+            P.elapsed0 = P.intervalLength;                    // inactive: no substeps
+        const auto &A = c->slots[a], &B = c->slots[b];
+        st[q].F = RecordPair<float>{A.f[0], B.f[0], A.f[1], B.f[1], A.f[2], B.f[2], A.f[3], B.f[3], A.f[4], B.f[4]};
+    }
+    HIPCHK(c, hipMemcpyAsync(c->ens.d_steps, st.data(), n * sizeof(EnsStep), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_fill_u32((uint32_t *)c->ens.d_stepIdx, 1, 0u, c->stream));
+    // the host vector must outlive the copy
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+}  // namespace
+
+int wc_ens_set(wc_ctx *c, const wc_particle_data *p, const uint8_t *scenario, size_t n, const double *sigma,
+               int n_scenarios) {
+    if (!c || (n && (!p || !scenario)) || !sigma) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    if (n_scenarios < 1 || n_scenarios > 256) return fail(c, WC_ERR_INVALID, "1..256 scenarios");
+    if (n > (size_t)INT32_MAX) return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", INT32_MAX);
+    const size_t plane = (size_t)c->eta * c->xi;
+    if ((size_t)n_scenarios * plane > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "surface grids too large");
+    for (size_t q = 0; q < n; ++q)
+        if (scenario[q] >= n_scenarios) return fail(c, WC_ERR_INVALID, "particle %zu: scenario %d", q, scenario[q]);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_ens(c);
+    auto &E = c->ens;
+    const size_t cap = n ? n : 1;
+    int rc = alloc_ens_soa(c, E.s, cap);
+    if (!rc) rc = alloc_ens_soa(c, E.save, cap);
+    if (rc) { free_ens(c); return rc; }
+    HIPCHK(c, hipMalloc(&E.d_sigma, n_scenarios * sizeof(float)));
+    HIPCHK(c, hipMalloc(&E.d_surface, (size_t)n_scenarios * plane * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&E.d_steps, kEnsMaxSteps * sizeof(EnsStep)));
+    HIPCHK(c, hipMalloc(&E.d_stepIdx, sizeof(int)));
+    HIPCHK(c, hipMalloc(&E.d_count, sizeof(unsigned long long)));
+    // host split into cell + FP32 offsets; a position outside the
+    // interpolation cells gets an invalid cell (it ends at its first substep,
+    // as validInterpolationCell fails, NumericalHelpers.hpp:12-14)
+    std::vector<uint64_t> id(n);
+    std::vector<uint32_t> cell(n);
+    std::vector<float> fj(n), fi(n), k(n), health(n), sg(n_scenarios);
+    std::vector<double> age(n), time(n);
+    for (size_t q = 0; q < n; ++q) {
+        const wc_particle_data &d = p[q];
+        const double jfl = std::floor(d.j), ifl = std::floor(d.i);
+        const bool inside = jfl >= 0 && ifl >= 0 && jfl + 1 < c->eta && ifl + 1 < c->xi;
+        float oj = inside ? (float)(d.j - jfl) : 0.f, oi = inside ? (float)(d.i - ifl) : 0.f;
+        long jc = (long)jfl, ic = (long)ifl;
+        if (oj >= 1.0f) { oj -= 1.0f; ++jc; }
+        if (oi >= 1.0f) { oi -= 1.0f; ++ic; }
+        id[q] = d.id;
+        cell[q] = inside ? (uint32_t)(jc * c->xi + ic) : 0xFFFFFFFFu;
+        fj[q] = oj; fi[q] = oi;
+        k[q] = (float)d.k;
+        health[q] = (float)d.health;
+        age[q] = d.age;
+        time[q] = d.time;
+    }
+    for (int q = 0; q < n_scenarios; ++q) sg[q] = (float)sigma[q];
+    if (n) {
+        HIPCHK(c, hipMemcpy(E.s.id, id.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.cell, cell.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.fj, fj.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.fi, fi.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.k, k.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.health, health.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.age, age.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.time, time.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(E.s.scen, scenario, n, hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipMemcpy(E.d_sigma, sg.data(), n_scenarios * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, launch_fill_u32(E.d_surface, (size_t)n_scenarios * plane, 0u, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    E.n = n;
+    E.nScen = n_scenarios;
+    E.have = true;
+    return WC_OK;
+}
+
+int wc_ens_set_launch(wc_ctx *c, int block, int ppt, int occ) {
+    if (!c) return WC_ERR_INVALID;
+    if ((block != 64 && block != 128 && block != 256) || ppt < 1 || ppt > 64 || (occ != 0 && occ != 3 && occ != 4))
+        return fail(c, WC_ERR_INVALID, "launch block=%d ppt=%d occ=%d not offered", block, ppt, occ);
+    free_ens_graph(c);
+    c->ens.block = block; c->ens.ppt = ppt; c->ens.occ = occ;
+    return WC_OK;
+}
+
+int wc_ens_get_launch(wc_ctx *c, int *block, int *ppt, int *occ) {
+    if (!c) return WC_ERR_INVALID;
+    if (block) *block = c->ens.block;
+    if (ppt) *ppt = c->ens.ppt;
+    if (occ) *occ = c->ens.occ;
+    return WC_OK;
+}
+
+int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double end, int slot0, int slot1,
+                    double *best_ms) {
+    if (!c || !cfg) return WC_ERR_INVALID;
+    if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
+    int rc = check_supported(c, cfg);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    auto &E = c->ens;
+    const double times[2] = {start, end};
+    const int slots[2] = {slot0, slot1};
+    rc = copy_ens_soa(c, E.save, E.s, E.n);                    // the population to restore
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const int blocks[3] = {64, 128, 256}, ppts[3] = {1, 2, 4}, occs[3] = {0, 3, 4};
+    double best = INFINITY;
+    int bb = E.block, bp = E.ppt, bo = E.occ;
+    for (int trial = -1; trial < 27 && !rc; ++trial) {
+        const int t = trial < 0 ? 26 : trial;                   // trial -1 warms up the last candidate
+        E.block = blocks[t % 3]; E.ppt = ppts[(t / 3) % 3]; E.occ = occs[t / 9];
+        if ((rc = copy_ens_soa(c, E.s, E.save, E.n))) break;
+        if ((rc = upload_ens_steps(c, cfg, times, slots, 1))) break;
+        const EnsLaunch L = ens_launch(c, false);
+        if (hipEventRecord(e0, c->stream) != hipSuccess || launch_ens_step(c->store, L, c->stream) != hipSuccess ||
+            hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(c, WC_ERR_HIP, "autotune launch block=%d ppt=%d occ=%d failed", E.block, E.ppt, E.occ);
+            break;
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (trial >= 0 && ms < best) { best = ms; bb = E.block; bp = E.ppt; bo = E.occ; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!rc) rc = copy_ens_soa(c, E.s, E.save, E.n);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (rc) return rc;
+    free_ens_graph(c);
+    E.block = bb; E.ppt = bp; E.occ = bo;
+    E.tuned = true;
+    if (best_ms) *best_ms = best;
+    return WC_OK;
+}
+
+int wc_ens_run(wc_ctx *c, const wc_config_data *cfg, const double *times, const int *slots, int n_intervals,
+               int use_graph, int surface, wc_ens_stats *stats) {
+    if (!c || !cfg || !times || !slots || n_intervals < 0) return WC_ERR_INVALID;
+    if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
+    if (n_intervals > kEnsMaxSteps) return fail(c, WC_ERR_INVALID, "at most %d intervals per call", kEnsMaxSteps);
+    int rc = check_supported(c, cfg);
+    if (rc) return rc;
+    if (stats) *stats = wc_ens_stats{};
+    if (n_intervals == 0) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    auto &E = c->ens;
+    rc = upload_ens_steps(c, cfg, times, slots, n_intervals);
+    if (rc) return rc;
+    HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_count), 2, 0u, c->stream));
+    const EnsLaunch L = ens_launch(c, surface != 0);
+    const bool wantSurface = surface != 0;
+    if (use_graph) {
+        // the captured step depends on the launch shape and on whether it grids
+        if (E.exec && (E.graphSurface != wantSurface)) free_ens_graph(c);
+        if (!E.exec) {
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = launch_ens_step(c->store, L, c->stream);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                return fail(c, WC_ERR_HIP, "capturing the ensemble step failed: %s / %s", hipGetErrorString(le),
+                            hipGetErrorString(ce));
+            }
+            E.graph = g;
+            HIPCHK(c, hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0));
+            E.graphSurface = wantSurface;
+        }
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t he = hipEventRecord(e0, c->stream);
+    for (int q = 0; q < n_intervals && he == hipSuccess; ++q)
+        he = use_graph ? hipGraphLaunch(E.exec, c->stream) : launch_ens_step(c->store, L, c->stream);
+    if (he == hipSuccess) he = hipEventRecord(e1, c->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    const auto t1 = std::chrono::steady_clock::now();
+    float ms = 0;
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (he != hipSuccess) return fail(c, WC_ERR_HIP, "ensemble run: %s", hipGetErrorString(he));
+    unsigned long long count = 0;
+    HIPCHK(c, hipMemcpy(&count, E.d_count, sizeof count, hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->device_ms = ms;
+        stats->wall_s = std::chrono::duration<double>(t1 - t0).count();
+        stats->particle_steps = count;
+        stats->intervals = n_intervals;
+    }
+    return WC_OK;
+}
+
+int wc_ens_surface(wc_ctx *c, uint32_t *counts) {
+    if (!c || !counts) return WC_ERR_INVALID;
+    if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t cells = (size_t)c->ens.nScen * c->eta * c->xi;
+    HIPCHK(c, hipMemcpyAsync(counts, c->ens.d_surface, cells * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+int wc_ens_get(wc_ctx *c, wc_particle_data *out, uint8_t *scenario, size_t n) {
+    if (!c || (n && !out)) return WC_ERR_INVALID;
+    if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
+    if (n > c->ens.n) return fail(c, WC_ERR_INVALID, "asked for %zu particles, the ensemble holds %zu", n, c->ens.n);
+    if (!n) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const EnsSoA &S = c->ens.s;
+    std::vector<uint64_t> id(n);
+    std::vector<uint32_t> cell(n);
+    std::vector<float> fj(n), fi(n), k(n), health(n);
+    std::vector<double> age(n), time(n);
+    std::vector<uint8_t> sc(n);
+    const hipMemcpyKind D2H = hipMemcpyDeviceToHost;
+    HIPCHK(c, hipMemcpyAsync(id.data(), S.id, n * 8, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cell.data(), S.cell, n * 4, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(fj.data(), S.fj, n * 4, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(fi.data(), S.fi, n * 4, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(k.data(), S.k, n * 4, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(health.data(), S.health, n * 4, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(age.data(), S.age, n * 8, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(time.data(), S.time, n * 8, D2H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sc.data(), S.scen, n, D2H, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < n; ++q) {
+        wc_particle_data d{};
+        d.id = id[q];
+        const bool valid = cell[q] != 0xFFFFFFFFu;
+        const uint32_t jc = valid ? cell[q] / (uint32_t)c->xi : 0, ic = valid ? cell[q] - jc * (uint32_t)c->xi : 0;
+        d.j = valid ? (double)jc + (double)fj[q] : NAN;
+        d.i = valid ? (double)ic + (double)fi[q] : NAN;
+        d.k = k[q];
+        d.health = health[q];
+        d.age = age[q];
+        d.time = time[q];
+        out[q] = d;
+        if (scenario) scenario[q] = sc[q];
+    }
     return WC_OK;
 }
 

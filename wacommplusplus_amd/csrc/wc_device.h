@@ -2,7 +2,7 @@
 // libwacomm_hip.so (gfx950).  The parity and gridding sources are compiled with
 // contraction off (-ffp-contract=off plus a file pragma) so that every
 // rounding point of the reference's Particle::move is kept; the mixed-precision
-// source enables contraction explicitly.
+// source fuses multiply-adds within each source expression (contract(on)).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,6 +45,24 @@ struct RecordPair {
 struct GridPtrs {
     const double *h, *depth, *dx, *dy, *lonRad, *latRad;
     const float *hf, *idxf, *idyf, *depthf;
+};
+
+// One interval of a batched-ensemble run, read on the device by the captured
+// step (scalars and the interval's record pair).
+struct EnsStep {
+    MoveParams P;
+    RecordPair<float> F;
+};
+
+// Compact ensemble state, structure of arrays: cell = jI*xi + iI with FP32
+// offsets fj, fi in [0, 1) and FP32 level k; FP32 health; FP64 age and
+// emission time; the particle's scenario.
+struct EnsSoA {
+    uint64_t *id;
+    uint32_t *cell;
+    float *fj, *fi, *k, *health;
+    double *age, *time;
+    uint8_t *scen;
 };
 
 // Particle state, structure of arrays (the context's canonical layout).

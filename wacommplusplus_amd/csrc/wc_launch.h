@@ -67,4 +67,25 @@ hipError_t launch_roms_normalize(const RomsDims &d, const RomsMissing &miss, con
 hipError_t launch_compare_records(const RecordPair<float> &F, int s_rho, size_t plane, unsigned long long *key,
                                   hipStream_t stream);
 
+// One captured ensemble step: pack the record pair of steps[*stepIdx], move
+// the population (block threads, ppt particles per thread, occupancy hint occ
+// = 0 (none), 3 or 4 waves per SIMD), clear and fill the per-scenario surface counts
+// (when surface != nullptr), advance *stepIdx.  Every kernel reads the step
+// from device memory, so the same launch sequence is valid for any interval.
+struct EnsLaunch {
+    const EnsStep *steps;
+    int *stepIdx;
+    void *packBuf;
+    GridPtrs G;
+    EnsSoA s;
+    int n;
+    const float *sigma;
+    int nScen;
+    uint32_t *surface;
+    unsigned long long *stepCounter;
+    int s_w, s_rho, eta, xi;
+    int block, ppt, occ;
+};
+hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream);
+
 }  // namespace wc

@@ -28,10 +28,15 @@
 // reference's SplitMix64 draws, not draw-for-draw identical.
 #include <hip/hip_fp16.h>
 
-#include "wc_device.h"
+#include "wc_launch.h"
 #include "wc_philox.h"
 
-#pragma clang fp contract(fast)
+// Multiply-adds are fused only within one source expression (contract(on),
+// not fast): fusion across statements depends on what the substep loop is
+// inlined into, and near-zero water columns (coast node lines, where the
+// reference's vertical step is 0/0) would then end differently in the
+// context kernel and the ensemble kernel.  With it, both give the same bits.
+#pragma clang fp contract(on)
 
 namespace wc {
 namespace {
@@ -159,49 +164,48 @@ __device__ __forceinline__ void advance(int cell, float off, float d, int &ncell
     }
 }
 
+// A particle of the mixed-precision path during one interval, in registers.
+struct Reg {
+    int jc, ic;
+    float fj, fi, k;
+    double health, age;
+    bool inside, healthFresh, changed;
+};
+
+// Converts an FP64 position into cell + offset; `inside` is the reference's
+// validInterpolationCell test (NumericalHelpers.hpp:12-14) and a finite k.
+__device__ __forceinline__ void to_cell(double jd, double id_, double kd, int eta, int xi, Reg &q) {
+    const double jfl = floor(jd), ifl = floor(id_);
+    q.inside = jfl >= 0 && ifl >= 0 && jfl + 1 < eta && ifl + 1 < xi && kd == kd;
+    q.jc = q.inside ? (int)jfl : 0;
+    q.ic = q.inside ? (int)ifl : 0;
+    q.fj = q.inside ? (float)(jd - jfl) : 0.f;
+    q.fi = q.inside ? (float)(id_ - ifl) : 0.f;
+    if (q.fj >= 1.0f) { q.fj -= 1.0f; ++q.jc; }
+    if (q.fi >= 1.0f) { q.fi -= 1.0f; ++q.ic; }
+}
+
+// The substeps of one forcing interval for one particle (Particle.cpp:
+// 178-815) on the cell + offset state; sigma is the horizontal diffusion
+// scale (config sigma, or the particle's scenario sigma in an ensemble).
+// Returns the live substeps taken.
 template <class T>
-__global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> S, SoA st, int n,
-                                                        unsigned long long *stepCounter) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Packed<T> &S, float sigma, uint64_t id,
+                                                 double ptime, Reg &q) {
     unsigned steps = 0;
-    if (idx < n) {
-        const uint64_t id = st.id[idx];
-        double kd = st.k[idx];
-        const double jd = st.j[idx], id_ = st.i[idx];
-        double health = st.health[idx], age = st.age[idx];
-        const double ptime = st.time[idx];
-        const float kLL = (float)P.kLowerLimit;
-        const int eta = P.eta, xi = P.xi;
-
-        // pre-step closures (Particle.cpp:134-176)
-        if (kd > 0) {
-            if (P.upper == WC_CLOSURE_CONSTRAINT) kd = 0;
-            else if (P.upper == WC_CLOSURE_KILL) health = -1;
-            else if (P.upper == WC_CLOSURE_REFLECTION) kd = -kd;
-        }
-        if (kd < P.kLowerLimit) {
-            if (P.lower == WC_CLOSURE_CONSTRAINT) kd = P.kLowerLimit;
-            else if (P.lower == WC_CLOSURE_KILL) health = -1;
-            else if (P.lower == WC_CLOSURE_REFLECTION) kd = 2.0 * P.kLowerLimit - kd;
-        }
-        // register state: cell + FP32 offsets
-        const double jfl = floor(jd), ifl = floor(id_);
-        bool inside = jfl >= 0 && ifl >= 0 && jfl + 1 < eta && ifl + 1 < xi && kd == kd;
-        int jc = inside ? (int)jfl : 0, ic = inside ? (int)ifl : 0;
-        float fj = inside ? (float)(jd - jfl) : 0.f, fi = inside ? (float)(id_ - ifl) : 0.f;
-        if (fj >= 1.0f) { fj -= 1.0f; ++jc; }
-        if (fi >= 1.0f) { fi -= 1.0f; ++ic; }
-        float k = (float)kd;
-        bool healthFresh = true;   // `health` holds the current value (else derive from age)
-        bool changed = false;
-
-        const float dir = (float)P.direction;
-        const float sv = (float)P.sv, shore = (float)P.shoreLimit;
-        const uint32_t seedLo = (uint32_t)P.seed, seedHi = (uint32_t)(P.seed >> 32);
-        const uint32_t ik = (uint32_t)P.intervalKey ^ (uint32_t)((uint64_t)P.intervalKey >> 32);
-        const double L = P.intervalLength;
-        const double invL = L == 0 ? 0.0 : 1.0 / L, invDti = 1.0 / P.dti;
-        double elapsed = P.elapsed0;
+    int jc = q.jc, ic = q.ic;
+    float fj = q.fj, fi = q.fi, k = q.k;
+    double health = q.health, age = q.age;
+    bool inside = q.inside, healthFresh = q.healthFresh, changed = q.changed;
+    const float kLL = (float)P.kLowerLimit;
+    const int eta = P.eta, xi = P.xi;
+    const float dir = (float)P.direction;
+    const float sv = (float)P.sv, shore = (float)P.shoreLimit;
+    const uint32_t seedLo = (uint32_t)P.seed, seedHi = (uint32_t)(P.seed >> 32);
+    const uint32_t ik = (uint32_t)P.intervalKey ^ (uint32_t)((uint64_t)P.intervalKey >> 32);
+    const double L = P.intervalLength;
+    const double invL = L == 0 ? 0.0 : 1.0 / L, invDti = 1.0 / P.dti;
+    double elapsed = P.elapsed0;
 
         while (elapsed < L) {
             const double stepDt = fmin(P.dti, L - elapsed);
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
             S.wa_at(kI, kF, c, w0, a1, alpha, P.randomOn != 0, ww, aa);
             float rx = 0, ry = 0, rz = 0;
             if (P.randomOn) {
-                const float sigmaDepth = (float)P.sigma * (1.0f - k / kLL);
+                const float sigmaDepth = sigma * (1.0f - k / kLL);
                 const uint32_t substep = (uint32_t)floor(elapsed * invDti + 1e-9);
                 const Philox4 r = philox4x32_10(substep, ik, (uint32_t)id, (uint32_t)(id >> 32), seedLo, seedHi);
                 const float u1 = ((float)(r.v[0] >> 8) + 0.5f) * 0x1.0p-24f;
@@ -368,16 +372,53 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
             }
             elapsed += stepDt;
         }
-        if (!healthFresh) health = exp(-age / P.tau0);
-        if (changed) {
-            st.k[idx] = (double)k;
-            st.j[idx] = (double)jc + (double)fj;
-            st.i[idx] = (double)ic + (double)fi;
-            st.health[idx] = health;
-            st.age[idx] = age;
-        } else if (kd != st.k[idx] || health != st.health[idx]) {  // pre-step closures only
+        q.jc = jc; q.ic = ic; q.fj = fj; q.fi = fi; q.k = k;
+    q.health = health; q.age = age;
+    q.inside = inside; q.healthFresh = healthFresh; q.changed = changed;
+    return steps;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> S, SoA st, int n,
+                                                        unsigned long long *stepCounter) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned steps = 0;
+    if (idx < n) {
+        const uint64_t id = st.id[idx];
+        double kd = st.k[idx];
+        const double jd = st.j[idx], id_ = st.i[idx];
+        double health = st.health[idx];
+        const double ptime = st.time[idx];
+
+        // pre-step closures (Particle.cpp:134-176)
+        if (kd > 0) {
+            if (P.upper == WC_CLOSURE_CONSTRAINT) kd = 0;
+            else if (P.upper == WC_CLOSURE_KILL) health = -1;
+            else if (P.upper == WC_CLOSURE_REFLECTION) kd = -kd;
+        }
+        if (kd < P.kLowerLimit) {
+            if (P.lower == WC_CLOSURE_CONSTRAINT) kd = P.kLowerLimit;
+            else if (P.lower == WC_CLOSURE_KILL) health = -1;
+            else if (P.lower == WC_CLOSURE_REFLECTION) kd = 2.0 * P.kLowerLimit - kd;
+        }
+        Reg q;
+        to_cell(jd, id_, kd, P.eta, P.xi, q);
+        q.k = (float)kd;
+        q.health = health;
+        q.age = st.age[idx];
+        q.healthFresh = true;   // `health` holds the current value (else derive from age)
+        q.changed = false;
+        steps = run_interval<T>(P, S, (float)P.sigma, id, ptime, q);
+        if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
+        if (q.changed) {
+            st.k[idx] = (double)q.k;
+            st.j[idx] = (double)q.jc + (double)q.fj;
+            st.i[idx] = (double)q.ic + (double)q.fi;
+            st.health[idx] = q.health;
+            st.age[idx] = q.age;
+        } else if (kd != st.k[idx] || q.health != st.health[idx]) {  // pre-step closures only
             st.k[idx] = kd;
-            st.health[idx] = health;
+            st.health[idx] = q.health;
         }
     }
     if (stepCounter) wave_add_u64(stepCounter, steps);
@@ -385,8 +426,9 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
 
 // Interleaves the interval's records (FP32 ring slots) into the gather layout.
 template <class T>
-__global__ void pack_kernel(RecordPair<float> F, const float *__restrict__ hf, UV4<T> *__restrict__ uv,
-                            WA8<T> *__restrict__ wa, ZH *__restrict__ z, int s_rho, int s_w, size_t plane) {
+__device__ __forceinline__ void pack_body(const RecordPair<float> &F, const float *__restrict__ hf,
+                                          UV4<T> *__restrict__ uv, WA8<T> *__restrict__ wa, ZH *__restrict__ z,
+                                          int s_rho, int s_w, size_t plane) {
     const int nlev = s_w - 1;                 // particle levels k = -(nlev-1) .. 0
     const size_t total = plane * nlev;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
@@ -408,6 +450,147 @@ __global__ void pack_kernel(RecordPair<float> F, const float *__restrict__ hf, U
         if (lv == 0) z[c] = ZH{tof<T>(fromf<T>(F.zeta0[c])), tof<T>(fromf<T>(F.zeta1[c])), hf[c], 0.0f};
     }
 }
+
+template <class T>
+__global__ void pack_kernel(RecordPair<float> F, const float *__restrict__ hf, UV4<T> *__restrict__ uv,
+                            WA8<T> *__restrict__ wa, ZH *__restrict__ z, int s_rho, int s_w, size_t plane) {
+    pack_body<T>(F, hf, uv, wa, z, s_rho, s_w, plane);
+}
+
+// ---- batched ensemble ------------------------------------------------------
+//
+// Many release scenarios share the grid, the forcing and one population.
+// Each particle carries its scenario (one byte), which selects its horizontal
+// diffusion scale; release offsets are particle emission times (the
+// reference's late activation, Particle.cpp:211-215).  State is compact: cell
+// index + FP32 offsets and level, FP32 health, FP64 age and emission time.
+// The interval's scalars and record pointers are read from a device array
+// indexed by a device step counter, so one captured graph (pack -> move ->
+// clear -> surface grid -> advance) replays interval after interval.
+
+template <class T>
+__global__ void ens_pack_kernel(const EnsStep *__restrict__ steps, const int *__restrict__ stepIdx,
+                                const float *__restrict__ hf, UV4<T> *__restrict__ uv, WA8<T> *__restrict__ wa,
+                                ZH *__restrict__ z, int s_rho, int s_w, size_t plane) {
+    const RecordPair<float> F = steps[*stepIdx].F;
+    pack_body<T>(F, hf, uv, wa, z, s_rho, s_w, plane);
+}
+
+// cell + offset -> FP64 coordinate -> cell + FP32 offset, as the FP64 state
+// of the context path stores and reloads it (to_cell); NaN stays NaN.
+__device__ __forceinline__ void requantize(int &cell, float &off) {
+    const double v = (double)cell + (double)off;
+    if (!(v == v)) return;
+    const double fl = floor(v);
+    cell = (int)fl;
+    off = (float)(v - fl);
+    if (off >= 1.0f) { off -= 1.0f; ++cell; }
+}
+
+// Occ = 0: no occupancy hint (the compiler's register allocation: 175-191
+// VGPRs, 2 waves per SIMD); Occ = 3: at most 168 VGPRs (3 waves per SIMD);
+// Occ = 4: at most 128 VGPRs (4 waves per SIMD, with spills).  The
+// autotuner times them.
+template <class T, int Occ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occ > 0 ? Occ : 1)))
+void move_ens_kernel(const EnsStep *__restrict__ steps, const int *__restrict__ stepIdx, Packed<T> S, EnsSoA st,
+                     int n, const float *__restrict__ sigmaTab, int ppt, unsigned long long *stepCounter) {
+    const MoveParams P = steps[*stepIdx].P;
+    unsigned total = 0;
+    const int stride = gridDim.x * blockDim.x;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int r = 0; r < ppt && idx < n; ++r, idx += stride) {
+        const uint32_t cell = st.cell[idx];
+        Reg q;
+        q.jc = (int)(cell / (uint32_t)P.xi);
+        q.ic = (int)(cell - (uint32_t)q.jc * (uint32_t)P.xi);
+        q.fj = st.fj[idx];
+        q.fi = st.fi[idx];
+        // an offset that rounded to 1.0f (the domain reflection can produce
+        // one) belongs to the next cell, as the FP64 reload of the context
+        // path (to_cell) makes it
+        if (q.fj >= 1.0f) { q.fj -= 1.0f; ++q.jc; }
+        if (q.fi >= 1.0f) { q.fi -= 1.0f; ++q.ic; }
+        float k0 = st.k[idx];
+        const float h0 = st.health[idx];
+        double health = h0;
+        const float kLL = (float)P.kLowerLimit;
+        float k = k0;
+        if (k > 0.f) {                                             // Particle.cpp:134-176
+            if (P.upper == WC_CLOSURE_CONSTRAINT) k = 0.f;
+            else if (P.upper == WC_CLOSURE_KILL) health = -1;
+            else if (P.upper == WC_CLOSURE_REFLECTION) k = -k;
+        }
+        if (k < kLL) {
+            if (P.lower == WC_CLOSURE_CONSTRAINT) k = kLL;
+            else if (P.lower == WC_CLOSURE_KILL) health = -1;
+            else if (P.lower == WC_CLOSURE_REFLECTION) k = 2.0f * kLL - k;
+        }
+        q.k = k;
+        q.inside = valid_cell(q.jc, q.ic, P.eta, P.xi) && q.fj == q.fj && q.fi == q.fi && k == k;
+        q.health = health;
+        q.age = st.age[idx];
+        q.healthFresh = true;
+        q.changed = false;
+        total += run_interval<T>(P, S, sigmaTab[st.scen[idx]], st.id[idx], st.time[idx], q);
+        if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
+        if (q.changed) {
+            // store the position the context path stores (FP64 cell +
+            // offset, read back into cell + FP32 offset): an offset below the
+            // FP64 resolution of the coordinate becomes 0 there, and both
+            // paths must continue from the same state
+            requantize(q.jc, q.fj);
+            requantize(q.ic, q.fi);
+            st.cell[idx] = (uint32_t)q.jc * (uint32_t)P.xi + (uint32_t)q.ic;
+            st.fj[idx] = q.fj;
+            st.fi[idx] = q.fi;
+            st.k[idx] = q.k;
+            st.health[idx] = (float)q.health;
+            st.age[idx] = q.age;
+        } else if (k != k0 || (float)q.health != h0) {
+            st.k[idx] = k;
+            st.health[idx] = (float)q.health;
+        }
+    }
+    if (stepCounter) wave_add_u64(stepCounter, total);
+}
+
+// Per-scenario surface-layer counts [scenario][eta][xi] at the end of the
+// step's interval: particles released by then (emission time not after the
+// interval end in forward tracking: the reference creates a particle in the
+// interval of its emission), alive, with rounded level 0 (k > -0.5,
+// Concentration.hpp:31-36 rounds half away from zero), at their rounded
+// (j, i); wave-ballot grouping of equal cells, one atomic per group.
+__global__ __launch_bounds__(256) void ens_surface_kernel(const EnsStep *__restrict__ steps,
+                                                          const int *__restrict__ stepIdx, EnsSoA st, int n,
+                                                          int eta, int xi, uint32_t *__restrict__ counts) {
+    const MoveParams &P = steps[*stepIdx].P;
+    const double tEnd = P.intervalStart + P.dirInt * P.intervalLength;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = false;
+    uint32_t key = 0;
+    if (idx < n && st.health[idx] > 0.f && st.k[idx] > -0.5f && (P.dirInt < 0 || st.time[idx] <= tEnd)) {
+        const uint32_t cell = st.cell[idx];
+        const int jc = (int)(cell / (uint32_t)xi), ic = (int)(cell - (uint32_t)jc * (uint32_t)xi);
+        const int jr = jc + (st.fj[idx] >= 0.5f ? 1 : 0), ir = ic + (st.fi[idx] >= 0.5f ? 1 : 0);
+        if (jr < eta && ir < xi) {
+            key = ((uint32_t)st.scen[idx] * (uint32_t)eta + (uint32_t)jr) * (uint32_t)xi + (uint32_t)ir;
+            active = true;
+        }
+    }
+    for (;;) {
+        const unsigned long long m = __ballot(active);
+        if (m == 0) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const bool same = active && key == lk;
+        const unsigned long long sm = __ballot(same);
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&counts[lk], (uint32_t)__popcll(sm));
+        if (same) active = false;
+    }
+}
+
+__global__ void ens_advance_kernel(int *stepIdx) { stepIdx[0] += 1; }
 
 int grid_blocks(size_t n) {
     size_t b = (n + 255) / 256;
@@ -441,7 +624,69 @@ hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packB
     return hipGetLastError();
 }
 
+template <class T>
+Packed<T> packed_view(void *packBuf, const GridPtrs &G, int s_w, int xi, size_t plane) {
+    const int nlev = s_w - 1;
+    Packed<T> S;
+    char *base = (char *)packBuf;
+    S.uv = (const UV4<T> *)base;
+    base += sizeof(UV4<T>) * plane * nlev;
+    S.wa = (const WA8<T> *)base;
+    base += sizeof(WA8<T>) * plane * nlev;
+    S.z = (const ZH *)base;
+    S.md = (const float2 *)G.idxf;
+    S.depthf = G.depthf;
+    S.xi = xi;
+    S.nlev = nlev;
+    S.plane = plane;
+    return S;
+}
+
+template <class T>
+hipError_t ens_launch_t(const EnsLaunch &L, hipStream_t stream) {
+    const size_t plane = (size_t)L.eta * L.xi;
+    const Packed<T> S = packed_view<T>(L.packBuf, L.G, L.s_w, L.xi, plane);
+    ens_pack_kernel<T><<<grid_blocks(plane * (L.s_w - 1)), 256, 0, stream>>>(
+        L.steps, L.stepIdx, L.G.hf, (UV4<T> *)S.uv, (WA8<T> *)S.wa, (ZH *)S.z, L.s_rho, L.s_w, plane);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (L.n > 0) {
+        const long long per = (long long)L.block * L.ppt;
+        const int blocks = (int)((L.n + per - 1) / per);
+        if (L.occ == 4)
+            move_ens_kernel<T, 4><<<blocks, L.block, 0, stream>>>(L.steps, L.stepIdx, S, L.s, L.n, L.sigma, L.ppt,
+                                                                   L.stepCounter);
+        else if (L.occ == 3)
+            move_ens_kernel<T, 3><<<blocks, L.block, 0, stream>>>(L.steps, L.stepIdx, S, L.s, L.n, L.sigma, L.ppt,
+                                                                   L.stepCounter);
+        else
+            move_ens_kernel<T, 0><<<blocks, L.block, 0, stream>>>(L.steps, L.stepIdx, S, L.s, L.n, L.sigma, L.ppt,
+                                                                   L.stepCounter);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (L.surface) {
+        const size_t cells = (size_t)L.nScen * plane;
+        e = launch_fill_u32(L.surface, cells, 0u, stream);
+        if (e != hipSuccess) return e;
+        if (L.n > 0) {
+            ens_surface_kernel<<<(L.n + 255) / 256, 256, 0, stream>>>(L.steps, L.stepIdx, L.s, L.n, L.eta, L.xi,
+                                                                      L.surface);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
+    ens_advance_kernel<<<1, 1, 0, stream>>>(L.stepIdx);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream) {
+    if (L.block != 64 && L.block != 128 && L.block != 256) return hipErrorInvalidValue;
+    if (L.ppt < 1 || (L.occ != 0 && L.occ != 3 && L.occ != 4)) return hipErrorInvalidValue;
+    if (store == WC_STORE_F16) return ens_launch_t<_Float16>(L, stream);
+    if (store == WC_STORE_BF16) return ens_launch_t<uint16_t>(L, stream);
+    return ens_launch_t<float>(L, stream);
+}
 
 size_t fast_pack_bytes(int store, int s_w, size_t plane) {
     const size_t nlev = (size_t)(s_w - 1);

@@ -206,3 +206,33 @@ def release(window: ForcingWindow, n_particles: int, n_sources: int, depth_m: fl
         p["j"] += spread * rng.standard_normal(n_particles)
         p["i"] += spread * rng.standard_normal(n_particles)
     return p
+
+
+def ensemble(window: ForcingWindow, n_scenarios: int = 256, total: int = 16_000_000, median: float = 40_000,
+             lo: int = 10_000, hi: int = 500_000, kh_range=(0.1, 10.0), max_offset_h: float = 6.0,
+             dti: float = 30.0, depth_m: float = 1.0, seed: int = 5489, spread: float = 0.25):
+    """The batched-ensemble workload: n_scenarios release scenarios on one
+    grid and forcing.  Particle counts are lognormal (median `median`,
+    sigma 1) clipped to [lo, hi] and rescaled so they sum to about `total`;
+    each scenario has its own ocean source cell, Kh log-uniform in kh_range
+    (sigma = sqrt(2 Kh dti)) and a release offset uniform in [0,
+    max_offset_h] hours after the window start.  Particles of a scenario are
+    contiguous, ids 0..N-1, each offset by N(0, spread^2) cells.
+    Returns (particles, scenario uint8, sigma per scenario, counts)."""
+    from .types import make_particles
+    rng = np.random.default_rng(seed + 7)
+    counts = np.clip(rng.lognormal(math.log(median), 1.0, n_scenarios), lo, hi)
+    counts = np.clip(np.round(counts * total / counts.sum()), lo, hi).astype(np.int64)
+    kh = np.exp(rng.uniform(math.log(kh_range[0]), math.log(kh_range[1]), n_scenarios))
+    sigma = np.sqrt(2.0 * kh * dti)
+    offset = rng.uniform(0.0, max_offset_h * 3600.0, n_scenarios)
+    js, is_ = point_sources(window, n_scenarios, seed + 11)
+    n = int(counts.sum())
+    p = make_particles(n, time=0.0)
+    scen = np.repeat(np.arange(n_scenarios, dtype=np.uint8), counts)
+    h_src = float(window.h[window.mask > 0].mean())
+    p["k"] = depth_to_k(depth_m, h_src, window.s_w)
+    p["j"] = np.repeat(js, counts) + spread * rng.standard_normal(n)
+    p["i"] = np.repeat(is_, counts) + spread * rng.standard_normal(n)
+    p["time"] = float(window.times[0]) + np.repeat(offset, counts)
+    return p, scen, sigma, counts
