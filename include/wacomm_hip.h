@@ -411,6 +411,40 @@ int wc_ens_surface(wc_ctx *ctx, uint32_t *counts);
  * when scenario is not NULL, each particle's scenario. */
 int wc_ens_get(wc_ctx *ctx, wc_particle_data *particles, uint8_t *scenario, size_t n);
 
+/* ---------------------------------------------------------------------------
+ * 4. Learned concentration reconstruction (no counterpart in the reference:
+ *    a convolutional network that maps a low-particle-count surface
+ *    histogram to the high-count one, trained on pairs the method generates)
+ * ------------------------------------------------------------------------- */
+
+/* A 4-level U-Net with 32 base channels (32-64-128-256, skips by channel
+ * concatenation, 2x2 average pooling, nearest upsampling, ReLU) and a
+ * residual 1x1 head: out = x + head(dec1).  Forward and backward run on BF16
+ * matrix cores with FP32 accumulation; Adam updates FP32 master parameters.
+ * Parameters are FP32 in this order: per conv layer (enc1a, enc1b, enc2a,
+ * enc2b, enc3a, enc3b, bott_a, bott_b, dec3a, dec3b, dec2a, dec2b, dec1a,
+ * dec1b) weights [cout][3][3][cin] then biases [cout]; then the head's 32
+ * weights and its bias.  cin of enc1a is 32 (the input is channel 0); the
+ * decoder's first convs take concat(skip, upsampled).  He-normal
+ * initialisation from `seed`.  Images are [n][h][w] FP32 on the host, h and w
+ * multiples of 8. */
+typedef struct wc_unet wc_unet;
+int wc_unet_create(int device, uint64_t seed, wc_unet **out);
+int wc_unet_destroy(wc_unet *net);
+const char *wc_unet_last_error(const wc_unet *net);
+size_t wc_unet_param_count(const wc_unet *net);
+int wc_unet_get_params(wc_unet *net, float *params);
+/* Replaces the parameters and resets the optimiser state. */
+int wc_unet_set_params(wc_unet *net, const float *params);
+/* y = network(x); *device_ms (optional) = device time of the forward pass. */
+int wc_unet_forward(wc_unet *net, const float *x, int n, int h, int w, float *y, double *device_ms);
+/* Gradient of loss = mean (network(x) - t)^2 with respect to the parameters
+ * (grad may be NULL); no update. */
+int wc_unet_gradients(wc_unet *net, const float *x, const float *t, int n, int h, int w, float *grad, float *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at learning rate lr on
+ * that loss; *loss receives the loss before the step. */
+int wc_unet_train_step(wc_unet *net, const float *x, const float *t, int n, int h, int w, float lr, float *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,98 @@
+"""GPU tests of the concentration-reconstruction network (wc_unet_*): its
+forward and backward kernels (BF16 MFMA, FP32 accumulation) against the
+FP64 oracle (oracle/unet_oracle.py, pinned by finite differences), which
+rounds to BF16 where the device does.  The remaining difference is FP32 vs
+FP64 accumulation, which occasionally moves a value across a BF16 rounding
+boundary (one BF16 ulp is 0.4 %); the bars are relative L2 errors:
+forward < 5e-3, every layer's weight and bias gradient < 3e-2, loss < 1e-3.
+A layout or indexing mistake gives O(1) errors.
+"""
+import numpy as np
+import pytest
+
+import unet_oracle as uo
+import wacommplusplus_amd as wc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def net():
+    n = wc.UNet(0, seed=11)
+    yield n
+    n.close()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_param_layout(net):
+    assert net.n_params == uo.param_count()
+    layers, wh, bh = uo.unpack(net.get_params())
+    W0 = layers[0][0]
+    assert np.all(W0[..., 1:] == 0) and np.any(W0[..., 0] != 0)      # enc1a sees channel 0 only
+    assert all(np.all(b == 0) for _, b in layers)
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 16), (1, 24, 40)])
+def test_forward_and_gradients_match_oracle(net, shape):
+    rng = np.random.default_rng(sum(shape))
+    p = uo.he_params(seed=4).astype(np.float32)
+    net.set_params(p)
+    x = rng.uniform(0.0, 3.0, shape).astype(np.float32)
+    t = (x + rng.normal(0.0, 0.3, shape)).astype(np.float32)
+    ora = uo.UNetOracle(p.astype(np.float64), emulate_bf16=True)
+    y, ms = net.forward(x)
+    y_ref = ora.forward(x)
+    print(f"{shape}: forward rel err {rel(y, y_ref):.2e} (device {ms:.3f} ms)", flush=True)
+    assert rel(y, y_ref) < 5e-3
+    loss, g = net.gradients(x, t)
+    loss_ref, g_ref = ora.gradients(x, t)
+    assert abs(loss - loss_ref) / loss_ref < 1e-3, (loss, loss_ref)
+    off = 0
+    worst = 0.0
+    for l, (co, c1, c2) in enumerate(uo.SPEC):
+        nw = co * 9 * (c1 + c2)
+        ew = rel(g[off:off + nw], g_ref[off:off + nw])
+        eb = rel(g[off + nw:off + nw + co], g_ref[off + nw:off + nw + co])
+        worst = max(worst, ew, eb)
+        assert ew < 3e-2 and eb < 3e-2, (l, ew, eb)
+        off += nw + co
+    eh = rel(g[off:], g_ref[off:])
+    assert eh < 3e-2, eh
+    print(f"{shape}: worst layer gradient rel err {worst:.2e}, head {eh:.2e}", flush=True)
+
+
+def blobs(rng, n, h, w, k=3):
+    """Smooth positive fields (sums of Gaussians) and noisy copies of them."""
+    yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    clean = np.zeros((n, h, w))
+    for q in range(n):
+        for _ in range(k):
+            cy, cx, s = rng.uniform(4, h - 4), rng.uniform(4, w - 4), rng.uniform(2, 5)
+            clean[q] += rng.uniform(20, 200) * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * s * s))
+    noisy = rng.poisson(clean / 10.0) * 10.0
+    return np.sqrt(noisy).astype(np.float32), np.sqrt(clean).astype(np.float32)
+
+
+def test_training_reduces_loss(net):
+    net.set_params(uo.he_params(seed=7).astype(np.float32))
+    rng = np.random.default_rng(9)
+    x, t = blobs(rng, 64, 32, 32)
+    losses = []
+    for step in range(120):
+        b = rng.choice(64, 8, replace=False)
+        losses.append(net.train_step(x[b], t[b], 1e-3))
+    first, last = np.mean(losses[:10]), np.mean(losses[-10:])
+    print(f"training loss {first:.3f} -> {last:.3f}", flush=True)
+    assert last < 0.5 * first
+    # the trained network beats its input on held-out fields
+    xh, th = blobs(rng, 16, 32, 32)
+    y, _ = net.forward(xh)
+    assert np.linalg.norm(y - th) < 0.8 * np.linalg.norm(xh - th)
+
+
+def test_rejects_bad_shapes(net):
+    with pytest.raises(wc.WcError):
+        net.forward(np.zeros((1, 12, 16), np.float32))

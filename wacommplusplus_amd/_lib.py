@@ -28,7 +28,8 @@ EXPORTS = (
     "wc_version", "wc_cell_metric_table", "wc_philox4x32_10", "wc_emit", "wc_run_window",
     "wc_set_roms_grid", "wc_upload_roms_record", "wc_get_record", "wc_check_roms_grid", "wc_compare_records",
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
-    "wc_ens_get",
+    "wc_ens_get", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
+    "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
 )
 
 
@@ -138,6 +139,23 @@ def load_library(path: str = LIB_PATH):
     L.wc_cell_metric_table.restype = None
     L.wc_philox4x32_10.argtypes = [vp, vp, vp]
     L.wc_philox4x32_10.restype = None
+    unet = {
+        "wc_unet_create": [i, C.c_uint64, C.POINTER(vp)],
+        "wc_unet_destroy": [vp],
+        "wc_unet_get_params": [vp, vp],
+        "wc_unet_set_params": [vp, vp],
+        "wc_unet_forward": [vp, vp, i, i, i, vp, C.POINTER(d)],
+        "wc_unet_gradients": [vp, vp, vp, i, i, i, vp, C.POINTER(C.c_float)],
+        "wc_unet_train_step": [vp, vp, vp, i, i, i, C.c_float, C.POINTER(C.c_float)],
+    }
+    for name, args in unet.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.wc_unet_last_error.argtypes = [vp]
+    L.wc_unet_last_error.restype = C.c_char_p
+    L.wc_unet_param_count.argtypes = [vp]
+    L.wc_unet_param_count.restype = C.c_size_t
     _lib = L
     return L
 
@@ -420,3 +438,66 @@ def philox4x32_10(ctr, key):
     out = np.zeros(4, np.uint32)
     L.wc_philox4x32_10(_p(c), _p(k), _p(out))
     return out
+
+
+class UNet:
+    """The concentration-reconstruction network (include/wacomm_hip.h,
+    section 4) on one device: images are [n][h][w] float32 arrays."""
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self.L = load_library()
+        h = vp()
+        rc = self.L.wc_unet_create(device, seed, C.byref(h))
+        if rc:
+            raise WcError(rc, f"wc_unet_create(device={device})")
+        self.h = h
+        self.n_params = self.L.wc_unet_param_count(h)
+
+    def _chk(self, rc):
+        if rc:
+            raise WcError(rc, self.L.wc_unet_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.wc_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self._chk(self.L.wc_unet_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, params):
+        p = _f32(params)
+        assert p.size == self.n_params
+        self._chk(self.L.wc_unet_set_params(self.h, _p(p)))
+
+    def forward(self, x):
+        """(y, device ms of the forward pass)."""
+        xx = _f32(x)
+        n, hh, ww = xx.shape
+        y = np.zeros_like(xx)
+        ms = C.c_double()
+        self._chk(self.L.wc_unet_forward(self.h, _p(xx), n, hh, ww, _p(y), C.byref(ms)))
+        return y, ms.value
+
+    def gradients(self, x, t):
+        xx, tt = _f32(x), _f32(t)
+        n, hh, ww = xx.shape
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        self._chk(self.L.wc_unet_gradients(self.h, _p(xx), _p(tt), n, hh, ww, _p(g), C.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, x, t, lr):
+        xx, tt = _f32(x), _f32(t)
+        n, hh, ww = xx.shape
+        loss = C.c_float()
+        self._chk(self.L.wc_unet_train_step(self.h, _p(xx), _p(tt), n, hh, ww, float(lr), C.byref(loss)))
+        return loss.value

@@ -88,4 +88,54 @@ struct EnsLaunch {
 };
 hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream);
 
+// ---- reconstruction network (wc_unet.hip) -----------------------------------
+// 3x3 convolution as an implicit GEMM on BF16 MFMA.  Inputs x1 (c1 channels)
+// and x2 (c2, may be 0) are NHWC, concatenated along channels, BF16 (or FP32
+// when fp32_input, converted on load); m1 (optional, BF16, shaped as x1)
+// masks x1 where m1 <= 0.  w_ = [nout][3][3][c1+c2] BF16, nout = cy1 + cy2.
+// mode 0: y1 = BF16 relu(conv + bias) (cy2 = 0); mode 1: FP32 output
+// channels [0, cy1) to y1 and [cy1, nout) to y2, each stored or accumulated.
+struct ConvArgs {
+    const void *x1;
+    const void *x2;
+    const uint16_t *m1;
+    int c1, c2;
+    const uint16_t *w_;
+    const float *bias;
+    void *y1;
+    void *y2;
+    int cy1, cy2;
+    bool acc1, acc2;
+    int mode;
+    int n, h, w;
+};
+// dW[co][tap][ci] += sum_p dY[p][co] * (Y[p][co] > 0) * X[p + tap][ci]
+struct WgradArgs {
+    const float *dy;
+    const uint16_t *y;
+    const uint16_t *x1;
+    const uint16_t *x2;
+    int c1, c2, cout;
+    float *dw;
+    int n, h, w;
+    int splits;
+};
+hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s);
+hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s);
+hipError_t launch_bias_grad(const float *dy, const uint16_t *y, long long P, int cout, float *db, hipStream_t s);
+hipError_t launch_pool_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
+hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, hipStream_t s);
+hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
+hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx, hipStream_t s);
+hipError_t launch_unet_input(const float *x, long long P, int c, uint16_t *y, hipStream_t s);
+hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, long long P, float *out,
+                           hipStream_t s);
+hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wh, long long P,
+                           float *dd, float *dwh, float *loss, hipStream_t s);
+hipError_t launch_flip(const uint16_t *w, int cout, int cin, uint16_t *wf, hipStream_t s);
+hipError_t launch_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,
+                       float eps, int t, hipStream_t s);
+hipError_t launch_to_bf16(const float *x, long long n, uint16_t *y, hipStream_t s);
+hipError_t launch_fill_f32(float *a, long long n, float v, hipStream_t s);
+
 }  // namespace wc

@@ -1,0 +1,432 @@
+// C ABI of the concentration-reconstruction network (include/wacomm_hip.h,
+// section 4): parameters, forward, gradients and Adam steps of a 4-level
+// U-Net with 32 base channels, run by the kernels of wc_unet.hip.
+//
+// Network (channels; L0 = full resolution, Lk = 1/2^k):
+//   in   = x as channel 0 of 32 (the rest zero)                  L0
+//   enc1 = relu(conv(relu(conv(in))))            32              L0  (skip s1)
+//   enc2 = 2 convs on avgpool(s1)                64              L1  (skip s2)
+//   enc3 = 2 convs on avgpool(s2)                128             L2  (skip s3)
+//   bott = 2 convs on avgpool(s3)                256             L3
+//   dec3 = 2 convs on concat(s3, up(bott))       384 -> 128      L2
+//   dec2 = 2 convs on concat(s2, up(dec3))       192 -> 64       L1
+//   dec1 = 2 convs on concat(s1, up(dec2))       96 -> 32        L0
+//   out  = x + head . dec1 + b_head (1x1 conv, residual)
+// Parameters, FP32, in this order: per conv layer w[cout][3][3][cin] then
+// b[cout]; then the head's 32 weights and its bias.  The loss is the mean
+// squared error of out against the target.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "wc_launch.h"
+
+using namespace wc;
+
+namespace {
+
+struct Layer {
+    int cout, c1, c2;        // input channels c1 (+ c2 concatenated)
+    size_t w_off, b_off;     // offsets into the FP32 parameters
+    size_t wb_off;           // offset into the BF16 weight copy
+};
+
+constexpr int kBase = 32;
+
+}  // namespace
+
+struct wc_unet {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<Layer> layers;
+    size_t nparam = 0, nwb = 0, head_off = 0;
+    float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
+    uint16_t *d_wb = nullptr, *d_wf = nullptr;     // BF16 weights, flipped scratch
+    float *d_loss = nullptr;
+    int adam_t = 0;
+    double fwd_ms = 0;
+    // activations / gradients for the current shape
+    int n = 0, h = 0, w = 0;
+    std::vector<void *> bufs;
+    uint16_t *in0 = nullptr, *a1 = nullptr, *s1 = nullptr, *p1 = nullptr, *a2 = nullptr, *s2 = nullptr,
+             *p2 = nullptr, *a3 = nullptr, *s3 = nullptr, *p3 = nullptr, *a4 = nullptr, *b4 = nullptr,
+             *u3 = nullptr, *d3a = nullptr, *d3 = nullptr, *u2 = nullptr, *d2a = nullptr, *d2 = nullptr,
+             *u1 = nullptr, *d1a = nullptr, *d1 = nullptr;
+    float *x = nullptr, *out = nullptr, *tgt = nullptr;
+    float *g_d1 = nullptr, *g_d1a = nullptr, *g_s1 = nullptr, *g_u1 = nullptr, *g_d2 = nullptr, *g_d2a = nullptr,
+          *g_s2 = nullptr, *g_u2 = nullptr, *g_d3 = nullptr, *g_d3a = nullptr, *g_s3 = nullptr, *g_u3 = nullptr,
+          *g_b4 = nullptr, *g_a4 = nullptr, *g_p3 = nullptr, *g_a3 = nullptr, *g_p2 = nullptr, *g_a2 = nullptr,
+          *g_p1 = nullptr, *g_a1 = nullptr;
+};
+
+namespace {
+
+int ufail(wc_unet *u, int code, const char *fmt, ...) {
+    if (u) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        u->err = buf;
+    }
+    return code;
+}
+
+#define UCHK(u, call)                                                                                \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return ufail(u, e_ == hipErrorOutOfMemory ? WC_ERR_NOMEM : WC_ERR_HIP, "%s: %s (%s:%d)", #call, \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                                 \
+    } while (0)
+
+void free_shape(wc_unet *u) {
+    for (void *b : u->bufs) hipFree(b);
+    u->bufs.clear();
+    u->n = u->h = u->w = 0;
+}
+
+template <class T>
+int alloc(wc_unet *u, T *&p, size_t count) {
+    void *q = nullptr;
+    UCHK(u, hipMalloc(&q, count * sizeof(T) + 256));
+    u->bufs.push_back(q);
+    p = (T *)q;
+    return WC_OK;
+}
+
+int ensure_shape(wc_unet *u, int n, int h, int w) {
+    if (n == u->n && h == u->h && w == u->w) return WC_OK;
+    if (n < 1 || h < 8 || w < 8 || h % 8 || w % 8)
+        return ufail(u, WC_ERR_INVALID, "images must be at least 8x8 with sides divisible by 8 (got %dx%d)", h, w);
+    if ((long long)n * h * w * 384 > (1LL << 40)) return ufail(u, WC_ERR_INVALID, "batch too large");
+    UCHK(u, hipStreamSynchronize(u->stream));
+    free_shape(u);
+    const size_t P0 = (size_t)n * h * w, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
+    const int C = kBase;
+    int rc = 0;
+    rc |= alloc(u, u->in0, P0 * C) | alloc(u, u->a1, P0 * C) | alloc(u, u->s1, P0 * C) | alloc(u, u->p1, P1 * C);
+    rc |= alloc(u, u->a2, P1 * 2 * C) | alloc(u, u->s2, P1 * 2 * C) | alloc(u, u->p2, P2 * 2 * C);
+    rc |= alloc(u, u->a3, P2 * 4 * C) | alloc(u, u->s3, P2 * 4 * C) | alloc(u, u->p3, P3 * 4 * C);
+    rc |= alloc(u, u->a4, P3 * 8 * C) | alloc(u, u->b4, P3 * 8 * C) | alloc(u, u->u3, P2 * 8 * C);
+    rc |= alloc(u, u->d3a, P2 * 4 * C) | alloc(u, u->d3, P2 * 4 * C) | alloc(u, u->u2, P1 * 4 * C);
+    rc |= alloc(u, u->d2a, P1 * 2 * C) | alloc(u, u->d2, P1 * 2 * C) | alloc(u, u->u1, P0 * 2 * C);
+    rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C);
+    rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0);
+    rc |= alloc(u, u->g_d1, P0 * C) | alloc(u, u->g_d1a, P0 * C) | alloc(u, u->g_s1, P0 * C) |
+          alloc(u, u->g_u1, P0 * 2 * C);
+    rc |= alloc(u, u->g_d2, P1 * 2 * C) | alloc(u, u->g_d2a, P1 * 2 * C) | alloc(u, u->g_s2, P1 * 2 * C) |
+          alloc(u, u->g_u2, P1 * 4 * C);
+    rc |= alloc(u, u->g_d3, P2 * 4 * C) | alloc(u, u->g_d3a, P2 * 4 * C) | alloc(u, u->g_s3, P2 * 4 * C) |
+          alloc(u, u->g_u3, P2 * 8 * C);
+    rc |= alloc(u, u->g_b4, P3 * 8 * C) | alloc(u, u->g_a4, P3 * 8 * C) | alloc(u, u->g_p3, P3 * 4 * C);
+    rc |= alloc(u, u->g_a3, P2 * 4 * C) | alloc(u, u->g_p2, P2 * 2 * C) | alloc(u, u->g_a2, P1 * 2 * C);
+    rc |= alloc(u, u->g_p1, P1 * C) | alloc(u, u->g_a1, P0 * C);
+    if (rc) {
+        free_shape(u);
+        return rc;
+    }
+    u->n = n; u->h = h; u->w = w;
+    return WC_OK;
+}
+
+// forward conv of layer L: y = relu(conv(concat(x1, x2)) + b)
+int conv_fwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2, uint16_t *y, int h, int w) {
+    ConvArgs a{};
+    a.x1 = x1; a.x2 = x2; a.m1 = nullptr; a.c1 = L.c1; a.c2 = L.c2;
+    a.w_ = u->d_wb + L.wb_off; a.bias = u->d_p + L.b_off;
+    a.y1 = y; a.y2 = nullptr; a.cy1 = L.cout; a.cy2 = 0; a.acc1 = a.acc2 = false; a.mode = 0;
+    a.n = u->n; a.h = h; a.w = w;
+    UCHK(u, launch_conv3x3(a, false, u->stream));
+    return WC_OK;
+}
+
+// backward of layer L, whose output y = relu(...) received gradient gy:
+// weight and bias gradients; when gx1 != null, the input gradients
+// (channels [0, c1) to gx1, [c1, c1+c2) to gx2), stored or accumulated.
+int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, const uint16_t *x1, const uint16_t *x2,
+             float *gx1, bool acc1, float *gx2, bool acc2, int h, int w) {
+    const long long P = (long long)u->n * h * w;
+    WgradArgs g{};
+    g.dy = gy; g.y = y; g.x1 = x1; g.x2 = x2; g.c1 = L.c1; g.c2 = L.c2; g.cout = L.cout;
+    g.dw = u->d_g + L.w_off; g.n = u->n; g.h = h; g.w = w;
+    long long sp = P / 2048;
+    g.splits = (int)(sp < 1 ? 1 : (sp > 1024 ? 1024 : sp));
+    UCHK(u, launch_wgrad(g, u->stream));
+    UCHK(u, launch_bias_grad(gy, y, P, L.cout, u->d_g + L.b_off, u->stream));
+    if (!gx1) return WC_OK;
+    const int cin = L.c1 + L.c2;
+    UCHK(u, launch_flip(u->d_wb + L.wb_off, L.cout, cin, u->d_wf, u->stream));
+    ConvArgs a{};
+    a.x1 = gy; a.x2 = nullptr; a.m1 = y; a.c1 = L.cout; a.c2 = 0;
+    a.w_ = u->d_wf; a.bias = nullptr;
+    a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
+    a.n = u->n; a.h = h; a.w = w;
+    UCHK(u, launch_conv3x3(a, true, u->stream));
+    return WC_OK;
+}
+
+int forward(wc_unet *u) {
+    const int H = u->h, W = u->w, C = kBase, n = u->n;
+    const auto &Ls = u->layers;
+    int rc;
+    UCHK(u, launch_unet_input(u->x, (long long)n * H * W, C, u->in0, u->stream));
+    if ((rc = conv_fwd(u, Ls[0], u->in0, nullptr, u->a1, H, W)) || (rc = conv_fwd(u, Ls[1], u->a1, nullptr, u->s1, H, W)))
+        return rc;
+    UCHK(u, launch_pool_fwd(u->s1, n, H, W, C, u->p1, u->stream));
+    if ((rc = conv_fwd(u, Ls[2], u->p1, nullptr, u->a2, H / 2, W / 2)) ||
+        (rc = conv_fwd(u, Ls[3], u->a2, nullptr, u->s2, H / 2, W / 2)))
+        return rc;
+    UCHK(u, launch_pool_fwd(u->s2, n, H / 2, W / 2, 2 * C, u->p2, u->stream));
+    if ((rc = conv_fwd(u, Ls[4], u->p2, nullptr, u->a3, H / 4, W / 4)) ||
+        (rc = conv_fwd(u, Ls[5], u->a3, nullptr, u->s3, H / 4, W / 4)))
+        return rc;
+    UCHK(u, launch_pool_fwd(u->s3, n, H / 4, W / 4, 4 * C, u->p3, u->stream));
+    if ((rc = conv_fwd(u, Ls[6], u->p3, nullptr, u->a4, H / 8, W / 8)) ||
+        (rc = conv_fwd(u, Ls[7], u->a4, nullptr, u->b4, H / 8, W / 8)))
+        return rc;
+    UCHK(u, launch_up_fwd(u->b4, n, H / 8, W / 8, 8 * C, u->u3, u->stream));
+    if ((rc = conv_fwd(u, Ls[8], u->s3, u->u3, u->d3a, H / 4, W / 4)) ||
+        (rc = conv_fwd(u, Ls[9], u->d3a, nullptr, u->d3, H / 4, W / 4)))
+        return rc;
+    UCHK(u, launch_up_fwd(u->d3, n, H / 4, W / 4, 4 * C, u->u2, u->stream));
+    if ((rc = conv_fwd(u, Ls[10], u->s2, u->u2, u->d2a, H / 2, W / 2)) ||
+        (rc = conv_fwd(u, Ls[11], u->d2a, nullptr, u->d2, H / 2, W / 2)))
+        return rc;
+    UCHK(u, launch_up_fwd(u->d2, n, H / 2, W / 2, 2 * C, u->u1, u->stream));
+    if ((rc = conv_fwd(u, Ls[12], u->s1, u->u1, u->d1a, H, W)) || (rc = conv_fwd(u, Ls[13], u->d1a, nullptr, u->d1, H, W)))
+        return rc;
+    UCHK(u, launch_head_fwd(u->d1, u->x, u->d_p + u->head_off, (long long)n * H * W, u->out, u->stream));
+    return WC_OK;
+}
+
+// gradients of the mean squared error into d_g (zeroed first); loss in d_loss
+int backward(wc_unet *u) {
+    const int H = u->h, W = u->w, C = kBase, n = u->n;
+    const auto &Ls = u->layers;
+    int rc;
+    UCHK(u, launch_fill_f32(u->d_g, (long long)u->nparam, 0.f, u->stream));
+    UCHK(u, launch_fill_f32(u->d_loss, 1, 0.f, u->stream));
+    UCHK(u, launch_head_bwd(u->d1, u->out, u->tgt, u->d_p + u->head_off, (long long)n * H * W, u->g_d1,
+                            u->d_g + u->head_off, u->d_loss, u->stream));
+    // decoder, level 0
+    if ((rc = conv_bwd(u, Ls[13], u->g_d1, u->d1, u->d1a, nullptr, u->g_d1a, false, nullptr, false, H, W))) return rc;
+    if ((rc = conv_bwd(u, Ls[12], u->g_d1a, u->d1a, u->s1, u->u1, u->g_s1, false, u->g_u1, false, H, W))) return rc;
+    UCHK(u, launch_up_bwd(u->g_u1, n, H / 2, W / 2, 2 * C, u->g_d2, u->stream));
+    // level 1
+    if ((rc = conv_bwd(u, Ls[11], u->g_d2, u->d2, u->d2a, nullptr, u->g_d2a, false, nullptr, false, H / 2, W / 2)))
+        return rc;
+    if ((rc = conv_bwd(u, Ls[10], u->g_d2a, u->d2a, u->s2, u->u2, u->g_s2, false, u->g_u2, false, H / 2, W / 2)))
+        return rc;
+    UCHK(u, launch_up_bwd(u->g_u2, n, H / 4, W / 4, 4 * C, u->g_d3, u->stream));
+    // level 2
+    if ((rc = conv_bwd(u, Ls[9], u->g_d3, u->d3, u->d3a, nullptr, u->g_d3a, false, nullptr, false, H / 4, W / 4)))
+        return rc;
+    if ((rc = conv_bwd(u, Ls[8], u->g_d3a, u->d3a, u->s3, u->u3, u->g_s3, false, u->g_u3, false, H / 4, W / 4)))
+        return rc;
+    UCHK(u, launch_up_bwd(u->g_u3, n, H / 8, W / 8, 8 * C, u->g_b4, u->stream));
+    // bottleneck
+    if ((rc = conv_bwd(u, Ls[7], u->g_b4, u->b4, u->a4, nullptr, u->g_a4, false, nullptr, false, H / 8, W / 8)))
+        return rc;
+    if ((rc = conv_bwd(u, Ls[6], u->g_a4, u->a4, u->p3, nullptr, u->g_p3, false, nullptr, false, H / 8, W / 8)))
+        return rc;
+    UCHK(u, launch_pool_bwd(u->g_p3, n, H / 4, W / 4, 4 * C, u->g_s3, true, u->stream));
+    // encoder
+    if ((rc = conv_bwd(u, Ls[5], u->g_s3, u->s3, u->a3, nullptr, u->g_a3, false, nullptr, false, H / 4, W / 4)))
+        return rc;
+    if ((rc = conv_bwd(u, Ls[4], u->g_a3, u->a3, u->p2, nullptr, u->g_p2, false, nullptr, false, H / 4, W / 4)))
+        return rc;
+    UCHK(u, launch_pool_bwd(u->g_p2, n, H / 2, W / 2, 2 * C, u->g_s2, true, u->stream));
+    if ((rc = conv_bwd(u, Ls[3], u->g_s2, u->s2, u->a2, nullptr, u->g_a2, false, nullptr, false, H / 2, W / 2)))
+        return rc;
+    if ((rc = conv_bwd(u, Ls[2], u->g_a2, u->a2, u->p1, nullptr, u->g_p1, false, nullptr, false, H / 2, W / 2)))
+        return rc;
+    UCHK(u, launch_pool_bwd(u->g_p1, n, H, W, C, u->g_s1, true, u->stream));
+    if ((rc = conv_bwd(u, Ls[1], u->g_s1, u->s1, u->a1, nullptr, u->g_a1, false, nullptr, false, H, W))) return rc;
+    if ((rc = conv_bwd(u, Ls[0], u->g_a1, u->a1, u->in0, nullptr, nullptr, false, nullptr, false, H, W))) return rc;
+    return WC_OK;
+}
+
+int upload(wc_unet *u, const float *x, const float *t, int n, int h, int w) {
+    int rc = ensure_shape(u, n, h, w);
+    if (rc) return rc;
+    const size_t P = (size_t)n * h * w;
+    UCHK(u, hipMemcpyAsync(u->x, x, P * 4, hipMemcpyHostToDevice, u->stream));
+    if (t) UCHK(u, hipMemcpyAsync(u->tgt, t, P * 4, hipMemcpyHostToDevice, u->stream));
+    return WC_OK;
+}
+
+int refresh_bf16(wc_unet *u) {
+    for (const Layer &L : u->layers)
+        UCHK(u, launch_to_bf16(u->d_p + L.w_off, (long long)L.cout * 9 * (L.c1 + L.c2), u->d_wb + L.wb_off, u->stream));
+    return WC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wc_unet_create(int device, uint64_t seed, wc_unet **out) {
+    if (!out) return WC_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return WC_ERR_NO_DEVICE;
+    auto *u = new wc_unet();
+    u->device = device;
+    const int C = kBase;
+    const int spec[14][3] = {{C, C, 0},          {C, C, 0},         {2 * C, C, 0},     {2 * C, 2 * C, 0},
+                             {4 * C, 2 * C, 0},  {4 * C, 4 * C, 0}, {8 * C, 4 * C, 0}, {8 * C, 8 * C, 0},
+                             {4 * C, 4 * C, 8 * C}, {4 * C, 4 * C, 0}, {2 * C, 2 * C, 4 * C}, {2 * C, 2 * C, 0},
+                             {C, C, 2 * C},      {C, C, 0}};
+    size_t off = 0, woff = 0, maxw = 0;
+    for (auto &s : spec) {
+        Layer L{s[0], s[1], s[2], 0, 0, 0};
+        const size_t nw = (size_t)L.cout * 9 * (L.c1 + L.c2);
+        L.w_off = off;
+        off += nw;
+        L.b_off = off;
+        off += L.cout;
+        L.wb_off = woff;
+        woff += nw;
+        if (nw > maxw) maxw = nw;
+        u->layers.push_back(L);
+    }
+    u->head_off = off;
+    off += C + 1;
+    u->nparam = off;
+    u->nwb = woff;
+    // He-normal initialisation (fan-in of the real inputs; the first layer
+    // sees one real channel), zero biases, small head weights
+    std::vector<float> p(u->nparam, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (size_t l = 0; l < u->layers.size(); ++l) {
+        const Layer &L = u->layers[l];
+        const int cin = L.c1 + L.c2, fan = 9 * (l == 0 ? 1 : cin);
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / fan));
+        for (int co = 0; co < L.cout; ++co)
+            for (int t = 0; t < 9; ++t)
+                for (int ci = 0; ci < cin; ++ci)
+                    p[L.w_off + ((size_t)co * 9 + t) * cin + ci] = (l == 0 && ci > 0) ? 0.0f : nd(rng);
+    }
+    std::normal_distribution<float> hd(0.0f, 0.01f);
+    for (int c = 0; c < C; ++c) p[u->head_off + c] = hd(rng);
+    auto bad = [&](int code) {
+        wc_unet_destroy(u);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&u->stream) != hipSuccess) return bad(WC_ERR_HIP);
+    if (hipMalloc(&u->d_p, u->nparam * 4) || hipMalloc(&u->d_g, u->nparam * 4) || hipMalloc(&u->d_m, u->nparam * 4) ||
+        hipMalloc(&u->d_v, u->nparam * 4) || hipMalloc(&u->d_wb, u->nwb * 2 + 256) ||
+        hipMalloc(&u->d_wf, maxw * 2 + 256) || hipMalloc(&u->d_loss, 256))
+        return bad(WC_ERR_NOMEM);
+    if (hipMemcpy(u->d_p, p.data(), u->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return bad(WC_ERR_HIP);
+    if (launch_fill_f32(u->d_m, (long long)u->nparam, 0.f, u->stream) != hipSuccess ||
+        launch_fill_f32(u->d_v, (long long)u->nparam, 0.f, u->stream) != hipSuccess || refresh_bf16(u) ||
+        hipStreamSynchronize(u->stream) != hipSuccess)
+        return bad(WC_ERR_HIP);
+    *out = u;
+    return WC_OK;
+}
+
+int wc_unet_destroy(wc_unet *u) {
+    if (!u) return WC_OK;
+    hipSetDevice(u->device);
+    if (u->stream) hipStreamSynchronize(u->stream);
+    free_shape(u);
+    hipFree(u->d_p); hipFree(u->d_g); hipFree(u->d_m); hipFree(u->d_v); hipFree(u->d_wb); hipFree(u->d_wf);
+    hipFree(u->d_loss);
+    if (u->stream) hipStreamDestroy(u->stream);
+    delete u;
+    return WC_OK;
+}
+
+const char *wc_unet_last_error(const wc_unet *u) { return u ? u->err.c_str() : "null network"; }
+
+size_t wc_unet_param_count(const wc_unet *u) { return u ? u->nparam : 0; }
+
+int wc_unet_get_params(wc_unet *u, float *p) {
+    if (!u || !p) return WC_ERR_INVALID;
+    UCHK(u, hipSetDevice(u->device));
+    UCHK(u, hipMemcpyAsync(p, u->d_p, u->nparam * 4, hipMemcpyDeviceToHost, u->stream));
+    UCHK(u, hipStreamSynchronize(u->stream));
+    return WC_OK;
+}
+
+int wc_unet_set_params(wc_unet *u, const float *p) {
+    if (!u || !p) return WC_ERR_INVALID;
+    UCHK(u, hipSetDevice(u->device));
+    UCHK(u, hipMemcpyAsync(u->d_p, p, u->nparam * 4, hipMemcpyHostToDevice, u->stream));
+    int rc = refresh_bf16(u);
+    if (rc) return rc;
+    UCHK(u, launch_fill_f32(u->d_m, (long long)u->nparam, 0.f, u->stream));
+    UCHK(u, launch_fill_f32(u->d_v, (long long)u->nparam, 0.f, u->stream));
+    u->adam_t = 0;
+    UCHK(u, hipStreamSynchronize(u->stream));
+    return WC_OK;
+}
+
+int wc_unet_forward(wc_unet *u, const float *x, int n, int h, int w, float *y, double *device_ms) {
+    if (!u || !x || !y) return WC_ERR_INVALID;
+    UCHK(u, hipSetDevice(u->device));
+    int rc = upload(u, x, nullptr, n, h, w);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    UCHK(u, hipEventCreate(&e0));
+    UCHK(u, hipEventCreate(&e1));
+    UCHK(u, hipEventRecord(e0, u->stream));
+    rc = forward(u);
+    if (!rc) {
+        UCHK(u, hipEventRecord(e1, u->stream));
+        UCHK(u, hipMemcpyAsync(y, u->out, (size_t)n * h * w * 4, hipMemcpyDeviceToHost, u->stream));
+        UCHK(u, hipStreamSynchronize(u->stream));
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        u->fwd_ms = ms;
+        if (device_ms) *device_ms = ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int wc_unet_gradients(wc_unet *u, const float *x, const float *t, int n, int h, int w, float *grad, float *loss) {
+    if (!u || !x || !t) return WC_ERR_INVALID;
+    UCHK(u, hipSetDevice(u->device));
+    int rc = upload(u, x, t, n, h, w);
+    if (!rc) rc = forward(u);
+    if (!rc) rc = backward(u);
+    if (rc) return rc;
+    if (grad) UCHK(u, hipMemcpyAsync(grad, u->d_g, u->nparam * 4, hipMemcpyDeviceToHost, u->stream));
+    float l = 0;
+    UCHK(u, hipMemcpyAsync(&l, u->d_loss, 4, hipMemcpyDeviceToHost, u->stream));
+    UCHK(u, hipStreamSynchronize(u->stream));
+    if (loss) *loss = l;
+    return WC_OK;
+}
+
+int wc_unet_train_step(wc_unet *u, const float *x, const float *t, int n, int h, int w, float lr, float *loss) {
+    if (!u || !x || !t || !(lr > 0)) return WC_ERR_INVALID;
+    UCHK(u, hipSetDevice(u->device));
+    int rc = upload(u, x, t, n, h, w);
+    if (!rc) rc = forward(u);
+    if (!rc) rc = backward(u);
+    if (rc) return rc;
+    u->adam_t += 1;
+    UCHK(u, launch_adam(u->d_p, u->d_g, u->d_m, u->d_v, (long long)u->nparam, lr, 0.9f, 0.999f, 1e-8f, u->adam_t,
+                        u->stream));
+    rc = refresh_bf16(u);
+    if (rc) return rc;
+    float l = 0;
+    UCHK(u, hipMemcpyAsync(&l, u->d_loss, 4, hipMemcpyDeviceToHost, u->stream));
+    UCHK(u, hipStreamSynchronize(u->stream));
+    if (loss) *loss = l;
+    return WC_OK;
+}
+
+}  // extern "C"
