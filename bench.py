@@ -199,6 +199,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-reconstruction workload")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the precision comparison (its diffusion-off runs launch the move kernels "
                          "on other populations, which a kernel-trace average would mix in)")
@@ -294,6 +295,15 @@ def main():
     roms = roms_normalize_bench(wc, ctx, W, args.steps)
     log(f"[roms] {roms}")
     ens = None if args.no_ensemble else ensemble_bench(wc, ctx, W, cfg, args.steps)
+    learned = None
+    if not args.no_learned:
+        from wacommplusplus_amd import reconstruct
+        log("[learned] 512 training + 128 held-out pairs (1M-particle targets, 50k inputs, 24 h)")
+        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=W.T, store="bf16")
+        for t in range(W.T):
+            ctx.upload_record(t, *W.record(t))
+        learned = reconstruct.run(ctx, W, n_train=512, n_test=128, epochs=40, log=log)
+        log(f"[learned] {learned['held_out']} {learned['equal_gpu_ms']}")
     ctx.close()
 
     main_r = results["f64"]
@@ -359,6 +369,7 @@ def main():
         "pcie_inclusive_value": pcie,
         "roms_normalize": roms,
         "ensemble": ens,
+        "learned_reconstruction": learned,
         "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),
         "alive_after": main_r["alive"],
     }

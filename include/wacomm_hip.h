@@ -372,6 +372,25 @@ void wc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 int wc_ens_set(wc_ctx *ctx, const wc_particle_data *particles, const uint8_t *scenario, size_t n,
                const double *sigma, int n_scenarios);
 
+/* A release point of a device-generated ensemble population. */
+typedef struct wc_ens_source {
+    double k, j, i;          /* release point (fractional indices) */
+    double spread;           /* std dev, in cells, of each particle's offset in j and i */
+    double time;             /* emission time (s) */
+    uint64_t count;          /* particles */
+    int scenario;            /* scenario of the particles */
+    int sub_scenario;        /* scenario of every sub_every-th particle of this source, or -1 */
+    uint32_t sub_every;      /* 0: no subset */
+} wc_ens_source;
+
+/* As wc_ens_set, but the population is generated on the device: particle q
+ * (id q, sources in order) starts at its source's point offset by spread *
+ * N(0, 1) in j and i (Philox4x32-10 keyed on seed and q); every sub_every-th
+ * particle of a source belongs to sub_scenario (an id-subsample that moves
+ * exactly as in the full population, since every draw is keyed by id). */
+int wc_ens_release(wc_ctx *ctx, const wc_ens_source *sources, int n_sources, const double *sigma, int n_scenarios,
+                   uint64_t seed);
+
 /* Launch shape of the ensemble move: block in {64, 128, 256} work-items,
  * ppt particles per work-item (1..64), occupancy hint occ in {0 (none),
  * 3, 4} waves per SIMD.  wc_ens_autotune picks them by timing. */

@@ -9,7 +9,7 @@ loudly when the library has not been built.
 from .types import (BACKWARD, CONSTRAINT, EULER, FORWARD, KILL, MIDPOINT, PARTICLE_DTYPE, REFLECTION,
                     ConfigData, default_config, make_particles, sigma_from_kh)
 from ._lib import (EXPORTS, FORCING_INTERVAL_BATCH, LIB_PATH, ROMS_MAX_MISSING, SINGLE_PULSE, UNIFORM_RATE,
-                   WC_ERR_HIP, WC_ERR_INVALID, WC_ERR_NO_DEVICE, WC_ERR_NOMEM, WC_ERR_UNSUPPORTED, WC_OK, Context, EnsStats,
+                   WC_ERR_HIP, WC_ERR_INVALID, WC_ERR_NO_DEVICE, WC_ERR_NOMEM, WC_ERR_UNSUPPORTED, WC_OK, Context, EnsSource, EnsStats,
                    RunStats, Source, UNet, WcError, cell_metric_table, load_library, philox4x32_10)
 
 __all__ = [
@@ -17,5 +17,5 @@ __all__ = [
     "ConfigData", "default_config", "make_particles", "sigma_from_kh", "EXPORTS", "LIB_PATH", "Context",
     "WcError", "cell_metric_table", "load_library", "philox4x32_10", "Source", "RunStats", "UNIFORM_RATE",
     "FORCING_INTERVAL_BATCH", "SINGLE_PULSE", "ROMS_MAX_MISSING", "WC_OK", "WC_ERR_HIP", "WC_ERR_INVALID",
-    "WC_ERR_UNSUPPORTED", "WC_ERR_NOMEM", "WC_ERR_NO_DEVICE", "EnsStats", "UNet",
+    "WC_ERR_UNSUPPORTED", "WC_ERR_NOMEM", "WC_ERR_NO_DEVICE", "EnsStats", "EnsSource", "UNet",
 ]

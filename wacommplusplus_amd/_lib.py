@@ -28,7 +28,7 @@ EXPORTS = (
     "wc_version", "wc_cell_metric_table", "wc_philox4x32_10", "wc_emit", "wc_run_window",
     "wc_set_roms_grid", "wc_upload_roms_record", "wc_get_record", "wc_check_roms_grid", "wc_compare_records",
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
-    "wc_ens_get", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
+    "wc_ens_get", "wc_ens_release", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
 )
 
@@ -64,6 +64,13 @@ class RomsGridDesc(C.Structure):
                [(n, C.c_void_p) for n in ("missing_u", "missing_v", "missing_angle")] + \
                [(n, C.c_int) for n in ("n_missing_u", "n_missing_v", "n_missing_angle")] + \
                [("s_rho_values", C.c_void_p)]
+
+
+class EnsSource(C.Structure):
+    """wc_ens_source: a release point of a device-generated ensemble."""
+    _fields_ = [("k", C.c_double), ("j", C.c_double), ("i", C.c_double), ("spread", C.c_double),
+                ("time", C.c_double), ("count", C.c_uint64), ("scenario", C.c_int), ("sub_scenario", C.c_int),
+                ("sub_every", C.c_uint32)]
 
 
 class EnsStats(C.Structure):
@@ -126,6 +133,7 @@ def load_library(path: str = LIB_PATH):
         "wc_ens_run": [vp, C.POINTER(ConfigData), vp, vp, i, i, i, C.POINTER(EnsStats)],
         "wc_ens_surface": [vp, vp],
         "wc_ens_get": [vp, vp, vp, sz],
+        "wc_ens_release": [vp, vp, i, vp, i, C.c_uint64],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -185,6 +193,7 @@ class Context:
         if rc:
             raise WcError(rc, f"wc_ctx_create(device={device})")
         self.h = h
+        self.device = device
         self.shape = None
 
     def _chk(self, rc):
@@ -365,6 +374,14 @@ class Context:
         self._chk(self.L.wc_ens_set(self.h, _p(p), _p(sc), len(p), _p(sg), len(sg)))
         self.ens_shape = (len(sg),) + self.shape[1:]
         self.ens_n = len(p)
+
+    def ens_release(self, sources, sigma, seed=5489):
+        """Device-generated ensemble population from EnsSource records."""
+        arr = (EnsSource * max(1, len(sources)))(*sources)
+        sg = _f64(sigma)
+        self._chk(self.L.wc_ens_release(self.h, arr, len(sources), _p(sg), len(sg), seed))
+        self.ens_shape = (len(sg),) + self.shape[1:]
+        self.ens_n = int(sum(s.count for s in sources))
 
     def ens_set_launch(self, block=256, ppt=1, occ=0):
         self._chk(self.L.wc_ens_set_launch(self.h, block, ppt, occ))

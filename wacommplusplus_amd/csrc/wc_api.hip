@@ -1148,6 +1148,63 @@ int wc_ens_set(wc_ctx *c, const wc_particle_data *p, const uint8_t *scenario, si
     return WC_OK;
 }
 
+int wc_ens_release(wc_ctx *c, const wc_ens_source *sources, int n_sources, const double *sigma, int n_scenarios,
+                   uint64_t seed) {
+    if (!c || (n_sources && !sources) || !sigma || n_sources < 0) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    if (n_scenarios < 1 || n_scenarios > 256) return fail(c, WC_ERR_INVALID, "1..256 scenarios");
+    const size_t plane = (size_t)c->eta * c->xi;
+    if ((size_t)n_scenarios * plane > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "surface grids too large");
+    std::vector<EnsSource> src(n_sources);
+    std::vector<unsigned long long> first(n_sources + 1, 0);
+    for (int q = 0; q < n_sources; ++q) {
+        const wc_ens_source &S = sources[q];
+        if (S.scenario < 0 || S.scenario >= n_scenarios || S.sub_scenario >= n_scenarios ||
+            !std::isfinite(S.j) || !std::isfinite(S.i) || !std::isfinite(S.k) || !(S.spread >= 0) ||
+            !std::isfinite(S.time))
+            return fail(c, WC_ERR_INVALID, "invalid ensemble source %d", q);
+        src[q] = EnsSource{S.k, S.j, S.i, S.spread, S.time, S.scenario, S.sub_scenario, S.sub_every};
+        first[q + 1] = first[q] + S.count;
+    }
+    const unsigned long long n = first[n_sources];
+    if (n > (unsigned long long)INT32_MAX) return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", INT32_MAX);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_ens(c);
+    auto &E = c->ens;
+    const size_t cap = n ? n : 1;
+    int rc = alloc_ens_soa(c, E.s, cap);
+    if (!rc) rc = alloc_ens_soa(c, E.save, cap);
+    if (rc) { free_ens(c); return rc; }
+    HIPCHK(c, hipMalloc(&E.d_sigma, n_scenarios * sizeof(float)));
+    HIPCHK(c, hipMalloc(&E.d_surface, (size_t)n_scenarios * plane * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&E.d_steps, kEnsMaxSteps * sizeof(EnsStep)));
+    HIPCHK(c, hipMalloc(&E.d_stepIdx, sizeof(int)));
+    HIPCHK(c, hipMalloc(&E.d_count, sizeof(unsigned long long)));
+    std::vector<float> sg(n_scenarios);
+    for (int q = 0; q < n_scenarios; ++q) sg[q] = (float)sigma[q];
+    HIPCHK(c, hipMemcpy(E.d_sigma, sg.data(), n_scenarios * sizeof(float), hipMemcpyHostToDevice));
+    EnsSource *d_src = nullptr;
+    unsigned long long *d_first = nullptr;
+    HIPCHK(c, hipMalloc(&d_src, (n_sources ? n_sources : 1) * sizeof(EnsSource)));
+    HIPCHK(c, hipMalloc(&d_first, (n_sources + 1) * sizeof(unsigned long long)));
+    hipError_t e = hipSuccess;
+    if (n_sources) e = hipMemcpy(d_src, src.data(), n_sources * sizeof(EnsSource), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d_first, first.data(), (n_sources + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_ens_release(d_src, d_first, n_sources, (long long)n, c->eta, c->xi, seed, E.s, c->stream);
+    if (e == hipSuccess) e = launch_fill_u32(E.d_surface, (size_t)n_scenarios * plane, 0u, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d_src);
+    hipFree(d_first);
+    if (e != hipSuccess) return fail(c, WC_ERR_HIP, "ensemble release: %s", hipGetErrorString(e));
+    E.n = n;
+    E.nScen = n_scenarios;
+    E.have = true;
+    return WC_OK;
+}
+
 int wc_ens_set_launch(wc_ctx *c, int block, int ppt, int occ) {
     if (!c) return WC_ERR_INVALID;
     if ((block != 64 && block != 128 && block != 256) || ppt < 1 || ppt > 64 || (occ != 0 && occ != 3 && occ != 4))

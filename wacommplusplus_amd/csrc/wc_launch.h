@@ -87,6 +87,14 @@ struct EnsLaunch {
     int block, ppt, occ;
 };
 hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream);
+// One release point of a device-generated ensemble population.
+struct EnsSource {
+    double k, j, i, spread, time;
+    int scenario, subScenario;
+    uint32_t subEvery;
+};
+hipError_t launch_ens_release(const EnsSource *src, const unsigned long long *first, int nsrc, long long n, int eta,
+                              int xi, uint64_t seed, const EnsSoA &st, hipStream_t stream);
 
 // ---- reconstruction network (wc_unet.hip) -----------------------------------
 // 3x3 convolution as an implicit GEMM on BF16 MFMA.  Inputs x1 (c1 channels)

@@ -592,6 +592,47 @@ __global__ __launch_bounds__(256) void ens_surface_kernel(const EnsStep *__restr
 
 __global__ void ens_advance_kernel(int *stepIdx) { stepIdx[0] += 1; }
 
+// Device release of an ensemble population: particle q (global id q) belongs
+// to the source s with first[s] <= q < first[s+1]; its j and i are offset by
+// spread * N(0, 1) draws of Philox4x32-10 keyed on (seed; q, 0xE5, 0, 0), so a
+// population is reproducible and independent of the launch; every
+// sub_every-th particle of a source (by its index in the source) goes to the
+// source's sub-scenario.
+__global__ void ens_release_kernel(const EnsSource *__restrict__ src, const unsigned long long *__restrict__ first,
+                                   int nsrc, long long n, int eta, int xi, uint64_t seed, EnsSoA st) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+         q += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nsrc - 1;                         // last s with first[s] <= q
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= (unsigned long long)q) lo = mid; else hi = mid - 1;
+        }
+        const EnsSource &S = src[lo];
+        const unsigned long long r = (unsigned long long)q - first[lo];
+        const Philox4 z = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), 0xE5u, 0u, (uint32_t)seed,
+                                        (uint32_t)(seed >> 32));
+        const double u1 = ((double)(z.v[0] >> 8) + 0.5) * 0x1.0p-24, u2 = (double)(z.v[1] >> 8) * 0x1.0p-24;
+        const double rad = sqrt(-2.0 * log(u1));
+        const double j = S.j + S.spread * rad * cos(6.283185307179586 * u2);
+        const double i = S.i + S.spread * rad * sin(6.283185307179586 * u2);
+        const double jfl = floor(j), ifl = floor(i);
+        const bool inside = jfl >= 0 && ifl >= 0 && jfl + 1 < eta && ifl + 1 < xi;
+        int jc = (int)jfl, ic = (int)ifl;
+        float fj = inside ? (float)(j - jfl) : 0.f, fi = inside ? (float)(i - ifl) : 0.f;
+        if (fj >= 1.0f) { fj -= 1.0f; ++jc; }
+        if (fi >= 1.0f) { fi -= 1.0f; ++ic; }
+        st.id[q] = (uint64_t)q;
+        st.cell[q] = inside ? (uint32_t)jc * (uint32_t)xi + (uint32_t)ic : 0xFFFFFFFFu;
+        st.fj[q] = fj;
+        st.fi[q] = fi;
+        st.k[q] = (float)S.k;
+        st.health[q] = 1.0f;
+        st.age[q] = 0.0;
+        st.time[q] = S.time;
+        st.scen[q] = (uint8_t)((S.subEvery && S.subScenario >= 0 && r % S.subEvery == 0) ? S.subScenario : S.scenario);
+    }
+}
+
 int grid_blocks(size_t n) {
     size_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -679,6 +720,15 @@ hipError_t ens_launch_t(const EnsLaunch &L, hipStream_t stream) {
 }
 
 }  // namespace
+
+hipError_t launch_ens_release(const EnsSource *src, const unsigned long long *first, int nsrc, long long n, int eta,
+                              int xi, uint64_t seed, const EnsSoA &st, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    long long b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    ens_release_kernel<<<(int)b, 256, 0, stream>>>(src, first, nsrc, n, eta, xi, seed, st);
+    return hipGetLastError();
+}
 
 hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream) {
     if (L.block != 64 && L.block != 128 && L.block != 256) return hipErrorInvalidValue;

@@ -1,0 +1,249 @@
+"""Learned concentration reconstruction: training pairs generated by the
+method itself, training and evaluation of the network (wc_unet_*).
+
+Pairs.  For each pair a release point (a random ocean cell), a horizontal
+diffusivity Kh (log-uniform) and a particle count N are drawn.  One batched
+ensemble run moves the N particles; every `sub_every`-th particle (by id)
+belongs to a second scenario, so the same run yields the surface-layer
+histogram of the n = N / sub_every subsample (the input) and of all N
+particles (the target).  Draws are keyed by particle id, so the subsample
+moves exactly as it would alone.  Images are crops around the release point.
+
+Network input and target are variance-stabilised: x = sqrt(sub_every * few)
+/ scale, t = sqrt(all) / scale (the square root of a Poisson count has
+roughly constant variance); the reconstruction is (max(y, 0) * scale)^2.
+Plain gridding -- the fallback -- is sub_every * few.
+
+Errors are relative L2 norms against the N-particle histogram.  That target
+has its own sampling noise, estimated from the Poisson variance of a
+histogram (E|h - lambda|^2 = sum lambda ~ sum h): e_target = sqrt(sum h) /
+|h|.  Errors against the unknown noise-free field are then estimated as
+sqrt(e^2 - e_target^2).
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from ._lib import Context, EnsSource, UNet
+from .synthetic import ForcingWindow, depth_to_k
+from .types import default_config
+
+
+def ocean_cells(W: ForcingWindow, n: int, margin: int, rng):
+    wet = W.mask > 0
+    ok = wet[:-1, :-1] & wet[1:, :-1] & wet[:-1, 1:] & wet[1:, 1:]
+    ok[:margin, :] = False
+    ok[-margin:, :] = False
+    ok[:, :margin] = False
+    ok[:, -margin:] = False
+    cells = np.argwhere(ok)
+    pick = cells[rng.choice(len(cells), n, replace=False)]
+    off = rng.uniform(0.05, 0.95, (n, 2))
+    return pick[:, 0] + off[:, 0], pick[:, 1] + off[:, 1]
+
+
+def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int = 1_000_000, sub_every: int = 20,
+                   hours: int = 24, crop: int = 128, seed: int = 5489, kh_range=(0.1, 10.0), dti: float = 30.0,
+                   depth_m: float = 1.0, batch_pairs: int = 128, log=None):
+    """Surface-layer histograms (few, all) of n_pairs releases after `hours`,
+    cropped to crop x crop around the centre of mass of the few-particle
+    histogram (what is known at reconstruction time; the release point when
+    it is empty).  The context must hold the window's grid and records
+    (wc_set_grid with n_slots >= hours + 1).  Returns dict(few, all: uint32
+    [n_pairs][crop][crop], kh, j, i, origin [n_pairs][2], seconds,
+    particle_steps)."""
+    rng = np.random.default_rng(seed)
+    js, is_ = ocean_cells(W, n_pairs, crop // 2 + 2, rng)
+    kh = np.exp(rng.uniform(math.log(kh_range[0]), math.log(kh_range[1]), n_pairs))
+    sigma = np.sqrt(2.0 * kh * dti)
+    k0 = depth_to_k(depth_m, float(W.h[W.mask > 0].mean()), W.s_w)
+    cfg = default_config(random=True, sigma=1.0, dti=dti)
+    times = [float(W.times[q]) for q in range(hours + 1)]
+    slots = list(range(hours + 1))
+    few = np.zeros((n_pairs, crop, crop), np.uint32)
+    allc = np.zeros((n_pairs, crop, crop), np.uint32)
+    origin = np.zeros((n_pairs, 2), np.int64)
+    jj, ii = np.arange(W.eta, dtype=np.float64), np.arange(W.xi, dtype=np.float64)
+    t0 = time.perf_counter()
+    steps = 0
+    for b0 in range(0, n_pairs, batch_pairs):
+        nb = min(batch_pairs, n_pairs - b0)
+        sources = [EnsSource(k=k0, j=float(js[b0 + q]), i=float(is_[b0 + q]), spread=0.25, time=times[0],
+                             count=n_target, scenario=2 * q + 1, sub_scenario=2 * q, sub_every=sub_every)
+                   for q in range(nb)]
+        sg = np.repeat(sigma[b0:b0 + nb], 2)
+        ctx.ens_release(sources, sg, seed=seed + 7919 * (b0 + 1))
+        st = ctx.ens_run(cfg, times, slots, use_graph=True, surface=True)
+        steps += st.particle_steps
+        surf = ctx.ens_surface()
+        for q in range(nb):
+            fs = surf[2 * q].astype(np.float64)
+            tot = fs.sum()
+            cj = (fs.sum(1) @ jj) / tot if tot > 0 else js[b0 + q]
+            ci = (fs.sum(0) @ ii) / tot if tot > 0 else is_[b0 + q]
+            r0 = int(np.clip(round(cj) - crop // 2, 0, W.eta - crop))
+            c0 = int(np.clip(round(ci) - crop // 2, 0, W.xi - crop))
+            origin[b0 + q] = (r0, c0)
+            f = surf[2 * q, r0:r0 + crop, c0:c0 + crop]
+            few[b0 + q] = f
+            allc[b0 + q] = f + surf[2 * q + 1, r0:r0 + crop, c0:c0 + crop]
+        if log:
+            log(f"  pairs {b0 + nb}/{n_pairs}: {st.particle_steps:.3g} particle-steps in {st.device_ms:.0f} ms")
+    return dict(few=few, all=allc, kh=kh, j=js, i=is_, origin=origin, seconds=time.perf_counter() - t0,
+                particle_steps=steps, n_target=n_target, sub_every=sub_every, hours=hours)
+
+
+def to_net(few, allc, sub_every, scale):
+    x = np.sqrt(sub_every * few.astype(np.float64)) / scale
+    t = None if allc is None else np.sqrt(allc.astype(np.float64)) / scale
+    return x.astype(np.float32), (None if t is None else t.astype(np.float32))
+
+
+def augment(x, t, rng):
+    k = int(rng.integers(4))
+    x, t = np.rot90(x, k, axes=(1, 2)), np.rot90(t, k, axes=(1, 2))
+    if rng.integers(2):
+        x, t = x[:, :, ::-1], t[:, :, ::-1]
+    return np.ascontiguousarray(x), np.ascontiguousarray(t)
+
+
+def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, lr_min=1e-4, seed=1, log=None):
+    """Adam with a cosine learning-rate decay; random rotations and flips."""
+    rng = np.random.default_rng(seed)
+    n = len(X)
+    steps_per_epoch = max(1, n // batch)
+    total = epochs * steps_per_epoch
+    hist = []
+    t0 = time.perf_counter()
+    step = 0
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        losses = []
+        for s in range(steps_per_epoch):
+            b = order[s * batch:(s + 1) * batch]
+            x, t = augment(X[b], T[b], rng)
+            a = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / total))
+            losses.append(net.train_step(x, t, a))
+            step += 1
+        hist.append(float(np.mean(losses)))
+        if log:
+            log(f"  epoch {ep + 1}/{epochs}: loss {hist[-1]:.4f}")
+    return dict(loss=hist, seconds=time.perf_counter() - t0, steps=step)
+
+
+def reconstruct(net: UNet, few, sub_every, scale, batch=16):
+    """Concentration estimate (counts of the full release) from few-particle
+    histograms, and the device ms of the forward passes."""
+    x, _ = to_net(few, None, sub_every, scale)
+    out = np.zeros_like(x)
+    ms = 0.0
+    for b in range(0, len(x), batch):
+        y, m = net.forward(x[b:b + batch])
+        out[b:b + batch] = y
+        ms += m
+    return (np.maximum(out, 0.0).astype(np.float64) * scale) ** 2, ms
+
+
+def rel_l2(est, ref):
+    """Per-image relative L2 error."""
+    d = (est - ref).reshape(len(ref), -1)
+    r = ref.reshape(len(ref), -1).astype(np.float64)
+    return np.linalg.norm(d, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-30)
+
+
+def evaluate(net: UNet, data, scale):
+    """Held-out errors of plain gridding and of the network against the
+    N-particle histograms, the targets' own noise, and the estimated errors
+    against the noise-free field.  Crops whose target is empty (every
+    particle left the surface layer or the crop) carry no information and
+    are left out (their number is reported)."""
+    keep = data["all"].reshape(len(data["all"]), -1).sum(1) > 0
+    few, allc, sub = data["few"][keep], data["all"][keep].astype(np.float64), data["sub_every"]
+    grid = sub * few.astype(np.float64)
+    rec, ms = reconstruct(net, few, sub, scale)
+    e_grid, e_net = rel_l2(grid, allc), rel_l2(rec, allc)
+    e_t = np.sqrt(allc.reshape(len(allc), -1).sum(1)) / np.linalg.norm(allc.reshape(len(allc), -1), axis=1)
+    truth = lambda e: np.sqrt(np.maximum(e ** 2 - e_t ** 2, 0.0))   # noqa: E731
+    return dict(e_grid=float(e_grid.mean()), e_net=float(e_net.mean()), e_target=float(e_t.mean()),
+                e_grid_truth=float(truth(e_grid).mean()), e_net_truth=float(truth(e_net).mean()),
+                net_better_fraction=float((e_net < e_grid).mean()), forward_ms_per_image=ms / len(few),
+                images=len(few), empty_targets=int((~keep).sum()))
+
+
+def train_flops_per_image(h: int, w: int) -> float:
+    """Multiply-add FLOPs of one training step per image (forward + both
+    backward products of every 3x3 conv): 3 x sum 2 H W 9 cin cout."""
+    from .types import UNET_SPEC
+    f, level = 0.0, [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0]
+    for (co, c1, c2), lv in zip(UNET_SPEC, level):
+        f += 2.0 * (h >> lv) * (w >> lv) * 9 * (c1 + c2) * co
+    return 3.0 * f
+
+
+def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, epochs: int = 40,
+        n_target: int = 1_000_000, sub_every: int = 20, hours: int = 24, crop: int = 128, batch: int = 16,
+        scale: float = 16.0, seed: int = 21, log=None):
+    """The whole learned-reconstruction experiment: pairs, training, held-out
+    errors, and the comparison with plain gridding at equal GPU time (the
+    move of n = n_target / sub_every particles per scenario, batched 256 at a
+    time, plus one forward pass per image)."""
+    tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, log=log)
+    te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, crop, seed + 1, log=log)
+    X, T = to_net(tr["few"], tr["all"], sub_every, scale)
+    net = UNet(ctx_device(ctx), seed=3)
+    h = train(net, X, T, epochs=epochs, batch=batch, log=log)
+    ev = evaluate(net, te, scale)
+    # GPU time of the few-particle move: 256 scenarios of n particles, batched
+    n = n_target // sub_every
+    srcs = [EnsSource(k=-0.27, j=float(tr["j"][q % n_train]), i=float(tr["i"][q % n_train]), spread=0.25,
+                      time=float(W.times[0]), count=n, scenario=q, sub_scenario=-1, sub_every=0) for q in range(256)]
+    ctx.ens_release(srcs, np.full(256, math.sqrt(60.0)), seed=seed + 2)
+    cfg = default_config(random=True, sigma=1.0)
+    st = ctx.ens_run(cfg, [float(W.times[q]) for q in range(hours + 1)], list(range(hours + 1)), use_graph=True,
+                     surface=True)
+    move_ms = st.device_ms / 256
+    net_ms = ev["forward_ms_per_image"]
+    n_equal = n * (move_ms + net_ms) / move_ms              # plain gridding's particles at the same GPU time
+    e_grid_equal = ev["e_grid_truth"] * math.sqrt(n / n_equal)
+    n_match = n * (ev["e_grid_truth"] / max(ev["e_net_truth"], 1e-12)) ** 2
+    flops = train_flops_per_image(crop, crop) * h["steps"] * batch
+    return dict(
+        pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop,
+                   seconds=tr["seconds"] + te["seconds"], particle_steps=int(tr["particle_steps"] + te["particle_steps"])),
+        training=dict(epochs=epochs, steps=h["steps"], seconds=h["seconds"], loss_first=h["loss"][0],
+                      loss_last=h["loss"][-1], tflops=flops / h["seconds"] / 1e12),
+        held_out=ev,
+        equal_gpu_ms=dict(move_ms_per_scenario=move_ms, net_ms_per_image=net_ms, plain_particles=n_equal,
+                          plain_e_truth=e_grid_equal, learned_e_truth=ev["e_net_truth"],
+                          plain_particles_to_match_learned=n_match),
+    )
+
+
+def ctx_device(ctx: Context) -> int:
+    return ctx.device
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+
+    from .synthetic import full_window
+
+    ap = argparse.ArgumentParser(description="train and evaluate the learned concentration reconstruction")
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--test", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--target", type=int, default=1_000_000)
+    ap.add_argument("--sub-every", type=int, default=20)
+    a = ap.parse_args()
+    say = lambda m: print(m, file=sys.stderr, flush=True)   # noqa: E731
+    W = full_window(T=25)
+    c = Context(0)
+    c.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=W.T, store="bf16")
+    for t in range(W.T):
+        c.upload_record(t, *W.record(t))
+    print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, log=say)), flush=True)

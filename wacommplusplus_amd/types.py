@@ -101,3 +101,9 @@ def make_particles(n, k=0.0, j=0.0, i=0.0, time=0.0, health=1.0, age=0.0, first_
     p["k"], p["j"], p["i"] = k, j, i
     p["health"], p["age"], p["time"] = health, age, time
     return p
+
+
+# (cout, c1, c2) of the reconstruction network's 3x3 conv layers, in parameter
+# order (include/wacomm_hip.h, section 4)
+UNET_SPEC = [(32, 32, 0), (32, 32, 0), (64, 32, 0), (64, 64, 0), (128, 64, 0), (128, 128, 0), (256, 128, 0),
+             (256, 256, 0), (128, 128, 256), (128, 128, 0), (64, 64, 128), (64, 64, 0), (32, 32, 64), (32, 32, 0)]
