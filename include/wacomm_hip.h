@@ -101,7 +101,8 @@ typedef enum wc_status {
     WC_ERR_UNSUPPORTED = 3,  /* configuration the device path does not implement
                                 (driftModel==1, jibing): use the CPU backend */
     WC_ERR_NOMEM = 4,        /* device allocation failed */
-    WC_ERR_NO_DEVICE = 5     /* no HIP device, or an invalid device index */
+    WC_ERR_NO_DEVICE = 5,    /* no HIP device, or an invalid device index */
+    WC_ERR_IO = 6            /* a file could not be written */
 } wc_status;
 
 /* Closure and integrator codes, Config.hpp:201-207 */
@@ -463,6 +464,32 @@ int wc_unet_gradients(wc_unet *net, const float *x, const float *t, int n, int h
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at learning rate lr on
  * that loss; *loss receives the loss before the step. */
 int wc_unet_train_step(wc_unet *net, const float *x, const float *t, int n, int h, int w, float lr, float *loss);
+
+/* ---------------------------------------------------------------------------
+ * 5. Output formats around the path
+ * ------------------------------------------------------------------------- */
+
+/* The concentration file of Wacomm::save (Wacomm.cpp:913-1020): dimensions
+ * ocean_time, s_rho, eta_rho, eta_xi; variables ocean_time, s_rho, lat_rho,
+ * lon_rho, mask_rho, h (double), zeta [T][eta][xi] and conc
+ * [T][s_rho][eta][xi] (float, _FillValue 9.99999993e36) with the reference's
+ * attributes, in the NetCDF classic format with 64-bit offsets (CDF-2).
+ * zeta / conc may be NULL (written as _FillValue, as the reference leaves
+ * conc unwritten when another rank writes it).  global_attributes: NULL or a
+ * NULL-terminated list of name, value strings (the provenance attributes of
+ * Provenance.hpp:19-29).  Written to <path>.partial, then renamed.
+ * Host-only: needs no device.  Returns WC_ERR_IO when the file cannot be
+ * written. */
+typedef struct wc_conc_file {
+    int T, s_rho, eta, xi;
+    const double *ocean_time;     /* [T] */
+    const double *s_rho_values;   /* [s_rho] */
+    const double *lat_rho, *lon_rho, *mask_rho, *h;   /* [eta][xi] */
+    const float *zeta;            /* [T][eta][xi] or NULL */
+    const float *conc;            /* [T][s_rho][eta][xi] or NULL */
+    const char *const *global_attributes;
+} wc_conc_file;
+int wc_write_concentration_nc(const char *path, const wc_conc_file *file);
 
 #ifdef __cplusplus
 }

@@ -204,6 +204,8 @@ struct OceanWindow {
     std::vector<float> zeta;                     // [T][eta][xi]
     std::vector<float> u, v;                     // [T][s_rho][eta][xi]
     std::vector<float> w, akt;                   // [T][s_rho+1][eta][xi]
+    std::vector<double> sRho;                    // [s_rho] sigma at rho levels (for the output file)
+    std::vector<double> lon, lat;                // [eta][xi] degrees (empty: lonRad / 0.0174533)
     int T() const { return (int)oceanTime.size(); }
     int s_w() const { return s_rho + 1; }
 };
@@ -255,6 +257,32 @@ public:
     }
 
     const std::vector<float> &Conc() const { return conc_; }
+
+    /* Wacomm::save (Wacomm.cpp:913-1020): the window's concentration frames
+       with its grid and zeta, in NetCDF classic format. */
+    void save(const std::string &fileName) const {
+        const OceanWindow &o = *ocean_;
+        const size_t plane = (size_t)o.eta * o.xi;
+        std::vector<double> lon = o.lon, lat = o.lat, sr = o.sRho, mask = o.mask;
+        if (lon.size() != plane || lat.size() != plane) {
+            lon.resize(plane);
+            lat.resize(plane);
+            for (size_t q = 0; q < plane; ++q) {
+                lon[q] = o.lonRad[q] / 0.0174533;              // the adapter's degree factor
+                lat[q] = o.latRad[q] / 0.0174533;
+            }
+        }
+        if ((int)sr.size() != o.s_rho) {
+            sr.resize(o.s_rho);
+            for (int k = 0; k < o.s_rho; ++k) sr[k] = (k - o.s_rho + 0.5) / o.s_rho;   // uniform sigma
+        }
+        if (mask.size() != plane) mask.assign(plane, 1.0);
+        const char *attrs[] = {"wacomm_random_key_scheme", "ordered-splitmix64-v2", nullptr, nullptr};
+        wc_conc_file f{o.T(), o.s_rho, o.eta, o.xi, o.oceanTime.data(), sr.data(), lat.data(), lon.data(),
+                       mask.data(), o.h.data(), o.zeta.data(), conc_.data(), attrs};
+        const int rc = wc_write_concentration_nc(fileName.c_str(), &f);
+        if (rc != WC_OK) throw Error(rc, "cannot write " + fileName);
+    }
     std::uint64_t ParticleSteps() const { return steps_; }
     Context &context() { return ctx_; }
 

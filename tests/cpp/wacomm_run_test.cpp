@@ -106,6 +106,20 @@ int main() {
     double total = 0;
     for (float x : mine) total += x;
     assert(total == 1005.0);
+    // Wacomm::save: the concentration file exists, complete (no .partial left)
+    wacomm.save("wc-conc-test.nc");
+    {
+        std::FILE *fp = std::fopen("wc-conc-test.nc", "rb");
+        assert(fp);
+        char magic[4];
+        assert(std::fread(magic, 1, 4, fp) == 4 && magic[0] == 'C' && magic[1] == 'D' && magic[2] == 'F' &&
+               magic[3] == 2);
+        std::fseek(fp, 0, SEEK_END);
+        assert((size_t)std::ftell(fp) > (size_t)T * s_rho * plane * 4);
+        std::fclose(fp);
+        assert(std::fopen("wc-conc-test.nc.partial", "rb") == nullptr);
+        std::remove("wc-conc-test.nc");
+    }
     std::cout << "wacomm_run_test: PASS" << std::endl;
     return 0;
 }

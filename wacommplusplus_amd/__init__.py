@@ -10,12 +10,13 @@ from .types import (BACKWARD, CONSTRAINT, EULER, FORWARD, KILL, MIDPOINT, PARTIC
                     ConfigData, default_config, make_particles, sigma_from_kh)
 from ._lib import (EXPORTS, FORCING_INTERVAL_BATCH, LIB_PATH, ROMS_MAX_MISSING, SINGLE_PULSE, UNIFORM_RATE,
                    WC_ERR_HIP, WC_ERR_INVALID, WC_ERR_NO_DEVICE, WC_ERR_NOMEM, WC_ERR_UNSUPPORTED, WC_OK, Context, EnsSource, EnsStats,
-                   RunStats, Source, UNet, WcError, cell_metric_table, load_library, philox4x32_10)
+                   RunStats, Source, UNet, WcError, cell_metric_table, load_library, philox4x32_10,
+                   write_concentration_nc, WC_ERR_IO)
 
 __all__ = [
     "BACKWARD", "CONSTRAINT", "EULER", "FORWARD", "KILL", "MIDPOINT", "PARTICLE_DTYPE", "REFLECTION",
     "ConfigData", "default_config", "make_particles", "sigma_from_kh", "EXPORTS", "LIB_PATH", "Context",
     "WcError", "cell_metric_table", "load_library", "philox4x32_10", "Source", "RunStats", "UNIFORM_RATE",
     "FORCING_INTERVAL_BATCH", "SINGLE_PULSE", "ROMS_MAX_MISSING", "WC_OK", "WC_ERR_HIP", "WC_ERR_INVALID",
-    "WC_ERR_UNSUPPORTED", "WC_ERR_NOMEM", "WC_ERR_NO_DEVICE", "EnsStats", "EnsSource", "UNet",
+    "WC_ERR_UNSUPPORTED", "WC_ERR_NOMEM", "WC_ERR_NO_DEVICE", "EnsStats", "EnsSource", "UNet", "write_concentration_nc", "WC_ERR_IO",
 ]

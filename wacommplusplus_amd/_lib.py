@@ -15,9 +15,9 @@ from .types import PARTICLE_DTYPE, PATH, STORE, ConfigData
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libwacomm_hip.so")
 
-WC_OK, WC_ERR_HIP, WC_ERR_INVALID, WC_ERR_UNSUPPORTED, WC_ERR_NOMEM, WC_ERR_NO_DEVICE = range(6)
+WC_OK, WC_ERR_HIP, WC_ERR_INVALID, WC_ERR_UNSUPPORTED, WC_ERR_NOMEM, WC_ERR_NO_DEVICE, WC_ERR_IO = range(7)
 _STATUS = {1: "WC_ERR_HIP", 2: "WC_ERR_INVALID", 3: "WC_ERR_UNSUPPORTED", 4: "WC_ERR_NOMEM",
-           5: "WC_ERR_NO_DEVICE"}
+           5: "WC_ERR_NO_DEVICE", 6: "WC_ERR_IO"}
 
 # every symbol include/wacomm_hip.h declares
 EXPORTS = (
@@ -30,6 +30,7 @@ EXPORTS = (
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
     "wc_ens_get", "wc_ens_release", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
+    "wc_write_concentration_nc",
 )
 
 
@@ -64,6 +65,12 @@ class RomsGridDesc(C.Structure):
                [(n, C.c_void_p) for n in ("missing_u", "missing_v", "missing_angle")] + \
                [(n, C.c_int) for n in ("n_missing_u", "n_missing_v", "n_missing_angle")] + \
                [("s_rho_values", C.c_void_p)]
+
+
+class ConcFile(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("T", "s_rho", "eta", "xi")] + \
+               [(n, C.c_void_p) for n in ("ocean_time", "s_rho_values", "lat_rho", "lon_rho", "mask_rho", "h",
+                                          "zeta", "conc", "global_attributes")]
 
 
 class EnsSource(C.Structure):
@@ -164,6 +171,8 @@ def load_library(path: str = LIB_PATH):
     L.wc_unet_last_error.restype = C.c_char_p
     L.wc_unet_param_count.argtypes = [vp]
     L.wc_unet_param_count.restype = C.c_size_t
+    L.wc_write_concentration_nc.argtypes = [C.c_char_p, C.POINTER(ConcFile)]
+    L.wc_write_concentration_nc.restype = C.c_int
     _lib = L
     return L
 
@@ -518,3 +527,30 @@ class UNet:
         loss = C.c_float()
         self._chk(self.L.wc_unet_train_step(self.h, _p(xx), _p(tt), n, hh, ww, float(lr), C.byref(loss)))
         return loss.value
+
+
+def write_concentration_nc(path, ocean_time, s_rho_values, lat_rho, lon_rho, mask_rho, h, zeta=None, conc=None,
+                           attributes=None):
+    """Wacomm::save's concentration file (Wacomm.cpp:913-1020) in NetCDF
+    classic format: zeta [T][eta][xi], conc [T][s_rho][eta][xi] float (None:
+    written as _FillValue), grid fields [eta][xi] in degrees."""
+    L = load_library()
+    t, sr = _f64(ocean_time), _f64(s_rho_values)
+    lat, lon, m, hh = (_f64(a) for a in (lat_rho, lon_rho, mask_rho, h))
+    eta, xi = lat.shape
+    z = None if zeta is None else _f32(zeta)
+    c = None if conc is None else _f32(conc)
+    if z is not None:
+        assert z.shape == (len(t), eta, xi)
+    if c is not None:
+        assert c.shape == (len(t), len(sr), eta, xi)
+    names = []
+    for k, v in (attributes or {}).items():
+        names += [str(k).encode(), str(v).encode()]
+    arr = (C.c_char_p * (len(names) + 2))(*names, None, None)
+    f = ConcFile(len(t), len(sr), eta, xi, _p(t).value, _p(sr).value, _p(lat).value, _p(lon).value, _p(m).value,
+                 _p(hh).value, None if z is None else _p(z).value, None if c is None else _p(c).value,
+                 C.cast(arr, C.c_void_p).value)
+    rc = L.wc_write_concentration_nc(str(path).encode(), C.byref(f))
+    if rc:
+        raise WcError(rc, f"writing {path}")
