@@ -146,8 +146,9 @@ def ensemble_bench(wc, ctx, W, cfg, steps):
     ctx.ens_set(p, scen, sigma)
     best_ms = ctx.ens_autotune(cfg, times[0], times[1], 0, 1)
     shape = ctx.ens_get_launch()
+    tile = ctx.ens_get_sort()[0]
     log(f"[ensemble] {len(p)} particles; autotuned block={shape[0]} ppt={shape[1]} occ={shape[2]} "
-        f"({best_ms:.2f} ms on the first interval)")
+        f"sort tile={tile} ({best_ms:.2f} ms on the first interval)")
     out = {}
     for use_graph in (False, True):
         ctx.ens_set(p, scen, sigma)
@@ -158,7 +159,7 @@ def ensemble_bench(wc, ctx, W, cfg, steps):
         log(f"[ensemble] {key}: {out[key]}")
     surf = ctx.ens_surface()
     return dict(metric="particle-steps/s", value=out["graph"]["value"], scenarios=len(sigma), particles=len(p),
-                intervals=steps, launch=dict(block=shape[0], ppt=shape[1], occupancy_hint=shape[2]),
+                intervals=steps, launch=dict(block=shape[0], ppt=shape[1], occupancy_hint=shape[2], sort_tile=tile),
                 autotune_first_interval_ms=best_ms, graph=out["graph"], direct=out["direct"],
                 surface_counts_last_interval=int(surf.sum()), dtype="f32 state, bf16 forcing",
                 counts=dict(min=int(counts.min()), median=float(np.median(counts)), max=int(counts.max())))

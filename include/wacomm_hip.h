@@ -397,10 +397,18 @@ int wc_ens_release(wc_ctx *ctx, const wc_ens_source *sources, int n_sources, con
  * 3, 4} waves per SIMD.  wc_ens_autotune picks them by timing. */
 int wc_ens_set_launch(wc_ctx *ctx, int block, int ppt, int occ);
 int wc_ens_get_launch(wc_ctx *ctx, int *block, int *ppt, int *occ);
+int wc_ens_get_sort(wc_ctx *ctx, int *tile, int *period);
 
-/* Times every offered launch shape on one interval [start, end] (records in
- * slot0, slot1) over the current population, which is restored afterwards,
- * and keeps the fastest.  *best_ms (optional) receives its time. */
+/* Reorders the ensemble population by the tile (tile x tile cells) of its cell
+ * before the move of every period-th step, so a wave's lanes gather from
+ * neighbouring cells; tile 0 never sorts (the default).  Results are per
+ * particle and do not depend on the order. */
+int wc_ens_set_sort(wc_ctx *ctx, int tile, int period);
+
+/* Times every offered launch shape and sort tile (none, 2, 4, 8; sorted every
+ * step) on one interval [start, end] (records in slot0, slot1) over the
+ * current population, which is restored afterwards, and keeps the fastest
+ * (108 trials).  *best_ms (optional) receives its time. */
 int wc_ens_autotune(wc_ctx *ctx, const wc_config_data *config, double start, double end, int slot0, int slot1,
                     double *best_ms);
 
@@ -426,9 +434,9 @@ int wc_ens_run(wc_ctx *ctx, const wc_config_data *config, const double *times, c
  * counts[n_scenarios][eta][xi]. */
 int wc_ens_surface(wc_ctx *ctx, uint32_t *counts);
 
-/* The ensemble population as particle_data (j = cell row + offset, i = cell
- * column + offset; NaN for a particle that started outside the grid) and,
- * when scenario is not NULL, each particle's scenario. */
+/* The ensemble population as particle_data in id order (j = cell row +
+ * offset, i = cell column + offset; NaN for a particle that started outside
+ * the grid) and, when scenario is not NULL, each particle's scenario. */
 int wc_ens_get(wc_ctx *ctx, wc_particle_data *particles, uint8_t *scenario, size_t n);
 
 /* ---------------------------------------------------------------------------

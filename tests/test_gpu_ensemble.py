@@ -167,6 +167,7 @@ def test_multi_interval_graph_run_and_autotune(ctx, full):
     shape = ctx.ens_get_launch()
     print(f"autotuned launch {shape}: {best:.3f} ms", flush=True)
     assert best > 0 and shape[0] in (64, 128, 256) and shape[1] in (1, 2, 4) and shape[2] in (0, 3, 4)
+    assert ctx.ens_get_sort()[0] in (0, 2, 4, 8)
     after, _ = ctx.ens_get()                     # the population is restored by the autotuner
     for f in ("k", "j", "i", "health", "age"):
         assert np.array_equal(after[f], initial[f], equal_nan=True), f
@@ -193,6 +194,27 @@ def test_permuted_population_same_per_id(ctx, full):
     for f in ("k", "j", "i", "health", "age"):
         assert np.array_equal(b[f][order], a[f], equal_nan=True), f
     assert np.array_equal(sb[order], sa)
+
+
+@pytest.mark.parametrize("tile,period", [(2, 1), (4, 1), (8, 3)])
+def test_cell_tile_sort_changes_no_result(ctx, full, tile, period):
+    """Sorting the population by cell tile (in the captured step) gives every
+    particle the same state and the same surface counts as no sorting."""
+    p, scen, sigma, counts = ensemble(full, n_scenarios=16, total=200_000, median=10_000, lo=2_000, hi=40_000)
+    cfg = wc.default_config(random=True, sigma=1.0)
+    load_window(ctx, full)
+    res = []
+    for t in (0, tile):
+        ctx.ens_set(p, scen, sigma)
+        ctx.ens_set_sort(t, period)
+        run_ens(ctx, full, cfg, 4, use_graph=True)
+        res.append((ctx.ens_get(), ctx.ens_surface()))
+    (a, sa), ga = res[0]
+    (b, sb), gb = res[1]
+    for f in ("id", "k", "j", "i", "health", "age", "time"):
+        assert np.array_equal(a[f], b[f], equal_nan=True), f
+    assert np.array_equal(sa, sb) and np.array_equal(ga, gb)
+    ctx.ens_set_sort(0, 1)
 
 
 def test_rejects_bad_input(ctx, full):

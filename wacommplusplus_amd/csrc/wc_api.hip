@@ -75,7 +75,9 @@ struct wc_ctx {
         bool have = false;
         size_t n = 0;
         int nScen = 0;
-        EnsSoA s{}, save{};
+        EnsSoA s{}, save{}, tmp{};
+        uint32_t *d_bins = nullptr;
+        int sortTile = 0, sortPeriod = 1;
         float *d_sigma = nullptr;
         uint32_t *d_surface = nullptr;
         EnsStep *d_steps = nullptr;
@@ -433,12 +435,15 @@ static void free_ens(wc_ctx *c) {
     free_ens_graph(c);
     free_ens_soa(c->ens.s);
     free_ens_soa(c->ens.save);
+    free_ens_soa(c->ens.tmp);
     hipFree(c->ens.d_sigma); hipFree(c->ens.d_surface); hipFree(c->ens.d_steps); hipFree(c->ens.d_stepIdx);
-    hipFree(c->ens.d_count);
+    hipFree(c->ens.d_count); hipFree(c->ens.d_bins);
     const int block = c->ens.block, ppt = c->ens.ppt, occ = c->ens.occ;
+    const int tile = c->ens.sortTile, period = c->ens.sortPeriod;
     const bool tuned = c->ens.tuned;
     c->ens = wc_ctx::Ens{};
     c->ens.block = block; c->ens.ppt = ppt; c->ens.occ = occ; c->ens.tuned = tuned;
+    c->ens.sortTile = tile; c->ens.sortPeriod = period;
 }
 
 static void free_grid(wc_ctx *c) {
@@ -1038,6 +1043,13 @@ int copy_ens_soa(wc_ctx *c, const EnsSoA &dst, const EnsSoA &src, size_t n) {
     return WC_OK;
 }
 
+// the cell-tile sort's counters for its smallest tile
+constexpr int kMinSortTile = 2;
+size_t ens_max_bins(const wc_ctx *c) {
+    const size_t tx = (c->xi + kMinSortTile - 1) / kMinSortTile, ty = (c->eta + kMinSortTile - 1) / kMinSortTile;
+    return tx * ty + 1;
+}
+
 EnsLaunch ens_launch(wc_ctx *c, bool surface) {
     EnsLaunch L{};
     L.steps = c->ens.d_steps;
@@ -1052,6 +1064,10 @@ EnsLaunch ens_launch(wc_ctx *c, bool surface) {
     L.stepCounter = c->ens.d_count;
     L.s_w = c->s_w; L.s_rho = c->s_rho; L.eta = c->eta; L.xi = c->xi;
     L.block = c->ens.block; L.ppt = c->ens.ppt; L.occ = c->ens.occ;
+    L.sortTile = c->ens.sortTile;
+    L.sortPeriod = c->ens.sortPeriod;
+    L.tmp = c->ens.tmp;
+    L.bins = c->ens.d_bins;
     return L;
 }
 
@@ -1098,7 +1114,9 @@ int wc_ens_set(wc_ctx *c, const wc_particle_data *p, const uint8_t *scenario, si
     const size_t cap = n ? n : 1;
     int rc = alloc_ens_soa(c, E.s, cap);
     if (!rc) rc = alloc_ens_soa(c, E.save, cap);
+    if (!rc) rc = alloc_ens_soa(c, E.tmp, cap);
     if (rc) { free_ens(c); return rc; }
+    HIPCHK(c, hipMalloc(&E.d_bins, ens_max_bins(c) * sizeof(uint32_t)));
     HIPCHK(c, hipMalloc(&E.d_sigma, n_scenarios * sizeof(float)));
     HIPCHK(c, hipMalloc(&E.d_surface, (size_t)n_scenarios * plane * sizeof(uint32_t)));
     HIPCHK(c, hipMalloc(&E.d_steps, kEnsMaxSteps * sizeof(EnsStep)));
@@ -1175,7 +1193,9 @@ int wc_ens_release(wc_ctx *c, const wc_ens_source *sources, int n_sources, const
     const size_t cap = n ? n : 1;
     int rc = alloc_ens_soa(c, E.s, cap);
     if (!rc) rc = alloc_ens_soa(c, E.save, cap);
+    if (!rc) rc = alloc_ens_soa(c, E.tmp, cap);
     if (rc) { free_ens(c); return rc; }
+    HIPCHK(c, hipMalloc(&E.d_bins, ens_max_bins(c) * sizeof(uint32_t)));
     HIPCHK(c, hipMalloc(&E.d_sigma, n_scenarios * sizeof(float)));
     HIPCHK(c, hipMalloc(&E.d_surface, (size_t)n_scenarios * plane * sizeof(uint32_t)));
     HIPCHK(c, hipMalloc(&E.d_steps, kEnsMaxSteps * sizeof(EnsStep)));
@@ -1214,6 +1234,23 @@ int wc_ens_set_launch(wc_ctx *c, int block, int ppt, int occ) {
     return WC_OK;
 }
 
+int wc_ens_set_sort(wc_ctx *c, int tile, int period) {
+    if (!c) return WC_ERR_INVALID;
+    if ((tile != 0 && (tile < kMinSortTile || tile > 64)) || period < 1)
+        return fail(c, WC_ERR_INVALID, "sort tile %d (0 or %d..64), period %d (>= 1)", tile, kMinSortTile, period);
+    free_ens_graph(c);
+    c->ens.sortTile = tile;
+    c->ens.sortPeriod = period;
+    return WC_OK;
+}
+
+int wc_ens_get_sort(wc_ctx *c, int *tile, int *period) {
+    if (!c) return WC_ERR_INVALID;
+    if (tile) *tile = c->ens.sortTile;
+    if (period) *period = c->ens.sortPeriod;
+    return WC_OK;
+}
+
 int wc_ens_get_launch(wc_ctx *c, int *block, int *ppt, int *occ) {
     if (!c) return WC_ERR_INVALID;
     if (block) *block = c->ens.block;
@@ -1237,12 +1274,14 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    const int blocks[3] = {64, 128, 256}, ppts[3] = {1, 2, 4}, occs[3] = {0, 3, 4};
+    const int blocks[3] = {64, 128, 256}, ppts[3] = {1, 2, 4}, occs[3] = {0, 3, 4}, tiles[4] = {0, 2, 4, 8};
+    constexpr int kTrials = 3 * 3 * 3 * 4;
     double best = INFINITY;
-    int bb = E.block, bp = E.ppt, bo = E.occ;
-    for (int trial = -1; trial < 27 && !rc; ++trial) {
-        const int t = trial < 0 ? 26 : trial;                   // trial -1 warms up the last candidate
-        E.block = blocks[t % 3]; E.ppt = ppts[(t / 3) % 3]; E.occ = occs[t / 9];
+    int bb = E.block, bp = E.ppt, bo = E.occ, bt = E.sortTile;
+    E.sortPeriod = 1;
+    for (int trial = -1; trial < kTrials && !rc; ++trial) {
+        const int t = trial < 0 ? kTrials - 1 : trial;         // trial -1 warms up the last candidate
+        E.block = blocks[t % 3]; E.ppt = ppts[(t / 3) % 3]; E.occ = occs[(t / 9) % 3]; E.sortTile = tiles[t / 27];
         if ((rc = copy_ens_soa(c, E.s, E.save, E.n))) break;
         if ((rc = upload_ens_steps(c, cfg, times, slots, 1))) break;
         const EnsLaunch L = ens_launch(c, false);
@@ -1253,7 +1292,7 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
         }
         float ms = 0;
         hipEventElapsedTime(&ms, e0, e1);
-        if (trial >= 0 && ms < best) { best = ms; bb = E.block; bp = E.ppt; bo = E.occ; }
+        if (trial >= 0 && ms < best) { best = ms; bb = E.block; bp = E.ppt; bo = E.occ; bt = E.sortTile; }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -1261,7 +1300,7 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (rc) return rc;
     free_ens_graph(c);
-    E.block = bb; E.ppt = bp; E.occ = bo;
+    E.block = bb; E.ppt = bp; E.occ = bo; E.sortTile = bt;
     E.tuned = true;
     if (best_ms) *best_ms = best;
     return WC_OK;
@@ -1360,7 +1399,20 @@ int wc_ens_get(wc_ctx *c, wc_particle_data *out, uint8_t *scenario, size_t n) {
     HIPCHK(c, hipMemcpyAsync(time.data(), S.time, n * 8, D2H, c->stream));
     HIPCHK(c, hipMemcpyAsync(sc.data(), S.scen, n, D2H, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t q = 0; q < n; ++q) {
+    // id order: the cell-tile sort leaves the population in an order of its own
+    std::vector<size_t> ord(n);
+    bool dense = c->ens.n == n;
+    std::vector<char> seen(dense ? n : 0, 0);
+    for (size_t q = 0; q < n && dense; ++q) {
+        if (id[q] >= n || seen[id[q]]) dense = false;
+        else { seen[id[q]] = 1; ord[id[q]] = q; }
+    }
+    if (!dense) {
+        for (size_t q = 0; q < n; ++q) ord[q] = q;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return id[a] < id[b]; });
+    }
+    for (size_t r = 0; r < n; ++r) {
+        const size_t q = ord[r];
         wc_particle_data d{};
         d.id = id[q];
         const bool valid = cell[q] != 0xFFFFFFFFu;
@@ -1371,8 +1423,8 @@ int wc_ens_get(wc_ctx *c, wc_particle_data *out, uint8_t *scenario, size_t n) {
         d.health = health[q];
         d.age = age[q];
         d.time = time[q];
-        out[q] = d;
-        if (scenario) scenario[q] = sc[q];
+        out[r] = d;
+        if (scenario) scenario[r] = sc[q];
     }
     return WC_OK;
 }

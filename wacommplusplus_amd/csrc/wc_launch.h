@@ -85,6 +85,11 @@ struct EnsLaunch {
     unsigned long long *stepCounter;
     int s_w, s_rho, eta, xi;
     int block, ppt, occ;
+    // cell-tile sort before the move every sortPeriod-th step (sortTile = 0:
+    // never); tmp = scratch population, bins = (tiles + 1) counters
+    int sortTile, sortPeriod;
+    EnsSoA tmp;
+    uint32_t *bins;
 };
 hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream);
 // One release point of a device-generated ensemble population.

@@ -592,6 +592,116 @@ __global__ __launch_bounds__(256) void ens_surface_kernel(const EnsStep *__restr
 
 __global__ void ens_advance_kernel(int *stepIdx) { stepIdx[0] += 1; }
 
+// ---- cell-tile sort of the ensemble population -----------------------------
+//
+// Every `period`-th step (read from the device step counter, so the captured
+// step stays the same) the population is reordered by the tile of its cell
+// (tile x tile cells, row-major tiles; a particle outside the grid goes
+// last): a histogram, an exclusive scan and a scatter into scratch, then a
+// copy back.  Lanes of a wave then gather from neighbouring cells.  Every
+// result is per particle (draws keyed by id, integer counts), so the order
+// changes no result; the order inside a tile is not fixed.
+
+__device__ __forceinline__ bool sort_step(const int *stepIdx, int period) {
+    return period > 0 && (*stepIdx % period) == 0;
+}
+
+__device__ __forceinline__ uint32_t tile_key(uint32_t cell, int xi, int tile, int tilesX, uint32_t nTiles) {
+    if (cell == 0xFFFFFFFFu) return nTiles;
+    const uint32_t j = cell / (uint32_t)xi, i = cell - j * (uint32_t)xi;
+    return (j / (uint32_t)tile) * (uint32_t)tilesX + i / (uint32_t)tile;
+}
+
+__global__ void ens_sort_clear_kernel(const int *stepIdx, int period, uint32_t *bins, uint32_t nBins) {
+    if (!sort_step(stepIdx, period)) return;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nBins; q += gridDim.x * blockDim.x) bins[q] = 0;
+}
+
+// histogram with equal keys of a wave added by one atomic
+__global__ __launch_bounds__(256) void ens_sort_hist_kernel(const int *stepIdx, int period, const uint32_t *cell,
+                                                            int n, int xi, int tile, int tilesX, uint32_t nTiles,
+                                                            uint32_t *bins) {
+    if (!sort_step(stepIdx, period)) return;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = idx < n;
+    const uint32_t key = active ? tile_key(cell[idx], xi, tile, tilesX, nTiles) : 0;
+    for (;;) {
+        const unsigned long long m = __ballot(active);
+        if (m == 0) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const bool same = active && key == lk;
+        const unsigned long long sm = __ballot(same);
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&bins[lk], (uint32_t)__popcll(sm));
+        if (same) active = false;
+    }
+}
+
+// exclusive scan of nBins counts in place, one block of 1024
+__global__ __launch_bounds__(1024) void ens_sort_scan_kernel(const int *stepIdx, int period, uint32_t *bins,
+                                                             uint32_t nBins) {
+    if (!sort_step(stepIdx, period)) return;
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x, per = (nBins + 1023) / 1024, b0 = t * per;
+    uint32_t s = 0;
+    for (uint32_t q = 0; q < per && b0 + q < nBins; ++q) s += bins[b0 + q];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t == 0 ? 0 : part[t - 1];
+    for (uint32_t q = 0; q < per && b0 + q < nBins; ++q) {
+        const uint32_t c = bins[b0 + q];
+        bins[b0 + q] = run;
+        run += c;
+    }
+}
+
+// scatter into dst at bins[key] (advanced per group of equal keys in a wave)
+__global__ __launch_bounds__(256) void ens_sort_scatter_kernel(const int *stepIdx, int period, EnsSoA src,
+                                                               EnsSoA dst, int n, int xi, int tile, int tilesX,
+                                                               uint32_t nTiles, uint32_t *bins) {
+    if (!sort_step(stepIdx, period)) return;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool active = idx < n;
+    const uint32_t key = active ? tile_key(src.cell[idx], xi, tile, tilesX, nTiles) : 0;
+    uint32_t pos = 0;
+    for (;;) {
+        const unsigned long long m = __ballot(active);
+        if (m == 0) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const bool same = active && key == lk;
+        const unsigned long long sm = __ballot(same);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&bins[lk], (uint32_t)__popcll(sm));
+        base = __shfl(base, leader, 64);
+        if (same) {
+            pos = base + (uint32_t)__popcll(sm & ((1ull << lane) - 1ull));
+            active = false;
+        }
+    }
+    if (idx < n) {
+        dst.id[pos] = src.id[idx]; dst.cell[pos] = src.cell[idx]; dst.fj[pos] = src.fj[idx];
+        dst.fi[pos] = src.fi[idx]; dst.k[pos] = src.k[idx]; dst.health[pos] = src.health[idx];
+        dst.age[pos] = src.age[idx]; dst.time[pos] = src.time[idx]; dst.scen[pos] = src.scen[idx];
+    }
+}
+
+__global__ void ens_sort_copy_kernel(const int *stepIdx, int period, EnsSoA src, EnsSoA dst, int n) {
+    if (!sort_step(stepIdx, period)) return;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        dst.id[q] = src.id[q]; dst.cell[q] = src.cell[q]; dst.fj[q] = src.fj[q]; dst.fi[q] = src.fi[q];
+        dst.k[q] = src.k[q]; dst.health[q] = src.health[q]; dst.age[q] = src.age[q]; dst.time[q] = src.time[q];
+        dst.scen[q] = src.scen[q];
+    }
+}
+
 // Device release of an ensemble population: particle q (global id q) belongs
 // to the source s with first[s] <= q < first[s+1]; its j and i are offset by
 // spread * N(0, 1) draws of Philox4x32-10 keyed on (seed; q, 0xE5, 0, 0), so a
@@ -683,10 +793,27 @@ Packed<T> packed_view(void *packBuf, const GridPtrs &G, int s_w, int xi, size_t 
     return S;
 }
 
+hipError_t ens_sort(const EnsLaunch &L, hipStream_t stream) {
+    if (L.sortTile <= 0 || L.sortPeriod <= 0 || L.n <= 0) return hipSuccess;
+    const int tilesX = (L.xi + L.sortTile - 1) / L.sortTile, tilesY = (L.eta + L.sortTile - 1) / L.sortTile;
+    const uint32_t nTiles = (uint32_t)tilesX * (uint32_t)tilesY, nBins = nTiles + 1;
+    const int nb = (L.n + 255) / 256;
+    ens_sort_clear_kernel<<<(nBins + 255) / 256, 256, 0, stream>>>(L.stepIdx, L.sortPeriod, L.bins, nBins);
+    ens_sort_hist_kernel<<<nb, 256, 0, stream>>>(L.stepIdx, L.sortPeriod, L.s.cell, L.n, L.xi, L.sortTile, tilesX,
+                                                 nTiles, L.bins);
+    ens_sort_scan_kernel<<<1, 1024, 0, stream>>>(L.stepIdx, L.sortPeriod, L.bins, nBins);
+    ens_sort_scatter_kernel<<<nb, 256, 0, stream>>>(L.stepIdx, L.sortPeriod, L.s, L.tmp, L.n, L.xi, L.sortTile,
+                                                    tilesX, nTiles, L.bins);
+    ens_sort_copy_kernel<<<nb < 16384 ? nb : 16384, 256, 0, stream>>>(L.stepIdx, L.sortPeriod, L.tmp, L.s, L.n);
+    return hipGetLastError();
+}
+
 template <class T>
 hipError_t ens_launch_t(const EnsLaunch &L, hipStream_t stream) {
     const size_t plane = (size_t)L.eta * L.xi;
     const Packed<T> S = packed_view<T>(L.packBuf, L.G, L.s_w, L.xi, plane);
+    hipError_t se = ens_sort(L, stream);
+    if (se != hipSuccess) return se;
     ens_pack_kernel<T><<<grid_blocks(plane * (L.s_w - 1)), 256, 0, stream>>>(
         L.steps, L.stepIdx, L.G.hf, (UV4<T> *)S.uv, (WA8<T> *)S.wa, (ZH *)S.z, L.s_rho, L.s_w, plane);
     hipError_t e = hipGetLastError();
