@@ -236,6 +236,26 @@ int wc_set_roms_grid(wc_ctx *ctx, const wc_roms_grid *grid, int n_slots, int sto
 int wc_upload_roms_record(wc_ctx *ctx, int slot, const float *zeta, const float *u, const float *v,
                           const float *w, const float *akt);
 
+/* The geographic grid the particle records need: lat, lon (degrees)
+ * [eta][xi] and the s_w W-level sigma values bottom first (the
+ * OceanModelAdapter's Lat, Lon and SW), host pointers; the bathymetry is the
+ * grid's h.  wc_set_roms_grid sets it from lon_rho, lat_rho and s_w. */
+int wc_set_geo(wc_ctx *ctx, const double *lat, const double *lon, const double *s_w);
+
+/* Particles::netcdfRecords (Particles.cpp:115-147) on the device: the
+ * context's particles with j >= 0, i >= 0, k <= 0 that
+ * OceanModelAdapter::kji2deplatlon (OceanModelAdapter.cpp:271-321) places
+ * inside the interpolation domain, in population order, with their depth
+ * (negative down), lat and lon by bilinear interpolation; bit-identical to
+ * the reference's FP64 arithmetic.  *count receives the number of records;
+ * with out non-NULL the non-NULL fields of out (host arrays of capacity
+ * entries) receive them.  WC_ERR_INVALID when capacity is too small. */
+typedef struct wc_records {
+    uint64_t *id;
+    double *lat, *lon, *depth, *k, *j, *i, *health, *age, *time;
+} wc_records;
+int wc_particle_records(wc_ctx *ctx, const wc_records *out, size_t capacity, size_t *count);
+
 /* The grid checks of OceanModelAdapter::appendBoundaryRecord
  * (OceanModelAdapter.cpp:199-214) for the next file of a run, against the
  * grid of the last wc_set_roms_grid: same dimensions; s_rho values (when both
@@ -498,6 +518,54 @@ typedef struct wc_conc_file {
     const char *const *global_attributes;
 } wc_conc_file;
 int wc_write_concentration_nc(const char *path, const wc_conc_file *file);
+
+/* A particle restart/history file, restart version 3, as
+ * Particles::saveAsNetCDF writes it (Particles.cpp:150-264): the global
+ * attributes global_attributes (provenance, Provenance.hpp:19-29; a
+ * wacomm_random_key_scheme of ordered-splitmix64-v2 is added unless given),
+ * wacomm_restart_version "3", tracking_direction, ocean_model,
+ * checkpoint_time (double) and random_seed (uint64); dimensions particles (n)
+ * and particle_time (1); variables particle_time, id (uint64), object_type
+ * (uint), drift_side (int), lat, lon, depth, k, j, i, health, age, time
+ * [particles][particle_time] (double) with the reference's attributes.  The
+ * reference writes NetCDF-4 (HDF5); this file is the classic format with
+ * 64-bit data (CDF-5), which holds uint64 and which the NetCDF library reads
+ * through the same calls.  object_type / drift_side NULL are written as 0 (the
+ * plain method's drift state).  An empty history (n = 0) keeps zero-length
+ * variables.  Written to <path>.partial, then renamed.  Host-only.  Returns
+ * WC_ERR_INVALID for a missing field or a global attribute that repeats
+ * another, WC_ERR_IO when the file cannot be written. */
+typedef struct wc_restart_file {
+    size_t n;
+    double particle_time;         /* the particle_time variable and checkpoint_time */
+    uint64_t random_seed;
+    int backward;                 /* tracking_direction: 1 "backward", 0 "forward" */
+    const char *ocean_model;
+    const char *const *global_attributes;
+    const uint64_t *id;
+    const uint32_t *object_type;
+    const int32_t *drift_side;
+    const double *lat, *lon, *depth, *k, *j, *i, *health, *age, *time;
+} wc_restart_file;
+int wc_write_restart_nc(const char *path, const wc_restart_file *file);
+
+/* Particles::loadFromNetCDF (Particles.cpp:373-480) for NetCDF classic files
+ * (CDF-1, CDF-2, CDF-5).  When the file has wacomm_restart_version, checks it
+ * as the reference does against the configuration (cfg->trackingDirection,
+ * randomSeed, driftModel == 1 for leeway) and ocean_model: version 2 or 3,
+ * version 2 not with leeway, the tracking direction, random_seed,
+ * wacomm_random_key_scheme and ocean_model when present; checkpoint_time, when
+ * present, is stored in *checkpoint_time (the reference's
+ * Config::RestartCheckpoint), which is otherwise left unchanged.  Reads id, k,
+ * j, i, health, age, time (the first particle_time column) and, when present,
+ * object_type and drift_side into out[0..n) in file order.  *count receives n;
+ * out NULL only counts.  err (err_len bytes, may be NULL) receives the message.
+ * Returns WC_ERR_INVALID with the reference's message where it throws,
+ * WC_ERR_UNSUPPORTED for a NetCDF-4 (HDF5) file, WC_ERR_IO for a file that
+ * cannot be read or is not NetCDF. */
+int wc_read_restart_nc(const char *path, const wc_config_data *cfg, const char *ocean_model,
+                       wc_particle_data *out, size_t capacity, size_t *count, double *checkpoint_time, char *err,
+                       size_t err_len);
 
 #ifdef __cplusplus
 }

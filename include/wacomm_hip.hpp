@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iomanip>
 #include <iterator>
@@ -192,6 +193,75 @@ public:
             count++;
         }
     }
+
+    /* Particles::saveAsNetCDF (Particles.cpp:150-264): restart version 3 of
+       the particles inside the interpolation domain (netcdfRecords, with
+       depth, lat and lon from kji2deplatlon computed on the device), in the
+       NetCDF classic format with 64-bit data.  ctx holds the grid and its
+       geographic fields (wc_set_roms_grid, or wc_set_grid + wc_set_geo); its
+       population is replaced by these particles.  provenance: NULL or a
+       NULL-terminated list of name, value strings. */
+    void saveAsNetCDF(Context &ctx, const std::string &fileName, double particleTime, const wc_config_data &config,
+                      const std::string &oceanModel, const char *const *provenance = nullptr) const {
+        wc_ctx *c = ctx.get();
+        // the records depend on position only; the drift state (which the
+        // device population does not carry) is taken from the host below
+        std::vector<wc_particle_data> pos(raw(), raw() + size());
+        for (wc_particle_data &d : pos) d.driftObjectType = 0, d.driftSide = 0;
+        ctx.check(wc_set_particles(c, pos.data(), pos.size()), "wc_set_particles");
+        size_t n = 0;
+        ctx.check(wc_particle_records(c, nullptr, 0, &n), "wc_particle_records");
+        std::vector<std::uint64_t> id(n);
+        std::vector<double> f[9];
+        for (auto &v : f) v.resize(n);
+        wc_records r{id.data(), f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(),
+                     f[5].data(), f[6].data(), f[7].data(), f[8].data()};
+        size_t m = 0;
+        ctx.check(wc_particle_records(c, &r, n, &m), "wc_particle_records");
+        // drift state of each record's particle: records are an order-keeping
+        // subsequence whose id, k, j, i equal their particle's
+        std::vector<std::uint32_t> object(n);
+        std::vector<std::int32_t> side(n);
+        size_t q = 0;
+        for (size_t k = 0; k < n; ++k) {
+            while (q < size() && !((*this)[q].Id() == id[k] && same((*this)[q].K(), f[3][k]) &&
+                                   same((*this)[q].J(), f[4][k]) && same((*this)[q].I(), f[5][k])))
+                ++q;
+            if (q == size()) throw std::runtime_error("record without its particle");
+            object[k] = (std::uint32_t)(*this)[q].DriftObject();
+            side[k] = (*this)[q].Side();
+            ++q;
+        }
+        wc_restart_file file{n, particleTime, config.randomSeed, config.trackingDirection == -1 ? 1 : 0,
+                             oceanModel.c_str(), provenance, id.data(), object.data(), side.data(),
+                             f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(), f[5].data(),
+                             f[6].data(), f[7].data(), f[8].data()};
+        const int rc = wc_write_restart_nc(fileName.c_str(), &file);
+        if (rc != WC_OK) throw Error(rc, "cannot write " + fileName);
+    }
+
+    /* Particles::loadFromNetCDF (Particles.cpp:373-480): appends the file's
+       particles after the reference's checks of its restart metadata against
+       config and oceanModel (std::runtime_error with the reference's message);
+       a checkpoint_time in the file becomes config.restartCheckpoint. */
+    void loadFromNetCDF(const std::string &fileName, wc_config_data &config, const std::string &oceanModel) {
+        size_t n = 0;
+        double checkpoint = config.restartCheckpoint;
+        char err[512];
+        int rc = wc_read_restart_nc(fileName.c_str(), &config, oceanModel.c_str(), nullptr, 0, &n, &checkpoint,
+                                    err, sizeof err);
+        std::vector<wc_particle_data> p(n);
+        if (rc == WC_OK && n)
+            rc = wc_read_restart_nc(fileName.c_str(), &config, oceanModel.c_str(), p.data(), n, &n, &checkpoint,
+                                    err, sizeof err);
+        if (rc == WC_ERR_INVALID) throw std::runtime_error(err);
+        if (rc != WC_OK) throw Error(rc, err);
+        config.restartCheckpoint = checkpoint;
+        for (const wc_particle_data &d : p) push_back(Particle(d));
+    }
+
+private:
+    static bool same(double a, double b) { return std::memcmp(&a, &b, sizeof a) == 0; }
 };
 
 /* The forcing window (oceanmodel_data, OceanModelAdapter.hpp:20-34), host

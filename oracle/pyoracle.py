@@ -142,6 +142,9 @@ def _load(path):
     lib.wco_roms_normalize.restype = C.c_int
     lib.wco_roms_normalize.argtypes = [C.POINTER(_RomsGrid)] + [C.c_void_p] * 10
     lib.wco_roms_static.argtypes = [C.c_int] * 3 + [C.c_void_p] * 9
+    lib.wco_kji2deplatlon.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [C.c_void_p] * 7
+    lib.wco_particle_records.restype = C.c_long
+    lib.wco_particle_records.argtypes = [C.c_void_p, C.c_long] + [C.c_int] * 3 + [C.c_void_p] * 8
     return lib
 
 
@@ -306,3 +309,28 @@ def roms_static(s_w_values, lon, lat, h, mask_rho):
     lib().wco_roms_static(len(sw) - 1, eta, xi, _ptr(sw), _ptr(lon), _ptr(lat), _ptr(h), _ptr(m), _ptr(lr),
                           _ptr(la), _ptr(ho), _ptr(di))
     return lr, la, ho, di
+
+
+def kji2deplatlon(k, j, i, s_w_values, lat, lon, h):
+    """(depth, lat, lon) of OceanModelAdapter::kji2deplatlon (OceanModelAdapter.cpp:271-321)."""
+    sw = np.ascontiguousarray(s_w_values, dtype=np.float64)
+    lat, lon, h = (np.ascontiguousarray(a, dtype=np.float64) for a in (lat, lon, h))
+    eta, xi = h.shape
+    d, la, lo = C.c_double(), C.c_double(), C.c_double()
+    lib().wco_kji2deplatlon(float(k), float(j), float(i), len(sw), eta, xi, _ptr(sw), _ptr(lat), _ptr(lon),
+                            _ptr(h), C.byref(d), C.byref(la), C.byref(lo))
+    return d.value, la.value, lo.value
+
+
+def particle_records(particles: np.ndarray, s_w_values, lat, lon, h):
+    """Particles::netcdfRecords (Particles.cpp:115-147): (src indices, depth, lat, lon)."""
+    particles = np.ascontiguousarray(particles)
+    sw = np.ascontiguousarray(s_w_values, dtype=np.float64)
+    lat, lon, h = (np.ascontiguousarray(a, dtype=np.float64) for a in (lat, lon, h))
+    eta, xi = h.shape
+    n = len(particles)
+    src = np.zeros(max(n, 1), np.int64)
+    dep, la, lo = (np.zeros(max(n, 1)) for _ in range(3))
+    r = lib().wco_particle_records(_ptr(particles), n, len(sw), eta, xi, _ptr(sw), _ptr(lat), _ptr(lon), _ptr(h),
+                                   _ptr(src), _ptr(dep), _ptr(la), _ptr(lo))
+    return src[:r], dep[:r], la[:r], lo[:r]

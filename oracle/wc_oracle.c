@@ -662,3 +662,62 @@ void wco_roms_static(int s_rho, int eta, int xi, const double *s_w_values, const
         latRad[c] = 0.0174533 * lat[c];
     }
 }
+
+/* (int)floor(x) as x86-64 converts it: values outside int range and NaN give
+ * INT_MIN (the reference's static_cast<int>(std::floor(x)),
+ * NumericalHelpers.hpp:20-26). */
+static int cell_of(double x) {
+    const double f = floor(x);
+    return (f >= -2147483648.0 && f < 2147483648.0) ? (int)f : INT32_MIN;
+}
+
+static int level_of(double x) {
+    const double c = ceil(x);
+    return (c >= -2147483648.0 && c < 2147483648.0) ? (int)c : INT32_MIN;
+}
+
+void wco_kji2deplatlon(double k, double j, double i, int s_w, int eta, int xi, const double *sW, const double *lat,
+                       const double *lon, const double *h, double *dep, double *olat, double *olon) {
+    const int kI = level_of(k);                          /* upperVerticalLevel */
+    const double kF = ceil(k) - k;                       /* lowerVerticalWeight */
+    const int jI = cell_of(j);
+    const double jF = j - jI;
+    const int iI = cell_of(i);
+    const double iF = i - iI;
+    /* validInterpolationCell (NumericalHelpers.hpp:12-14) and the level range
+       (OceanModelAdapter.cpp:284-291) */
+    if (!(jI >= 0 && iI >= 0 && (long)jI + 1 < eta && (long)iI + 1 < xi) || kI > 0 || kI <= -s_w + 1) {
+        *dep = *olat = *olon = 1e37;
+        return;
+    }
+    const size_t c00 = (size_t)jI * xi + iI, c10 = c00 + xi, c11 = c10 + 1, c01 = c00 + 1;
+    /* :293-320, each term left to right, summed in order */
+    *olon = lon[c00] * (1.0 - iF) * (1.0 - jF) + lon[c10] * (1.0 - iF) * jF + lon[c11] * iF * jF +
+            lon[c01] * iF * (1.0 - jF);
+    *olat = lat[c00] * (1.0 - iF) * (1.0 - jF) + lat[c10] * (1.0 - iF) * jF + lat[c11] * iF * jF +
+            lat[c01] * iF * (1.0 - jF);
+    const double hh = h[c00] * (1.0 - iF) * (1.0 - jF) + h[c10] * (1.0 - iF) * jF + h[c11] * iF * jF +
+                      h[c01] * iF * (1.0 - jF);
+    const double sLo = sW[kI - 1 + s_w - 1], sUp = sW[kI + s_w - 1];
+    const double aDep = fabs(fabs(hh * sLo) - fabs(hh * sUp));           /* :315-318 */
+    *dep = -(hh * fabs(sUp) + fabs(kF * aDep));                          /* :319 */
+}
+
+long wco_particle_records(const wco_particle *ps, long n, int s_w, int eta, int xi, const double *sW,
+                          const double *lat, const double *lon, const double *h, long *src, double *dep,
+                          double *olat, double *olon) {
+    long r = 0;
+    for (long q = 0; q < n; q++) {
+        const wco_particle *p = &ps[q];
+        if (p->j < 0 || p->i < 0 || p->k > 0) continue;                /* Particles.cpp:121-123 */
+        double d, la, lo;
+        wco_kji2deplatlon(p->k, p->j, p->i, s_w, eta, xi, sW, lat, lon, h, &d, &la, &lo);
+        if (d == 1e37 || la == 1e37 || lo == 1e37) continue;          /* :128-130 */
+        src[r] = q;
+        dep[r] = d;
+        olat[r] = la;
+        olon[r] = lo;
+        r++;
+    }
+    return r;
+}

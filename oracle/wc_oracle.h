@@ -31,6 +31,12 @@
  *     show that each pin catches a plausible error.
  *   - wco_grid_count (Concentration::evaluate): tests/ConcentrationTest.cpp
  *     and brute force on tiny inputs.
+ *   - wco_kji2deplatlon / wco_particle_records: the inverse deplatlon2kji
+ *     fixture of tests/ROMSAdapterTest.cpp (source at 1 m over h = 50 m with
+ *     s_w spacing 0.1 is k = -0.2), closed forms (a rectilinear grid gives
+ *     lat/lon linear in j/i exactly at nodes; depth -h|sW(k)| at integer k and
+ *     linear in k between), the sentinel cases of the reference's range checks,
+ *     and order preservation.
  */
 #ifndef WC_ORACLE_H
 #define WC_ORACLE_H
@@ -189,6 +195,23 @@ int wco_roms_normalize(const wco_roms_grid *g, const float *zeta, const float *u
 void wco_roms_static(int s_rho, int eta, int xi, const double *s_w_values, const double *lon, const double *lat,
                      const double *h, const double *mask_rho, double *lonRad, double *latRad, double *h_out,
                      double *depthIntervals);
+
+/* OceanModelAdapter::kji2deplatlon (OceanModelAdapter.cpp:271-321): the
+ * depth (negative down), latitude and longitude of fractional position
+ * (k, j, i) by bilinear interpolation of lat, lon, h [eta][xi] and the W-level
+ * sigma values sW (s_w values, bottom first: logical level k is storage index
+ * k+s_w-1).  Outside the interpolation cell range or the level range all three
+ * are 1e37. */
+void wco_kji2deplatlon(double k, double j, double i, int s_w, int eta, int xi, const double *sW, const double *lat,
+                       const double *lon, const double *h, double *dep, double *olat, double *olon);
+
+/* Particles::netcdfRecords (Particles.cpp:115-147): the particles with
+ * j >= 0, i >= 0, k <= 0 whose kji2deplatlon is not 1e37, in input order.
+ * Record r comes from particle src[r] and has depth dep[r], lat[r], lon[r].
+ * Returns the number of records. */
+long wco_particle_records(const wco_particle *ps, long n, int s_w, int eta, int xi, const double *sW,
+                          const double *lat, const double *lon, const double *h, long *src, double *dep,
+                          double *olat, double *olon);
 
 /* Haversine cell sizes of Particle.cpp:548-574 for cell (j,i): dx toward
  * (j,i+1), dy toward (j+1,i). */

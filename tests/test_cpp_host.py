@@ -30,3 +30,18 @@ def test_wacomm_run_and_concentration(tmp_path):
     """Wacomm::run and Concentration::evaluate of the C++ layer on the
     device: uniform-current closed form and exact counts."""
     run("wacomm_run_test", tmp_path)
+
+
+def test_particles_netcdf_restart(tmp_path):
+    """Particles::loadFromNetCDF (Particles.cpp:373-480) of the C++ layer on
+    files written through the C ABI: 64-bit ids, full precision, drift state,
+    checkpoint hand-over and the reference's rejections (CPU only)."""
+    run("particles_nc_test", tmp_path)
+
+
+@pytest.mark.gpu
+def test_particles_netcdf_restart_from_device(tmp_path):
+    """Particles::saveAsNetCDF with the records computed on the device, read
+    back with loadFromNetCDF: kept set and order, drift state, closed-form
+    lat, lon and depth."""
+    run("particles_nc_device_test", tmp_path)

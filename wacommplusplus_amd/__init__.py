@@ -11,7 +11,7 @@ from .types import (BACKWARD, CONSTRAINT, EULER, FORWARD, KILL, MIDPOINT, PARTIC
 from ._lib import (EXPORTS, FORCING_INTERVAL_BATCH, LIB_PATH, ROMS_MAX_MISSING, SINGLE_PULSE, UNIFORM_RATE,
                    WC_ERR_HIP, WC_ERR_INVALID, WC_ERR_NO_DEVICE, WC_ERR_NOMEM, WC_ERR_UNSUPPORTED, WC_OK, Context, EnsSource, EnsStats,
                    RunStats, Source, UNet, WcError, cell_metric_table, load_library, philox4x32_10,
-                   write_concentration_nc, WC_ERR_IO)
+                   write_concentration_nc, WC_ERR_IO, RECORD_FIELDS, read_restart_nc, write_restart_nc)
 
 __all__ = [
     "BACKWARD", "CONSTRAINT", "EULER", "FORWARD", "KILL", "MIDPOINT", "PARTICLE_DTYPE", "REFLECTION",
@@ -19,4 +19,5 @@ __all__ = [
     "WcError", "cell_metric_table", "load_library", "philox4x32_10", "Source", "RunStats", "UNIFORM_RATE",
     "FORCING_INTERVAL_BATCH", "SINGLE_PULSE", "ROMS_MAX_MISSING", "WC_OK", "WC_ERR_HIP", "WC_ERR_INVALID",
     "WC_ERR_UNSUPPORTED", "WC_ERR_NOMEM", "WC_ERR_NO_DEVICE", "EnsStats", "EnsSource", "UNet", "write_concentration_nc", "WC_ERR_IO",
+    "RECORD_FIELDS", "read_restart_nc", "write_restart_nc",
 ]

@@ -30,7 +30,8 @@ EXPORTS = (
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
     "wc_ens_get", "wc_ens_release", "wc_ens_set_sort", "wc_ens_get_sort", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
-    "wc_write_concentration_nc",
+    "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
+    "wc_particle_records",
 )
 
 
@@ -71,6 +72,22 @@ class ConcFile(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("T", "s_rho", "eta", "xi")] + \
                [(n, C.c_void_p) for n in ("ocean_time", "s_rho_values", "lat_rho", "lon_rho", "mask_rho", "h",
                                           "zeta", "conc", "global_attributes")]
+
+
+class RestartFile(C.Structure):
+    """wc_restart_file: a restart-version-3 particle file (Particles.cpp:150-264)."""
+    _fields_ = [("n", C.c_size_t), ("particle_time", C.c_double), ("random_seed", C.c_uint64),
+                ("backward", C.c_int), ("ocean_model", C.c_char_p), ("global_attributes", C.c_void_p),
+                ("id", C.c_void_p), ("object_type", C.c_void_p), ("drift_side", C.c_void_p)] + \
+               [(n, C.c_void_p) for n in ("lat", "lon", "depth", "k", "j", "i", "health", "age", "time")]
+
+
+RECORD_FIELDS = ("id", "lat", "lon", "depth", "k", "j", "i", "health", "age", "time")
+
+
+class Records(C.Structure):
+    """wc_records: host arrays receiving Particles::netcdfRecords."""
+    _fields_ = [(n, C.c_void_p) for n in RECORD_FIELDS]
 
 
 class EnsSource(C.Structure):
@@ -143,6 +160,8 @@ def load_library(path: str = LIB_PATH):
         "wc_ens_release": [vp, vp, i, vp, i, C.c_uint64],
         "wc_ens_set_sort": [vp, i, i],
         "wc_ens_get_sort": [vp, C.POINTER(i), C.POINTER(i)],
+        "wc_set_geo": [vp, vp, vp, vp],
+        "wc_particle_records": [vp, C.POINTER(Records), sz, C.POINTER(sz)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -175,6 +194,11 @@ def load_library(path: str = LIB_PATH):
     L.wc_unet_param_count.restype = C.c_size_t
     L.wc_write_concentration_nc.argtypes = [C.c_char_p, C.POINTER(ConcFile)]
     L.wc_write_concentration_nc.restype = C.c_int
+    L.wc_write_restart_nc.argtypes = [C.c_char_p, C.POINTER(RestartFile)]
+    L.wc_write_restart_nc.restype = C.c_int
+    L.wc_read_restart_nc.argtypes = [C.c_char_p, C.POINTER(ConfigData), C.c_char_p, vp, sz, C.POINTER(sz),
+                                     C.POINTER(d), C.c_char_p, sz]
+    L.wc_read_restart_nc.restype = C.c_int
     _lib = L
     return L
 
@@ -326,6 +350,24 @@ class Context:
         n = self.count()
         out = np.zeros(n, dtype=PARTICLE_DTYPE)
         self._chk(self.L.wc_get_particles(self.h, _p(out), n))
+        return out
+
+    def set_geo(self, lat, lon, s_w_values):
+        """The grid's lat, lon (degrees, [eta][xi]) and s_w values (bottom first)
+        used by particle_records."""
+        la, lo, sw = _f64(lat), _f64(lon), _f64(s_w_values)
+        self._chk(self.L.wc_set_geo(self.h, _p(la), _p(lo), _p(sw)))
+
+    def particle_records(self) -> dict:
+        """Particles::netcdfRecords (Particles.cpp:115-147) computed on the
+        device: a dict of the RECORD_FIELDS arrays."""
+        n = C.c_size_t()
+        self._chk(self.L.wc_particle_records(self.h, None, 0, C.byref(n)))
+        out = {f: np.zeros(n.value, np.uint64 if f == "id" else np.float64) for f in RECORD_FIELDS}
+        r = Records(*[_p(out[f]).value for f in RECORD_FIELDS])
+        m = C.c_size_t()
+        self._chk(self.L.wc_particle_records(self.h, C.byref(r), n.value, C.byref(m)))
+        assert m.value == n.value
         return out
 
     # -- the hot path -------------------------------------------------------
@@ -564,3 +606,49 @@ def write_concentration_nc(path, ocean_time, s_rho_values, lat_rho, lon_rho, mas
     rc = L.wc_write_concentration_nc(str(path).encode(), C.byref(f))
     if rc:
         raise WcError(rc, f"writing {path}")
+
+
+def write_restart_nc(path, records: dict, particle_time: float, random_seed: int, backward: bool = False,
+                     ocean_model: str = "roms", attributes=None, object_type=None, drift_side=None):
+    """Particles::saveAsNetCDF's restart file (Particles.cpp:150-264), restart
+    version 3, in the NetCDF classic format with 64-bit data (CDF-5).
+    records: the RECORD_FIELDS arrays (as particle_records returns them)."""
+    L = load_library()
+    n = len(records["id"])
+    ids = np.ascontiguousarray(records["id"], dtype=np.uint64)
+    cols = {f: _f64(records[f]) for f in RECORD_FIELDS if f != "id"}
+    for a in cols.values():
+        assert a.shape == (n,)
+    ot = None if object_type is None else np.ascontiguousarray(object_type, dtype=np.uint32)
+    ds = None if drift_side is None else np.ascontiguousarray(drift_side, dtype=np.int32)
+    names = []
+    for k, v in (attributes or {}).items():
+        names += [str(k).encode(), str(v).encode()]
+    arr = (C.c_char_p * (len(names) + 2))(*names, None, None)
+    pt = lambda a: None if a is None or len(a) == 0 else _p(a).value
+    f = RestartFile(n, float(particle_time), int(random_seed), 1 if backward else 0, ocean_model.encode(),
+                    C.cast(arr, C.c_void_p).value, pt(ids), pt(ot), pt(ds),
+                    *[pt(cols[c]) for c in ("lat", "lon", "depth", "k", "j", "i", "health", "age", "time")])
+    rc = L.wc_write_restart_nc(str(path).encode(), C.byref(f))
+    if rc:
+        raise WcError(rc, f"writing {path}")
+
+
+def read_restart_nc(path, config: ConfigData, ocean_model: str = "roms"):
+    """Particles::loadFromNetCDF (Particles.cpp:373-480): (particles, checkpoint
+    time or None) from a NetCDF classic restart file, after the reference's
+    metadata checks against config and ocean_model."""
+    L = load_library()
+    n = C.c_size_t()
+    cp = C.c_double(float("nan"))
+    err = C.create_string_buffer(512)
+    rc = L.wc_read_restart_nc(str(path).encode(), C.byref(config), ocean_model.encode(), None, 0, C.byref(n),
+                              C.byref(cp), err, 512)
+    if rc:
+        raise WcError(rc, err.value.decode())
+    out = np.zeros(n.value, dtype=PARTICLE_DTYPE)
+    rc = L.wc_read_restart_nc(str(path).encode(), C.byref(config), ocean_model.encode(),
+                              _p(out).value if n.value else None, n.value, C.byref(n), C.byref(cp), err, 512)
+    if rc:
+        raise WcError(rc, err.value.decode())
+    return out, (None if np.isnan(cp.value) else cp.value)

@@ -68,6 +68,13 @@ struct wc_ctx {
     // duplicate-record comparison key
     unsigned long long *d_key = nullptr;
 
+    // geographic grid of the particle records (wc_set_geo): lat, lon
+    // [eta][xi], s_w values; the records' depth, lat, lon (3 x recCap)
+    bool haveGeo = false;
+    double *d_lat = nullptr, *d_lon = nullptr, *d_sw = nullptr;
+    double *d_rec = nullptr;
+    size_t recCap = 0;
+
     // batched ensemble (wc_ens_*): compact state, a saved copy for the
     // autotuner, scenario sigmas, per-scenario surface counts, the device
     // step array and counter read by the captured step, the graph
@@ -468,6 +475,9 @@ static void free_grid(wc_ctx *c) {
     hipFree(c->d_key);
     c->d_key = nullptr;
     c->haveRoms = false;
+    hipFree(c->d_lat); hipFree(c->d_lon); hipFree(c->d_sw);
+    c->d_lat = c->d_lon = c->d_sw = nullptr;
+    c->haveGeo = false;
     c->d_counts = nullptr;
     c->d_conc = nullptr;
     c->haveGrid = false;
@@ -484,6 +494,7 @@ int wc_ctx_destroy(wc_ctx *c) {
     hipFree(c->d_blockCounts);
     hipFree(c->d_total);
     hipFree(c->d_steps);
+    hipFree(c->d_rec);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     for (int k = 0; k < kTimedKernels; ++k)
@@ -674,6 +685,55 @@ int wc_set_roms_grid(wc_ctx *c, const wc_roms_grid *g, int n_slots, int store) {
     m.u[m.nu++] = (double)kNcFillFloat; m.u[m.nu++] = kNcFillDouble;
     m.v[m.nv++] = (double)kNcFillFloat; m.v[m.nv++] = kNcFillDouble;
     c->haveRoms = true;
+    return wc_set_geo(c, g->lat_rho, g->lon_rho, g->s_w);
+}
+
+int wc_set_geo(wc_ctx *c, const double *lat, const double *lon, const double *s_w) {
+    if (!c || !lat || !lon || !s_w) return WC_ERR_INVALID;
+    if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t plane = (size_t)c->eta * c->xi;
+    if (!c->d_lat) HIPCHK(c, hipMalloc(&c->d_lat, plane * sizeof(double)));
+    if (!c->d_lon) HIPCHK(c, hipMalloc(&c->d_lon, plane * sizeof(double)));
+    if (!c->d_sw) HIPCHK(c, hipMalloc(&c->d_sw, c->s_w * sizeof(double)));
+    HIPCHK(c, hipMemcpy(c->d_lat, lat, plane * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_lon, lon, plane * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_sw, s_w, c->s_w * sizeof(double), hipMemcpyHostToDevice));
+    c->haveGeo = true;
+    return WC_OK;
+}
+
+int wc_particle_records(wc_ctx *c, const wc_records *out, size_t capacity, size_t *count) {
+    if (!c || !count) return WC_ERR_INVALID;
+    *count = 0;
+    if (!c->haveGeo) return fail(c, WC_ERR_INVALID, "wc_set_geo or wc_set_roms_grid first");
+    if (c->n == 0) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->recCap < c->cap) {
+        hipFree(c->d_rec);
+        c->d_rec = nullptr;
+        c->recCap = 0;
+        HIPCHK(c, hipMalloc(&c->d_rec, 3 * c->cap * sizeof(double)));
+        c->recCap = c->cap;
+    }
+    double *dep = c->d_rec, *lat = dep + c->recCap, *lon = lat + c->recCap;
+    const GeoPtrs g{c->d_lat, c->d_lon, c->d_h, c->d_sw, c->eta, c->xi, c->s_w};
+    HIPCHK(c, launch_particle_records(c->s, (int)c->n, g, c->s2, dep, lat, lon, c->d_blockCounts, c->d_total,
+                                      c->stream));
+    unsigned long long total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, c->d_total, sizeof total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *count = (size_t)total;
+    if (!out || !total) return WC_OK;
+    if (capacity < total) return fail(c, WC_ERR_INVALID, "%llu records, capacity %zu", total, capacity);
+    const size_t b = total * sizeof(double);
+    const struct { void *dst; const void *src; } f[10] = {
+        {out->id, c->s2.id}, {out->lat, lat}, {out->lon, lon}, {out->depth, dep}, {out->k, c->s2.k},
+        {out->j, c->s2.j}, {out->i, c->s2.i}, {out->health, c->s2.health}, {out->age, c->s2.age},
+        {out->time, c->s2.time}};
+    for (const auto &e : f)
+        if (e.dst) HIPCHK(c, hipMemcpyAsync(e.dst, e.src, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return WC_OK;
 }
 

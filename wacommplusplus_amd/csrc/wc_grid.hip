@@ -13,6 +13,7 @@
 // stream compaction of the SoA state: block counts -> exclusive scan -> scatter
 // with wave-ballot prefix sums inside each block.
 #include "wc_device.h"
+#include "wc_launch.h"
 
 #pragma clang fp contract(off)
 
@@ -156,6 +157,74 @@ __global__ __launch_bounds__(kBlock) void scatter_alive_kernel(SoA src, SoA dst,
     }
 }
 
+// ---- particle history records ------------------------------------------
+
+// Particles::netcdfRecords (Particles.cpp:115-147) with
+// OceanModelAdapter::kji2deplatlon (OceanModelAdapter.cpp:271-321): a particle
+// is kept when j >= 0, i >= 0, k <= 0, its cell (floor j, floor i) and its
+// next cell lie on the grid, ceil(k) lies in (-s_w+1, 0], and none of depth,
+// lat, lon is the 1e37 sentinel.  The range tests compare in double, so NaN or
+// out-of-int-range coordinates are rejected as the host's int conversion
+// (INT_MIN) rejects them.  The bilinear sums keep the reference's term order.
+__device__ __forceinline__ bool record_geo(const GeoPtrs &g, double k, double j, double i, double &dep, double &lat,
+                                           double &lon) {
+    if (j < 0 || i < 0 || k > 0) return false;
+    const double kc = ceil(k), jf = floor(j), fi = floor(i);
+    if (!(jf >= 0.0 && fi >= 0.0 && jf + 1.0 < (double)g.eta && fi + 1.0 < (double)g.xi && kc <= 0.0 &&
+          kc > (double)(-g.s_w + 1)))
+        return false;
+    const int jI = (int)jf, iI = (int)fi, kI = (int)kc;
+    const double kF = kc - k, jF = j - jf, iF = i - fi;
+    const size_t c00 = (size_t)jI * g.xi + iI, c10 = c00 + g.xi, c11 = c10 + 1, c01 = c00 + 1;
+    lon = g.lon[c00] * (1.0 - iF) * (1.0 - jF) + g.lon[c10] * (1.0 - iF) * jF + g.lon[c11] * iF * jF +
+          g.lon[c01] * iF * (1.0 - jF);
+    lat = g.lat[c00] * (1.0 - iF) * (1.0 - jF) + g.lat[c10] * (1.0 - iF) * jF + g.lat[c11] * iF * jF +
+          g.lat[c01] * iF * (1.0 - jF);
+    const double h = g.h[c00] * (1.0 - iF) * (1.0 - jF) + g.h[c10] * (1.0 - iF) * jF + g.h[c11] * iF * jF +
+                     g.h[c01] * iF * (1.0 - jF);
+    const double sLo = g.sW[kI - 1 + g.s_w - 1], sUp = g.sW[kI + g.s_w - 1];
+    const double aDep = fabs(fabs(h * sLo) - fabs(h * sUp));
+    dep = -(h * fabs(sUp) + fabs(kF * aDep));
+    return !(dep == 1e37 || lat == 1e37 || lon == 1e37);
+}
+
+__global__ __launch_bounds__(kBlock) void records_count_kernel(SoA src, int n, GeoPtrs g,
+                                                               uint32_t *__restrict__ blockCounts) {
+    __shared__ uint32_t waveCount[kWaves];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double d, la, lo;
+    const bool keep = idx < n && record_geo(g, src.k[idx], src.j[idx], src.i[idx], d, la, lo);
+    const unsigned long long m = __ballot(keep);
+    if (lane_id() == 0) waveCount[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < kWaves; ++w) s += waveCount[w];
+        blockCounts[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void records_scatter_kernel(SoA src, int n, GeoPtrs g, SoA dst,
+                                                                 double *__restrict__ dep, double *__restrict__ lat,
+                                                                 double *__restrict__ lon,
+                                                                 const uint32_t *__restrict__ blockOffsets) {
+    __shared__ uint32_t waveBase[kWaves];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0, la = 0, lo = 0;
+    const bool keep = idx < n && record_geo(g, src.k[idx], src.j[idx], src.i[idx], d, la, lo);
+    const unsigned long long m = __ballot(keep);
+    if (lane_id() == 0) waveBase[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = blockOffsets[blockIdx.x];
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) base += waveBase[w];
+    if (keep) {
+        const uint32_t o = base + (uint32_t)lanes_below(m);
+        dst.id[o] = src.id[idx]; dst.k[o] = src.k[idx]; dst.j[o] = src.j[idx]; dst.i[o] = src.i[idx];
+        dst.health[o] = src.health[idx]; dst.age[o] = src.age[idx]; dst.time[o] = src.time[idx];
+        dep[o] = d; lat[o] = la; lon[o] = lo;
+    }
+}
+
 int grid_blocks(size_t n) {
     size_t b = (n + kBlock - 1) / kBlock;
     if (b > 65536) b = 65536;
@@ -203,6 +272,22 @@ hipError_t launch_compact(const SoA &src, const SoA &dst, int n, uint32_t *block
     alive_count_kernel<<<nb, kBlock, 0, stream>>>(src.health, n, blockCounts);
     scan_blocks_kernel<<<1, 1024, 0, stream>>>(blockCounts, nb, total);
     scatter_alive_kernel<<<nb, kBlock, 0, stream>>>(src, dst, n, blockCounts);
+    return hipGetLastError();
+}
+
+}  // namespace wc
+
+namespace wc {
+
+// blockCounts must hold ceil(n/256) entries; total a device u64.
+hipError_t launch_particle_records(const SoA &src, int n, const GeoPtrs &g, const SoA &dst, double *dep,
+                                   double *lat, double *lon, uint32_t *blockCounts, unsigned long long *total,
+                                   hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + kBlock - 1) / kBlock;
+    records_count_kernel<<<nb, kBlock, 0, stream>>>(src, n, g, blockCounts);
+    scan_blocks_kernel<<<1, 1024, 0, stream>>>(blockCounts, nb, total);
+    records_scatter_kernel<<<nb, kBlock, 0, stream>>>(src, n, g, dst, dep, lat, lon, blockCounts);
     return hipGetLastError();
 }
 

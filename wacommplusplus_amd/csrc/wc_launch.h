@@ -34,6 +34,18 @@ hipError_t launch_counts_to_conc(const uint32_t *counts, const double *mask, int
 hipError_t launch_compact(const SoA &src, const SoA &dst, int n, uint32_t *blockCounts, unsigned long long *total,
                           hipStream_t stream);
 
+// Particle history records (Particles::netcdfRecords): the grid's lat, lon
+// (degrees) and h [eta][xi], the s_w W-level sigma values bottom first.
+struct GeoPtrs {
+    const double *lat, *lon, *h, *sW;
+    int eta, xi, s_w;
+};
+// Order-preserving: the particles of src inside the interpolation domain go
+// to dst (state) and dep, lat, lon; total[0] receives their number.
+hipError_t launch_particle_records(const SoA &src, int n, const GeoPtrs &g, const SoA &dst, double *dep,
+                                   double *lat, double *lon, uint32_t *blockCounts, unsigned long long *total,
+                                   hipStream_t stream);
+
 hipError_t launch_emit(const SoA &s, size_t at, int n, const double *times, const int *src, const double *pos,
                        uint64_t firstId, uint64_t seed, int randomSources, hipStream_t stream);
 bool emission_times(int mode, double value, double sourceStart, double sourceEnd, double intervalStart,
