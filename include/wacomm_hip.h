@@ -418,6 +418,15 @@ int wc_ens_release(wc_ctx *ctx, const wc_ens_source *sources, int n_sources, con
 int wc_ens_set_launch(wc_ctx *ctx, int block, int ppt, int occ);
 int wc_ens_get_launch(wc_ctx *ctx, int *block, int *ppt, int *occ);
 int wc_ens_get_sort(wc_ctx *ctx, int *tile, int *period);
+/* Stages the forcing around each workgroup's particles in LDS: the interval
+ * runs in chunks of `chunk` substeps, and before each chunk every workgroup
+ * of 256 particles copies the packed forcing of its particles' bounding box
+ * (widened by `halo` cells and one level each way) into LDS; stencils inside
+ * the box read LDS, the others global memory.  The results are those of the
+ * plain move, bit for bit.  chunk 0 (the default) turns it off; halo 0..16,
+ * default 2.  Needs occupancy hint 0 or 3. */
+int wc_ens_set_lds(wc_ctx *ctx, int chunk, int halo);
+int wc_ens_get_lds(wc_ctx *ctx, int *chunk, int *halo);
 
 /* Reorders the ensemble population by the tile (tile x tile cells) of its cell
  * before the move of every period-th step, so a wave's lanes gather from
@@ -425,10 +434,12 @@ int wc_ens_get_sort(wc_ctx *ctx, int *tile, int *period);
  * particle and do not depend on the order. */
 int wc_ens_set_sort(wc_ctx *ctx, int tile, int period);
 
-/* Times every offered launch shape and sort tile (none, 2, 4, 8; sorted every
- * step) on one interval [start, end] (records in slot0, slot1) over the
- * current population, which is restored afterwards, and keeps the fastest
- * (108 trials).  *best_ms (optional) receives its time. */
+/* Times every offered launch on one interval [start, end] (records in slot0,
+ * slot1) over the current population, which is restored afterwards, and
+ * keeps the fastest (124 trials): the plain move over block x particles per
+ * work-item x occupancy hint, and the LDS-tiled move (occupancy hint 0 or 3,
+ * chunk 10 or 30 substeps, halo kept), each with sort tile none, 2, 4 or 8
+ * (sorted every step).  *best_ms (optional) receives its time. */
 int wc_ens_autotune(wc_ctx *ctx, const wc_config_data *config, double start, double end, int slot0, int slot1,
                     double *best_ms);
 

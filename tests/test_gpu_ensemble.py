@@ -167,7 +167,7 @@ def test_multi_interval_graph_run_and_autotune(ctx, full):
     shape = ctx.ens_get_launch()
     print(f"autotuned launch {shape}: {best:.3f} ms", flush=True)
     assert best > 0 and shape[0] in (64, 128, 256) and shape[1] in (1, 2, 4) and shape[2] in (0, 3, 4)
-    assert ctx.ens_get_sort()[0] in (0, 2, 4, 8)
+    assert ctx.ens_get_sort()[0] in (0, 2, 4, 8) and ctx.ens_get_lds()[0] in (0, 10, 30)
     after, _ = ctx.ens_get()                     # the population is restored by the autotuner
     for f in ("k", "j", "i", "health", "age"):
         assert np.array_equal(after[f], initial[f], equal_nan=True), f
@@ -217,6 +217,35 @@ def test_cell_tile_sort_changes_no_result(ctx, full, tile, period):
     ctx.ens_set_sort(0, 1)
 
 
+@pytest.mark.parametrize("chunk,halo,tile,occ", [(1, 2, 0, 0), (7, 2, 4, 3), (30, 0, 0, 3), (30, 2, 4, 0),
+                                                 (120, 16, 2, 3)])
+def test_lds_tiled_move_changes_no_result(ctx, full, chunk, halo, tile, occ):
+    """The LDS-tiled move (forcing staged per workgroup, restaged every `chunk`
+    substeps; halo 0 sends many stencils to global memory) gives every
+    particle the plain move's state and surface counts, bit for bit."""
+    p, scen, sigma, counts = ensemble(full, n_scenarios=16, total=200_000, median=10_000, lo=2_000, hi=40_000,
+                                      max_offset_h=1.0)
+    cfg = wc.default_config(random=True, sigma=1.0)
+    load_window(ctx, full)
+    res = []
+    for lds in (0, chunk):
+        ctx.ens_set(p, scen, sigma)
+        ctx.ens_set_lds(0, 2)
+        ctx.ens_set_launch(256, 1, occ)
+        ctx.ens_set_sort(tile, 1)
+        ctx.ens_set_lds(lds, halo)
+        assert ctx.ens_get_lds() == (lds, halo)
+        st = run_ens(ctx, full, cfg, 3, use_graph=True)
+        res.append((ctx.ens_get(), ctx.ens_surface(), st.particle_steps))
+    (a, sa), ga, na = res[0]
+    (b, sb), gb, nb = res[1]
+    for f in ("id", "k", "j", "i", "health", "age", "time"):
+        assert np.array_equal(a[f], b[f], equal_nan=True), f
+    assert np.array_equal(sa, sb) and np.array_equal(ga, gb) and na == nb
+    ctx.ens_set_lds(0, 2)
+    ctx.ens_set_sort(0, 1)
+
+
 def test_rejects_bad_input(ctx, full):
     load_window(ctx, full)
     p = wc.make_particles(10, j=10.5, i=10.5)
@@ -224,6 +253,13 @@ def test_rejects_bad_input(ctx, full):
         ctx.ens_set(p, np.full(10, 3, np.uint8), [1.0, 2.0])          # scenario >= n_scenarios
     with pytest.raises(wc.WcError):
         ctx.ens_set_launch(100, 1, 0)
+    with pytest.raises(wc.WcError):
+        ctx.ens_set_lds(-1, 2)
+    ctx.ens_set_launch(256, 1, 0)
+    ctx.ens_set_lds(10, 2)
+    with pytest.raises(wc.WcError):
+        ctx.ens_set_launch(256, 1, 4)                                   # the LDS move needs hint 0 or 3
+    ctx.ens_set_lds(0, 2)
     cfg = wc.default_config(driftModel=1)
     ctx.ens_set(p, np.zeros(10, np.uint8), [1.0])
     with pytest.raises(wc.WcError) as e:

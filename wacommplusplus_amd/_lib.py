@@ -28,7 +28,7 @@ EXPORTS = (
     "wc_version", "wc_cell_metric_table", "wc_philox4x32_10", "wc_emit", "wc_run_window",
     "wc_set_roms_grid", "wc_upload_roms_record", "wc_get_record", "wc_check_roms_grid", "wc_compare_records",
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
-    "wc_ens_get", "wc_ens_release", "wc_ens_set_sort", "wc_ens_get_sort", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
+    "wc_ens_get", "wc_ens_release", "wc_ens_set_sort", "wc_ens_get_sort", "wc_ens_set_lds", "wc_ens_get_lds", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
     "wc_particle_records",
@@ -160,6 +160,8 @@ def load_library(path: str = LIB_PATH):
         "wc_ens_release": [vp, vp, i, vp, i, C.c_uint64],
         "wc_ens_set_sort": [vp, i, i],
         "wc_ens_get_sort": [vp, C.POINTER(i), C.POINTER(i)],
+        "wc_ens_set_lds": [vp, i, i],
+        "wc_ens_get_lds": [vp, C.POINTER(i), C.POINTER(i)],
         "wc_set_geo": [vp, vp, vp, vp],
         "wc_particle_records": [vp, C.POINTER(Records), sz, C.POINTER(sz)],
     }
@@ -441,6 +443,16 @@ class Context:
 
     def ens_set_sort(self, tile=8, period=1):
         self._chk(self.L.wc_ens_set_sort(self.h, tile, period))
+
+    def ens_set_lds(self, chunk: int, halo: int = 2):
+        """Forcing staged in LDS per workgroup, restaged every `chunk`
+        substeps (0: off); the results do not change."""
+        self._chk(self.L.wc_ens_set_lds(self.h, int(chunk), int(halo)))
+
+    def ens_get_lds(self):
+        c, h = C.c_int(), C.c_int()
+        self._chk(self.L.wc_ens_get_lds(self.h, C.byref(c), C.byref(h)))
+        return c.value, h.value
 
     def ens_get_sort(self):
         t, p = C.c_int(), C.c_int()

@@ -85,6 +85,7 @@ struct wc_ctx {
         EnsSoA s{}, save{}, tmp{};
         uint32_t *d_bins = nullptr;
         int sortTile = 0, sortPeriod = 1;
+        int ldsChunk = 0, ldsHalo = 2;
         float *d_sigma = nullptr;
         uint32_t *d_surface = nullptr;
         EnsStep *d_steps = nullptr;
@@ -1128,6 +1129,8 @@ EnsLaunch ens_launch(wc_ctx *c, bool surface) {
     L.sortPeriod = c->ens.sortPeriod;
     L.tmp = c->ens.tmp;
     L.bins = c->ens.d_bins;
+    L.ldsChunk = c->ens.ldsChunk;
+    L.ldsHalo = c->ens.ldsHalo;
     return L;
 }
 
@@ -1289,6 +1292,8 @@ int wc_ens_set_launch(wc_ctx *c, int block, int ppt, int occ) {
     if (!c) return WC_ERR_INVALID;
     if ((block != 64 && block != 128 && block != 256) || ppt < 1 || ppt > 64 || (occ != 0 && occ != 3 && occ != 4))
         return fail(c, WC_ERR_INVALID, "launch block=%d ppt=%d occ=%d not offered", block, ppt, occ);
+    if (occ == 4 && c->ens.ldsChunk > 0)
+        return fail(c, WC_ERR_INVALID, "the LDS-tiled move runs at occupancy hint 0 or 3");
     free_ens_graph(c);
     c->ens.block = block; c->ens.ppt = ppt; c->ens.occ = occ;
     return WC_OK;
@@ -1301,6 +1306,25 @@ int wc_ens_set_sort(wc_ctx *c, int tile, int period) {
     free_ens_graph(c);
     c->ens.sortTile = tile;
     c->ens.sortPeriod = period;
+    return WC_OK;
+}
+
+int wc_ens_set_lds(wc_ctx *c, int chunk, int halo) {
+    if (!c) return WC_ERR_INVALID;
+    if (chunk < 0 || chunk > 1024 || halo < 0 || halo > 16)
+        return fail(c, WC_ERR_INVALID, "LDS chunk %d (0..1024), halo %d (0..16)", chunk, halo);
+    if (chunk > 0 && c->ens.occ == 4)
+        return fail(c, WC_ERR_INVALID, "the LDS-tiled move runs at occupancy hint 0 or 3");
+    free_ens_graph(c);
+    c->ens.ldsChunk = chunk;
+    c->ens.ldsHalo = halo;
+    return WC_OK;
+}
+
+int wc_ens_get_lds(wc_ctx *c, int *chunk, int *halo) {
+    if (!c) return WC_ERR_INVALID;
+    if (chunk) *chunk = c->ens.ldsChunk;
+    if (halo) *halo = c->ens.ldsHalo;
     return WC_OK;
 }
 
@@ -1334,25 +1358,42 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    const int blocks[3] = {64, 128, 256}, ppts[3] = {1, 2, 4}, occs[3] = {0, 3, 4}, tiles[4] = {0, 2, 4, 8};
-    constexpr int kTrials = 3 * 3 * 3 * 4;
+    // candidates: the plain move over block x particles per work-item x
+    // occupancy hint, and the LDS-tiled move (256 x 1) over occupancy hint x
+    // chunk; each with every sort tile
+    struct Cand {
+        int block, ppt, occ, tile, chunk;
+    };
+    std::vector<Cand> cands;
+    for (int tile : {0, 2, 4, 8}) {
+        for (int occ : {0, 3, 4})
+            for (int ppt : {1, 2, 4})
+                for (int block : {64, 128, 256}) cands.push_back(Cand{block, ppt, occ, tile, 0});
+        for (int occ : {0, 3})
+            for (int chunk : {10, 30}) cands.push_back(Cand{256, 1, occ, tile, chunk});
+    }
     double best = INFINITY;
-    int bb = E.block, bp = E.ppt, bo = E.occ, bt = E.sortTile;
+    Cand keep{E.block, E.ppt, E.occ, E.sortTile, E.ldsChunk};
     E.sortPeriod = 1;
-    for (int trial = -1; trial < kTrials && !rc; ++trial) {
-        const int t = trial < 0 ? kTrials - 1 : trial;         // trial -1 warms up the last candidate
-        E.block = blocks[t % 3]; E.ppt = ppts[(t / 3) % 3]; E.occ = occs[(t / 9) % 3]; E.sortTile = tiles[t / 27];
+    const int nc = (int)cands.size();
+    for (int trial = -1; trial < nc && !rc; ++trial) {
+        const Cand &t = cands[trial < 0 ? nc - 1 : trial];       // trial -1 warms up the last candidate
+        E.block = t.block; E.ppt = t.ppt; E.occ = t.occ; E.sortTile = t.tile; E.ldsChunk = t.chunk;
         if ((rc = copy_ens_soa(c, E.s, E.save, E.n))) break;
         if ((rc = upload_ens_steps(c, cfg, times, slots, 1))) break;
         const EnsLaunch L = ens_launch(c, false);
         if (hipEventRecord(e0, c->stream) != hipSuccess || launch_ens_step(c->store, L, c->stream) != hipSuccess ||
             hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-            rc = fail(c, WC_ERR_HIP, "autotune launch block=%d ppt=%d occ=%d failed", E.block, E.ppt, E.occ);
+            rc = fail(c, WC_ERR_HIP, "autotune launch block=%d ppt=%d occ=%d tile=%d chunk=%d failed", E.block,
+                      E.ppt, E.occ, E.sortTile, E.ldsChunk);
             break;
         }
         float ms = 0;
         hipEventElapsedTime(&ms, e0, e1);
-        if (trial >= 0 && ms < best) { best = ms; bb = E.block; bp = E.ppt; bo = E.occ; bt = E.sortTile; }
+        if (trial >= 0 && ms < best) {
+            best = ms;
+            keep = t;
+        }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -1360,7 +1401,7 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (rc) return rc;
     free_ens_graph(c);
-    E.block = bb; E.ppt = bp; E.occ = bo; E.sortTile = bt;
+    E.block = keep.block; E.ppt = keep.ppt; E.occ = keep.occ; E.sortTile = keep.tile; E.ldsChunk = keep.chunk;
     E.tuned = true;
     if (best_ms) *best_ms = best;
     return WC_OK;

@@ -102,7 +102,15 @@ struct EnsLaunch {
     int sortTile, sortPeriod;
     EnsSoA tmp;
     uint32_t *bins;
+    // forcing staged in LDS per workgroup (move_ens_lds_kernel): chunk =
+    // substeps between restagings (0: the plain kernel), halo in cells
+    int ldsChunk, ldsHalo;
 };
+// LDS of one move_ens_lds_kernel workgroup (256 work-items): three per CU at
+// occupancy hint 3, two otherwise (160 KB per CU, at most 64 KB per workgroup
+// including its static LDS)
+constexpr int kEnsLdsBytes3 = 47 * 1024;
+constexpr int kEnsLdsBytes2 = 63 * 1024;
 hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream);
 // One release point of a device-generated ensemble population.
 struct EnsSource {

@@ -76,6 +76,54 @@ __device__ __forceinline__ uint16_t fromf<uint16_t>(float x) {
 
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 
+// Corner arithmetic of the samplers (shared by the global and the LDS-tiled
+// sampler, so both give the same bits).
+struct W4 {
+    float w00, w01, w10, w11;
+};
+__device__ __forceinline__ W4 weights(float fi, float fj) {
+    const float gi = 1.0f - fi, gj = 1.0f - fj;
+    return W4{gi * gj, fi * gj, gi * fj, fi * fj};
+}
+__device__ __forceinline__ float hc4(const ZH &a, const ZH &b, const ZH &d, const ZH &e, const W4 &w, float al) {
+    const float zz = w.w00 * lerpf(a.z0, a.z1, al) + w.w01 * lerpf(b.z0, b.z1, al) +
+                     w.w10 * lerpf(d.z0, d.z1, al) + w.w11 * lerpf(e.z0, e.z1, al);
+    const float hh = w.w00 * a.h + w.w01 * b.h + w.w10 * d.h + w.w11 * e.h;
+    return hh + zz;
+}
+template <class T>
+__device__ __forceinline__ void uv4(const UV4<T> &a, const UV4<T> &b, const UV4<T> &d, const UV4<T> &e, const W4 &w,
+                                    float al, float &uu, float &vv) {
+    uu = w.w00 * lerpf(tof(a.v[0]), tof(a.v[1]), al) + w.w01 * lerpf(tof(b.v[0]), tof(b.v[1]), al) +
+         w.w10 * lerpf(tof(d.v[0]), tof(d.v[1]), al) + w.w11 * lerpf(tof(e.v[0]), tof(e.v[1]), al);
+    vv = w.w00 * lerpf(tof(a.v[2]), tof(a.v[3]), al) + w.w01 * lerpf(tof(b.v[2]), tof(b.v[3]), al) +
+         w.w10 * lerpf(tof(d.v[2]), tof(d.v[3]), al) + w.w11 * lerpf(tof(e.v[2]), tof(e.v[3]), al);
+}
+// w (and AKt when wantA) at W levels kI (weight 1-kF) and kI-1 (weight kF)
+template <class T>
+__device__ __forceinline__ void wa4(const WA8<T> &a, const WA8<T> &b, const WA8<T> &d, const WA8<T> &e, float kF,
+                                    const W4 &w, float alw, float ala, bool wantA, float &ww, float &aa) {
+    auto corner = [&](const WA8<T> &q, float &wc, float &ac) {
+        const float wt = lerpf(tof(q.v[0]), tof(q.v[1]), alw), wb = lerpf(tof(q.v[2]), tof(q.v[3]), alw);
+        wc = lerpf(wt, wb, kF);
+        if (wantA) {
+            const float at = lerpf(tof(q.v[4]), tof(q.v[5]), ala), ab = lerpf(tof(q.v[6]), tof(q.v[7]), ala);
+            ac = lerpf(at, ab, kF);
+        } else {
+            ac = 0.f;
+        }
+    };
+    float w0, w1, w2, w3, a0, a1, a2, a3;
+    corner(a, w0, a0);
+    corner(b, w1, a1);
+    corner(d, w2, a2);
+    corner(e, w3, a3);
+    ww = w.w00 * w0 + w.w01 * w1 + w.w10 * w2 + w.w11 * w3;
+    aa = w.w00 * a0 + w.w01 * a1 + w.w10 * a2 + w.w11 * a3;
+}
+
+// The interval's gather layout in global memory.  Stencils at cell (j, i)
+// read (j, i), (j, i+1), (j+1, i), (j+1, i+1).
 template <class T>
 struct Packed {
     const UV4<T> *uv;     // [s_w-1 levels: k = kLL..0][eta][xi]   (rho levels)
@@ -87,55 +135,70 @@ struct Packed {
     int nlev;             // s_w - 1
     size_t plane;
 
-    // bilinear weights of the four corners
-    struct W4 {
-        float w00, w01, w10, w11;
-    };
-    __device__ __forceinline__ static W4 weights(float fi, float fj) {
-        const float gi = 1.0f - fi, gj = 1.0f - fj;
-        return W4{gi * gj, fi * gj, gi * fj, fi * fj};
-    }
     // level index of logical level k (k = -(nlev-1) .. 0)
     __device__ __forceinline__ size_t lev(int k) const { return (size_t)(k + nlev - 1) * plane; }
+    __device__ __forceinline__ size_t cell(int j, int i) const { return (size_t)j * xi + i; }
 
-    __device__ __forceinline__ float hc(size_t c, const W4 &w, float al) const {
-        const ZH a = z[c], b = z[c + 1], d = z[c + xi], e = z[c + xi + 1];
-        const float zz = w.w00 * lerpf(a.z0, a.z1, al) + w.w01 * lerpf(b.z0, b.z1, al) +
-                         w.w10 * lerpf(d.z0, d.z1, al) + w.w11 * lerpf(e.z0, e.z1, al);
-        const float hh = w.w00 * a.h + w.w01 * b.h + w.w10 * d.h + w.w11 * e.h;
-        return hh + zz;
+    __device__ __forceinline__ float hc(int j, int i, const W4 &w, float al) const {
+        const size_t c = cell(j, i);
+        return hc4(z[c], z[c + 1], z[c + xi], z[c + xi + 1], w, al);
     }
-    __device__ __forceinline__ void uv_at(int kI, size_t c, const W4 &w, float al, float &uu, float &vv) const {
-        const UV4<T> *p = uv + lev(kI) + c;
-        const UV4<T> a = p[0], b = p[1], d = p[xi], e = p[xi + 1];
-        uu = w.w00 * lerpf(tof(a.v[0]), tof(a.v[1]), al) + w.w01 * lerpf(tof(b.v[0]), tof(b.v[1]), al) +
-             w.w10 * lerpf(tof(d.v[0]), tof(d.v[1]), al) + w.w11 * lerpf(tof(e.v[0]), tof(e.v[1]), al);
-        vv = w.w00 * lerpf(tof(a.v[2]), tof(a.v[3]), al) + w.w01 * lerpf(tof(b.v[2]), tof(b.v[3]), al) +
-             w.w10 * lerpf(tof(d.v[2]), tof(d.v[3]), al) + w.w11 * lerpf(tof(e.v[2]), tof(e.v[3]), al);
+    __device__ __forceinline__ void uv_at(int kI, int j, int i, const W4 &w, float al, float &uu, float &vv) const {
+        const UV4<T> *p = uv + lev(kI) + cell(j, i);
+        uv4<T>(p[0], p[1], p[xi], p[xi + 1], w, al, uu, vv);
     }
-    // w (and AKt when wantA) at W levels kI (weight 1-kF) and kI-1 (weight kF)
-    __device__ __forceinline__ void wa_at(int kI, float kF, size_t c, const W4 &w, float alw, float ala, bool wantA,
-                                          float &ww, float &aa) const {
-        const WA8<T> *p = wa + lev(kI) + c;
-        const WA8<T> a = p[0], b = p[1], d = p[xi], e = p[xi + 1];
-        auto corner = [&](const WA8<T> &q, float &wc, float &ac) {
-            const float wt = lerpf(tof(q.v[0]), tof(q.v[1]), alw), wb = lerpf(tof(q.v[2]), tof(q.v[3]), alw);
-            wc = lerpf(wt, wb, kF);
-            if (wantA) {
-                const float at = lerpf(tof(q.v[4]), tof(q.v[5]), ala), ab = lerpf(tof(q.v[6]), tof(q.v[7]), ala);
-                ac = lerpf(at, ab, kF);
-            } else {
-                ac = 0.f;
-            }
-        };
-        float w0, w1, w2, w3, a0, a1, a2, a3;
-        corner(a, w0, a0);
-        corner(b, w1, a1);
-        corner(d, w2, a2);
-        corner(e, w3, a3);
-        ww = w.w00 * w0 + w.w01 * w1 + w.w10 * w2 + w.w11 * w3;
-        aa = w.w00 * a0 + w.w01 * a1 + w.w10 * a2 + w.w11 * a3;
+    __device__ __forceinline__ void wa_at(int kI, float kF, int j, int i, const W4 &w, float alw, float ala,
+                                          bool wantA, float &ww, float &aa) const {
+        const WA8<T> *p = wa + lev(kI) + cell(j, i);
+        wa4<T>(p[0], p[1], p[xi], p[xi + 1], kF, w, alw, ala, wantA, ww, aa);
     }
+    __device__ __forceinline__ float2 md_at(int j, int i) const { return md[cell(j, i)]; }
+    __device__ __forceinline__ float depth_at(int q) const { return depthf[q]; }
+};
+
+// The same layout with a box of cells [j0, j0+bh) x [i0, i0+bw) and levels
+// [l0, l0+nl) (level indices) staged in LDS by the workgroup: a stencil
+// inside the box reads LDS, any other reads global memory, so the values and
+// the results are the same wherever a particle goes.
+template <class T>
+struct TilePacked {
+    Packed<T> g;
+    const WA8<T> *wL;     // [nl][bh][bw]
+    const ZH *zL;         // [bh][bw]
+    const UV4<T> *uL;     // [nl][bh][bw]
+    const float2 *mL;     // [bh][bw]
+    int j0, i0, bh, bw, l0, nl;
+
+    __device__ __forceinline__ bool stencil_in(int j, int i) const {
+        return (unsigned)(j - j0) < (unsigned)(bh - 1) && (unsigned)(i - i0) < (unsigned)(bw - 1);
+    }
+    __device__ __forceinline__ bool level_in(int kI) const { return (unsigned)(kI + g.nlev - 1 - l0) < (unsigned)nl; }
+    __device__ __forceinline__ int local(int j, int i) const { return (j - j0) * bw + (i - i0); }
+    __device__ __forceinline__ int local(int kI, int j, int i) const {
+        return ((kI + g.nlev - 1 - l0) * bh + (j - j0)) * bw + (i - i0);
+    }
+
+    __device__ __forceinline__ float hc(int j, int i, const W4 &w, float al) const {
+        if (!stencil_in(j, i)) return g.hc(j, i, w, al);
+        const ZH *p = zL + local(j, i);
+        return hc4(p[0], p[1], p[bw], p[bw + 1], w, al);
+    }
+    __device__ __forceinline__ void uv_at(int kI, int j, int i, const W4 &w, float al, float &uu, float &vv) const {
+        if (!(stencil_in(j, i) && level_in(kI))) return g.uv_at(kI, j, i, w, al, uu, vv);
+        const UV4<T> *p = uL + local(kI, j, i);
+        uv4<T>(p[0], p[1], p[bw], p[bw + 1], w, al, uu, vv);
+    }
+    __device__ __forceinline__ void wa_at(int kI, float kF, int j, int i, const W4 &w, float alw, float ala,
+                                          bool wantA, float &ww, float &aa) const {
+        if (!(stencil_in(j, i) && level_in(kI))) return g.wa_at(kI, kF, j, i, w, alw, ala, wantA, ww, aa);
+        const WA8<T> *p = wL + local(kI, j, i);
+        wa4<T>(p[0], p[1], p[bw], p[bw + 1], kF, w, alw, ala, wantA, ww, aa);
+    }
+    __device__ __forceinline__ float2 md_at(int j, int i) const {
+        if ((unsigned)(j - j0) < (unsigned)bh && (unsigned)(i - i0) < (unsigned)bw) return mL[local(j, i)];
+        return g.md_at(j, i);
+    }
+    __device__ __forceinline__ float depth_at(int q) const { return g.depthf[q]; }
 };
 
 __device__ __forceinline__ bool valid_cell(int j, int i, int eta, int xi) {
@@ -169,7 +232,9 @@ struct Reg {
     int jc, ic;
     float fj, fi, k;
     double health, age;
+    double elapsed;       // time into the interval
     bool inside, healthFresh, changed;
+    bool done;            // the substep loop has ended for this interval
 };
 
 // Converts an FP64 position into cell + offset; `inside` is the reference's
@@ -188,15 +253,19 @@ __device__ __forceinline__ void to_cell(double jd, double id_, double kd, int et
 // The substeps of one forcing interval for one particle (Particle.cpp:
 // 178-815) on the cell + offset state; sigma is the horizontal diffusion
 // scale (config sigma, or the particle's scenario sigma in an ensemble).
-// Returns the live substeps taken.
-template <class T>
-__device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Packed<T> &S, float sigma, uint64_t id,
-                                                 double ptime, Reg &q) {
+// Runs the substeps that start before tEnd (the interval length, or the end
+// of a chunk of it: a later call continues from q.elapsed); q.done marks the
+// end of the particle's loop for the interval.  Returns the live substeps
+// taken.  S is a Packed<T> or a TilePacked<T>.
+template <class T, class Smp>
+__device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp &S, float sigma, uint64_t id,
+                                                 double ptime, Reg &q, double tEnd) {
     unsigned steps = 0;
+    if (q.done) return 0;
     int jc = q.jc, ic = q.ic;
     float fj = q.fj, fi = q.fi, k = q.k;
     double health = q.health, age = q.age;
-    bool inside = q.inside, healthFresh = q.healthFresh, changed = q.changed;
+    bool inside = q.inside, healthFresh = q.healthFresh, changed = q.changed, done = false;
     const float kLL = (float)P.kLowerLimit;
     const int eta = P.eta, xi = P.xi;
     const float dir = (float)P.direction;
@@ -205,9 +274,9 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Pack
     const uint32_t ik = (uint32_t)P.intervalKey ^ (uint32_t)((uint64_t)P.intervalKey >> 32);
     const double L = P.intervalLength;
     const double invL = L == 0 ? 0.0 : 1.0 / L, invDti = 1.0 / P.dti;
-    double elapsed = P.elapsed0;
+    double elapsed = q.elapsed;
 
-        while (elapsed < L) {
+        while (elapsed < tEnd) {
             const double stepDt = fmin(P.dti, L - elapsed);
             const double physicalTime = P.intervalStart + P.dirInt * elapsed;
             if (P.dirInt > 0 && ptime > physicalTime) {           // not yet emitted (Particle.cpp:211-215)
@@ -218,12 +287,14 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Pack
                 health = -1;
                 healthFresh = true;
                 changed = true;
+                done = true;
                 break;
             }
             if (!inside || !(k == k)) {                           // open boundary (:241-248)
                 health = -1;
                 healthFresh = true;
                 changed = true;
+                done = true;
                 break;
             }
             ++steps;
@@ -233,17 +304,16 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Pack
             const float dt = (float)stepDt;
             const int kI = (int)ceilf(k);
             const float kF = (float)kI - k;
-            const size_t c = (size_t)jc * xi + ic;
-            const auto w0 = Packed<T>::weights(fi, fj);
+            const W4 w0 = weights(fi, fj);
 
-            const float hc = S.hc(c, w0, alpha);
+            const float hc = S.hc(jc, ic, w0, alpha);
             bool kill = false;
             if (hc <= shore && P.horizontal == WC_CLOSURE_KILL) kill = true;
 
             const float a1 = P.midpoint ? alphaStart : alpha;
             float uu, vv, ww, aa;
-            S.uv_at(kI, c, w0, a1, uu, vv);
-            S.wa_at(kI, kF, c, w0, a1, alpha, P.randomOn != 0, ww, aa);
+            S.uv_at(kI, jc, ic, w0, a1, uu, vv);
+            S.wa_at(kI, kF, jc, ic, w0, a1, alpha, P.randomOn != 0, ww, aa);
             float rx = 0, ry = 0, rz = 0;
             if (P.randomOn) {
                 const float sigmaDepth = sigma * (1.0f - k / kLL);
@@ -262,9 +332,9 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Pack
                 ry = scale * ra * s1;
                 rz = scale * rb * c2 * aa * (float)P.crid;
             }
-            const float2 md = S.md[c];
+            const float2 md = S.md_at(jc, ic);
             const float idx_ = md.x, idy_ = md.y;
-            const float zdist = hc * S.depthf[kI + P.s_w - 2];
+            const float zdist = hc * S.depth_at(kI + P.s_w - 2);
             // Vertical step in levels: the leap capped at one cell height and
             // divided by it (Particle.cpp:585-594, 644-657).  Where the water
             // column has no depth (hc == 0: a particle exactly on a land node
@@ -285,29 +355,27 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Pack
                 const float kMid = fminf(0.0f, fmaxf(kLL, k + levels(kHalf)));
                 bool midValid = valid_cell(jm, im, eta, xi);
                 float hcMid = 0;
-                size_t cm = 0;
-                typename Packed<T>::W4 wm{};
+                W4 wm{};
                 if (midValid) {
-                    cm = (size_t)jm * xi + im;
-                    wm = Packed<T>::weights(fim, fjm);
-                    hcMid = S.hc(cm, wm, alpha);
+                    wm = weights(fim, fjm);
+                    hcMid = S.hc(jm, im, wm, alpha);
                     midValid = hcMid > shore && hcMid > 0;
                 }
                 if (midValid) {
                     const int kMI = (int)ceilf(kMid);
                     const float kMF = (float)kMI - kMid;
                     float um, vm, wmv, dummy;
-                    S.uv_at(kMI, cm, wm, alpha, um, vm);
-                    S.wa_at(kMI, kMF, cm, wm, alpha, alpha, false, wmv, dummy);
+                    S.uv_at(kMI, jm, im, wm, alpha, um, vm);
+                    S.wa_at(kMI, kMF, jm, im, wm, alpha, alpha, false, wmv, dummy);
                     // leap in metres of the start cell: u* dt dx/dx*
-                    const float2 mdm = S.md[cm];
+                    const float2 mdm = S.md_at(jm, im);
                     dxl = dir * um * dt * (mdm.x / idx_);
                     dyl = dir * vm * dt * (mdm.y / idy_);
-                    dzl = dir * (sv + wmv) * dt * (zdist / (hcMid * S.depthf[kMI + P.s_w - 2]));
+                    dzl = dir * (sv + wmv) * dt * (zdist / (hcMid * S.depth_at(kMI + P.s_w - 2)));
                 } else {
                     float ue, ve, we, dummy;
-                    S.uv_at(kI, c, w0, alpha, ue, ve);
-                    S.wa_at(kI, kF, c, w0, alpha, alpha, false, we, dummy);
+                    S.uv_at(kI, jc, ic, w0, alpha, ue, ve);
+                    S.wa_at(kI, kF, jc, ic, w0, alpha, alpha, false, we, dummy);
                     dxl = dir * ue * dt;
                     dyl = dir * ve * dt;
                     dzl = dir * (sv + we) * dt;
@@ -331,8 +399,7 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Pack
                 else if (P.lower == WC_CLOSURE_REFLECTION) kdet = 2.0f * kLL - kdet;
             }
             if (valid_cell(jn, in_, eta, xi)) {                    // :709-776
-                const size_t cn = (size_t)jn * xi + in_;
-                const float hcdet = S.hc(cn, Packed<T>::weights(fin, fjn), alpha);
+                const float hcdet = S.hc(jn, in_, weights(fin, fjn), alpha);
                 if (hcdet <= shore) {
                     if (P.horizontal == WC_CLOSURE_CONSTRAINT) {
                         jn = jc; fjn = fj; in_ = ic; fin = fi;
@@ -375,6 +442,8 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Pack
         q.jc = jc; q.ic = ic; q.fj = fj; q.fi = fi; q.k = k;
     q.health = health; q.age = age;
     q.inside = inside; q.healthFresh = healthFresh; q.changed = changed;
+    q.elapsed = elapsed;
+    q.done = done;
     return steps;
 }
 
@@ -408,7 +477,9 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
         q.age = st.age[idx];
         q.healthFresh = true;   // `health` holds the current value (else derive from age)
         q.changed = false;
-        steps = run_interval<T>(P, S, (float)P.sigma, id, ptime, q);
+        q.elapsed = P.elapsed0;
+        q.done = false;
+        steps = run_interval<T>(P, S, (float)P.sigma, id, ptime, q, P.intervalLength);
         if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
         if (q.changed) {
             st.k[idx] = (double)q.k;
@@ -532,7 +603,9 @@ void move_ens_kernel(const EnsStep *__restrict__ steps, const int *__restrict__ 
         q.age = st.age[idx];
         q.healthFresh = true;
         q.changed = false;
-        total += run_interval<T>(P, S, sigmaTab[st.scen[idx]], st.id[idx], st.time[idx], q);
+        q.elapsed = P.elapsed0;
+        q.done = false;
+        total += run_interval<T>(P, S, sigmaTab[st.scen[idx]], st.id[idx], st.time[idx], q, P.intervalLength);
         if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
         if (q.changed) {
             // store the position the context path stores (FP64 cell +
@@ -549,6 +622,158 @@ void move_ens_kernel(const EnsStep *__restrict__ steps, const int *__restrict__ 
             st.age[idx] = q.age;
         } else if (k != k0 || (float)q.health != h0) {
             st.k[idx] = k;
+            st.health[idx] = (float)q.health;
+        }
+    }
+    if (stepCounter) wave_add_u64(stepCounter, total);
+}
+
+// The ensemble move with the forcing staged in LDS.  One particle per work-
+// item, 256 per workgroup.  The interval runs in chunks of `chunk` substeps;
+// before each chunk the workgroup reduces the cells and levels of its
+// particles that will step in it, widens that box by `halo` cells (and one
+// level each way plus the stencil's next cell), and, when the box fits in
+// `ldsBytes`, copies its packed u/v, w/AKt, zeta/h and metric into LDS with
+// coalesced loads.  The substeps then sample through TilePacked (LDS inside
+// the box, global memory outside), so the result is the one of
+// move_ens_kernel for any box.  With the population sorted by cell tile the
+// box stays small.
+template <class T, int Occ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occ > 0 ? Occ : 1)))
+void move_ens_lds_kernel(const EnsStep *__restrict__ steps, const int *__restrict__ stepIdx, Packed<T> S, EnsSoA st,
+                         int n, const float *__restrict__ sigmaTab, int chunk, int halo, int ldsBytes,
+                         unsigned long long *stepCounter) {
+    extern __shared__ __align__(32) unsigned char lds[];
+    __shared__ int part[4][6];
+    __shared__ int box[6];
+    const MoveParams P = steps[*stepIdx].P;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int idx = blockIdx.x * 256 + tid;
+    const bool have = idx < n;
+    Reg q{};
+    float k0 = 0.f, h0 = 0.f, kStart = 0.f, sigma = 0.f;
+    uint64_t id = 0;
+    double ptime = 0;
+    q.done = true;
+    if (have) {
+        const uint32_t cell = st.cell[idx];
+        q.jc = (int)(cell / (uint32_t)P.xi);
+        q.ic = (int)(cell - (uint32_t)q.jc * (uint32_t)P.xi);
+        q.fj = st.fj[idx];
+        q.fi = st.fi[idx];
+        if (q.fj >= 1.0f) { q.fj -= 1.0f; ++q.jc; }
+        if (q.fi >= 1.0f) { q.fi -= 1.0f; ++q.ic; }
+        k0 = st.k[idx];
+        h0 = st.health[idx];
+        double health = h0;
+        const float kLL = (float)P.kLowerLimit;
+        float k = k0;
+        if (k > 0.f) {                                             // Particle.cpp:134-176
+            if (P.upper == WC_CLOSURE_CONSTRAINT) k = 0.f;
+            else if (P.upper == WC_CLOSURE_KILL) health = -1;
+            else if (P.upper == WC_CLOSURE_REFLECTION) k = -k;
+        }
+        if (k < kLL) {
+            if (P.lower == WC_CLOSURE_CONSTRAINT) k = kLL;
+            else if (P.lower == WC_CLOSURE_KILL) health = -1;
+            else if (P.lower == WC_CLOSURE_REFLECTION) k = 2.0f * kLL - k;
+        }
+        kStart = k;
+        q.k = k;
+        q.inside = valid_cell(q.jc, q.ic, P.eta, P.xi) && q.fj == q.fj && q.fi == q.fi && k == k;
+        q.health = health;
+        q.age = st.age[idx];
+        q.healthFresh = true;
+        q.changed = false;
+        q.elapsed = P.elapsed0;
+        q.done = false;
+        sigma = sigmaTab[st.scen[idx]];
+        id = st.id[idx];
+        ptime = st.time[idx];
+    }
+    const double L = P.intervalLength, span = (double)chunk * P.dti;
+    const int nChunks = L > P.elapsed0 ? (int)ceil((L - P.elapsed0) / span) : 0;
+    const int nlev = S.nlev;
+    unsigned total = 0;
+    for (int ch = 0; ch < nChunks; ++ch) {
+        const double tEnd = ch + 1 == nChunks ? L : fmin(L, P.elapsed0 + (double)(ch + 1) * span);
+        // box of the particles that step in this chunk (forward: emitted before its end)
+        const bool act = !q.done && q.inside && q.elapsed < tEnd && q.k >= -1e6f && q.k <= 1e6f &&
+                         (P.dirInt < 0 || ptime - P.intervalStart < tEnd);
+        const int kI = act ? (int)ceilf(q.k) : 0;
+        int v[6] = {act ? q.jc : INT_MAX, act ? -q.jc : INT_MAX, act ? q.ic : INT_MAX,
+                    act ? -q.ic : INT_MAX, act ? kI : INT_MAX, act ? -kI : INT_MAX};
+        for (int off = 32; off > 0; off >>= 1)
+            for (int r = 0; r < 6; ++r) v[r] = min(v[r], __shfl_xor(v[r], off, 64));
+        if (lane == 0)
+            for (int r = 0; r < 6; ++r) part[wave][r] = v[r];
+        __syncthreads();                 // also: every read of the previous chunk's tile is done
+        if (tid == 0) {
+            int m[6];
+            for (int r = 0; r < 6; ++r) m[r] = min(min(part[0][r], part[1][r]), min(part[2][r], part[3][r]));
+            // j0, bh, i0, bw, l0, nl; no box: a 1 x 1 box at (-2, -2) with no
+            // levels, which no cell or stencil lies in
+            int b[6] = {-2, 1, -2, 1, 0, 0};
+            const int lLo = max(0, m[4] - 2 + nlev - 1), lHi = min(nlev - 1, -m[5] + 1 + nlev - 1);
+            const int nl = lHi - lLo + 1;
+            if (m[0] != INT_MAX && nl >= 1) {
+                for (int h = halo; h >= 0; --h) {
+                    const int j0 = max(0, m[0] - h), j1 = min(P.eta - 1, -m[1] + 1 + h);
+                    const int i0 = max(0, m[2] - h), i1 = min(P.xi - 1, -m[3] + 1 + h);
+                    const int bh = j1 - j0 + 1, bw = i1 - i0 + 1, cells = bh * bw;
+                    const long long need = (long long)nl * cells * (long long)(sizeof(WA8<T>) + sizeof(UV4<T>)) +
+                                           (long long)cells * (long long)(sizeof(ZH) + sizeof(float2));
+                    if (need <= ldsBytes) {
+                        b[0] = j0; b[1] = bh; b[2] = i0; b[3] = bw; b[4] = lLo; b[5] = nl;
+                        break;
+                    }
+                }
+            }
+            for (int r = 0; r < 6; ++r) box[r] = b[r];
+        }
+        __syncthreads();
+        TilePacked<T> TS;
+        TS.g = S;
+        TS.j0 = box[0]; TS.bh = box[1]; TS.i0 = box[2]; TS.bw = box[3]; TS.l0 = box[4]; TS.nl = box[5];
+        const int cells = TS.bh * TS.bw;
+        WA8<T> *wL = reinterpret_cast<WA8<T> *>(lds);
+        ZH *zL = reinterpret_cast<ZH *>(lds + (size_t)TS.nl * cells * sizeof(WA8<T>));
+        UV4<T> *uL = reinterpret_cast<UV4<T> *>(reinterpret_cast<unsigned char *>(zL) + (size_t)cells * sizeof(ZH));
+        float2 *mL = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(uL) +
+                                                (size_t)TS.nl * cells * sizeof(UV4<T>));
+        TS.wL = wL; TS.zL = zL; TS.uL = uL; TS.mL = mL;
+        if (TS.nl > 0) {
+            for (int c = tid; c < cells; c += 256) {
+                const int jj = c / TS.bw, ii = c - jj * TS.bw;
+                const size_t g = (size_t)(TS.j0 + jj) * S.xi + (TS.i0 + ii);
+                zL[c] = S.z[g];
+                mL[c] = S.md[g];
+            }
+            const int items = TS.nl * cells;
+            for (int c = tid; c < items; c += 256) {
+                const int l = c / cells, r = c - l * cells;
+                const int jj = r / TS.bw, ii = r - jj * TS.bw;
+                const size_t g = (size_t)(TS.l0 + l) * S.plane + (size_t)(TS.j0 + jj) * S.xi + (TS.i0 + ii);
+                wL[c] = S.wa[g];
+                uL[c] = S.uv[g];
+            }
+        }
+        __syncthreads();
+        if (have) total += run_interval<T>(P, TS, sigma, id, ptime, q, tEnd);
+    }
+    if (have) {
+        if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
+        if (q.changed) {
+            requantize(q.jc, q.fj);
+            requantize(q.ic, q.fi);
+            st.cell[idx] = (uint32_t)q.jc * (uint32_t)P.xi + (uint32_t)q.ic;
+            st.fj[idx] = q.fj;
+            st.fi[idx] = q.fi;
+            st.k[idx] = q.k;
+            st.health[idx] = (float)q.health;
+            st.age[idx] = q.age;
+        } else if (kStart != k0 || (float)q.health != h0) {
+            st.k[idx] = kStart;
             st.health[idx] = (float)q.health;
         }
     }
@@ -818,7 +1043,16 @@ hipError_t ens_launch_t(const EnsLaunch &L, hipStream_t stream) {
         L.steps, L.stepIdx, L.G.hf, (UV4<T> *)S.uv, (WA8<T> *)S.wa, (ZH *)S.z, L.s_rho, L.s_w, plane);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (L.n > 0) {
+    if (L.n > 0 && L.ldsChunk > 0) {
+        const int blocks = (L.n + 255) / 256;
+        if (L.occ == 3)
+            move_ens_lds_kernel<T, 3><<<blocks, 256, kEnsLdsBytes3, stream>>>(
+                L.steps, L.stepIdx, S, L.s, L.n, L.sigma, L.ldsChunk, L.ldsHalo, kEnsLdsBytes3, L.stepCounter);
+        else
+            move_ens_lds_kernel<T, 0><<<blocks, 256, kEnsLdsBytes2, stream>>>(
+                L.steps, L.stepIdx, S, L.s, L.n, L.sigma, L.ldsChunk, L.ldsHalo, kEnsLdsBytes2, L.stepCounter);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    } else if (L.n > 0) {
         const long long per = (long long)L.block * L.ppt;
         const int blocks = (int)((L.n + per - 1) / per);
         if (L.occ == 4)
@@ -860,6 +1094,7 @@ hipError_t launch_ens_release(const EnsSource *src, const unsigned long long *fi
 hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream) {
     if (L.block != 64 && L.block != 128 && L.block != 256) return hipErrorInvalidValue;
     if (L.ppt < 1 || (L.occ != 0 && L.occ != 3 && L.occ != 4)) return hipErrorInvalidValue;
+    if (L.ldsChunk < 0 || L.ldsHalo < 0 || (L.ldsChunk > 0 && L.occ == 4)) return hipErrorInvalidValue;
     if (store == WC_STORE_F16) return ens_launch_t<_Float16>(L, stream);
     if (store == WC_STORE_BF16) return ens_launch_t<uint16_t>(L, stream);
     return ens_launch_t<float>(L, stream);
