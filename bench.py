@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4     # wave64 VALU instructions per second (MI355X)
 
 
 def log(msg):
@@ -315,10 +316,25 @@ def main():
     bytes_per_launch = steps_per_launch * bps + args.particles * STATE_BYTES
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
+    valu = None
     tpath = os.path.join(ROOT, "profiles", "move_parity_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        valu = prof.get("valu_per_wave_substep")
+    # instruction issue: wave-level VALU instructions per second against the
+    # chip's issue rate (one wave64 VALU instruction per SIMD every 4 cycles:
+    # 256 CUs x 4 SIMDs x 2.4 GHz / 4, the rate behind the 78.6 TFLOP/s FP64
+    # vector peak = that x 64 lanes x 2 flop per FMA)
+    issue = None
+    if valu:
+        wave_substeps = steps_per_launch / 64.0
+        issue_rate = valu * wave_substeps / avg_launch_s
+        issue = {"what": "VALU instructions per wave-substep (rocprofv3 SQ_INSTS_VALU) x wave-substeps per launch "
+                         "/ the launch time measured here",
+                 "valu_per_wave_substep": valu, "achieved": issue_rate, "peak": VALU_ISSUE_PEAK,
+                 "unit": "wave-instr/s", "frac": issue_rate / VALU_ISSUE_PEAK}
 
     cpu = None if args.no_cpu else cpu_baseline(W, cfg, p0)
 
@@ -357,6 +373,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "move_parity_kernel<SoAAccess>",
             "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_particle_step": bps,
+            "issue": issue,
         },
         "cpu_baseline": cpu,
         "variants": variants_out,
