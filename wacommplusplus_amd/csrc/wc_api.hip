@@ -1373,7 +1373,9 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
             for (int chunk : {10, 30}) cands.push_back(Cand{256, 1, occ, tile, chunk});
     }
     double best = INFINITY;
-    Cand keep{E.block, E.ppt, E.occ, E.sortTile, E.ldsChunk};
+    const Cand orig{E.block, E.ppt, E.occ, E.sortTile, E.ldsChunk};
+    const int origPeriod = E.sortPeriod;
+    Cand keep = orig;
     E.sortPeriod = 1;
     const int nc = (int)cands.size();
     for (int trial = -1; trial < nc && !rc; ++trial) {
@@ -1398,6 +1400,10 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (!rc) rc = copy_ens_soa(c, E.s, E.save, E.n);
+    if (rc) {                                   // leave the caller's launch as it was
+        E.block = orig.block; E.ppt = orig.ppt; E.occ = orig.occ; E.sortTile = orig.tile;
+        E.ldsChunk = orig.chunk; E.sortPeriod = origPeriod;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (rc) return rc;
     free_ens_graph(c);
