@@ -152,10 +152,22 @@ struct WgradArgs {
     float *dw;
     int n, h, w;
     int splits;
+    // workspace of the tiled path (per-workgroup partial sums, then a
+    // reduction): partCap floats; nullptr selects the atomic path
+    float *part;
+    size_t partCap;
+    // the masked output gradient already in BF16 (or nullptr: dy and y)
+    const uint16_t *dyb;
 };
+// floats of workspace that launch_wgrad's tiled path may use
+constexpr size_t kWgradPartFloats = (size_t)1024 * 9 * 1024;
 hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s);
 hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s);
-hipError_t launch_bias_grad(const float *dy, const uint16_t *y, long long P, int cout, float *db, hipStream_t s);
+// out[e] = bf16(y[e] > 0 ? dy[e] : 0), a layer's ReLU-masked output gradient
+// (E = pixels x cout), and its bias gradient db[co] += sum of the masked dy
+// (partial sums in part, partCap floats)
+hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *db,
+                            float *part, size_t partCap, hipStream_t s);
 hipError_t launch_pool_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
 hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, hipStream_t s);
 hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);

@@ -223,25 +223,192 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
     }
 }
 
-// db[co] += sum_p dY[p][co] * (Y[p][co] > 0); one block per 32 channels x pixel slice
-__global__ __launch_bounds__(256) void bias_grad_kernel(const float *__restrict__ dy, const uint16_t *__restrict__ y,
-                                                        long long P, int cout, float *__restrict__ db) {
-    const int c = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int lanegrp = threadIdx.x >> 5;                        // 8 pixel lanes per channel
-    float s = 0.f;
-    if (c < cout)
-        for (long long p = (long long)blockIdx.y * 8 + lanegrp; p < P; p += (long long)gridDim.y * 8) {
-            const size_t o = (size_t)p * cout + c;
-            if (from_bf16(y[o]) > 0.f) s += dy[o];
-        }
-    __shared__ float part[8][32];
-    part[lanegrp][threadIdx.x & 31] = s;
-    __syncthreads();
-    if (threadIdx.x < 32 && c < cout) {
-        float t = 0.f;
-        for (int g = 0; g < 8; ++g) t += part[g][threadIdx.x];
-        if (t != 0.f) atomicAdd(&db[c], t);
+// Tiled weight gradient for widths that are a multiple of 8.  A workgroup
+// owns 32 output channels x 32 input channels and all 9 taps; it walks
+// stages of R = 256 / w image rows (whole rows of one image).  Per stage it
+// stages in LDS, channel-major with pixels contiguous:
+//   sdy[co][r][x]      the ReLU-masked output gradient (BF16), R rows;
+//   sx[ci][rr][8 + x]  the input rows y0-1 .. y0+R with an 8-element zero
+//                      pad each side (rows outside the image are zero).
+// Each wave takes one 16 x 16 (co, ci) tile and keeps 9 tap accumulators.
+// For a chunk of 32 pixels a lane's MFMA k-group is 8 consecutive pixels of
+// one row: A = 8 pixels of dY (one aligned 16-byte read); for each row
+// offset dy, three aligned 16-byte reads of the X row give the windows for
+// dx = -1, 0, +1 by byte alignment.  9 MFMAs per chunk.  Partial sums go to
+// part[z][co][tap][ci] (no atomics); wgrad_reduce_kernel adds them into dW.
+constexpr int kWgPix = 256;
+
+__device__ __forceinline__ bf16x8 window(const uint4 &lo, const uint4 &mid, const uint4 &hi, int dx) {
+    Pack8 r;
+    if (dx == 0) {
+        r.u = mid;
+    } else if (dx < 0) {      // elements -1 .. 6 of mid
+        r.u.x = __builtin_amdgcn_alignbyte(mid.x, lo.w, 2);
+        r.u.y = __builtin_amdgcn_alignbyte(mid.y, mid.x, 2);
+        r.u.z = __builtin_amdgcn_alignbyte(mid.z, mid.y, 2);
+        r.u.w = __builtin_amdgcn_alignbyte(mid.w, mid.z, 2);
+    } else {                  // elements 1 .. 8 of mid
+        r.u.x = __builtin_amdgcn_alignbyte(mid.y, mid.x, 2);
+        r.u.y = __builtin_amdgcn_alignbyte(mid.z, mid.y, 2);
+        r.u.z = __builtin_amdgcn_alignbyte(mid.w, mid.z, 2);
+        r.u.w = __builtin_amdgcn_alignbyte(hi.x, mid.w, 2);
     }
+    return r.v;
+}
+
+__global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int rowsPerImg, int stages) {
+    extern __shared__ __align__(16) uint16_t smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int co0 = blockIdx.x * 32, ci0 = blockIdx.y * 32;
+    const int W = a.w, H = a.h, hw = H * W;
+    const int cin = a.c1 + a.c2;
+    const bool first = ci0 < a.c1;
+    const uint16_t *xs = first ? a.x1 : a.x2;
+    const int cc = first ? a.c1 : a.c2, cl0 = first ? ci0 : ci0 - a.c1;
+    const int RW = R * W, rowX = W + 16;
+    const int planeD = RW + 8, planeX = (R + 2) * rowX + 8;      // padded channel planes
+    uint16_t *sdy = smem;
+    uint16_t *sx = smem + 32 * planeD;
+    const int ti = wave >> 1, tj = wave & 1, row = lane & 15, kq = lane >> 4;
+    f32x4 acc[9];
+    for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int st = blockIdx.z; st < stages; st += gridDim.z) {
+        const int img = st / rowsPerImg, y0 = (st - img * rowsPerImg) * R;
+        const int rv = min(R, H - y0);                 // rows of this stage inside the image
+        __syncthreads();                               // the previous stage's reads are done
+        // masked dY in BF16: 8 channels (16 B) of one pixel per work-item
+        for (int e = tid; e < RW * 4; e += 256) {
+            const int pix = e >> 2, c8 = (e & 3) * 8;
+            const int r = pix / W;
+            Pack8 v;
+            v.u = make_uint4(0, 0, 0, 0);
+            if (r < rv) {
+                const size_t o = ((size_t)img * hw + (size_t)(y0 + r) * W + (pix - r * W)) * a.cout + co0 + c8;
+                if (a.dyb) {
+                    v.u = *(const uint4 *)(a.dyb + o);
+                } else {
+                    const float4 d0 = *(const float4 *)(a.dy + o), d1 = *(const float4 *)(a.dy + o + 4);
+                    const uint4 m = *(const uint4 *)(a.y + o);
+                    const float d[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+                    const uint32_t mw[4] = {m.x, m.y, m.z, m.w};
+                    for (int q = 0; q < 8; ++q)
+                        v.h[q] = to_bf16(from_bf16((uint16_t)(mw[q >> 1] >> (16 * (q & 1)))) > 0.f ? d[q] : 0.f);
+                }
+            }
+            for (int q = 0; q < 8; ++q) sdy[(c8 + q) * planeD + pix] = v.h[q];
+        }
+        // X rows y0-1 .. y0+R, padded: 8 channels (16 B) of one position per work-item
+        const int positions = (R + 2) * rowX;
+        for (int e = tid; e < positions * 4; e += 256) {
+            const int pos = e >> 2, c8 = (e & 3) * 8;
+            const int rr = pos / rowX, xx = pos - rr * rowX - 8;
+            const int y = y0 - 1 + rr;
+            Pack8 v;
+            v.u = make_uint4(0, 0, 0, 0);
+            if (y >= 0 && y < H && xx >= 0 && xx < W)
+                v.u = *(const uint4 *)(xs + ((size_t)img * hw + (size_t)y * W + xx) * cc + cl0 + c8);
+            for (int q = 0; q < 8; ++q) sx[(c8 + q) * planeX + pos] = v.h[q];
+        }
+        __syncthreads();
+        const uint16_t *dRow = sdy + (ti * 16 + row) * planeD;
+        const uint16_t *xPlane = sx + (tj * 16 + row) * planeX;
+        for (int pb = 0; pb < RW; pb += 32) {
+            // k-groups past the stage's pixels (R * W not a multiple of 32)
+            // contribute A = 0 and read row 0
+            const int g = pb + 8 * kq;
+            const bool gv = g < RW;
+            const int r = gv ? g / W : 0, x = gv ? g - r * W : 0;     // x is a multiple of 8
+            Pack8 A;
+            A.u = gv ? *(const uint4 *)(dRow + g) : make_uint4(0, 0, 0, 0);   // zero rows beyond the image
+            for (int dy = 0; dy < 3; ++dy) {
+                const uint16_t *xr = xPlane + (r + dy) * rowX + 8 + x;
+                const uint4 lo = *(const uint4 *)(xr - 8), mid = *(const uint4 *)xr, hi = *(const uint4 *)(xr + 8);
+                for (int dx = 0; dx < 3; ++dx)
+                    acc[dy * 3 + dx] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, window(lo, mid, hi, dx - 1),
+                                                                                acc[dy * 3 + dx], 0, 0, 0);
+            }
+        }
+    }
+    // D[4 kq + r][row]: co = co0 + 16 ti + 4 kq + r, ci = ci0 + 16 tj + row
+    float *dst = a.part + (size_t)blockIdx.z * a.cout * 9 * cin;
+    for (int t = 0; t < 9; ++t)
+        for (int r = 0; r < 4; ++r) {
+            const int co = co0 + ti * 16 + kq * 4 + r, ci = ci0 + tj * 16 + row;
+            dst[((size_t)co * 9 + t) * cin + ci] = acc[t][r];
+        }
+}
+
+// Deterministic reduction of part[zn][E] into dW in two passes: pass 1 sums
+// each chunk of kWgChunk slices into the chunk's first slice (grid: E blocks
+// x chunks); pass 2 adds the chunk sums into dW.
+constexpr int kWgChunk = 32;
+
+__global__ void wgrad_reduce_chunks_kernel(float *__restrict__ part, int zn, size_t E) {
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const int z0 = blockIdx.y * kWgChunk, z1 = min(zn, z0 + kWgChunk);
+    if (e >= E) return;
+    float s = 0.f;
+    for (int z = z0; z < z1; ++z) s += part[(size_t)z * E + e];
+    part[(size_t)z0 * E + e] = s;
+}
+
+__global__ void wgrad_reduce_kernel(const float *__restrict__ part, int zn, size_t E, float *__restrict__ dw) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < E; e += (size_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int z = 0; z < zn; z += kWgChunk) s += part[(size_t)z * E + e];
+        dw[e] += s;
+    }
+}
+
+// The ReLU-masked output gradient of a layer in BF16, and each workgroup's
+// sums of the masked dY per channel (FP32) in part[block][cout], which
+// bias_reduce_kernel adds into the bias gradient.  A work-item takes 4
+// channels; cout divides 1024, so its channels stay the same over the
+// grid-stride loop.
+__global__ __launch_bounds__(256) void mask_bf16_kernel(const float *__restrict__ dy, const uint16_t *__restrict__ y,
+                                                        long long E4, int cout, uint16_t *__restrict__ out,
+                                                        float *__restrict__ part) {
+    const int tid = threadIdx.x;
+    float sb[4] = {0.f, 0.f, 0.f, 0.f};
+    for (long long q = blockIdx.x * 256LL + tid; q < E4; q += (long long)gridDim.x * 256) {
+        const float4 d = ((const float4 *)dy)[q];
+        const uint2 m = ((const uint2 *)y)[q];
+        const float g0 = from_bf16((uint16_t)m.x) > 0.f ? d.x : 0.f;
+        const float g1 = from_bf16((uint16_t)(m.x >> 16)) > 0.f ? d.y : 0.f;
+        const float g2 = from_bf16((uint16_t)m.y) > 0.f ? d.z : 0.f;
+        const float g3 = from_bf16((uint16_t)(m.y >> 16)) > 0.f ? d.w : 0.f;
+        sb[0] += g0; sb[1] += g1; sb[2] += g2; sb[3] += g3;
+        uint2 r;
+        r.x = (uint32_t)to_bf16(g0) | (uint32_t)to_bf16(g1) << 16;
+        r.y = (uint32_t)to_bf16(g2) | (uint32_t)to_bf16(g3) << 16;
+        ((uint2 *)out)[q] = r;
+    }
+    __shared__ float red[256][4];
+    for (int k = 0; k < 4; ++k) red[tid][k] = sb[k];
+    __syncthreads();
+    const int groups = cout / 4;               // work-items of one channel set: tid, tid + groups, ...
+    if (tid < groups)
+        for (int k = 0; k < 4; ++k) {
+            float s = 0.f;
+            for (int j = tid; j < 256; j += groups) s += red[j][k];
+            part[(size_t)blockIdx.x * cout + 4 * tid + k] = s;
+        }
+}
+
+// db[c] += sum_b part[b][c]: one workgroup per channel, a fixed-order tree
+__global__ __launch_bounds__(256) void bias_reduce_kernel(const float *__restrict__ part, int nb, int cout,
+                                                          float *__restrict__ db) {
+    const int c = blockIdx.x, tid = threadIdx.x;
+    float s = 0.f;
+    for (int b = tid; b < nb; b += 256) s += part[(size_t)b * cout + c];
+    __shared__ float red[256];
+    red[tid] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) red[tid] += red[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) db[c] += red[0];
 }
 
 // 2x2 average pooling, BF16 NHWC (h, w even)
@@ -331,37 +498,43 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *__restric
                                                        const float *__restrict__ t, const float *__restrict__ wh,
                                                        long long P, float *__restrict__ dd,
                                                        float *__restrict__ dwh, float *__restrict__ loss) {
-    __shared__ float part[256];
+    // four work-items per pixel, 8 of its kHead channels each (16-byte loads)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cg = tid & 3;
     const float inv = 1.0f / (float)P;
-    float l = 0.f;
-    float gw[kHead + 1];
-    for (int q = 0; q <= kHead; ++q) gw[q] = 0.f;
-    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < P; p += (long long)gridDim.x * blockDim.x) {
+    float v[10];                          // dwh of the 8 channels, the bias, the squared error
+    for (int q = 0; q < 10; ++q) v[q] = 0.f;
+    float w8[8];
+    for (int q = 0; q < 8; ++q) w8[q] = wh[8 * cg + q];
+    for (long long p = (blockIdx.x * 256LL + tid) >> 2; p < P; p += (long long)gridDim.x * 64) {
         const float e = out[p] - t[p];
-        l += e * e;
         const float g = 2.0f * e * inv;
-        for (int q = 0; q < kHead; ++q) {
-            gw[q] += g * from_bf16(d[p * kHead + q]);
-            dd[p * kHead + q] = g * wh[q];
+        if (cg == 0) {
+            v[8] += g;
+            v[9] += e * e;
         }
-        gw[kHead] += g;
+        Pack8 dv;
+        dv.u = *(const uint4 *)(d + p * kHead + 8 * cg);
+        for (int q = 0; q < 8; ++q) v[q] += g * from_bf16(dv.h[q]);
+        float *o = dd + p * kHead + 8 * cg;
+        *(float4 *)o = make_float4(g * w8[0], g * w8[1], g * w8[2], g * w8[3]);
+        *(float4 *)(o + 4) = make_float4(g * w8[4], g * w8[5], g * w8[6], g * w8[7]);
     }
-    part[threadIdx.x] = l;
+    // sum over the lanes of a wave with the same cg, then over the waves
+    for (int off = 4; off < 64; off <<= 1)
+        for (int q = 0; q < 10; ++q) v[q] += __shfl_xor(v[q], off, 64);
+    __shared__ float red[4][4][10];
+    if (lane < 4)
+        for (int q = 0; q < 10; ++q) red[wave][lane][q] = v[q];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, part[0] * inv);
-    for (int q = 0; q <= kHead; ++q) {
-        __syncthreads();
-        part[threadIdx.x] = gw[q];
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
-            __syncthreads();
+    if (tid < 40) {
+        const int c = tid / 10, q = tid - c * 10;
+        const float s = red[0][c][q] + red[1][c][q] + red[2][c][q] + red[3][c][q];
+        if (q < 8) {
+            if (s != 0.f) atomicAdd(&dwh[8 * c + q], s);
+        } else if (c == 0) {
+            if (q == 8 && s != 0.f) atomicAdd(&dwh[kHead], s);
+            if (q == 9) atomicAdd(loss, s * inv);
         }
-        if (threadIdx.x == 0 && part[0] != 0.f) atomicAdd(&dwh[q], part[0]);
     }
 }
 
@@ -426,14 +599,44 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
 hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
     const int cin = a.c1 + a.c2;
     if ((cin % 32) || (a.cout % 32) || a.splits < 1) return hipErrorInvalidValue;
+    // tiled path: widths that are a multiple of 8, whose stage fits in LDS
+    if (a.part && a.w % 8 == 0 && a.w <= kWgPix) {
+        const int R = kWgPix / a.w < a.h ? kWgPix / a.w : a.h;
+        const size_t lds = (size_t)(32 * (R * a.w + 8) + 32 * ((R + 2) * (a.w + 16) + 8)) * sizeof(uint16_t);
+        const int rowsPerImg = (a.h + R - 1) / R, stages = a.n * rowsPerImg;
+        const int blocks = (a.cout / 32) * (cin / 32);
+        int zn = 512 / blocks;
+        if (zn > stages) zn = stages;
+        if (zn < 1) zn = 1;
+        const size_t E = (size_t)a.cout * 9 * cin;
+        if (lds <= 64 * 1024 && E * zn <= a.partCap) {
+            const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)zn);
+            wgrad_tile_kernel<<<grid, 256, lds, s>>>(a, R, rowsPerImg, stages);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            const dim3 g1((unsigned)((E + 255) / 256), (unsigned)((zn + kWgChunk - 1) / kWgChunk));
+            wgrad_reduce_chunks_kernel<<<g1, 256, 0, s>>>(a.part, zn, E);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            wgrad_reduce_kernel<<<blocks_for((long long)E), 256, 0, s>>>(a.part, zn, E, a.dw);
+            return hipGetLastError();
+        }
+    }
     const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)(9 * a.splits));
     wgrad_kernel<<<grid, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
-hipError_t launch_bias_grad(const float *dy, const uint16_t *y, long long P, int cout, float *db, hipStream_t s) {
-    const dim3 grid((unsigned)((cout + 31) / 32), 256);
-    bias_grad_kernel<<<grid, 256, 0, s>>>(dy, y, P, cout, db);
+// E = pixels x cout; cout a multiple of 4 that divides 1024; part holds
+// blocks x cout floats (at most 16384 x cout)
+hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *db,
+                            float *part, size_t partCap, hipStream_t s) {
+    if (cout < 4 || cout > 1024 || cout % 4 || 1024 % cout || E % cout) return hipErrorInvalidValue;
+    const int b = blocks_for(E / 4);
+    if ((size_t)b * cout > partCap) return hipErrorInvalidValue;
+    mask_bf16_kernel<<<b, 256, 0, s>>>(dy, y, E / 4, cout, out, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    bias_reduce_kernel<<<cout, 256, 0, s>>>(part, b, cout, db);
     return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ struct wc_unet {
     float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
     uint16_t *d_wb = nullptr, *d_wf = nullptr;     // BF16 weights, flipped scratch
     float *d_loss = nullptr;
+    float *d_part = nullptr;                     // weight-gradient partial sums (kWgradPartFloats)
     int adam_t = 0;
     double fwd_ms = 0;
     // activations / gradients for the current shape
@@ -57,6 +58,7 @@ struct wc_unet {
              *p2 = nullptr, *a3 = nullptr, *s3 = nullptr, *p3 = nullptr, *a4 = nullptr, *b4 = nullptr,
              *u3 = nullptr, *d3a = nullptr, *d3 = nullptr, *u2 = nullptr, *d2a = nullptr, *d2 = nullptr,
              *u1 = nullptr, *d1a = nullptr, *d1 = nullptr;
+    uint16_t *gmask = nullptr;      // a layer's ReLU-masked output gradient in BF16 (backward scratch)
     float *x = nullptr, *out = nullptr, *tgt = nullptr;
     float *g_d1 = nullptr, *g_d1a = nullptr, *g_s1 = nullptr, *g_u1 = nullptr, *g_d2 = nullptr, *g_d2a = nullptr,
           *g_s2 = nullptr, *g_u2 = nullptr, *g_d3 = nullptr, *g_d3a = nullptr, *g_s3 = nullptr, *g_u3 = nullptr,
@@ -117,7 +119,7 @@ int ensure_shape(wc_unet *u, int n, int h, int w) {
     rc |= alloc(u, u->a4, P3 * 8 * C) | alloc(u, u->b4, P3 * 8 * C) | alloc(u, u->u3, P2 * 8 * C);
     rc |= alloc(u, u->d3a, P2 * 4 * C) | alloc(u, u->d3, P2 * 4 * C) | alloc(u, u->u2, P1 * 4 * C);
     rc |= alloc(u, u->d2a, P1 * 2 * C) | alloc(u, u->d2, P1 * 2 * C) | alloc(u, u->u1, P0 * 2 * C);
-    rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C);
+    rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C) | alloc(u, u->gmask, P0 * C);
     rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0);
     rc |= alloc(u, u->g_d1, P0 * C) | alloc(u, u->g_d1a, P0 * C) | alloc(u, u->g_s1, P0 * C) |
           alloc(u, u->g_u1, P0 * 2 * C);
@@ -158,17 +160,23 @@ int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, con
     g.dw = u->d_g + L.w_off; g.n = u->n; g.h = h; g.w = w;
     long long sp = P / 2048;
     g.splits = (int)(sp < 1 ? 1 : (sp > 1024 ? 1024 : sp));
+    g.part = u->d_part;
+    g.partCap = kWgradPartFloats;
+    // the masked gradient in BF16, once for the weight gradient and the data
+    // gradient, with the bias gradient
+    UCHK(u, launch_mask_bf16(gy, y, P * L.cout, L.cout, u->gmask, u->d_g + L.b_off, u->d_part, kWgradPartFloats,
+                             u->stream));
+    g.dyb = u->gmask;
     UCHK(u, launch_wgrad(g, u->stream));
-    UCHK(u, launch_bias_grad(gy, y, P, L.cout, u->d_g + L.b_off, u->stream));
     if (!gx1) return WC_OK;
     const int cin = L.c1 + L.c2;
     UCHK(u, launch_flip(u->d_wb + L.wb_off, L.cout, cin, u->d_wf, u->stream));
     ConvArgs a{};
-    a.x1 = gy; a.x2 = nullptr; a.m1 = y; a.c1 = L.cout; a.c2 = 0;
+    a.x1 = u->gmask; a.x2 = nullptr; a.m1 = nullptr; a.c1 = L.cout; a.c2 = 0;
     a.w_ = u->d_wf; a.bias = nullptr;
     a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
     a.n = u->n; a.h = h; a.w = w;
-    UCHK(u, launch_conv3x3(a, true, u->stream));
+    UCHK(u, launch_conv3x3(a, false, u->stream));
     return WC_OK;
 }
 
@@ -323,7 +331,8 @@ int wc_unet_create(int device, uint64_t seed, wc_unet **out) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&u->stream) != hipSuccess) return bad(WC_ERR_HIP);
     if (hipMalloc(&u->d_p, u->nparam * 4) || hipMalloc(&u->d_g, u->nparam * 4) || hipMalloc(&u->d_m, u->nparam * 4) ||
         hipMalloc(&u->d_v, u->nparam * 4) || hipMalloc(&u->d_wb, u->nwb * 2 + 256) ||
-        hipMalloc(&u->d_wf, maxw * 2 + 256) || hipMalloc(&u->d_loss, 256))
+        hipMalloc(&u->d_wf, maxw * 2 + 256) || hipMalloc(&u->d_loss, 256) ||
+        hipMalloc(&u->d_part, kWgradPartFloats * sizeof(float)))
         return bad(WC_ERR_NOMEM);
     if (hipMemcpy(u->d_p, p.data(), u->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return bad(WC_ERR_HIP);
     if (launch_fill_f32(u->d_m, (long long)u->nparam, 0.f, u->stream) != hipSuccess ||
@@ -340,7 +349,7 @@ int wc_unet_destroy(wc_unet *u) {
     if (u->stream) hipStreamSynchronize(u->stream);
     free_shape(u);
     hipFree(u->d_p); hipFree(u->d_g); hipFree(u->d_m); hipFree(u->d_v); hipFree(u->d_wb); hipFree(u->d_wf);
-    hipFree(u->d_loss);
+    hipFree(u->d_loss); hipFree(u->d_part);
     if (u->stream) hipStreamDestroy(u->stream);
     delete u;
     return WC_OK;
