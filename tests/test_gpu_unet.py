@@ -2,10 +2,15 @@
 forward and backward kernels (BF16 MFMA, FP32 accumulation) against the
 FP64 oracle (oracle/unet_oracle.py, pinned by finite differences), which
 rounds to BF16 where the device does.  The remaining difference is FP32 vs
-FP64 accumulation, which occasionally moves a value across a BF16 rounding
-boundary (one BF16 ulp is 0.4 %); the bars are relative L2 errors:
-forward < 5e-3, every layer's weight and bias gradient < 3e-2, loss < 1e-3.
-A layout or indexing mistake gives O(1) errors.
+FP64 accumulation, which now and then moves a value across a BF16 rounding
+boundary (one BF16 ulp is 0.4 %) or flips a ReLU mask; on the small coarse
+levels of these images (60 and 15 pixels for 24 x 40) a few such flips move
+a layer's gradient by percents.  Measured over 2x16x16, 1x24x40 and 4x32x32
+with two kernels of different summation order: whole gradient 2e-4 .. 1.3e-2,
+worst layer 1.5e-3 .. 4.4e-2, and both kernels equally far from the exact
+FP64 oracle.  The bars are relative L2 errors: forward < 5e-3, loss < 1e-3,
+whole gradient < 3e-2, every layer's weight and bias gradient < 1e-1.  A
+layout or indexing mistake gives O(1) errors in the layer it touches.
 """
 import numpy as np
 import pytest
@@ -35,7 +40,7 @@ def test_param_layout(net):
     assert all(np.all(b == 0) for _, b in layers)
 
 
-@pytest.mark.parametrize("shape", [(2, 16, 16), (1, 24, 40)])
+@pytest.mark.parametrize("shape", [(2, 16, 16), (1, 24, 40), (4, 32, 32)])
 def test_forward_and_gradients_match_oracle(net, shape):
     rng = np.random.default_rng(sum(shape))
     p = uo.he_params(seed=4).astype(np.float32)
@@ -57,11 +62,13 @@ def test_forward_and_gradients_match_oracle(net, shape):
         ew = rel(g[off:off + nw], g_ref[off:off + nw])
         eb = rel(g[off + nw:off + nw + co], g_ref[off + nw:off + nw + co])
         worst = max(worst, ew, eb)
-        assert ew < 3e-2 and eb < 3e-2, (l, ew, eb)
+        assert ew < 1e-1 and eb < 1e-1, (l, ew, eb)
         off += nw + co
     eh = rel(g[off:], g_ref[off:])
     assert eh < 3e-2, eh
-    print(f"{shape}: worst layer gradient rel err {worst:.2e}, head {eh:.2e}", flush=True)
+    whole = rel(g, g_ref)
+    assert whole < 3e-2, whole
+    print(f"{shape}: gradient rel err {whole:.2e}, worst layer {worst:.2e}, head {eh:.2e}", flush=True)
 
 
 def blobs(rng, n, h, w, k=3):

@@ -164,6 +164,115 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
     }
 }
 
+// LDS-tiled 3x3 convolution (BF16 input, no input mask).  A workgroup owns R
+// whole rows of one image (R * w <= 256 pixels) and 32 output channels.  Per
+// 32-channel input chunk each wave loads its B operands of the 9 taps (the
+// chunk's weights of its 32 output channels) into registers, and the
+// workgroup stages in LDS
+//   sx[(rr, xx)][40]   input rows y0-1 .. y0+R, columns -1 .. w (zero outside
+//                      the image), 32 channels padded to 40 (80 B per pixel:
+//                      16-byte aligned, banks spread);
+// then each wave computes 64 pixels (4 MFMA row tiles) x 32 channels: per
+// tap 4 A reads from LDS feed 8 MFMAs.  The epilogue is the one of
+// conv3x3_kernel.
+constexpr int kCvPS = 40;
+
+__global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, int rowsPerImg) {
+    extern __shared__ __align__(16) uint16_t smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int W = a.w, H = a.h, hw = H * W, Wp = W + 2;
+    const int img = blockIdx.x / rowsPerImg, y0 = (blockIdx.x - img * rowsPerImg) * R;
+    const int rv = min(R, H - y0), RW = R * W;
+    const int co0 = blockIdx.y * 32;
+    const int nout = a.cy1 + a.cy2, cin = a.c1 + a.c2;
+    uint16_t *sx = smem;                                  // [(R+2) * (W+2)][kCvPS]
+    const int row = lane & 15, kq = lane >> 4;
+    // this lane's A rows: stage pixel 64 wave + 16 i + row (its centre-tap
+    // LDS offset; pixels past the stage read pixel 0 and are not stored)
+    int base[4];
+    for (int i = 0; i < 4; ++i) {
+        const int q = 64 * wave + 16 * i + row;
+        const int r = q < RW ? q / W : 0, x = q < RW ? q - r * W : 0;
+        base[i] = ((r + 1) * Wp + (x + 1)) * kCvPS + 8 * kq;
+    }
+    const bool active = 64 * wave < RW;
+    f32x4 acc[4][2];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int npos = (R + 2) * Wp;
+    for (int c0 = 0; c0 < cin; c0 += 32) {
+        const bool first = c0 < a.c1;
+        const uint16_t *xs = (const uint16_t *)(first ? a.x1 : a.x2);
+        const int cc = first ? a.c1 : a.c2, cl0 = first ? c0 : c0 - a.c1;
+        // B of the 9 taps: channels c0 + 8 kq .. +7 of output channels co0 + 16 j + row
+        Pack8 B[9][2];
+        if (active)
+            for (int tap = 0; tap < 9; ++tap)
+                for (int j = 0; j < 2; ++j) {
+                    const int co = co0 + 16 * j + row;
+                    B[tap][j].u = co < nout ? *(const uint4 *)(a.w_ + ((size_t)co * 9 + tap) * cin + c0 + 8 * kq)
+                                            : make_uint4(0, 0, 0, 0);
+                }
+        __syncthreads();                                  // the previous chunk's reads are done
+        int rr = (tid >> 2) / Wp, xx = (tid >> 2) - rr * Wp;          // position of e >> 2, by counting
+        for (int e = tid; e < npos * 4; e += 256) {
+            const int pos = e >> 2, g = e & 3;
+            const int y = y0 - 1 + rr, x = xx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (y >= 0 && y < H && x >= 0 && x < W)
+                v = *(const uint4 *)(xs + ((size_t)img * hw + (size_t)y * W + x) * cc + cl0 + 8 * g);
+            *(uint4 *)(sx + pos * kCvPS + 8 * g) = v;
+            for (xx += 64; xx >= Wp; xx -= Wp) ++rr;
+        }
+        __syncthreads();
+        if (active)
+            for (int tap = 0; tap < 9; ++tap) {
+                const int off = ((tap / 3 - 1) * Wp + (tap % 3 - 1)) * kCvPS;
+                for (int i = 0; i < 4; ++i) {
+                    Pack8 A;
+                    A.u = *(const uint4 *)(sx + base[i] + off);
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, B[tap][j].v, acc[i][j], 0, 0, 0);
+                }
+            }
+    }
+    if (!active) return;
+    // D[4 kq + r][row] of tile (i, j): stage pixel 64 wave + 16 i + 4 kq + r,
+    // channel co0 + 16 j + row; rows of the stage pixels by counting
+    int er[4][4];
+    for (int i = 0; i < 4; ++i) {
+        const int q0 = 64 * wave + 16 * i + 4 * kq;
+        int rr = q0 / W, xx = q0 - rr * W;
+        for (int r = 0; r < 4; ++r) {
+            er[i][r] = rr;
+            if (++xx == W) { xx = 0; ++rr; }
+        }
+    }
+    for (int j = 0; j < 2; ++j) {
+        const int co = co0 + j * 16 + row;
+        if (co >= nout) continue;
+        const bool first = co < a.cy1;
+        const int cl = first ? co : co - a.cy1;
+        const int cc = first ? a.cy1 : a.cy2;
+        for (int i = 0; i < 4; ++i)
+            for (int r = 0; r < 4; ++r) {
+                const int q = 64 * wave + 16 * i + 4 * kq + r;
+                if (q >= RW || er[i][r] >= rv) continue;
+                // stage pixel q of image row y0 + er is image pixel y0 * W + q
+                const long long p = (long long)img * hw + (long long)y0 * W + q;
+                float v = acc[i][j][r];
+                if (a.mode == 0) {                                  // forward: bias + ReLU, BF16
+                    v += a.bias[co];
+                    ((uint16_t *)a.y1)[p * a.cy1 + co] = to_bf16(v > 0.f ? v : 0.f);
+                } else {
+                    float *y = (float *)(first ? a.y1 : a.y2) + p * cc + cl;
+                    const bool accum = first ? a.acc1 : a.acc2;
+                    *y = accum ? *y + v : v;
+                }
+            }
+    }
+}
+
 // dW[co][tap][ci] += sum_p dY[p][co] * (Y[p][co] > 0) * X[p + tap][ci];
 // X = concat(x1, x2) along channels.  Grid: (cout/32, cin/32, 9 * splits).
 __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
@@ -276,14 +385,16 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
         const int img = st / rowsPerImg, y0 = (st - img * rowsPerImg) * R;
         const int rv = min(R, H - y0);                 // rows of this stage inside the image
         __syncthreads();                               // the previous stage's reads are done
-        // masked dY in BF16: 8 channels (16 B) of one pixel per work-item
+        // masked dY in BF16: 8 channels (16 B) of one pixel per work-item;
+        // pixel pix = (e >> 2) advances by 64 per step, its row r and column
+        // x by counting (no division in the loop)
+        int r = (tid >> 2) / W, x0 = (tid >> 2) - r * W;
         for (int e = tid; e < RW * 4; e += 256) {
             const int pix = e >> 2, c8 = (e & 3) * 8;
-            const int r = pix / W;
             Pack8 v;
             v.u = make_uint4(0, 0, 0, 0);
             if (r < rv) {
-                const size_t o = ((size_t)img * hw + (size_t)(y0 + r) * W + (pix - r * W)) * a.cout + co0 + c8;
+                const size_t o = ((size_t)img * hw + (size_t)(y0 + r) * W + x0) * a.cout + co0 + c8;
                 if (a.dyb) {
                     v.u = *(const uint4 *)(a.dyb + o);
                 } else {
@@ -296,28 +407,33 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
                 }
             }
             for (int q = 0; q < 8; ++q) sdy[(c8 + q) * planeD + pix] = v.h[q];
+            for (x0 += 64; x0 >= W; x0 -= W) ++r;
         }
         // X rows y0-1 .. y0+R, padded: 8 channels (16 B) of one position per work-item
         const int positions = (R + 2) * rowX;
+        int rr = (tid >> 2) / rowX, xp = (tid >> 2) - rr * rowX;     // position (rr, xp) of e >> 2
         for (int e = tid; e < positions * 4; e += 256) {
             const int pos = e >> 2, c8 = (e & 3) * 8;
-            const int rr = pos / rowX, xx = pos - rr * rowX - 8;
+            const int xx = xp - 8;
             const int y = y0 - 1 + rr;
             Pack8 v;
             v.u = make_uint4(0, 0, 0, 0);
             if (y >= 0 && y < H && xx >= 0 && xx < W)
                 v.u = *(const uint4 *)(xs + ((size_t)img * hw + (size_t)y * W + xx) * cc + cl0 + c8);
             for (int q = 0; q < 8; ++q) sx[(c8 + q) * planeX + pos] = v.h[q];
+            for (xp += 64; xp >= rowX; xp -= rowX) ++rr;
         }
         __syncthreads();
         const uint16_t *dRow = sdy + (ti * 16 + row) * planeD;
         const uint16_t *xPlane = sx + (tj * 16 + row) * planeX;
-        for (int pb = 0; pb < RW; pb += 32) {
+        int gr = (8 * kq) / W, gx = 8 * kq - gr * W;       // row and column of pixel pb + 8 kq
+        for (int pb = 0; pb < RW; pb += 32, gx += 32) {
+            for (; gx >= W; gx -= W) ++gr;
             // k-groups past the stage's pixels (R * W not a multiple of 32)
             // contribute A = 0 and read row 0
             const int g = pb + 8 * kq;
             const bool gv = g < RW;
-            const int r = gv ? g / W : 0, x = gv ? g - r * W : 0;     // x is a multiple of 8
+            const int r = gv ? gr : 0, x = gv ? gx : 0;                // x is a multiple of 8
             Pack8 A;
             A.u = gv ? *(const uint4 *)(dRow + g) : make_uint4(0, 0, 0, 0);   // zero rows beyond the image
             for (int dy = 0; dy < 3; ++dy) {
@@ -588,6 +704,21 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
     const int nout = a.cy1 + a.cy2;
     if (P <= 0 || nout <= 0) return hipSuccess;
     if ((a.c1 % 32) || (a.c2 % 32) || (nout % 32) || ((a.cy1 % 32) && a.cy2)) return hipErrorInvalidValue;
+    if (!fp32_input && !a.m1) {
+        // tiled path: R rows per workgroup (R * w <= 256), halved while the
+        // grid has fewer than 512 workgroups, as long as the stage fits in LDS
+        int R = 256 / a.w < a.h ? 256 / a.w : a.h;
+        if (R < 1) R = 1;
+        auto lds = [&](int r) { return (size_t)(r + 2) * (a.w + 2) * kCvPS * sizeof(uint16_t); };
+        while (R > 1 && lds(R) > 64 * 1024) --R;
+        while (R > 1 && (long long)a.n * ((a.h + R - 1) / R) * (nout / 32) < 512) R = (R + 1) / 2;
+        if (lds(R) <= 64 * 1024) {
+            const int rowsPerImg = (a.h + R - 1) / R;
+            const dim3 grid((unsigned)(a.n * rowsPerImg), (unsigned)(nout / 32));
+            conv3x3_tile_kernel<<<grid, 256, lds(R), s>>>(a, R, rowsPerImg);
+            return hipGetLastError();
+        }
+    }
     const dim3 grid((unsigned)((P + 127) / 128), (unsigned)(nout / 32));
     if (fp32_input)
         conv3x3_kernel<float><<<grid, 256, 0, s>>>(a);
