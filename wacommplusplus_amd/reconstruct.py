@@ -226,6 +226,26 @@ def ctx_device(ctx: Context) -> int:
     return ctx.device
 
 
+def time_training(steps: int, batch: int = 16, size: int = 128, seed: int = 1) -> dict:
+    """Times `steps` training steps and one forward pass of the network on a
+    fixed random batch (the training kernels' throughput, without data)."""
+    rng = np.random.default_rng(seed)
+    net = UNet(0, seed=seed)
+    x = rng.uniform(0.0, 2.0, (batch, size, size)).astype(np.float32)
+    t = (x + rng.normal(0.0, 0.1, x.shape)).astype(np.float32)
+    for _ in range(5):
+        net.train_step(x, t, 1e-3)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.train_step(x, t, 1e-3)
+    dt = time.perf_counter() - t0
+    _, fwd_ms = net.forward(x)
+    net.close()
+    f = train_flops_per_image(size, size) * batch
+    return dict(steps=steps, batch=batch, size=size, ms_per_step=1e3 * dt / steps, forward_ms=fwd_ms,
+                train_tflops=f * steps / dt / 1e12, forward_tflops=f / 3 / (fwd_ms * 1e-3) / 1e12)
+
+
 if __name__ == "__main__":
     import argparse
     import json
@@ -239,7 +259,12 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--target", type=int, default=1_000_000)
     ap.add_argument("--sub-every", type=int, default=20)
+    ap.add_argument("--time-training", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS training steps on a random batch of 16 128x128 images")
     a = ap.parse_args()
+    if a.time_training:
+        print(json.dumps(time_training(a.time_training)), flush=True)
+        sys.exit(0)
     say = lambda m: print(m, file=sys.stderr, flush=True)   # noqa: E731
     W = full_window(T=25)
     c = Context(0)
