@@ -148,7 +148,8 @@ def ensemble_bench(wc, ctx, W, cfg, steps):
     best_ms = ctx.ens_autotune(cfg, times[0], times[1], 0, 1)
     shape = ctx.ens_get_launch()
     tile = ctx.ens_get_sort()[0]
-    log(f"[ensemble] {len(p)} particles; autotuned block={shape[0]} ppt={shape[1]} occ={shape[2]} "
+    lds_chunk = ctx.ens_get_lds()[0]
+    log(f"[ensemble] {len(p)} particles; autotuned block={shape[0]} ppt={shape[1]} occ={shape[2]} lds chunk={lds_chunk} "
         f"sort tile={tile} ({best_ms:.2f} ms on the first interval)")
     out = {}
     for use_graph in (False, True):
@@ -160,7 +161,8 @@ def ensemble_bench(wc, ctx, W, cfg, steps):
         log(f"[ensemble] {key}: {out[key]}")
     surf = ctx.ens_surface()
     return dict(metric="particle-steps/s", value=out["graph"]["value"], scenarios=len(sigma), particles=len(p),
-                intervals=steps, launch=dict(block=shape[0], ppt=shape[1], occupancy_hint=shape[2], sort_tile=tile),
+                intervals=steps, launch=dict(block=shape[0], ppt=shape[1], occupancy_hint=shape[2], sort_tile=tile,
+                                                 lds_chunk=lds_chunk),
                 autotune_first_interval_ms=best_ms, graph=out["graph"], direct=out["direct"],
                 surface_counts_last_interval=int(surf.sum()), dtype="f32 state, bf16 forcing",
                 counts=dict(min=int(counts.min()), median=float(np.median(counts)), max=int(counts.max())))
