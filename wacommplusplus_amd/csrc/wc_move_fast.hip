@@ -139,18 +139,22 @@ struct Packed {
     __device__ __forceinline__ size_t lev(int k) const { return (size_t)(k + nlev - 1) * plane; }
     __device__ __forceinline__ size_t cell(int j, int i) const { return (size_t)j * xi + i; }
 
+    // corners are copied whole (one 16-byte or 8-byte load each), not field by field
     __device__ __forceinline__ float hc(int j, int i, const W4 &w, float al) const {
         const size_t c = cell(j, i);
-        return hc4(z[c], z[c + 1], z[c + xi], z[c + xi + 1], w, al);
+        const ZH a = z[c], b = z[c + 1], d = z[c + xi], e = z[c + xi + 1];
+        return hc4(a, b, d, e, w, al);
     }
     __device__ __forceinline__ void uv_at(int kI, int j, int i, const W4 &w, float al, float &uu, float &vv) const {
         const UV4<T> *p = uv + lev(kI) + cell(j, i);
-        uv4<T>(p[0], p[1], p[xi], p[xi + 1], w, al, uu, vv);
+        const UV4<T> a = p[0], b = p[1], d = p[xi], e = p[xi + 1];
+        uv4<T>(a, b, d, e, w, al, uu, vv);
     }
     __device__ __forceinline__ void wa_at(int kI, float kF, int j, int i, const W4 &w, float alw, float ala,
                                           bool wantA, float &ww, float &aa) const {
         const WA8<T> *p = wa + lev(kI) + cell(j, i);
-        wa4<T>(p[0], p[1], p[xi], p[xi + 1], kF, w, alw, ala, wantA, ww, aa);
+        const WA8<T> a = p[0], b = p[1], d = p[xi], e = p[xi + 1];
+        wa4<T>(a, b, d, e, kF, w, alw, ala, wantA, ww, aa);
     }
     __device__ __forceinline__ float2 md_at(int j, int i) const { return md[cell(j, i)]; }
     __device__ __forceinline__ float depth_at(int q) const { return depthf[q]; }
@@ -181,18 +185,21 @@ struct TilePacked {
     __device__ __forceinline__ float hc(int j, int i, const W4 &w, float al) const {
         if (!stencil_in(j, i)) return g.hc(j, i, w, al);
         const ZH *p = zL + local(j, i);
-        return hc4(p[0], p[1], p[bw], p[bw + 1], w, al);
+        const ZH a = p[0], b = p[1], d = p[bw], e = p[bw + 1];
+        return hc4(a, b, d, e, w, al);
     }
     __device__ __forceinline__ void uv_at(int kI, int j, int i, const W4 &w, float al, float &uu, float &vv) const {
         if (!(stencil_in(j, i) && level_in(kI))) return g.uv_at(kI, j, i, w, al, uu, vv);
         const UV4<T> *p = uL + local(kI, j, i);
-        uv4<T>(p[0], p[1], p[bw], p[bw + 1], w, al, uu, vv);
+        const UV4<T> a = p[0], b = p[1], d = p[bw], e = p[bw + 1];
+        uv4<T>(a, b, d, e, w, al, uu, vv);
     }
     __device__ __forceinline__ void wa_at(int kI, float kF, int j, int i, const W4 &w, float alw, float ala,
                                           bool wantA, float &ww, float &aa) const {
         if (!(stencil_in(j, i) && level_in(kI))) return g.wa_at(kI, kF, j, i, w, alw, ala, wantA, ww, aa);
         const WA8<T> *p = wL + local(kI, j, i);
-        wa4<T>(p[0], p[1], p[bw], p[bw + 1], kF, w, alw, ala, wantA, ww, aa);
+        const WA8<T> a = p[0], b = p[1], d = p[bw], e = p[bw + 1];
+        wa4<T>(a, b, d, e, kF, w, alw, ala, wantA, ww, aa);
     }
     __device__ __forceinline__ float2 md_at(int j, int i) const {
         if ((unsigned)(j - j0) < (unsigned)bh && (unsigned)(i - i0) < (unsigned)bw) return mL[local(j, i)];
