@@ -76,6 +76,13 @@ __device__ __forceinline__ uint16_t fromf<uint16_t>(float x) {
 
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 
+// The mixed-precision path is held to the FP64 path statistically, not bit
+// for bit, so its quotients and square roots use the hardware reciprocal and
+// square root (v_rcp_f32, v_sqrt_f32: 1 ulp) instead of the IEEE sequences.
+// A zero divisor gives inf, so 0/0 still ends as NaN, as with the division.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Corner arithmetic of the samplers (shared by the global and the LDS-tiled
 // sampler, so both give the same bits).
 struct W4 {
@@ -330,11 +337,11 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
                 const float u2 = (float)(r.v[1] >> 8) * 0x1.0p-24f;
                 const float u3 = ((float)(r.v[2] >> 8) + 0.5f) * 0x1.0p-24f;
                 const float u4 = (float)(r.v[3] >> 8) * 0x1.0p-24f;
-                const float ra = sqrtf(-2.0f * __logf(u1)), rb = sqrtf(-2.0f * __logf(u3));
+                const float ra = fsqrt(-2.0f * __logf(u1)), rb = fsqrt(-2.0f * __logf(u3));
                 float s1, c1, s2, c2;
                 __sincosf(6.2831853071795865f * u2, &s1, &c1);
                 __sincosf(6.2831853071795865f * u4, &s2, &c2);
-                const float scale = sqrtf((float)(stepDt * invDti)) * sigmaDepth;
+                const float scale = fsqrt((float)(stepDt * invDti)) * sigmaDepth;
                 rx = scale * ra * c1;
                 ry = scale * ra * s1;
                 rz = scale * rb * c2 * aa * (float)P.crid;
@@ -349,7 +356,7 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
             // the non-finite level then ends the particle at its next substep.
             auto levels = [&](float leap) -> float {
                 if (fabsf(leap) > zdist) leap = copysignf(fabsf(zdist), leap);
-                return leap / zdist;
+                return leap * frcp(zdist);
             };
             float dxl = dir * uu * dt, dyl = dir * vv * dt, dzl = dir * (sv + ww) * dt;
 
@@ -376,9 +383,9 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
                     S.wa_at(kMI, kMF, jm, im, wm, alpha, alpha, false, wmv, dummy);
                     // leap in metres of the start cell: u* dt dx/dx*
                     const float2 mdm = S.md_at(jm, im);
-                    dxl = dir * um * dt * (mdm.x / idx_);
-                    dyl = dir * vm * dt * (mdm.y / idy_);
-                    dzl = dir * (sv + wmv) * dt * (zdist / (hcMid * S.depth_at(kMI + P.s_w - 2)));
+                    dxl = dir * um * dt * (mdm.x * frcp(idx_));
+                    dyl = dir * vm * dt * (mdm.y * frcp(idy_));
+                    dzl = dir * (sv + wmv) * dt * (zdist * frcp(hcMid * S.depth_at(kMI + P.s_w - 2)));
                 } else {
                     float ue, ve, we, dummy;
                     S.uv_at(kI, jc, ic, w0, alpha, ue, ve);
