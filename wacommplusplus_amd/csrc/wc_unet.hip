@@ -6,27 +6,24 @@
 // gradients NHWC in FP32, weights [cout][3][3][cin] in BF16 (FP32 master
 // copies live in the optimiser).  Every channel count is a multiple of 32.
 //
-// conv3x3_kernel: 3x3, stride 1, zero padding 1, as an implicit GEMM
+// Convolutions are 3x3, stride 1, zero padding 1, as implicit GEMMs
 //   out[p][co] = sum_{tap, ci} in[p + tap][ci] * w[co][tap][ci]
-// on v_mfma_f32_16x16x32_bf16.  One wave computes a 32-pixel x 32-channel
-// tile (2 x 2 MFMA tiles); each lane loads 8 consecutive channels (16 B) of
-// one pixel for A and of one output channel's weights for B, straight from
-// global memory (NHWC makes both contiguous).  The same kernel serves
+// on v_mfma_f32_16x16x32_bf16.  The same products serve
 //   - forward: two inputs concatenated along channels (decoder skip + up),
 //     bias + ReLU, BF16 out;
-//   - backward data: the output gradient (FP32, masked by the ReLU of the
-//     forward output) convolved with the flipped, transposed weights,
-//     FP32 out split over two tensors (the concat's two inputs), stored or
-//     accumulated.
+//   - input gradient: the ReLU-masked output gradient (BF16, made once per
+//     layer by mask_bf16_kernel) convolved with the flipped, transposed
+//     weights, FP32 out split over the concat's two inputs, stored or
+//     accumulated;
+//   - weight gradient: dW[co][tap][ci] = sum_p dYm[p][co] X[p + tap][ci], a
+//     GEMM whose reduction runs over pixels.
+// conv3x3_tile_kernel (forward, input gradient) and wgrad_tile_kernel
+// (weight gradient) stage their operands in LDS; conv3x3_kernel (operands
+// straight from global memory) and wgrad_kernel (split-K with FP32 atomics)
+// take the shapes whose stage does not fit.
 // MFMA operand layout (16x16x32 bf16): lane l holds A[l%16][8*(l/16)..+7]
 // and B[8*(l/16)..+7][l%16]; the accumulator holds D[4*(l/16)+r][l%16],
 // r = 0..3.
-//
-// wgrad_kernel: dW[co][tap][ci] += sum_p dYm[p][co] * X[p + tap][ci], a GEMM
-// whose reduction runs over pixels: a workgroup stages 64 pixels of dYm
-// (32 co) and X (32 ci) transposed in LDS, four waves take the 2 x 2 16x16
-// tiles, and each workgroup adds its pixel range's partial sums into dW with
-// FP32 atomics (split-K).
 #include "wc_launch.h"
 
 #pragma clang fp contract(on)
