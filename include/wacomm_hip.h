@@ -483,12 +483,17 @@ int wc_ens_get(wc_ctx *ctx, wc_particle_data *particles, uint8_t *scenario, size
  * Parameters are FP32 in this order: per conv layer (enc1a, enc1b, enc2a,
  * enc2b, enc3a, enc3b, bott_a, bott_b, dec3a, dec3b, dec2a, dec2b, dec1a,
  * dec1b) weights [cout][3][3][cin] then biases [cout]; then the head's 32
- * weights and its bias.  cin of enc1a is 32 (the input is channel 0); the
- * decoder's first convs take concat(skip, upsampled).  He-normal
- * initialisation from `seed`.  Images are [n][h][w] FP32 on the host, h and w
- * multiples of 8. */
+ * weights and its bias.  cin of enc1a is 32 (the input channels first, then
+ * zeros); the decoder's first convs take concat(skip, upsampled).  He-normal
+ * initialisation from `seed`.  Images are [n][h][w] FP32 on the host (one
+ * input channel), h and w multiples of 8. */
 typedef struct wc_unet wc_unet;
 int wc_unet_create(int device, uint64_t seed, wc_unet **out);
+/* The same network with `channels` (1..32) input channels: images are then
+ * [n][channels][h][w] FP32 (the first layer's input channels 0..channels-1;
+ * the residual adds channel 0), e.g. a low-count histogram at several output
+ * times.  wc_unet_create is channels = 1. */
+int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out);
 int wc_unet_destroy(wc_unet *net);
 const char *wc_unet_last_error(const wc_unet *net);
 size_t wc_unet_param_count(const wc_unet *net);

@@ -146,13 +146,17 @@ class UNetOracle:
 
     # -- network --------------------------------------------------------------
     def forward(self, x):
-        """x [n][h][w] -> out [n][h][w]; keeps the activations for backward."""
+        """x [n][h][w] (one input channel) or [n][C][h][w] (C input channels,
+        channel 0 the residual's) -> out [n][h][w]; keeps the activations for
+        backward."""
         x = np.asarray(x, dtype=np.float64)
-        n, h, w = x.shape
+        xs = x[:, None] if x.ndim == 3 else x
+        n, cin, h, w = xs.shape
+        x = xs[:, 0]
         a = {}
         a["x"] = x
         in0 = np.zeros((n, h, w, BASE))
-        in0[..., 0] = self._r(x)
+        in0[..., :cin] = self._r(np.moveaxis(xs, 1, 3))
         a["in0"] = in0
         a["a1"] = self._layer(0, in0)
         a["s1"] = self._layer(1, a["a1"])
@@ -244,16 +248,17 @@ class UNetOracle:
         return loss, pack(grads, dwh, dbh)
 
 
-def he_params(seed=1):
-    """He-normal weights (fan-in of the real inputs), small random biases and
-    head: a generic point for gradient checks."""
+def he_params(seed=1, channels=1):
+    """He-normal weights (fan-in of the real inputs: `channels` input
+    channels), small random biases and head: a generic point for gradient
+    checks."""
     rng = np.random.default_rng(seed)
     layers = []
     for l, (co, c1, c2) in enumerate(SPEC):
         cin = c1 + c2
-        fan = 9 * (1 if l == 0 else cin)
+        fan = 9 * (channels if l == 0 else cin)
         W = rng.normal(0.0, np.sqrt(2.0 / fan), (co, 3, 3, cin))
         if l == 0:
-            W[..., 1:] = 0.0
+            W[..., channels:] = 0.0
         layers.append((W, rng.normal(0.0, 0.05, co)))
     return pack(layers, rng.normal(0.0, 0.1, BASE), 0.01)

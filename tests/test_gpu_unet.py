@@ -71,6 +71,31 @@ def test_forward_and_gradients_match_oracle(net, shape):
     print(f"{shape}: gradient rel err {whole:.2e}, worst layer {worst:.2e}, head {eh:.2e}", flush=True)
 
 
+def test_three_channel_input_matches_oracle():
+    """A 3-channel network ([n][3][h][w] input) against the oracle, as above."""
+    net3 = wc.UNet(0, seed=5, channels=3)
+    try:
+        rng = np.random.default_rng(17)
+        p = uo.he_params(seed=6, channels=3).astype(np.float32)
+        net3.set_params(p)
+        x = rng.uniform(0.0, 3.0, (2, 3, 32, 32)).astype(np.float32)
+        t = (x[:, 0] + rng.normal(0.0, 0.3, (2, 32, 32))).astype(np.float32)
+        ora = uo.UNetOracle(p.astype(np.float64), emulate_bf16=True)
+        y, _ = net3.forward(x)
+        ey = rel(y, ora.forward(x))
+        loss, g = net3.gradients(x, t)
+        loss_ref, g_ref = ora.gradients(x, t)
+        eg = rel(g, g_ref)
+        print(f"3 channels: forward rel err {ey:.2e}, gradient {eg:.2e}", flush=True)
+        assert ey < 5e-3 and abs(loss - loss_ref) / loss_ref < 1e-3 and eg < 3e-2
+        W0 = g[:32 * 9 * 32].reshape(32, 3, 3, 32)
+        assert np.all(W0[..., 3:] == 0) and np.any(W0[..., 2] != 0)
+        with pytest.raises(ValueError):
+            net3.forward(x[:, 0])
+    finally:
+        net3.close()
+
+
 def blobs(rng, n, h, w, k=3):
     """Smooth positive fields (sums of Gaussians) and noisy copies of them."""
     yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")

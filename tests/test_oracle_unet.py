@@ -47,6 +47,39 @@ def test_finite_differences_fp64():
     assert finite_difference_check() < 1e-6
 
 
+def test_multichannel_input():
+    """Several input channels ([n][C][h][w]): with the extra channels zero the
+    output equals the one-channel network's; the loss gradient for first-layer
+    weights on channels 1 and 2 (and every other sampled parameter) matches
+    central finite differences, so the extra channels reach the network; the
+    residual adds channel 0 only."""
+    rng = np.random.default_rng(8)
+    p = uo.he_params(4, channels=3)
+    x1 = rng.uniform(0.0, 3.0, (1, 8, 8))
+    x3 = np.concatenate([x1[:, None], np.zeros((1, 2, 8, 8))], axis=1)
+    a = uo.UNetOracle(p, emulate_bf16=False).forward(x1)
+    b = uo.UNetOracle(p, emulate_bf16=False).forward(x3)
+    assert np.array_equal(a, b)
+    x3 = np.concatenate([x1[:, None], rng.uniform(0.0, 3.0, (1, 2, 8, 8))], axis=1)
+    c = uo.UNetOracle(p, emulate_bf16=False).forward(x3)
+    assert np.abs(c - a).max() > 1e-3                      # channels 1, 2 change the output
+    t = x1 + rng.normal(0.0, 0.5, x1.shape)
+    _, g = uo.UNetOracle(p, emulate_bf16=False).gradients(x3, t)
+    W0 = np.arange(32 * 9 * 32).reshape(32, 3, 3, 32)
+    idx = [int(W0[5, 1, 1, 1]), int(W0[7, 0, 2, 2]), int(W0[3, 2, 0, 0]), 32 * 9 * 32 + 4]
+    worst = 0.0
+    for i in idx:
+        eps = 1e-6 * max(1.0, abs(p[i]))
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (uo.UNetOracle(pp, emulate_bf16=False).loss(x3, t) -
+              uo.UNetOracle(pm, emulate_bf16=False).loss(x3, t)) / (2 * eps)
+        worst = max(worst, abs(fd - g[i]) / max(abs(fd), abs(g[i]), 1e-4 * np.max(np.abs(g))))
+    assert worst < 1e-6, worst
+    assert np.all(g[int(W0[0, 0, 0, 3]):int(W0[0, 0, 0, 3]) + 1] == 0)      # channel 3 carries no input
+
+
 def test_planted_wrong_taps_fails_the_pin():
     assert finite_difference_check(plant="wrong_taps") > 1e-2
 

@@ -30,6 +30,7 @@ EXPORTS = (
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
     "wc_ens_get", "wc_ens_release", "wc_ens_set_sort", "wc_ens_get_sort", "wc_ens_set_lds", "wc_ens_get_lds", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
+    "wc_unet_create_ex",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
     "wc_particle_records",
 )
@@ -179,6 +180,7 @@ def load_library(path: str = LIB_PATH):
     L.wc_philox4x32_10.restype = None
     unet = {
         "wc_unet_create": [i, C.c_uint64, C.POINTER(vp)],
+        "wc_unet_create_ex": [i, C.c_uint64, i, C.POINTER(vp)],
         "wc_unet_destroy": [vp],
         "wc_unet_get_params": [vp, vp],
         "wc_unet_set_params": [vp, vp],
@@ -532,20 +534,31 @@ def philox4x32_10(ctr, key):
 
 class UNet:
     """The concentration-reconstruction network (include/wacomm_hip.h,
-    section 4) on one device: images are [n][h][w] float32 arrays."""
+    section 4) on one device: images are [n][h][w] float32 arrays, or
+    [n][channels][h][w] with `channels` input channels (channel 0 feeds the
+    residual); outputs are [n][h][w]."""
 
-    def __init__(self, device: int = 0, seed: int = 1):
+    def __init__(self, device: int = 0, seed: int = 1, channels: int = 1):
         self.L = load_library()
         h = vp()
-        rc = self.L.wc_unet_create(device, seed, C.byref(h))
+        rc = self.L.wc_unet_create_ex(device, seed, int(channels), C.byref(h))
         if rc:
-            raise WcError(rc, f"wc_unet_create(device={device})")
+            raise WcError(rc, f"wc_unet_create_ex(device={device}, channels={channels})")
         self.h = h
+        self.channels = int(channels)
         self.n_params = self.L.wc_unet_param_count(h)
 
     def _chk(self, rc):
         if rc:
             raise WcError(rc, self.L.wc_unet_last_error(self.h).decode())
+
+    def _shape(self, xx):
+        """(n, h, w) of an input batch, checked against the channel count."""
+        if self.channels == 1 and xx.ndim == 3:
+            return xx.shape
+        if xx.ndim != 4 or xx.shape[1] != self.channels:
+            raise ValueError(f"input of shape {xx.shape} for a {self.channels}-channel network")
+        return xx.shape[0], xx.shape[2], xx.shape[3]
 
     def close(self):
         if self.h:
@@ -571,15 +584,16 @@ class UNet:
     def forward(self, x):
         """(y, device ms of the forward pass)."""
         xx = _f32(x)
-        n, hh, ww = xx.shape
-        y = np.zeros_like(xx)
+        n, hh, ww = self._shape(xx)
+        y = np.zeros((n, hh, ww), np.float32)
         ms = C.c_double()
         self._chk(self.L.wc_unet_forward(self.h, _p(xx), n, hh, ww, _p(y), C.byref(ms)))
         return y, ms.value
 
     def gradients(self, x, t):
         xx, tt = _f32(x), _f32(t)
-        n, hh, ww = xx.shape
+        n, hh, ww = self._shape(xx)
+        assert tt.shape == (n, hh, ww)
         g = np.zeros(self.n_params, np.float32)
         loss = C.c_float()
         self._chk(self.L.wc_unet_gradients(self.h, _p(xx), _p(tt), n, hh, ww, _p(g), C.byref(loss)))
@@ -587,7 +601,8 @@ class UNet:
 
     def train_step(self, x, t, lr):
         xx, tt = _f32(x), _f32(t)
-        n, hh, ww = xx.shape
+        n, hh, ww = self._shape(xx)
+        assert tt.shape == (n, hh, ww)
         loss = C.c_float()
         self._chk(self.L.wc_unet_train_step(self.h, _p(xx), _p(tt), n, hh, ww, float(lr), C.byref(loss)))
         return loss.value

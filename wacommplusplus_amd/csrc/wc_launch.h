@@ -172,7 +172,10 @@ hipError_t launch_pool_fwd(const uint16_t *x, int n, int h, int w, int c, uint16
 hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, hipStream_t s);
 hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
 hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx, hipStream_t s);
-hipError_t launch_unet_input(const float *x, long long P, int c, uint16_t *y, hipStream_t s);
+// x [n][cin][h][w] -> the first layer's NHWC BF16 input with c channels
+// (P = n h w, hw = h w); x0 (optional) receives channel 0 as [n][h][w]
+hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c, uint16_t *y, float *x0,
+                             hipStream_t s);
 hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, long long P, float *out,
                            hipStream_t s);
 hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wh, long long P,

@@ -583,12 +583,23 @@ __global__ void up_bwd_kernel(const float *__restrict__ dy, int n, int h, int w,
     }
 }
 
-// network input: x [n][h][w] FP32 -> NHWC BF16 with `c` channels, channel 0 = x
-__global__ void input_kernel(const float *__restrict__ x, long long P, int c, uint16_t *__restrict__ y) {
+// network input: x [n][cin][h][w] FP32 -> NHWC BF16 with `c` channels
+// (input channel q in channel q, zeros above cin); with x0 != null also
+// channel 0 as [n][h][w] FP32 (the residual's input)
+__global__ void input_kernel(const float *__restrict__ x, long long P, int hw, int cin, int c,
+                             uint16_t *__restrict__ y, float *__restrict__ x0) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < P * c;
          e += (long long)gridDim.x * blockDim.x) {
         const long long p = e / c;
-        y[e] = (e - p * c) == 0 ? to_bf16(x[p]) : (uint16_t)0;
+        const int ch = (int)(e - p * c);
+        if (ch >= cin) {
+            y[e] = 0;
+            continue;
+        }
+        const long long ni = p / hw, pix = p - ni * hw;
+        const float v = x[(ni * cin + ch) * hw + pix];
+        y[e] = to_bf16(v);
+        if (x0 && ch == 0) x0[p] = v;
     }
 }
 
@@ -784,8 +795,10 @@ hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx,
     up_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx);
     return hipGetLastError();
 }
-hipError_t launch_unet_input(const float *x, long long P, int c, uint16_t *y, hipStream_t s) {
-    input_kernel<<<blocks_for(P * c), 256, 0, s>>>(x, P, c, y);
+hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c, uint16_t *y, float *x0,
+                             hipStream_t s) {
+    if (cin < 1 || cin > c) return hipErrorInvalidValue;
+    input_kernel<<<blocks_for(P * c), 256, 0, s>>>(x, P, hw, cin, c, y, x0);
     return hipGetLastError();
 }
 hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, long long P, float *out,

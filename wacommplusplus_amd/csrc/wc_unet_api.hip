@@ -60,6 +60,8 @@ struct wc_unet {
              *u1 = nullptr, *d1a = nullptr, *d1 = nullptr;
     uint16_t *gmask = nullptr;      // a layer's ReLU-masked output gradient in BF16 (backward scratch)
     float *x = nullptr, *out = nullptr, *tgt = nullptr;
+    int cin = 1;                // input channels (x holds channel 0, xc all of them when cin > 1)
+    float *xc = nullptr;
     float *g_d1 = nullptr, *g_d1a = nullptr, *g_s1 = nullptr, *g_u1 = nullptr, *g_d2 = nullptr, *g_d2a = nullptr,
           *g_s2 = nullptr, *g_u2 = nullptr, *g_d3 = nullptr, *g_d3a = nullptr, *g_s3 = nullptr, *g_u3 = nullptr,
           *g_b4 = nullptr, *g_a4 = nullptr, *g_p3 = nullptr, *g_a3 = nullptr, *g_p2 = nullptr, *g_a2 = nullptr,
@@ -121,6 +123,7 @@ int ensure_shape(wc_unet *u, int n, int h, int w) {
     rc |= alloc(u, u->d2a, P1 * 2 * C) | alloc(u, u->d2, P1 * 2 * C) | alloc(u, u->u1, P0 * 2 * C);
     rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C) | alloc(u, u->gmask, P0 * C);
     rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0);
+    if (u->cin > 1) rc |= alloc(u, u->xc, P0 * u->cin);
     rc |= alloc(u, u->g_d1, P0 * C) | alloc(u, u->g_d1a, P0 * C) | alloc(u, u->g_s1, P0 * C) |
           alloc(u, u->g_u1, P0 * 2 * C);
     rc |= alloc(u, u->g_d2, P1 * 2 * C) | alloc(u, u->g_d2a, P1 * 2 * C) | alloc(u, u->g_s2, P1 * 2 * C) |
@@ -184,7 +187,8 @@ int forward(wc_unet *u) {
     const int H = u->h, W = u->w, C = kBase, n = u->n;
     const auto &Ls = u->layers;
     int rc;
-    UCHK(u, launch_unet_input(u->x, (long long)n * H * W, C, u->in0, u->stream));
+    UCHK(u, launch_unet_input(u->cin > 1 ? u->xc : u->x, (long long)n * H * W, H * W, u->cin, C, u->in0,
+                              u->cin > 1 ? u->x : nullptr, u->stream));
     if ((rc = conv_fwd(u, Ls[0], u->in0, nullptr, u->a1, H, W)) || (rc = conv_fwd(u, Ls[1], u->a1, nullptr, u->s1, H, W)))
         return rc;
     UCHK(u, launch_pool_fwd(u->s1, n, H, W, C, u->p1, u->stream));
@@ -265,7 +269,7 @@ int upload(wc_unet *u, const float *x, const float *t, int n, int h, int w) {
     int rc = ensure_shape(u, n, h, w);
     if (rc) return rc;
     const size_t P = (size_t)n * h * w;
-    UCHK(u, hipMemcpyAsync(u->x, x, P * 4, hipMemcpyHostToDevice, u->stream));
+    UCHK(u, hipMemcpyAsync(u->cin > 1 ? u->xc : u->x, x, P * u->cin * 4, hipMemcpyHostToDevice, u->stream));
     if (t) UCHK(u, hipMemcpyAsync(u->tgt, t, P * 4, hipMemcpyHostToDevice, u->stream));
     return WC_OK;
 }
@@ -280,13 +284,17 @@ int refresh_bf16(wc_unet *u) {
 
 extern "C" {
 
-int wc_unet_create(int device, uint64_t seed, wc_unet **out) {
+int wc_unet_create(int device, uint64_t seed, wc_unet **out) { return wc_unet_create_ex(device, seed, 1, out); }
+
+int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
     if (!out) return WC_ERR_INVALID;
     *out = nullptr;
+    if (channels < 1 || channels > kBase) return WC_ERR_INVALID;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return WC_ERR_NO_DEVICE;
     auto *u = new wc_unet();
     u->device = device;
+    u->cin = channels;
     const int C = kBase;
     const int spec[14][3] = {{C, C, 0},          {C, C, 0},         {2 * C, C, 0},     {2 * C, 2 * C, 0},
                              {4 * C, 2 * C, 0},  {4 * C, 4 * C, 0}, {8 * C, 4 * C, 0}, {8 * C, 8 * C, 0},
@@ -310,17 +318,17 @@ int wc_unet_create(int device, uint64_t seed, wc_unet **out) {
     u->nparam = off;
     u->nwb = woff;
     // He-normal initialisation (fan-in of the real inputs; the first layer
-    // sees one real channel), zero biases, small head weights
+    // sees `channels` real channels), zero biases, small head weights
     std::vector<float> p(u->nparam, 0.0f);
     std::mt19937_64 rng(seed);
     for (size_t l = 0; l < u->layers.size(); ++l) {
         const Layer &L = u->layers[l];
-        const int cin = L.c1 + L.c2, fan = 9 * (l == 0 ? 1 : cin);
+        const int cin = L.c1 + L.c2, fan = 9 * (l == 0 ? channels : cin);
         std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / fan));
         for (int co = 0; co < L.cout; ++co)
             for (int t = 0; t < 9; ++t)
                 for (int ci = 0; ci < cin; ++ci)
-                    p[L.w_off + ((size_t)co * 9 + t) * cin + ci] = (l == 0 && ci > 0) ? 0.0f : nd(rng);
+                    p[L.w_off + ((size_t)co * 9 + t) * cin + ci] = (l == 0 && ci >= channels) ? 0.0f : nd(rng);
     }
     std::normal_distribution<float> hd(0.0f, 0.01f);
     for (int c = 0; c < C; ++c) p[u->head_off + c] = hd(rng);

@@ -47,13 +47,17 @@ def ocean_cells(W: ForcingWindow, n: int, margin: int, rng):
 
 def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int = 1_000_000, sub_every: int = 20,
                    hours: int = 24, crop: int = 128, seed: int = 5489, kh_range=(0.1, 10.0), dti: float = 30.0,
-                   depth_m: float = 1.0, batch_pairs: int = 128, log=None):
+                   depth_m: float = 1.0, batch_pairs: int = 128, frames: int = 1, log=None):
     """Surface-layer histograms (few, all) of n_pairs releases after `hours`,
     cropped to crop x crop around the centre of mass of the few-particle
     histogram (what is known at reconstruction time; the release point when
-    it is empty).  The context must hold the window's grid and records
-    (wc_set_grid with n_slots >= hours + 1).  Returns dict(few, all: uint32
-    [n_pairs][crop][crop], kh, j, i, origin [n_pairs][2], seconds,
+    it is empty).  With frames > 1 the few-particle histograms of the
+    frames - 1 hours before are kept too (a low-count run writes hourly
+    frames), cropped at the same place.  The context must hold the window's
+    grid and records (wc_set_grid with n_slots >= hours + 1).  Returns
+    dict(few: uint32 [n_pairs][crop][crop], or [n_pairs][frames][crop][crop]
+    with frame 0 at `hours`, frame f at hours - f; all: uint32
+    [n_pairs][crop][crop]; kh, j, i, origin [n_pairs][2], seconds,
     particle_steps)."""
     rng = np.random.default_rng(seed)
     js, is_ = ocean_cells(W, n_pairs, crop // 2 + 2, rng)
@@ -63,7 +67,7 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
     cfg = default_config(random=True, sigma=1.0, dti=dti)
     times = [float(W.times[q]) for q in range(hours + 1)]
     slots = list(range(hours + 1))
-    few = np.zeros((n_pairs, crop, crop), np.uint32)
+    few = np.zeros((n_pairs, frames, crop, crop), np.uint32)
     allc = np.zeros((n_pairs, crop, crop), np.uint32)
     origin = np.zeros((n_pairs, 2), np.int64)
     jj, ii = np.arange(W.eta, dtype=np.float64), np.arange(W.xi, dtype=np.float64)
@@ -76,9 +80,20 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
                    for q in range(nb)]
         sg = np.repeat(sigma[b0:b0 + nb], 2)
         ctx.ens_release(sources, sg, seed=seed + 7919 * (b0 + 1))
-        st = ctx.ens_run(cfg, times, slots, use_graph=True, surface=True)
+        # run to hours - (frames - 1), then one hour at a time, keeping the
+        # surface counts after each of the last `frames` hours
+        first = hours - (frames - 1)
+        st = ctx.ens_run(cfg, times[:first + 1], slots[:first + 1], use_graph=True, surface=True)
         steps += st.particle_steps
-        surf = ctx.ens_surface()
+        ms = st.device_ms
+        kept = [ctx.ens_surface()]
+        for hh in range(first, hours):
+            st = ctx.ens_run(cfg, times[hh:hh + 2], slots[hh:hh + 2], use_graph=True, surface=True)
+            steps += st.particle_steps
+            ms += st.device_ms
+            kept.append(ctx.ens_surface())
+        kept = kept[::-1]                                 # frame 0 = the last hour
+        surf = kept[0]
         for q in range(nb):
             fs = surf[2 * q].astype(np.float64)
             tot = fs.sum()
@@ -88,12 +103,14 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
             c0 = int(np.clip(round(ci) - crop // 2, 0, W.xi - crop))
             origin[b0 + q] = (r0, c0)
             f = surf[2 * q, r0:r0 + crop, c0:c0 + crop]
-            few[b0 + q] = f
+            for fr in range(frames):
+                few[b0 + q, fr] = kept[fr][2 * q, r0:r0 + crop, c0:c0 + crop]
             allc[b0 + q] = f + surf[2 * q + 1, r0:r0 + crop, c0:c0 + crop]
         if log:
-            log(f"  pairs {b0 + nb}/{n_pairs}: {st.particle_steps:.3g} particle-steps in {st.device_ms:.0f} ms")
-    return dict(few=few, all=allc, kh=kh, j=js, i=is_, origin=origin, seconds=time.perf_counter() - t0,
-                particle_steps=steps, n_target=n_target, sub_every=sub_every, hours=hours)
+            log(f"  pairs {b0 + nb}/{n_pairs}: {steps:.3g} particle-steps so far, last batch {ms:.0f} ms")
+    return dict(few=few[:, 0] if frames == 1 else few, all=allc, kh=kh, j=js, i=is_, origin=origin,
+                seconds=time.perf_counter() - t0, particle_steps=steps, n_target=n_target, sub_every=sub_every,
+                hours=hours, frames=frames)
 
 
 def to_net(few, allc, sub_every, scale):
@@ -103,10 +120,12 @@ def to_net(few, allc, sub_every, scale):
 
 
 def augment(x, t, rng):
+    """The same random rotation and flip of the image axes of x ([n][h][w] or
+    [n][frames][h][w]) and t."""
     k = int(rng.integers(4))
-    x, t = np.rot90(x, k, axes=(1, 2)), np.rot90(t, k, axes=(1, 2))
+    x, t = np.rot90(x, k, axes=(-2, -1)), np.rot90(t, k, axes=(-2, -1))
     if rng.integers(2):
-        x, t = x[:, :, ::-1], t[:, :, ::-1]
+        x, t = x[..., ::-1], t[..., ::-1]
     return np.ascontiguousarray(x), np.ascontiguousarray(t)
 
 
@@ -138,7 +157,7 @@ def reconstruct(net: UNet, few, sub_every, scale, batch=16):
     """Concentration estimate (counts of the full release) from few-particle
     histograms, and the device ms of the forward passes."""
     x, _ = to_net(few, None, sub_every, scale)
-    out = np.zeros_like(x)
+    out = np.zeros((len(x),) + x.shape[-2:], np.float32)
     ms = 0.0
     for b in range(0, len(x), batch):
         y, m = net.forward(x[b:b + batch])
@@ -162,7 +181,7 @@ def evaluate(net: UNet, data, scale):
     are left out (their number is reported)."""
     keep = data["all"].reshape(len(data["all"]), -1).sum(1) > 0
     few, allc, sub = data["few"][keep], data["all"][keep].astype(np.float64), data["sub_every"]
-    grid = sub * few.astype(np.float64)
+    grid = sub * (few[:, 0] if few.ndim == 4 else few).astype(np.float64)      # the last hour's histogram
     rec, ms = reconstruct(net, few, sub, scale)
     e_grid, e_net = rel_l2(grid, allc), rel_l2(rec, allc)
     e_t = np.sqrt(allc.reshape(len(allc), -1).sum(1)) / np.linalg.norm(allc.reshape(len(allc), -1), axis=1)
@@ -185,15 +204,16 @@ def train_flops_per_image(h: int, w: int) -> float:
 
 def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, epochs: int = 40,
         n_target: int = 1_000_000, sub_every: int = 20, hours: int = 24, crop: int = 128, batch: int = 16,
-        scale: float = 16.0, seed: int = 21, log=None):
+        scale: float = 16.0, seed: int = 21, frames: int = 1, log=None):
     """The whole learned-reconstruction experiment: pairs, training, held-out
     errors, and the comparison with plain gridding at equal GPU time (the
     move of n = n_target / sub_every particles per scenario, batched 256 at a
-    time, plus one forward pass per image)."""
-    tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, log=log)
-    te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, crop, seed + 1, log=log)
+    time, plus one forward pass per image).  frames > 1: the network also
+    sees the few-particle histograms of the frames - 1 hours before."""
+    tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
+    te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, crop, seed + 1, frames=frames, log=log)
     X, T = to_net(tr["few"], tr["all"], sub_every, scale)
-    net = UNet(ctx_device(ctx), seed=3)
+    net = UNet(ctx_device(ctx), seed=3, channels=frames)
     h = train(net, X, T, epochs=epochs, batch=batch, log=log)
     ev = evaluate(net, te, scale)
     # GPU time of the few-particle move: 256 scenarios of n particles, batched
@@ -211,7 +231,7 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     n_match = n * (ev["e_grid_truth"] / max(ev["e_net_truth"], 1e-12)) ** 2
     flops = train_flops_per_image(crop, crop) * h["steps"] * batch
     return dict(
-        pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop,
+        pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop, frames=frames,
                    seconds=tr["seconds"] + te["seconds"], particle_steps=int(tr["particle_steps"] + te["particle_steps"])),
         training=dict(epochs=epochs, steps=h["steps"], seconds=h["seconds"], loss_first=h["loss"][0],
                       loss_last=h["loss"][-1], tflops=flops / h["seconds"] / 1e12),
@@ -259,6 +279,7 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--target", type=int, default=1_000_000)
     ap.add_argument("--sub-every", type=int, default=20)
+    ap.add_argument("--frames", type=int, default=1, help="hourly few-particle frames the network sees")
     ap.add_argument("--time-training", type=int, default=0, metavar="STEPS",
                     help="only time STEPS training steps on a random batch of 16 128x128 images")
     a = ap.parse_args()
@@ -271,4 +292,5 @@ if __name__ == "__main__":
     c.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=W.T, store="bf16")
     for t in range(W.T):
         c.upload_record(t, *W.record(t))
-    print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, log=say)), flush=True)
+    print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames, log=say)),
+          flush=True)
