@@ -134,8 +134,10 @@ enum { WC_PATH_PARITY = 0, WC_PATH_FAST = 1 };
  * kernel.cu:63-69).  mask, numThread, numGPU are unused (as in the reference);
  * windU10..stokesV must be NULL or are ignored for passive particles.
  * stream is a hipStream_t (NULL = default stream).  Asynchronous: results are
- * valid after the stream is synchronised.  Returns WC_ERR_UNSUPPORTED when
- * config->driftModel==1 or leewayJibeProbabilityHourly>0. */
+ * valid after the stream is synchronised.  Non-passive particles move
+ * passively (driftModel 0) and, with leewayJibeProbabilityHourly > 0, flip
+ * their side as Particle.cpp:805-813 does.  Returns WC_ERR_UNSUPPORTED when
+ * config->driftModel==1. */
 int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, int timeIndex, int oceanTime,
                           int sW, int sRho, int eta, int xi, double *times, double *mask, double *lonRad,
                           double *latRad, double *depthIntervals, double *h, float *zeta, float *u, float *v,
@@ -253,6 +255,8 @@ int wc_set_geo(wc_ctx *ctx, const double *lat, const double *lon, const double *
 typedef struct wc_records {
     uint64_t *id;
     double *lat, *lon, *depth, *k, *j, *i, *health, *age, *time;
+    uint32_t *object_type;   /* driftObjectType (Particles.cpp:144) */
+    int32_t *drift_side;     /* driftSide (Particles.cpp:145) */
 } wc_records;
 int wc_particle_records(wc_ctx *ctx, const wc_records *out, size_t capacity, size_t *count);
 
@@ -274,7 +278,13 @@ int wc_compare_records(wc_ctx *ctx, int slot_a, int slot_b, int *conflict, long 
 
 /* Particle state: host AoS in and out (Particles.hpp:17 holds particle_data).
  * wc_set_particles replaces the device population; wc_append_particles adds
- * to it (emission). */
+ * to it (emission).  Every field travels, the drift object type and side
+ * included: with driftModel 0 a non-passive particle moves as a passive one
+ * (Particle.cpp:380-454 act only under driftModel 1), and the hourly side
+ * flip of leewayJibeProbabilityHourly > 0 (Particle.cpp:805-813) runs on
+ * WC_PATH_PARITY; WC_PATH_FAST returns WC_ERR_UNSUPPORTED for a positive
+ * jibe probability once a non-passive particle has been uploaded or
+ * released. */
 int wc_set_particles(wc_ctx *ctx, const wc_particle_data *particles, size_t n);
 int wc_append_particles(wc_ctx *ctx, const wc_particle_data *particles, size_t n);
 int wc_get_particles(wc_ctx *ctx, wc_particle_data *particles, size_t n);
@@ -313,7 +323,11 @@ typedef struct wc_source {
  * released in [intervalStart, intervalEnd) in time order, ids continuing from
  * the last particle's id (0 when empty); with config->randomSources each
  * coordinate gets 0.25*normal(seed, id, SOURCE_POSITION_RANDOM_INTERVAL, 0, c)
- * (SourceEmission.hpp:18-23).  *emitted receives the count.  Returns
+ * (SourceEmission.hpp:18-23).  Each particle takes config->driftObjectType
+ * and config->driftSide, or with config->leewayRandomSide the side
+ * sampleDriftSide draws (RIGHT when uniform(seed, id, LEEWAY_SIDE_RANDOM_INTERVAL,
+ * 0, 0) < leewayRightSideProbability, else LEFT; Source.cpp:38-42,
+ * DriftModel.hpp:146-154).  *emitted receives the count.  Returns
  * WC_ERR_INVALID for an invalid schedule (where the reference throws). */
 int wc_emit(wc_ctx *ctx, const wc_config_data *config, const wc_source *sources, int n_sources,
             double intervalStart, double intervalEnd, size_t *emitted);
@@ -389,7 +403,8 @@ void wc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
  * time is its release (late activation, Particle.cpp:211-215).  State is
  * compact: integer cell + FP32 offsets and level, FP32 health, FP64 age and
  * emission time.  Per-scenario surface-layer count grids
- * [n_scenarios][eta][xi] (uint32) are allocated here. */
+ * [n_scenarios][eta][xi] (uint32) are allocated here.  Ensemble particles
+ * are passive tracers: a non-passive one is WC_ERR_UNSUPPORTED. */
 int wc_ens_set(wc_ctx *ctx, const wc_particle_data *particles, const uint8_t *scenario, size_t n,
                const double *sigma, int n_scenarios);
 

@@ -25,7 +25,6 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
-#include <cstring>
 #include <fstream>
 #include <iomanip>
 #include <iterator>
@@ -204,34 +203,18 @@ public:
     void saveAsNetCDF(Context &ctx, const std::string &fileName, double particleTime, const wc_config_data &config,
                       const std::string &oceanModel, const char *const *provenance = nullptr) const {
         wc_ctx *c = ctx.get();
-        // the records depend on position only; the drift state (which the
-        // device population does not carry) is taken from the host below
-        std::vector<wc_particle_data> pos(raw(), raw() + size());
-        for (wc_particle_data &d : pos) d.driftObjectType = 0, d.driftSide = 0;
-        ctx.check(wc_set_particles(c, pos.data(), pos.size()), "wc_set_particles");
+        ctx.check(wc_set_particles(c, raw(), size()), "wc_set_particles");
         size_t n = 0;
         ctx.check(wc_particle_records(c, nullptr, 0, &n), "wc_particle_records");
         std::vector<std::uint64_t> id(n);
         std::vector<double> f[9];
         for (auto &v : f) v.resize(n);
-        wc_records r{id.data(), f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(),
-                     f[5].data(), f[6].data(), f[7].data(), f[8].data()};
-        size_t m = 0;
-        ctx.check(wc_particle_records(c, &r, n, &m), "wc_particle_records");
-        // drift state of each record's particle: records are an order-keeping
-        // subsequence whose id, k, j, i equal their particle's
         std::vector<std::uint32_t> object(n);
         std::vector<std::int32_t> side(n);
-        size_t q = 0;
-        for (size_t k = 0; k < n; ++k) {
-            while (q < size() && !((*this)[q].Id() == id[k] && same((*this)[q].K(), f[3][k]) &&
-                                   same((*this)[q].J(), f[4][k]) && same((*this)[q].I(), f[5][k])))
-                ++q;
-            if (q == size()) throw std::runtime_error("record without its particle");
-            object[k] = (std::uint32_t)(*this)[q].DriftObject();
-            side[k] = (*this)[q].Side();
-            ++q;
-        }
+        wc_records r{id.data(), f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(),
+                     f[5].data(), f[6].data(), f[7].data(), f[8].data(), object.data(), side.data()};
+        size_t m = 0;
+        ctx.check(wc_particle_records(c, &r, n, &m), "wc_particle_records");
         wc_restart_file file{n, particleTime, config.randomSeed, config.trackingDirection == -1 ? 1 : 0,
                              oceanModel.c_str(), provenance, id.data(), object.data(), side.data(),
                              f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(), f[5].data(),
@@ -260,8 +243,6 @@ public:
         for (const wc_particle_data &d : p) push_back(Particle(d));
     }
 
-private:
-    static bool same(double a, double b) { return std::memcmp(&a, &b, sizeof a) == 0; }
 };
 
 /* The forcing window (oceanmodel_data, OceanModelAdapter.hpp:20-34), host

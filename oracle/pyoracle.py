@@ -123,6 +123,10 @@ def _load(path):
         f = getattr(lib, name)
         f.restype = C.c_double
         f.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_uint64, C.c_uint64]
+    lib.wco_jibe_step_probability.restype = C.c_double
+    lib.wco_jibe_step_probability.argtypes = [C.c_double, C.c_double]
+    lib.wco_sample_drift_side.restype = C.c_int
+    lib.wco_sample_drift_side.argtypes = [C.c_uint64, C.c_uint64, C.c_double]
     lib.wco_restart_elapsed.restype = C.c_double
     lib.wco_restart_elapsed.argtypes = [C.c_double] * 3
     lib.wco_reflect_cell.restype = C.c_double
@@ -186,7 +190,30 @@ def ref_lib():
                                      C.c_double, C.c_void_p, C.c_long]
     L.ref_source_position_offset.restype = C.c_double
     L.ref_source_position_offset.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    if hasattr(L, "ref_jibe_step_probability"):
+        L.ref_jibe_step_probability.restype = C.c_double
+        L.ref_jibe_step_probability.argtypes = [C.c_double, C.c_double]
+        L.ref_sample_drift_side.restype = C.c_int
+        L.ref_sample_drift_side.argtypes = [C.c_uint64, C.c_uint64, C.c_double]
+        L.ref_jibe_component.restype = C.c_uint64
+    if hasattr(L, "ref_partition_count"):
+        for name in ("ref_partition_count", "ref_partition_offset"):
+            f = getattr(L, name)
+            f.restype = C.c_uint64
+            f.argtypes = [C.c_uint64] * 3
     return L
+
+
+def partition(total, workers, worker):
+    """(offset, count) of `worker`'s contiguous particle range when `total`
+    particles are split over `workers` ranks: every rank gets total // workers
+    and rank 0 also the remainder (NumericalHelpers::partitionCount /
+    partitionOffset, NumericalHelpers.hpp:76-82; MpiDecompositionTest.cpp:16-20
+    scatters by exactly these ranges)."""
+    base = total // workers
+    count = base + (total % workers if worker == 0 else 0)
+    offset = 0 if worker == 0 else base + total % workers + (worker - 1) * base
+    return offset, count
 
 
 class Window:

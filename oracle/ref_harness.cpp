@@ -1,7 +1,9 @@
 // C-ABI shim over reference headers compiled in place from the reference tree
 // (include path set by oracle/Makefile).  Test infrastructure only: it lets the
 // CPU tests compare the oracle restatement with the reference's own
-// NumericalHelpers.hpp and SourceEmission.hpp bit for bit.
+// NumericalHelpers.hpp, SourceEmission.hpp and the inline helpers of
+// DriftModel.hpp bit for bit.
+#include "DriftModel.hpp"
 #include "NumericalHelpers.hpp"
 #include "SourceEmission.hpp"
 
@@ -35,6 +37,24 @@ double ref_reflect_domain(double coordinate, double maximum) {
 }
 
 const char *ref_scheme() { return NumericalHelpers::RANDOM_KEY_SCHEME; }
+
+double ref_jibe_step_probability(double hourlyProbability, double elapsedSeconds) {
+    return jibeStepProbability(hourlyProbability, elapsedSeconds);
+}
+
+int ref_sample_drift_side(std::uint64_t seed, std::uint64_t particle, double rightProbability) {
+    return static_cast<int>(sampleDriftSide(seed, particle, rightProbability));
+}
+
+std::uint64_t ref_jibe_component() { return LEEWAY_JIBE_RANDOM_COMPONENT; }
+
+std::uint64_t ref_partition_count(std::uint64_t total, std::uint64_t workers, std::uint64_t worker) {
+    return NumericalHelpers::partitionCount(total, workers, worker);
+}
+
+std::uint64_t ref_partition_offset(std::uint64_t total, std::uint64_t workers, std::uint64_t worker) {
+    return NumericalHelpers::partitionOffset(total, workers, worker);
+}
 
 // Same return convention as wco_emission_times: count, -1 invalid (throws), -2 cap too small.
 long ref_emission_times(int mode, double value, double sourceStart, double sourceEnd, double intervalStart,

@@ -55,6 +55,19 @@ double wco_normal(uint64_t seed, uint64_t particle, int64_t interval, uint64_t s
     return sqrt(-2.0 * log(u1)) * cos(6.28318530717958647692 * u2);
 }
 
+/* ---- drift side helpers: DriftModel.hpp:110-160 --------------------------- */
+
+double wco_jibe_step_probability(double hourlyProbability, double elapsedSeconds) {   /* :156-160 */
+    if (hourlyProbability <= 0 || elapsedSeconds <= 0) return 0;
+    if (hourlyProbability >= 1) return 1;
+    return 1 - exp(log(1 - hourlyProbability) * elapsedSeconds / 3600);
+}
+
+int wco_sample_drift_side(uint64_t seed, uint64_t particle, double rightProbability) {   /* :146-154 */
+    double u = wco_uniform(seed, particle, WCO_LEEWAY_SIDE_INTERVAL, 0, 0);
+    return u < rightProbability ? 1 : -1;                 /* RIGHT : LEFT */
+}
+
 /* ---- numerical helpers: NumericalHelpers.hpp:45-74 ----------------------- */
 
 double wco_restart_elapsed(double s, double e, double checkpoint) {   /* :45-52 */
@@ -441,9 +454,15 @@ void wco_move(const wco_config *cfg, const wco_grid *g, double intervalStart, do
             }
         }
 
-        if (P.health > 0) {                                 /* :797-814 (passive: no jibe) */
+        if (P.health > 0) {                                 /* :797-804 */
             P.age = P.age + stepDt;
             P.health = health0 * exp(-P.age / tau0);
+            if (cfg->leewayJibeProbabilityHourly > 0 && P.driftObjectType != 0) {   /* :805-813 */
+                uint64_t substep = (uint64_t)floor(elapsed / dti);
+                double probability = wco_jibe_step_probability(cfg->leewayJibeProbabilityHourly, stepDt);
+                if (wco_uniform(cfg->randomSeed, P.id, intervalKey, substep, WCO_JIBE_COMPONENT) < probability)
+                    P.driftSide = (int8_t)-P.driftSide;
+            }
         }
         elapsed += stepDt;                                  /* :821 */
     }

@@ -19,6 +19,13 @@
  *     oracle/_ref.
  *   - wco_emission_times: tests/SourceEmissionTest.cpp cases and bit-equality
  *     with SourceEmission.hpp compiled from the reference sources (oracle/_ref).
+ *   - wco_jibe_step_probability / wco_sample_drift_side and the jibe
+ *     component: bit-equality with the inline helpers of DriftModel.hpp
+ *     compiled from the reference sources (oracle/_ref).  The side flip inside
+ *     wco_move: the side outcomes of tests/ParticleMoveTest.cpp:192-215 (three
+ *     live substeps at probability 1 turn RIGHT into LEFT; restart splits give
+ *     the full run's side) and the closed form of the flip fraction after n
+ *     substeps, (1 - (1 - 2q)^n) / 2.
  *   - wco_move (Particle::move): Particle.cpp needs NetCDF and log4cplus
  *     headers that this image lacks, so it cannot be built here.  It is pinned by
  *     the closed forms and invariants of tests/ParticleMoveTest.cpp (exact
@@ -110,6 +117,14 @@ enum { WCO_EULER = 1, WCO_MIDPOINT = 2 };                        /* Config.hpp:2
 uint64_t wco_mix(uint64_t value);
 double wco_uniform(uint64_t seed, uint64_t particle, int64_t interval, uint64_t substep, uint64_t component);
 double wco_normal(uint64_t seed, uint64_t particle, int64_t interval, uint64_t substep, uint64_t component);
+
+/* Drift side helpers (DriftModel.hpp:110-160): the hourly jibe probability
+ * over one substep, and the random side a source gives a released particle
+ * (+1 RIGHT, -1 LEFT). */
+#define WCO_JIBE_COMPONENT 0x4a494245ULL                       /* LEEWAY_JIBE_RANDOM_COMPONENT */
+#define WCO_LEEWAY_SIDE_INTERVAL (INT64_MIN + 1)               /* LEEWAY_SIDE_RANDOM_INTERVAL */
+double wco_jibe_step_probability(double hourlyProbability, double elapsedSeconds);
+int wco_sample_drift_side(uint64_t seed, uint64_t particle, double rightProbability);
 
 /* Numerical helpers (NumericalHelpers.hpp:45-74). */
 double wco_restart_elapsed(double intervalStart, double intervalEnd, double checkpoint);

@@ -60,17 +60,18 @@ int main() {
     }
     assert(cfg.restartCheckpoint == 7200.0);
 
-    // the records themselves (lat, lon, depth) from the device
+    // the records themselves (lat, lon, depth, drift state) from the device
     size_t n = 0;
-    ps[0].Drift(0, 0);
-    ps[5].Drift(0, 0);
     ctx.check(wc_set_particles(ctx.get(), ps.raw(), ps.size()), "wc_set_particles");
     std::vector<std::uint64_t> id(6);
     std::vector<double> la(6), lo(6), dep(6), k(6), j(6), i(6), he(6), ag(6), ti(6);
+    std::vector<std::uint32_t> ot(6);
+    std::vector<std::int32_t> sd(6);
     wc_records rec{id.data(), la.data(), lo.data(), dep.data(), k.data(), j.data(), i.data(), he.data(),
-                   ag.data(), ti.data()};
+                   ag.data(), ti.data(), ot.data(), sd.data()};
     ctx.check(wc_particle_records(ctx.get(), &rec, 6, &n), "wc_particle_records");
     assert(n == 3);
+    assert(ot[0] == 3 && sd[0] == -1 && ot[1] == 0 && sd[1] == 0 && ot[2] == 2 && sd[2] == 1);
     for (size_t r = 0; r < n; ++r) {
         assert(std::fabs(la[r] - (40 + 0.1 * j[r])) < 1e-12 && std::fabs(lo[r] - (10 + 0.1 * i[r])) < 1e-12);
         const double hh = 80 + 20 * i[r] + 10 * j[r];

@@ -34,13 +34,17 @@ def oracle_emit(cfg, sources, start, end, particles):
             k = k + 0.25 * L.wco_normal(cfg.randomSeed, next_id, SOURCE_POSITION_RANDOM_INTERVAL, 0, 0)
             j = j + 0.25 * L.wco_normal(cfg.randomSeed, next_id, SOURCE_POSITION_RANDOM_INTERVAL, 0, 1)
             i = i + 0.25 * L.wco_normal(cfg.randomSeed, next_id, SOURCE_POSITION_RANDOM_INTERVAL, 0, 2)
-        out.append((next_id, k, j, i, x))
+        side = cfg.driftSide                              # Source.cpp:38-42
+        if cfg.leewayRandomSide:
+            side = L.wco_sample_drift_side(cfg.randomSeed, next_id, cfg.leewayRightSideProbability)
+        out.append((next_id, k, j, i, x, side))
         next_id += 1
     res = np.zeros(len(particles) + len(out), dtype=wc.PARTICLE_DTYPE)   # keeps the 64-byte layout
     res[: len(particles)] = particles
-    for q, (pid, k, j, i, x) in enumerate(out):
+    for q, (pid, k, j, i, x, side) in enumerate(out):
         r = res[len(particles) + q]
         r["id"], r["k"], r["j"], r["i"], r["health"], r["age"], r["time"] = pid, k, j, i, 1.0, 0.0, x
+        r["driftObjectType"], r["driftSide"] = cfg.driftObjectType, side
     return res
 
 
@@ -56,10 +60,15 @@ def sources_for(W):
             wc.Source(-0.5, 30.2, 50.4, wc.UNIFORM_RATE, 77.0, 900.0, 5000.0)]
 
 
+DRIFT = {"passive": {}, "typed": dict(driftObjectType=3, driftSide=-1),
+         "random_side": dict(driftObjectType=6, leewayRandomSide=True, leewayRightSideProbability=0.3)}
+
+
+@pytest.mark.parametrize("drift", list(DRIFT))
 @pytest.mark.parametrize("random_sources", [False, True])
-def test_emit_matches_oracle(small, random_sources):
+def test_emit_matches_oracle(small, random_sources, drift):
     W = small
-    cfg = wc.default_config(randomSources=random_sources, randomSeed=42)
+    cfg = wc.default_config(randomSources=random_sources, randomSeed=42, **DRIFT[drift])
     src = sources_for(W)
     with wc.Context(0) as ctx:
         ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask)
@@ -70,6 +79,10 @@ def test_emit_matches_oracle(small, random_sources):
         dev = ctx.get_particles()
     assert len(dev) == len(ref) > 0
     assert np.array_equal(dev["id"], ref["id"]) and np.array_equal(dev["time"], ref["time"])
+    assert np.array_equal(dev["driftObjectType"], ref["driftObjectType"])
+    assert np.array_equal(dev["driftSide"], ref["driftSide"])
+    if drift == "random_side":
+        assert set(np.unique(dev["driftSide"])) == {-1, 1}
     tol = 1e-12 if random_sources else 0.0            # device log/cos vs glibc: ulps of 0.25*normal
     for f in ("k", "j", "i"):
         assert np.max(np.abs(dev[f] - ref[f])) <= tol

@@ -2,7 +2,8 @@
 Particles::netcdfRecords, Particles.cpp:115-147, with
 OceanModelAdapter::kji2deplatlon, OceanModelAdapter.cpp:271-321) against the
 oracle, through the C ABI, and the device -> restart file -> context round
-trip (Particles::saveAsNetCDF / loadFromNetCDF).
+trip (Particles::saveAsNetCDF / loadFromNetCDF), drift object type and side
+included.
 
 Bar: bit-identical.  The kept set and its order equal the oracle's, every
 record's depth, lat and lon has the oracle's bits (FP64, no contraction, the
@@ -40,6 +41,8 @@ def positions(n, s_w, eta, xi, seed):
     p["age"] = rng.uniform(0, 1e5, n)
     p["time"] = rng.uniform(0, 1e5, n)
     p["id"] = rng.permutation(n).astype(np.uint64) + np.uint64(2 ** 53)
+    p["driftObjectType"] = np.where(rng.random(n) < 0.5, 0, rng.integers(1, 12, n))
+    p["driftSide"] = rng.choice([-1, 0, 1], n)
     edge = [(0.0, 0.0, 0.0), (-0.0, -0.0, -0.0), (-(s_w - 2.0), 1.0, 1.0), (-(s_w - 1.0), 1.0, 1.0),
             (-0.5, eta - 1.0, 2.0), (-0.5, 2.0, xi - 1.0), (-0.5, eta - 1.0 - 1e-12, xi - 1.0 - 1e-12),
             (np.nan, 1.0, 1.0), (-0.5, np.nan, 1.0), (-0.5, 1.0, np.inf), (-np.inf, 1.0, 1.0), (-0.5, 3e9, 1.0),
@@ -57,6 +60,8 @@ def check(ctx, p, s_w_values, lat, lon, h):
     assert np.array_equal(got["id"], p["id"][src])
     for f in ("k", "j", "i", "health", "age", "time"):
         assert np.array_equal(got[f].view(np.uint64), p[f][src].view(np.uint64)), f
+    assert np.array_equal(got["object_type"], p["driftObjectType"][src])     # Particles.cpp:144-145
+    assert np.array_equal(got["drift_side"], p["driftSide"][src])
     for f, ref in (("depth", dep), ("lat", la), ("lon", lo)):
         assert np.array_equal(got[f].view(np.uint64), ref.view(np.uint64)), f
     return got
@@ -114,3 +119,5 @@ def test_restart_round_trip_through_device(ctx, tmp_path):
     assert np.array_equal(again["id"], p["id"][src])
     for f in ("k", "j", "i", "health", "age", "time"):
         assert np.array_equal(again[f].view(np.uint64), p[f][src].view(np.uint64)), f
+    assert np.array_equal(again["driftObjectType"], p["driftObjectType"][src])
+    assert np.array_equal(again["driftSide"], p["driftSide"][src])

@@ -218,6 +218,142 @@ def test_move_parallel_equals_serial():
     assert np.array_equal(a, b)
 
 
+def test_partition_ranges():
+    # NumericalHelpers.hpp:76-82: the ranges tile [0, total) in rank order,
+    # rank 0 takes the remainder, the others total // workers each
+    for total in (0, 1, 10, 11, 4_000_000, 4_000_003):
+        for workers in (1, 2, 3, 7, 8):
+            ranges = [po.partition(total, workers, r) for r in range(workers)]
+            assert ranges[0][0] == 0
+            for (o0, n0), (o1, _) in zip(ranges, ranges[1:]):
+                assert o1 == o0 + n0
+            assert sum(n for _, n in ranges) == total
+            assert all(n == total // workers for _, n in ranges[1:])
+    # MpiDecompositionTest.cpp:15-20 with 11 particles over 4 ranks
+    assert [po.partition(11, 4, r) for r in range(4)] == [(0, 5), (5, 2), (7, 2), (9, 2)]
+
+
+@needs_ref
+def test_partition_matches_reference_build():
+    if not hasattr(REF, "ref_partition_count"):
+        pytest.skip("oracle/_ref built before the partition helpers were added")
+    for total in (0, 1, 11, 4_000_003, 2 ** 40 + 5):
+        for workers in (1, 2, 3, 8, 13):
+            for r in range(workers):
+                assert po.partition(total, workers, r) == (REF.ref_partition_offset(total, workers, r),
+                                                           REF.ref_partition_count(total, workers, r))
+
+
+def test_move_decomposed_equals_serial():
+    # MpiDecompositionTest.cpp:15-80: 11 particles scattered over the ranks by
+    # partitionCount / partitionOffset, each range moved on its own, gathered
+    # in rank order: bit-identical to one serial move, for every world size
+    W = tiny_window(times=(0.0, 60.0))
+    cfg = move_cfg(random=True, randomSeed=5489, deltat=60, dti=30, sigma=.1)
+    p = po.make_particles(11, k=-.5, j=.25, i=.25, first_id=9007199254740993)
+    serial = p.copy()
+    po.move(W, cfg, 0, serial, nthreads=1)
+    assert not np.array_equal(serial["i"], p["i"])
+    for workers in (1, 2, 3, 4, 11):
+        parts = []
+        for r in range(workers):
+            o, n = po.partition(len(p), workers, r)
+            local = p[o:o + n].copy()
+            po.move(W, cfg, 0, local, nthreads=1)
+            parts.append(local)
+        assert np.array_equal(np.concatenate(parts), serial)
+
+
+# --------------------------------------------------------------------------
+# drift object type and side: carried by the move, flipped by the hourly jibe
+# (Particle.cpp:805-813, DriftModel.hpp:110-160)
+# --------------------------------------------------------------------------
+
+PERSON_IN_WATER, RIGHT, LEFT = 1, 1, -1
+
+
+def typed(id_, k, j, i, time, side=RIGHT, kind=PERSON_IN_WATER):
+    p = one(id_, k, j, i, time)
+    p["driftObjectType"], p["driftSide"] = kind, side
+    return p
+
+
+def test_jibe_reference_cases():
+    # tests/ParticleMoveTest.cpp:192-215 with driftModel 0: the side outcome
+    # does not depend on the drift model (the positions there need leeway)
+    W = tiny_window()                                      # ocean time 0 .. 65: substeps of 30, 30, 5 s
+    cfg = move_cfg(leewayJibeProbabilityHourly=1.0)
+    full = typed(30, -.5, .25, .25, 0)
+    po.move(W, cfg, 0, full)
+    r = typed(30, -.5, .25, .25, 0)
+    po.move(tiny_window(times=(0.0, 30.0)), cfg, 0, r)
+    po.move(W, move_cfg(leewayJibeProbabilityHourly=1.0, restartCheckpoint=30), 0, r)
+    assert full["driftSide"][0] == LEFT and r["driftSide"][0] == full["driftSide"][0]
+    assert full["driftObjectType"][0] == PERSON_IN_WATER
+    # backward from 65, and restarted at 35 after the late window 35 .. 65
+    back = move_cfg(leewayJibeProbabilityHourly=1.0, trackingDirection=-1)
+    bf = typed(31, -.5, .25, .25, 65)
+    po.move(W, back, 1, bf)
+    br = typed(31, -.5, .25, .25, 65)
+    po.move(tiny_window(times=(35.0, 65.0)), back, 1, br)
+    po.move(W, move_cfg(leewayJibeProbabilityHourly=1.0, trackingDirection=-1, restartCheckpoint=35), 1, br)
+    assert bf["driftSide"][0] == br["driftSide"][0] == LEFT
+    # passive particles and an undefined side never change; no jibe without a probability
+    passive = one(32, -.5, .25, .25, 0)
+    undefined = typed(33, -.5, .25, .25, 0, side=0)
+    po.move(W, cfg, 0, passive)
+    po.move(W, cfg, 0, undefined)
+    assert passive["driftSide"][0] == 0 and undefined["driftSide"][0] == 0
+    off = typed(34, -.5, .25, .25, 0)
+    po.move(W, move_cfg(), 0, off)
+    assert off["driftSide"][0] == RIGHT
+
+
+def test_jibe_flip_fraction_closed_form():
+    # per substep the side flips with q = 1 - (1-p)^(dt/3600); after n
+    # independent substeps it is flipped with probability (1 - (1-2q)^n)/2
+    p_hour, n_part = 0.5, 20000
+    W = tiny_window(times=(0.0, 3600.0))
+    cfg = move_cfg(leewayJibeProbabilityHourly=p_hour, randomSeed=77)
+    ps = po.make_particles(n_part, k=-.5, j=.25, i=.25, first_id=1000)
+    ps["driftObjectType"], ps["driftSide"] = PERSON_IN_WATER, RIGHT
+    po.move(W, cfg, 0, ps, nthreads=8)
+    q = 1 - (1 - p_hour) ** (30 / 3600)
+    expect = (1 - (1 - 2 * q) ** 120) / 2
+    got = float(np.mean(ps["driftSide"] == LEFT))
+    assert abs(got - expect) < 5 * math.sqrt(expect * (1 - expect) / n_part), (got, expect)
+    assert set(np.unique(ps["driftSide"])) <= {LEFT, RIGHT}
+
+
+@needs_ref
+def test_drift_helpers_match_reference_build():
+    if not hasattr(REF, "ref_jibe_step_probability"):
+        pytest.skip("oracle/_ref built before the drift helpers were added")
+    rng = np.random.default_rng(4)
+    for h in (-0.5, 0.0, 1e-9, 0.05, 0.5, 0.999, 1.0, 2.0):
+        for dt in (-1.0, 0.0, 1e-3, 5.0, 30.0, 3600.0, 7200.5):
+            assert L.wco_jibe_step_probability(h, dt) == REF.ref_jibe_step_probability(h, dt)
+    for _ in range(2000):
+        seed, pid = (int(x) for x in rng.integers(0, 2 ** 63, 2))
+        prob = float(rng.uniform(-0.1, 1.1))
+        assert L.wco_sample_drift_side(seed, pid, prob) == REF.ref_sample_drift_side(seed, pid, prob)
+    # the flip inside the move uses the reference's jibe component and step
+    # probability on the substep key: recompute the side from those
+    W = tiny_window(times=(0.0, 3600.0))
+    cfg = move_cfg(leewayJibeProbabilityHourly=0.3, randomSeed=9)
+    ps = po.make_particles(300, k=-.5, j=.25, i=.25, first_id=50)
+    ps["driftObjectType"], ps["driftSide"] = PERSON_IN_WATER, RIGHT
+    po.move(W, cfg, 0, ps, nthreads=1)
+    prob = REF.ref_jibe_step_probability(0.3, 30.0)
+    comp = REF.ref_jibe_component()
+    for r in ps:
+        side = RIGHT
+        for sub in range(120):
+            if REF.ref_uniform(9, int(r["id"]), 0, sub, comp) < prob:
+                side = -side
+        assert r["driftSide"] == side
+
+
 def test_move_restart_forward_backward():
     # ParticleMoveTest.cpp:102-129
     _, full = case_time_linear()

@@ -88,7 +88,7 @@ RECORD_FIELDS = ("id", "lat", "lon", "depth", "k", "j", "i", "health", "age", "t
 
 class Records(C.Structure):
     """wc_records: host arrays receiving Particles::netcdfRecords."""
-    _fields_ = [(n, C.c_void_p) for n in RECORD_FIELDS]
+    _fields_ = [(n, C.c_void_p) for n in RECORD_FIELDS] + [("object_type", C.c_void_p), ("drift_side", C.c_void_p)]
 
 
 class EnsSource(C.Structure):
@@ -364,11 +364,14 @@ class Context:
 
     def particle_records(self) -> dict:
         """Particles::netcdfRecords (Particles.cpp:115-147) computed on the
-        device: a dict of the RECORD_FIELDS arrays."""
+        device: a dict of the RECORD_FIELDS arrays plus object_type (uint32)
+        and drift_side (int32)."""
         n = C.c_size_t()
         self._chk(self.L.wc_particle_records(self.h, None, 0, C.byref(n)))
         out = {f: np.zeros(n.value, np.uint64 if f == "id" else np.float64) for f in RECORD_FIELDS}
-        r = Records(*[_p(out[f]).value for f in RECORD_FIELDS])
+        out["object_type"] = np.zeros(n.value, np.uint32)
+        out["drift_side"] = np.zeros(n.value, np.int32)
+        r = Records(*[_p(out[f]).value for f in RECORD_FIELDS + ("object_type", "drift_side")])
         m = C.c_size_t()
         self._chk(self.L.wc_particle_records(self.h, C.byref(r), n.value, C.byref(m)))
         assert m.value == n.value
@@ -639,9 +642,14 @@ def write_restart_nc(path, records: dict, particle_time: float, random_seed: int
                      ocean_model: str = "roms", attributes=None, object_type=None, drift_side=None):
     """Particles::saveAsNetCDF's restart file (Particles.cpp:150-264), restart
     version 3, in the NetCDF classic format with 64-bit data (CDF-5).
-    records: the RECORD_FIELDS arrays (as particle_records returns them)."""
+    records: the RECORD_FIELDS arrays (as particle_records returns them);
+    object_type / drift_side default to the records' own, if they hold them."""
     L = load_library()
     n = len(records["id"])
+    if object_type is None:
+        object_type = records.get("object_type")
+    if drift_side is None:
+        drift_side = records.get("drift_side")
     ids = np.ascontiguousarray(records["id"], dtype=np.uint64)
     cols = {f: _f64(records[f]) for f in RECORD_FIELDS if f != "id"}
     for a in cols.values():

@@ -47,6 +47,7 @@ struct wc_ctx {
     // particles (SoA, double-buffered for compaction)
     size_t n = 0, cap = 0;
     SoA s{}, s2{};
+    bool typed = false;   // a non-passive particle may be held (set at upload and release)
     uint32_t *d_blockCounts = nullptr;
     unsigned long long *d_total = nullptr;
     unsigned long long *d_steps = nullptr;
@@ -174,6 +175,8 @@ bool make_params(const wc_config_data *cfg, double start, double end, int s_w, i
     P.ageKill = cfg->survprob > 0 ? -cfg->tau0 * std::log(cfg->survprob) : INFINITY;
     P.seed = cfg->randomSeed;
     P.intervalKey = (int64_t)std::llround(std_min(start, end));
+    P.jibeHourly = cfg->leewayJibeProbabilityHourly;
+    P.jibe = P.jibeHourly > 0;
     P.upper = cfg->upperClosure;
     P.lower = cfg->lowerClosure;
     P.horizontal = cfg->horizontalClosure;
@@ -187,11 +190,15 @@ bool make_params(const wc_config_data *cfg, double start, double end, int s_w, i
     return true;
 }
 
-int check_supported(wc_ctx *c, const wc_config_data *cfg) {
+// jibe: whether the caller's kernels apply the hourly side flip of non-passive
+// particles (Particle.cpp:805-813); where they do not, a positive probability
+// is accepted only when no non-passive particle can be present.
+int check_supported(wc_ctx *c, const wc_config_data *cfg, bool jibe) {
     if (cfg->driftModel == 1)
         return fail(c, WC_ERR_UNSUPPORTED, "driftModel==1 (leeway) is not implemented on the device path");
-    if (cfg->leewayJibeProbabilityHourly > 0)
-        return fail(c, WC_ERR_UNSUPPORTED, "leeway jibing is not implemented on the device path");
+    if (cfg->leewayJibeProbabilityHourly > 0 && !jibe)
+        return fail(c, WC_ERR_UNSUPPORTED,
+                    "leeway jibing of non-passive particles is implemented on the FP64 parity path only");
     if (!(cfg->dti > 0)) return fail(c, WC_ERR_INVALID, "dti must be > 0");
     if (cfg->trackingDirection != 1 && cfg->trackingDirection != -1)
         return fail(c, WC_ERR_INVALID, "trackingDirection must be +1 or -1");
@@ -200,6 +207,7 @@ int check_supported(wc_ctx *c, const wc_config_data *cfg) {
 
 void free_soa(SoA &s) {
     hipFree(s.id); hipFree(s.k); hipFree(s.j); hipFree(s.i); hipFree(s.health); hipFree(s.age); hipFree(s.time);
+    hipFree(s.tag);
     s = SoA{};
 }
 
@@ -211,6 +219,7 @@ int alloc_soa(wc_ctx *c, SoA &s, size_t cap) {
     HIPCHK(c, hipMalloc(&s.health, cap * sizeof(double)));
     HIPCHK(c, hipMalloc(&s.age, cap * sizeof(double)));
     HIPCHK(c, hipMalloc(&s.time, cap * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.tag, cap * sizeof(uint32_t)));
     return WC_OK;
 }
 
@@ -234,6 +243,7 @@ int reserve(wc_ctx *c, size_t need) {
         HIPCHK(c, hipMemcpyAsync(a.health, c->s.health, n * 8, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(a.age, c->s.age, n * 8, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(a.time, c->s.time, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.tag, c->s.tag, n * 4, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     free_soa(c->s);
@@ -247,23 +257,26 @@ int reserve(wc_ctx *c, size_t need) {
     return WC_OK;
 }
 
-// Host AoS -> device SoA at offset `at`.
+// Host AoS -> device SoA at offset `at`.  Non-passive particles (a drift
+// object type other than PASSIVE) move passively with driftModel 0, as in the
+// reference; their type and side travel with them.
 int upload_particles(wc_ctx *c, const wc_particle_data *p, size_t n, size_t at) {
     std::vector<uint64_t> id(n);
     std::vector<double> f[6];
+    std::vector<uint32_t> tag(n);
     for (auto &v : f) v.resize(n);
     for (size_t q = 0; q < n; ++q) {
         id[q] = p[q].id;
         f[0][q] = p[q].k; f[1][q] = p[q].j; f[2][q] = p[q].i;
         f[3][q] = p[q].health; f[4][q] = p[q].age; f[5][q] = p[q].time;
-        if (p[q].driftObjectType != 0)
-            return fail(c, WC_ERR_UNSUPPORTED, "particle %zu is not passive (driftObjectType=%u)", q,
-                        (unsigned)p[q].driftObjectType);
+        tag[q] = make_tag(p[q].driftObjectType, p[q].driftSide);
+        if (p[q].driftObjectType != 0) c->typed = true;
     }
     double *dst[6] = {c->s.k, c->s.j, c->s.i, c->s.health, c->s.age, c->s.time};
     HIPCHK(c, hipMemcpyAsync(c->s.id + at, id.data(), n * 8, hipMemcpyHostToDevice, c->stream));
     for (int q = 0; q < 6; ++q)
         HIPCHK(c, hipMemcpyAsync(dst[q] + at, f[q].data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->s.tag + at, tag.data(), n * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return WC_OK;
 }
@@ -371,7 +384,7 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
     if (hipMemcpyAsync(t.data(), times, oceanTime * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
         return WC_ERR_HIP;
     if (hipStreamSynchronize(st) != hipSuccess) return WC_ERR_HIP;
-    int rc = check_supported(nullptr, &cfg);
+    int rc = check_supported(nullptr, &cfg, true);
     if (rc) return rc;
     // interval and record pair: Particle.cpp:110-115, 183-184
     const int dir = cfg.trackingDirection;
@@ -734,7 +747,16 @@ int wc_particle_records(wc_ctx *c, const wc_records *out, size_t capacity, size_
         {out->time, c->s2.time}};
     for (const auto &e : f)
         if (e.dst) HIPCHK(c, hipMemcpyAsync(e.dst, e.src, b, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> tag;
+    if (out->object_type || out->drift_side) {
+        tag.resize(total);
+        HIPCHK(c, hipMemcpyAsync(tag.data(), c->s2.tag, total * 4, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < tag.size(); ++q) {
+        if (out->object_type) out->object_type[q] = tag_type(tag[q]);
+        if (out->drift_side) out->drift_side[q] = tag_side(tag[q]);
+    }
     return WC_OK;
 }
 
@@ -839,6 +861,7 @@ int wc_set_particles(wc_ctx *c, const wc_particle_data *p, size_t n) {
     int rc = reserve(c, n);
     if (rc) return rc;
     c->n = 0;
+    c->typed = false;
     if (n) {
         rc = upload_particles(c, p, n, 0);
         if (rc) return rc;
@@ -867,18 +890,20 @@ int wc_get_particles(wc_ctx *c, wc_particle_data *p, size_t n) {
     std::vector<uint64_t> id(n);
     std::vector<double> f[6];
     for (auto &v : f) v.resize(n);
+    std::vector<uint32_t> tag(n);
     const double *src[6] = {c->s.k, c->s.j, c->s.i, c->s.health, c->s.age, c->s.time};
     HIPCHK(c, hipMemcpyAsync(id.data(), c->s.id, n * 8, hipMemcpyDeviceToHost, c->stream));
     for (int q = 0; q < 6; ++q)
         HIPCHK(c, hipMemcpyAsync(f[q].data(), src[q], n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(tag.data(), c->s.tag, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t q = 0; q < n; ++q) {
         wc_particle_data d{};
         d.id = id[q];
         d.k = f[0][q]; d.j = f[1][q]; d.i = f[2][q];
         d.health = f[3][q]; d.age = f[4][q]; d.time = f[5][q];
-        d.driftObjectType = 0;
-        d.driftSide = 0;
+        d.driftObjectType = tag_type(tag[q]);
+        d.driftSide = tag_side(tag[q]);
         p[q] = d;
     }
     return WC_OK;
@@ -893,7 +918,7 @@ int wc_particle_count(const wc_ctx *c, size_t *n) {
 int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double end, int slot0, int slot1,
                      int path) {
     if (!c || !cfg) return WC_ERR_INVALID;
-    int rc = check_supported(c, cfg);
+    int rc = check_supported(c, cfg, path == WC_PATH_PARITY || !c->typed);
     if (rc) return rc;
     if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
     const int ns = (int)c->slots.size();
@@ -994,8 +1019,11 @@ int wc_emit(wc_ctx *c, const wc_config_data *cfg, const wc_source *sources, int 
     HIPCHK(c, hipMemcpy(d_times, times.data(), m * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d_src, src.data(), m * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d_pos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
+    const EmitDrift drift{cfg->driftObjectType, cfg->driftSide, cfg->leewayRandomSide ? 1 : 0,
+                          cfg->leewayRightSideProbability};
     HIPCHK(c, launch_emit(c->s, c->n, (int)m, d_times, d_src, d_pos, firstId, cfg->randomSeed,
-                          cfg->randomSources ? 1 : 0, c->stream));
+                          cfg->randomSources ? 1 : 0, drift, c->stream));
+    if (cfg->driftObjectType != 0) c->typed = true;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     hipFree(d_times); hipFree(d_src); hipFree(d_pos);
     c->n += m;
@@ -1007,7 +1035,7 @@ int wc_run_window(wc_ctx *c, const wc_config_data *cfg, const wc_window *win, co
                   int n_sources, int path, float *conc, int mask_output, wc_run_stats *stats) {
     if (!c || !cfg || !win || !win->oceanTime || win->T < 1) return WC_ERR_INVALID;
     if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
-    int rc = check_supported(c, cfg);
+    int rc = check_supported(c, cfg, path == WC_PATH_PARITY || (!c->typed && cfg->driftObjectType == 0));
     if (rc) return rc;
     const int T = win->T, dir = cfg->trackingDirection;
     for (int t = 1; t < T; ++t)                               // ROMSAdapter.cpp:197-201
@@ -1168,8 +1196,12 @@ int wc_ens_set(wc_ctx *c, const wc_particle_data *p, const uint8_t *scenario, si
     if (n > (size_t)INT32_MAX) return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", INT32_MAX);
     const size_t plane = (size_t)c->eta * c->xi;
     if ((size_t)n_scenarios * plane > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "surface grids too large");
-    for (size_t q = 0; q < n; ++q)
+    for (size_t q = 0; q < n; ++q) {
         if (scenario[q] >= n_scenarios) return fail(c, WC_ERR_INVALID, "particle %zu: scenario %d", q, scenario[q]);
+        if (p[q].driftObjectType != 0)
+            return fail(c, WC_ERR_UNSUPPORTED, "ensemble particles are passive tracers (particle %zu has type %u)", q,
+                        (unsigned)p[q].driftObjectType);
+    }
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_ens(c);
@@ -1347,7 +1379,7 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
                     double *best_ms) {
     if (!c || !cfg) return WC_ERR_INVALID;
     if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
-    int rc = check_supported(c, cfg);
+    int rc = check_supported(c, cfg, true);      // ensemble particles are passive: no side to flip
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     auto &E = c->ens;
@@ -1418,7 +1450,7 @@ int wc_ens_run(wc_ctx *c, const wc_config_data *cfg, const double *times, const 
     if (!c || !cfg || !times || !slots || n_intervals < 0) return WC_ERR_INVALID;
     if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
     if (n_intervals > kEnsMaxSteps) return fail(c, WC_ERR_INVALID, "at most %d intervals per call", kEnsMaxSteps);
-    int rc = check_supported(c, cfg);
+    int rc = check_supported(c, cfg, true);      // ensemble particles are passive: no side to flip
     if (rc) return rc;
     if (stats) *stats = wc_ens_stats{};
     if (n_intervals == 0) return WC_OK;

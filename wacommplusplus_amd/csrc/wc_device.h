@@ -24,6 +24,8 @@ struct MoveParams {
     double ageKill;         // exp(-age/tau0) < survprob  <=>  age > ageKill (mixed-precision path)
     uint64_t seed;
     int64_t intervalKey;    // llround(min(start, end))
+    double jibeHourly;      // leewayJibeProbabilityHourly
+    int jibe;               // jibeHourly > 0: non-passive particles may flip their drift side
     int upper, lower, horizontal;
     int dirInt;
     int randomOn;           // random && (direction > 0 || backwardDiffusion)
@@ -65,25 +67,49 @@ struct EnsSoA {
     uint8_t *scen;
 };
 
+// Drift object type and crosswind side of a particle (particle_data's
+// driftObjectType and driftSide) packed in one word: type in bits 0-15, the
+// side's two's-complement byte in bits 16-23.  Type 0 is PASSIVE.
+__host__ __device__ __forceinline__ uint32_t make_tag(uint16_t type, int8_t side) {
+    return (uint32_t)type | ((uint32_t)(uint8_t)side << 16);
+}
+__host__ __device__ __forceinline__ uint16_t tag_type(uint32_t tag) { return (uint16_t)(tag & 0xffffu); }
+__host__ __device__ __forceinline__ int8_t tag_side(uint32_t tag) { return (int8_t)(uint8_t)(tag >> 16); }
+
+// Drift fields of released particles (Source.cpp:38-42): the configured type,
+// and the configured side or, with randomSide, RIGHT when the keyed uniform
+// of the particle is below rightProbability, else LEFT.
+struct EmitDrift {
+    uint16_t type;
+    int8_t side;
+    int randomSide;
+    double rightProbability;
+};
+
 // Particle state, structure of arrays (the context's canonical layout).
 struct SoA {
     uint64_t *id;
     double *k, *j, *i, *health, *age, *time;
+    uint32_t *tag;
 };
 
 struct State {
     uint64_t id;
     double k, j, i, health, age, time;
+    uint32_t tag;
 };
 
+// Only a non-passive particle's side can change during a move, so the tag is
+// written back for those alone.
 struct SoAAccess {
     SoA s;
     __device__ __forceinline__ void load(int idx, State &p) const {
         p.id = s.id[idx]; p.k = s.k[idx]; p.j = s.j[idx]; p.i = s.i[idx];
-        p.health = s.health[idx]; p.age = s.age[idx]; p.time = s.time[idx];
+        p.health = s.health[idx]; p.age = s.age[idx]; p.time = s.time[idx]; p.tag = s.tag[idx];
     }
     __device__ __forceinline__ void store(int idx, const State &p) const {
         s.k[idx] = p.k; s.j[idx] = p.j; s.i[idx] = p.i; s.health[idx] = p.health; s.age[idx] = p.age;
+        if (tag_type(p.tag) != 0) s.tag[idx] = p.tag;
     }
 };
 
@@ -92,10 +118,12 @@ struct AoSAccess {
     __device__ __forceinline__ void load(int idx, State &q) const {
         const wc_particle_data &d = p[idx];
         q.id = d.id; q.k = d.k; q.j = d.j; q.i = d.i; q.health = d.health; q.age = d.age; q.time = d.time;
+        q.tag = make_tag(d.driftObjectType, d.driftSide);
     }
     __device__ __forceinline__ void store(int idx, const State &q) const {
         wc_particle_data &d = p[idx];
         d.k = q.k; d.j = q.j; d.i = q.i; d.health = q.health; d.age = q.age;
+        if (tag_type(q.tag) != 0) d.driftSide = tag_side(q.tag);
     }
 };
 

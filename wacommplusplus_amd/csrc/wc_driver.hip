@@ -31,6 +31,7 @@ namespace wc {
 namespace {
 
 constexpr int64_t kSourcePositionInterval = (-9223372036854775807LL - 1) + 2;   // SourceEmission.hpp:18
+constexpr int64_t kLeewaySideInterval = (-9223372036854775807LL - 1) + 1;       // DriftModel.hpp:110-111
 
 __device__ double keyed_normal_full(uint64_t seed, uint64_t particle, int64_t interval, uint64_t substep,
                                     uint64_t component) {
@@ -43,7 +44,8 @@ __device__ double keyed_normal_full(uint64_t seed, uint64_t particle, int64_t in
 }
 
 __global__ void emit_kernel(SoA s, size_t at, int n, const double *__restrict__ times, const int *__restrict__ src,
-                            const double *__restrict__ pos, uint64_t firstId, uint64_t seed, int randomSources) {
+                            const double *__restrict__ pos, uint64_t firstId, uint64_t seed, int randomSources,
+                            EmitDrift drift) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const uint64_t id = firstId + (uint64_t)q;
@@ -62,14 +64,21 @@ __global__ void emit_kernel(SoA s, size_t at, int n, const double *__restrict__ 
     s.health[o] = 1.0;                                        // Particle(id,k,j,i,time): health0, age 0
     s.age[o] = 0.0;
     s.time[o] = times[q];
+    int8_t side = drift.side;
+    if (drift.randomSide) {                                   // sampleDriftSide, DriftModel.hpp:146-154
+        const uint64_t key = mix64(mix64(mix64(mix64(mix64(seed) ^ id) ^ (uint64_t)kLeewaySideInterval) ^ 0ULL) ^ 0ULL);
+        side = (double)(key >> 11) * 0x1.0p-53 < drift.rightProbability ? 1 : -1;
+    }
+    s.tag[o] = make_tag(drift.type, side);
 }
 
 }  // namespace
 
 hipError_t launch_emit(const SoA &s, size_t at, int n, const double *times, const int *src, const double *pos,
-                       uint64_t firstId, uint64_t seed, int randomSources, hipStream_t stream) {
+                       uint64_t firstId, uint64_t seed, int randomSources, EmitDrift drift, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    emit_kernel<<<(n + 255) / 256, 256, 0, stream>>>(s, at, n, times, src, pos, firstId, seed, randomSources);
+    emit_kernel<<<(n + 255) / 256, 256, 0, stream>>>(s, at, n, times, src, pos, firstId, seed, randomSources,
+                                                      drift);
     return hipGetLastError();
 }
 

@@ -154,6 +154,7 @@ __global__ __launch_bounds__(kBlock) void scatter_alive_kernel(SoA src, SoA dst,
         const uint32_t o = base + (uint32_t)lanes_below(m);
         dst.id[o] = src.id[idx]; dst.k[o] = src.k[idx]; dst.j[o] = src.j[idx]; dst.i[o] = src.i[idx];
         dst.health[o] = src.health[idx]; dst.age[o] = src.age[idx]; dst.time[o] = src.time[idx];
+        dst.tag[o] = src.tag[idx];
     }
 }
 
@@ -221,6 +222,7 @@ __global__ __launch_bounds__(kBlock) void records_scatter_kernel(SoA src, int n,
         const uint32_t o = base + (uint32_t)lanes_below(m);
         dst.id[o] = src.id[idx]; dst.k[o] = src.k[idx]; dst.j[o] = src.j[idx]; dst.i[o] = src.i[idx];
         dst.health[o] = src.health[idx]; dst.age[o] = src.age[idx]; dst.time[o] = src.time[idx];
+        dst.tag[o] = src.tag[idx];
         dep[o] = d; lat[o] = la; lon[o] = lo;
     }
 }

@@ -47,7 +47,7 @@ hipError_t launch_particle_records(const SoA &src, int n, const GeoPtrs &g, cons
                                    hipStream_t stream);
 
 hipError_t launch_emit(const SoA &s, size_t at, int n, const double *times, const int *src, const double *pos,
-                       uint64_t firstId, uint64_t seed, int randomSources, hipStream_t stream);
+                       uint64_t firstId, uint64_t seed, int randomSources, EmitDrift drift, hipStream_t stream);
 bool emission_times(int mode, double value, double sourceStart, double sourceEnd, double intervalStart,
                     double intervalEnd, std::vector<double> &out);
 
