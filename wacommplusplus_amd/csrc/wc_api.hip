@@ -112,6 +112,17 @@ struct wc_ctx {
 
 namespace {
 
+// Device scratch of one call, freed on every return path.
+struct DeviceTemp {
+    void *p = nullptr;
+    DeviceTemp() = default;
+    DeviceTemp(const DeviceTemp &) = delete;
+    DeviceTemp &operator=(const DeviceTemp &) = delete;
+    ~DeviceTemp() { if (p) (void)hipFree(p); }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
 int fail(wc_ctx *c, int code, const char *fmt, ...) {
     if (c) {
         char buf[512];
@@ -1011,11 +1022,12 @@ int wc_emit(wc_ctx *c, const wc_config_data *cfg, const wc_source *sources, int 
     for (int q = 0; q < n_sources; ++q) {
         pos[3 * q] = sources[q].k; pos[3 * q + 1] = sources[q].j; pos[3 * q + 2] = sources[q].i;
     }
-    double *d_times = nullptr, *d_pos = nullptr;
-    int *d_src = nullptr;
-    HIPCHK(c, hipMalloc(&d_times, m * 8));
-    HIPCHK(c, hipMalloc(&d_src, m * 4));
-    HIPCHK(c, hipMalloc(&d_pos, pos.size() * 8));
+    DeviceTemp t_times, t_src, t_pos;
+    HIPCHK(c, hipMalloc(&t_times.p, m * 8));
+    HIPCHK(c, hipMalloc(&t_src.p, m * 4));
+    HIPCHK(c, hipMalloc(&t_pos.p, pos.size() * 8));
+    double *d_times = t_times.as<double>(), *d_pos = t_pos.as<double>();
+    int *d_src = t_src.as<int>();
     HIPCHK(c, hipMemcpy(d_times, times.data(), m * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d_src, src.data(), m * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d_pos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
@@ -1025,7 +1037,6 @@ int wc_emit(wc_ctx *c, const wc_config_data *cfg, const wc_source *sources, int 
                           cfg->randomSources ? 1 : 0, drift, c->stream));
     if (cfg->driftObjectType != 0) c->typed = true;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    hipFree(d_times); hipFree(d_src); hipFree(d_pos);
     c->n += m;
     if (emitted) *emitted = m;
     return WC_OK;
@@ -1299,10 +1310,11 @@ int wc_ens_release(wc_ctx *c, const wc_ens_source *sources, int n_sources, const
     std::vector<float> sg(n_scenarios);
     for (int q = 0; q < n_scenarios; ++q) sg[q] = (float)sigma[q];
     HIPCHK(c, hipMemcpy(E.d_sigma, sg.data(), n_scenarios * sizeof(float), hipMemcpyHostToDevice));
-    EnsSource *d_src = nullptr;
-    unsigned long long *d_first = nullptr;
-    HIPCHK(c, hipMalloc(&d_src, (n_sources ? n_sources : 1) * sizeof(EnsSource)));
-    HIPCHK(c, hipMalloc(&d_first, (n_sources + 1) * sizeof(unsigned long long)));
+    DeviceTemp t_src, t_first;
+    HIPCHK(c, hipMalloc(&t_src.p, (n_sources ? n_sources : 1) * sizeof(EnsSource)));
+    HIPCHK(c, hipMalloc(&t_first.p, (n_sources + 1) * sizeof(unsigned long long)));
+    EnsSource *d_src = t_src.as<EnsSource>();
+    unsigned long long *d_first = t_first.as<unsigned long long>();
     hipError_t e = hipSuccess;
     if (n_sources) e = hipMemcpy(d_src, src.data(), n_sources * sizeof(EnsSource), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -1311,8 +1323,6 @@ int wc_ens_release(wc_ctx *c, const wc_ens_source *sources, int n_sources, const
         e = launch_ens_release(d_src, d_first, n_sources, (long long)n, c->eta, c->xi, seed, E.s, c->stream);
     if (e == hipSuccess) e = launch_fill_u32(E.d_surface, (size_t)n_scenarios * plane, 0u, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d_src);
-    hipFree(d_first);
     if (e != hipSuccess) return fail(c, WC_ERR_HIP, "ensemble release: %s", hipGetErrorString(e));
     E.n = n;
     E.nScen = n_scenarios;
