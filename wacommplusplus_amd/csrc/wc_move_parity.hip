@@ -310,23 +310,26 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
                         zm.c * iMF * jMF + zm.d * iMF * (1.0 - jMF);
                 midValid = hcMid > P.shoreLimit && hcMid > 0;
             }
+            // second stage at the midpoint, or the Euler fallback at the
+            // start position; both sample at alpha, so one gather serves
+            // either (the lanes of a wave may take different ones)
+            const int kS = midValid ? (int)ceil(kMid) : kI;
+            const double kSF = midValid ? ceil(kMid) - kMid : kF;
+            const int jS = midValid ? jMI : jI, iS = midValid ? iMI : iI;
+            const double jSF = midValid ? jMF : jF, iSF = midValid ? iMF : iF;
+            float us, vs, ws;
+            velocity(S, kS, kSF, jS, jSF, iS, iSF, alpha, us, vs, ws);
             if (midValid) {
-                const int kMI = (int)ceil(kMid);
-                const double kMF = ceil(kMid) - kMid;
-                float um, vm, wm;
-                velocity(S, kMI, kMF, jMI, jMF, iMI, iMF, alpha, um, vm, wm);
                 double xdistMid, ydistMid;
                 S.metric(jMI, iMI, xdistMid, ydistMid);
-                const double zdistMid = hcMid * S.depth(kMI);
-                dxleap = direction * um * stepDt * xdist / xdistMid;
-                dyleap = direction * vm * stepDt * ydist / ydistMid;
-                dzleap = direction * (P.sv + wm) * stepDt * zdist / zdistMid;
+                const double zdistMid = hcMid * S.depth(kS);
+                dxleap = direction * us * stepDt * xdist / xdistMid;
+                dyleap = direction * vs * stepDt * ydist / ydistMid;
+                dzleap = direction * (P.sv + ws) * stepDt * zdist / zdistMid;
             } else {
-                float ue, ve, we;
-                velocity(S, kI, kF, jI, jF, iI, iF, alpha, ue, ve, we);
-                dxleap = direction * ue * stepDt;
-                dyleap = direction * ve * stepDt;
-                dzleap = direction * (P.sv + we) * stepDt;
+                dxleap = direction * us * stepDt;
+                dyleap = direction * vs * stepDt;
+                dzleap = direction * (P.sv + ws) * stepDt;
             }
             xleap = dxleap + rxleap;
             yleap = dyleap + ryleap;
