@@ -90,3 +90,33 @@ def test_synthetic_full_window_properties():
     speed = np.sqrt(u[-1] ** 2 + v[-1] ** 2)[~land]
     assert 0.15 < np.sqrt((speed ** 2).mean()) < 0.45
     assert (w[0] == 0).all()                      # no flow through the bottom
+
+
+def test_reconstruction_inputs():
+    """Network input of the learned reconstruction: histogram frames then the
+    extra planes as channels; rotations and flips act on every channel alike."""
+    from wacommplusplus_amd.reconstruct import augment, extra_planes, to_net
+    rng = np.random.default_rng(0)
+    n, h, w = 3, 16, 16
+    data = dict(all=rng.integers(0, 50, (n, h, w)).astype(np.uint32), kh=np.array([0.1, 1.0, 10.0]),
+                mask=(rng.random((n, h, w)) < 0.7).astype(np.uint8))
+    few = rng.integers(0, 5, (n, h, w)).astype(np.uint32)
+    ex = extra_planes(data, ("kh", "mask"))
+    assert ex.shape == (n, 2, h, w) and ex.dtype == np.float32
+    assert np.allclose(ex[:, 0, 3, 5], [-1.0, 0.0, 1.0])
+    assert np.array_equal(ex[:, 1], data["mask"])
+    x, t = to_net(few, data["all"], 20, 16.0, ex)
+    assert x.shape == (n, 3, h, w) and t.shape == (n, h, w)
+    assert np.allclose(x[:, 0], np.sqrt(20.0 * few) / 16.0)
+    assert extra_planes(data, ()) is None
+    x1, _ = to_net(few, None, 20, 16.0)
+    assert x1.shape == (n, h, w)
+    for seed in range(8):
+        xa, ta = augment(x, t, np.random.default_rng(seed))
+        # one rotation/flip, the same for every channel and the target
+        transforms = [lambda a, q=q, f=f: (np.rot90(a, q, axes=(-2, -1))[..., ::-1] if f
+                                          else np.rot90(a, q, axes=(-2, -1))) for q in range(4) for f in (0, 1)]
+        hits = [T for T in transforms if np.array_equal(T(x), xa) and np.array_equal(T(t), ta)]
+        assert hits, seed
+    with pytest.raises(ValueError):
+        extra_planes(data, ("wind",))

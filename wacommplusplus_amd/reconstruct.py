@@ -14,6 +14,10 @@ Network input and target are variance-stabilised: x = sqrt(sub_every * few)
 roughly constant variance); the reconstruction is (max(y, 0) * scale)^2.
 Plain gridding -- the fallback -- is sub_every * few.
 
+Extra input planes (`extras`) carry what is known at reconstruction time
+besides the histogram: "kh", the run's horizontal diffusivity as a constant
+plane log10(Kh); "mask", the crop's ocean mask (1 wet, 0 land).
+
 Errors are relative L2 norms against the N-particle histogram.  That target
 has its own sampling noise, estimated from the Poisson variance of a
 histogram (E|h - lambda|^2 = sum lambda ~ sum h): e_target = sqrt(sum h) /
@@ -57,7 +61,8 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
     grid and records (wc_set_grid with n_slots >= hours + 1).  Returns
     dict(few: uint32 [n_pairs][crop][crop], or [n_pairs][frames][crop][crop]
     with frame 0 at `hours`, frame f at hours - f; all: uint32
-    [n_pairs][crop][crop]; kh, j, i, origin [n_pairs][2], seconds,
+    [n_pairs][crop][crop]; mask: uint8 ocean mask of each crop; kh, j, i,
+    origin [n_pairs][2], seconds,
     particle_steps)."""
     rng = np.random.default_rng(seed)
     js, is_ = ocean_cells(W, n_pairs, crop // 2 + 2, rng)
@@ -69,6 +74,7 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
     slots = list(range(hours + 1))
     few = np.zeros((n_pairs, frames, crop, crop), np.uint32)
     allc = np.zeros((n_pairs, crop, crop), np.uint32)
+    wet = np.zeros((n_pairs, crop, crop), np.uint8)
     origin = np.zeros((n_pairs, 2), np.int64)
     jj, ii = np.arange(W.eta, dtype=np.float64), np.arange(W.xi, dtype=np.float64)
     t0 = time.perf_counter()
@@ -102,21 +108,46 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
             r0 = int(np.clip(round(cj) - crop // 2, 0, W.eta - crop))
             c0 = int(np.clip(round(ci) - crop // 2, 0, W.xi - crop))
             origin[b0 + q] = (r0, c0)
+            wet[b0 + q] = W.mask[r0:r0 + crop, c0:c0 + crop] > 0
             f = surf[2 * q, r0:r0 + crop, c0:c0 + crop]
             for fr in range(frames):
                 few[b0 + q, fr] = kept[fr][2 * q, r0:r0 + crop, c0:c0 + crop]
             allc[b0 + q] = f + surf[2 * q + 1, r0:r0 + crop, c0:c0 + crop]
         if log:
             log(f"  pairs {b0 + nb}/{n_pairs}: {steps:.3g} particle-steps so far, last batch {ms:.0f} ms")
-    return dict(few=few[:, 0] if frames == 1 else few, all=allc, kh=kh, j=js, i=is_, origin=origin,
+    return dict(few=few[:, 0] if frames == 1 else few, all=allc, mask=wet, kh=kh, j=js, i=is_, origin=origin,
                 seconds=time.perf_counter() - t0, particle_steps=steps, n_target=n_target, sub_every=sub_every,
                 hours=hours, frames=frames)
 
 
-def to_net(few, allc, sub_every, scale):
-    x = np.sqrt(sub_every * few.astype(np.float64)) / scale
+EXTRAS = ("kh", "mask")
+
+
+def extra_planes(data, extras):
+    """The extra input planes of `data` ([n][len(extras)][h][w] float32), or
+    None without extras."""
+    if not extras:
+        return None
+    n, h, w = data["all"].shape
+    planes = []
+    for e in extras:
+        if e == "kh":
+            planes.append(np.broadcast_to(np.log10(data["kh"]).astype(np.float32)[:, None, None], (n, h, w)))
+        elif e == "mask":
+            planes.append(data["mask"].astype(np.float32))
+        else:
+            raise ValueError(f"unknown extra input {e!r} (known: {', '.join(EXTRAS)})")
+    return np.ascontiguousarray(np.stack(planes, 1))
+
+
+def to_net(few, allc, sub_every, scale, extra=None):
+    """Network input (histogram frames, then the extra planes, as channels)
+    and target, variance-stabilised."""
+    x = (np.sqrt(sub_every * few.astype(np.float64)) / scale).astype(np.float32)
+    if extra is not None:
+        x = np.concatenate([x if x.ndim == 4 else x[:, None], extra], axis=1)
     t = None if allc is None else np.sqrt(allc.astype(np.float64)) / scale
-    return x.astype(np.float32), (None if t is None else t.astype(np.float32))
+    return x, (None if t is None else t.astype(np.float32))
 
 
 def augment(x, t, rng):
@@ -153,10 +184,11 @@ def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, lr_min=1e-4, seed=1, lo
     return dict(loss=hist, seconds=time.perf_counter() - t0, steps=step)
 
 
-def reconstruct(net: UNet, few, sub_every, scale, batch=16):
+def reconstruct(net: UNet, few, sub_every, scale, batch=16, extra=None):
     """Concentration estimate (counts of the full release) from few-particle
-    histograms, and the device ms of the forward passes."""
-    x, _ = to_net(few, None, sub_every, scale)
+    histograms (and the extra planes the network was trained with), and the
+    device ms of the forward passes."""
+    x, _ = to_net(few, None, sub_every, scale, extra)
     out = np.zeros((len(x),) + x.shape[-2:], np.float32)
     ms = 0.0
     for b in range(0, len(x), batch):
@@ -173,7 +205,7 @@ def rel_l2(est, ref):
     return np.linalg.norm(d, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-30)
 
 
-def evaluate(net: UNet, data, scale):
+def evaluate(net: UNet, data, scale, extras=()):
     """Held-out errors of plain gridding and of the network against the
     N-particle histograms, the targets' own noise, and the estimated errors
     against the noise-free field.  Crops whose target is empty (every
@@ -181,8 +213,9 @@ def evaluate(net: UNet, data, scale):
     are left out (their number is reported)."""
     keep = data["all"].reshape(len(data["all"]), -1).sum(1) > 0
     few, allc, sub = data["few"][keep], data["all"][keep].astype(np.float64), data["sub_every"]
+    extra = extra_planes(data, extras)
     grid = sub * (few[:, 0] if few.ndim == 4 else few).astype(np.float64)      # the last hour's histogram
-    rec, ms = reconstruct(net, few, sub, scale)
+    rec, ms = reconstruct(net, few, sub, scale, extra=None if extra is None else extra[keep])
     e_grid, e_net = rel_l2(grid, allc), rel_l2(rec, allc)
     e_t = np.sqrt(allc.reshape(len(allc), -1).sum(1)) / np.linalg.norm(allc.reshape(len(allc), -1), axis=1)
     truth = lambda e: np.sqrt(np.maximum(e ** 2 - e_t ** 2, 0.0))   # noqa: E731
@@ -204,18 +237,24 @@ def train_flops_per_image(h: int, w: int) -> float:
 
 def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, epochs: int = 40,
         n_target: int = 1_000_000, sub_every: int = 20, hours: int = 24, crop: int = 128, batch: int = 16,
-        scale: float = 16.0, seed: int = 21, frames: int = 1, log=None):
+        scale: float = 16.0, seed: int = 21, frames: int = 1, extras=("kh", "mask"), log=None, pairs=None):
     """The whole learned-reconstruction experiment: pairs, training, held-out
     errors, and the comparison with plain gridding at equal GPU time (the
     move of n = n_target / sub_every particles per scenario, batched 256 at a
     time, plus one forward pass per image).  frames > 1: the network also
-    sees the few-particle histograms of the frames - 1 hours before."""
-    tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
-    te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, crop, seed + 1, frames=frames, log=log)
-    X, T = to_net(tr["few"], tr["all"], sub_every, scale)
-    net = UNet(ctx_device(ctx), seed=3, channels=frames)
+    sees the few-particle histograms of the frames - 1 hours before; extras:
+    the extra input planes (EXTRAS).  pairs: (train, test) data from an
+    earlier call with the same sizes, reused instead of generated."""
+    if pairs is None:
+        tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
+        te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, crop, seed + 1, frames=frames, log=log)
+    else:
+        tr, te = pairs
+    X, T = to_net(tr["few"], tr["all"], sub_every, scale, extra_planes(tr, extras))
+    net = UNet(ctx_device(ctx), seed=3, channels=frames + len(extras))
     h = train(net, X, T, epochs=epochs, batch=batch, log=log)
-    ev = evaluate(net, te, scale)
+    ev = evaluate(net, te, scale, extras)
+    net.close()
     # GPU time of the few-particle move: 256 scenarios of n particles, batched
     n = n_target // sub_every
     srcs = [EnsSource(k=-0.27, j=float(tr["j"][q % n_train]), i=float(tr["i"][q % n_train]), spread=0.25,
@@ -232,6 +271,7 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     flops = train_flops_per_image(crop, crop) * h["steps"] * batch
     return dict(
         pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop, frames=frames,
+                   extras=list(extras),
                    seconds=tr["seconds"] + te["seconds"], particle_steps=int(tr["particle_steps"] + te["particle_steps"])),
         training=dict(epochs=epochs, steps=h["steps"], seconds=h["seconds"], loss_first=h["loss"][0],
                       loss_last=h["loss"][-1], tflops=flops / h["seconds"] / 1e12),
@@ -280,6 +320,9 @@ if __name__ == "__main__":
     ap.add_argument("--target", type=int, default=1_000_000)
     ap.add_argument("--sub-every", type=int, default=20)
     ap.add_argument("--frames", type=int, default=1, help="hourly few-particle frames the network sees")
+    ap.add_argument("--extras", default="kh,mask", metavar="SETS",
+                    help="extra input planes, comma-separated (kh, mask); several sets separated by ';' "
+                         "train one network each on the same pairs ('none' = no extras)")
     ap.add_argument("--time-training", type=int, default=0, metavar="STEPS",
                     help="only time STEPS training steps on a random batch of 16 128x128 images")
     a = ap.parse_args()
@@ -292,5 +335,16 @@ if __name__ == "__main__":
     c.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=W.T, store="bf16")
     for t in range(W.T):
         c.upload_record(t, *W.record(t))
-    print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames, log=say)),
-          flush=True)
+    sets = [tuple(e for e in grp.split(",") if e and e != "none") for grp in (a.extras or "none").split(";")]
+    if len(sets) == 1:
+        print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames,
+                             extras=sets[0], log=say)), flush=True)
+    else:
+        pairs = (generate_pairs(c, W, a.train, a.target, a.sub_every, seed=21, frames=a.frames, log=say),
+                 generate_pairs(c, W, a.test, a.target, a.sub_every, seed=22, frames=a.frames, log=say))
+        out = {}
+        for st in sets:
+            say(f"== extras: {','.join(st) or 'none'}")
+            out[",".join(st) or "none"] = run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every,
+                                               frames=a.frames, extras=st, log=say, pairs=pairs)
+        print(json.dumps({"sets": out}), flush=True)
