@@ -523,6 +523,12 @@ int wc_unet_gradients(wc_unet *net, const float *x, const float *t, int n, int h
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at learning rate lr on
  * that loss; *loss receives the loss before the step. */
 int wc_unet_train_step(wc_unet *net, const float *x, const float *t, int n, int h, int w, float lr, float *loss);
+/* The same two calls for the per-pixel weighted loss mean weight (network(x)
+ * - t)^2, weight [n][h][w] (NULL: all ones, the calls above). */
+int wc_unet_gradients_weighted(wc_unet *net, const float *x, const float *t, const float *weight, int n, int h,
+                               int w, float *grad, float *loss);
+int wc_unet_train_step_weighted(wc_unet *net, const float *x, const float *t, const float *weight, int n, int h,
+                                int w, float lr, float *loss);
 
 /* ---------------------------------------------------------------------------
  * 5. Output formats around the path

@@ -182,9 +182,11 @@ class UNetOracle:
         self.act = a
         return a["out"]
 
-    def loss(self, x, t):
+    def loss(self, x, t, w=None):
+        """mean over pixels of w (out - t)^2 (w = 1 when not given)."""
         out = self.forward(x)
-        return float(np.mean((out - np.asarray(t, dtype=np.float64)) ** 2))
+        w = 1.0 if w is None else np.asarray(w, dtype=np.float64)
+        return float(np.mean(w * (out - np.asarray(t, dtype=np.float64)) ** 2))
 
     def _conv_bwd(self, l, gy, y, x1, x2=None, need_dx=True):
         W, _ = self.layers[l]
@@ -199,14 +201,15 @@ class UNetOracle:
         c1 = x1.shape[3]
         return dW, db, gx[..., :c1], (gx[..., c1:] if x2 is not None else None)
 
-    def gradients(self, x, t):
-        """(loss, flat gradient in the parameter order)."""
+    def gradients(self, x, t, w=None):
+        """(loss, flat gradient in the parameter order) of the loss above."""
         t = np.asarray(t, dtype=np.float64)
+        w = 1.0 if w is None else np.asarray(w, dtype=np.float64)
         out = self.forward(x)
         a = self.act
         P = out.size
-        loss = float(np.mean((out - t) ** 2))
-        g = 2.0 * (out - t) / P
+        loss = float(np.mean(w * (out - t) ** 2))
+        g = 2.0 * w * (out - t) / P
         dwh = np.einsum("nhw,nhwc->c", g, a["d1"])
         dbh = g.sum()
         grads = [None] * len(SPEC)

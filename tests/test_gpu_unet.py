@@ -71,6 +71,27 @@ def test_forward_and_gradients_match_oracle(net, shape):
     print(f"{shape}: gradient rel err {whole:.2e}, worst layer {worst:.2e}, head {eh:.2e}", flush=True)
 
 
+def test_weighted_loss_matches_oracle(net):
+    """Per-pixel weighted loss mean w (y - t)^2 against the oracle, to the bars
+    above; unit weights give the unweighted loss and gradient."""
+    rng = np.random.default_rng(29)
+    shape = (2, 16, 16)
+    p = uo.he_params(seed=4).astype(np.float32)
+    net.set_params(p)
+    x = rng.uniform(0.0, 3.0, shape).astype(np.float32)
+    t = (x + rng.normal(0.0, 0.3, shape)).astype(np.float32)
+    w = rng.uniform(0.0, 4.0, shape).astype(np.float32)
+    ora = uo.UNetOracle(p.astype(np.float64), emulate_bf16=True)
+    loss, g = net.gradients(x, t, w)
+    loss_ref, g_ref = ora.gradients(x, t, w.astype(np.float64))
+    assert abs(loss - loss_ref) / loss_ref < 1e-3, (loss, loss_ref)
+    assert rel(g, g_ref) < 3e-2, rel(g, g_ref)
+    loss0, g0 = net.gradients(x, t)
+    loss1, g1 = net.gradients(x, t, np.ones(shape, np.float32))
+    assert abs(loss1 - loss0) <= 1e-6 * loss0 and rel(g1, g0) < 1e-6
+    assert rel(g, g0) > 1e-2                      # the weights do change the gradient
+
+
 def test_three_channel_input_matches_oracle():
     """A 3-channel network ([n][3][h][w] input) against the oracle, as above."""
     net3 = wc.UNet(0, seed=5, channels=3)

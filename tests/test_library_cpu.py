@@ -120,3 +120,25 @@ def test_reconstruction_inputs():
         assert hits, seed
     with pytest.raises(ValueError):
         extra_planes(data, ("wind",))
+
+
+def test_reconstruction_loss_weights():
+    """The count-aligned loss weights: positive, mean one, growing with the
+    (box-filtered) target, and moved by augmentation with the images."""
+    from wacommplusplus_amd.reconstruct import augment, loss_weights
+    rng = np.random.default_rng(3)
+    T = np.zeros((4, 16, 16), np.float32)
+    T[:, 6:10, 6:10] = rng.uniform(1.0, 2.0, (4, 4, 4))
+    w = loss_weights(T)
+    assert w.shape == T.shape and w.dtype == np.float32
+    assert np.all(w > 0) and abs(float(w.mean()) - 1.0) < 1e-5
+    assert np.all(w[:, 8, 8] > 10 * w[:, 0, 0])
+    x = rng.uniform(0, 1, (4, 2, 16, 16)).astype(np.float32)
+    xa, ta, wa = augment(x, T, np.random.default_rng(1), w)
+    for q in range(4):
+        for f in (0, 1):
+            R = lambda a: np.rot90(a, q, axes=(-2, -1))[..., ::-1] if f else np.rot90(a, q, axes=(-2, -1))  # noqa: E731
+            if np.array_equal(R(T), ta):
+                assert np.array_equal(R(w), wa) and np.array_equal(R(x), xa)
+                return
+    raise AssertionError("augmentation is not one rotation/flip")

@@ -14,13 +14,14 @@ import pytest
 import unet_oracle as uo
 
 
-def finite_difference_check(plant=None, n_params=40, seed=3):
+def finite_difference_check(plant=None, n_params=40, seed=3, weighted=False):
     rng = np.random.default_rng(seed)
     p = uo.he_params(seed)
     x = rng.uniform(0.0, 3.0, (1, 8, 8))
     t = x + rng.normal(0.0, 0.5, x.shape)
+    w = rng.uniform(0.0, 4.0, x.shape) if weighted else None
     net = uo.UNetOracle(p, emulate_bf16=False, plant=plant)
-    _, g = net.gradients(x, t)
+    _, g = net.gradients(x, t, w)
     # parameters from every layer (weights and biases) and the head
     idx, off = [], 0
     for co, c1, c2 in uo.SPEC:
@@ -37,7 +38,8 @@ def finite_difference_check(plant=None, n_params=40, seed=3):
         pp, pm = p.copy(), p.copy()
         pp[i] += eps
         pm[i] -= eps
-        fd = (uo.UNetOracle(pp, emulate_bf16=False).loss(x, t) - uo.UNetOracle(pm, emulate_bf16=False).loss(x, t)) / (2 * eps)
+        fd = (uo.UNetOracle(pp, emulate_bf16=False).loss(x, t, w)
+              - uo.UNetOracle(pm, emulate_bf16=False).loss(x, t, w)) / (2 * eps)
         scale = max(abs(fd), abs(g[i]), 1e-4 * np.max(np.abs(g)))
         worst = max(worst, abs(fd - g[i]) / scale)
     return worst
@@ -45,6 +47,20 @@ def finite_difference_check(plant=None, n_params=40, seed=3):
 
 def test_finite_differences_fp64():
     assert finite_difference_check() < 1e-6
+
+
+def test_finite_differences_weighted_loss():
+    # per-pixel weights w: loss mean(w (out - t)^2)
+    assert finite_difference_check(weighted=True, seed=5) < 1e-6
+    # unit weights are the plain loss
+    rng = np.random.default_rng(1)
+    p = uo.he_params(2)
+    x = rng.uniform(0.0, 3.0, (2, 8, 8))
+    t = x + rng.normal(0.0, 0.5, x.shape)
+    net = uo.UNetOracle(p, emulate_bf16=False)
+    l0, g0 = net.gradients(x, t)
+    l1, g1 = net.gradients(x, t, np.ones_like(x))
+    assert l0 == l1 and np.array_equal(g0, g1)
 
 
 def test_multichannel_input():

@@ -30,7 +30,7 @@ EXPORTS = (
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
     "wc_ens_get", "wc_ens_release", "wc_ens_set_sort", "wc_ens_get_sort", "wc_ens_set_lds", "wc_ens_get_lds", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
-    "wc_unet_create_ex",
+    "wc_unet_create_ex", "wc_unet_gradients_weighted", "wc_unet_train_step_weighted",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
     "wc_particle_records",
 )
@@ -187,6 +187,8 @@ def load_library(path: str = LIB_PATH):
         "wc_unet_forward": [vp, vp, i, i, i, vp, C.POINTER(d)],
         "wc_unet_gradients": [vp, vp, vp, i, i, i, vp, C.POINTER(C.c_float)],
         "wc_unet_train_step": [vp, vp, vp, i, i, i, C.c_float, C.POINTER(C.c_float)],
+        "wc_unet_gradients_weighted": [vp, vp, vp, vp, i, i, i, vp, C.POINTER(C.c_float)],
+        "wc_unet_train_step_weighted": [vp, vp, vp, vp, i, i, i, C.c_float, C.POINTER(C.c_float)],
     }
     for name, args in unet.items():
         f = getattr(L, name)
@@ -593,21 +595,33 @@ class UNet:
         self._chk(self.L.wc_unet_forward(self.h, _p(xx), n, hh, ww, _p(y), C.byref(ms)))
         return y, ms.value
 
-    def gradients(self, x, t):
+    def _weight(self, weight, shape):
+        if weight is None:
+            return None
+        w = _f32(weight)
+        assert w.shape == shape
+        return w
+
+    def gradients(self, x, t, weight=None):
+        """(loss, gradient) of mean weight (network(x) - t)^2 (weight None: 1)."""
         xx, tt = _f32(x), _f32(t)
         n, hh, ww = self._shape(xx)
         assert tt.shape == (n, hh, ww)
+        w = self._weight(weight, tt.shape)
         g = np.zeros(self.n_params, np.float32)
         loss = C.c_float()
-        self._chk(self.L.wc_unet_gradients(self.h, _p(xx), _p(tt), n, hh, ww, _p(g), C.byref(loss)))
+        self._chk(self.L.wc_unet_gradients_weighted(self.h, _p(xx), _p(tt), None if w is None else _p(w), n, hh, ww,
+                                                    _p(g), C.byref(loss)))
         return loss.value, g
 
-    def train_step(self, x, t, lr):
+    def train_step(self, x, t, lr, weight=None):
         xx, tt = _f32(x), _f32(t)
         n, hh, ww = self._shape(xx)
         assert tt.shape == (n, hh, ww)
+        w = self._weight(weight, tt.shape)
         loss = C.c_float()
-        self._chk(self.L.wc_unet_train_step(self.h, _p(xx), _p(tt), n, hh, ww, float(lr), C.byref(loss)))
+        self._chk(self.L.wc_unet_train_step_weighted(self.h, _p(xx), _p(tt), None if w is None else _p(w), n, hh, ww,
+                                                     float(lr), C.byref(loss)))
         return loss.value
 
 

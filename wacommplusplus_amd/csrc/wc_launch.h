@@ -178,7 +178,8 @@ hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c
                              hipStream_t s);
 hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, long long P, float *out,
                            hipStream_t s);
-hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wh, long long P,
+hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wgt, const float *wh,
+                           long long P,
                            float *dd, float *dwh, float *loss, hipStream_t s);
 hipError_t launch_flip(const uint16_t *w, int cout, int cin, uint16_t *wf, hipStream_t s);
 hipError_t launch_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,

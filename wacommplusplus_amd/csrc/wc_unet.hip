@@ -616,12 +616,13 @@ __global__ void head_fwd_kernel(const uint16_t *__restrict__ d, const float *__r
     }
 }
 
-// loss += mean (out - t)^2; g = d loss / d out; head gradients dwh;
-// dd[p][c] = g[p] wh[c]
+// loss += mean wgt (out - t)^2 (wgt = 1 when null); g = d loss / d out;
+// head gradients dwh; dd[p][c] = g[p] wh[c]
 __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *__restrict__ d, const float *__restrict__ out,
-                                                       const float *__restrict__ t, const float *__restrict__ wh,
-                                                       long long P, float *__restrict__ dd,
-                                                       float *__restrict__ dwh, float *__restrict__ loss) {
+                                                       const float *__restrict__ t, const float *__restrict__ wgt,
+                                                       const float *__restrict__ wh, long long P,
+                                                       float *__restrict__ dd, float *__restrict__ dwh,
+                                                       float *__restrict__ loss) {
     // four work-items per pixel, 8 of its kHead channels each (16-byte loads)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cg = tid & 3;
     const float inv = 1.0f / (float)P;
@@ -631,10 +632,11 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *__restric
     for (int q = 0; q < 8; ++q) w8[q] = wh[8 * cg + q];
     for (long long p = (blockIdx.x * 256LL + tid) >> 2; p < P; p += (long long)gridDim.x * 64) {
         const float e = out[p] - t[p];
-        const float g = 2.0f * e * inv;
+        const float wv = wgt ? wgt[p] : 1.0f;
+        const float g = 2.0f * e * wv * inv;
         if (cg == 0) {
             v[8] += g;
-            v[9] += e * e;
+            v[9] += wv * e * e;
         }
         Pack8 dv;
         dv.u = *(const uint4 *)(d + p * kHead + 8 * cg);
@@ -806,11 +808,12 @@ hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, l
     head_fwd_kernel<<<blocks_for(P), 256, 0, s>>>(d, x, wh, P, out);
     return hipGetLastError();
 }
-hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wh, long long P,
+hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wgt, const float *wh,
+                           long long P,
                            float *dd, float *dwh, float *loss, hipStream_t s) {
     int b = blocks_for(P);
     if (b > 1024) b = 1024;
-    head_bwd_kernel<<<b, 256, 0, s>>>(d, out, t, wh, P, dd, dwh, loss);
+    head_bwd_kernel<<<b, 256, 0, s>>>(d, out, t, wgt, wh, P, dd, dwh, loss);
     return hipGetLastError();
 }
 hipError_t launch_flip(const uint16_t *w, int cout, int cin, uint16_t *wf, hipStream_t s) {

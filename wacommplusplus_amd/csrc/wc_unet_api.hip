@@ -60,6 +60,8 @@ struct wc_unet {
              *u1 = nullptr, *d1a = nullptr, *d1 = nullptr;
     uint16_t *gmask = nullptr;      // a layer's ReLU-masked output gradient in BF16 (backward scratch)
     float *x = nullptr, *out = nullptr, *tgt = nullptr;
+    float *wgt = nullptr;     // per-pixel loss weights of the current batch
+    bool useWgt = false;
     int cin = 1;                // input channels (x holds channel 0, xc all of them when cin > 1)
     float *xc = nullptr;
     float *g_d1 = nullptr, *g_d1a = nullptr, *g_s1 = nullptr, *g_u1 = nullptr, *g_d2 = nullptr, *g_d2a = nullptr,
@@ -122,7 +124,7 @@ int ensure_shape(wc_unet *u, int n, int h, int w) {
     rc |= alloc(u, u->d3a, P2 * 4 * C) | alloc(u, u->d3, P2 * 4 * C) | alloc(u, u->u2, P1 * 4 * C);
     rc |= alloc(u, u->d2a, P1 * 2 * C) | alloc(u, u->d2, P1 * 2 * C) | alloc(u, u->u1, P0 * 2 * C);
     rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C) | alloc(u, u->gmask, P0 * C);
-    rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0);
+    rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0) | alloc(u, u->wgt, P0);
     if (u->cin > 1) rc |= alloc(u, u->xc, P0 * u->cin);
     rc |= alloc(u, u->g_d1, P0 * C) | alloc(u, u->g_d1a, P0 * C) | alloc(u, u->g_s1, P0 * C) |
           alloc(u, u->g_u1, P0 * 2 * C);
@@ -225,7 +227,8 @@ int backward(wc_unet *u) {
     int rc;
     UCHK(u, launch_fill_f32(u->d_g, (long long)u->nparam, 0.f, u->stream));
     UCHK(u, launch_fill_f32(u->d_loss, 1, 0.f, u->stream));
-    UCHK(u, launch_head_bwd(u->d1, u->out, u->tgt, u->d_p + u->head_off, (long long)n * H * W, u->g_d1,
+    UCHK(u, launch_head_bwd(u->d1, u->out, u->tgt, u->useWgt ? u->wgt : nullptr, u->d_p + u->head_off,
+                            (long long)n * H * W, u->g_d1,
                             u->d_g + u->head_off, u->d_loss, u->stream));
     // decoder, level 0
     if ((rc = conv_bwd(u, Ls[13], u->g_d1, u->d1, u->d1a, nullptr, u->g_d1a, false, nullptr, false, H, W))) return rc;
@@ -265,12 +268,14 @@ int backward(wc_unet *u) {
     return WC_OK;
 }
 
-int upload(wc_unet *u, const float *x, const float *t, int n, int h, int w) {
+int upload(wc_unet *u, const float *x, const float *t, const float *wgt, int n, int h, int w) {
     int rc = ensure_shape(u, n, h, w);
     if (rc) return rc;
     const size_t P = (size_t)n * h * w;
     UCHK(u, hipMemcpyAsync(u->cin > 1 ? u->xc : u->x, x, P * u->cin * 4, hipMemcpyHostToDevice, u->stream));
     if (t) UCHK(u, hipMemcpyAsync(u->tgt, t, P * 4, hipMemcpyHostToDevice, u->stream));
+    u->useWgt = wgt != nullptr;
+    if (wgt) UCHK(u, hipMemcpyAsync(u->wgt, wgt, P * 4, hipMemcpyHostToDevice, u->stream));
     return WC_OK;
 }
 
@@ -391,7 +396,7 @@ int wc_unet_set_params(wc_unet *u, const float *p) {
 int wc_unet_forward(wc_unet *u, const float *x, int n, int h, int w, float *y, double *device_ms) {
     if (!u || !x || !y) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
-    int rc = upload(u, x, nullptr, n, h, w);
+    int rc = upload(u, x, nullptr, nullptr, n, h, w);
     if (rc) return rc;
     hipEvent_t e0, e1;
     UCHK(u, hipEventCreate(&e0));
@@ -413,9 +418,14 @@ int wc_unet_forward(wc_unet *u, const float *x, int n, int h, int w, float *y, d
 }
 
 int wc_unet_gradients(wc_unet *u, const float *x, const float *t, int n, int h, int w, float *grad, float *loss) {
+    return wc_unet_gradients_weighted(u, x, t, nullptr, n, h, w, grad, loss);
+}
+
+int wc_unet_gradients_weighted(wc_unet *u, const float *x, const float *t, const float *weight, int n, int h, int w,
+                               float *grad, float *loss) {
     if (!u || !x || !t) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
-    int rc = upload(u, x, t, n, h, w);
+    int rc = upload(u, x, t, weight, n, h, w);
     if (!rc) rc = forward(u);
     if (!rc) rc = backward(u);
     if (rc) return rc;
@@ -428,9 +438,14 @@ int wc_unet_gradients(wc_unet *u, const float *x, const float *t, int n, int h, 
 }
 
 int wc_unet_train_step(wc_unet *u, const float *x, const float *t, int n, int h, int w, float lr, float *loss) {
+    return wc_unet_train_step_weighted(u, x, t, nullptr, n, h, w, lr, loss);
+}
+
+int wc_unet_train_step_weighted(wc_unet *u, const float *x, const float *t, const float *weight, int n, int h, int w,
+                                float lr, float *loss) {
     if (!u || !x || !t || !(lr > 0)) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
-    int rc = upload(u, x, t, n, h, w);
+    int rc = upload(u, x, t, weight, n, h, w);
     if (!rc) rc = forward(u);
     if (!rc) rc = backward(u);
     if (rc) return rc;

@@ -14,6 +14,14 @@ Network input and target are variance-stabilised: x = sqrt(sub_every * few)
 roughly constant variance); the reconstruction is (max(y, 0) * scale)^2.
 Plain gridding -- the fallback -- is sub_every * few.
 
+Loss.  The plain loss is the mean squared error in those units.  The error
+that matters is the relative L2 norm in counts; to first order a unit of
+error at a pixel of target t costs 4 s^4 t^2 there, over |c|^2 = s^4 sum t^4
+for the image.  loss_weight="count" trains with per-pixel weights
+t_b^2 / mean(t_b^4) (t_b: the target box-filtered over 3 x 3, so that its
+noise does not steer the weights), plus a floor of 10 % of the image's mean,
+scaled to a mean of one over the training set.
+
 Extra input planes (`extras`) carry what is known at reconstruction time
 besides the histogram: "kh", the run's horizontal diffusivity as a constant
 plane log10(Kh); "mask", the crop's ocean mask (1 wet, 0 land).
@@ -150,18 +158,36 @@ def to_net(few, allc, sub_every, scale, extra=None):
     return x, (None if t is None else t.astype(np.float32))
 
 
-def augment(x, t, rng):
+def loss_weights(T):
+    """Per-pixel loss weights of the "count" loss for targets T [n][h][w]
+    (variance-stabilised units)."""
+    Tp = np.pad(T.astype(np.float64), ((0, 0), (1, 1), (1, 1)), mode="edge")
+    h, w = T.shape[1:]
+    tb = sum(Tp[:, a:a + h, b:b + w] for a in range(3) for b in range(3)) / 9.0
+    t2 = tb * tb
+    m2 = t2.mean(axis=(1, 2), keepdims=True)
+    m4 = np.maximum((t2 * t2).mean(axis=(1, 2), keepdims=True), 1e-30)
+    wgt = (t2 + 0.1 * m2) / m4
+    return (wgt / wgt.mean()).astype(np.float32)
+
+
+def augment(x, t, rng, w=None):
     """The same random rotation and flip of the image axes of x ([n][h][w] or
-    [n][frames][h][w]) and t."""
+    [n][frames][h][w]), t (and the loss weights w)."""
     k = int(rng.integers(4))
-    x, t = np.rot90(x, k, axes=(-2, -1)), np.rot90(t, k, axes=(-2, -1))
-    if rng.integers(2):
-        x, t = x[..., ::-1], t[..., ::-1]
-    return np.ascontiguousarray(x), np.ascontiguousarray(t)
+    flip = bool(rng.integers(2))
+
+    def tr(a):
+        if a is None:
+            return None
+        a = np.rot90(a, k, axes=(-2, -1))
+        return np.ascontiguousarray(a[..., ::-1] if flip else a)
+    return (tr(x), tr(t)) if w is None else (tr(x), tr(t), tr(w))
 
 
-def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, lr_min=1e-4, seed=1, log=None):
-    """Adam with a cosine learning-rate decay; random rotations and flips."""
+def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, lr_min=1e-4, seed=1, log=None, weights=None):
+    """Adam with a cosine learning-rate decay; random rotations and flips;
+    per-pixel loss weights when given."""
     rng = np.random.default_rng(seed)
     n = len(X)
     steps_per_epoch = max(1, n // batch)
@@ -174,9 +200,13 @@ def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, lr_min=1e-4, seed=1, lo
         losses = []
         for s in range(steps_per_epoch):
             b = order[s * batch:(s + 1) * batch]
-            x, t = augment(X[b], T[b], rng)
             a = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / total))
-            losses.append(net.train_step(x, t, a))
+            if weights is None:
+                x, t = augment(X[b], T[b], rng)
+                losses.append(net.train_step(x, t, a))
+            else:
+                x, t, w = augment(X[b], T[b], rng, weights[b])
+                losses.append(net.train_step(x, t, a, weight=w))
             step += 1
         hist.append(float(np.mean(losses)))
         if log:
@@ -237,14 +267,16 @@ def train_flops_per_image(h: int, w: int) -> float:
 
 def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, epochs: int = 40,
         n_target: int = 1_000_000, sub_every: int = 20, hours: int = 24, crop: int = 128, batch: int = 16,
-        scale: float = 16.0, seed: int = 21, frames: int = 1, extras=("kh", "mask"), log=None, pairs=None):
+        scale: float = 16.0, seed: int = 21, frames: int = 1, extras=("kh", "mask"), loss_weight: str = "none",
+        log=None, pairs=None):
     """The whole learned-reconstruction experiment: pairs, training, held-out
     errors, and the comparison with plain gridding at equal GPU time (the
     move of n = n_target / sub_every particles per scenario, batched 256 at a
     time, plus one forward pass per image).  frames > 1: the network also
     sees the few-particle histograms of the frames - 1 hours before; extras:
-    the extra input planes (EXTRAS).  pairs: (train, test) data from an
-    earlier call with the same sizes, reused instead of generated."""
+    the extra input planes (EXTRAS); loss_weight: "none" or "count" (the
+    module docstring).  pairs: (train, test) data from an earlier call with
+    the same sizes, reused instead of generated."""
     if pairs is None:
         tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
         te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, crop, seed + 1, frames=frames, log=log)
@@ -252,7 +284,10 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
         tr, te = pairs
     X, T = to_net(tr["few"], tr["all"], sub_every, scale, extra_planes(tr, extras))
     net = UNet(ctx_device(ctx), seed=3, channels=frames + len(extras))
-    h = train(net, X, T, epochs=epochs, batch=batch, log=log)
+    if loss_weight not in ("none", "count"):
+        raise ValueError(f"loss_weight {loss_weight!r}: 'none' or 'count'")
+    h = train(net, X, T, epochs=epochs, batch=batch, log=log,
+              weights=loss_weights(T) if loss_weight == "count" else None)
     ev = evaluate(net, te, scale, extras)
     net.close()
     # GPU time of the few-particle move: 256 scenarios of n particles, batched
@@ -271,7 +306,7 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     flops = train_flops_per_image(crop, crop) * h["steps"] * batch
     return dict(
         pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop, frames=frames,
-                   extras=list(extras),
+                   extras=list(extras), loss_weight=loss_weight,
                    seconds=tr["seconds"] + te["seconds"], particle_steps=int(tr["particle_steps"] + te["particle_steps"])),
         training=dict(epochs=epochs, steps=h["steps"], seconds=h["seconds"], loss_first=h["loss"][0],
                       loss_last=h["loss"][-1], tflops=flops / h["seconds"] / 1e12),
@@ -320,6 +355,9 @@ if __name__ == "__main__":
     ap.add_argument("--target", type=int, default=1_000_000)
     ap.add_argument("--sub-every", type=int, default=20)
     ap.add_argument("--frames", type=int, default=1, help="hourly few-particle frames the network sees")
+    ap.add_argument("--loss-weight", default="none", metavar="SETS",
+                    help="loss weighting, none or count; several separated by ';' (with one extras set) train "
+                         "one network each on the same pairs")
     ap.add_argument("--extras", default="kh,mask", metavar="SETS",
                     help="extra input planes, comma-separated (kh, mask); several sets separated by ';' "
                          "train one network each on the same pairs ('none' = no extras)")
@@ -336,15 +374,18 @@ if __name__ == "__main__":
     for t in range(W.T):
         c.upload_record(t, *W.record(t))
     sets = [tuple(e for e in grp.split(",") if e and e != "none") for grp in (a.extras or "none").split(";")]
-    if len(sets) == 1:
+    weights = [x for x in a.loss_weight.split(";") if x]
+    combos = [(st, lw) for st in sets for lw in weights]
+    if len(combos) == 1:
         print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames,
-                             extras=sets[0], log=say)), flush=True)
+                             extras=combos[0][0], loss_weight=combos[0][1], log=say)), flush=True)
     else:
         pairs = (generate_pairs(c, W, a.train, a.target, a.sub_every, seed=21, frames=a.frames, log=say),
                  generate_pairs(c, W, a.test, a.target, a.sub_every, seed=22, frames=a.frames, log=say))
         out = {}
-        for st in sets:
-            say(f"== extras: {','.join(st) or 'none'}")
-            out[",".join(st) or "none"] = run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every,
-                                               frames=a.frames, extras=st, log=say, pairs=pairs)
+        for st, lw in combos:
+            name = f"{','.join(st) or 'none'}/{lw}"
+            say(f"== extras / loss weight: {name}")
+            out[name] = run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames, extras=st,
+                            loss_weight=lw, log=say, pairs=pairs)
         print(json.dumps({"sets": out}), flush=True)
