@@ -158,16 +158,22 @@ def to_net(few, allc, sub_every, scale, extra=None):
     return x, (None if t is None else t.astype(np.float32))
 
 
-def loss_weights(T):
-    """Per-pixel loss weights of the "count" loss for targets T [n][h][w]
-    (variance-stabilised units)."""
+def loss_weights(T, kind="count"):
+    """Per-pixel loss weights for targets T [n][h][w] (variance-stabilised
+    units): "count" (above), or "core", (t_b^2 + m2) / (2 m2) with m2 the
+    image's mean t_b^2 -- the plume core emphasised within each image,
+    every image weighted alike."""
     Tp = np.pad(T.astype(np.float64), ((0, 0), (1, 1), (1, 1)), mode="edge")
     h, w = T.shape[1:]
     tb = sum(Tp[:, a:a + h, b:b + w] for a in range(3) for b in range(3)) / 9.0
     t2 = tb * tb
     m2 = t2.mean(axis=(1, 2), keepdims=True)
-    m4 = np.maximum((t2 * t2).mean(axis=(1, 2), keepdims=True), 1e-30)
-    wgt = (t2 + 0.1 * m2) / m4
+    if kind == "core":
+        wgt = (t2 + m2) / np.maximum(2.0 * m2, 1e-30)
+        wgt = np.where(m2 > 0, wgt, 1.0)
+    else:
+        m4 = np.maximum((t2 * t2).mean(axis=(1, 2), keepdims=True), 1e-30)
+        wgt = (t2 + 0.1 * m2) / m4
     return (wgt / wgt.mean()).astype(np.float32)
 
 
@@ -274,8 +280,8 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     move of n = n_target / sub_every particles per scenario, batched 256 at a
     time, plus one forward pass per image).  frames > 1: the network also
     sees the few-particle histograms of the frames - 1 hours before; extras:
-    the extra input planes (EXTRAS); loss_weight: "none" or "count" (the
-    module docstring).  pairs: (train, test) data from an earlier call with
+    the extra input planes (EXTRAS); loss_weight: "none", "count" or "core"
+    (loss_weights).  pairs: (train, test) data from an earlier call with
     the same sizes, reused instead of generated."""
     if pairs is None:
         tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
@@ -284,10 +290,10 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
         tr, te = pairs
     X, T = to_net(tr["few"], tr["all"], sub_every, scale, extra_planes(tr, extras))
     net = UNet(ctx_device(ctx), seed=3, channels=frames + len(extras))
-    if loss_weight not in ("none", "count"):
-        raise ValueError(f"loss_weight {loss_weight!r}: 'none' or 'count'")
+    if loss_weight not in ("none", "count", "core"):
+        raise ValueError(f"loss_weight {loss_weight!r}: 'none', 'count' or 'core'")
     h = train(net, X, T, epochs=epochs, batch=batch, log=log,
-              weights=loss_weights(T) if loss_weight == "count" else None)
+              weights=None if loss_weight == "none" else loss_weights(T, loss_weight))
     ev = evaluate(net, te, scale, extras)
     net.close()
     # GPU time of the few-particle move: 256 scenarios of n particles, batched
@@ -356,7 +362,7 @@ if __name__ == "__main__":
     ap.add_argument("--sub-every", type=int, default=20)
     ap.add_argument("--frames", type=int, default=1, help="hourly few-particle frames the network sees")
     ap.add_argument("--loss-weight", default="none", metavar="SETS",
-                    help="loss weighting, none or count; several separated by ';' (with one extras set) train "
+                    help="loss weighting, none, count or core; several separated by ';' train "
                          "one network each on the same pairs")
     ap.add_argument("--extras", default="kh,mask", metavar="SETS",
                     help="extra input planes, comma-separated (kh, mask); several sets separated by ';' "
