@@ -216,8 +216,9 @@ def main():
                          "on other populations, which a kernel-trace average would mix in)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is described "
-                                   "in DESIGN.md and not run", "n_gpus": args.gpus}), flush=True)
+        print(json.dumps({"error": "this benchmark measures one GPU only: the particle ranges of several "
+                                   "GPUs move independently and only the count frames are summed, which "
+                                   "is not run here", "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     assert 1 <= args.steps <= 24, "the window holds 24 intervals"
     import wacommplusplus_amd as wc
