@@ -49,7 +49,7 @@ def algorithmic_bytes_per_step(store_bytes: int, static_bytes: int, midpoint: bo
     return n_dyn * store_bytes + n_stat * static_bytes
 
 
-STATE_BYTES = 96   # FP64 SoA: read id,k,j,i,health,age,time (56 B) + write k,j,i,health,age (40 B)
+STATE_BYTES = 100  # FP64 SoA: read id,k,j,i,health,age,time (56 B) and the drift tag (4 B) + write k,j,i,health,age (40 B)
 
 
 def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
@@ -186,7 +186,7 @@ def cpu_baseline(W, cfg, p0, n_sample=100_000, n_intervals=4):
     p = p0[::stride].copy()
     ocfg = cfg_to_oracle(cfg)
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    windows = [oracle_pair(W, t, t + 1) for t in range(n_intervals)]
+    windows = [oracle_pair(W, t, t + 1)[0] for t in range(n_intervals)]
     steps = 0
     t0 = time.perf_counter()
     for t in range(n_intervals):

@@ -9,11 +9,16 @@ import pyoracle as po
 
 
 def oracle_pair(W, t0, t1):
-    """Oracle view of records t0 -> t1 of a synthetic ForcingWindow."""
-    r0, r1 = W.record(t0), W.record(t1)
+    """Oracle view of the interval from record t0 to record t1 of a synthetic
+    ForcingWindow: the two records in chronological order, as the reference
+    holds them, and the time index of the interval start t0 (1 when tracking
+    backward: Particle::move goes from ocean_time(idx) to
+    ocean_time(idx + direction), Particle.cpp:110-115)."""
+    lo, hi = min(t0, t1), max(t0, t1)
+    r0, r1 = W.record(lo), W.record(hi)
     stack = [np.stack([r0[q], r1[q]]) for q in range(5)]
-    times = np.array([W.times[t0], W.times[t1]])
-    return po.Window(times, W.lonRad, W.latRad, W.h, W.depthIntervals, *stack)
+    times = np.array([W.times[lo], W.times[hi]])
+    return po.Window(times, W.lonRad, W.latRad, W.h, W.depthIntervals, *stack), (0 if t1 >= t0 else 1)
 
 
 def cfg_to_oracle(cfg):
@@ -39,8 +44,8 @@ def run_oracle(W, cfg, particles, n_intervals, first=None, nthreads=0, remove_de
     p = particles.copy()
     ocfg = cfg_to_oracle(cfg)
     for t, nxt in intervals(W, cfg.trackingDirection, n_intervals, first):
-        win = oracle_pair(W, t, nxt)
-        po.move(win, ocfg, 0, p, nthreads=nthreads)
+        win, idx = oracle_pair(W, t, nxt)
+        po.move(win, ocfg, idx, p, nthreads=nthreads)
         if remove_dead:
             p = p[p["health"] > 0]
     return p

@@ -105,7 +105,8 @@ def test_run_window_matches_oracle(small):
     land = W.mask != 1
     for t in range(W.T - 1):
         ref = oracle_emit(cfg, src, W.times[t], W.times[t + 1], ref)
-        po.move(oracle_pair(W, t, t + 1), ocfg, 0, ref)
+        win, idx = oracle_pair(W, t, t + 1)
+        po.move(win, ocfg, idx, ref)
         ref = np.ascontiguousarray(ref[ref["health"] > 0])
         frame = po.grid_count(ref, W.s_rho, W.eta, W.xi).astype(np.float32)
         frame[:, land] = np.float32(1e37)

@@ -233,6 +233,42 @@ def test_random_windows_all_closures(ctx, seed):
                 assert ok.mean() >= 0.97, (case, float(ok.mean()), np.where(~ok)[0][:5])
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_random_windows_backward_and_restart(ctx, seed):
+    """Backward tracking from the last record (Particle.cpp:81-125: reversed
+    interval, direction -1; diffusion only with backwardDiffusion) and a
+    restart checkpoint inside the first interval (restartElapsed,
+    NumericalHelpers.hpp:45-52), on the land-heavy random windows."""
+    rng = np.random.default_rng(200 + seed)
+    W = random_window(seed)
+    n = 400
+    p = wc.make_particles(n)
+    p["k"] = rng.uniform(-3.2, 0.2, n)
+    p["j"] = rng.uniform(-0.5, W.eta - 0.5, n)
+    p["i"] = rng.uniform(-0.5, W.xi - 0.5, n)
+    p["time"] = rng.choice([0.0, 900.0, 2000.0, 1e5], n)
+    p["id"] = rng.integers(0, 2**63, n, dtype=np.uint64)
+    cases = []
+    for integ in (1, 2):
+        for random, bwd in ((False, 0), (True, 0), (True, 1)):
+            cases.append(dict(integrator=integ, random=random, backwardDiffusion=bwd, trackingDirection=-1))
+        for random in (False, True):
+            cases.append(dict(integrator=integ, random=random, restartCheckpoint=float(W.times[0]) + 1000.0))
+    for kw in cases:
+        cfg = wc.default_config(sigma=30.0, survprob=1e-4, tau0=20000.0, randomSeed=seed, upperClosure=1,
+                                lowerClosure=2, horizontalClosure=3, **kw)
+        ref = run_oracle(W, cfg, p, 3)
+        dev = run_device(ctx, W, cfg, p, 3)
+        diffusing = kw["random"] and (kw.get("trackingDirection", 1) > 0 or kw.get("backwardDiffusion", 0))
+        if not diffusing:
+            print(f"{kw}: bit-identical check", flush=True)
+            assert_same(dev, ref, tol=0.0)
+            continue
+        ok = np.all(np.stack([close(dev[f], ref[f], 1e-9) for f in ("k", "j", "i")]), axis=0)
+        print(f"{kw}: {ok.mean():.4f} within 1e-9", flush=True)
+        assert ok.mean() >= 0.97, (kw, float(ok.mean()))
+
+
 # --------------------------------------------------------------------------
 # the plain CPU configuration: 10,000 particles, 24 h
 # --------------------------------------------------------------------------
