@@ -140,6 +140,36 @@ def test_fast_path_uniform_current_matches_parity(ctx):
     assert r["both_alive"] == 1.0 and r["max"] < 1e-4, r
 
 
+def test_fast_path_backward_matches_parity(ctx):
+    """Backward tracking (direction -1 from the last record) on the plain
+    configuration: the FP32 fast path within 1e-4 cells of the FP64 path after
+    12 h, and a forward run followed by the backward run returns every
+    particle to its release position: within 1e-6 cells on the FP64 path
+    (the oracle's own return error here is 6.5e-8: the row-dependent cell
+    width makes the midpoint return inexact at second order) and within
+    1e-5 cells on the FP32 path (measured 2.4e-7)."""
+    W = plain_window(T=13)
+    p = release(W, 2000, 1, depth_m=1.0)
+    p["j"], p["i"] = 59.5, 49.5
+    p["time"] = float(W.times[-1])
+    back = wc.default_config(random=False, trackingDirection=-1)
+    ref = run_device(ctx, W, back, p, 12, path="parity")
+    dev = run_device(ctx, W, back, p, 12, path="fast", store="f32")
+    r = deviation(dev, ref)
+    assert r["both_alive"] == 1.0 and r["max"] < 1e-4, r
+    assert np.all(ref["j"] < 59.5) and np.all(ref["i"] < 49.5)        # moved against the current
+    q = p.copy()
+    q["time"] = float(W.times[0])
+    for path, store, tol in (("parity", "f32", 1e-6), ("fast", "f32", 1e-5)):
+        fwd = run_device(ctx, W, wc.default_config(random=False), q, 12, path=path, store=store)
+        fwd["time"] = float(W.times[-1])
+        ret = run_device(ctx, W, back, fwd, 12, path=path, store=store)
+        for f in ("j", "i"):
+            err = float(np.max(np.abs(ret[f] - p[f])))
+            print(f"{path}: return error in {f} {err:.2e} cells", flush=True)
+            assert err < tol, (path, f, err)
+
+
 def test_fast_path_full_size_invariants(ctx, full):
     p = release(full, 4_000_000, 50)
     cfg = wc.default_config(random=True, sigma=wc.sigma_from_kh(1.0, 30.0))
