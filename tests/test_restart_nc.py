@@ -282,3 +282,87 @@ def test_writer_rejects_repeated_attribute(tmp_path):
     r = records(3)
     with pytest.raises(wc.WcError):
         wc.write_restart_nc(tmp_path / "x.nc", r, 0.0, 1, attributes={"random_seed": "1"})
+
+
+FUZZ = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import wacommplusplus_amd as wc
+src, n_cases, seed = sys.argv[2], int(sys.argv[3]), int(sys.argv[4])
+good = open(src, "rb").read()
+rng = np.random.default_rng(seed)
+cfg = wc.default_config(randomSeed=5489)
+path = src + ".fuzz"
+accepted = rejected = 0
+for case in range(n_cases):
+    b = bytearray(good)
+    kind = case % 4
+    if kind == 0:                                   # flip bytes in and just after the header
+        for _ in range(int(rng.integers(1, 5))):
+            b[int(rng.integers(4, min(len(b), 1400)))] ^= 1 << int(rng.integers(8))
+    elif kind == 1:                                 # truncate
+        del b[int(rng.integers(0, len(b))):]
+    elif kind == 2:                                 # a header word set to a huge or random value
+        at = int(rng.integers(1, min(len(b), 1400) // 4)) * 4
+        b[at:at + 4] = bytes([0xFF] * 4) if rng.integers(2) else rng.integers(0, 256, 4, dtype=np.uint8).tobytes()
+    else:                                           # an 8-byte field set to a huge or random value
+        at = int(rng.integers(1, min(len(b), 1400) // 8)) * 8
+        pick = int(rng.integers(3))
+        b[at:at + 8] = (bytes([0x7F] + [0xFF] * 7) if pick == 0 else bytes([0xFF] * 7 + [0xF8]) if pick == 1
+                        else rng.integers(0, 256, 8, dtype=np.uint8).tobytes())
+    open(path, "wb").write(bytes(b))
+    try:
+        p, _ = wc.read_restart_nc(path, cfg, "roms")
+        assert len(p) * 8 <= len(b), (case, len(p), len(b))
+        accepted += 1
+    except wc.WcError:
+        rejected += 1
+print("ok", accepted, rejected, flush=True)
+"""
+
+
+def test_reader_survives_corrupt_files(tmp_path):
+    """A restart file from disk is untrusted input: byte flips, truncations
+    and huge header values give a clean error or a bounded population, never
+    an out-of-bounds read.  Run in a child process, so that a crash fails the
+    test instead of the test session."""
+    import os
+    import subprocess
+    import sys
+    src = tmp_path / "good.nc"
+    wc.write_restart_nc(src, records(40, seed=9), 3600.0, 5489, ocean_model="roms", attributes=PROV)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", FUZZ, root, str(src), "2000", "7"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+    ok, accepted, rejected = r.stdout.split()
+    assert ok == "ok" and int(accepted) + int(rejected) == 2000
+    assert int(rejected) > 0
+
+
+def test_reader_under_address_sanitizer(tmp_path):
+    """The reader's source built with g++ under AddressSanitizer and
+    UndefinedBehaviorSanitizer (host code only), fed 4,000 corrupted restart
+    files (tests/cpp/restart_fuzz.cpp): no invalid access, and a population
+    reported only when it fits the file.  The reader before the overflow-safe
+    offsets reported 4,099,880,709 particles from a 6,708-byte file here."""
+    import os
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "restart_fuzz"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-I", os.path.join(root, "include"), "-x", "c++",
+                    os.path.join(root, "wacommplusplus_amd", "csrc", "wc_netcdf.hip"), "-x", "none",
+                    os.path.join(root, "tests", "cpp", "restart_fuzz.cpp"), "-o", str(exe)], check=True)
+    src = tmp_path / "good.nc"
+    wc.write_restart_nc(src, records(40, seed=9), 3600.0, 5489, ocean_model="roms", attributes=PROV)
+    env = dict(os.environ, ASAN_OPTIONS="verify_asan_link_order=0:detect_leaks=1")
+    r = subprocess.run([str(exe), str(src), "4000", "7", str(tmp_path / "case.nc")], capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout[-1000:] + r.stderr[-3000:]
+    ok, accepted, rejected = r.stdout.split()
+    assert ok == "ok" and int(accepted) + int(rejected) == 4000 and int(rejected) > 0

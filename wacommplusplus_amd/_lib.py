@@ -239,7 +239,7 @@ class Context:
 
     def _chk(self, rc):
         if rc:
-            raise WcError(rc, self.L.wc_last_error(self.h).decode())
+            raise WcError(rc, self.L.wc_last_error(self.h).decode(errors="replace"))
 
     def close(self):
         if self.h:
@@ -555,7 +555,7 @@ class UNet:
 
     def _chk(self, rc):
         if rc:
-            raise WcError(rc, self.L.wc_unet_last_error(self.h).decode())
+            raise WcError(rc, self.L.wc_unet_last_error(self.h).decode(errors="replace"))
 
     def _shape(self, xx):
         """(n, h, w) of an input batch, checked against the channel count."""
@@ -694,10 +694,10 @@ def read_restart_nc(path, config: ConfigData, ocean_model: str = "roms"):
     rc = L.wc_read_restart_nc(str(path).encode(), C.byref(config), ocean_model.encode(), None, 0, C.byref(n),
                               C.byref(cp), err, 512)
     if rc:
-        raise WcError(rc, err.value.decode())
+        raise WcError(rc, err.value.decode(errors="replace"))
     out = np.zeros(n.value, dtype=PARTICLE_DTYPE)
     rc = L.wc_read_restart_nc(str(path).encode(), C.byref(config), ocean_model.encode(),
                               _p(out).value if n.value else None, n.value, C.byref(n), C.byref(cp), err, 512)
     if rc:
-        raise WcError(rc, err.value.decode())
+        raise WcError(rc, err.value.decode(errors="replace"))
     return out, (None if np.isnan(cp.value) else cp.value)

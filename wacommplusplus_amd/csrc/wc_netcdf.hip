@@ -444,13 +444,19 @@ Num read_num(const uint8_t *p, int type) {
 }
 
 // Element offset of variable v at flat index q of its first dimension with
-// the other indices 0 (getVar({q,0,..},{1,1,..})).
+// the other indices 0 (getVar({q,0,..},{1,1,..})).  False when the element
+// does not lie inside the file; every product and sum of header values is
+// checked for overflow, so a corrupt header cannot wrap an offset back into
+// range.
 bool element_offset(const NcFile &f, const RVar &v, uint64_t q, uint64_t &off) {
-    const int sz = type_size(v.type);
-    uint64_t inner = 1;
-    for (size_t d = 1; d < v.dims.size(); ++d) inner *= f.dims[v.dims[d]].second;
-    off = v.record ? v.begin + q * f.recsize : v.begin + q * inner * sz;
-    return off + sz <= f.bytes.size();
+    const uint64_t sz = (uint64_t)type_size(v.type);
+    uint64_t inner = 1, stride = 0, rel = 0;
+    for (size_t d = 1; d < v.dims.size(); ++d)
+        if (__builtin_mul_overflow(inner, f.dims[v.dims[d]].second, &inner)) return false;
+    if (v.record) stride = f.recsize;
+    else if (__builtin_mul_overflow(inner, sz, &stride)) return false;
+    if (__builtin_mul_overflow(q, stride, &rel) || __builtin_add_overflow(v.begin, rel, &off)) return false;
+    return sz > 0 && off <= f.bytes.size() && sz <= f.bytes.size() - off;
 }
 
 uint64_t first_dim_size(const NcFile &f, const RVar &v) {
@@ -669,6 +675,16 @@ extern "C" int wc_read_restart_nc(const char *path, const wc_config_data *cfg, c
     if ((vo && (vo->type == NC_CHAR || first_dim_size(f, *vo) < n)) ||
         (vs && (vs->type == NC_CHAR || first_dim_size(f, *vs) < n)))
         return bad("Restart drift variables do not match id");
+    // the count comes from the header: every variable's last element must lie
+    // in the file before it is reported (offsets grow with the index)
+    if (n > 0) {
+        uint64_t off;
+        for (const RVar *x : {v[0], v[1], v[2], v[3], v[4], v[5], v[6], vo, vs})
+            if (x && !element_offset(f, *x, n - 1, off)) {
+                set_err(err, err_len, "truncated NetCDF data");
+                return WC_ERR_IO;
+            }
+    }
     *count = (size_t)n;
     if (!out) return WC_OK;
     if (capacity < n) return bad("Restart holds " + std::to_string(n) + " particles, capacity " +
