@@ -282,9 +282,7 @@ int wc_compare_records(wc_ctx *ctx, int slot_a, int slot_b, int *conflict, long 
  * included: with driftModel 0 a non-passive particle moves as a passive one
  * (Particle.cpp:380-454 act only under driftModel 1), and the hourly side
  * flip of leewayJibeProbabilityHourly > 0 (Particle.cpp:805-813) runs on
- * WC_PATH_PARITY; WC_PATH_FAST returns WC_ERR_UNSUPPORTED for a positive
- * jibe probability once a non-passive particle has been uploaded or
- * released. */
+ * both move paths with the reference's keyed draw. */
 int wc_set_particles(wc_ctx *ctx, const wc_particle_data *particles, size_t n);
 int wc_append_particles(wc_ctx *ctx, const wc_particle_data *particles, size_t n);
 int wc_get_particles(wc_ctx *ctx, wc_particle_data *particles, size_t n);

@@ -47,7 +47,6 @@ struct wc_ctx {
     // particles (SoA, double-buffered for compaction)
     size_t n = 0, cap = 0;
     SoA s{}, s2{};
-    bool typed = false;   // a non-passive particle may be held (set at upload and release)
     uint32_t *d_blockCounts = nullptr;
     unsigned long long *d_total = nullptr;
     unsigned long long *d_steps = nullptr;
@@ -201,15 +200,12 @@ bool make_params(const wc_config_data *cfg, double start, double end, int s_w, i
     return true;
 }
 
-// jibe: whether the caller's kernels apply the hourly side flip of non-passive
-// particles (Particle.cpp:805-813); where they do not, a positive probability
-// is accepted only when no non-passive particle can be present.
-int check_supported(wc_ctx *c, const wc_config_data *cfg, bool jibe) {
+// The hourly side flip of non-passive particles (Particle.cpp:805-813) runs
+// on both move paths; the ensemble's particles are passive, so it has nothing
+// to act on there.
+int check_supported(wc_ctx *c, const wc_config_data *cfg) {
     if (cfg->driftModel == 1)
         return fail(c, WC_ERR_UNSUPPORTED, "driftModel==1 (leeway) is not implemented on the device path");
-    if (cfg->leewayJibeProbabilityHourly > 0 && !jibe)
-        return fail(c, WC_ERR_UNSUPPORTED,
-                    "leeway jibing of non-passive particles is implemented on the FP64 parity path only");
     if (!(cfg->dti > 0)) return fail(c, WC_ERR_INVALID, "dti must be > 0");
     if (cfg->trackingDirection != 1 && cfg->trackingDirection != -1)
         return fail(c, WC_ERR_INVALID, "trackingDirection must be +1 or -1");
@@ -281,7 +277,6 @@ int upload_particles(wc_ctx *c, const wc_particle_data *p, size_t n, size_t at) 
         f[0][q] = p[q].k; f[1][q] = p[q].j; f[2][q] = p[q].i;
         f[3][q] = p[q].health; f[4][q] = p[q].age; f[5][q] = p[q].time;
         tag[q] = make_tag(p[q].driftObjectType, p[q].driftSide);
-        if (p[q].driftObjectType != 0) c->typed = true;
     }
     double *dst[6] = {c->s.k, c->s.j, c->s.i, c->s.health, c->s.age, c->s.time};
     HIPCHK(c, hipMemcpyAsync(c->s.id + at, id.data(), n * 8, hipMemcpyHostToDevice, c->stream));
@@ -395,7 +390,7 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
     if (hipMemcpyAsync(t.data(), times, oceanTime * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
         return WC_ERR_HIP;
     if (hipStreamSynchronize(st) != hipSuccess) return WC_ERR_HIP;
-    int rc = check_supported(nullptr, &cfg, true);
+    int rc = check_supported(nullptr, &cfg);
     if (rc) return rc;
     // interval and record pair: Particle.cpp:110-115, 183-184
     const int dir = cfg.trackingDirection;
@@ -872,7 +867,6 @@ int wc_set_particles(wc_ctx *c, const wc_particle_data *p, size_t n) {
     int rc = reserve(c, n);
     if (rc) return rc;
     c->n = 0;
-    c->typed = false;
     if (n) {
         rc = upload_particles(c, p, n, 0);
         if (rc) return rc;
@@ -929,7 +923,7 @@ int wc_particle_count(const wc_ctx *c, size_t *n) {
 int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double end, int slot0, int slot1,
                      int path) {
     if (!c || !cfg) return WC_ERR_INVALID;
-    int rc = check_supported(c, cfg, path == WC_PATH_PARITY || !c->typed);
+    int rc = check_supported(c, cfg);
     if (rc) return rc;
     if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
     const int ns = (int)c->slots.size();
@@ -1035,7 +1029,6 @@ int wc_emit(wc_ctx *c, const wc_config_data *cfg, const wc_source *sources, int 
                           cfg->leewayRightSideProbability};
     HIPCHK(c, launch_emit(c->s, c->n, (int)m, d_times, d_src, d_pos, firstId, cfg->randomSeed,
                           cfg->randomSources ? 1 : 0, drift, c->stream));
-    if (cfg->driftObjectType != 0) c->typed = true;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n += m;
     if (emitted) *emitted = m;
@@ -1046,7 +1039,7 @@ int wc_run_window(wc_ctx *c, const wc_config_data *cfg, const wc_window *win, co
                   int n_sources, int path, float *conc, int mask_output, wc_run_stats *stats) {
     if (!c || !cfg || !win || !win->oceanTime || win->T < 1) return WC_ERR_INVALID;
     if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
-    int rc = check_supported(c, cfg, path == WC_PATH_PARITY || (!c->typed && cfg->driftObjectType == 0));
+    int rc = check_supported(c, cfg);
     if (rc) return rc;
     const int T = win->T, dir = cfg->trackingDirection;
     for (int t = 1; t < T; ++t)                               // ROMSAdapter.cpp:197-201
@@ -1389,7 +1382,7 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
                     double *best_ms) {
     if (!c || !cfg) return WC_ERR_INVALID;
     if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
-    int rc = check_supported(c, cfg, true);      // ensemble particles are passive: no side to flip
+    int rc = check_supported(c, cfg);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     auto &E = c->ens;
@@ -1460,7 +1453,7 @@ int wc_ens_run(wc_ctx *c, const wc_config_data *cfg, const double *times, const 
     if (!c || !cfg || !times || !slots || n_intervals < 0) return WC_ERR_INVALID;
     if (!c->ens.have) return fail(c, WC_ERR_INVALID, "wc_ens_set first");
     if (n_intervals > kEnsMaxSteps) return fail(c, WC_ERR_INVALID, "at most %d intervals per call", kEnsMaxSteps);
-    int rc = check_supported(c, cfg, true);      // ensemble particles are passive: no side to flip
+    int rc = check_supported(c, cfg);
     if (rc) return rc;
     if (stats) *stats = wc_ens_stats{};
     if (n_intervals == 0) return WC_OK;

@@ -136,6 +136,22 @@ __device__ __forceinline__ uint64_t mix64(uint64_t v) {
     return v ^ (v >> 31);
 }
 
+// The hourly jibe of a non-passive particle after a live substep
+// (Particle.cpp:805-813): true when the keyed uniform of component 'JIBE'
+// at substep floor(elapsed / dti) falls below jibeStepProbability
+// (DriftModel.hpp:156-160).  keyPrefix = the key after seed, id and
+// interval.  FP64 with the reference's operation order on every path.
+__device__ __forceinline__ bool jibe_flips(const MoveParams &P, uint64_t keyPrefix, double elapsed, double stepDt) {
+#pragma clang fp contract(off)
+    const double prob = stepDt <= 0 ? 0.0 : P.jibeHourly >= 1 ? 1.0 : 1 - exp(log(1 - P.jibeHourly) * stepDt / 3600);
+    const uint64_t sk = mix64(keyPrefix ^ (uint64_t)floor(elapsed / P.dti));
+    return (double)(mix64(sk ^ 0x4a494245ULL) >> 11) * 0x1.0p-53 < prob;
+}
+
+__device__ __forceinline__ uint32_t flip_side(uint32_t tag) {
+    return make_tag(tag_type(tag), (int8_t)-(int)tag_side(tag));
+}
+
 // Sum of a 64-bit value over the wavefront, added once per wave to *counter.
 __device__ __forceinline__ void wave_add_u64(unsigned long long *counter, unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -249,6 +249,7 @@ struct Reg {
     double elapsed;       // time into the interval
     bool inside, healthFresh, changed;
     bool done;            // the substep loop has ended for this interval
+    uint32_t tag;         // drift type and side (read and flipped only with the jibe on)
 };
 
 // Converts an FP64 position into cell + offset; `inside` is the reference's
@@ -270,8 +271,10 @@ __device__ __forceinline__ void to_cell(double jd, double id_, double kd, int et
 // Runs the substeps that start before tEnd (the interval length, or the end
 // of a chunk of it: a later call continues from q.elapsed); q.done marks the
 // end of the particle's loop for the interval.  Returns the live substeps
-// taken.  S is a Packed<T> or a TilePacked<T>.
-template <class T, class Smp>
+// taken.  S is a Packed<T> or a TilePacked<T>.  Jibe: apply the hourly side
+// flip of a non-passive particle (q.tag) after each live substep, with the
+// reference's keyed draw.
+template <class T, class Smp, bool Jibe = false>
 __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp &S, float sigma, uint64_t id,
                                                  double ptime, Reg &q, double tEnd) {
     unsigned steps = 0;
@@ -450,6 +453,10 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
             } else if (!(healthFresh && health <= 0)) {            // :797-804
                 age = age + stepDt;
                 healthFresh = false;
+                if (Jibe && tag_type(q.tag) != 0) {                // :805-813
+                    const uint64_t prefix = mix64(mix64(mix64(P.seed) ^ id) ^ (uint64_t)P.intervalKey);
+                    if (jibe_flips(P, prefix, elapsed, stepDt)) q.tag = flip_side(q.tag);
+                }
             }
             elapsed += stepDt;
         }
@@ -461,7 +468,7 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
     return steps;
 }
 
-template <class T>
+template <class T, bool Jibe>
 __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> S, SoA st, int n,
                                                         unsigned long long *stepCounter) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -493,8 +500,10 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
         q.changed = false;
         q.elapsed = P.elapsed0;
         q.done = false;
-        steps = run_interval<T>(P, S, (float)P.sigma, id, ptime, q, P.intervalLength);
+        q.tag = Jibe ? st.tag[idx] : 0u;
+        steps = run_interval<T, Packed<T>, Jibe>(P, S, (float)P.sigma, id, ptime, q, P.intervalLength);
         if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
+        if (Jibe && tag_type(q.tag) != 0) st.tag[idx] = q.tag;
         if (q.changed) {
             st.k[idx] = (double)q.k;
             st.j[idx] = (double)q.jc + (double)q.fj;
@@ -1010,7 +1019,10 @@ hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packB
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n <= 0) return e;
     const int block = 256;
-    move_fast_kernel<T><<<(n + block - 1) / block, block, 0, stream>>>(P, S, s, n, ctr);
+    if (P.jibe)
+        move_fast_kernel<T, true><<<(n + block - 1) / block, block, 0, stream>>>(P, S, s, n, ctr);
+    else
+        move_fast_kernel<T, false><<<(n + block - 1) / block, block, 0, stream>>>(P, S, s, n, ctr);
     return hipGetLastError();
 }
 

@@ -386,16 +386,8 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
         if (p.health > 0) {                                 // :797-804
             p.age = p.age + stepDt;
             p.health = 1.0 * exp(-p.age / P.tau0);
-            if (P.jibe && tag_type(p.tag) != 0) {           // :805-813
-                // the side flips with jibeStepProbability (DriftModel.hpp:156-160)
-                // against the keyed uniform of component 'JIBE'
-                const double prob = stepDt <= 0        ? 0.0
-                                  : P.jibeHourly >= 1 ? 1.0
-                                                      : 1 - exp(log(1 - P.jibeHourly) * stepDt / 3600);
-                const uint64_t sk = mix64(keyPrefix ^ (uint64_t)floor(elapsed / P.dti));
-                const double u = (double)(mix64(sk ^ 0x4a494245ULL) >> 11) * 0x1.0p-53;
-                if (u < prob) p.tag = make_tag(tag_type(p.tag), (int8_t)-(int)tag_side(p.tag));
-            }
+            if (P.jibe && tag_type(p.tag) != 0 && jibe_flips(P, keyPrefix, elapsed, stepDt))   // :805-813
+                p.tag = flip_side(p.tag);
         }
         elapsed += stepDt;                                  // :821
     }
