@@ -49,7 +49,8 @@ def algorithmic_bytes_per_step(store_bytes: int, static_bytes: int, midpoint: bo
     return n_dyn * store_bytes + n_stat * static_bytes
 
 
-STATE_BYTES = 100  # FP64 SoA: read id,k,j,i,health,age,time (56 B) and the drift tag (4 B) + write k,j,i,health,age (40 B)
+STATE_BYTES = 96   # FP64 SoA: read id,k,j,i,health,age,time (56 B) + write k,j,i,health,age (40 B); the drift
+                   # tag is touched only with the jibe on
 
 
 def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):

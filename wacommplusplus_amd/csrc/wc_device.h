@@ -99,17 +99,19 @@ struct State {
     uint32_t tag;
 };
 
-// Only a non-passive particle's side can change during a move, so the tag is
-// written back for those alone.
+// The tag is read only when the move can change it (tags: the jibe is on),
+// and written back for non-passive particles alone.
 struct SoAAccess {
     SoA s;
+    int tags;
     __device__ __forceinline__ void load(int idx, State &p) const {
         p.id = s.id[idx]; p.k = s.k[idx]; p.j = s.j[idx]; p.i = s.i[idx];
-        p.health = s.health[idx]; p.age = s.age[idx]; p.time = s.time[idx]; p.tag = s.tag[idx];
+        p.health = s.health[idx]; p.age = s.age[idx]; p.time = s.time[idx];
+        p.tag = tags ? s.tag[idx] : 0u;
     }
     __device__ __forceinline__ void store(int idx, const State &p) const {
         s.k[idx] = p.k; s.j[idx] = p.j; s.i[idx] = p.i; s.health[idx] = p.health; s.age[idx] = p.age;
-        if (tag_type(p.tag) != 0) s.tag[idx] = p.tag;
+        if (tags && tag_type(p.tag) != 0) s.tag[idx] = p.tag;
     }
 };
 

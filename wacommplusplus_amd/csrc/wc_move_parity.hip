@@ -456,7 +456,7 @@ hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n <= 0) return e;
     const int block = 256;
-    move_parity_kernel<PackedSampler, SoAAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, SoAAccess{s}, n,
+    move_parity_kernel<PackedSampler, SoAAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, SoAAccess{s, P.jibe}, n,
                                                                                                 stepCounter);
     return hipGetLastError();
 }
