@@ -142,3 +142,17 @@ def test_reconstruction_loss_weights():
                 assert np.array_equal(R(w), wa) and np.array_equal(R(x), xa)
                 return
     raise AssertionError("augmentation is not one rotation/flip")
+
+
+def test_reconstruction_tta_views_invert():
+    """Each of the 8 rotation/flip views used for test-time averaging maps
+    back to the original image, and the 8 views are distinct."""
+    from wacommplusplus_amd.reconstruct import _dihedral
+    a = np.arange(2 * 3 * 8 * 8, dtype=np.float32).reshape(2, 3, 8, 8)
+    views = []
+    for k in range(4):
+        for f in (False, True):
+            v = _dihedral(a, k, f)
+            assert np.array_equal(_dihedral(v, k, f, inverse=True), a)
+            views.append(v.tobytes())
+    assert len(set(views)) == 8

@@ -220,18 +220,34 @@ def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, lr_min=1e-4, seed=1, lo
     return dict(loss=hist, seconds=time.perf_counter() - t0, steps=step)
 
 
-def reconstruct(net: UNet, few, sub_every, scale, batch=16, extra=None):
+def _dihedral(a, k, flip, inverse=False):
+    """Rotation by k quarter turns (then a flip of the last axis) of the
+    image axes, or its inverse."""
+    if not inverse:
+        a = np.rot90(a, k, axes=(-2, -1))
+        return a[..., ::-1] if flip else a
+    a = a[..., ::-1] if flip else a
+    return np.rot90(a, -k, axes=(-2, -1))
+
+
+def reconstruct(net: UNet, few, sub_every, scale, batch=16, extra=None, tta=False):
     """Concentration estimate (counts of the full release) from few-particle
     histograms (and the extra planes the network was trained with), and the
-    device ms of the forward passes."""
+    device ms of the forward passes.  tta: average the network's output over
+    the 8 rotations and flips of the input (each mapped back), in the
+    network's units."""
     x, _ = to_net(few, None, sub_every, scale, extra)
-    out = np.zeros((len(x),) + x.shape[-2:], np.float32)
+    out = np.zeros((len(x),) + x.shape[-2:], np.float64)
     ms = 0.0
-    for b in range(0, len(x), batch):
-        y, m = net.forward(x[b:b + batch])
-        out[b:b + batch] = y
-        ms += m
-    return (np.maximum(out, 0.0).astype(np.float64) * scale) ** 2, ms
+    views = [(k, f) for k in range(4) for f in (False, True)] if tta else [(0, False)]
+    for k, f in views:
+        xv = np.ascontiguousarray(_dihedral(x, k, f))
+        for b in range(0, len(x), batch):
+            y, m = net.forward(xv[b:b + batch])
+            out[b:b + batch] += _dihedral(y, k, f, inverse=True)
+            ms += m
+    out /= len(views)
+    return (np.maximum(out, 0.0) * scale) ** 2, ms
 
 
 def rel_l2(est, ref):
@@ -241,7 +257,7 @@ def rel_l2(est, ref):
     return np.linalg.norm(d, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-30)
 
 
-def evaluate(net: UNet, data, scale, extras=()):
+def evaluate(net: UNet, data, scale, extras=(), tta=False):
     """Held-out errors of plain gridding and of the network against the
     N-particle histograms, the targets' own noise, and the estimated errors
     against the noise-free field.  Crops whose target is empty (every
@@ -251,7 +267,7 @@ def evaluate(net: UNet, data, scale, extras=()):
     few, allc, sub = data["few"][keep], data["all"][keep].astype(np.float64), data["sub_every"]
     extra = extra_planes(data, extras)
     grid = sub * (few[:, 0] if few.ndim == 4 else few).astype(np.float64)      # the last hour's histogram
-    rec, ms = reconstruct(net, few, sub, scale, extra=None if extra is None else extra[keep])
+    rec, ms = reconstruct(net, few, sub, scale, extra=None if extra is None else extra[keep], tta=tta)
     e_grid, e_net = rel_l2(grid, allc), rel_l2(rec, allc)
     e_t = np.sqrt(allc.reshape(len(allc), -1).sum(1)) / np.linalg.norm(allc.reshape(len(allc), -1), axis=1)
     truth = lambda e: np.sqrt(np.maximum(e ** 2 - e_t ** 2, 0.0))   # noqa: E731
@@ -274,14 +290,15 @@ def train_flops_per_image(h: int, w: int) -> float:
 def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, epochs: int = 40,
         n_target: int = 1_000_000, sub_every: int = 20, hours: int = 24, crop: int = 128, batch: int = 16,
         scale: float = 16.0, seed: int = 21, frames: int = 1, extras=("kh", "mask"), loss_weight: str = "none",
-        log=None, pairs=None):
+        tta: bool = True, log=None, pairs=None):
     """The whole learned-reconstruction experiment: pairs, training, held-out
     errors, and the comparison with plain gridding at equal GPU time (the
     move of n = n_target / sub_every particles per scenario, batched 256 at a
     time, plus one forward pass per image).  frames > 1: the network also
     sees the few-particle histograms of the frames - 1 hours before; extras:
     the extra input planes (EXTRAS); loss_weight: "none", "count" or "core"
-    (loss_weights).  pairs: (train, test) data from an earlier call with
+    (loss_weights); tta: average over the 8 rotations and flips at
+    inference (the single-pass errors are reported beside).  pairs: (train, test) data from an earlier call with
     the same sizes, reused instead of generated."""
     if pairs is None:
         tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
@@ -294,7 +311,8 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
         raise ValueError(f"loss_weight {loss_weight!r}: 'none', 'count' or 'core'")
     h = train(net, X, T, epochs=epochs, batch=batch, log=log,
               weights=None if loss_weight == "none" else loss_weights(T, loss_weight))
-    ev = evaluate(net, te, scale, extras)
+    ev = evaluate(net, te, scale, extras, tta=tta)
+    single = evaluate(net, te, scale, extras) if tta else None
     net.close()
     # GPU time of the few-particle move: 256 scenarios of n particles, batched
     n = n_target // sub_every
@@ -312,11 +330,12 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     flops = train_flops_per_image(crop, crop) * h["steps"] * batch
     return dict(
         pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop, frames=frames,
-                   extras=list(extras), loss_weight=loss_weight,
+                   extras=list(extras), loss_weight=loss_weight, tta=tta,
                    seconds=tr["seconds"] + te["seconds"], particle_steps=int(tr["particle_steps"] + te["particle_steps"])),
         training=dict(epochs=epochs, steps=h["steps"], seconds=h["seconds"], loss_first=h["loss"][0],
                       loss_last=h["loss"][-1], tflops=flops / h["seconds"] / 1e12),
         held_out=ev,
+        held_out_single_pass=single,
         equal_gpu_ms=dict(move_ms_per_scenario=move_ms, net_ms_per_image=net_ms, plain_particles=n_equal,
                           plain_e_truth=e_grid_equal, learned_e_truth=ev["e_net_truth"],
                           plain_particles_to_match_learned=n_match),
@@ -364,6 +383,8 @@ if __name__ == "__main__":
     ap.add_argument("--loss-weight", default="none", metavar="SETS",
                     help="loss weighting, none, count or core; several separated by ';' train "
                          "one network each on the same pairs")
+    ap.add_argument("--no-tta", dest="tta", action="store_false",
+                    help="one forward pass instead of the average over the 8 rotations and flips")
     ap.add_argument("--extras", default="kh,mask", metavar="SETS",
                     help="extra input planes, comma-separated (kh, mask); several sets separated by ';' "
                          "train one network each on the same pairs ('none' = no extras)")
@@ -384,7 +405,7 @@ if __name__ == "__main__":
     combos = [(st, lw) for st in sets for lw in weights]
     if len(combos) == 1:
         print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames,
-                             extras=combos[0][0], loss_weight=combos[0][1], log=say)), flush=True)
+                             extras=combos[0][0], loss_weight=combos[0][1], tta=a.tta, log=say)), flush=True)
     else:
         pairs = (generate_pairs(c, W, a.train, a.target, a.sub_every, seed=21, frames=a.frames, log=say),
                  generate_pairs(c, W, a.test, a.target, a.sub_every, seed=22, frames=a.frames, log=say))
@@ -393,5 +414,5 @@ if __name__ == "__main__":
             name = f"{','.join(st) or 'none'}/{lw}"
             say(f"== extras / loss weight: {name}")
             out[name] = run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames, extras=st,
-                            loss_weight=lw, log=say, pairs=pairs)
+                            loss_weight=lw, tta=a.tta, log=say, pairs=pairs)
         print(json.dumps({"sets": out}), flush=True)
