@@ -230,23 +230,26 @@ def _dihedral(a, k, flip, inverse=False):
     return np.rot90(a, -k, axes=(-2, -1))
 
 
-def reconstruct(net: UNet, few, sub_every, scale, batch=16, extra=None, tta=False):
+def reconstruct(net, few, sub_every, scale, batch=16, extra=None, tta=False):
     """Concentration estimate (counts of the full release) from few-particle
     histograms (and the extra planes the network was trained with), and the
-    device ms of the forward passes.  tta: average the network's output over
-    the 8 rotations and flips of the input (each mapped back), in the
-    network's units."""
+    device ms of the forward passes.  net: a UNet or a list of them (their
+    outputs averaged).  tta: average each network's output over the 8
+    rotations and flips of the input (each mapped back), in the network's
+    units."""
+    nets = net if isinstance(net, (list, tuple)) else [net]
     x, _ = to_net(few, None, sub_every, scale, extra)
     out = np.zeros((len(x),) + x.shape[-2:], np.float64)
     ms = 0.0
     views = [(k, f) for k in range(4) for f in (False, True)] if tta else [(0, False)]
-    for k, f in views:
-        xv = np.ascontiguousarray(_dihedral(x, k, f))
-        for b in range(0, len(x), batch):
-            y, m = net.forward(xv[b:b + batch])
-            out[b:b + batch] += _dihedral(y, k, f, inverse=True)
-            ms += m
-    out /= len(views)
+    for nt in nets:
+        for k, f in views:
+            xv = np.ascontiguousarray(_dihedral(x, k, f))
+            for b in range(0, len(x), batch):
+                y, m = nt.forward(xv[b:b + batch])
+                out[b:b + batch] += _dihedral(y, k, f, inverse=True)
+                ms += m
+    out /= len(views) * len(nets)
     return (np.maximum(out, 0.0) * scale) ** 2, ms
 
 
@@ -257,7 +260,7 @@ def rel_l2(est, ref):
     return np.linalg.norm(d, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-30)
 
 
-def evaluate(net: UNet, data, scale, extras=(), tta=False):
+def evaluate(net, data, scale, extras=(), tta=False):
     """Held-out errors of plain gridding and of the network against the
     N-particle histograms, the targets' own noise, and the estimated errors
     against the noise-free field.  Crops whose target is empty (every
@@ -290,7 +293,7 @@ def train_flops_per_image(h: int, w: int) -> float:
 def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, epochs: int = 40,
         n_target: int = 1_000_000, sub_every: int = 20, hours: int = 24, crop: int = 128, batch: int = 16,
         scale: float = 16.0, seed: int = 21, frames: int = 1, extras=("kh", "mask"), loss_weight: str = "none",
-        tta: bool = True, log=None, pairs=None):
+        tta: bool = True, members: int = 1, log=None, pairs=None):
     """The whole learned-reconstruction experiment: pairs, training, held-out
     errors, and the comparison with plain gridding at equal GPU time (the
     move of n = n_target / sub_every particles per scenario, batched 256 at a
@@ -298,7 +301,9 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     sees the few-particle histograms of the frames - 1 hours before; extras:
     the extra input planes (EXTRAS); loss_weight: "none", "count" or "core"
     (loss_weights); tta: average over the 8 rotations and flips at
-    inference (the single-pass errors are reported beside).  pairs: (train, test) data from an earlier call with
+    inference (the single-pass errors are reported beside); members:
+    networks trained from different initialisations whose outputs are
+    averaged.  pairs: (train, test) data from an earlier call with
     the same sizes, reused instead of generated."""
     if pairs is None:
         tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
@@ -306,14 +311,19 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     else:
         tr, te = pairs
     X, T = to_net(tr["few"], tr["all"], sub_every, scale, extra_planes(tr, extras))
-    net = UNet(ctx_device(ctx), seed=3, channels=frames + len(extras))
     if loss_weight not in ("none", "count", "core"):
         raise ValueError(f"loss_weight {loss_weight!r}: 'none', 'count' or 'core'")
-    h = train(net, X, T, epochs=epochs, batch=batch, log=log,
-              weights=None if loss_weight == "none" else loss_weights(T, loss_weight))
-    ev = evaluate(net, te, scale, extras, tta=tta)
-    single = evaluate(net, te, scale, extras) if tta else None
-    net.close()
+    nets, hs = [], []
+    for m in range(members):
+        net = UNet(ctx_device(ctx), seed=3 + m, channels=frames + len(extras))
+        hs.append(train(net, X, T, epochs=epochs, batch=batch, log=log, seed=1 + m,
+                        weights=None if loss_weight == "none" else loss_weights(T, loss_weight)))
+        nets.append(net)
+    h = dict(loss=hs[0]["loss"], seconds=sum(x["seconds"] for x in hs), steps=sum(x["steps"] for x in hs))
+    ev = evaluate(nets, te, scale, extras, tta=tta)
+    single = evaluate(nets[0], te, scale, extras) if (tta or members > 1) else None
+    for net in nets:
+        net.close()
     # GPU time of the few-particle move: 256 scenarios of n particles, batched
     n = n_target // sub_every
     srcs = [EnsSource(k=-0.27, j=float(tr["j"][q % n_train]), i=float(tr["i"][q % n_train]), spread=0.25,
@@ -330,7 +340,7 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     flops = train_flops_per_image(crop, crop) * h["steps"] * batch
     return dict(
         pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop, frames=frames,
-                   extras=list(extras), loss_weight=loss_weight, tta=tta,
+                   extras=list(extras), loss_weight=loss_weight, tta=tta, members=members,
                    seconds=tr["seconds"] + te["seconds"], particle_steps=int(tr["particle_steps"] + te["particle_steps"])),
         training=dict(epochs=epochs, steps=h["steps"], seconds=h["seconds"], loss_first=h["loss"][0],
                       loss_last=h["loss"][-1], tflops=flops / h["seconds"] / 1e12),
@@ -383,6 +393,7 @@ if __name__ == "__main__":
     ap.add_argument("--loss-weight", default="none", metavar="SETS",
                     help="loss weighting, none, count or core; several separated by ';' train "
                          "one network each on the same pairs")
+    ap.add_argument("--members", type=int, default=1, help="networks averaged (different initialisations)")
     ap.add_argument("--no-tta", dest="tta", action="store_false",
                     help="one forward pass instead of the average over the 8 rotations and flips")
     ap.add_argument("--extras", default="kh,mask", metavar="SETS",
@@ -405,7 +416,8 @@ if __name__ == "__main__":
     combos = [(st, lw) for st in sets for lw in weights]
     if len(combos) == 1:
         print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames,
-                             extras=combos[0][0], loss_weight=combos[0][1], tta=a.tta, log=say)), flush=True)
+                             extras=combos[0][0], loss_weight=combos[0][1], tta=a.tta, members=a.members,
+                             log=say)), flush=True)
     else:
         pairs = (generate_pairs(c, W, a.train, a.target, a.sub_every, seed=21, frames=a.frames, log=say),
                  generate_pairs(c, W, a.test, a.target, a.sub_every, seed=22, frames=a.frames, log=say))
@@ -414,5 +426,5 @@ if __name__ == "__main__":
             name = f"{','.join(st) or 'none'}/{lw}"
             say(f"== extras / loss weight: {name}")
             out[name] = run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames, extras=st,
-                            loss_weight=lw, tta=a.tta, log=say, pairs=pairs)
+                            loss_weight=lw, tta=a.tta, members=a.members, log=say, pairs=pairs)
         print(json.dumps({"sets": out}), flush=True)
