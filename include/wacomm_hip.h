@@ -278,7 +278,9 @@ int wc_compare_records(wc_ctx *ctx, int slot_a, int slot_b, int *conflict, long 
 
 /* Particle state: host AoS in and out (Particles.hpp:17 holds particle_data).
  * wc_set_particles replaces the device population; wc_append_particles adds
- * to it (emission).  Every field travels, the drift object type and side
+ * to it (emission).  A context holds at most 2,147,483,392 particles (the
+ * kernels index with 32-bit integers; the drop-in entry points take the same
+ * limit for particleCount).  Every field travels, the drift object type and side
  * included: with driftModel 0 a non-passive particle moves as a passive one
  * (Particle.cpp:380-454 act only under driftModel 1), and the hourly side
  * flip of leewayJibeProbabilityHourly > 0 (Particle.cpp:805-813) runs on

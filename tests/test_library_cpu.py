@@ -156,3 +156,16 @@ def test_reconstruction_tta_views_invert():
             assert np.array_equal(_dihedral(v, k, f, inverse=True), a)
             views.append(v.tobytes())
     assert len(set(views)) == 8
+
+
+def test_dropin_rejects_populations_beyond_32bit_indexing():
+    """Populations above 2^31 - 256 would overflow the kernels' 32-bit
+    indexing: the drop-in entry points refuse them before any device call
+    (so this runs without a GPU)."""
+    L = wc.load_library()
+    frame = np.zeros(8, np.float32)
+    big = 0x7FFFFF00 + 1
+    assert L.wc_hip_concentration(None, big, 2, 2, 2, frame.ctypes.data_as(C.c_void_p), None) == wc.WC_ERR_INVALID
+    dummy = np.zeros(64, np.float64)
+    d = dummy.ctypes.data_as(C.c_void_p)
+    assert L.wc_hip_move_particles(d, d, 0, 2, 3, 2, 2, 2, *([d] * 6), *([d] * 9), big, 1, 1, None) == wc.WC_ERR_INVALID

@@ -232,10 +232,11 @@ int alloc_soa(wc_ctx *c, SoA &s, size_t cap) {
 
 int reserve(wc_ctx *c, size_t need) {
     if (need <= c->cap) return WC_OK;
-    if (need > (size_t)INT32_MAX) return fail(c, WC_ERR_INVALID, "at most %d particles per context", INT32_MAX);
+    if (need > (size_t)kMaxParticles)
+        return fail(c, WC_ERR_INVALID, "at most %d particles per context", kMaxParticles);
     size_t cap = c->cap ? c->cap : 1024;
     while (cap < need) cap *= 2;
-    if (cap > (size_t)INT32_MAX) cap = INT32_MAX;
+    if (cap > (size_t)kMaxParticles) cap = kMaxParticles;
     SoA a{}, b{};
     int rc = alloc_soa(c, a, cap);
     if (rc) { free_soa(a); return rc; }
@@ -377,8 +378,8 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
     (void)windU10; (void)windV10; (void)stokesU; (void)stokesV;
     if (!config || !times || !lonRad || !latRad || !depthIntervals || !h || !zeta || !u || !v || !w || !akt)
         return WC_ERR_INVALID;
-    if (particleCount < 0 || oceanTime <= 0 || timeIndex < 0 || timeIndex >= oceanTime || sW != sRho + 1 ||
-        sW < 2 || eta < 2 || xi < 2)
+    if (particleCount < 0 || particleCount > kMaxParticles || oceanTime <= 0 || timeIndex < 0 ||
+        timeIndex >= oceanTime || sW != sRho + 1 || sW < 2 || eta < 2 || xi < 2)
         return WC_ERR_INVALID;
     if (particleCount == 0) return WC_OK;
     if (!particles) return WC_ERR_INVALID;
@@ -413,7 +414,8 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
 
 int wc_hip_concentration(const wc_particle_data *particles, int particleCount, int s_rho, int eta, int xi,
                          float *conc, void *stream) {
-    if (particleCount < 0 || s_rho < 1 || eta < 1 || xi < 1 || !conc) return WC_ERR_INVALID;
+    if (particleCount < 0 || particleCount > kMaxParticles || s_rho < 1 || eta < 1 || xi < 1 || !conc)
+        return WC_ERR_INVALID;
     if ((size_t)s_rho * eta * xi > 0xFFFFFFFFull) return WC_ERR_INVALID;
     if (particleCount == 0) return WC_OK;
     if (!particles) return WC_ERR_INVALID;
@@ -1197,7 +1199,7 @@ int wc_ens_set(wc_ctx *c, const wc_particle_data *p, const uint8_t *scenario, si
     if (!c || (n && (!p || !scenario)) || !sigma) return WC_ERR_INVALID;
     if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
     if (n_scenarios < 1 || n_scenarios > 256) return fail(c, WC_ERR_INVALID, "1..256 scenarios");
-    if (n > (size_t)INT32_MAX) return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", INT32_MAX);
+    if (n > (size_t)kMaxParticles) return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", kMaxParticles);
     const size_t plane = (size_t)c->eta * c->xi;
     if ((size_t)n_scenarios * plane > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "surface grids too large");
     for (size_t q = 0; q < n; ++q) {
@@ -1284,7 +1286,8 @@ int wc_ens_release(wc_ctx *c, const wc_ens_source *sources, int n_sources, const
         first[q + 1] = first[q] + S.count;
     }
     const unsigned long long n = first[n_sources];
-    if (n > (unsigned long long)INT32_MAX) return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", INT32_MAX);
+    if (n > (unsigned long long)kMaxParticles)
+        return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", kMaxParticles);
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_ens(c);

@@ -86,6 +86,11 @@ struct EmitDrift {
     double rightProbability;
 };
 
+// Largest population of one launch: every kernel indexes particles with a
+// 32-bit int (blockIdx.x * blockDim.x + threadIdx.x) and sizes its grid as
+// (n + block - 1) / block, which both stay below 2^31 for n up to this.
+constexpr int kMaxParticles = 0x7FFFFF00;
+
 // Particle state, structure of arrays (the context's canonical layout).
 struct SoA {
     uint64_t *id;
