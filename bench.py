@@ -288,18 +288,22 @@ def main():
             precision["conc_l2"][key] = float(np.linalg.norm(grids[(key, "rnd")] - g0) / nrm)
         log(f"[precision] {precision}")
 
-    # PCIe-inclusive rate: two resident slots, one record uploaded per interval
+    # PCIe-inclusive rate: a three-slot ring, one record uploaded per
+    # interval from host memory -- the record after next, copied on the
+    # copy stream while the move of the current interval runs
     pcie = None
     if not args.no_variants:
-        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=2, store="f32")
+        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=3, store="f32")
         ctx.set_particles(p0)
-        ctx.upload_record(0, *W.record(0))
         ctx.reset_step_count()
         ctx.sync()
         t0 = time.perf_counter()
+        ctx.upload_record(0, *W.record(0))
+        ctx.upload_record(1, *W.record(1))
         for s in range(args.steps):
-            ctx.upload_record((s + 1) % 2, *W.record(s + 1))
-            ctx.move_interval(cfg, float(W.times[s]), float(W.times[s + 1]), s % 2, (s + 1) % 2, path="parity")
+            ctx.move_interval(cfg, float(W.times[s]), float(W.times[s + 1]), s % 3, (s + 1) % 3, path="parity")
+            if s + 2 < W.T:
+                ctx.upload_record((s + 2) % 3, *W.record(s + 2))
             ctx.remove_dead()
             ctx.grid(counts=False)
         ctx.sync()

@@ -181,7 +181,10 @@ int wc_set_grid(wc_ctx *ctx, int s_w, int s_rho, int eta, int xi, const double *
  * [s_rho][eta][xi], w, akt [s_w][eta][xi]) into ring slot `slot`; slots hold
  * FP32 and each interval's pack converts the record pair to the context's
  * storage type.  Replaces the whole-window re-upload of Wacomm.cpp:417-477
- * by one record per interval. */
+ * by one record per interval.  The copy runs on the context's copy stream and
+ * waits only for enqueued moves that read this slot, so with three or more
+ * slots the record after next uploads while the current move runs.  Returns
+ * when the copy is complete: the host arrays may be reused at once. */
 int wc_upload_record(wc_ctx *ctx, int slot, const float *zeta, const float *u, const float *v, const float *w,
                      const float *akt);
 

@@ -88,15 +88,18 @@ def test_emit_matches_oracle(small, random_sources, drift):
         assert np.max(np.abs(dev[f] - ref[f])) <= tol
 
 
-def test_run_window_matches_oracle(small):
+@pytest.mark.parametrize("n_slots", [2, 3, 7])
+def test_run_window_matches_oracle(small, n_slots):
     """wc_run_window (emit, move, remove dead, grid per interval) against the
-    same loop driven on the oracle; deterministic, so bit-exact."""
+    same loop driven on the oracle; deterministic, so bit-exact.  Two slots
+    alternate uploads and moves; three or more upload the record after next
+    on the copy stream while the move runs."""
     W = small
     cfg = wc.default_config(random=False, randomSources=False)
     src = sources_for(W)
     zeta, u, v, w, a = W.stacked()
     with wc.Context(0) as ctx:
-        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask)
+        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=n_slots)
         conc, st = ctx.run_window(cfg, W.times, zeta, u, v, w, a, sources=src, mask_output=True)
         dev = ctx.get_particles()
     assert st.intervals == W.T - 1

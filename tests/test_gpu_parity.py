@@ -311,6 +311,30 @@ def test_full_configuration_sample_24h(ctx, full, random):
         assert frac >= 0.999, frac
 
 
+def test_upload_waits_for_the_move_reading_its_slot(ctx, full):
+    """Uploads run on their own stream: an upload into a slot that an
+    enqueued move still reads waits for that move (the per-slot event), and
+    one into a free slot runs beside it.  2M particles of the full-size
+    release keep the move (~13 ms) longer than a record's copy (~6 ms)."""
+    p = release(full, 2_000_000, 50)
+    cfg = wc.default_config(random=True, sigma=wc.sigma_from_kh(1.0, 30.0))
+    ref = run_device(ctx, full, cfg, p, 1)
+    from runner import load_grid
+    load_grid(ctx, full, 3)
+    ctx.upload_record(0, *full.record(0))
+    ctx.upload_record(1, *full.record(1))
+    ctx.set_particles(p)
+    ctx.move_interval(cfg, float(full.times[0]), float(full.times[1]), 0, 1)     # enqueued, not waited for
+    ctx.upload_record(1, *full.record(2))           # the slot the move reads: must wait for it
+    ctx.upload_record(2, *full.record(3))           # a free slot: overlaps the move
+    got = ctx.get_particles()
+    for f in ref.dtype.names:
+        assert np.array_equal(got[f], ref[f], equal_nan=f not in ("id", "driftObjectType", "driftSide")), f
+    # the slots hold the new records
+    z1 = ctx.get_record(1)[0]
+    assert np.array_equal(z1, np.asarray(full.record(2)[0], np.float32))
+
+
 # --------------------------------------------------------------------------
 # gridding and dead removal
 # --------------------------------------------------------------------------

@@ -41,8 +41,15 @@ struct wc_ctx {
         float *f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};      // zeta, u, v, w, akt
         bool filled = false;
         int record = -1;      // window record held by the slot (wc_run_window)
+        // recorded on the compute stream after the last enqueued work that
+        // reads or writes the slot; an upload into the slot waits for it
+        hipEvent_t released = nullptr;
     };
     std::vector<Slot> slots;
+
+    // record uploads run on their own stream, so the next record's copy
+    // overlaps the move already enqueued on `stream`
+    hipStream_t copy = nullptr;
 
     // particles (SoA, double-buffered for compaction)
     size_t n = 0, cap = 0;
@@ -436,6 +443,7 @@ int wc_ctx_create(int device, wc_ctx **out) {
     wc_ctx *c = new wc_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipMalloc(&c->d_total, sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&c->d_steps, sizeof(unsigned long long)) != hipSuccess ||
@@ -483,8 +491,10 @@ static void free_grid(wc_ctx *c) {
     c->d_metric = nullptr;
     c->d_hf = c->d_md = c->d_depthf = nullptr;
     c->d_pack = nullptr;
-    for (auto &s : c->slots)
+    for (auto &s : c->slots) {
         for (int f = 0; f < 5; ++f) hipFree(s.f[f]);
+        if (s.released) hipEventDestroy(s.released);
+    }
     c->slots.clear();
     hipFree(c->d_counts);
     hipFree(c->d_conc);
@@ -509,6 +519,7 @@ int wc_ctx_destroy(wc_ctx *c) {
     if (!c) return WC_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->copy) hipStreamSynchronize(c->copy);
     free_grid(c);
     free_ens(c);
     free_soa(c->s);
@@ -523,6 +534,7 @@ int wc_ctx_destroy(wc_ctx *c) {
         for (auto &pr : c->pending[k]) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
     for (auto e : c->spare) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->copy) hipStreamDestroy(c->copy);
     delete c;
     return WC_OK;
 }
@@ -588,6 +600,8 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
     c->slots.resize(n_slots);
     for (auto &s : c->slots) {
         for (int f = 0; f < 5; ++f) HIPCHK(c, hipMalloc(&s.f[f], field_count(c, f) * sizeof(float)));
+        HIPCHK(c, hipEventCreateWithFlags(&s.released, hipEventDisableTiming));
+        HIPCHK(c, hipEventRecord(s.released, c->stream));
     }
     HIPCHK(c, hipMalloc(&c->d_counts, plane * s_rho * sizeof(uint32_t)));
     c->haveGrid = true;
@@ -604,9 +618,15 @@ int wc_upload_record(wc_ctx *c, int slot, const float *zeta, const float *u, con
         if (!src[f]) return fail(c, WC_ERR_INVALID, "null forcing field %d", f);
     HIPCHK(c, hipSetDevice(c->device));
     auto &s = c->slots[slot];
+    // the copy waits only for the enqueued work that uses this slot; work on
+    // other slots (the move of the current interval) runs meanwhile.  The
+    // call returns when the copy is done, so the caller may reuse its arrays,
+    // and every later launch sees the record.
+    s.filled = false;
+    HIPCHK(c, hipStreamWaitEvent(c->copy, s.released, 0));
     for (int f = 0; f < 5; ++f)
-        HIPCHK(c, hipMemcpyAsync(s.f[f], src[f], field_count(c, f) * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpyAsync(s.f[f], src[f], field_count(c, f) * sizeof(float), hipMemcpyHostToDevice, c->copy));
+    HIPCHK(c, hipStreamSynchronize(c->copy));
     s.filled = true;
     s.record = -1;
     return WC_OK;
@@ -945,6 +965,9 @@ int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double 
         HIPCHK(c, launch_move_parity_soa(P, F, G, c->d_pack, c->d_metric, c->s, (int)c->n, c->d_steps, c->stream));
     else
         HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps, c->stream));
+    // an upload into either slot now waits for this launch
+    HIPCHK(c, hipEventRecord(c->slots[slot0].released, c->stream));
+    HIPCHK(c, hipEventRecord(c->slots[slot1].released, c->stream));
     return timing_end(c, 0);
 }
 
@@ -1060,12 +1083,16 @@ int wc_run_window(wc_ctx *c, const wc_config_data *cfg, const wc_window *win, co
     uint64_t steps0 = 0;
     rc = wc_step_count(c, &steps0);
     if (rc) return rc;
-    auto ensure = [&](int rec) -> int {                       // record rec -> slot rec % 2
-        auto &s = c->slots[rec % 2];
+    // record rec -> slot rec % ring.  With three or more slots the record
+    // after next is uploaded while the move runs (the copy waits only for the
+    // move that last used its slot); with two, uploads and moves alternate.
+    const int ring = c->slots.size() >= 3 ? 3 : 2;
+    auto ensure = [&](int rec) -> int {
+        auto &s = c->slots[rec % ring];
         if (s.filled && s.record == rec) return WC_OK;
         const float *src[5];
         for (int f = 0; f < 5; ++f) src[f] = fields[f] + nf[f] * rec;
-        int r = wc_upload_record(c, rec % 2, src[0], src[1], src[2], src[3], src[4]);
+        int r = wc_upload_record(c, rec % ring, src[0], src[1], src[2], src[3], src[4]);
         if (r) return r;
         s.record = rec;
         return WC_OK;
@@ -1081,8 +1108,10 @@ int wc_run_window(wc_ctx *c, const wc_config_data *cfg, const wc_window *win, co
             if (rc) break;
         }
         if ((rc = ensure(idx)) || (rc = ensure(idx + dir))) break;
-        rc = wc_move_interval(c, cfg, start, end, idx % 2, (idx + dir) % 2, path);
+        rc = wc_move_interval(c, cfg, start, end, idx % ring, (idx + dir) % ring, path);
         if (rc) break;
+        const int ahead = idx + 2 * dir;
+        if (ring == 3 && ahead >= 0 && ahead < T && (rc = ensure(ahead))) break;
         size_t remaining = 0;
         if ((rc = wc_remove_dead(c, &remaining))) break;     // :710-713
         if ((rc = wc_grid(c, nullptr, conc ? conc + frame * (idx + dir) : nullptr, mask_output))) break;   // :722-739
