@@ -306,21 +306,22 @@ hipEvent_t take_event(wc_ctx *c) {
     return e;
 }
 
-void timing_begin(wc_ctx *c) {
+// Event pair around the next launches on stream s (the compute stream by default).
+void timing_begin(wc_ctx *c, hipStream_t s = nullptr) {
     c->cur = nullptr;
     if (!c->timing) return;
     c->cur = take_event(c);
-    if (c->cur && hipEventRecord(c->cur, c->stream) != hipSuccess) {
+    if (c->cur && hipEventRecord(c->cur, s ? s : c->stream) != hipSuccess) {
         c->spare.push_back(c->cur);
         c->cur = nullptr;
     }
 }
 
-int timing_end(wc_ctx *c, int kernel) {
+int timing_end(wc_ctx *c, int kernel, hipStream_t s = nullptr) {
     if (!c->timing || !c->cur) return WC_OK;
     hipEvent_t e = take_event(c);
     if (!e) return fail(c, WC_ERR_HIP, "hipEventCreate failed");
-    HIPCHK(c, hipEventRecord(e, c->stream));
+    HIPCHK(c, hipEventRecord(e, s ? s : c->stream));
     c->pending[kernel].emplace_back(c->cur, e);
     c->cur = nullptr;
     return WC_OK;
@@ -800,20 +801,25 @@ int wc_upload_roms_record(wc_ctx *c, int slot, const float *zeta, const float *u
     auto &s = c->slots[slot];
     s.filled = false;
     s.record = -1;
-    HIPCHK(c, hipMemcpyAsync(s.f[0], zeta, field_count(c, 0) * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(s.f[3], w, field_count(c, 3) * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(s.f[4], akt, field_count(c, 4) * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    if (pu) HIPCHK(c, hipMemcpyAsync(c->d_u_raw, u, pu * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    if (pv) HIPCHK(c, hipMemcpyAsync(c->d_v_raw, v, pv * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_fill_u32(c->d_flag, 1, 0u, c->stream));
-    timing_begin(c);
+    // on the copy stream, after the enqueued work that uses this slot (as
+    // wc_upload_record): the copies and the normalisation overlap a move on
+    // the other slots
+    hipStream_t cs = c->copy;
+    HIPCHK(c, hipStreamWaitEvent(cs, s.released, 0));
+    HIPCHK(c, hipMemcpyAsync(s.f[0], zeta, field_count(c, 0) * sizeof(float), hipMemcpyHostToDevice, cs));
+    HIPCHK(c, hipMemcpyAsync(s.f[3], w, field_count(c, 3) * sizeof(float), hipMemcpyHostToDevice, cs));
+    HIPCHK(c, hipMemcpyAsync(s.f[4], akt, field_count(c, 4) * sizeof(float), hipMemcpyHostToDevice, cs));
+    if (pu) HIPCHK(c, hipMemcpyAsync(c->d_u_raw, u, pu * sizeof(float), hipMemcpyHostToDevice, cs));
+    if (pv) HIPCHK(c, hipMemcpyAsync(c->d_v_raw, v, pv * sizeof(float), hipMemcpyHostToDevice, cs));
+    HIPCHK(c, launch_fill_u32(c->d_flag, 1, 0u, cs));
+    timing_begin(c, cs);
     HIPCHK(c, launch_roms_normalize(d, c->romsMissing, c->d_face, c->d_rot, c->d_u_raw, c->d_v_raw, s.f[1],
-                                    s.f[2], s.f[3], s.f[4], s.f[0], c->d_flag, c->stream));
-    int trc = timing_end(c, 2);
+                                    s.f[2], s.f[3], s.f[4], s.f[0], c->d_flag, cs));
+    int trc = timing_end(c, 2, cs);
     if (trc) return trc;
     uint32_t flag = 0;
-    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag, sizeof flag, hipMemcpyDeviceToHost, cs));
+    HIPCHK(c, hipStreamSynchronize(cs));
     if (flag & 1u) return fail(c, WC_ERR_INVALID, "ROMS u is missing or non-finite at a wet face");
     if (flag & 2u) return fail(c, WC_ERR_INVALID, "ROMS v is missing or non-finite at a wet face");
     if (flag & 4u) return fail(c, WC_ERR_INVALID, "ROMS normalized velocity is outside finite float range");
