@@ -283,8 +283,9 @@ public:
     int run(double &time, double &part, double &cuda, int &nParticles, int &idx) {
         const OceanWindow &o = *ocean_;
         wc_ctx *c = ctx_.get();
+        // three forcing slots: the record after next uploads while the move runs
         ctx_.check(wc_set_grid(c, o.s_w(), o.s_rho, o.eta, o.xi, o.lonRad.data(), o.latRad.data(), o.h.data(),
-                               o.mask.empty() ? nullptr : o.mask.data(), o.depthIntervals.data(), 2, WC_STORE_F32),
+                               o.mask.empty() ? nullptr : o.mask.data(), o.depthIntervals.data(), 3, WC_STORE_F32),
                    "wc_set_grid");
         ctx_.check(wc_set_particles(c, particles_->raw(), particles_->size()), "wc_set_particles");
         wc_window win{o.T(), o.oceanTime.data(), o.zeta.data(), o.u.data(), o.v.data(), o.w.data(), o.akt.data()};
