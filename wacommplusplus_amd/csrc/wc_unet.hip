@@ -556,6 +556,72 @@ __global__ void pool_bwd_kernel(const float *__restrict__ dy, int n, int h, int 
     }
 }
 
+// Vector forms of the pooling and upsampling kernels: one work-item per 8
+// BF16 or 4 FP32 channels of a pixel (16-byte accesses) and 32-bit index
+// arithmetic, for channel counts that divide and groups that fit in 31 bits.
+// Every element is computed by the same expression as in the scalar forms.
+struct PixIdx {
+    unsigned ni, y, x, g;
+};
+__device__ __forceinline__ PixIdx pix_idx(unsigned e, unsigned groups, unsigned w, unsigned h) {
+    const unsigned pix = e / groups, t = pix / w;
+    return PixIdx{t / h, t - (t / h) * h, pix - t * w, e - pix * groups};
+}
+
+__global__ void pool_fwd_vec_kernel(const uint4 *__restrict__ x, unsigned n, unsigned h, unsigned w, unsigned groups,
+                                    uint4 *__restrict__ y) {
+    const unsigned ho = h / 2, wo = w / 2, total = n * ho * wo * groups;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const PixIdx q = pix_idx(e, groups, wo, ho);
+        const size_t b = (((size_t)q.ni * h + 2 * q.y) * w + 2 * q.x) * groups + q.g;
+        Pack8 A, B, C, D, o;
+        A.u = x[b]; B.u = x[b + groups]; C.u = x[b + (size_t)w * groups]; D.u = x[b + (size_t)w * groups + groups];
+        for (int k = 0; k < 8; ++k) {
+            const float s = from_bf16(A.h[k]) + from_bf16(B.h[k]) + from_bf16(C.h[k]) + from_bf16(D.h[k]);
+            o.h[k] = to_bf16(0.25f * s);
+        }
+        y[e] = o.u;
+    }
+}
+
+__global__ void pool_bwd_vec_kernel(const float4 *__restrict__ dy, unsigned n, unsigned h, unsigned w,
+                                    unsigned groups, float4 *__restrict__ dx, int accumulate) {
+    const unsigned ho = h / 2, wo = w / 2, total = n * h * w * groups;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const PixIdx q = pix_idx(e, groups, w, h);
+        const float4 d = dy[(((size_t)q.ni * ho + q.y / 2) * wo + q.x / 2) * groups + q.g];
+        const float4 g = make_float4(0.25f * d.x, 0.25f * d.y, 0.25f * d.z, 0.25f * d.w);
+        if (accumulate) {
+            const float4 a = dx[e];
+            dx[e] = make_float4(a.x + g.x, a.y + g.y, a.z + g.z, a.w + g.w);
+        } else {
+            dx[e] = g;
+        }
+    }
+}
+
+__global__ void up_fwd_vec_kernel(const uint4 *__restrict__ x, unsigned n, unsigned h, unsigned w, unsigned groups,
+                                  uint4 *__restrict__ y) {
+    const unsigned ho = 2 * h, wo = 2 * w, total = n * ho * wo * groups;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const PixIdx q = pix_idx(e, groups, wo, ho);
+        y[e] = x[(((size_t)q.ni * h + q.y / 2) * w + q.x / 2) * groups + q.g];
+    }
+}
+
+__global__ void up_bwd_vec_kernel(const float4 *__restrict__ dy, unsigned n, unsigned h, unsigned w, unsigned groups,
+                                  float4 *__restrict__ dx) {
+    const unsigned wo = 2 * w, total = n * h * w * groups;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const PixIdx q = pix_idx(e, groups, w, h);
+        const size_t b = (((size_t)q.ni * 2 * h + 2 * q.y) * wo + 2 * q.x) * groups + q.g;
+        const float4 a = dy[b], bb = dy[b + groups], c = dy[b + (size_t)wo * groups],
+                     d = dy[b + (size_t)wo * groups + groups];
+        dx[e] = make_float4(a.x + bb.x + c.x + d.x, a.y + bb.y + c.y + d.y, a.z + bb.z + c.z + d.z,
+                            a.w + bb.w + c.w + d.w);
+    }
+}
+
 // nearest-neighbour 2x upsampling, BF16 NHWC
 __global__ void up_fwd_kernel(const uint16_t *__restrict__ x, int n, int h, int w, int c, uint16_t *__restrict__ y) {
     const int ho = 2 * h, wo = 2 * w;
@@ -781,20 +847,41 @@ hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int
     return hipGetLastError();
 }
 
+// the vector forms apply when the channels split into 16-byte groups and the
+// group count of the larger tensor stays below 2^31
+static bool vec_ok(long long groups_total, int c, int per) { return c % per == 0 && groups_total < (1LL << 31); }
+
 hipError_t launch_pool_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s) {
-    pool_fwd_kernel<<<blocks_for((long long)n * (h / 2) * (w / 2) * c), 256, 0, s>>>(x, n, h, w, c, y);
+    if (vec_ok((long long)n * h * w * (c / 8), c, 8)) {
+        const long long g = (long long)n * (h / 2) * (w / 2) * (c / 8);
+        pool_fwd_vec_kernel<<<blocks_for(g), 256, 0, s>>>((const uint4 *)x, n, h, w, c / 8, (uint4 *)y);
+    } else {
+        pool_fwd_kernel<<<blocks_for((long long)n * (h / 2) * (w / 2) * c), 256, 0, s>>>(x, n, h, w, c, y);
+    }
     return hipGetLastError();
 }
 hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, hipStream_t s) {
-    pool_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx, accumulate ? 1 : 0);
+    if (vec_ok((long long)n * h * w * (c / 4), c, 4))
+        pool_bwd_vec_kernel<<<blocks_for((long long)n * h * w * (c / 4)), 256, 0, s>>>(
+            (const float4 *)dy, n, h, w, c / 4, (float4 *)dx, accumulate ? 1 : 0);
+    else
+        pool_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx, accumulate ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s) {
-    up_fwd_kernel<<<blocks_for((long long)n * 4 * h * w * c), 256, 0, s>>>(x, n, h, w, c, y);
+    if (vec_ok((long long)n * 4 * h * w * (c / 8), c, 8))
+        up_fwd_vec_kernel<<<blocks_for((long long)n * 4 * h * w * (c / 8)), 256, 0, s>>>((const uint4 *)x, n, h, w,
+                                                                                     c / 8, (uint4 *)y);
+    else
+        up_fwd_kernel<<<blocks_for((long long)n * 4 * h * w * c), 256, 0, s>>>(x, n, h, w, c, y);
     return hipGetLastError();
 }
 hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx, hipStream_t s) {
-    up_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx);
+    if (vec_ok((long long)n * 4 * h * w * (c / 4), c, 4))
+        up_bwd_vec_kernel<<<blocks_for((long long)n * h * w * (c / 4)), 256, 0, s>>>((const float4 *)dy, n, h, w,
+                                                                                 c / 4, (float4 *)dx);
+    else
+        up_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx);
     return hipGetLastError();
 }
 hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c, uint16_t *y, float *x0,
