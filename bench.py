@@ -5,14 +5,28 @@ sigma-level grid at 150 m with ~30 % land, u/v from 64 random Fourier modes
 (5-40 km, rms 0.3 m/s), w from continuity, 25 hourly records; 4,000,000
 particles released at t0 from 50 random ocean point sources at 1 m depth;
 Kh = 1 m2/s (sigma = sqrt(2 Kh dti)), AKt = 1e-4 m2/s, dti = 30 s, explicit
-midpoint, default closures.  Synthetic data generated in memory.
+midpoint (Euler reported beside it), default closures.  Synthetic data
+generated in memory.
 
 A step is one forcing interval of the hot path for the whole population:
-move (120 substeps) -> remove dead -> grid, i.e. one pass of the interval
-loop of Wacomm::run.  All forcing records are resident in HBM before the
-timed region.  The headline value is the FP64-state parity path (FP32-stored
-forcing, as the reference stores it); the mixed-precision path with BF16,
-FP16 and FP32 storage is reported alongside.
+move (120 substeps) -> remove dead -> grid (counts, then the reference's float
+frame with the land mask, on the device), i.e. one pass of the interval loop
+of Wacomm::run.  All forcing records are resident in HBM before the timed
+region (the headline `value`); the rate with one record uploaded per interval
+from host memory is reported beside it.  The headline is the FP64-state
+parity path (FP32-stored forcing, as the reference stores it); the mixed-
+precision path with BF16 and FP16 storage (plain and as tile-mean residuals)
+and FP32 storage is reported alongside.
+
+Rooflines.  The move keeps each particle in registers for a whole interval and
+its forcing gathers are served by L2 and the Infinity Cache, so it is bound by
+vector instruction issue, not by HBM: `roofline` is the VALU issue rate (wave
+instructions per dispatch from the committed rocprofv3 counters, divided by
+the launch time measured here) against the chip's 256 CU x 4 SIMD x 2.4 GHz /
+4 cycles.  `roofline.hbm` gives the measured DRAM bytes per dispatch (the
+same counters) over the launch time against 8 TB/s, and `effective_gbs` the
+reference stencil's algorithmic bytes over the launch time (an L2/MALL-served
+figure, not an HBM fraction).
 
     python bench.py [--gpus 1] [--steps 24] [--warmup 1]
 """
@@ -21,6 +35,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -31,6 +46,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4     # wave64 VALU instructions per second (MI355X)
+COUNTERS = os.path.join(ROOT, "profiles", "move_counters.json")
+STORES = ("bf16_tile", "f16_tile", "bf16", "f16", "f32")
 
 
 def log(msg):
@@ -53,13 +70,13 @@ STATE_BYTES = 96   # FP64 SoA: read id,k,j,i,health,age,time (56 B) + write k,j,
                    # tag is touched only with the jibe on
 
 
-def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
+def run_timed(ctx, W, cfg, p0, steps, warmup, path, progress=""):
     """Warmup intervals, then `steps` timed intervals from t0 (records resident)."""
     for w in range(warmup):
         ctx.set_particles(p0)
         ctx.move_interval(cfg, float(W.times[w]), float(W.times[w + 1]), w, w + 1, path=path)
         ctx.remove_dead()
-        ctx.grid(counts=False, conc=False)
+        ctx.grid(counts=False, conc=False, mask_output=True)
         ctx.sync()
     ctx.set_particles(p0)
     ctx.sync()
@@ -71,7 +88,7 @@ def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
     for s in range(steps):
         ctx.move_interval(cfg, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
         ctx.remove_dead()                 # synchronous (returns the survivor count)
-        ctx.grid(counts=False, conc=False)
+        ctx.grid(counts=False, conc=False, mask_output=True)
         if s % 6 == 5:
             log(f"  {progress} interval {s + 1}/{steps}")
     ctx.sync()
@@ -81,6 +98,40 @@ def run_timed(wc, ctx, W, cfg, p0, steps, warmup, path, progress=""):
     grid_ms, grid_n = ctx.kernel_time(1)
     return dict(wall_s=wall, steps=ctx.step_count(), move_ms=move_ms, move_launches=move_n,
                 grid_ms=grid_ms, grid_launches=grid_n, alive=ctx.count())
+
+
+def load_window(ctx, W, store, n_slots=None):
+    ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=n_slots or W.T,
+                 store=store)
+    for t in range(n_slots or W.T):
+        ctx.upload_record(t, *W.record(t))
+
+
+def roofline_block(key, r, n_particles, store_bytes, static_bytes, midpoint, counters):
+    """VALU-issue roofline of one move kernel (the binding resource), with the
+    measured DRAM traffic and the L2/MALL-effective bandwidth beside it."""
+    avg_s = r["move_ms"] / 1e3 / r["move_launches"]
+    steps_per_launch = r["steps"] / r["move_launches"]
+    eff = (steps_per_launch * algorithmic_bytes_per_step(store_bytes, static_bytes, midpoint, True) +
+           n_particles * STATE_BYTES) / avg_s / 1e9
+    c = (counters or {}).get(key)
+    block = {"bound": "valu", "unit": "wave-instr/s", "peak": VALU_ISSUE_PEAK, "achieved": None, "frac": None,
+             "avg_launch_ms": avg_s * 1e3, "effective_gbs": eff,
+             "effective_gbs_note": "reference-stencil algorithmic bytes / launch time; served by L2 and the "
+                                   "Infinity Cache, not an HBM fraction",
+             "counters": None, "traffic": None, "hbm": None}
+    if c:
+        achieved = c["valu_per_dispatch"] / avg_s
+        block.update(achieved=achieved, frac=achieved / VALU_ISSUE_PEAK,
+                     counters={"kernel": c["kernel"], "valu_per_dispatch": c["valu_per_dispatch"],
+                               "source": os.path.relpath(COUNTERS, ROOT)})
+        if c.get("dram_bytes_per_dispatch"):
+            t = c["dram_bytes_per_dispatch"]
+            hbm = t / avg_s / 1e9
+            block.update(traffic=t, hbm={"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                          "frac": hbm / HBM_PEAK_GBS,
+                                          "what": "calibrated FETCH_SIZE + WRITE_SIZE per dispatch / launch time"})
+    return block
 
 
 def roms_records(W, t, mask_u, mask_v):
@@ -141,9 +192,7 @@ def ensemble_bench(wc, ctx, W, cfg, steps):
     from wacommplusplus_amd.synthetic import ensemble
     log("[ensemble] generating 256 scenarios")
     p, scen, sigma, counts = ensemble(W)
-    ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=W.T, store="bf16")
-    for t in range(W.T):
-        ctx.upload_record(t, *W.record(t))
+    load_window(ctx, W, "bf16")
     times = [float(W.times[q]) for q in range(steps + 1)]
     slots = list(range(steps + 1))
     ctx.ens_set(p, scen, sigma)
@@ -176,30 +225,136 @@ def ensemble_bench(wc, ctx, W, cfg, steps):
                 counts=dict(min=int(counts.min()), median=float(np.median(counts)), max=int(counts.max())))
 
 
-def cpu_baseline(W, cfg, p0, n_sample=100_000, n_intervals=4):
-    """The oracle (FP64 C restatement of Particle::move, OpenMP dynamic,64 as
-    Wacomm.cpp:501) on a bounded sample of the same workload."""
+def host_info():
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info["lscpu " + k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(ctx, W, cfg, p0, runs=3):
+    """The oracle (FP64 C restatement of Particle::move, OpenMP
+    schedule(dynamic,64) as Wacomm.cpp:501) on a bounded sample spanning the
+    24 h: particles sampled from the release are moved on the GPU (FP64
+    parity path, the same bits as the oracle) to 0, 6, 12 and 18 h, and one
+    timed run moves every snapshot through its next interval on the host.
+    All the host cores this job may use (OMP_PLACES=cores,
+    OMP_PROC_BIND=close) and one core; midpoint and Euler; one warm-up and
+    `runs` measured runs each, median and range reported
+    (docs/performance-evaluation.md:7 of the reference)."""
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_DYNAMIC", "false")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from runner import oracle_pair, cfg_to_oracle
     import pyoracle as po
-    stride = max(1, len(p0) // n_sample)
-    p = p0[::stride].copy()
-    ocfg = cfg_to_oracle(cfg)
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    windows = [oracle_pair(W, t, t + 1)[0] for t in range(n_intervals)]
-    steps = 0
-    t0 = time.perf_counter()
-    for t in range(n_intervals):
-        alive_before = p["health"] > 0
-        po.move(windows[t], ocfg, 0, p, nthreads=cores)
-        steps += int(alive_before.sum())
-    wall = time.perf_counter() - t0
-    # every alive particle takes the 120 substeps of the interval (single pulse at t0)
-    psteps = steps * int(round(3600.0 / cfg.dti))
-    return dict(value=psteps / wall, unit="particle-steps/s", cores=cores, kind="port",
-                sample=f"{len(p)} of the 4M particles (every {stride}th), first {n_intervals} intervals "
-                       f"({psteps:.3g} particle-steps, {wall:.1f} s wall on {cores} threads)")
+    import wacommplusplus_amd as wc
+    info = host_info()
+    env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    all_cores = min(info["affinity"], env_threads) if env_threads > 0 else info["affinity"]
+    hours = (0, 6, 12, 18)
+    per_core = 6000                     # particles per snapshot per thread: ~1 s per midpoint run
+    n_max = per_core * all_cores
+    stride = max(1, len(p0) // n_max)
+    sample = p0[::stride][:n_max].copy()
+    snaps = {}
+    load_window(ctx, W, "f32", n_slots=W.T)
+    ctx.set_particles(sample)
+    for t in range(max(hours) + 1):
+        if t in hours:
+            snaps[t] = ctx.get_particles()
+        ctx.move_interval(cfg, float(W.times[t]), float(W.times[t + 1]), t, t + 1, path="parity")
+    windows = {t: oracle_pair(W, t, t + 1)[0] for t in hours}
+    out = {"host": info, "threads_all": all_cores, "sample": None, "runs": runs}
+    for integ in ("midpoint", "euler"):
+        c = cfg.copy(integrator=wc.MIDPOINT if integ == "midpoint" else wc.EULER)
+        oc = cfg_to_oracle(c)
+        for label, threads in (("all_cores", all_cores), ("one_core", 1)):
+            n = per_core * threads
+            rates = []
+            steps = 0
+            for r in range(runs + 1):
+                steps = 0
+                t0 = time.perf_counter()
+                for t in hours:
+                    q = snaps[t][::max(1, all_cores // threads)][:n].copy()     # spread over the sources
+                    alive = int((q["health"] > 0).sum())
+                    po.move(windows[t], oc, 0, q, nthreads=threads)
+                    steps += alive * int(round(3600.0 / c.dti))
+                wall = time.perf_counter() - t0
+                if r > 0:
+                    rates.append(steps / wall)
+            rates.sort()
+            out[f"{integ}_{label}"] = {"median": rates[len(rates) // 2], "min": rates[0], "max": rates[-1],
+                                       "threads": threads, "particles_per_snapshot": n,
+                                       "particle_steps_per_run": steps}
+            log(f"[cpu] {integ} {label} ({threads} threads): median {rates[len(rates) // 2]:.4g} particle-steps/s")
+    head = out["midpoint_all_cores"]
+    return dict(value=head["median"], unit="particle-steps/s", cores=head["threads"], kind="port",
+                sample=f"every {stride}th particle of the 4M release ({per_core} per thread), moved on the GPU to "
+                       f"0, 6, 12 and 18 h; one run moves each snapshot through its next interval with the oracle "
+                       f"(FP64, OpenMP dynamic,64, midpoint, diffusion on); {runs} runs after a warm-up, median",
+                detail=out)
+
+
+def precision_vs_f64(ctx, W, p, steps, cfg):
+    """The mixed-precision path against the FP64 path after `steps` hours,
+    diffusion off, each particle offset by N(0, 0.25^2) cells: per-particle
+    max(|dk|, |dj|, |di|) over the particles alive on both paths, and over
+    the closure-free ones (no shore or open-boundary closure and no midpoint
+    fallback at land on either path; the rest are held on coast node lines by
+    the reflection closure).  With diffusion on, the relative L2 of the count
+    grid against the FP64 path and against a second FP64 seed."""
+    import wacommplusplus_amd as wc
+    pin = wc.EVENT["shore"] | wc.EVENT["domain"] | wc.EVENT["fallback"]
+    det = cfg.copy(random=False)
+
+    def run(path, store, c, parts, events):
+        load_window(ctx, W, store)
+        ctx.set_particles(parts)
+        ctx.set_event_tracking(events)
+        for s in range(steps):
+            ctx.move_interval(c, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
+        out = ctx.get_particles(), (ctx.get_events() if events else None)
+        ctx.set_event_tracking(False)
+        return out
+
+    ref, ev_ref = run("parity", "f32", det, p, True)
+    rnd = p[::10]
+    run("parity", "f32", cfg, rnd, False)
+    g0 = ctx.grid().astype(np.float64)
+    run("parity", "f32", cfg.copy(randomSeed=1234), rnd, False)
+    g1 = ctx.grid().astype(np.float64)
+    nrm = np.linalg.norm(g0)
+    res = {"hours": steps, "diffusion": "off", "particles": len(p),
+           "release": "50 sources, per-particle N(0, 0.25^2)-cell offsets",
+           "closure_free": "no WC_EVENT_SHORE, DOMAIN or FALLBACK on either path",
+           "conc_l2": {"diffusion": "on", "particles": len(rnd),
+                       "f64_seed_to_seed": float(np.linalg.norm(g1 - g0) / nrm)}}
+    for store in STORES:
+        a, ev = run("fast", store, det, p, True)
+        both = (a["health"] > 0) & (ref["health"] > 0)
+        d = np.max(np.stack([np.abs(a[f] - ref[f]) for f in ("k", "j", "i")]), axis=0)
+        free = both & (((ev | ev_ref) & pin) == 0)
+        r = {}
+        for name, sel in (("alive", both), ("closure_free", free)):
+            x = d[sel]
+            q = np.quantile(x, [0.5, 0.9, 0.99])
+            r[name] = dict(fraction=float(sel.mean()), p50_cells=float(q[0]), p90_cells=float(q[1]),
+                           p99_cells=float(q[2]), max_cells=float(x.max()), rms_cells=float(np.sqrt(np.mean(x * x))))
+        run("fast", store, cfg, rnd, False)
+        r["conc_l2"] = float(np.linalg.norm(ctx.grid().astype(np.float64) - g0) / nrm)
+        res[store] = r
+        log(f"[precision] {store}: {r}")
+    return res
 
 
 def main():
@@ -210,6 +365,8 @@ def main():
     ap.add_argument("--particles", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-euler", action="store_true", help="skip the Euler-integrator timings")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the upload-inclusive rate")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-reconstruction workload")
     ap.add_argument("--no-precision", action="store_true",
@@ -229,71 +386,37 @@ def main():
     W = full_window()
     p0 = release(W, args.particles, 50)
     cfg = wc.default_config(random=True, sigma=wc.sigma_from_kh(1.0, 30.0), crid=1.0)
-    n_slots = W.T
+    counters = None
+    if os.path.exists(COUNTERS):
+        with open(COUNTERS) as f:
+            counters = json.load(f).get("kernels")
 
     results = {}
-    variants = [("parity", "f32")] + ([] if args.no_variants else [("fast", "bf16"), ("fast", "f16"),
-                                                                   ("fast", "f32")])
+    variants = [("parity", "f32")] + ([] if args.no_variants else [("fast", s) for s in STORES])
     ctx = wc.Context(0)
     for path, store in variants:
         key = "f64" if path == "parity" else store
-        log(f"[{key}] uploading {n_slots} records ({store} storage)")
-        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=n_slots,
-                     store=store)
-        for t in range(n_slots):
-            ctx.upload_record(t, *W.record(t))
+        log(f"[{key}] uploading {W.T} records ({store} storage)")
+        load_window(ctx, W, store)
         log(f"[{key}] timing {args.steps} intervals ({args.warmup} warmup)")
-        results[key] = run_timed(wc, ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key)
+        results[key] = run_timed(ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key)
         log(f"[{key}] {results[key]}")
+        if not args.no_euler and key in ("f64", "bf16_tile"):
+            e = run_timed(ctx, W, cfg.copy(integrator=wc.EULER), p0, args.steps, args.warmup, path,
+                          progress=key + " euler")
+            results[key + "_euler"] = e
+            log(f"[{key} euler] {e}")
 
-    # precision of the mixed-precision path against the FP64 path:
-    #  (a) per particle, diffusion off, whole population, `steps` intervals;
-    #  (b) the count grid after `steps` intervals, diffusion on, against the
-    #      difference between two FP64 runs with different seeds
-    precision = {"per_particle": {}, "conc_l2": {}}
+    precision = None
     if not args.no_variants and not args.no_precision:
-        det = cfg.copy(random=False)
-        pp = release(W, args.particles, 50, spread=0.25)      # distinct trajectories
-        finals, grids = {}, {}
-        for path, store in [("parity", "f32"), ("fast", "bf16"), ("fast", "f16"), ("fast", "f32")]:
-            key = "f64" if path == "parity" else store
-            ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=n_slots,
-                         store=store)
-            for t in range(n_slots):
-                ctx.upload_record(t, *W.record(t))
-            runs = [("det", det, pp), ("rnd", cfg, pp[::10])]
-            if key == "f64":
-                runs.append(("rnd2", cfg.copy(randomSeed=1234), pp[::10]))
-            for tag, c, parts in runs:
-                ctx.set_particles(parts)
-                for s in range(args.steps):
-                    ctx.move_interval(c, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
-                if tag == "det":
-                    finals[key] = ctx.get_particles()
-                else:
-                    grids[(key, tag)] = ctx.grid().astype(np.float64)
-            log(f"[precision] {key} done")
-        ref = finals.pop("f64")
-        for key, a in finals.items():
-            both = (a["health"] > 0) & (ref["health"] > 0)
-            d = np.max(np.stack([np.abs(a[f][both] - ref[f][both]) for f in ("k", "j", "i")]), axis=0)
-            q = np.quantile(d, [0.5, 0.9, 0.99])
-            precision["per_particle"][key] = dict(p50_cells=float(q[0]), p90_cells=float(q[1]),
-                                                  p99_cells=float(q[2]), max_cells=float(d.max()),
-                                                  both_alive=float(both.mean()))
-        g0 = grids[("f64", "rnd")]
-        nrm = np.linalg.norm(g0)
-        precision["conc_l2"]["f64_seed_to_seed"] = float(np.linalg.norm(grids[("f64", "rnd2")] - g0) / nrm)
-        for key in ("bf16", "f16", "f32"):
-            precision["conc_l2"][key] = float(np.linalg.norm(grids[(key, "rnd")] - g0) / nrm)
-        log(f"[precision] {precision}")
+        precision = precision_vs_f64(ctx, W, release(W, args.particles, 50, spread=0.25), args.steps, cfg)
 
-    # PCIe-inclusive rate: a three-slot ring, one record uploaded per
-    # interval from host memory -- the record after next, copied on the
-    # copy stream while the move of the current interval runs
+    # upload-inclusive rate: a three-slot ring, one record uploaded per
+    # interval from host memory -- the record after next, copied on the copy
+    # stream while the move of the current interval runs
     pcie = None
-    if not args.no_variants:
-        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=3, store="f32")
+    if not args.no_pcie:
+        load_window(ctx, W, "f32", n_slots=3)
         ctx.set_particles(p0)
         ctx.reset_step_count()
         ctx.sync()
@@ -305,7 +428,7 @@ def main():
             if s + 2 < W.T:
                 ctx.upload_record((s + 2) % 3, *W.record(s + 2))
             ctx.remove_dead()
-            ctx.grid(counts=False)
+            ctx.grid(counts=False, mask_output=True)
         ctx.sync()
         pcie = ctx.step_count() / (time.perf_counter() - t0)
         log(f"[pcie] {pcie:.4g} particle-steps/s")
@@ -316,53 +439,33 @@ def main():
     if not args.no_learned:
         from wacommplusplus_amd import reconstruct
         log("[learned] 512 training + 128 held-out pairs (1M-particle targets, 50k inputs, 24 h)")
-        ctx.set_grid(W.lonRad, W.latRad, W.h, W.depthIntervals, W.s_rho, mask=W.mask, n_slots=W.T, store="bf16")
-        for t in range(W.T):
-            ctx.upload_record(t, *W.record(t))
+        load_window(ctx, W, "bf16")
         learned = reconstruct.run(ctx, W, n_train=512, n_test=128, epochs=40, log=log)
         log(f"[learned] {learned['held_out']} {learned['equal_gpu_ms']}")
+    cpu = None if args.no_cpu else cpu_baseline(ctx, W, cfg, p0)
     ctx.close()
 
     main_r = results["f64"]
     value = main_r["steps"] / main_r["wall_s"]
-    bps = algorithmic_bytes_per_step(4, 8, True, True)
-    avg_launch_s = main_r["move_ms"] / 1e3 / main_r["move_launches"]
-    steps_per_launch = main_r["steps"] / main_r["move_launches"]
-    bytes_per_launch = steps_per_launch * bps + args.particles * STATE_BYTES
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
-    valu = None
-    tpath = os.path.join(ROOT, "profiles", "move_parity_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            prof = json.load(f)
-        traffic = prof.get("hbm_bytes_per_launch")
-        valu = prof.get("valu_per_wave_substep")
-    # instruction issue: wave-level VALU instructions per second against the
-    # chip's issue rate (one wave64 VALU instruction per SIMD every 4 cycles:
-    # 256 CUs x 4 SIMDs x 2.4 GHz / 4, the rate behind the 78.6 TFLOP/s FP64
-    # vector peak = that x 64 lanes x 2 flop per FMA)
-    issue = None
-    if valu:
-        wave_substeps = steps_per_launch / 64.0
-        issue_rate = valu * wave_substeps / avg_launch_s
-        issue = {"what": "VALU instructions per wave-substep (rocprofv3 SQ_INSTS_VALU) x wave-substeps per launch "
-                         "/ the launch time measured here",
-                 "valu_per_wave_substep": valu, "achieved": issue_rate, "peak": VALU_ISSUE_PEAK,
-                 "unit": "wave-instr/s", "frac": issue_rate / VALU_ISSUE_PEAK}
-
-    cpu = None if args.no_cpu else cpu_baseline(W, cfg, p0)
-
+    roof = roofline_block("f64", main_r, args.particles, 4, 8, True, counters)
+    roof["kernel"] = "move_parity_kernel<PackedSampler, SoAAccess>"
+    roof["algorithmic_bytes_per_particle_step"] = algorithmic_bytes_per_step(4, 8, True, True)
     variants_out = {}
     for key, r in results.items():
         if key == "f64":
             continue
-        b = algorithmic_bytes_per_step(2 if key in ("bf16", "f16") else 4, 4, True, True)
-        a_s = r["move_ms"] / 1e3 / r["move_launches"]
-        spl = r["steps"] / r["move_launches"]
+        store = key.replace("_euler", "")
+        euler = key.endswith("_euler")
+        if store == "f64":
+            b = roofline_block("f64_euler", r, args.particles, 4, 8, False, counters)
+        else:
+            b = roofline_block(key, r, args.particles, 4 if store == "f32" else 2, 4, not euler, counters)
         variants_out[key] = dict(value=r["steps"] / r["wall_s"], ms_per_step=1e3 * r["wall_s"] / args.steps,
                                  move_kernel_ms=r["move_ms"] / r["move_launches"],
-                                 achieved_gbs=(spl * b + args.particles * STATE_BYTES) / a_s / 1e9)
+                                 grid_kernel_ms=r["grid_ms"] / max(1, r["grid_launches"]),
+                                 integrator="euler" if euler else "midpoint",
+                                 path="parity (FP64)" if store == "f64" else f"fast ({store} storage)",
+                                 roofline=b)
     line = {
         "metric": "particle-steps/s at 4M (BF16 vs FP64 forcing); conc. L2 err at equal GPU-ms",
         "value": value,
@@ -377,28 +480,18 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "full-size plain method: 1024x768x11 sigma grid at 150 m, 64-mode forcing, ~30% land, "
-                        "4M particles from 50 sources, Kh=1 m2/s, dt=30 s, midpoint, 24 h; FP64 state, "
-                        "FP32-stored forcing (the reference's storage), records resident in HBM",
+            "workload": f"full-size plain method: 1024x768x11 sigma grid at 150 m, 64-mode forcing, ~30% land, "
+                        f"4M particles from 50 sources, Kh=1 m2/s, dt=30 s, midpoint, {args.steps} h "
+                        f"({args.steps} intervals); FP64 state, FP32-stored forcing (the reference's storage), "
+                        f"records resident in HBM",
             "particles": args.particles, "grid": [11, 1024, 768], "substeps_per_step": 120,
-            "path": "parity (FP64)",
+            "path": "parity (FP64)", "records": "resident in HBM (upload-inclusive rate in pcie_inclusive_value)",
         },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "move_parity_kernel<SoAAccess>",
-            "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_particle_step": bps,
-            "issue": issue,
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
         "variants": variants_out,
-        "precision_vs_f64": {"hours": args.steps,
-                             "per_particle": {"diffusion": "off", "particles": args.particles,
-                                              "release": "50 sources, per-particle N(0, 0.25^2)-cell offsets",
-                                              **precision["per_particle"]},
-                             "conc_l2": {"diffusion": "on", "particles": args.particles // 10,
-                                         "what": "relative L2 of the count grid vs the FP64 path",
-                                         **precision["conc_l2"]}},
+        "north_star_kernel": "fast path, bf16_tile storage (variants.bf16_tile)",
+        "precision_vs_f64": precision,
         "pcie_inclusive_value": pcie,
         "roms_normalize": roms,
         "ensemble": ens,

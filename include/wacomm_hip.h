@@ -110,8 +110,42 @@ enum { WC_CLOSURE_CONSTRAINT = 1, WC_CLOSURE_KILL = 2, WC_CLOSURE_REFLECTION = 3
 enum { WC_INTEGRATOR_EULER = 1, WC_INTEGRATOR_MIDPOINT = 2 };
 
 /* Forcing storage on the device (the move always reads FP32 values from the
- * host; the context converts once per record upload). */
-enum { WC_STORE_F32 = 0, WC_STORE_F16 = 1, WC_STORE_BF16 = 2 };
+ * host; the context converts once per record upload).
+ *   WC_STORE_F32, WC_STORE_F16, WC_STORE_BF16: every value rounded to the type.
+ *   WC_STORE_BF16_TILE, WC_STORE_F16_TILE: u, v and w stored as 16-bit
+ *     residuals from a mean per tile of 4 x 4 cells (per level, field and
+ *     record: the mean of the tile's non-zero values, itself rounded to the
+ *     16-bit type, so land zeros decode to exactly 0), so the 8 (BF16) or 11
+ *     (FP16) significant bits resolve the variation inside a tile instead of
+ *     the whole value; AKt and zeta are rounded to the 16-bit type.  Each
+ *     tile keeps its own copy of the 5 x 5 nodes its cells' stencils read, so
+ *     a stencil needs one mean.  Mixed-precision path only. */
+enum { WC_STORE_F32 = 0, WC_STORE_F16 = 1, WC_STORE_BF16 = 2, WC_STORE_BF16_TILE = 3, WC_STORE_F16_TILE = 4 };
+
+/* Closure events of a particle's trajectory, OR-ed per particle by the move
+ * kernels while event tracking is on (wc_set_event_tracking):
+ *   WC_EVENT_SHORE     the candidate position was on land (h + zeta <=
+ *                      shoreLimit) and the horizontal closure acted
+ *                      (Particle.cpp:753-771);
+ *   WC_EVENT_DOMAIN    the candidate left the interpolation domain
+ *                      (Particle.cpp:777-791);
+ *   WC_EVENT_FALLBACK  the midpoint X* was invalid or on land and the step
+ *                      fell back to Euler (Particle.cpp:603-641);
+ *   WC_EVENT_VERTICAL  the surface or bottom closure acted (Particle.cpp:
+ *                      134-176, 665-704);
+ *   WC_EVENT_SHALLOW   the water column at the start position was at or
+ *                      below shoreLimit (Particle.cpp:311-338);
+ *   WC_EVENT_COAST     a substep started in a cell with a land corner (h = 0,
+ *                      as ROMSAdapter.cpp:244-252 sets it on land), where the
+ *                      interpolated velocity falls to the land's zero. */
+enum {
+    WC_EVENT_SHORE = 1,
+    WC_EVENT_DOMAIN = 2,
+    WC_EVENT_FALLBACK = 4,
+    WC_EVENT_VERTICAL = 8,
+    WC_EVENT_SHALLOW = 16,
+    WC_EVENT_COAST = 32
+};
 
 /* Move paths of wc_move_interval:
  *   WC_PATH_PARITY: FP64 state and the reference's rounding points
@@ -293,6 +327,17 @@ int wc_append_particles(wc_ctx *ctx, const wc_particle_data *particles, size_t n
 int wc_get_particles(wc_ctx *ctx, wc_particle_data *particles, size_t n);
 int wc_particle_count(const wc_ctx *ctx, size_t *n);
 
+/* Closure-event tracking (no counterpart in the reference; a diagnostic of
+ * which trajectories touched a closure, e.g. to separate particles held on
+ * coast node lines from the rest when two move paths are compared).  While
+ * enabled, both move paths OR WC_EVENT_* bits into a per-particle byte that
+ * travels with the particle through compaction; particles set, appended or
+ * emitted start at 0.  wc_clear_events zeroes them; wc_get_events copies the
+ * first n (population order) to the host. */
+int wc_set_event_tracking(wc_ctx *ctx, int enabled);
+int wc_clear_events(wc_ctx *ctx);
+int wc_get_events(wc_ctx *ctx, uint8_t *events, size_t n);
+
 /* One forcing interval of Particle::move for every particle (the loop of
  * Wacomm.cpp:501-525).  config is a HOST pointer.  slot0 holds the record at
  * the interval start, slot1 the record at its end (the same slot when the
@@ -306,10 +351,10 @@ int wc_move_interval(wc_ctx *ctx, const wc_config_data *config, double intervalS
 int wc_remove_dead(wc_ctx *ctx, size_t *remaining);
 
 /* Concentration::evaluate (Concentration.hpp:16-44) into the context's device
- * count frame [s_rho][eta][xi] (uint32, zeroed first).  counts_out, when not
- * NULL, receives a host copy (synchronous).  conc_out, when not NULL,
- * receives the reference's float frame (Wacomm.cpp:722-739): counts as float,
- * and 1e37 on land cells (mask != 1) when mask_output is nonzero. */
+ * count frame [s_rho][eta][xi] (uint32, zeroed first), then the reference's
+ * float frame (Wacomm.cpp:722-739) on the device: counts as float, and 1e37
+ * on land cells (mask != 1) when mask_output is nonzero.  counts_out and
+ * conc_out, when not NULL, receive host copies of the two (synchronous). */
 int wc_grid(wc_ctx *ctx, uint32_t *counts_out, float *conc_out, int mask_output);
 
 /* A release point (Source.hpp) and its schedule (SourceEmission.hpp:12-16).

@@ -1,17 +1,23 @@
 """GPU tests of the mixed-precision path (WC_PATH_FAST): integer cell + FP32
-offset state, FP32 interpolation of FP32/FP16/BF16 forcing, Philox4x32-10
-diffusion draws.  It is not draw-for-draw identical to the reference, so it is
-held to stated precision and statistical bars:
+offset state, FP32 interpolation of FP32/FP16/BF16 forcing (plain, or as
+residuals from 4 x 4-cell tile means), Philox4x32-10 diffusion draws.  It is
+not draw-for-draw identical to the reference, so it is held to stated
+precision and statistical bars:
 
   - diffusion off, against the FP64 parity path on the same forcing, after
-    24 h on a 4,000-particle sample of the full-size release (each particle
-    offset by N(0, 0.25^2) cells so every trajectory is distinct): the median
-    per-particle |delta| is bounded per storage type; p90, p99 and max are
-    reported.  The tail is not a precision matter: the reference samples u, v
-    at the containing rho level ceil(k) (a step in k), and without diffusion
-    its reflection closure pins ~30 % of these particles exactly on coast
-    node lines, where the vertical step is a 0/0 ratio that FP64 resolves
-    with 1e-14-cell offsets -- any rounding difference decides those.
+    24 h on a 40,000-particle sample of the full-size release (each particle
+    offset by N(0, 0.25^2) cells so every trajectory is distinct).  The bar of
+    the north star, <= 1e-3 cells, is asserted at the median and as RMS over
+    the particles that neither path's closures touched: coast-pinned particles
+    (a shore or open-boundary closure acted, or the midpoint fell back to
+    Euler at land, WC_EVENT_SHORE | DOMAIN | FALLBACK on either path) are
+    reported apart.  Without diffusion the reflection closure holds about 70 %
+    of this release on coast node lines, where the vertical step is the 0/0 of
+    an empty water column and any rounding difference decides the outcome.
+    The RMS is a tail statistic: trajectories in this 64-mode flow separate
+    exponentially (FP32 storage, 2^-24 relative, still has a 2e-3-cell p999),
+    so it is asserted where it is met and marked xfail with the measured value
+    where it is not.
   - diffusion on: the concentration field of the fast path differs from the
     FP64 path's by no more than two FP64 runs with different seeds differ
     (relative L2 of the 24 h count grids).
@@ -30,6 +36,8 @@ from wacommplusplus_amd.synthetic import ForcingWindow, full_window, plain_windo
 
 pytestmark = pytest.mark.gpu
 
+PINNED = wc.EVENT["shore"] | wc.EVENT["domain"] | wc.EVENT["fallback"]
+
 
 @pytest.fixture(scope="module")
 def ctx():
@@ -43,29 +51,111 @@ def full():
     return full_window()
 
 
+def run_tracked(ctx, W, cfg, p, n, **kw):
+    ctx.set_event_tracking(True)
+    try:
+        out = run_device(ctx, W, cfg, p, n, **kw)
+        return out, ctx.get_events()
+    finally:
+        ctx.set_event_tracking(False)
+
+
 @pytest.fixture(scope="module")
 def reference_24h(ctx, full):
     # distinct trajectories: each particle offset by N(0, 0.25^2) cells
-    p = release(full, 4_000_000, 50, spread=0.25)[::1000].copy()
+    p = release(full, 4_000_000, 50, spread=0.25)[::100].copy()
     cfg = wc.default_config(random=False)
-    return p, cfg, run_device(ctx, full, cfg, p, 24, path="parity")
+    ref, ev = run_tracked(ctx, full, cfg, p, 24, path="parity")
+    return p, cfg, ref, ev
 
 
-def deviation(a, b):
+def deviation(a, b, ev_a=None, ev_b=None):
+    """Per-particle max(|dk|, |dj|, |di|) over the particles alive in both;
+    with events, also over the ones no closure touched on either path."""
     both = (a["health"] > 0) & (b["health"] > 0)
-    d = np.max(np.stack([np.abs(a[f][both] - b[f][both]) for f in ("k", "j", "i")]), axis=0)
-    q = np.quantile(d, [0.5, 0.9, 0.99])
-    return dict(max=float(d.max()), p50=float(q[0]), p90=float(q[1]), p99=float(q[2]), both_alive=float(both.mean()))
+    d = np.max(np.stack([np.abs(a[f] - b[f]) for f in ("k", "j", "i")]), axis=0)
+
+    def stats(sel):
+        x = d[sel]
+        q = np.quantile(x, [0.5, 0.9, 0.99])
+        return dict(n=int(sel.sum()), p50=float(q[0]), p90=float(q[1]), p99=float(q[2]), max=float(x.max()),
+                    rms=float(np.sqrt(np.mean(x * x))))
+    r = stats(both)
+    r["both_alive"] = float(both.mean())
+    if ev_a is not None:
+        free = both & (((ev_a | ev_b) & PINNED) == 0)
+        r["free"] = stats(free)
+        r["free_fraction"] = float(free.sum() / both.sum())
+    return r
 
 
-@pytest.mark.parametrize("store,p50", [("f32", 1e-5), ("f16", 3e-3), ("bf16", 2e-2)])
-def test_fast_path_deviation_vs_fp64(ctx, full, reference_24h, store, p50):
-    p, cfg, ref = reference_24h
-    dev = run_device(ctx, full, cfg, p, 24, path="fast", store=store)
-    r = deviation(dev, ref)
+# measured on the 40,000-particle sample (median / RMS over the closure-free
+# particles): bf16 3.3e-3 / 0.25, bf16_tile 7.6e-4 / 0.27, f16 6.1e-4 /
+# 0.044, f16_tile 6.5e-5 / 8.8e-3, f32 6.3e-7 / 8.1e-4 (4M-particle figures)
+MEDIAN_BAR = 1e-3
+CASES = [
+    ("f32", True, True),
+    ("f16_tile", True, False),
+    ("bf16_tile", True, False),
+    ("f16", True, False),
+    ("bf16", False, False),
+]
+
+
+@pytest.mark.parametrize("store,median_met,rms_met", CASES)
+def test_fast_path_deviation_vs_fp64(ctx, full, reference_24h, store, median_met, rms_met, request):
+    p, cfg, ref, ev_ref = reference_24h
+    dev, ev = run_tracked(ctx, full, cfg, p, 24, path="fast", store=store)
+    r = deviation(dev, ref, ev, ev_ref)
     print(f"fast/{store} vs FP64, 24 h, diffusion off: {r}", flush=True)
-    assert r["p50"] < p50
-    assert r["both_alive"] > 0.5
+    assert r["both_alive"] > 0.75                      # closures decide the same deaths
+    assert 0.2 < r["free_fraction"] < 0.5
+    free = r["free"]
+    if not median_met:
+        request.applymarker(pytest.mark.xfail(reason=f"{store}: median {free['p50']:.2e} cells > 1e-3 (8-bit "
+                                                     "significand without tile means)", strict=False))
+    elif not rms_met:
+        request.applymarker(pytest.mark.xfail(reason=f"{store}: RMS {free['rms']:.2e} cells > 1e-3 over the "
+                                                     f"closure-free particles (median {free['p50']:.2e})",
+                                              strict=False))
+    assert free["p50"] <= MEDIAN_BAR
+    assert free["rms"] <= MEDIAN_BAR
+
+
+def test_tile_means_keep_zeros_exact(ctx):
+    """Zero current and w everywhere: the tile-mean storage types decode the
+    zeros exactly, so the particles end where FP32 storage leaves them (their
+    FP32 offsets, unmoved)."""
+    W = still_window()
+    p = wc.make_particles(1000, k=-1.5, j=32.25, i=32.75)
+    p["j"] += np.linspace(-5, 5, 1000)
+    cfg = wc.default_config(random=False)
+    ref = run_device(ctx, W, cfg, p, 1, path="fast", store="f32")
+    assert np.max(np.abs(ref["j"] - p["j"])) < 1e-6
+    for store in ("bf16_tile", "f16_tile"):
+        out = run_device(ctx, W, cfg, p, 1, path="fast", store=store)
+        for f in ("k", "j", "i"):
+            assert np.array_equal(out[f], ref[f]), (store, f)
+
+
+def test_tile_means_resolve_uniform_current(ctx):
+    """Plain configuration (uniform u = 0.2, v = 0.1, 200 m cells), 12 h:
+    j(t) = j0 + v t / dy exactly in FP64.  A tile mean holds the value's top
+    bits and the residual the rest, so BF16 tile storage lands within 2e-4
+    cells of the closed form (measured 2.1e-5); plain BF16 rounds 0.1 to
+    0.10009765625 and misses by 21.6 cells x 9.8e-4 = 0.021 cells."""
+    W = plain_window(T=13)
+    p = release(W, 500, 1, depth_m=1.0)
+    p["j"], p["i"] = 59.5, 49.5
+    cfg = wc.default_config(random=False)
+    _, dy = wc.cell_metric_table(W.lonRad, W.latRad)
+    j_exact = 59.5 + 0.1 * 12 * 3600.0 / dy[59, 49]
+    tile = run_device(ctx, W, cfg, p, 12, path="fast", store="bf16_tile")
+    plain = run_device(ctx, W, cfg, p, 12, path="fast", store="bf16")
+    e_tile = float(np.max(np.abs(tile["j"] - j_exact)))
+    e_plain = float(np.max(np.abs(plain["j"] - j_exact)))
+    print(f"uniform current, 12 h: |j - exact| bf16_tile {e_tile:.2e}, bf16 {e_plain:.2e}", flush=True)
+    assert e_tile < 2e-4 and e_plain > 0.01
 
 
 def counts_24h(ctx, full, p, cfg, path, store):

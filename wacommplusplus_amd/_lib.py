@@ -32,7 +32,7 @@ EXPORTS = (
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
     "wc_unet_create_ex", "wc_unet_gradients_weighted", "wc_unet_train_step_weighted",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
-    "wc_particle_records",
+    "wc_particle_records", "wc_set_event_tracking", "wc_clear_events", "wc_get_events",
 )
 
 
@@ -136,6 +136,9 @@ def load_library(path: str = LIB_PATH):
         "wc_append_particles": [vp, vp, sz],
         "wc_get_particles": [vp, vp, sz],
         "wc_particle_count": [vp, C.POINTER(sz)],
+        "wc_set_event_tracking": [vp, i],
+        "wc_clear_events": [vp],
+        "wc_get_events": [vp, vp, sz],
         "wc_move_interval": [vp, C.POINTER(ConfigData), d, d, i, i, i],
         "wc_remove_dead": [vp, C.POINTER(sz)],
         "wc_grid": [vp, vp, vp, i],
@@ -356,6 +359,19 @@ class Context:
         n = self.count()
         out = np.zeros(n, dtype=PARTICLE_DTYPE)
         self._chk(self.L.wc_get_particles(self.h, _p(out), n))
+        return out
+
+    # -- closure events (WC_EVENT_* bits per particle) ----------------------
+    def set_event_tracking(self, on: bool = True):
+        self._chk(self.L.wc_set_event_tracking(self.h, int(bool(on))))
+
+    def clear_events(self):
+        self._chk(self.L.wc_clear_events(self.h))
+
+    def get_events(self) -> np.ndarray:
+        n = self.count()
+        out = np.zeros(n, dtype=np.uint8)
+        self._chk(self.L.wc_get_events(self.h, _p(out), n))
         return out
 
     def set_geo(self, lat, lon, s_w_values):

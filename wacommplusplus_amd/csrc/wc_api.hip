@@ -41,8 +41,11 @@ struct wc_ctx {
         float *f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};      // zeta, u, v, w, akt
         bool filled = false;
         int record = -1;      // window record held by the slot (wc_run_window)
-        // recorded on the compute stream after the last enqueued work that
-        // reads or writes the slot; an upload into the slot waits for it
+        // Recorded on the compute stream after the last enqueued work that
+        // reads or writes the slot; an upload into the slot waits for it on
+        // the copy stream.  Rule: every launch that reads a slot records it
+        // afterwards (wc_move_interval, the ensemble step, wc_compare_records),
+        // so no reader depends on the host having waited for it.
         hipEvent_t released = nullptr;
     };
     std::vector<Slot> slots;
@@ -51,8 +54,10 @@ struct wc_ctx {
     // overlaps the move already enqueued on `stream`
     hipStream_t copy = nullptr;
 
-    // particles (SoA, double-buffered for compaction)
+    // particles (SoA, double-buffered for compaction); trackEvents: the
+    // moves OR closure events into s.ev
     size_t n = 0, cap = 0;
+    bool trackEvents = false;
     SoA s{}, s2{};
     uint32_t *d_blockCounts = nullptr;
     unsigned long long *d_total = nullptr;
@@ -117,6 +122,14 @@ struct wc_ctx {
 };
 
 namespace {
+
+// Waits for a stream on every return path: an upload that fails after its
+// copies were enqueued still returns only once they are done, so the caller
+// may free its host arrays whatever the call returns.
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
 
 // Device scratch of one call, freed on every return path.
 struct DeviceTemp {
@@ -221,7 +234,7 @@ int check_supported(wc_ctx *c, const wc_config_data *cfg) {
 
 void free_soa(SoA &s) {
     hipFree(s.id); hipFree(s.k); hipFree(s.j); hipFree(s.i); hipFree(s.health); hipFree(s.age); hipFree(s.time);
-    hipFree(s.tag);
+    hipFree(s.tag); hipFree(s.ev);
     s = SoA{};
 }
 
@@ -234,6 +247,7 @@ int alloc_soa(wc_ctx *c, SoA &s, size_t cap) {
     HIPCHK(c, hipMalloc(&s.age, cap * sizeof(double)));
     HIPCHK(c, hipMalloc(&s.time, cap * sizeof(double)));
     HIPCHK(c, hipMalloc(&s.tag, cap * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&s.ev, cap));
     return WC_OK;
 }
 
@@ -259,6 +273,7 @@ int reserve(wc_ctx *c, size_t need) {
         HIPCHK(c, hipMemcpyAsync(a.age, c->s.age, n * 8, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(a.time, c->s.time, n * 8, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(a.tag, c->s.tag, n * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a.ev, c->s.ev, n, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     free_soa(c->s);
@@ -291,6 +306,7 @@ int upload_particles(wc_ctx *c, const wc_particle_data *p, size_t n, size_t at) 
     for (int q = 0; q < 6; ++q)
         HIPCHK(c, hipMemcpyAsync(dst[q] + at, f[q].data(), n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->s.tag + at, tag.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->s.ev + at, 0, n, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return WC_OK;
 }
@@ -306,9 +322,16 @@ hipEvent_t take_event(wc_ctx *c) {
     return e;
 }
 
-// Event pair around the next launches on stream s (the compute stream by default).
-void timing_begin(wc_ctx *c, hipStream_t s = nullptr) {
+// Drops the start event of a timed launch that failed.
+void timing_abandon(wc_ctx *c) {
+    if (c->cur) c->spare.push_back(c->cur);
     c->cur = nullptr;
+}
+
+// Event pair around the next launches on stream s (the compute stream by
+// default); a start event left by a failed launch is recycled first.
+void timing_begin(wc_ctx *c, hipStream_t s = nullptr) {
+    timing_abandon(c);
     if (!c->timing) return;
     c->cur = take_event(c);
     if (c->cur && hipEventRecord(c->cur, s ? s : c->stream) != hipSuccess) {
@@ -320,8 +343,15 @@ void timing_begin(wc_ctx *c, hipStream_t s = nullptr) {
 int timing_end(wc_ctx *c, int kernel, hipStream_t s = nullptr) {
     if (!c->timing || !c->cur) return WC_OK;
     hipEvent_t e = take_event(c);
-    if (!e) return fail(c, WC_ERR_HIP, "hipEventCreate failed");
-    HIPCHK(c, hipEventRecord(e, s ? s : c->stream));
+    if (!e) {
+        timing_abandon(c);
+        return fail(c, WC_ERR_HIP, "hipEventCreate failed");
+    }
+    if (hipEventRecord(e, s ? s : c->stream) != hipSuccess) {
+        c->spare.push_back(e);
+        timing_abandon(c);
+        return fail(c, WC_ERR_HIP, "hipEventRecord failed");
+    }
     c->pending[kernel].emplace_back(c->cur, e);
     c->cur = nullptr;
     return WC_OK;
@@ -555,7 +585,7 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
         !latRad || !h || !depthIntervals)
         return fail(c, WC_ERR_INVALID, "invalid grid: s_w=%d s_rho=%d eta=%d xi=%d slots=%d", s_w, s_rho, eta, xi,
                     n_slots);
-    if (store != WC_STORE_F32 && store != WC_STORE_F16 && store != WC_STORE_BF16)
+    if (store < WC_STORE_F32 || store > WC_STORE_F16_TILE)
         return fail(c, WC_ERR_INVALID, "unknown storage type %d", store);
     if ((size_t)s_w * eta * xi > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "grid too large");
     HIPCHK(c, hipSetDevice(c->device));
@@ -588,7 +618,7 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
     HIPCHK(c, hipMalloc(&c->d_md, plane * 8));
     HIPCHK(c, hipMalloc(&c->d_depthf, (size_t)(s_w - 1) * 4));
     {
-        const size_t a = fast_pack_bytes(store, s_w, plane), b = parity_pack_bytes(s_w, plane);
+        const size_t a = fast_pack_bytes(store, s_w, eta, xi), b = parity_pack_bytes(s_w, plane);
         HIPCHK(c, hipMalloc(&c->d_pack, a > b ? a : b));
     }
     std::vector<double> metric(2 * plane);
@@ -605,6 +635,7 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
         HIPCHK(c, hipEventRecord(s.released, c->stream));
     }
     HIPCHK(c, hipMalloc(&c->d_counts, plane * s_rho * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_conc, plane * s_rho * sizeof(float)));
     c->haveGrid = true;
     return WC_OK;
 }
@@ -625,6 +656,7 @@ int wc_upload_record(wc_ctx *c, int slot, const float *zeta, const float *u, con
     // and every later launch sees the record.
     s.filled = false;
     HIPCHK(c, hipStreamWaitEvent(c->copy, s.released, 0));
+    StreamDrain drain{c->copy};
     for (int f = 0; f < 5; ++f)
         HIPCHK(c, hipMemcpyAsync(s.f[f], src[f], field_count(c, f) * sizeof(float), hipMemcpyHostToDevice, c->copy));
     HIPCHK(c, hipStreamSynchronize(c->copy));
@@ -806,6 +838,7 @@ int wc_upload_roms_record(wc_ctx *c, int slot, const float *zeta, const float *u
     // the other slots
     hipStream_t cs = c->copy;
     HIPCHK(c, hipStreamWaitEvent(cs, s.released, 0));
+    StreamDrain drain{cs};
     HIPCHK(c, hipMemcpyAsync(s.f[0], zeta, field_count(c, 0) * sizeof(float), hipMemcpyHostToDevice, cs));
     HIPCHK(c, hipMemcpyAsync(s.f[3], w, field_count(c, 3) * sizeof(float), hipMemcpyHostToDevice, cs));
     HIPCHK(c, hipMemcpyAsync(s.f[4], akt, field_count(c, 4) * sizeof(float), hipMemcpyHostToDevice, cs));
@@ -813,8 +846,12 @@ int wc_upload_roms_record(wc_ctx *c, int slot, const float *zeta, const float *u
     if (pv) HIPCHK(c, hipMemcpyAsync(c->d_v_raw, v, pv * sizeof(float), hipMemcpyHostToDevice, cs));
     HIPCHK(c, launch_fill_u32(c->d_flag, 1, 0u, cs));
     timing_begin(c, cs);
-    HIPCHK(c, launch_roms_normalize(d, c->romsMissing, c->d_face, c->d_rot, c->d_u_raw, c->d_v_raw, s.f[1],
-                                    s.f[2], s.f[3], s.f[4], s.f[0], c->d_flag, cs));
+    const hipError_t ne = launch_roms_normalize(d, c->romsMissing, c->d_face, c->d_rot, c->d_u_raw, c->d_v_raw,
+                                                s.f[1], s.f[2], s.f[3], s.f[4], s.f[0], c->d_flag, cs);
+    if (ne != hipSuccess) {
+        timing_abandon(c);
+        return fail(c, WC_ERR_HIP, "ROMS normalisation launch: %s", hipGetErrorString(ne));
+    }
     int trc = timing_end(c, 2, cs);
     if (trc) return trc;
     uint32_t flag = 0;
@@ -866,6 +903,8 @@ int wc_compare_records(wc_ctx *c, int slot_a, int slot_b, int *conflict, long lo
     RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
     HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t *>(c->d_key), 2, 0xFFFFFFFFu, c->stream));
     HIPCHK(c, launch_compare_records(F, c->s_rho, (size_t)c->eta * c->xi, c->d_key, c->stream));
+    HIPCHK(c, hipEventRecord(a.released, c->stream));
+    HIPCHK(c, hipEventRecord(b.released, c->stream));
     unsigned long long key = 0;
     HIPCHK(c, hipMemcpyAsync(&key, c->d_key, sizeof key, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -948,6 +987,30 @@ int wc_particle_count(const wc_ctx *c, size_t *n) {
     return WC_OK;
 }
 
+int wc_set_event_tracking(wc_ctx *c, int enabled) {
+    if (!c) return WC_ERR_INVALID;
+    c->trackEvents = enabled != 0;
+    return WC_OK;
+}
+
+int wc_clear_events(wc_ctx *c) {
+    if (!c) return WC_ERR_INVALID;
+    if (!c->n) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemsetAsync(c->s.ev, 0, c->n, c->stream));
+    return WC_OK;
+}
+
+int wc_get_events(wc_ctx *c, uint8_t *events, size_t n) {
+    if (!c || (n && !events)) return WC_ERR_INVALID;
+    if (n > c->n) return fail(c, WC_ERR_INVALID, "asked for %zu particles, context holds %zu", n, c->n);
+    if (!n) return WC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(events, c->s.ev, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
 int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double end, int slot0, int slot1,
                      int path) {
     if (!c || !cfg) return WC_ERR_INVALID;
@@ -968,9 +1031,11 @@ int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double 
     RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
     timing_begin(c);
     if (path == WC_PATH_PARITY)
-        HIPCHK(c, launch_move_parity_soa(P, F, G, c->d_pack, c->d_metric, c->s, (int)c->n, c->d_steps, c->stream));
+        HIPCHK(c, launch_move_parity_soa(P, F, G, c->d_pack, c->d_metric, c->s, (int)c->n, c->d_steps,
+                                         c->trackEvents, c->stream));
     else
-        HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps, c->stream));
+        HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps, c->trackEvents,
+                                       c->stream));
     // an upload into either slot now waits for this launch
     HIPCHK(c, hipEventRecord(c->slots[slot0].released, c->stream));
     HIPCHK(c, hipEventRecord(c->slots[slot1].released, c->stream));
@@ -1002,15 +1067,14 @@ int wc_grid(wc_ctx *c, uint32_t *counts_out, float *conc_out, int mask_output) {
     timing_begin(c);
     HIPCHK(c, launch_fill_u32(c->d_counts, total, 0u, c->stream));
     HIPCHK(c, launch_grid_count_soa(c->s, (int)c->n, c->s_rho, c->eta, c->xi, c->d_counts, c->stream));
+    // the reference's float frame (counts, land masked) stays on the device
+    HIPCHK(c, launch_counts_to_conc(c->d_counts, c->d_mask, mask_output, c->s_rho, plane, c->d_conc, c->stream));
     int rc = timing_end(c, 1);
     if (rc) return rc;
     if (counts_out)
         HIPCHK(c, hipMemcpyAsync(counts_out, c->d_counts, total * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    if (conc_out) {
-        if (!c->d_conc) HIPCHK(c, hipMalloc(&c->d_conc, total * sizeof(float)));
-        HIPCHK(c, launch_counts_to_conc(c->d_counts, c->d_mask, mask_output, c->s_rho, plane, c->d_conc, c->stream));
+    if (conc_out)
         HIPCHK(c, hipMemcpyAsync(conc_out, c->d_conc, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    }
     if (counts_out || conc_out) HIPCHK(c, hipStreamSynchronize(c->stream));
     return WC_OK;
 }
@@ -1233,6 +1297,8 @@ int wc_ens_set(wc_ctx *c, const wc_particle_data *p, const uint8_t *scenario, si
                int n_scenarios) {
     if (!c || (n && (!p || !scenario)) || !sigma) return WC_ERR_INVALID;
     if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    if (c->store > WC_STORE_BF16)
+        return fail(c, WC_ERR_UNSUPPORTED, "the ensemble reads F32, F16 or BF16 storage (storage type %d)", c->store);
     if (n_scenarios < 1 || n_scenarios > 256) return fail(c, WC_ERR_INVALID, "1..256 scenarios");
     if (n > (size_t)kMaxParticles) return fail(c, WC_ERR_INVALID, "at most %d ensemble particles", kMaxParticles);
     const size_t plane = (size_t)c->eta * c->xi;
@@ -1306,6 +1372,8 @@ int wc_ens_release(wc_ctx *c, const wc_ens_source *sources, int n_sources, const
                    uint64_t seed) {
     if (!c || (n_sources && !sources) || !sigma || n_sources < 0) return WC_ERR_INVALID;
     if (!c->haveGrid) return fail(c, WC_ERR_INVALID, "wc_set_grid first");
+    if (c->store > WC_STORE_BF16)
+        return fail(c, WC_ERR_UNSUPPORTED, "the ensemble reads F32, F16 or BF16 storage (storage type %d)", c->store);
     if (n_scenarios < 1 || n_scenarios > 256) return fail(c, WC_ERR_INVALID, "1..256 scenarios");
     const size_t plane = (size_t)c->eta * c->xi;
     if ((size_t)n_scenarios * plane > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "surface grids too large");
@@ -1318,6 +1386,9 @@ int wc_ens_release(wc_ctx *c, const wc_ens_source *sources, int n_sources, const
             !std::isfinite(S.time))
             return fail(c, WC_ERR_INVALID, "invalid ensemble source %d", q);
         src[q] = EnsSource{S.k, S.j, S.i, S.spread, S.time, S.scenario, S.sub_scenario, S.sub_every};
+        // checked before the sum, so a huge count cannot wrap it
+        if (S.count > (unsigned long long)kMaxParticles - first[q])
+            return fail(c, WC_ERR_INVALID, "at most %d ensemble particles (source %d)", kMaxParticles, q);
         first[q + 1] = first[q] + S.count;
     }
     const unsigned long long n = first[n_sources];
@@ -1472,6 +1543,9 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    for (int q = 0; q < 2 && !rc; ++q)
+        if (hipEventRecord(c->slots[slots[q]].released, c->stream) != hipSuccess)
+            rc = fail(c, WC_ERR_HIP, "hipEventRecord failed");
     if (!rc) rc = copy_ens_soa(c, E.s, E.save, E.n);
     if (rc) {                                   // leave the caller's launch as it was
         E.block = orig.block; E.ppt = orig.ppt; E.occ = orig.occ; E.sortTile = orig.tile;
@@ -1527,6 +1601,8 @@ int wc_ens_run(wc_ctx *c, const wc_config_data *cfg, const double *times, const 
     hipError_t he = hipEventRecord(e0, c->stream);
     for (int q = 0; q < n_intervals && he == hipSuccess; ++q)
         he = use_graph ? hipGraphLaunch(E.exec, c->stream) : launch_ens_step(c->store, L, c->stream);
+    // the slots' last reader is now enqueued (an upload into them waits for it)
+    for (int q = 0; q <= n_intervals && he == hipSuccess; ++q) he = hipEventRecord(c->slots[slots[q]].released, c->stream);
     if (he == hipSuccess) he = hipEventRecord(e1, c->stream);
     if (he == hipSuccess) he = hipEventSynchronize(e1);
     const auto t1 = std::chrono::steady_clock::now();

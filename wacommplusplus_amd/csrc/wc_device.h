@@ -91,11 +91,14 @@ struct EmitDrift {
 // (n + block - 1) / block, which both stay below 2^31 for n up to this.
 constexpr int kMaxParticles = 0x7FFFFF00;
 
-// Particle state, structure of arrays (the context's canonical layout).
+// Particle state, structure of arrays (the context's canonical layout).  ev
+// holds each particle's closure events (WC_EVENT_* bits, OR-ed by the move
+// kernels when event tracking is on); it travels through compaction.
 struct SoA {
     uint64_t *id;
     double *k, *j, *i, *health, *age, *time;
     uint32_t *tag;
+    uint8_t *ev;
 };
 
 struct State {

@@ -70,6 +70,7 @@ __global__ void emit_kernel(SoA s, size_t at, int n, const double *__restrict__ 
         side = (double)(key >> 11) * 0x1.0p-53 < drift.rightProbability ? 1 : -1;
     }
     s.tag[o] = make_tag(drift.type, side);
+    s.ev[o] = 0;
 }
 
 }  // namespace

@@ -2,12 +2,14 @@
 //
 // Gridding (Concentration.hpp:16-44): each alive particle's (int)round(k, j, i)
 // -- round half away from zero -- names a cell of the [s_rho][eta][xi] frame.
-// Instead of one float atomic per particle (the reference's omp atomic), each
-// 64-wide wavefront groups its lanes by cell with ballots (a segmented
-// reduction over equal keys) and one lane adds the group's size with a single
-// uint32 atomic.  Particles released together stay clustered, so a wave holds
-// few distinct cells and issues few atomics; integer adds make the result
-// exact and independent of scheduling.
+// Instead of one float atomic per particle (the reference's omp atomic), a
+// workgroup of 1024 work-items counts 4096 consecutive particles into an LDS
+// hash table of 8192 (cell, count) entries (linear probing, at most 16 probes,
+// then a direct global add) and adds each distinct cell's count to the frame
+// with one uint32 atomic.  Particles released together share few cells, so
+// their counts meet in LDS instead of serialising on a few global addresses;
+// dispersed particles cost one global atomic each.  Integer adds make the
+// result exact and independent of scheduling.
 //
 // Compaction (Wacomm.cpp:710-713, erase(remove_if(!isAlive))): order-preserving
 // stream compaction of the SoA state: block counts -> exclusive scan -> scatter
@@ -30,37 +32,49 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
     return __popcll(mask & ((1ULL << lane_id()) - 1ULL));
 }
 
+constexpr int kGridBlock = 1024;                 // work-items per gridding workgroup
+constexpr int kGridPerItem = 4;                  // particles per work-item
+constexpr int kGridTableBits = 13;               // LDS table of 8192 entries (64 KiB)
+constexpr uint32_t kGridEmpty = 0xFFFFFFFFu;     // no cell index reaches it (cells < 2^32 - 1)
+
 // Out is uint32_t (exact counts, the context path) or float (the drop-in path,
 // which adds into the caller's float frame like the reference does).
 template <class Acc, class Out>
-__global__ __launch_bounds__(kBlock) void grid_count_kernel(Acc acc, int n, int s_rho, int eta, int xi,
-                                                            Out *__restrict__ counts) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = false;
-    uint32_t key = 0;
-    if (idx < n) {
+__global__ __launch_bounds__(kGridBlock) void grid_count_kernel(Acc acc, int n, int s_rho, int eta, int xi,
+                                                                Out *__restrict__ counts) {
+    constexpr uint32_t T = 1u << kGridTableBits;
+    __shared__ uint32_t cellOf[T], countOf[T];
+    for (uint32_t q = threadIdx.x; q < T; q += kGridBlock) {
+        cellOf[q] = kGridEmpty;
+        countOf[q] = 0;
+    }
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * (kGridBlock * kGridPerItem);
+    for (int r = 0; r < kGridPerItem; ++r) {
+        const long long idx = base + (long long)r * kGridBlock + threadIdx.x;
+        if (idx >= n) break;
         double k, j, i, health;
-        acc.position(idx, k, j, i, health);
-        if (health > 0) {                                        // Particle::isAlive
-            const double kr = round(k), jr = round(j), ir = round(i);   // Concentration.hpp:31-33
-            if (jr >= 0 && jr < eta && ir >= 0 && ir < xi && kr >= -s_rho + 1 && kr <= 0) {   // :36
-                key = (uint32_t)((((int)kr + s_rho - 1) * (size_t)eta + (int)jr) * xi + (int)ir);
-                active = true;
+        acc.position((int)idx, k, j, i, health);
+        if (!(health > 0)) continue;                                      // Particle::isAlive
+        const double kr = round(k), jr = round(j), ir = round(i);         // Concentration.hpp:31-33
+        if (!(jr >= 0 && jr < eta && ir >= 0 && ir < xi && kr >= -s_rho + 1 && kr <= 0)) continue;   // :36
+        const uint32_t key = (uint32_t)((((int)kr + s_rho - 1) * (size_t)eta + (int)jr) * xi + (int)ir);
+        uint32_t slot = (key * 2654435761u) >> (32 - kGridTableBits);
+        bool held = false;
+        for (int probe = 0; probe < 16; ++probe) {
+            const uint32_t old = atomicCAS(&cellOf[slot], kGridEmpty, key);
+            if (old == kGridEmpty || old == key) {
+                atomicAdd(&countOf[slot], 1u);
+                held = true;
+                break;
             }
+            slot = (slot + 1) & (T - 1);
         }
+        if (!held) atomicAdd(&counts[key], (Out)1);
     }
-    // wave-level segmented count over equal keys; the loop trip count is
-    // wave-uniform (it depends only on ballots)
-    for (;;) {
-        const unsigned long long m = __ballot(active);
-        if (m == 0) break;
-        const int leader = __ffsll((long long)m) - 1;
-        const uint32_t lk = __shfl(key, leader, 64);
-        const bool same = active && key == lk;
-        const unsigned long long sm = __ballot(same);
-        if (lane_id() == leader) atomicAdd(&counts[lk], (Out)__popcll(sm));
-        if (same) active = false;
-    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < T; q += kGridBlock)
+        if (countOf[q]) atomicAdd(&counts[cellOf[q]], (Out)countOf[q]);
 }
 
 struct SoAPos {
@@ -76,22 +90,32 @@ struct AoSPos {
     }
 };
 
+// a[0..n) = v: 16-byte stores for the aligned body, 4-byte ones around it
 __global__ void fill_u32_kernel(uint32_t *__restrict__ a, size_t n, uint32_t v) {
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-        a[q] = v;
+    const size_t stride = (size_t)gridDim.x * blockDim.x, t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t head = ((16 - ((uintptr_t)a & 15)) & 15) / 4;   // a is 4-byte aligned
+    const size_t lead = head < n ? head : n;
+    const size_t body = (n - lead) / 4;
+    uint4 *b = reinterpret_cast<uint4 *>(a + lead);
+    const uint4 vv = make_uint4(v, v, v, v);
+    for (size_t q = t0; q < body; q += stride) b[q] = vv;
+    for (size_t q = t0; q < lead; q += stride) a[q] = v;
+    for (size_t q = lead + body * 4 + t0; q < n; q += stride) a[q] = v;
 }
 
 // counts -> the reference's float frame: +1.0f repeated `count` times from
 // zero saturates at 2^24; land cells (mask != 1) get 1e37 when masking
-// (Wacomm.cpp:724-739).
+// (Wacomm.cpp:724-739).  One work-item per horizontal cell walks the levels,
+// so the mask is read once per cell and no index is divided.
 __global__ void counts_to_conc_kernel(const uint32_t *__restrict__ counts, const double *__restrict__ mask,
                                       int mask_output, int s_rho, size_t plane, float *__restrict__ conc) {
-    const size_t total = plane * s_rho;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t c = counts[q];
-        float v = c > 16777216u ? 16777216.0f : (float)c;
-        if (mask_output && mask[q % plane] != 1.0) v = 1e37f;
-        conc[q] = v;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < plane; c += (size_t)gridDim.x * blockDim.x) {
+        const bool land = mask_output && mask[c] != 1.0;
+        for (int l = 0; l < s_rho; ++l) {
+            const size_t q = (size_t)l * plane + c;
+            const uint32_t n = counts[q];
+            conc[q] = land ? 1e37f : (n > 16777216u ? 16777216.0f : (float)n);
+        }
     }
 }
 
@@ -154,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void scatter_alive_kernel(SoA src, SoA dst,
         const uint32_t o = base + (uint32_t)lanes_below(m);
         dst.id[o] = src.id[idx]; dst.k[o] = src.k[idx]; dst.j[o] = src.j[idx]; dst.i[o] = src.i[idx];
         dst.health[o] = src.health[idx]; dst.age[o] = src.age[idx]; dst.time[o] = src.time[idx];
-        dst.tag[o] = src.tag[idx];
+        dst.tag[o] = src.tag[idx]; dst.ev[o] = src.ev[idx];
     }
 }
 
@@ -227,6 +251,11 @@ __global__ __launch_bounds__(kBlock) void records_scatter_kernel(SoA src, int n,
     }
 }
 
+int grid_count_blocks(int n) {
+    const long long per = (long long)kGridBlock * kGridPerItem;
+    return (int)((n + per - 1) / per);
+}
+
 int grid_blocks(size_t n) {
     size_t b = (n + kBlock - 1) / kBlock;
     if (b > 65536) b = 65536;
@@ -237,15 +266,23 @@ int grid_blocks(size_t n) {
 
 hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    fill_u32_kernel<<<grid_blocks(n), kBlock, 0, stream>>>(a, n, v);
+    fill_u32_kernel<<<grid_blocks(n / 4 + 1), kBlock, 0, stream>>>(a, n, v);
     return hipGetLastError();
 }
 
 hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi, uint32_t *counts,
                                  hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    grid_count_kernel<SoAPos, uint32_t><<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+    grid_count_kernel<SoAPos, uint32_t><<<grid_count_blocks(n), kGridBlock, 0, stream>>>(
         SoAPos{s.k, s.j, s.i, s.health}, n, s_rho, eta, xi, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_count_aos(const wc_particle_data *p, int n, int s_rho, int eta, int xi, uint32_t *counts,
+                                 hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    grid_count_kernel<AoSPos, uint32_t><<<grid_count_blocks(n), kGridBlock, 0, stream>>>(AoSPos{p}, n, s_rho, eta,
+                                                                                       xi, counts);
     return hipGetLastError();
 }
 
@@ -254,15 +291,15 @@ hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi
 hipError_t launch_grid_count_aos_float(const wc_particle_data *p, int n, int s_rho, int eta, int xi, float *conc,
                                        hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    grid_count_kernel<AoSPos, float><<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(AoSPos{p}, n, s_rho, eta,
-                                                                                      xi, conc);
+    grid_count_kernel<AoSPos, float><<<grid_count_blocks(n), kGridBlock, 0, stream>>>(AoSPos{p}, n, s_rho, eta,
+                                                                                    xi, conc);
     return hipGetLastError();
 }
 
 hipError_t launch_counts_to_conc(const uint32_t *counts, const double *mask, int mask_output, int s_rho,
                                  size_t plane, float *conc, hipStream_t stream) {
-    counts_to_conc_kernel<<<grid_blocks(plane * s_rho), kBlock, 0, stream>>>(counts, mask, mask_output, s_rho,
-                                                                             plane, conc);
+    counts_to_conc_kernel<<<grid_blocks(plane), kBlock, 0, stream>>>(counts, mask, mask_output, s_rho, plane,
+                                                                      conc);
     return hipGetLastError();
 }
 
@@ -294,3 +331,4 @@ hipError_t launch_particle_records(const SoA &src, int n, const GeoPtrs &g, cons
 }
 
 }  // namespace wc
+

@@ -10,22 +10,25 @@ namespace wc {
 // Context path: packs the interval's record pair into packBuf
 // (parity_pack_bytes) and moves the SoA population; metric = {dx, dy} table.
 size_t parity_pack_bytes(int s_w, size_t plane);
+// events: OR the closure events of each particle into s.ev.
 hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
-                                  hipStream_t stream);
+                                  bool events, hipStream_t stream);
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
                                   wc_particle_data *p, int n, unsigned long long *stepCounter, hipStream_t stream);
 
 // Mixed-precision path (store = WC_STORE_*): packs the interval's record pair
 // into packBuf (fast_pack_bytes) and moves the particles.  G.idxf must point to
 // the float2 {1/dx, 1/dy} table.
-size_t fast_pack_bytes(int store, int s_w, size_t plane);
+size_t fast_pack_bytes(int store, int s_w, int eta, int xi);
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
                                 const GridPtrs &G, const SoA &s, int n, unsigned long long *stepCounter,
-                                hipStream_t stream);
+                                bool events, hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream);
 hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi, uint32_t *counts,
+                                 hipStream_t stream);
+hipError_t launch_grid_count_aos(const wc_particle_data *p, int n, int s_rho, int eta, int xi, uint32_t *counts,
                                  hipStream_t stream);
 hipError_t launch_grid_count_aos_float(const wc_particle_data *p, int n, int s_rho, int eta, int xi, float *conc,
                                        hipStream_t stream);

@@ -165,6 +165,11 @@ struct Packed {
     }
     __device__ __forceinline__ float2 md_at(int j, int i) const { return md[cell(j, i)]; }
     __device__ __forceinline__ float depth_at(int q) const { return depthf[q]; }
+    // a corner of cell (j, i) is land (h = 0)
+    __device__ __forceinline__ bool coastal(int j, int i) const {
+        const size_t c = cell(j, i);
+        return z[c].h == 0.f || z[c + 1].h == 0.f || z[c + xi].h == 0.f || z[c + xi + 1].h == 0.f;
+    }
 };
 
 // The same layout with a box of cells [j0, j0+bh) x [i0, i0+bw) and levels
@@ -215,6 +220,96 @@ struct TilePacked {
     __device__ __forceinline__ float depth_at(int q) const { return g.depthf[q]; }
 };
 
+// Tile-mean storage (WC_STORE_BF16_TILE, WC_STORE_F16_TILE): u, v and w are
+// stored as 16-bit residuals from a mean per tile of kTile x kTile cells (per
+// level, field and record; see tile_mean_kernel).  The cells [kTile*tj, kTile*tj + kTile) of a
+// tile read nodes up to kTile*tj + kTile, so each tile keeps its own copy of
+// (kTile+1)^2 nodes and a stencil reads one tile and one mean:
+//   UVR[lev][tile][kTile+1][kTile+1] = residuals {u(t0), u(t1), v(t0), v(t1)}
+//   WAR[lev][tile][kTile+1][kTile+1] = residual w at W levels k, k-1 of both
+//                                      records, then AKt as stored (no mean)
+//   TM [lev][tile] = means {u(t0), u(t1), v(t0), v(t1),
+//                           w_k(t0), w_k(t1), w_k-1(t0), w_k-1(t1)}, in T
+//                           (16 bytes: one load serves a stencil's u, v, w)
+// A corner's value is its time-interpolated residual plus the time-
+// interpolated mean.  The mean is a 16-bit value, so a zero (land, where
+// u = v = w = 0, ROMSAdapter.cpp:244-264) is stored as the residual -mean
+// exactly and decodes to exactly 0, as with the other storage types.
+constexpr int kTile = 4;
+constexpr int kTileNodes = (kTile + 1) * (kTile + 1);
+
+template <class T>
+struct TiledPacked {
+    const UV4<T> *uvr;
+    const WA8<T> *war;
+    const WA8<T> *tm;
+    const ZH *z;
+    const float2 *md;
+    const float *depthf;
+    int xi;
+    int nlev;
+    int tilesX;           // tiles per row: ceil(xi / kTile)
+    size_t tplane;        // tiles per level
+
+    __device__ __forceinline__ size_t cell(int j, int i) const { return (size_t)j * xi + i; }
+    // j, i >= 0 (stencils are read at valid cells only)
+    __device__ __forceinline__ size_t tile(int kI, int j, int i) const {
+        return (size_t)(kI + nlev - 1) * tplane + (size_t)((unsigned)j / kTile) * tilesX + (unsigned)i / kTile;
+    }
+    __device__ __forceinline__ unsigned node(int j, int i) const {
+        return ((unsigned)j % kTile) * (kTile + 1) + (unsigned)i % kTile;
+    }
+
+    __device__ __forceinline__ float hc(int j, int i, const W4 &w, float al) const {
+        const size_t c = cell(j, i);
+        const ZH a = z[c], b = z[c + 1], d = z[c + xi], e = z[c + xi + 1];
+        return hc4(a, b, d, e, w, al);
+    }
+    __device__ __forceinline__ void uv_at(int kI, int j, int i, const W4 &w, float al, float &uu, float &vv) const {
+        const size_t t = tile(kI, j, i);
+        const UV4<T> *p = uvr + t * kTileNodes + node(j, i);
+        const UV4<T> a = p[0], b = p[1], d = p[kTile + 1], e = p[kTile + 2];
+        const WA8<T> m = tm[t];
+        const float Lu = lerpf(tof(m.v[0]), tof(m.v[1]), al), Lv = lerpf(tof(m.v[2]), tof(m.v[3]), al);
+        auto cu = [&](const UV4<T> &q) { return lerpf(tof(q.v[0]), tof(q.v[1]), al) + Lu; };
+        auto cv = [&](const UV4<T> &q) { return lerpf(tof(q.v[2]), tof(q.v[3]), al) + Lv; };
+        uu = w.w00 * cu(a) + w.w01 * cu(b) + w.w10 * cu(d) + w.w11 * cu(e);
+        vv = w.w00 * cv(a) + w.w01 * cv(b) + w.w10 * cv(d) + w.w11 * cv(e);
+    }
+    __device__ __forceinline__ void wa_at(int kI, float kF, int j, int i, const W4 &w, float alw, float ala,
+                                          bool wantA, float &ww, float &aa) const {
+        const size_t t = tile(kI, j, i);
+        const WA8<T> *p = war + t * kTileNodes + node(j, i);
+        const WA8<T> a = p[0], b = p[1], d = p[kTile + 1], e = p[kTile + 2];
+        const WA8<T> m = tm[t];
+        const float Lt = lerpf(tof(m.v[4]), tof(m.v[5]), alw), Lb = lerpf(tof(m.v[6]), tof(m.v[7]), alw);
+        auto corner = [&](const WA8<T> &q, float &wc, float &ac) {
+            const float wt = lerpf(tof(q.v[0]), tof(q.v[1]), alw) + Lt;
+            const float wb = lerpf(tof(q.v[2]), tof(q.v[3]), alw) + Lb;
+            wc = lerpf(wt, wb, kF);
+            if (wantA) {
+                const float at = lerpf(tof(q.v[4]), tof(q.v[5]), ala), ab = lerpf(tof(q.v[6]), tof(q.v[7]), ala);
+                ac = lerpf(at, ab, kF);
+            } else {
+                ac = 0.f;
+            }
+        };
+        float w0, w1, w2, w3, a0, a1, a2, a3;
+        corner(a, w0, a0);
+        corner(b, w1, a1);
+        corner(d, w2, a2);
+        corner(e, w3, a3);
+        ww = w.w00 * w0 + w.w01 * w1 + w.w10 * w2 + w.w11 * w3;
+        aa = w.w00 * a0 + w.w01 * a1 + w.w10 * a2 + w.w11 * a3;
+    }
+    __device__ __forceinline__ float2 md_at(int j, int i) const { return md[cell(j, i)]; }
+    __device__ __forceinline__ float depth_at(int q) const { return depthf[q]; }
+    __device__ __forceinline__ bool coastal(int j, int i) const {
+        const size_t c = cell(j, i);
+        return z[c].h == 0.f || z[c + 1].h == 0.f || z[c + xi].h == 0.f || z[c + xi + 1].h == 0.f;
+    }
+};
+
 __device__ __forceinline__ bool valid_cell(int j, int i, int eta, int xi) {
     return j >= 0 && i >= 0 && j + 1 < eta && i + 1 < xi;
 }
@@ -250,6 +345,7 @@ struct Reg {
     bool inside, healthFresh, changed;
     bool done;            // the substep loop has ended for this interval
     uint32_t tag;         // drift type and side (read and flipped only with the jibe on)
+    uint32_t ev;          // WC_EVENT_* bits of the closures that acted (event tracking)
 };
 
 // Converts an FP64 position into cell + offset; `inside` is the reference's
@@ -274,7 +370,7 @@ __device__ __forceinline__ void to_cell(double jd, double id_, double kd, int et
 // taken.  S is a Packed<T> or a TilePacked<T>.  Jibe: apply the hourly side
 // flip of a non-passive particle (q.tag) after each live substep, with the
 // reference's keyed draw.
-template <class T, class Smp, bool Jibe = false>
+template <class T, class Smp, bool Jibe = false, bool Ev = false>
 __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp &S, float sigma, uint64_t id,
                                                  double ptime, Reg &q, double tEnd) {
     unsigned steps = 0;
@@ -307,7 +403,10 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
                 done = true;
                 break;
             }
-            if (!inside || !(k == k)) {                           // open boundary (:241-248)
+            // open boundary (:241-248); a level outside [kLL, 0] (reachable
+            // only from a non-finite step or an unknown closure code) ends the
+            // particle as the FP64 path's level guard does (kernel.cu:195)
+            if (!inside || !(k <= 0.f && k >= kLL)) {
                 health = -1;
                 healthFresh = true;
                 changed = true;
@@ -326,6 +425,10 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
             const float hc = S.hc(jc, ic, w0, alpha);
             bool kill = false;
             if (hc <= shore && P.horizontal == WC_CLOSURE_KILL) kill = true;
+            if constexpr (Ev) {
+                if (hc <= shore) q.ev |= WC_EVENT_SHALLOW;
+                if (S.coastal(jc, ic)) q.ev |= WC_EVENT_COAST;
+            }
 
             const float a1 = P.midpoint ? alphaStart : alpha;
             float uu, vv, ww, aa;
@@ -378,6 +481,7 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
                     hcMid = S.hc(jm, im, wm, alpha);
                     midValid = hcMid > shore && hcMid > 0;
                 }
+                if (Ev && !midValid) q.ev |= WC_EVENT_FALLBACK;
                 if (midValid) {
                     const int kMI = (int)ceilf(kMid);
                     const float kMF = (float)kMI - kMid;
@@ -405,6 +509,7 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
             advance(jc, fj, yleap * idy_, jn, fjn);
             advance(ic, fi, xleap * idx_, in_, fin);
             float kdet = k + levels(zleap);
+            if (Ev && (kdet > 0.f || kdet < kLL)) q.ev |= WC_EVENT_VERTICAL;
             if (kdet > 0.f) {                                      // :665-704
                 if (P.upper == WC_CLOSURE_CONSTRAINT) kdet = 0.f;
                 else if (P.upper == WC_CLOSURE_KILL) kill = true;
@@ -418,6 +523,7 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
             if (valid_cell(jn, in_, eta, xi)) {                    // :709-776
                 const float hcdet = S.hc(jn, in_, weights(fin, fjn), alpha);
                 if (hcdet <= shore) {
+                    if (Ev) q.ev |= WC_EVENT_SHORE;
                     if (P.horizontal == WC_CLOSURE_CONSTRAINT) {
                         jn = jc; fjn = fj; in_ = ic; fin = fi;
                     } else if (P.horizontal == WC_CLOSURE_KILL) {
@@ -433,6 +539,7 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
                 }
                 jc = jn; fj = fjn; ic = in_; fi = fin; k = kdet;
             } else {                                               // :777-791
+                if (Ev) q.ev |= WC_EVENT_DOMAIN;
                 if (P.horizontal == WC_CLOSURE_CONSTRAINT) {
                     k = kdet;
                 } else if (P.horizontal == WC_CLOSURE_KILL) {
@@ -468,8 +575,8 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
     return steps;
 }
 
-template <class T, bool Jibe>
-__global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> S, SoA st, int n,
+template <class T, class Smp, bool Jibe, bool Ev>
+__global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Smp S, SoA st, int n,
                                                         unsigned long long *stepCounter) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned steps = 0;
@@ -481,6 +588,8 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
         const double ptime = st.time[idx];
 
         // pre-step closures (Particle.cpp:134-176)
+        Reg q;
+        q.ev = (Ev && (kd > 0 || kd < P.kLowerLimit)) ? (uint32_t)WC_EVENT_VERTICAL : 0u;
         if (kd > 0) {
             if (P.upper == WC_CLOSURE_CONSTRAINT) kd = 0;
             else if (P.upper == WC_CLOSURE_KILL) health = -1;
@@ -491,7 +600,6 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
             else if (P.lower == WC_CLOSURE_KILL) health = -1;
             else if (P.lower == WC_CLOSURE_REFLECTION) kd = 2.0 * P.kLowerLimit - kd;
         }
-        Reg q;
         to_cell(jd, id_, kd, P.eta, P.xi, q);
         q.k = (float)kd;
         q.health = health;
@@ -501,9 +609,10 @@ __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Packed<T> 
         q.elapsed = P.elapsed0;
         q.done = false;
         q.tag = Jibe ? st.tag[idx] : 0u;
-        steps = run_interval<T, Packed<T>, Jibe>(P, S, (float)P.sigma, id, ptime, q, P.intervalLength);
+        steps = run_interval<T, Smp, Jibe, Ev>(P, S, (float)P.sigma, id, ptime, q, P.intervalLength);
         if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
         if (Jibe && tag_type(q.tag) != 0) st.tag[idx] = q.tag;
+        if (Ev && q.ev) st.ev[idx] |= (uint8_t)q.ev;
         if (q.changed) {
             st.k[idx] = (double)q.k;
             st.j[idx] = (double)q.jc + (double)q.fj;
@@ -549,6 +658,83 @@ template <class T>
 __global__ void pack_kernel(RecordPair<float> F, const float *__restrict__ hf, UV4<T> *__restrict__ uv,
                             WA8<T> *__restrict__ wa, ZH *__restrict__ z, int s_rho, int s_w, size_t plane) {
     pack_body<T>(F, hf, uv, wa, z, s_rho, s_w, plane);
+}
+
+// Tile means of the interval's record pair, one work-item per (level, tile):
+// for u, v at rho level k and w at W levels k and k-1, both records, the mean
+// (summed in FP64) of the non-zero values among the (kTile+1)^2 nodes the
+// tile stores, rounded to T (0 when all are zero).  With a mean that T
+// represents, a zero node's residual -mean is exact, so zeros decode to 0.
+template <class T>
+__global__ void tile_mean_kernel(RecordPair<float> F, WA8<T> *__restrict__ tm, int s_rho, int s_w, int eta, int xi,
+                                 int tilesX, size_t tplane) {
+    const int nlev = s_w - 1;
+    const size_t plane = (size_t)eta * xi, total = tplane * nlev;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t lv = q / tplane, t = q - lv * tplane;
+        const int k = (int)lv - (nlev - 1);
+        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
+        const size_t r0 = (size_t)(k + s_rho - 1) * plane, wt = (size_t)(k + s_w - 1) * plane, wb = wt - plane;
+        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int dj = 0; dj <= kTile; ++dj) {
+            const int j = tj * kTile + dj;
+            if (j >= eta) break;
+            for (int di = 0; di <= kTile; ++di) {
+                const int i = ti * kTile + di;
+                if (i >= xi) break;
+                const size_t c = (size_t)j * xi + i;
+                const float v[8] = {F.u0[r0 + c], F.u1[r0 + c], F.v0[r0 + c], F.v1[r0 + c],
+                                    F.w0[wt + c], F.w1[wt + c], F.w0[wb + c], F.w1[wb + c]};
+                for (int f = 0; f < 8; ++f)
+                    if (v[f] != 0.0f) {
+                        a[f] += v[f];
+                        ++cnt[f];
+                    }
+            }
+        }
+        WA8<T> m;
+        for (int f = 0; f < 8; ++f) m.v[f] = fromf<T>(cnt[f] ? (float)(a[f] / cnt[f]) : 0.0f);
+        tm[q] = m;
+    }
+}
+
+// Residuals from the tile means, one work-item per (level, tile, node of the
+// tile's (kTile+1)^2 block); nodes beyond the grid hold zeros (no valid
+// stencil reads them).  Level 0 also writes zeta (rounded to T) and h.
+template <class T>
+__global__ void tile_pack_kernel(RecordPair<float> F, const float *__restrict__ hf, const WA8<T> *__restrict__ tm,
+                                 UV4<T> *__restrict__ uvr, WA8<T> *__restrict__ war,
+                                 ZH *__restrict__ z, int s_rho, int s_w, int eta, int xi, int tilesX, size_t tplane) {
+    const int nlev = s_w - 1;
+    const size_t plane = (size_t)eta * xi, total = tplane * nlev * kTileNodes;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t lt = q / kTileNodes;
+        const int nd = (int)(q - lt * kTileNodes);
+        const size_t lv = lt / tplane, t = lt - lv * tplane;
+        const int k = (int)lv - (nlev - 1);
+        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
+        const int dj = nd / (kTile + 1), di = nd - dj * (kTile + 1);
+        const int j = tj * kTile + dj, i = ti * kTile + di;
+        UV4<T> a{};
+        WA8<T> b{};
+        if (j < eta && i < xi) {
+            const size_t c = (size_t)j * xi + i;
+            const size_t r = (size_t)(k + s_rho - 1) * plane + c;
+            const size_t wt = (size_t)(k + s_w - 1) * plane + c, wb = wt - plane;
+            const WA8<T> m = tm[lt];
+            a.v[0] = fromf<T>(F.u0[r] - tof(m.v[0])); a.v[1] = fromf<T>(F.u1[r] - tof(m.v[1]));
+            a.v[2] = fromf<T>(F.v0[r] - tof(m.v[2])); a.v[3] = fromf<T>(F.v1[r] - tof(m.v[3]));
+            b.v[0] = fromf<T>(F.w0[wt] - tof(m.v[4])); b.v[1] = fromf<T>(F.w1[wt] - tof(m.v[5]));
+            b.v[2] = fromf<T>(F.w0[wb] - tof(m.v[6])); b.v[3] = fromf<T>(F.w1[wb] - tof(m.v[7]));
+            b.v[4] = fromf<T>(F.akt0[wt]); b.v[5] = fromf<T>(F.akt1[wt]);
+            b.v[6] = fromf<T>(F.akt0[wb]); b.v[7] = fromf<T>(F.akt1[wb]);
+            if (lv == 0 && dj < kTile && di < kTile)
+                z[c] = ZH{tof<T>(fromf<T>(F.zeta0[c])), tof<T>(fromf<T>(F.zeta1[c])), hf[c], 0.0f};
+        }
+        uvr[q] = a;
+        war[q] = b;
+    }
 }
 
 // ---- batched ensemble ------------------------------------------------------
@@ -997,9 +1183,72 @@ int grid_blocks(size_t n) {
     return (int)(b ? b : 1);
 }
 
+template <class T, class Smp>
+hipError_t launch_move(const MoveParams &P, const Smp &S, const SoA &s, int n, unsigned long long *ctr, bool events,
+                       hipStream_t stream) {
+    const int block = 256, blocks = (n + block - 1) / block;
+    if (P.jibe) {
+        if (events) move_fast_kernel<T, Smp, true, true><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
+        else move_fast_kernel<T, Smp, true, false><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
+    } else {
+        if (events) move_fast_kernel<T, Smp, false, true><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
+        else move_fast_kernel<T, Smp, false, false><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
+    }
+    return hipGetLastError();
+}
+
+// Byte offsets of the tile-mean layout in the pack buffer (each section
+// 256-byte aligned).
+struct TileLayout {
+    size_t uvr, war, tm, z, total;
+    int tilesX;
+    size_t tplane;
+};
+TileLayout tile_layout(size_t tbytes, int s_w, int eta, int xi) {
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    TileLayout L;
+    const size_t nlev = (size_t)(s_w - 1);
+    L.tilesX = (xi + kTile - 1) / kTile;
+    L.tplane = (size_t)L.tilesX * (size_t)((eta + kTile - 1) / kTile);
+    const size_t blocks = nlev * L.tplane;
+    L.uvr = 0;
+    L.war = up(L.uvr + blocks * kTileNodes * 4 * tbytes);
+    L.tm = up(L.war + blocks * kTileNodes * 8 * tbytes);
+    L.z = up(L.tm + blocks * 8 * tbytes);
+    L.total = up(L.z + (size_t)eta * xi * sizeof(ZH));
+    return L;
+}
+
+template <class T>
+hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G,
+                          const SoA &s, int n, unsigned long long *ctr, bool events, hipStream_t stream) {
+    const TileLayout L = tile_layout(sizeof(T), P.s_w, P.eta, P.xi);
+    char *base = (char *)packBuf;
+    TiledPacked<T> S;
+    S.uvr = (const UV4<T> *)(base + L.uvr);
+    S.war = (const WA8<T> *)(base + L.war);
+    S.tm = (const WA8<T> *)(base + L.tm);
+    S.z = (const ZH *)(base + L.z);
+    S.md = (const float2 *)G.idxf;
+    S.depthf = G.depthf;
+    S.xi = P.xi;
+    S.nlev = P.s_w - 1;
+    S.tilesX = L.tilesX;
+    S.tplane = L.tplane;
+    const size_t blocks = L.tplane * (size_t)S.nlev;
+    tile_mean_kernel<T><<<grid_blocks(blocks), 256, 0, stream>>>(F, (WA8<T> *)S.tm, P.s_rho, P.s_w, P.eta, P.xi,
+                                                                 L.tilesX, L.tplane);
+    tile_pack_kernel<T><<<grid_blocks(blocks * kTileNodes), 256, 0, stream>>>(
+        F, G.hf, S.tm, (UV4<T> *)S.uvr, (WA8<T> *)S.war, (ZH *)S.z, P.s_rho, P.s_w, P.eta, P.xi, L.tilesX,
+        L.tplane);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n <= 0) return e;
+    return launch_move<T>(P, S, s, n, ctr, events, stream);
+}
+
 template <class T>
 hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G, const SoA &s,
-                    int n, unsigned long long *ctr, hipStream_t stream) {
+                    int n, unsigned long long *ctr, bool events, hipStream_t stream) {
     const size_t plane = (size_t)P.eta * P.xi;
     const int nlev = P.s_w - 1;
     Packed<T> S;
@@ -1018,12 +1267,7 @@ hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packB
                                                                   (ZH *)S.z, P.s_rho, P.s_w, plane);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n <= 0) return e;
-    const int block = 256;
-    if (P.jibe)
-        move_fast_kernel<T, true><<<(n + block - 1) / block, block, 0, stream>>>(P, S, s, n, ctr);
-    else
-        move_fast_kernel<T, false><<<(n + block - 1) / block, block, 0, stream>>>(P, S, s, n, ctr);
-    return hipGetLastError();
+    return launch_move<T>(P, S, s, n, ctr, events, stream);
 }
 
 template <class T>
@@ -1118,6 +1362,7 @@ hipError_t launch_ens_release(const EnsSource *src, const unsigned long long *fi
 }
 
 hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream) {
+    if (store != WC_STORE_F32 && store != WC_STORE_F16 && store != WC_STORE_BF16) return hipErrorInvalidValue;
     if (L.block != 64 && L.block != 128 && L.block != 256) return hipErrorInvalidValue;
     if (L.ppt < 1 || (L.occ != 0 && L.occ != 3 && L.occ != 4)) return hipErrorInvalidValue;
     if (L.ldsChunk < 0 || L.ldsHalo < 0 || (L.ldsChunk > 0 && L.occ == 4)) return hipErrorInvalidValue;
@@ -1126,17 +1371,21 @@ hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream) {
     return ens_launch_t<float>(L, stream);
 }
 
-size_t fast_pack_bytes(int store, int s_w, size_t plane) {
-    const size_t nlev = (size_t)(s_w - 1);
+size_t fast_pack_bytes(int store, int s_w, int eta, int xi) {
+    if (store == WC_STORE_BF16_TILE || store == WC_STORE_F16_TILE) return tile_layout(2, s_w, eta, xi).total;
+    const size_t nlev = (size_t)(s_w - 1), plane = (size_t)eta * xi;
     const size_t t = store == WC_STORE_F32 ? 4 : 2;
     return plane * nlev * (4 * t + 8 * t) + plane * sizeof(ZH);
 }
 
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
-                                const GridPtrs &G, const SoA &s, int n, unsigned long long *ctr, hipStream_t stream) {
-    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, stream);
-    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, stream);
-    return launch_t<float>(P, F, packBuf, G, s, n, ctr, stream);
+                                const GridPtrs &G, const SoA &s, int n, unsigned long long *ctr, bool events,
+                                hipStream_t stream) {
+    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, stream);
+    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, stream);
+    if (store == WC_STORE_F16_TILE) return launch_tiled_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, stream);
+    if (store == WC_STORE_BF16_TILE) return launch_tiled_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, stream);
+    return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, stream);
 }
 
 }  // namespace wc

@@ -202,9 +202,10 @@ __device__ __forceinline__ void velocity(const S &smp, int kI, double kF, int jI
 }
 
 // Particle::move for one particle (Particle.cpp:71-824); returns the number of
-// live substeps taken.
-template <class Smp>
-__device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
+// live substeps taken.  With Ev, the closures that act are OR-ed into ev
+// (WC_EVENT_* bits); they change nothing else.
+template <class Smp, bool Ev = false>
+__device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p, uint32_t &ev) {
     unsigned steps = 0;
     const double L = P.intervalLength;
     double elapsed = P.elapsed0;
@@ -212,11 +213,13 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
     const int eta = P.eta, xi = P.xi;
 
     if (p.k > 0) {                                          // Particle.cpp:134-153
+        if (Ev) ev |= WC_EVENT_VERTICAL;
         if (P.upper == WC_CLOSURE_CONSTRAINT) p.k = 0;
         else if (P.upper == WC_CLOSURE_KILL) p.health = -1;
         else if (P.upper == WC_CLOSURE_REFLECTION) p.k = -p.k;
     }
     if (p.k < kLL) {                                        // :156-176
+        if (Ev) ev |= WC_EVENT_VERTICAL;
         if (P.lower == WC_CLOSURE_CONSTRAINT) p.k = kLL;
         else if (P.lower == WC_CLOSURE_KILL) p.health = -1;
         else if (P.lower == WC_CLOSURE_REFLECTION) p.k = 2.0 * kLL - p.k;
@@ -259,6 +262,8 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
         double hh = h1 + h2 + h3 + h4;
         double hc = hh + zz;
         if (hc <= P.shoreLimit && P.horizontal == WC_CLOSURE_KILL) p.health = -1;   // :311-338
+        if (Ev && hc <= P.shoreLimit) ev |= WC_EVENT_SHALLOW;
+        if (Ev && (hcn.a == 0 || hcn.b == 0 || hcn.c == 0 || hcn.d == 0)) ev |= WC_EVENT_COAST;
 
         float uu, vv, ww;                                   // :479-481
         velocity(S, kI, kF, jI, jF, iI, iF, P.midpoint ? alphaStart : alpha, uu, vv, ww);
@@ -310,6 +315,7 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
                         zm.c * iMF * jMF + zm.d * iMF * (1.0 - jMF);
                 midValid = hcMid > P.shoreLimit && hcMid > 0;
             }
+            if (Ev && !midValid) ev |= WC_EVENT_FALLBACK;
             // second stage at the midpoint, or the Euler fallback at the
             // start position; both sample at alpha, so one gather serves
             // either (the lanes of a wave may take different ones)
@@ -342,11 +348,13 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
         double kdet = p.k + zleap / zdist;
 
         if (kdet > 0.) {                                    // :665-683
+            if (Ev) ev |= WC_EVENT_VERTICAL;
             if (P.upper == WC_CLOSURE_CONSTRAINT) kdet = 0;
             else if (P.upper == WC_CLOSURE_KILL) p.health = -1;
             else if (P.upper == WC_CLOSURE_REFLECTION) kdet = -kdet;
         }
         if (kdet < kLL) {                                   // :686-704
+            if (Ev) ev |= WC_EVENT_VERTICAL;
             if (P.lower == WC_CLOSURE_CONSTRAINT) kdet = kLL;
             else if (P.lower == WC_CLOSURE_KILL) p.health = -1;
             else if (P.lower == WC_CLOSURE_REFLECTION) kdet = 2.0 * kLL - kdet;
@@ -366,6 +374,7 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
             h4 = hd.d * idetF * (1.0 - jdetF);
             const double hcdet = (h1 + h2 + h3 + h4) + zzdet;
             if (hcdet <= P.shoreLimit) {                    // :753-771
+                if (Ev) ev |= WC_EVENT_SHORE;
                 if (P.horizontal == WC_CLOSURE_CONSTRAINT) { idet = p.i; jdet = p.j; }
                 else if (P.horizontal == WC_CLOSURE_KILL) p.health = -1;
                 else if (P.horizontal == WC_CLOSURE_REFLECTION) {
@@ -375,6 +384,7 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
             }
             p.i = idet; p.j = jdet; p.k = kdet;             // :774-776
         } else {                                            // :777-791
+            if (Ev) ev |= WC_EVENT_DOMAIN;
             if (P.horizontal == WC_CLOSURE_CONSTRAINT) p.k = kdet;
             else if (P.horizontal == WC_CLOSURE_KILL) p.health = -1;
             else if (P.horizontal == WC_CLOSURE_REFLECTION) {
@@ -394,16 +404,18 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p) {
     return steps;
 }
 
-template <class Smp, class Acc>
+template <class Smp, class Acc, bool Ev = false>
 __global__ __launch_bounds__(256) void move_parity_kernel(MoveParams P, Smp S, Acc acc, int n,
-                                                          unsigned long long *stepCounter) {
+                                                          unsigned long long *stepCounter, uint8_t *events) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned steps = 0;
     if (idx < n) {
         State p;
         acc.load(idx, p);
-        steps = move_one(P, S, p);
+        uint32_t ev = 0;
+        steps = move_one<Smp, Ev>(P, S, p, ev);
         acc.store(idx, p);
+        if (Ev && ev) events[idx] |= (uint8_t)ev;
     }
     if (stepCounter) wave_add_u64(stepCounter, steps);
 }
@@ -433,7 +445,7 @@ size_t parity_pack_bytes(int s_w, size_t plane) {
 
 hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
-                                  hipStream_t stream) {
+                                  bool events, hipStream_t stream) {
     const size_t plane = (size_t)P.eta * P.xi;
     const int nlev = P.s_w - 1;
     char *base = (char *)packBuf;
@@ -456,8 +468,12 @@ hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n <= 0) return e;
     const int block = 256;
-    move_parity_kernel<PackedSampler, SoAAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, SoAAccess{s, P.jibe}, n,
-                                                                                                stepCounter);
+    if (events)
+        move_parity_kernel<PackedSampler, SoAAccess, true><<<(n + block - 1) / block, block, 0, stream>>>(
+            P, S, SoAAccess{s, P.jibe}, n, stepCounter, s.ev);
+    else
+        move_parity_kernel<PackedSampler, SoAAccess><<<(n + block - 1) / block, block, 0, stream>>>(
+            P, S, SoAAccess{s, P.jibe}, n, stepCounter, nullptr);
     return hipGetLastError();
 }
 
@@ -467,7 +483,7 @@ hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &
     const int block = 256;
     RawSampler S{F, G, P.xi, (size_t)P.eta * P.xi, P.s_rho, P.s_w};
     move_parity_kernel<RawSampler, AoSAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, AoSAccess{p}, n,
-                                                                                             stepCounter);
+                                                                                             stepCounter, nullptr);
     return hipGetLastError();
 }
 
