@@ -187,6 +187,24 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
 int wc_hip_concentration(const wc_particle_data *particles, int particleCount, int s_rho, int eta, int xi,
                          float *conc, void *stream);
 
+/* Allocation-free gridding.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  The result of wc_grid's float frame for a device
+ * particle_data array: conc[s_rho][eta][xi] (device) is overwritten with the
+ * count of alive particles per cell (Concentration.hpp:16-44), as float
+ * (saturating at 2^24), and 1e37 on land cells (mask != 1) when mask_output is
+ * nonzero (Wacomm.cpp:724-739; mask: device [eta][xi] doubles, required
+ * then).  The exact counts accumulate in a static device buffer of
+ * WC_GRID_NOALLOC_MAX_CELLS uint32 (64 MiB) that each device the library
+ * runs on holds until the process ends; s_rho * eta * xi above it returns
+ * WC_ERR_INVALID.  Calls no hipMalloc, hipFree or allocating routine; the
+ * kernels run asynchronously on stream (a hipStream_t, NULL = default).
+ * Every call on a device shares its buffer (each call clears the cells it
+ * uses first), so calls that overlap on one device race. */
+#define WC_GRID_NOALLOC_MAX_CELLS 16777216
+int wc_grid_noalloc(const wc_particle_data *particles, int particleCount, int s_rho, int eta, int xi,
+                    const double *mask, int mask_output, float *conc, void *stream);
+
 /* ---------------------------------------------------------------------------
  * 2. Native context API
  * ------------------------------------------------------------------------- */

@@ -32,7 +32,7 @@ EXPORTS = (
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
     "wc_unet_create_ex", "wc_unet_gradients_weighted", "wc_unet_train_step_weighted",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
-    "wc_particle_records", "wc_set_event_tracking", "wc_clear_events", "wc_get_events",
+    "wc_particle_records", "wc_set_event_tracking", "wc_clear_events", "wc_get_events", "wc_grid_noalloc",
 )
 
 
@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH):
     sig = {
         "wc_hip_move_particles": [vp, vp, i, i, i, i, i, i] + [vp] * 6 + [vp] * 9 + [i, i, i, vp],
         "wc_hip_concentration": [vp, i, i, i, i, vp, vp],
+        "wc_grid_noalloc": [vp, i, i, i, i, vp, i, vp, vp],
         "wc_ctx_create": [i, C.POINTER(vp)],
         "wc_ctx_destroy": [vp],
         "wc_sync": [vp],

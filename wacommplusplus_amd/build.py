@@ -21,7 +21,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libwacomm_hip.so")
 SOURCES = ["wc_api.hip", "wc_move_parity.hip", "wc_move_fast.hip", "wc_grid.hip", "wc_driver.hip",
            "wc_roms.hip", "wc_unet.hip", "wc_unet_api.hip",
-           "wc_netcdf.hip"]
+           "wc_netcdf.hip", "wc_noalloc.hip"]
 ARCH = os.environ.get("WC_OFFLOAD_ARCH", "gfx950")
 
 
