@@ -163,6 +163,11 @@ enum { WC_PATH_PARITY = 0, WC_PATH_FAST = 1 };
 /* Replaces cudaMoveParticle (cuda/kernel.h:7, cuda/kernel.cu:370-382): moves
  * particleCount particles in place over the forcing interval that starts at
  * times[timeIndex], exactly as Particle::move (Particle.cpp:71-824) does.
+ * The haversine cell sizes (Particle.cpp:544-574) come from a table the
+ * library computes on the host once per lon/lat grid and keeps per device
+ * (a content hash of lon/lat, taken on the device at every call, detects a
+ * new grid), so results equal the CPU path's bit for bit without diffusion;
+ * calls on one device serialise on the host while they enqueue.
  * config, particles, times and all field arrays are DEVICE pointers laid out
  * as the reference's (index4 = ((t*levels + k+levels-1)*eta + j)*xi + i,
  * kernel.cu:63-69).  mask, numThread, numGPU are unused (as in the reference);

@@ -115,6 +115,15 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _records(particles):
+    """particle_data records as the C side reads them: 64-byte, contiguous
+    (numpy can pack a structured array, e.g. in concatenate)."""
+    if particles.dtype.itemsize != 64 or not particles.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"particles must be contiguous 64-byte particle_data records, got {particles.dtype.itemsize}-"
+                         f"byte records (use .astype(PARTICLE_DTYPE))")
+    return _ptr(particles)
+
+
 def _load(path):
     lib = C.CDLL(path)
     lib.wco_mix.restype = C.c_uint64
@@ -272,13 +281,13 @@ def move(window: Window, cfg: Config, ocean_time_idx: int, particles: np.ndarray
     s, e, t0, t1 = window.interval(ocean_time_idx, cfg.trackingDirection, cfg.deltat)
     r0, r1 = window.record(t0), window.record(t1)
     lib(plant).wco_move_all(C.byref(cfg), C.byref(window._grid), s, e, C.byref(r0), C.byref(r1),
-                            _ptr(particles), len(particles), nthreads)
+                            _records(particles), len(particles), nthreads)
     return particles
 
 
 def grid_count(particles: np.ndarray, s_rho: int, eta: int, xi: int, plant: int = 0) -> np.ndarray:
     counts = np.zeros((s_rho, eta, xi), dtype=np.uint32)
-    lib(plant).wco_grid_count(_ptr(particles), len(particles), s_rho, eta, xi, _ptr(counts))
+    lib(plant).wco_grid_count(_records(particles), len(particles), s_rho, eta, xi, _ptr(counts))
     return counts
 
 
@@ -358,6 +367,6 @@ def particle_records(particles: np.ndarray, s_w_values, lat, lon, h):
     n = len(particles)
     src = np.zeros(max(n, 1), np.int64)
     dep, la, lo = (np.zeros(max(n, 1)) for _ in range(3))
-    r = lib().wco_particle_records(_ptr(particles), n, len(sw), eta, xi, _ptr(sw), _ptr(lat), _ptr(lon), _ptr(h),
+    r = lib().wco_particle_records(_records(particles), n, len(sw), eta, xi, _ptr(sw), _ptr(lat), _ptr(lon), _ptr(h),
                                    _ptr(src), _ptr(dep), _ptr(la), _ptr(lo))
     return src[:r], dep[:r], la[:r], lo[:r]

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -374,6 +375,66 @@ int timing_collect(wc_ctx *c) {
     return WC_OK;
 }
 
+// The {dx, dy} table of the drop-in entry point's lon/lat grid, one per
+// device: built on the host with the expression of Particle.cpp:544-574 (the
+// table the context path uses, so the two paths give the same bits) the
+// first time a grid is seen, and reused while the grid's content hash, taken
+// on the device at every call, stays the same.
+struct DropinMetric {
+    unsigned long long *d_hash = nullptr;
+    double2 *table = nullptr;
+    size_t cells = 0;
+    int eta = 0, xi = 0;
+    unsigned long long hash = 0;
+    bool valid = false;
+};
+constexpr int kDropinDevices = 64;
+std::mutex dropinLock;
+DropinMetric dropinMetric[kDropinDevices];
+
+// Caller holds dropinLock.
+int dropin_metric(int eta, int xi, const double *lon, const double *lat, hipStream_t st, const double2 **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return WC_ERR_HIP;
+    if (dev < 0 || dev >= kDropinDevices) return WC_ERR_INVALID;
+    DropinMetric &M = dropinMetric[dev];
+    const size_t plane = (size_t)eta * xi;
+    if (!M.d_hash && hipMalloc(&M.d_hash, sizeof(unsigned long long)) != hipSuccess) return WC_ERR_NOMEM;
+    unsigned long long h = 0;
+    if (launch_fill_u32(reinterpret_cast<uint32_t *>(M.d_hash), 2, 0u, st) != hipSuccess ||
+        launch_grid_hash(lon, lat, plane, M.d_hash, st) != hipSuccess ||
+        hipMemcpyAsync(&h, M.d_hash, sizeof h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return WC_ERR_HIP;
+    if (!(M.valid && M.hash == h && M.eta == eta && M.xi == xi)) {
+        std::vector<double> hlon(plane), hlat(plane), dx(plane), dy(plane);
+        if (hipMemcpy(hlon.data(), lon, plane * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hlat.data(), lat, plane * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            return WC_ERR_HIP;
+        wc_cell_metric_table(eta, xi, hlon.data(), hlat.data(), dx.data(), dy.data());
+        std::vector<double2> tab(plane);
+        for (size_t q = 0; q < plane; ++q) tab[q] = make_double2(dx[q], dy[q]);
+        M.valid = false;
+        // earlier launches may still read the old table
+        if (hipDeviceSynchronize() != hipSuccess) return WC_ERR_HIP;
+        if (M.cells != plane) {
+            hipFree(M.table);
+            M.table = nullptr;
+            M.cells = 0;
+            if (hipMalloc(&M.table, plane * sizeof(double2)) != hipSuccess) return WC_ERR_NOMEM;
+            M.cells = plane;
+        }
+        if (hipMemcpy(M.table, tab.data(), plane * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess)
+            return WC_ERR_HIP;
+        M.hash = h;
+        M.eta = eta;
+        M.xi = xi;
+        M.valid = true;
+    }
+    *out = M.table;
+    return WC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -440,14 +501,18 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
     MoveParams P;
     if (!make_params(&cfg, start, end, sW, sRho, eta, xi, P)) return WC_OK;
     const size_t plane = (size_t)eta * xi;
+    std::lock_guard<std::mutex> guard(dropinLock);
+    const double2 *metric = nullptr;
+    rc = dropin_metric(eta, xi, lonRad, latRad, st, &metric);
+    if (rc) return rc;
     RecordPair<float> F{zeta + plane * timeIndex, zeta + plane * next,
                         u + plane * sRho * timeIndex, u + plane * sRho * next,
                         v + plane * sRho * timeIndex, v + plane * sRho * next,
                         w + plane * sW * timeIndex, w + plane * sW * next,
                         akt + plane * sW * timeIndex, akt + plane * sW * next};
-    GridPtrs G{h, depthIntervals, nullptr, nullptr, lonRad, latRad, nullptr, nullptr, nullptr, nullptr};  // metric inline
-    return launch_move_parity_aos(P, F, G, particles, particleCount, nullptr, st) == hipSuccess ? WC_OK
-                                                                                                : WC_ERR_HIP;
+    GridPtrs G{h, depthIntervals, nullptr, nullptr, lonRad, latRad, nullptr, nullptr, nullptr, nullptr};
+    return launch_move_parity_aos(P, F, G, metric, particles, particleCount, nullptr, st) == hipSuccess ? WC_OK
+                                                                                                        : WC_ERR_HIP;
 }
 
 int wc_hip_concentration(const wc_particle_data *particles, int particleCount, int s_rho, int eta, int xi,

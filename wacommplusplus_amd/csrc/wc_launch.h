@@ -14,8 +14,13 @@ size_t parity_pack_bytes(int s_w, size_t plane);
 hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
                                   bool events, hipStream_t stream);
+// Drop-in path: the reference's array layout; metric = {dx, dy} table.
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
-                                  wc_particle_data *p, int n, unsigned long long *stepCounter, hipStream_t stream);
+                                  const double2 *metric, wc_particle_data *p, int n, unsigned long long *stepCounter,
+                                  hipStream_t stream);
+// *hash += a content hash of lon[0..n) and lat[0..n) (zero it first).
+hipError_t launch_grid_hash(const double *lon, const double *lat, size_t n, unsigned long long *hash,
+                            hipStream_t stream);
 
 // Mixed-precision path (store = WC_STORE_*): packs the interval's record pair
 // into packBuf (fast_pack_bytes) and moves the particles.  G.idxf must point to

@@ -96,10 +96,13 @@ __device__ __forceinline__ float tril(const C4f &t, const C4f &b, double iF, dou
 }
 
 // Sampler over the reference's own array layout (drop-in entry point):
-// index4 = ((t*levels + k+levels-1)*eta + j)*xi + i per record.
+// index4 = ((t*levels + k+levels-1)*eta + j)*xi + i per record.  The cell
+// metric comes from a {dx, dy} table the host computed with the expression
+// of Particle.cpp:544-574 (as for the context path).
 struct RawSampler {
     RecordPair<float> F;
     GridPtrs G;
+    const double2 *md;
     int xi;
     size_t plane;
     int s_rho, s_w;
@@ -126,17 +129,24 @@ struct RawSampler {
         ab = at4(F.akt0, F.akt1, lvl(kI - 1, s_w), j, i, al);
     }
     __device__ __forceinline__ double depth(int kI) const { return G.depth[kI + s_w - 2]; }
-    // haversine cell sizes recomputed with the expression of Particle.cpp:548-574
     __device__ __forceinline__ void metric(int j, int i, double &dx, double &dy) const {
-        const size_t c = (size_t)j * xi + i;
-        double dLat = G.latRad[c + xi] - G.latRad[c];
-        double t = pow(sin(0.5 * dLat), 2);
-        dy = 2.0 * atan2(pow(t, .5), pow(1.0 - t, .5)) * 6371000.0;
-        double dLon = G.lonRad[c + 1] - G.lonRad[c];
-        t = pow(sin(0.5 * dLon), 2) * cos(G.latRad[c + 1]) * cos(G.latRad[c]);
-        dx = 2.0 * atan2(pow(t, .5), pow(1.0 - t, .5)) * 6371000.0;
+        const double2 m = md[(size_t)j * xi + i];
+        dx = m.x;
+        dy = m.y;
     }
 };
+
+// Content hash of a lon/lat grid: sum over cells of a SplitMix64 mix of each
+// value's bits and its index, added once per wave into *hash (zeroed before).
+__global__ void grid_hash_kernel(const double *__restrict__ lon, const double *__restrict__ lat, size_t n,
+                                 unsigned long long *hash) {
+    unsigned long long h = 0;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        h += mix64((uint64_t)__double_as_longlong(lon[q]) ^ (q * 0x9E3779B97F4A7C15ULL));
+        h += mix64((uint64_t)__double_as_longlong(lat[q]) ^ ((q + n) * 0x9E3779B97F4A7C15ULL));
+    }
+    wave_add_u64(hash, h);
+}
 
 // Sampler over the context's packed interval layout: one 16-byte load gives
 // u and v of both records at a corner, one 32-byte load w and AKt of both
@@ -477,11 +487,20 @@ hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &
     return hipGetLastError();
 }
 
+hipError_t launch_grid_hash(const double *lon, const double *lat, size_t n, unsigned long long *hash,
+                            hipStream_t stream) {
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    grid_hash_kernel<<<(int)(blocks ? blocks : 1), 256, 0, stream>>>(lon, lat, n, hash);
+    return hipGetLastError();
+}
+
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
-                                  wc_particle_data *p, int n, unsigned long long *stepCounter, hipStream_t stream) {
+                                  const double2 *metric, wc_particle_data *p, int n, unsigned long long *stepCounter,
+                                  hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int block = 256;
-    RawSampler S{F, G, P.xi, (size_t)P.eta * P.xi, P.s_rho, P.s_w};
+    RawSampler S{F, G, metric, P.xi, (size_t)P.eta * P.xi, P.s_rho, P.s_w};
     move_parity_kernel<RawSampler, AoSAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, AoSAccess{p}, n,
                                                                                              stepCounter, nullptr);
     return hipGetLastError();
