@@ -25,6 +25,9 @@ from wacommplusplus_amd.synthetic import ForcingWindow, full_window, plain_windo
 pytestmark = pytest.mark.gpu
 
 NAN = float("nan")
+# closure events of the device path (WC_EVENT_*), the coast-cell bit aside
+CLOSURE_EVENTS = (wc.EVENT["shore"] | wc.EVENT["domain"] | wc.EVENT["fallback"] | wc.EVENT["vertical"] |
+                  wc.EVENT["shallow"])
 
 
 @pytest.fixture(scope="module")
@@ -221,16 +224,21 @@ def test_random_windows_all_closures(ctx, seed):
                                         horizontalClosure=closures[2], random=random, sigma=30.0,
                                         survprob=1e-4, tau0=20000.0, randomSeed=seed)
                 ref = run_oracle(W, cfg, p, 3)
+                ctx.set_event_tracking(True)
                 dev = run_device(ctx, W, cfg, p, 3)
+                ev = ctx.get_events()
+                ctx.set_event_tracking(False)
                 if not random:
                     assert_same(dev, ref, tol=0.0), case
                     continue
                 ok = np.all(np.stack([close(dev[f], ref[f], 1e-9) for f in ("k", "j", "i")]), axis=0)
                 # an ulp of the device log/cos can move a particle across a
                 # closure threshold (shore limit, land, domain edge); this
-                # window is built to hit closures constantly, so allow 3 %
+                # window is built to hit closures constantly, so allow 3 %,
+                # and every particle beyond 1e-9 took a closure branch
                 print(f"{case}: {ok.mean():.4f} within 1e-9", flush=True)
                 assert ok.mean() >= 0.97, (case, float(ok.mean()), np.where(~ok)[0][:5])
+                assert np.all((ev[~ok] & CLOSURE_EVENTS) != 0), (case, np.where(~ok & (ev == 0))[0][:5])
 
 
 @pytest.mark.parametrize("seed", range(3))
@@ -299,16 +307,21 @@ def test_full_configuration_sample_24h(ctx, full, random):
     p = release(full, 4_000_000, 50)[::1000].copy()          # 4,000 particles, all 50 sources
     cfg = wc.default_config(random=random, sigma=wc.sigma_from_kh(1.0, 30.0), crid=1.0)
     ref = run_oracle(full, cfg, p, 24)
+    ctx.set_event_tracking(True)
     dev = run_device(ctx, full, cfg, p, 24)
+    ev = ctx.get_events()
+    ctx.set_event_tracking(False)
     if not random:
         assert_same(dev, ref, tol=0.0)
     else:
         ok = np.all(np.stack([close(dev[f], ref[f], 1e-9) for f in ("k", "j", "i")]), axis=0)
         # every particle within 1e-9 cells, except where an ulp difference of
-        # the device log/cos flips a closure branch; at most 1 in 1000
+        # the device log/cos flips a closure branch; at most 1 in 1000, and
+        # each of those took a closure branch
         frac = float(np.mean(ok))
         print(f"diffusion on: {frac:.5f} of particles within 1e-9 cells", flush=True)
         assert frac >= 0.999, frac
+        assert np.all((ev[~ok] & CLOSURE_EVENTS) != 0)
 
 
 def test_upload_waits_for_the_move_reading_its_slot(ctx, full):

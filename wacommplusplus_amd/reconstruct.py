@@ -7,7 +7,9 @@ ensemble run moves the N particles; every `sub_every`-th particle (by id)
 belongs to a second scenario, so the same run yields the surface-layer
 histogram of the n = N / sub_every subsample (the input) and of all N
 particles (the target).  Draws are keyed by particle id, so the subsample
-moves exactly as it would alone.  Images are crops around the release point.
+moves exactly as it would alone.  Training images are crops around the
+few-particle histogram's centre of mass; held-out images may be the whole
+surface layer (crop=None).
 
 Network input and target are variance-stabilised: x = sqrt(sub_every * few)
 / scale, t = sqrt(all) / scale (the square root of a Poisson count has
@@ -58,12 +60,15 @@ def ocean_cells(W: ForcingWindow, n: int, margin: int, rng):
 
 
 def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int = 1_000_000, sub_every: int = 20,
-                   hours: int = 24, crop: int = 128, seed: int = 5489, kh_range=(0.1, 10.0), dti: float = 30.0,
-                   depth_m: float = 1.0, batch_pairs: int = 128, frames: int = 1, log=None):
+                   hours: int = 24, crop: int | None = 128, seed: int = 5489, kh_range=(0.1, 10.0),
+                   dti: float = 30.0, depth_m: float = 1.0, batch_pairs: int = 128, frames: int = 1, log=None,
+                   launch=(128, 2, 3, 4)):
     """Surface-layer histograms (few, all) of n_pairs releases after `hours`,
     cropped to crop x crop around the centre of mass of the few-particle
     histogram (what is known at reconstruction time; the release point when
-    it is empty).  With frames > 1 the few-particle histograms of the
+    it is empty), or the whole eta x xi surface with crop=None.  launch:
+    (block, particles per work-item, occupancy hint, sort tile) of the
+    ensemble move, or None for the context's current one.  With frames > 1 the few-particle histograms of the
     frames - 1 hours before are kept too (a low-count run writes hourly
     frames), cropped at the same place.  The context must hold the window's
     grid and records (wc_set_grid with n_slots >= hours + 1).  Returns
@@ -73,16 +78,18 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
     origin [n_pairs][2], seconds,
     particle_steps)."""
     rng = np.random.default_rng(seed)
-    js, is_ = ocean_cells(W, n_pairs, crop // 2 + 2, rng)
+    full = crop is None
+    ch, cw = (W.eta, W.xi) if full else (crop, crop)
+    js, is_ = ocean_cells(W, n_pairs, (128 if full else crop) // 2 + 2, rng)
     kh = np.exp(rng.uniform(math.log(kh_range[0]), math.log(kh_range[1]), n_pairs))
     sigma = np.sqrt(2.0 * kh * dti)
     k0 = depth_to_k(depth_m, float(W.h[W.mask > 0].mean()), W.s_w)
     cfg = default_config(random=True, sigma=1.0, dti=dti)
     times = [float(W.times[q]) for q in range(hours + 1)]
     slots = list(range(hours + 1))
-    few = np.zeros((n_pairs, frames, crop, crop), np.uint32)
-    allc = np.zeros((n_pairs, crop, crop), np.uint32)
-    wet = np.zeros((n_pairs, crop, crop), np.uint8)
+    few = np.zeros((n_pairs, frames, ch, cw), np.uint32)
+    allc = np.zeros((n_pairs, ch, cw), np.uint32)
+    wet = np.zeros((n_pairs, ch, cw), np.uint8)
     origin = np.zeros((n_pairs, 2), np.int64)
     jj, ii = np.arange(W.eta, dtype=np.float64), np.arange(W.xi, dtype=np.float64)
     t0 = time.perf_counter()
@@ -94,6 +101,9 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
                    for q in range(nb)]
         sg = np.repeat(sigma[b0:b0 + nb], 2)
         ctx.ens_release(sources, sg, seed=seed + 7919 * (b0 + 1))
+        if launch is not None:
+            ctx.ens_set_launch(*launch[:3])
+            ctx.ens_set_sort(launch[3], 1)
         # run to hours - (frames - 1), then one hour at a time, keeping the
         # surface counts after each of the last `frames` hours
         first = hours - (frames - 1)
@@ -113,19 +123,19 @@ def generate_pairs(ctx: Context, W: ForcingWindow, n_pairs: int, n_target: int =
             tot = fs.sum()
             cj = (fs.sum(1) @ jj) / tot if tot > 0 else js[b0 + q]
             ci = (fs.sum(0) @ ii) / tot if tot > 0 else is_[b0 + q]
-            r0 = int(np.clip(round(cj) - crop // 2, 0, W.eta - crop))
-            c0 = int(np.clip(round(ci) - crop // 2, 0, W.xi - crop))
+            r0 = 0 if full else int(np.clip(round(cj) - ch // 2, 0, W.eta - ch))
+            c0 = 0 if full else int(np.clip(round(ci) - cw // 2, 0, W.xi - cw))
             origin[b0 + q] = (r0, c0)
-            wet[b0 + q] = W.mask[r0:r0 + crop, c0:c0 + crop] > 0
-            f = surf[2 * q, r0:r0 + crop, c0:c0 + crop]
+            wet[b0 + q] = W.mask[r0:r0 + ch, c0:c0 + cw] > 0
+            f = surf[2 * q, r0:r0 + ch, c0:c0 + cw]
             for fr in range(frames):
-                few[b0 + q, fr] = kept[fr][2 * q, r0:r0 + crop, c0:c0 + crop]
-            allc[b0 + q] = f + surf[2 * q + 1, r0:r0 + crop, c0:c0 + crop]
+                few[b0 + q, fr] = kept[fr][2 * q, r0:r0 + ch, c0:c0 + cw]
+            allc[b0 + q] = f + surf[2 * q + 1, r0:r0 + ch, c0:c0 + cw]
         if log:
             log(f"  pairs {b0 + nb}/{n_pairs}: {steps:.3g} particle-steps so far, last batch {ms:.0f} ms")
     return dict(few=few[:, 0] if frames == 1 else few, all=allc, mask=wet, kh=kh, j=js, i=is_, origin=origin,
                 seconds=time.perf_counter() - t0, particle_steps=steps, n_target=n_target, sub_every=sub_every,
-                hours=hours, frames=frames)
+                hours=hours, frames=frames, full=full)
 
 
 EXTRAS = ("kh", "mask")
@@ -260,17 +270,49 @@ def rel_l2(est, ref):
     return np.linalg.norm(d, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-30)
 
 
-def evaluate(net, data, scale, extras=(), tta=False):
+def windowed(net, few, sub, scale, window, extra=None, tta=False):
+    """Reconstruction of whole-surface histograms few [n][H][W] (or
+    [n][frames][H][W]): the network runs on the window x window crop around
+    the last frame's centre of mass (the plume; the release point is not
+    needed) and plain gridding is the fallback outside it.  Returns the
+    estimates [n][H][W] and the device ms of the forward passes."""
+    last = few[:, 0] if few.ndim == 4 else few
+    n, H, W = last.shape
+    jj, ii = np.arange(H, dtype=np.float64), np.arange(W, dtype=np.float64)
+    est = sub * last.astype(np.float64)
+    org = []
+    for q in range(n):
+        f = last[q].astype(np.float64)
+        tot = f.sum()
+        cj = (f.sum(1) @ jj) / tot if tot > 0 else H / 2
+        ci = (f.sum(0) @ ii) / tot if tot > 0 else W / 2
+        org.append((int(np.clip(round(cj) - window // 2, 0, H - window)),
+                    int(np.clip(round(ci) - window // 2, 0, W - window))))
+    crop = lambda a, q: a[..., org[q][0]:org[q][0] + window, org[q][1]:org[q][1] + window]   # noqa: E731
+    fc = np.stack([crop(few[q], q) for q in range(n)])
+    xc = None if extra is None else np.stack([crop(extra[q], q) for q in range(n)])
+    rec, ms = reconstruct(net, fc, sub, scale, extra=xc, tta=tta)
+    for q in range(n):
+        r0, c0 = org[q]
+        est[q, r0:r0 + window, c0:c0 + window] = rec[q]
+    return est, ms
+
+
+def evaluate(net, data, scale, extras=(), tta=False, window=256):
     """Held-out errors of plain gridding and of the network against the
     N-particle histograms, the targets' own noise, and the estimated errors
-    against the noise-free field.  Crops whose target is empty (every
+    against the noise-free field.  Images whose target is empty (every
     particle left the surface layer or the crop) carry no information and
-    are left out (their number is reported)."""
+    are left out (their number is reported).  Whole-surface images are
+    reconstructed in a window x window plume window (windowed)."""
     keep = data["all"].reshape(len(data["all"]), -1).sum(1) > 0
     few, allc, sub = data["few"][keep], data["all"][keep].astype(np.float64), data["sub_every"]
     extra = extra_planes(data, extras)
     grid = sub * (few[:, 0] if few.ndim == 4 else few).astype(np.float64)      # the last hour's histogram
-    rec, ms = reconstruct(net, few, sub, scale, extra=None if extra is None else extra[keep], tta=tta)
+    if data.get("full"):
+        rec, ms = windowed(net, few, sub, scale, window, extra=None if extra is None else extra[keep], tta=tta)
+    else:
+        rec, ms = reconstruct(net, few, sub, scale, extra=None if extra is None else extra[keep], tta=tta)
     e_grid, e_net = rel_l2(grid, allc), rel_l2(rec, allc)
     e_t = np.sqrt(allc.reshape(len(allc), -1).sum(1)) / np.linalg.norm(allc.reshape(len(allc), -1), axis=1)
     truth = lambda e: np.sqrt(np.maximum(e ** 2 - e_t ** 2, 0.0))   # noqa: E731
@@ -293,7 +335,7 @@ def train_flops_per_image(h: int, w: int) -> float:
 def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, epochs: int = 40,
         n_target: int = 1_000_000, sub_every: int = 20, hours: int = 24, crop: int = 128, batch: int = 16,
         scale: float = 16.0, seed: int = 21, frames: int = 1, extras=("kh", "mask"), loss_weight: str = "none",
-        tta: bool = True, members: int = 1, log=None, pairs=None):
+        tta: bool = True, members: int = 1, log=None, pairs=None, eval_full: bool = False):
     """The whole learned-reconstruction experiment: pairs, training, held-out
     errors, and the comparison with plain gridding at equal GPU time (the
     move of n = n_target / sub_every particles per scenario, batched 256 at a
@@ -304,10 +346,12 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     inference (the single-pass errors are reported beside); members:
     networks trained from different initialisations whose outputs are
     averaged.  pairs: (train, test) data from an earlier call with
-    the same sizes, reused instead of generated."""
+    the same sizes, reused instead of generated.  eval_full: the held-out
+    images are the whole eta x xi surface layer, not crops."""
     if pairs is None:
         tr = generate_pairs(ctx, W, n_train, n_target, sub_every, hours, crop, seed, frames=frames, log=log)
-        te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, crop, seed + 1, frames=frames, log=log)
+        te = generate_pairs(ctx, W, n_test, n_target, sub_every, hours, None if eval_full else crop, seed + 1,
+                            frames=frames, log=log)
     else:
         tr, te = pairs
     X, T = to_net(tr["few"], tr["all"], sub_every, scale, extra_planes(tr, extras))
@@ -320,8 +364,8 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
                         weights=None if loss_weight == "none" else loss_weights(T, loss_weight)))
         nets.append(net)
     h = dict(loss=hs[0]["loss"], seconds=sum(x["seconds"] for x in hs), steps=sum(x["steps"] for x in hs))
-    ev = evaluate(nets, te, scale, extras, tta=tta)
-    single = evaluate(nets[0], te, scale, extras) if (tta or members > 1) else None
+    ev = evaluate(nets, te, scale, extras, tta=tta, window=crop)
+    single = evaluate(nets[0], te, scale, extras, window=crop) if (tta or members > 1) else None
     for net in nets:
         net.close()
     # GPU time of the few-particle move: 256 scenarios of n particles, batched
@@ -340,6 +384,9 @@ def run(ctx: Context, W: ForcingWindow, n_train: int = 512, n_test: int = 128, e
     flops = train_flops_per_image(crop, crop) * h["steps"] * batch
     return dict(
         pairs=dict(train=n_train, test=n_test, n_target=n_target, n_input=n, hours=hours, crop=crop, frames=frames,
+                   held_out_images=(f"whole surface {W.eta}x{W.xi}; the network in the {crop}x{crop} window around "
+                                    "the input's centre of mass, plain gridding outside") if eval_full
+                   else f"{crop}x{crop} crops",
                    extras=list(extras), loss_weight=loss_weight, tta=tta, members=members,
                    seconds=tr["seconds"] + te["seconds"], particle_steps=int(tr["particle_steps"] + te["particle_steps"])),
         training=dict(epochs=epochs, steps=h["steps"], seconds=h["seconds"], loss_first=h["loss"][0],
@@ -389,6 +436,10 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--target", type=int, default=1_000_000)
     ap.add_argument("--sub-every", type=int, default=20)
+    ap.add_argument("--hours", type=int, default=24, help="hours each release is moved")
+    ap.add_argument("--crop", type=int, default=128, help="training crop size (cells)")
+    ap.add_argument("--eval-full", action="store_true",
+                    help="held-out images are the whole 1024 x 768 surface layer instead of crops")
     ap.add_argument("--frames", type=int, default=1, help="hourly few-particle frames the network sees")
     ap.add_argument("--loss-weight", default="none", metavar="SETS",
                     help="loss weighting, none, count or core; several separated by ';' train "
@@ -415,9 +466,9 @@ if __name__ == "__main__":
     weights = [x for x in a.loss_weight.split(";") if x]
     combos = [(st, lw) for st in sets for lw in weights]
     if len(combos) == 1:
-        print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, frames=a.frames,
-                             extras=combos[0][0], loss_weight=combos[0][1], tta=a.tta, members=a.members,
-                             log=say)), flush=True)
+        print(json.dumps(run(c, W, a.train, a.test, a.epochs, a.target, a.sub_every, hours=a.hours, crop=a.crop,
+                             frames=a.frames, extras=combos[0][0], loss_weight=combos[0][1], tta=a.tta,
+                             members=a.members, log=say, eval_full=a.eval_full)), flush=True)
     else:
         pairs = (generate_pairs(c, W, a.train, a.target, a.sub_every, seed=21, frames=a.frames, log=say),
                  generate_pairs(c, W, a.test, a.target, a.sub_every, seed=22, frames=a.frames, log=say))
