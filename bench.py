@@ -70,8 +70,23 @@ STATE_BYTES = 96   # FP64 SoA: read id,k,j,i,health,age,time (56 B) + write k,j,
                    # tag is touched only with the jibe on
 
 
-def run_timed(ctx, W, cfg, p0, steps, warmup, path, progress=""):
-    """Warmup intervals, then `steps` timed intervals from t0 (records resident)."""
+def tune_sort(ctx, W, cfg, p0, path, at=12):
+    """The cell-tile sort (wc_set_sort) the autotuner picks on a representative
+    state: the population moved to `at` h (plumes dispersed), timed on the
+    next interval; the population is then reloaded by the caller."""
+    ctx.set_sort(0)
+    ctx.set_particles(p0)
+    for s in range(at):
+        ctx.move_interval(cfg, float(W.times[s]), float(W.times[s + 1]), s, s + 1, path=path)
+        ctx.remove_dead()
+    tile, _ = ctx.autotune_sort(cfg, float(W.times[at]), float(W.times[at + 1]), at, at + 1, path=path)
+    return tile
+
+
+def run_timed(ctx, W, cfg, p0, steps, warmup, path, progress="", tile=0):
+    """Warmup intervals, then `steps` timed intervals from t0 (records
+    resident), with the cell-tile sort `tile` (0: none)."""
+    ctx.set_sort(tile, 1)
     for w in range(warmup):
         ctx.set_particles(p0)
         ctx.move_interval(cfg, float(W.times[w]), float(W.times[w + 1]), w, w + 1, path=path)
@@ -96,8 +111,11 @@ def run_timed(ctx, W, cfg, p0, steps, warmup, path, progress=""):
     ctx.set_timing(False)
     move_ms, move_n = ctx.kernel_time(0)
     grid_ms, grid_n = ctx.kernel_time(1)
+    sort_ms, sort_n = ctx.kernel_time(3)
+    ctx.set_sort(0)
     return dict(wall_s=wall, steps=ctx.step_count(), move_ms=move_ms, move_launches=move_n,
-                grid_ms=grid_ms, grid_launches=grid_n, alive=ctx.count())
+                grid_ms=grid_ms, grid_launches=grid_n, sort_tile=tile, sort_ms=sort_ms, sort_launches=sort_n,
+                alive=ctx.count())
 
 
 def load_window(ctx, W, store, n_slots=None):
@@ -366,6 +384,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-euler", action="store_true", help="skip the Euler-integrator timings")
+    ap.add_argument("--no-sort", action="store_true", help="no cell-tile sort (skip its autotuning)")
     ap.add_argument("--no-pcie", action="store_true", help="skip the upload-inclusive rate")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-reconstruction workload")
@@ -399,11 +418,16 @@ def main():
         log(f"[{key}] uploading {W.T} records ({store} storage)")
         load_window(ctx, W, store)
         log(f"[{key}] timing {args.steps} intervals ({args.warmup} warmup)")
-        results[key] = run_timed(ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key)
+        results[key + "_unsorted"] = run_timed(ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key)
+        log(f"[{key} unsorted] {results[key + '_unsorted']}")
+        tile = 0 if args.no_sort else tune_sort(ctx, W, cfg, p0, path)
+        log(f"[{key}] autotuned cell-tile sort: {tile}")
+        results[key] = (run_timed(ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key, tile=tile)
+                        if tile else results[key + "_unsorted"])
         log(f"[{key}] {results[key]}")
         if not args.no_euler and key in ("f64", "bf16_tile"):
             e = run_timed(ctx, W, cfg.copy(integrator=wc.EULER), p0, args.steps, args.warmup, path,
-                          progress=key + " euler")
+                          progress=key + " euler", tile=tile)
             results[key + "_euler"] = e
             log(f"[{key} euler] {e}")
 
@@ -447,22 +471,26 @@ def main():
 
     main_r = results["f64"]
     value = main_r["steps"] / main_r["wall_s"]
-    roof = roofline_block("f64", main_r, args.particles, 4, 8, True, counters)
+    roof = roofline_block("f64" if main_r["sort_tile"] == 0 else "f64_sorted", main_r, args.particles, 4, 8, True,
+                          counters)
     roof["kernel"] = "move_parity_kernel<PackedSampler, SoAAccess>"
+    roof["sort_tile"] = main_r["sort_tile"]
     roof["algorithmic_bytes_per_particle_step"] = algorithmic_bytes_per_step(4, 8, True, True)
     variants_out = {}
     for key, r in results.items():
         if key == "f64":
             continue
-        store = key.replace("_euler", "")
+        store = key.replace("_euler", "").replace("_unsorted", "")
         euler = key.endswith("_euler")
+        ckey = key if (r["sort_tile"] == 0 or key.endswith("_unsorted")) else key + "_sorted"
         if store == "f64":
-            b = roofline_block("f64_euler", r, args.particles, 4, 8, False, counters)
+            b = roofline_block(ckey, r, args.particles, 4, 8, not euler, counters)
         else:
-            b = roofline_block(key, r, args.particles, 4 if store == "f32" else 2, 4, not euler, counters)
+            b = roofline_block(ckey, r, args.particles, 4 if store == "f32" else 2, 4, not euler, counters)
         variants_out[key] = dict(value=r["steps"] / r["wall_s"], ms_per_step=1e3 * r["wall_s"] / args.steps,
                                  move_kernel_ms=r["move_ms"] / r["move_launches"],
                                  grid_kernel_ms=r["grid_ms"] / max(1, r["grid_launches"]),
+                                 sort_tile=r["sort_tile"], sort_ms_per_interval=r["sort_ms"] / args.steps,
                                  integrator="euler" if euler else "midpoint",
                                  path="parity (FP64)" if store == "f64" else f"fast ({store} storage)",
                                  roofline=b)
@@ -486,6 +514,7 @@ def main():
                         f"records resident in HBM",
             "particles": args.particles, "grid": [11, 1024, 768], "substeps_per_step": 120,
             "path": "parity (FP64)", "records": "resident in HBM (upload-inclusive rate in pcie_inclusive_value)",
+            "cell_tile_sort": main_r["sort_tile"],
         },
         "roofline": roof,
         "cpu_baseline": cpu,

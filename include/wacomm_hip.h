@@ -358,6 +358,22 @@ int wc_particle_count(const wc_ctx *ctx, size_t *n);
  * emitted start at 0.  wc_clear_events zeroes them; wc_get_events copies the
  * first n (population order) to the host. */
 int wc_set_event_tracking(wc_ctx *ctx, int enabled);
+
+/* Cell-tile order of the moves (no counterpart in the reference): with
+ * tile > 0, wc_move_interval and wc_grid visit the particles tile by tile of
+ * tile x tile cells, so the lanes of a wavefront gather forcing from
+ * neighbouring cells.  The order is a permutation kept beside the population,
+ * which itself stays in the reference's order (every result, and
+ * wc_get_particles, is unchanged); it is recomputed every period-th move and
+ * follows compaction, release and appends.  tile 0 (the default) turns it
+ * off.  wc_autotune_sort times one move over [start, end] with tile 0, 1, 2,
+ * 4 and 8 on the current population (restored afterwards, as is the step
+ * count), keeps the fastest with period 1, and returns it in *tile (and its
+ * time in *best_ms); the sort's own time is timed as kernel group 3. */
+int wc_set_sort(wc_ctx *ctx, int tile, int period);
+int wc_get_sort(wc_ctx *ctx, int *tile, int *period);
+int wc_autotune_sort(wc_ctx *ctx, const wc_config_data *config, double start, double end, int slot0, int slot1,
+                     int path, int *tile, double *best_ms);
 int wc_clear_events(wc_ctx *ctx);
 int wc_get_events(wc_ctx *ctx, uint8_t *events, size_t n);
 
@@ -434,7 +450,8 @@ int wc_run_window(wc_ctx *ctx, const wc_config_data *config, const wc_window *wi
 
 /* Accumulated device time of the move kernels launched since the last reset,
  * measured with HIP events on the context stream around each launch (enable
- * with wc_set_timing).  kernel: 0 = move, 1 = grid, 2 = ROMS normalisation. */
+ * with wc_set_timing).  kernel: 0 = move, 1 = grid, 2 = ROMS normalisation,
+ * 3 = the cell-tile sort of wc_set_sort. */
 int wc_set_timing(wc_ctx *ctx, int enabled);
 int wc_kernel_time(wc_ctx *ctx, int kernel, double *total_ms, long *launches);
 int wc_reset_kernel_time(wc_ctx *ctx);

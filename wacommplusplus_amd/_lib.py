@@ -33,6 +33,7 @@ EXPORTS = (
     "wc_unet_create_ex", "wc_unet_gradients_weighted", "wc_unet_train_step_weighted",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
     "wc_particle_records", "wc_set_event_tracking", "wc_clear_events", "wc_get_events", "wc_grid_noalloc",
+    "wc_set_sort", "wc_get_sort", "wc_autotune_sort",
 )
 
 
@@ -138,6 +139,9 @@ def load_library(path: str = LIB_PATH):
         "wc_get_particles": [vp, vp, sz],
         "wc_particle_count": [vp, C.POINTER(sz)],
         "wc_set_event_tracking": [vp, i],
+        "wc_set_sort": [vp, i, i],
+        "wc_get_sort": [vp, C.POINTER(i), C.POINTER(i)],
+        "wc_autotune_sort": [vp, C.POINTER(ConfigData), d, d, i, i, i, C.POINTER(i), C.POINTER(d)],
         "wc_clear_events": [vp],
         "wc_get_events": [vp, vp, sz],
         "wc_move_interval": [vp, C.POINTER(ConfigData), d, d, i, i, i],
@@ -361,6 +365,21 @@ class Context:
         out = np.zeros(n, dtype=PARTICLE_DTYPE)
         self._chk(self.L.wc_get_particles(self.h, _p(out), n))
         return out
+
+    # -- cell-tile order of the moves ----------------------------------------
+    def set_sort(self, tile: int, period: int = 1):
+        self._chk(self.L.wc_set_sort(self.h, int(tile), int(period)))
+
+    def get_sort(self):
+        t, p = C.c_int(), C.c_int()
+        self._chk(self.L.wc_get_sort(self.h, C.byref(t), C.byref(p)))
+        return t.value, p.value
+
+    def autotune_sort(self, config: ConfigData, start: float, end: float, slot0: int, slot1: int, path="parity"):
+        t, ms = C.c_int(), C.c_double()
+        self._chk(self.L.wc_autotune_sort(self.h, C.byref(config), start, end, slot0, slot1, PATH[path],
+                                          C.byref(t), C.byref(ms)))
+        return t.value, ms.value
 
     # -- closure events (WC_EVENT_* bits per particle) ----------------------
     def set_event_tracking(self, on: bool = True):

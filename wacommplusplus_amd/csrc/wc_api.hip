@@ -20,8 +20,8 @@
 
 using namespace wc;
 
-// timed kernel groups: 0 move, 1 grid, 2 ROMS normalisation
-constexpr int kTimedKernels = 3;
+// timed kernel groups: 0 move, 1 grid, 2 ROMS normalisation, 3 cell-tile sort
+constexpr int kTimedKernels = 4;
 
 struct wc_ctx {
     int device = 0;
@@ -59,6 +59,15 @@ struct wc_ctx {
     // moves OR closure events into s.ev
     size_t n = 0, cap = 0;
     bool trackEvents = false;
+    // cell-tile order of the population (wc_set_sort): perm lists the
+    // population's indices tile by tile; the moves and the gridding visit the
+    // particles in that order.  perm2 and remap are scratch of the sort and
+    // of compaction (which keeps perm valid); sortBins: tiles + 1 counters.
+    uint32_t *d_perm = nullptr, *d_perm2 = nullptr, *d_remap = nullptr, *d_sortBins = nullptr;
+    uint32_t *d_scanSums = nullptr;       // kScanSumsCap entries: scratch of the multi-block scans
+    int sortTile = 0, sortPeriod = 1;
+    long sortCount = 0;
+    bool permValid = false;
     SoA s{}, s2{};
     uint32_t *d_blockCounts = nullptr;
     unsigned long long *d_total = nullptr;
@@ -280,11 +289,17 @@ int reserve(wc_ctx *c, size_t need) {
     free_soa(c->s);
     free_soa(c->s2);
     hipFree(c->d_blockCounts);
+    hipFree(c->d_perm); hipFree(c->d_perm2); hipFree(c->d_remap);
     c->d_blockCounts = nullptr;
+    c->d_perm = c->d_perm2 = c->d_remap = nullptr;
+    c->permValid = false;
     c->s = a;
     c->s2 = b;
     c->cap = cap;
     HIPCHK(c, hipMalloc(&c->d_blockCounts, ((cap + 255) / 256) * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_perm, cap * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_perm2, cap * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_remap, cap * sizeof(uint32_t)));
     return WC_OK;
 }
 
@@ -372,6 +387,24 @@ int timing_collect(wc_ctx *c) {
         }
         c->pending[k].clear();
     }
+    return WC_OK;
+}
+
+// Brings the cell-tile order up to date before a move (wc_set_sort): a new
+// sort every sortPeriod-th move, or when the population changed in a way
+// the order does not follow (set, a new grid).  Timed as kernel group 3.
+int sort_population(wc_ctx *c) {
+    if (c->sortTile <= 0 || c->n == 0) return WC_OK;
+    if (!c->permValid || c->sortCount % c->sortPeriod == 0) {
+        timing_begin(c);
+        HIPCHK(c, launch_sort_perm(c->s, (int)c->n, c->eta, c->xi, c->sortTile, c->permValid ? c->d_perm : nullptr,
+                                   c->d_perm2, c->d_sortBins, c->d_total, c->d_scanSums, c->stream));
+        std::swap(c->d_perm, c->d_perm2);
+        c->permValid = true;
+        const int rc = timing_end(c, 3);
+        if (rc) return rc;
+    }
+    ++c->sortCount;
     return WC_OK;
 }
 
@@ -543,6 +576,7 @@ int wc_ctx_create(int device, wc_ctx **out) {
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipMalloc(&c->d_total, sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&c->d_steps, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_scanSums, kScanSumsCap * sizeof(uint32_t)) != hipSuccess ||
         hipMemset(c->d_steps, 0, sizeof(unsigned long long)) != hipSuccess) {
         wc_ctx_destroy(c);
         return WC_ERR_HIP;
@@ -621,6 +655,8 @@ int wc_ctx_destroy(wc_ctx *c) {
     free_soa(c->s);
     free_soa(c->s2);
     hipFree(c->d_blockCounts);
+    hipFree(c->d_perm); hipFree(c->d_perm2); hipFree(c->d_remap);
+    hipFree(c->d_scanSums);
     hipFree(c->d_total);
     hipFree(c->d_steps);
     hipFree(c->d_rec);
@@ -701,6 +737,8 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
     }
     HIPCHK(c, hipMalloc(&c->d_counts, plane * s_rho * sizeof(uint32_t)));
     HIPCHK(c, hipMalloc(&c->d_conc, plane * s_rho * sizeof(float)));
+    HIPCHK(c, hipMalloc(&c->d_sortBins, (plane + 1) * sizeof(uint32_t)));     // tiles of one cell, plus one
+    c->permValid = false;
     c->haveGrid = true;
     return WC_OK;
 }
@@ -999,6 +1037,7 @@ int wc_set_particles(wc_ctx *c, const wc_particle_data *p, size_t n) {
     int rc = reserve(c, n);
     if (rc) return rc;
     c->n = 0;
+    c->permValid = false;
     if (n) {
         rc = upload_particles(c, p, n, 0);
         if (rc) return rc;
@@ -1015,6 +1054,7 @@ int wc_append_particles(wc_ctx *c, const wc_particle_data *p, size_t n) {
     if (rc) return rc;
     rc = upload_particles(c, p, n, c->n);
     if (rc) return rc;
+    if (c->permValid) HIPCHK(c, launch_perm_fill(c->d_perm, c->n, c->n + n, c->stream));   // appended last
     c->n += n;
     return WC_OK;
 }
@@ -1066,6 +1106,89 @@ int wc_clear_events(wc_ctx *c) {
     return WC_OK;
 }
 
+int wc_set_sort(wc_ctx *c, int tile, int period) {
+    if (!c) return WC_ERR_INVALID;
+    if (tile < 0 || tile > 4096 || period < 1) return fail(c, WC_ERR_INVALID, "sort tile %d / period %d", tile, period);
+    c->sortTile = tile;
+    c->sortPeriod = period;
+    c->sortCount = 0;
+    c->permValid = false;
+    return WC_OK;
+}
+
+int wc_get_sort(wc_ctx *c, int *tile, int *period) {
+    if (!c) return WC_ERR_INVALID;
+    if (tile) *tile = c->sortTile;
+    if (period) *period = c->sortPeriod;
+    return WC_OK;
+}
+
+int wc_autotune_sort(wc_ctx *c, const wc_config_data *cfg, double start, double end, int slot0, int slot1, int path,
+                     int *best_tile, double *best_ms) {
+    if (!c || !cfg) return WC_ERR_INVALID;
+    if (!c->haveGrid || c->n == 0) return fail(c, WC_ERR_INVALID, "wc_set_grid and particles first");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n = c->n;
+    // the population and the step counter are restored after the trials
+    auto copy = [&](const SoA &dst, const SoA &src) -> int {
+        HIPCHK(c, hipMemcpyAsync(dst.id, src.id, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        double *d[6] = {dst.k, dst.j, dst.i, dst.health, dst.age, dst.time};
+        double *s[6] = {src.k, src.j, src.i, src.health, src.age, src.time};
+        for (int q = 0; q < 6; ++q) HIPCHK(c, hipMemcpyAsync(d[q], s[q], n * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dst.tag, src.tag, n * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dst.ev, src.ev, n, hipMemcpyDeviceToDevice, c->stream));
+        return WC_OK;
+    };
+    unsigned long long steps0 = 0;
+    HIPCHK(c, hipMemcpy(&steps0, c->d_steps, sizeof steps0, hipMemcpyDeviceToHost));
+    int rc = copy(c->s2, c->s);
+    if (rc) return rc;
+    const int tiles[] = {0, 1, 2, 4, 8};
+    const int origTile = c->sortTile, origPeriod = c->sortPeriod;
+    const bool origTiming = c->timing;
+    c->timing = false;
+    int keep = origTile;
+    float best = 1e30f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    for (int trial = -1; trial < 5 && !rc; ++trial) {       // trial -1 warms up
+        const int tile = tiles[trial < 0 ? 1 : trial];
+        if ((rc = copy(c->s, c->s2))) break;
+        c->sortTile = tile;
+        c->sortPeriod = 1;
+        c->sortCount = 0;
+        c->permValid = false;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(c, WC_ERR_HIP, "hipEventRecord"); break; }
+        if ((rc = wc_move_interval(c, cfg, start, end, slot0, slot1, path))) break;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(c, WC_ERR_HIP, "autotune trial with sort tile %d failed", tile);
+            break;
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (trial >= 0 && ms < best) {
+            best = ms;
+            keep = tile;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!rc) rc = copy(c->s, c->s2);
+    c->timing = origTiming;
+    c->sortTile = rc ? origTile : keep;
+    c->sortPeriod = rc ? origPeriod : 1;
+    c->sortCount = 0;
+    c->permValid = false;
+    HIPCHK(c, hipMemcpyAsync(c->d_steps, &steps0, sizeof steps0, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (rc) return rc;
+    if (best_tile) *best_tile = keep;
+    if (best_ms) *best_ms = best;
+    return WC_OK;
+}
+
 int wc_get_events(wc_ctx *c, uint8_t *events, size_t n) {
     if (!c || (n && !events)) return WC_ERR_INVALID;
     if (n > c->n) return fail(c, WC_ERR_INVALID, "asked for %zu particles, context holds %zu", n, c->n);
@@ -1094,13 +1217,16 @@ int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double 
     const auto &a = c->slots[slot0], &b = c->slots[slot1];
     GridPtrs G{c->d_h, c->d_depth, c->d_dx, c->d_dy, nullptr, nullptr, c->d_hf, c->d_md, nullptr, c->d_depthf};
     RecordPair<float> F{a.f[0], b.f[0], a.f[1], b.f[1], a.f[2], b.f[2], a.f[3], b.f[3], a.f[4], b.f[4]};
+    int rc2 = sort_population(c);
+    if (rc2) return rc2;
+    const uint32_t *perm = c->sortTile > 0 ? c->d_perm : nullptr;
     timing_begin(c);
     if (path == WC_PATH_PARITY)
         HIPCHK(c, launch_move_parity_soa(P, F, G, c->d_pack, c->d_metric, c->s, (int)c->n, c->d_steps,
-                                         c->trackEvents, c->stream));
+                                         c->trackEvents, perm, c->stream));
     else
         HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps, c->trackEvents,
-                                       c->stream));
+                                       perm, c->stream));
     // an upload into either slot now waits for this launch
     HIPCHK(c, hipEventRecord(c->slots[slot0].released, c->stream));
     HIPCHK(c, hipEventRecord(c->slots[slot1].released, c->stream));
@@ -1114,10 +1240,19 @@ int wc_remove_dead(wc_ctx *c, size_t *remaining) {
         return WC_OK;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, launch_compact(c->s, c->s2, (int)c->n, c->d_blockCounts, c->d_total, c->stream));
+    const bool keepPerm = c->permValid && c->sortTile > 0;
+    HIPCHK(c, launch_compact(c->s, c->s2, (int)c->n, c->d_blockCounts, c->d_total, keepPerm ? c->d_remap : nullptr,
+                             c->d_scanSums, c->stream));
     unsigned long long total = 0;
     HIPCHK(c, hipMemcpyAsync(&total, c->d_total, sizeof total, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (keepPerm) {             // the cell-tile order of the survivors, at their new indices
+        HIPCHK(c, launch_perm_remap(c->d_perm, c->d_remap, (int)c->n, c->d_perm2, c->d_blockCounts, c->d_total,
+                                    c->d_scanSums, c->stream));
+        std::swap(c->d_perm, c->d_perm2);
+    } else {
+        c->permValid = false;
+    }
     std::swap(c->s, c->s2);
     c->n = (size_t)total;
     if (remaining) *remaining = c->n;
@@ -1131,7 +1266,8 @@ int wc_grid(wc_ctx *c, uint32_t *counts_out, float *conc_out, int mask_output) {
     const size_t plane = (size_t)c->eta * c->xi, total = plane * c->s_rho;
     timing_begin(c);
     HIPCHK(c, launch_fill_u32(c->d_counts, total, 0u, c->stream));
-    HIPCHK(c, launch_grid_count_soa(c->s, (int)c->n, c->s_rho, c->eta, c->xi, c->d_counts, c->stream));
+    // population order: the LDS aggregation works as well, and the reads stay contiguous
+    HIPCHK(c, launch_grid_count_soa(c->s, (int)c->n, c->s_rho, c->eta, c->xi, c->d_counts, nullptr, c->stream));
     // the reference's float frame (counts, land masked) stays on the device
     HIPCHK(c, launch_counts_to_conc(c->d_counts, c->d_mask, mask_output, c->s_rho, plane, c->d_conc, c->stream));
     int rc = timing_end(c, 1);
@@ -1187,6 +1323,7 @@ int wc_emit(wc_ctx *c, const wc_config_data *cfg, const wc_source *sources, int 
     HIPCHK(c, hipMemcpy(d_pos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
     const EmitDrift drift{cfg->driftObjectType, cfg->driftSide, cfg->leewayRandomSide ? 1 : 0,
                           cfg->leewayRightSideProbability};
+    if (c->permValid) HIPCHK(c, launch_perm_fill(c->d_perm, c->n, c->n + m, c->stream));   // emitted last
     HIPCHK(c, launch_emit(c->s, c->n, (int)m, d_times, d_src, d_pos, firstId, cfg->randomSeed,
                           cfg->randomSources ? 1 : 0, drift, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

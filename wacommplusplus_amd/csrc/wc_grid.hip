@@ -77,9 +77,12 @@ __global__ __launch_bounds__(kGridBlock) void grid_count_kernel(Acc acc, int n, 
         if (countOf[q]) atomicAdd(&counts[cellOf[q]], (Out)countOf[q]);
 }
 
+// perm (optional): visit the particles in this order (a cell-tile sort)
 struct SoAPos {
     const double *k, *j, *i, *health;
-    __device__ __forceinline__ void position(int idx, double &K, double &J, double &I, double &H) const {
+    const uint32_t *perm;
+    __device__ __forceinline__ void position(int t, double &K, double &J, double &I, double &H) const {
+        const int idx = perm ? (int)perm[t] : t;
         K = k[idx]; J = j[idx]; I = i[idx]; H = health[idx];
     }
 };
@@ -164,8 +167,63 @@ __global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t *__restrict_
     if (t == 1023) total[0] = part[1023];
 }
 
+// remap (optional): remap[idx] = the survivor's new index, or kDead
+constexpr uint32_t kDead = 0xFFFFFFFFu;
+// Exclusive scan of n uint32 in place for large n, in three passes: the sum
+// of every chunk of kScanChunk entries, a one-block scan of the sums, and
+// the chunks' scans plus their offsets.  Each work-item takes 4 consecutive
+// entries (one 16-byte load when aligned), a wave scans with shuffles.
+constexpr int kScanChunk = kBlock * 4;
+
+__device__ __forceinline__ void load4(const uint32_t *a, size_t n, size_t q, uint32_t v[4]) {
+    for (int r = 0; r < 4; ++r) v[r] = q + r < n ? a[q + r] : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void chunk_sum_kernel(const uint32_t *__restrict__ a, size_t n,
+                                                           uint32_t *__restrict__ sums) {
+    __shared__ uint32_t waveSum[kWaves];
+    uint32_t v[4];
+    load4(a, n, (size_t)blockIdx.x * kScanChunk + 4 * threadIdx.x, v);
+    uint32_t s = v[0] + v[1] + v[2] + v[3];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane_id() == 0) waveSum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kWaves; ++w) t += waveSum[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void chunk_scan_kernel(uint32_t *__restrict__ a, size_t n,
+                                                            const uint32_t *__restrict__ offsets) {
+    __shared__ uint32_t waveSum[kWaves];
+    const size_t q = (size_t)blockIdx.x * kScanChunk + 4 * threadIdx.x;
+    uint32_t v[4];
+    load4(a, n, q, v);
+    const uint32_t own = v[0] + v[1] + v[2] + v[3];
+    uint32_t incl = own;                                   // inclusive scan over the wave
+    const int lane = lane_id();
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) waveSum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint32_t base = offsets[blockIdx.x];
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) base += waveSum[w];
+    uint32_t run = base + incl - own;
+    for (int r = 0; r < 4; ++r)
+        if (q + r < n) {
+            const uint32_t x = v[r];
+            a[q + r] = run;
+            run += x;
+        }
+}
+
 __global__ __launch_bounds__(kBlock) void scatter_alive_kernel(SoA src, SoA dst, int n,
-                                                               const uint32_t *__restrict__ blockOffsets) {
+                                                               const uint32_t *__restrict__ blockOffsets,
+                                                               uint32_t *__restrict__ remap) {
     __shared__ uint32_t waveBase[kWaves];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const bool alive = idx < n && src.health[idx] > 0;
@@ -179,7 +237,148 @@ __global__ __launch_bounds__(kBlock) void scatter_alive_kernel(SoA src, SoA dst,
         dst.id[o] = src.id[idx]; dst.k[o] = src.k[idx]; dst.j[o] = src.j[idx]; dst.i[o] = src.i[idx];
         dst.health[o] = src.health[idx]; dst.age[o] = src.age[idx]; dst.time[o] = src.time[idx];
         dst.tag[o] = src.tag[idx]; dst.ev[o] = src.ev[idx];
+        if (remap) remap[idx] = o;
+    } else if (remap && idx < n) {
+        remap[idx] = kDead;
     }
+}
+
+// ---- cell-tile order of the population (a permutation) -----------------
+//
+// perm lists the population's indices with particles of one tile of
+// tile x tile cells together (tiles row-major; particles outside the grid
+// last), so the move's lanes gather from neighbouring cells.  A counting
+// sort: histogram of tile keys, exclusive scan, scatter.  Both passes read
+// the particles in the previous order, where a workgroup's particles share
+// few tiles.
+
+__device__ __forceinline__ uint32_t tile_of(double j, double i, int eta, int xi, int tile, int tilesX,
+                                            uint32_t nTiles) {
+    const double jf = floor(j), if_ = floor(i);
+    if (!(jf >= 0 && if_ >= 0 && jf < eta && if_ < xi)) return nTiles;
+    return (uint32_t)((int)jf / tile) * (uint32_t)tilesX + (uint32_t)((int)if_ / tile);
+}
+
+// Histogram and scatter of the tile keys, per workgroup of kGridBlock work-
+// items x kGridPerItem particles, aggregated in an LDS hash table (as the
+// gridding): one global atomic per distinct tile per workgroup.  The scatter
+// reserves each tile's run of positions with one returning atomic and places
+// the workgroup's particles of the tile in it.
+struct TileTable {
+    uint32_t key[1u << kGridTableBits], count[1u << kGridTableBits], base[1u << kGridTableBits];
+};
+
+__device__ __forceinline__ void table_clear(TileTable &T) {
+    for (uint32_t q = threadIdx.x; q < (1u << kGridTableBits); q += kGridBlock) {
+        T.key[q] = kGridEmpty;
+        T.count[q] = 0;
+    }
+}
+
+// slot of key (inserted if new), or -1 when 16 probes find none
+__device__ __forceinline__ int table_slot(TileTable &T, uint32_t key) {
+    uint32_t slot = (key * 2654435761u) >> (32 - kGridTableBits);
+    for (int probe = 0; probe < 16; ++probe) {
+        const uint32_t old = atomicCAS(&T.key[slot], kGridEmpty, key);
+        if (old == kGridEmpty || old == key) return (int)slot;
+        slot = (slot + 1) & ((1u << kGridTableBits) - 1);
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(kGridBlock) void sort_hist_kernel(const double *__restrict__ j,
+                                                               const double *__restrict__ i,
+                                                               const uint32_t *__restrict__ perm, int n, int eta,
+                                                               int xi, int tile, int tilesX, uint32_t nTiles,
+                                                               uint32_t *__restrict__ bins) {
+    __shared__ TileTable T;
+    table_clear(T);
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * (kGridBlock * kGridPerItem);
+    for (int r = 0; r < kGridPerItem; ++r) {
+        const long long t = base + (long long)r * kGridBlock + threadIdx.x;
+        if (t >= n) break;
+        const int idx = perm ? (int)perm[t] : (int)t;
+        const uint32_t key = tile_of(j[idx], i[idx], eta, xi, tile, tilesX, nTiles);
+        const int slot = table_slot(T, key);
+        if (slot >= 0) atomicAdd(&T.count[slot], 1u);
+        else atomicAdd(&bins[key], 1u);
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (1u << kGridTableBits); q += kGridBlock)
+        if (T.count[q]) atomicAdd(&bins[T.key[q]], T.count[q]);
+}
+
+__global__ __launch_bounds__(kGridBlock) void sort_scatter_kernel(const double *__restrict__ j,
+                                                                  const double *__restrict__ i,
+                                                                  const uint32_t *__restrict__ perm, int n, int eta,
+                                                                  int xi, int tile, int tilesX, uint32_t nTiles,
+                                                                  uint32_t *__restrict__ bins,
+                                                                  uint32_t *__restrict__ out) {
+    __shared__ TileTable T;
+    table_clear(T);
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * (kGridBlock * kGridPerItem);
+    int idxs[kGridPerItem], slots[kGridPerItem];
+    uint32_t local[kGridPerItem];
+    for (int r = 0; r < kGridPerItem; ++r) {
+        const long long t = base + (long long)r * kGridBlock + threadIdx.x;
+        idxs[r] = -1;
+        if (t >= n) continue;
+        const int idx = perm ? (int)perm[t] : (int)t;
+        const uint32_t key = tile_of(j[idx], i[idx], eta, xi, tile, tilesX, nTiles);
+        const int slot = table_slot(T, key);
+        idxs[r] = idx;
+        slots[r] = slot;
+        if (slot >= 0) local[r] = atomicAdd(&T.count[slot], 1u);
+        else out[atomicAdd(&bins[key], 1u)] = (uint32_t)idx;       // table full: placed directly
+        if (slot < 0) idxs[r] = -1;
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (1u << kGridTableBits); q += kGridBlock)
+        if (T.count[q]) T.base[q] = atomicAdd(&bins[T.key[q]], T.count[q]);
+    __syncthreads();
+    for (int r = 0; r < kGridPerItem; ++r)
+        if (idxs[r] >= 0) out[T.base[slots[r]] + local[r]] = (uint32_t)idxs[r];
+}
+
+// perm after a compaction: the surviving entries, in perm order, mapped to
+// the survivors' new indices (remap from scatter_alive_kernel)
+__global__ __launch_bounds__(kBlock) void perm_count_kernel(const uint32_t *__restrict__ perm,
+                                                            const uint32_t *__restrict__ remap, int n,
+                                                            uint32_t *__restrict__ blockCounts) {
+    __shared__ uint32_t waveCount[kWaves];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool keep = t < n && remap[perm[t]] != kDead;
+    const unsigned long long m = __ballot(keep);
+    if (lane_id() == 0) waveCount[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < kWaves; ++w) s += waveCount[w];
+        blockCounts[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void perm_scatter_kernel(const uint32_t *__restrict__ perm,
+                                                              const uint32_t *__restrict__ remap, int n,
+                                                              const uint32_t *__restrict__ blockOffsets,
+                                                              uint32_t *__restrict__ out) {
+    __shared__ uint32_t waveBase[kWaves];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t to = t < n ? remap[perm[t]] : kDead;
+    const bool keep = to != kDead;
+    const unsigned long long m = __ballot(keep);
+    if (lane_id() == 0) waveBase[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = blockOffsets[blockIdx.x];
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) base += waveBase[w];
+    if (keep) out[base + (uint32_t)lanes_below(m)] = to;
+}
+
+__global__ void perm_fill_kernel(uint32_t *__restrict__ perm, size_t from, size_t to) {
+    for (size_t q = from + blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < to; q += (size_t)gridDim.x * blockDim.x)
+        perm[q] = (uint32_t)q;
 }
 
 // ---- particle history records ------------------------------------------
@@ -251,6 +450,19 @@ __global__ __launch_bounds__(kBlock) void records_scatter_kernel(SoA src, int n,
     }
 }
 
+// Exclusive scan of a[0..n) in place; total[0] = the sum.  sums: scratch of
+// kScanSumsCap entries (nullptr, or too few: the one-block scan).
+void scan_u32(uint32_t *a, size_t n, uint32_t *sums, unsigned long long *total, hipStream_t stream) {
+    const size_t chunks = (n + kScanChunk - 1) / kScanChunk;
+    if (!sums || chunks <= 1 || chunks > kScanSumsCap) {
+        scan_blocks_kernel<<<1, 1024, 0, stream>>>(a, (int)n, total);
+        return;
+    }
+    chunk_sum_kernel<<<(int)chunks, kBlock, 0, stream>>>(a, n, sums);
+    scan_blocks_kernel<<<1, 1024, 0, stream>>>(sums, (int)chunks, total);
+    chunk_scan_kernel<<<(int)chunks, kBlock, 0, stream>>>(a, n, sums);
+}
+
 int grid_count_blocks(int n) {
     const long long per = (long long)kGridBlock * kGridPerItem;
     return (int)((n + per - 1) / per);
@@ -271,10 +483,10 @@ hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream
 }
 
 hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi, uint32_t *counts,
-                                 hipStream_t stream) {
+                                 const uint32_t *perm, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     grid_count_kernel<SoAPos, uint32_t><<<grid_count_blocks(n), kGridBlock, 0, stream>>>(
-        SoAPos{s.k, s.j, s.i, s.health}, n, s_rho, eta, xi, counts);
+        SoAPos{s.k, s.j, s.i, s.health, perm}, n, s_rho, eta, xi, counts);
     return hipGetLastError();
 }
 
@@ -305,12 +517,44 @@ hipError_t launch_counts_to_conc(const uint32_t *counts, const double *mask, int
 
 // blockCounts must hold ceil(n/256) entries; total a device u64.
 hipError_t launch_compact(const SoA &src, const SoA &dst, int n, uint32_t *blockCounts,
-                          unsigned long long *total, hipStream_t stream) {
+                          unsigned long long *total, uint32_t *remap, uint32_t *scanSums, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int nb = (n + kBlock - 1) / kBlock;
     alive_count_kernel<<<nb, kBlock, 0, stream>>>(src.health, n, blockCounts);
-    scan_blocks_kernel<<<1, 1024, 0, stream>>>(blockCounts, nb, total);
-    scatter_alive_kernel<<<nb, kBlock, 0, stream>>>(src, dst, n, blockCounts);
+    scan_u32(blockCounts, (size_t)nb, scanSums, total, stream);
+    scatter_alive_kernel<<<nb, kBlock, 0, stream>>>(src, dst, n, blockCounts, remap);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_perm(const SoA &s, int n, int eta, int xi, int tile, const uint32_t *permIn,
+                            uint32_t *permOut, uint32_t *bins, unsigned long long *total, uint32_t *scanSums,
+                            hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int tilesX = (xi + tile - 1) / tile, tilesY = (eta + tile - 1) / tile;
+    const uint32_t nTiles = (uint32_t)tilesX * (uint32_t)tilesY, nBins = nTiles + 1;
+    const int nb = grid_count_blocks(n);
+    fill_u32_kernel<<<grid_blocks(nBins / 4 + 1), kBlock, 0, stream>>>(bins, nBins, 0u);
+    sort_hist_kernel<<<nb, kGridBlock, 0, stream>>>(s.j, s.i, permIn, n, eta, xi, tile, tilesX, nTiles, bins);
+    scan_u32(bins, nBins, scanSums, total, stream);
+    sort_scatter_kernel<<<nb, kGridBlock, 0, stream>>>(s.j, s.i, permIn, n, eta, xi, tile, tilesX, nTiles, bins,
+                                                       permOut);
+    return hipGetLastError();
+}
+
+hipError_t launch_perm_remap(const uint32_t *perm, const uint32_t *remap, int n, uint32_t *out,
+                             uint32_t *blockCounts, unsigned long long *total, uint32_t *scanSums,
+                             hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + kBlock - 1) / kBlock;
+    perm_count_kernel<<<nb, kBlock, 0, stream>>>(perm, remap, n, blockCounts);
+    scan_u32(blockCounts, (size_t)nb, scanSums, total, stream);
+    perm_scatter_kernel<<<nb, kBlock, 0, stream>>>(perm, remap, n, blockCounts, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_perm_fill(uint32_t *perm, size_t from, size_t to, hipStream_t stream) {
+    if (to <= from) return hipSuccess;
+    perm_fill_kernel<<<grid_blocks(to - from), kBlock, 0, stream>>>(perm, from, to);
     return hipGetLastError();
 }
 

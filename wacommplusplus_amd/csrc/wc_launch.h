@@ -11,9 +11,10 @@ namespace wc {
 // (parity_pack_bytes) and moves the SoA population; metric = {dx, dy} table.
 size_t parity_pack_bytes(int s_w, size_t plane);
 // events: OR the closure events of each particle into s.ev.
+// perm (optional): the order in which work-items take particles.
 hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
-                                  bool events, hipStream_t stream);
+                                  bool events, const uint32_t *perm, hipStream_t stream);
 // Drop-in path: the reference's array layout; metric = {dx, dy} table.
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
                                   const double2 *metric, wc_particle_data *p, int n, unsigned long long *stepCounter,
@@ -28,19 +29,37 @@ hipError_t launch_grid_hash(const double *lon, const double *lat, size_t n, unsi
 size_t fast_pack_bytes(int store, int s_w, int eta, int xi);
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
                                 const GridPtrs &G, const SoA &s, int n, unsigned long long *stepCounter,
-                                bool events, hipStream_t stream);
+                                bool events, const uint32_t *perm, hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream);
+// perm (optional): visit the particles in that order
 hipError_t launch_grid_count_soa(const SoA &s, int n, int s_rho, int eta, int xi, uint32_t *counts,
-                                 hipStream_t stream);
+                                 const uint32_t *perm, hipStream_t stream);
 hipError_t launch_grid_count_aos(const wc_particle_data *p, int n, int s_rho, int eta, int xi, uint32_t *counts,
                                  hipStream_t stream);
 hipError_t launch_grid_count_aos_float(const wc_particle_data *p, int n, int s_rho, int eta, int xi, float *conc,
                                        hipStream_t stream);
 hipError_t launch_counts_to_conc(const uint32_t *counts, const double *mask, int mask_output, int s_rho,
                                  size_t plane, float *conc, hipStream_t stream);
+// remap (optional, n entries): each particle's new index, or 0xFFFFFFFF if it
+// died.  scanSums (optional): kScanSumsCap entries of scratch for the
+// multi-block scans (scans of up to 64M values; longer ones use one block).
+constexpr size_t kScanSumsCap = 65536;
 hipError_t launch_compact(const SoA &src, const SoA &dst, int n, uint32_t *blockCounts, unsigned long long *total,
-                          hipStream_t stream);
+                          uint32_t *remap, uint32_t *scanSums, hipStream_t stream);
+// Cell-tile order: permOut = the population's indices grouped by tile of
+// tile x tile cells (particles outside the grid last), read in the order
+// permIn (nullptr: population order); bins: tiles + 1 counters.
+hipError_t launch_sort_perm(const SoA &s, int n, int eta, int xi, int tile, const uint32_t *permIn,
+                            uint32_t *permOut, uint32_t *bins, unsigned long long *total, uint32_t *scanSums,
+                            hipStream_t stream);
+// out = the entries of perm whose particle survived a compaction, mapped
+// through remap, in perm order; total[0] = their number
+hipError_t launch_perm_remap(const uint32_t *perm, const uint32_t *remap, int n, uint32_t *out,
+                             uint32_t *blockCounts, unsigned long long *total, uint32_t *scanSums,
+                             hipStream_t stream);
+// perm[q] = q for q in [from, to)
+hipError_t launch_perm_fill(uint32_t *perm, size_t from, size_t to, hipStream_t stream);
 
 // Particle history records (Particles::netcdfRecords): the grid's lat, lon
 // (degrees) and h [eta][xi], the s_w W-level sigma values bottom first.

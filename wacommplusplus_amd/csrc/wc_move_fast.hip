@@ -575,12 +575,15 @@ __device__ __forceinline__ unsigned run_interval(const MoveParams &P, const Smp 
     return steps;
 }
 
+// perm (optional): work-item t moves particle perm[t] (a cell-tile order)
 template <class T, class Smp, bool Jibe, bool Ev>
 __global__ __launch_bounds__(256) void move_fast_kernel(MoveParams P, Smp S, SoA st, int n,
-                                                        unsigned long long *stepCounter) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+                                                        unsigned long long *stepCounter,
+                                                        const uint32_t *__restrict__ perm) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned steps = 0;
-    if (idx < n) {
+    if (t < n) {
+        const int idx = perm ? (int)perm[t] : t;
         const uint64_t id = st.id[idx];
         double kd = st.k[idx];
         const double jd = st.j[idx], id_ = st.i[idx];
@@ -1185,14 +1188,14 @@ int grid_blocks(size_t n) {
 
 template <class T, class Smp>
 hipError_t launch_move(const MoveParams &P, const Smp &S, const SoA &s, int n, unsigned long long *ctr, bool events,
-                       hipStream_t stream) {
+                       const uint32_t *perm, hipStream_t stream) {
     const int block = 256, blocks = (n + block - 1) / block;
     if (P.jibe) {
-        if (events) move_fast_kernel<T, Smp, true, true><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
-        else move_fast_kernel<T, Smp, true, false><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
+        if (events) move_fast_kernel<T, Smp, true, true><<<blocks, block, 0, stream>>>(P, S, s, n, ctr, perm);
+        else move_fast_kernel<T, Smp, true, false><<<blocks, block, 0, stream>>>(P, S, s, n, ctr, perm);
     } else {
-        if (events) move_fast_kernel<T, Smp, false, true><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
-        else move_fast_kernel<T, Smp, false, false><<<blocks, block, 0, stream>>>(P, S, s, n, ctr);
+        if (events) move_fast_kernel<T, Smp, false, true><<<blocks, block, 0, stream>>>(P, S, s, n, ctr, perm);
+        else move_fast_kernel<T, Smp, false, false><<<blocks, block, 0, stream>>>(P, S, s, n, ctr, perm);
     }
     return hipGetLastError();
 }
@@ -1221,7 +1224,8 @@ TileLayout tile_layout(size_t tbytes, int s_w, int eta, int xi) {
 
 template <class T>
 hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G,
-                          const SoA &s, int n, unsigned long long *ctr, bool events, hipStream_t stream) {
+                          const SoA &s, int n, unsigned long long *ctr, bool events, const uint32_t *perm,
+                          hipStream_t stream) {
     const TileLayout L = tile_layout(sizeof(T), P.s_w, P.eta, P.xi);
     char *base = (char *)packBuf;
     TiledPacked<T> S;
@@ -1243,12 +1247,12 @@ hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void 
         L.tplane);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n <= 0) return e;
-    return launch_move<T>(P, S, s, n, ctr, events, stream);
+    return launch_move<T>(P, S, s, n, ctr, events, perm, stream);
 }
 
 template <class T>
 hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G, const SoA &s,
-                    int n, unsigned long long *ctr, bool events, hipStream_t stream) {
+                    int n, unsigned long long *ctr, bool events, const uint32_t *perm, hipStream_t stream) {
     const size_t plane = (size_t)P.eta * P.xi;
     const int nlev = P.s_w - 1;
     Packed<T> S;
@@ -1267,7 +1271,7 @@ hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packB
                                                                   (ZH *)S.z, P.s_rho, P.s_w, plane);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n <= 0) return e;
-    return launch_move<T>(P, S, s, n, ctr, events, stream);
+    return launch_move<T>(P, S, s, n, ctr, events, perm, stream);
 }
 
 template <class T>
@@ -1380,12 +1384,14 @@ size_t fast_pack_bytes(int store, int s_w, int eta, int xi) {
 
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
                                 const GridPtrs &G, const SoA &s, int n, unsigned long long *ctr, bool events,
-                                hipStream_t stream) {
-    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, stream);
-    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, stream);
-    if (store == WC_STORE_F16_TILE) return launch_tiled_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, stream);
-    if (store == WC_STORE_BF16_TILE) return launch_tiled_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, stream);
-    return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, stream);
+                                const uint32_t *perm, hipStream_t stream) {
+    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
+    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
+    if (store == WC_STORE_F16_TILE)
+        return launch_tiled_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
+    if (store == WC_STORE_BF16_TILE)
+        return launch_tiled_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
+    return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
 }
 
 }  // namespace wc

@@ -414,12 +414,15 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p, uint32
     return steps;
 }
 
+// perm (optional): work-item t moves particle perm[t] (a cell-tile order)
 template <class Smp, class Acc, bool Ev = false>
 __global__ __launch_bounds__(256) void move_parity_kernel(MoveParams P, Smp S, Acc acc, int n,
-                                                          unsigned long long *stepCounter, uint8_t *events) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          unsigned long long *stepCounter, uint8_t *events,
+                                                          const uint32_t *__restrict__ perm) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned steps = 0;
-    if (idx < n) {
+    if (t < n) {
+        const int idx = perm ? (int)perm[t] : t;
         State p;
         acc.load(idx, p);
         uint32_t ev = 0;
@@ -455,7 +458,7 @@ size_t parity_pack_bytes(int s_w, size_t plane) {
 
 hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
-                                  bool events, hipStream_t stream) {
+                                  bool events, const uint32_t *perm, hipStream_t stream) {
     const size_t plane = (size_t)P.eta * P.xi;
     const int nlev = P.s_w - 1;
     char *base = (char *)packBuf;
@@ -480,10 +483,10 @@ hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &
     const int block = 256;
     if (events)
         move_parity_kernel<PackedSampler, SoAAccess, true><<<(n + block - 1) / block, block, 0, stream>>>(
-            P, S, SoAAccess{s, P.jibe}, n, stepCounter, s.ev);
+            P, S, SoAAccess{s, P.jibe}, n, stepCounter, s.ev, perm);
     else
         move_parity_kernel<PackedSampler, SoAAccess><<<(n + block - 1) / block, block, 0, stream>>>(
-            P, S, SoAAccess{s, P.jibe}, n, stepCounter, nullptr);
+            P, S, SoAAccess{s, P.jibe}, n, stepCounter, nullptr, perm);
     return hipGetLastError();
 }
 
@@ -502,7 +505,8 @@ hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &
     const int block = 256;
     RawSampler S{F, G, metric, P.xi, (size_t)P.eta * P.xi, P.s_rho, P.s_w};
     move_parity_kernel<RawSampler, AoSAccess><<<(n + block - 1) / block, block, 0, stream>>>(P, S, AoSAccess{p}, n,
-                                                                                             stepCounter, nullptr);
+                                                                                             stepCounter, nullptr,
+                                                                                             nullptr);
     return hipGetLastError();
 }
 
