@@ -127,7 +127,10 @@ def load_window(ctx, W, store, n_slots=None):
 
 def roofline_block(key, r, n_particles, store_bytes, static_bytes, midpoint, counters):
     """VALU-issue roofline of one move kernel (the binding resource), with the
-    measured DRAM traffic and the L2/MALL-effective bandwidth beside it."""
+    measured L2-to-memory traffic and the L2/MALL-effective bandwidth beside
+    it.  `key` names the run (variant, integrator, cell-tile sort); the
+    counters hold one value per interval from t0, and the run's timed
+    intervals 0 .. steps-1 are averaged."""
     avg_s = r["move_ms"] / 1e3 / r["move_launches"]
     steps_per_launch = r["steps"] / r["move_launches"]
     eff = (steps_per_launch * algorithmic_bytes_per_step(store_bytes, static_bytes, midpoint, True) +
@@ -138,17 +141,21 @@ def roofline_block(key, r, n_particles, store_bytes, static_bytes, midpoint, cou
              "effective_gbs_note": "reference-stencil algorithmic bytes / launch time; served by L2 and the "
                                    "Infinity Cache, not an HBM fraction",
              "counters": None, "traffic": None, "hbm": None}
-    if c:
-        achieved = c["valu_per_dispatch"] / avg_s
+    n = r["move_launches"]
+    if c and len(c["valu_per_interval"]) >= n:
+        valu = float(np.mean(c["valu_per_interval"][:n]))
+        achieved = valu / avg_s
         block.update(achieved=achieved, frac=achieved / VALU_ISSUE_PEAK,
-                     counters={"kernel": c["kernel"], "valu_per_dispatch": c["valu_per_dispatch"],
-                               "source": os.path.relpath(COUNTERS, ROOT)})
-        if c.get("dram_bytes_per_dispatch"):
-            t = c["dram_bytes_per_dispatch"]
+                     counters={"key": key, "kernel": c["kernel"], "valu_per_dispatch": valu,
+                               "intervals": n, "source": os.path.relpath(COUNTERS, ROOT)})
+        if c.get("mem_bytes_per_interval"):
+            t = float(np.mean(c["mem_bytes_per_interval"][:n]))
             hbm = t / avg_s / 1e9
             block.update(traffic=t, hbm={"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                           "frac": hbm / HBM_PEAK_GBS,
-                                          "what": "calibrated FETCH_SIZE + WRITE_SIZE per dispatch / launch time"})
+                                          "what": "FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by its "
+                                                  "reading for a plain copy of known size (L2-to-memory "
+                                                  "traffic: HBM or the Infinity Cache) / launch time"})
     return block
 
 
@@ -375,6 +382,33 @@ def precision_vs_f64(ctx, W, p, steps, cfg):
     return res
 
 
+EULER_VARIANTS = ("f64", "bf16_tile")
+
+
+def profile_runs(ctx, W, cfg, p0, variants, tiles, args):
+    """The runs the committed move-kernel counters come from: each variant
+    unsorted and at every listed cell-tile sort, midpoint and (for
+    EULER_VARIANTS) Euler, `warmup` + `steps` launches each, in the order
+    printed.  The counters are per launch, so their per-interval values
+    serve any run of the same variant, integrator and sort."""
+    import wacommplusplus_amd as wc
+    runs = []
+    for path, store in variants:
+        key = "f64" if path == "parity" else store
+        load_window(ctx, W, store)
+        for euler in ((False, True) if not args.no_euler and key in EULER_VARIANTS else (False,)):
+            c = cfg.copy(integrator=wc.EULER) if euler else cfg
+            for tile in [0] + tiles:
+                name = f"{key}{'_euler' if euler else ''}@t{tile}"
+                r = run_timed(ctx, W, c, p0, args.steps, args.warmup, path, progress=name, tile=tile)
+                runs.append(dict(key=name, variant=key, path=path, store=store, sort_tile=tile,
+                                 warmup_launches=args.warmup, timed_launches=r["move_launches"],
+                                 move_kernel_ms=r["move_ms"] / r["move_launches"]))
+                log(f"[{name}] {r}")
+    print(json.dumps({"profile_runs": runs, "steps": args.steps, "warmup": args.warmup,
+                      "particles": args.particles}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,6 +425,10 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the precision comparison (its diffusion-off runs launch the move kernels "
                          "on other populations, which a kernel-trace average would mix in)")
+    ap.add_argument("--sort-tiles", default=None,
+                    help="counter-collection runs: comma-separated cell-tile sorts; every variant is timed "
+                         "unsorted and at each of them (midpoint, and Euler where reported), without the "
+                         "autotuner, and one JSON line lists the runs in launch order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU only: the particle ranges of several "
@@ -413,6 +451,10 @@ def main():
     results = {}
     variants = [("parity", "f32")] + ([] if args.no_variants else [("fast", s) for s in STORES])
     ctx = wc.Context(0)
+    if args.sort_tiles is not None:
+        profile_runs(ctx, W, cfg, p0, variants, [int(t) for t in args.sort_tiles.split(",") if t], args)
+        ctx.close()
+        return
     for path, store in variants:
         key = "f64" if path == "parity" else store
         log(f"[{key}] uploading {W.T} records ({store} storage)")
@@ -425,7 +467,7 @@ def main():
         results[key] = (run_timed(ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key, tile=tile)
                         if tile else results[key + "_unsorted"])
         log(f"[{key}] {results[key]}")
-        if not args.no_euler and key in ("f64", "bf16_tile"):
+        if not args.no_euler and key in EULER_VARIANTS:
             e = run_timed(ctx, W, cfg.copy(integrator=wc.EULER), p0, args.steps, args.warmup, path,
                           progress=key + " euler", tile=tile)
             results[key + "_euler"] = e
@@ -471,8 +513,7 @@ def main():
 
     main_r = results["f64"]
     value = main_r["steps"] / main_r["wall_s"]
-    roof = roofline_block("f64" if main_r["sort_tile"] == 0 else "f64_sorted", main_r, args.particles, 4, 8, True,
-                          counters)
+    roof = roofline_block(f"f64@t{main_r['sort_tile']}", main_r, args.particles, 4, 8, True, counters)
     roof["kernel"] = "move_parity_kernel<PackedSampler, SoAAccess>"
     roof["sort_tile"] = main_r["sort_tile"]
     roof["algorithmic_bytes_per_particle_step"] = algorithmic_bytes_per_step(4, 8, True, True)
@@ -482,7 +523,7 @@ def main():
             continue
         store = key.replace("_euler", "").replace("_unsorted", "")
         euler = key.endswith("_euler")
-        ckey = key if (r["sort_tile"] == 0 or key.endswith("_unsorted")) else key + "_sorted"
+        ckey = f"{key.replace('_unsorted', '')}@t{r['sort_tile']}"
         if store == "f64":
             b = roofline_block(ckey, r, args.particles, 4, 8, not euler, counters)
         else:
