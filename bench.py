@@ -112,10 +112,11 @@ def run_timed(ctx, W, cfg, p0, steps, warmup, path, progress="", tile=0):
     move_ms, move_n = ctx.kernel_time(0)
     grid_ms, grid_n = ctx.kernel_time(1)
     sort_ms, sort_n = ctx.kernel_time(3)
+    pack_ms, pack_n = ctx.kernel_time(4)
     ctx.set_sort(0)
     return dict(wall_s=wall, steps=ctx.step_count(), move_ms=move_ms, move_launches=move_n,
                 grid_ms=grid_ms, grid_launches=grid_n, sort_tile=tile, sort_ms=sort_ms, sort_launches=sort_n,
-                alive=ctx.count())
+                pack_ms=pack_ms, pack_launches=pack_n, alive=ctx.count())
 
 
 def load_window(ctx, W, store, n_slots=None):
@@ -531,6 +532,7 @@ def main():
         variants_out[key] = dict(value=r["steps"] / r["wall_s"], ms_per_step=1e3 * r["wall_s"] / args.steps,
                                  move_kernel_ms=r["move_ms"] / r["move_launches"],
                                  grid_kernel_ms=r["grid_ms"] / max(1, r["grid_launches"]),
+                                 pack_ms_per_interval=r["pack_ms"] / max(1, r["pack_launches"]),
                                  sort_tile=r["sort_tile"], sort_ms_per_interval=r["sort_ms"] / args.steps,
                                  integrator="euler" if euler else "midpoint",
                                  path="parity (FP64)" if store == "f64" else f"fast ({store} storage)",
@@ -556,6 +558,7 @@ def main():
             "particles": args.particles, "grid": [11, 1024, 768], "substeps_per_step": 120,
             "path": "parity (FP64)", "records": "resident in HBM (upload-inclusive rate in pcie_inclusive_value)",
             "cell_tile_sort": main_r["sort_tile"],
+            "sort_autotuned": not args.no_sort,
         },
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -567,6 +570,8 @@ def main():
         "ensemble": ens,
         "learned_reconstruction": learned,
         "grid_kernel_ms": main_r["grid_ms"] / max(1, main_r["grid_launches"]),
+        "pack_ms_per_interval": main_r["pack_ms"] / max(1, main_r["pack_launches"]),
+        "sort_ms_per_interval": main_r["sort_ms"] / args.steps,
         "alive_after": main_r["alive"],
     }
     print(json.dumps(line), flush=True)

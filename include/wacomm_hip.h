@@ -451,7 +451,8 @@ int wc_run_window(wc_ctx *ctx, const wc_config_data *config, const wc_window *wi
 /* Accumulated device time of the move kernels launched since the last reset,
  * measured with HIP events on the context stream around each launch (enable
  * with wc_set_timing).  kernel: 0 = move, 1 = grid, 2 = ROMS normalisation,
- * 3 = the cell-tile sort of wc_set_sort. */
+ * 3 = the cell-tile sort of wc_set_sort, 4 = the interval pack that precedes
+ * each move (the record pair rearranged for the move's gathers). */
 int wc_set_timing(wc_ctx *ctx, int enabled);
 int wc_kernel_time(wc_ctx *ctx, int kernel, double *total_ms, long *launches);
 int wc_reset_kernel_time(wc_ctx *ctx);

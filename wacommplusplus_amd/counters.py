@@ -1,6 +1,6 @@
 """Hardware-counter summaries of the move kernels, from rocprofv3 runs.
 
-Two commands:
+Three commands:
 
   python3 -m wacommplusplus_amd.counters copy [--mib 1024] [--reps 5]
       A plain device-to-device copy of known size, run under rocprofv3 with
@@ -17,6 +17,12 @@ Two commands:
       bytes.  The move launches of one variant share one kernel symbol, so
       each symbol's dispatches are assigned to that variant's runs in launch
       order (warm-up launches skipped).
+
+  python3 -m wacommplusplus_amd.counters bench-trace --bench OUT.json
+        --moves MOVES.csv [--stats STATS.csv] --md FILE
+      Splits the move launches of a default `bench.py` run's kernel trace
+      into the bench's runs and sets their mean durations beside the
+      HIP-event means of the bench line.
 """
 from __future__ import annotations
 
@@ -250,6 +256,75 @@ def write_md(a, out, runs) -> None:
         f.write("\n".join(lines) + "\n")
 
 
+AUTOTUNE_LAUNCHES = 6          # wc_autotune_sort: one warm-up trial, then tiles 0, 1, 2, 4, 8
+TUNE_HOURS = 12                # bench.tune_sort moves the population to 12 h before tuning
+
+
+def bench_trace(a) -> None:
+    """The move launches of a default bench run (its kernel trace filtered to
+    the move kernels) split into the bench's runs, and each run's mean trace
+    duration set beside the HIP-event mean the bench line reports."""
+    with open(a.bench) as f:
+        line = [json.loads(x) for x in f if x.startswith("{")][-1]
+    steps, warmup = line["steps"], line["warmup"]
+    v = line["variants"]
+    runs = []
+    for key in ["f64"] + [k for k in v if "_" not in k.replace("bf16_tile", "bt").replace("f16_tile", "ft")]:
+        tile = line["config"]["cell_tile_sort"] if key == "f64" else v[key]["sort_tile"]
+        runs.append(dict(key=key + "_unsorted", variant=key, warmup_launches=warmup, timed_launches=steps))
+        if line["config"].get("sort_autotuned", True):
+            runs.append(dict(key=key + "_tuning", variant=key, warmup_launches=TUNE_HOURS,
+                             timed_launches=AUTOTUNE_LAUNCHES))
+        if tile:
+            runs.append(dict(key=key, variant=key, warmup_launches=warmup, timed_launches=steps))
+        if key + "_euler" in v:
+            runs.append(dict(key=key + "_euler", variant=key, warmup_launches=warmup, timed_launches=steps))
+    disp = OrderedDict()
+    with open(a.moves) as f:
+        rd = csv.reader(f)
+        for r in rd:
+            if r[0] == "Kind":
+                continue
+            disp[int(r[5])] = {"name": r[7], "ns": int(r[10]) - int(r[9])}
+    groups = move_dispatches(OrderedDict(sorted(disp.items())))
+    # launches after the variants' runs (precision, upload-inclusive rate, CPU-baseline snapshots) are
+    # left over at the end of the parity symbol's list and ignored here
+    variants = list(OrderedDict((r["variant"], None) for r in runs))
+    rows = []
+    for vk, (name, ds) in zip(variants, groups.items()):
+        pos = 0
+        for r in (r for r in runs if r["variant"] == vk):
+            pos += r["warmup_launches"]
+            seg = ds[pos:pos + r["timed_launches"]]
+            pos += r["timed_launches"]
+            if r["key"].endswith("_tuning"):
+                continue
+            bench_ms = (line["roofline"]["avg_launch_ms"] if r["key"] == "f64" else
+                        v[r["key"]]["move_kernel_ms"])
+            trace_ms = sum(e["ns"] for e in seg) / len(seg) / 1e6
+            rows.append(f"| {r['key']} | {short_name(name)} | {len(seg)} | {bench_ms:.3f} | {trace_ms:.3f} | "
+                        f"{100 * (trace_ms / bench_ms - 1):+.1f} % |")
+    roof = line["roofline"]
+    lines = [f"# Move kernels of the default benchmark: trace vs bench timing ({line['steps']} intervals, "
+             f"{line['warmup']} warm-up)", "", "Produced by:", "", "```", *(a.command or []), "```", "",
+             f"Bench line: value {line['value']:.4g} particle-steps/s (FP64 parity path, cell-tile sort "
+             f"{line['config']['cell_tile_sort']}); roofline frac {roof['frac']} "
+             f"({roof['counters']['key'] if roof.get('counters') else 'no counters'}).", "",
+             "Each run's timed move launches, found in the kernel trace by launch order, against the mean "
+             "the bench measured with HIP events on the context stream:", "",
+             "| run | kernel | launches | bench ms | trace ms | trace - bench |", "|---|---|---|---|---|---|",
+             *rows]
+    if a.stats:
+        lines += ["", "rocprofv3 `--stats` of the same run (every dispatch; one symbol covers several runs):", "",
+                  "| kernel | calls | total ms | mean us | % |", "|---|---|---|---|---|"]
+        with open(a.stats) as f:
+            for r in list(csv.DictReader(f))[:40]:
+                lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
+                             f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    with open(a.md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(prog="python3 -m wacommplusplus_amd.counters")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -264,9 +339,17 @@ def main(argv=None) -> None:
     p.add_argument("--out", default="profiles/move_counters.json")
     p.add_argument("--md", default=None)
     p.add_argument("--command", action="append", help="a command the counters came from (repeatable)")
+    b = sub.add_parser("bench-trace")
+    b.add_argument("--bench", required=True, help="the bench run's stdout (its JSON line)")
+    b.add_argument("--moves", required=True, help="its kernel-trace CSV, header and move-kernel rows only")
+    b.add_argument("--stats", default=None, help="its rocprofv3 kernel_stats CSV")
+    b.add_argument("--md", required=True)
+    b.add_argument("--command", action="append")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_probe(a.mib, a.reps)
+    elif a.cmd == "bench-trace":
+        bench_trace(a)
     else:
         parse(a)
 

@@ -21,7 +21,7 @@
 using namespace wc;
 
 // timed kernel groups: 0 move, 1 grid, 2 ROMS normalisation, 3 cell-tile sort
-constexpr int kTimedKernels = 4;
+constexpr int kTimedKernels = 5;
 
 struct wc_ctx {
     int device = 0;
@@ -1220,17 +1220,24 @@ int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double 
     int rc2 = sort_population(c);
     if (rc2) return rc2;
     const uint32_t *perm = c->sortTile > 0 ? c->d_perm : nullptr;
-    timing_begin(c);
-    if (path == WC_PATH_PARITY)
-        HIPCHK(c, launch_move_parity_soa(P, F, G, c->d_pack, c->d_metric, c->s, (int)c->n, c->d_steps,
-                                         c->trackEvents, perm, c->stream));
-    else
-        HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps, c->trackEvents,
-                                       perm, c->stream));
-    // an upload into either slot now waits for this launch
-    HIPCHK(c, hipEventRecord(c->slots[slot0].released, c->stream));
-    HIPCHK(c, hipEventRecord(c->slots[slot1].released, c->stream));
-    return timing_end(c, 0);
+    // the pack (timed as group 4) and the move kernel (group 0)
+    for (int stage : {kStagePack, kStageMove}) {
+        timing_begin(c);
+        if (path == WC_PATH_PARITY)
+            HIPCHK(c, launch_move_parity_soa(P, F, G, c->d_pack, c->d_metric, c->s, (int)c->n, c->d_steps,
+                                             c->trackEvents, perm, stage, c->stream));
+        else
+            HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps,
+                                           c->trackEvents, perm, stage, c->stream));
+        if (stage == kStageMove) {
+            // an upload into either slot now waits for this launch
+            HIPCHK(c, hipEventRecord(c->slots[slot0].released, c->stream));
+            HIPCHK(c, hipEventRecord(c->slots[slot1].released, c->stream));
+        }
+        const int trc = timing_end(c, stage == kStageMove ? 0 : 4);
+        if (trc) return trc;
+    }
+    return WC_OK;
 }
 
 int wc_remove_dead(wc_ctx *c, size_t *remaining) {

@@ -7,14 +7,18 @@
 
 namespace wc {
 
+// Stages of a context-path move launch: the interval pack, the move kernel.
+constexpr int kStagePack = 1, kStageMove = 2, kStagesAll = 3;
+
 // Context path: packs the interval's record pair into packBuf
 // (parity_pack_bytes) and moves the SoA population; metric = {dx, dy} table.
 size_t parity_pack_bytes(int s_w, size_t plane);
 // events: OR the closure events of each particle into s.ev.
 // perm (optional): the order in which work-items take particles.
+// stages: kStage* bits; the move stage reads the pack the pack stage wrote.
 hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
-                                  bool events, const uint32_t *perm, hipStream_t stream);
+                                  bool events, const uint32_t *perm, int stages, hipStream_t stream);
 // Drop-in path: the reference's array layout; metric = {dx, dy} table.
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
                                   const double2 *metric, wc_particle_data *p, int n, unsigned long long *stepCounter,
@@ -29,7 +33,7 @@ hipError_t launch_grid_hash(const double *lon, const double *lat, size_t n, unsi
 size_t fast_pack_bytes(int store, int s_w, int eta, int xi);
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
                                 const GridPtrs &G, const SoA &s, int n, unsigned long long *stepCounter,
-                                bool events, const uint32_t *perm, hipStream_t stream);
+                                bool events, const uint32_t *perm, int stages, hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream);
 // perm (optional): visit the particles in that order

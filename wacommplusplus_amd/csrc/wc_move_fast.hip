@@ -1225,7 +1225,7 @@ TileLayout tile_layout(size_t tbytes, int s_w, int eta, int xi) {
 template <class T>
 hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G,
                           const SoA &s, int n, unsigned long long *ctr, bool events, const uint32_t *perm,
-                          hipStream_t stream) {
+                          int stages, hipStream_t stream) {
     const TileLayout L = tile_layout(sizeof(T), P.s_w, P.eta, P.xi);
     char *base = (char *)packBuf;
     TiledPacked<T> S;
@@ -1239,20 +1239,24 @@ hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void 
     S.nlev = P.s_w - 1;
     S.tilesX = L.tilesX;
     S.tplane = L.tplane;
-    const size_t blocks = L.tplane * (size_t)S.nlev;
-    tile_mean_kernel<T><<<grid_blocks(blocks), 256, 0, stream>>>(F, (WA8<T> *)S.tm, P.s_rho, P.s_w, P.eta, P.xi,
-                                                                 L.tilesX, L.tplane);
-    tile_pack_kernel<T><<<grid_blocks(blocks * kTileNodes), 256, 0, stream>>>(
-        F, G.hf, S.tm, (UV4<T> *)S.uvr, (WA8<T> *)S.war, (ZH *)S.z, P.s_rho, P.s_w, P.eta, P.xi, L.tilesX,
-        L.tplane);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || n <= 0) return e;
+    if (stages & kStagePack) {
+        const size_t blocks = L.tplane * (size_t)S.nlev;
+        tile_mean_kernel<T><<<grid_blocks(blocks), 256, 0, stream>>>(F, (WA8<T> *)S.tm, P.s_rho, P.s_w, P.eta,
+                                                                     P.xi, L.tilesX, L.tplane);
+        tile_pack_kernel<T><<<grid_blocks(blocks * kTileNodes), 256, 0, stream>>>(
+            F, G.hf, S.tm, (UV4<T> *)S.uvr, (WA8<T> *)S.war, (ZH *)S.z, P.s_rho, P.s_w, P.eta, P.xi, L.tilesX,
+            L.tplane);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (!(stages & kStageMove) || n <= 0) return hipSuccess;
     return launch_move<T>(P, S, s, n, ctr, events, perm, stream);
 }
 
 template <class T>
 hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G, const SoA &s,
-                    int n, unsigned long long *ctr, bool events, const uint32_t *perm, hipStream_t stream) {
+                    int n, unsigned long long *ctr, bool events, const uint32_t *perm, int stages,
+                    hipStream_t stream) {
     const size_t plane = (size_t)P.eta * P.xi;
     const int nlev = P.s_w - 1;
     Packed<T> S;
@@ -1267,10 +1271,13 @@ hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packB
     S.xi = P.xi;
     S.nlev = nlev;
     S.plane = plane;
-    pack_kernel<T><<<grid_blocks(plane * nlev), 256, 0, stream>>>(F, G.hf, (UV4<T> *)S.uv, (WA8<T> *)S.wa,
-                                                                  (ZH *)S.z, P.s_rho, P.s_w, plane);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || n <= 0) return e;
+    if (stages & kStagePack) {
+        pack_kernel<T><<<grid_blocks(plane * nlev), 256, 0, stream>>>(F, G.hf, (UV4<T> *)S.uv, (WA8<T> *)S.wa,
+                                                                      (ZH *)S.z, P.s_rho, P.s_w, plane);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (!(stages & kStageMove) || n <= 0) return hipSuccess;
     return launch_move<T>(P, S, s, n, ctr, events, perm, stream);
 }
 
@@ -1384,14 +1391,14 @@ size_t fast_pack_bytes(int store, int s_w, int eta, int xi) {
 
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
                                 const GridPtrs &G, const SoA &s, int n, unsigned long long *ctr, bool events,
-                                const uint32_t *perm, hipStream_t stream) {
-    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
-    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
+                                const uint32_t *perm, int stages, hipStream_t stream) {
+    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
+    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
     if (store == WC_STORE_F16_TILE)
-        return launch_tiled_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
+        return launch_tiled_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
     if (store == WC_STORE_BF16_TILE)
-        return launch_tiled_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
-    return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, perm, stream);
+        return launch_tiled_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
+    return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
 }
 
 }  // namespace wc

@@ -16,6 +16,7 @@
 //     instead of 30 and produces identical bits.
 // Passive particles only (driftModel==0); the host rejects the rest.
 #include "wc_device.h"
+#include "wc_launch.h"
 
 #pragma clang fp contract(off)
 
@@ -458,7 +459,7 @@ size_t parity_pack_bytes(int s_w, size_t plane) {
 
 hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G, void *packBuf,
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
-                                  bool events, const uint32_t *perm, hipStream_t stream) {
+                                  bool events, const uint32_t *perm, int stages, hipStream_t stream) {
     const size_t plane = (size_t)P.eta * P.xi;
     const int nlev = P.s_w - 1;
     char *base = (char *)packBuf;
@@ -474,12 +475,15 @@ hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &
     S.nlev = nlev;
     S.s_w = P.s_w;
     S.plane = plane;
-    size_t blocks = (plane * nlev + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    pack_parity_kernel<<<(int)blocks, 256, 0, stream>>>(F, G.h, (PackUV *)S.uv, (PackWA *)S.wa, (PackZD *)S.z,
-                                                        P.s_rho, P.s_w, plane);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || n <= 0) return e;
+    if (stages & kStagePack) {
+        size_t blocks = (plane * nlev + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        pack_parity_kernel<<<(int)blocks, 256, 0, stream>>>(F, G.h, (PackUV *)S.uv, (PackWA *)S.wa, (PackZD *)S.z,
+                                                            P.s_rho, P.s_w, plane);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (!(stages & kStageMove) || n <= 0) return hipSuccess;
     const int block = 256;
     if (events)
         move_parity_kernel<PackedSampler, SoAAccess, true><<<(n + block - 1) / block, block, 0, stream>>>(
