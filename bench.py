@@ -47,7 +47,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4     # wave64 VALU instructions per second (MI355X)
 COUNTERS = os.path.join(ROOT, "profiles", "move_counters.json")
-STORES = ("bf16_tile", "f16_tile", "bf16", "f16", "f32")
+STORES = ("i16_tile", "bf16_tile", "f16_tile", "bf16", "f16", "f32")
 
 
 def log(msg):
@@ -334,13 +334,16 @@ def cpu_baseline(ctx, W, cfg, p0, runs=3):
 def precision_vs_f64(ctx, W, p, steps, cfg):
     """The mixed-precision path against the FP64 path after `steps` hours,
     diffusion off, each particle offset by N(0, 0.25^2) cells: per-particle
-    max(|dk|, |dj|, |di|) over the particles alive on both paths, and over
-    the closure-free ones (no shore or open-boundary closure and no midpoint
+    max(|dk|, |dj|, |di|) over the particles alive on both paths, over the
+    closure-free ones (no shore or open-boundary closure and no midpoint
     fallback at land on either path; the rest are held on coast node lines by
-    the reflection closure).  With diffusion on, the relative L2 of the count
-    grid against the FP64 path and against a second FP64 seed."""
+    the reflection closure), and over the open-water ones (closure-free, and
+    no stencil with a land node on either path, where the bilinear operator
+    jumps to the land zeros).  With diffusion on, the relative L2 of the
+    count grid against the FP64 path and against a second FP64 seed."""
     import wacommplusplus_amd as wc
     pin = wc.EVENT["shore"] | wc.EVENT["domain"] | wc.EVENT["fallback"]
+    coastal = pin | wc.EVENT["coast"]
     det = cfg.copy(random=False)
 
     def run(path, store, c, parts, events):
@@ -363,6 +366,7 @@ def precision_vs_f64(ctx, W, p, steps, cfg):
     res = {"hours": steps, "diffusion": "off", "particles": len(p),
            "release": "50 sources, per-particle N(0, 0.25^2)-cell offsets",
            "closure_free": "no WC_EVENT_SHORE, DOMAIN or FALLBACK on either path",
+           "open_water": "closure-free and no WC_EVENT_COAST (a stencil with a land node) on either path",
            "conc_l2": {"diffusion": "on", "particles": len(rnd),
                        "f64_seed_to_seed": float(np.linalg.norm(g1 - g0) / nrm)}}
     for store in STORES:
@@ -370,8 +374,9 @@ def precision_vs_f64(ctx, W, p, steps, cfg):
         both = (a["health"] > 0) & (ref["health"] > 0)
         d = np.max(np.stack([np.abs(a[f] - ref[f]) for f in ("k", "j", "i")]), axis=0)
         free = both & (((ev | ev_ref) & pin) == 0)
+        open_water = both & (((ev | ev_ref) & coastal) == 0)
         r = {}
-        for name, sel in (("alive", both), ("closure_free", free)):
+        for name, sel in (("alive", both), ("closure_free", free), ("open_water", open_water)):
             x = d[sel]
             q = np.quantile(x, [0.5, 0.9, 0.99])
             r[name] = dict(fraction=float(sel.mean()), p50_cells=float(q[0]), p90_cells=float(q[1]),
@@ -383,7 +388,7 @@ def precision_vs_f64(ctx, W, p, steps, cfg):
     return res
 
 
-EULER_VARIANTS = ("f64", "bf16_tile")
+EULER_VARIANTS = ("f64", "i16_tile", "bf16")
 
 
 def profile_runs(ctx, W, cfg, p0, variants, tiles, args):
@@ -563,7 +568,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "variants": variants_out,
-        "north_star_kernel": "fast path, bf16_tile storage (variants.bf16_tile)",
+        "north_star_kernel": "fast path, 16-bit storage: i16_tile (block-scaled int16) for the drift bar, bf16 "
+                             "for throughput (variants.i16_tile, variants.bf16)",
         "precision_vs_f64": precision,
         "pcie_inclusive_value": pcie,
         "roms_normalize": roms,

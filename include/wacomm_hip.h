@@ -119,8 +119,22 @@ enum { WC_INTEGRATOR_EULER = 1, WC_INTEGRATOR_MIDPOINT = 2 };
  *     (FP16) significant bits resolve the variation inside a tile instead of
  *     the whole value; AKt and zeta are rounded to the 16-bit type.  Each
  *     tile keeps its own copy of the 5 x 5 nodes its cells' stencils read, so
- *     a stencil needs one mean.  Mixed-precision path only. */
-enum { WC_STORE_F32 = 0, WC_STORE_F16 = 1, WC_STORE_BF16 = 2, WC_STORE_BF16_TILE = 3, WC_STORE_F16_TILE = 4 };
+ *     a stencil needs one mean.  Mixed-precision path only.
+ *   WC_STORE_I16_TILE: the same tiles, each residual an int16 multiple of a
+ *     power-of-two scale per tile (one per field and W level, shared by the
+ *     two records; the mean an integer multiple of it), so every node of a
+ *     tile is resolved to 2^-16 of the tile's largest deviation from its mean
+ *     (block floating point); AKt as int16 multiples of its own tile scale,
+ *     zeta kept in FP32.  Land zeros decode to exactly 0.  Mixed-precision
+ *     path only. */
+enum {
+    WC_STORE_F32 = 0,
+    WC_STORE_F16 = 1,
+    WC_STORE_BF16 = 2,
+    WC_STORE_BF16_TILE = 3,
+    WC_STORE_F16_TILE = 4,
+    WC_STORE_I16_TILE = 5
+};
 
 /* Closure events of a particle's trajectory, OR-ed per particle by the move
  * kernels while event tracking is on (wc_set_event_tracking):

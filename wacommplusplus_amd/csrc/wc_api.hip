@@ -686,7 +686,7 @@ int wc_set_grid(wc_ctx *c, int s_w, int s_rho, int eta, int xi, const double *lo
         !latRad || !h || !depthIntervals)
         return fail(c, WC_ERR_INVALID, "invalid grid: s_w=%d s_rho=%d eta=%d xi=%d slots=%d", s_w, s_rho, eta, xi,
                     n_slots);
-    if (store < WC_STORE_F32 || store > WC_STORE_F16_TILE)
+    if (store < WC_STORE_F32 || store > WC_STORE_I16_TILE)
         return fail(c, WC_ERR_INVALID, "unknown storage type %d", store);
     if ((size_t)s_w * eta * xi > 0xFFFFFFFFull) return fail(c, WC_ERR_INVALID, "grid too large");
     HIPCHK(c, hipSetDevice(c->device));

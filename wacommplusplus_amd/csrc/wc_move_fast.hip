@@ -238,11 +238,9 @@ struct TilePacked {
 constexpr int kTile = 4;
 constexpr int kTileNodes = (kTile + 1) * (kTile + 1);
 
-template <class T>
-struct TiledPacked {
-    const UV4<T> *uvr;
-    const WA8<T> *war;
-    const WA8<T> *tm;
+// Geometry shared by the tiled samplers: the tile of a cell, the node of a
+// stencil's first corner inside the tile's (kTile+1)^2 block, zeta/h, metric.
+struct TileBase {
     const ZH *z;
     const float2 *md;
     const float *depthf;
@@ -259,12 +257,25 @@ struct TiledPacked {
     __device__ __forceinline__ unsigned node(int j, int i) const {
         return ((unsigned)j % kTile) * (kTile + 1) + (unsigned)i % kTile;
     }
-
     __device__ __forceinline__ float hc(int j, int i, const W4 &w, float al) const {
         const size_t c = cell(j, i);
         const ZH a = z[c], b = z[c + 1], d = z[c + xi], e = z[c + xi + 1];
         return hc4(a, b, d, e, w, al);
     }
+    __device__ __forceinline__ float2 md_at(int j, int i) const { return md[cell(j, i)]; }
+    __device__ __forceinline__ float depth_at(int q) const { return depthf[q]; }
+    __device__ __forceinline__ bool coastal(int j, int i) const {
+        const size_t c = cell(j, i);
+        return z[c].h == 0.f || z[c + 1].h == 0.f || z[c + xi].h == 0.f || z[c + xi + 1].h == 0.f;
+    }
+};
+
+template <class T>
+struct TiledPacked : TileBase {
+    const UV4<T> *uvr;
+    const WA8<T> *war;
+    const WA8<T> *tm;
+
     __device__ __forceinline__ void uv_at(int kI, int j, int i, const W4 &w, float al, float &uu, float &vv) const {
         const size_t t = tile(kI, j, i);
         const UV4<T> *p = uvr + t * kTileNodes + node(j, i);
@@ -302,11 +313,72 @@ struct TiledPacked {
         ww = w.w00 * w0 + w.w01 * w1 + w.w10 * w2 + w.w11 * w3;
         aa = w.w00 * a0 + w.w01 * a1 + w.w10 * a2 + w.w11 * a3;
     }
-    __device__ __forceinline__ float2 md_at(int j, int i) const { return md[cell(j, i)]; }
-    __device__ __forceinline__ float depth_at(int q) const { return depthf[q]; }
-    __device__ __forceinline__ bool coastal(int j, int i) const {
-        const size_t c = cell(j, i);
-        return z[c].h == 0.f || z[c + 1].h == 0.f || z[c + xi].h == 0.f || z[c + xi + 1].h == 0.f;
+};
+
+// Block-scaled storage (WC_STORE_I16_TILE): the tile layout above, each
+// residual an int16 multiple q of a power-of-two scale s of its tile (one
+// per field and W level, shared by both records) and each mean an integer
+// multiple of s (tile_scale_kernel).  A corner's value is
+//   s * lerp(q(t0), q(t1), al) + lerp(m(t0), m(t1), al),
+// off by at most s/2 per record: about 2^-16 of the tile's largest deviation
+// from its mean.  Scaling by a power of two commutes with rounding, so with
+// m(t) = s * M(t) the two terms of a land node (q(t) = -M(t)) cancel exactly
+// and zeros decode to exactly 0.  AKt has no mean (q = AKt / s_akt).
+struct alignas(16) TileScaleUV {
+    float m[4];           // u(t0), u(t1), v(t0), v(t1)
+    float s[2];           // u, v
+    float pad[2];
+};
+struct alignas(16) TileScaleWA {
+    float m[4];           // w_k(t0), w_k(t1), w_k-1(t0), w_k-1(t1)
+    float s[4];           // w_k, w_k-1, AKt_k, AKt_k-1
+};
+
+__device__ __forceinline__ float i16f(int16_t x) { return (float)x; }
+
+struct TiledI16 : TileBase {
+    const UV4<int16_t> *uvr;
+    const WA8<int16_t> *war;
+    const TileScaleUV *tuv;
+    const TileScaleWA *twa;
+
+    __device__ __forceinline__ void uv_at(int kI, int j, int i, const W4 &w, float al, float &uu, float &vv) const {
+        const size_t t = tile(kI, j, i);
+        const UV4<int16_t> *p = uvr + t * kTileNodes + node(j, i);
+        const UV4<int16_t> a = p[0], b = p[1], d = p[kTile + 1], e = p[kTile + 2];
+        const TileScaleUV m = tuv[t];
+        const float Lu = lerpf(m.m[0], m.m[1], al), Lv = lerpf(m.m[2], m.m[3], al);
+        auto cu = [&](const UV4<int16_t> &q) { return fmaf(m.s[0], lerpf(i16f(q.v[0]), i16f(q.v[1]), al), Lu); };
+        auto cv = [&](const UV4<int16_t> &q) { return fmaf(m.s[1], lerpf(i16f(q.v[2]), i16f(q.v[3]), al), Lv); };
+        uu = w.w00 * cu(a) + w.w01 * cu(b) + w.w10 * cu(d) + w.w11 * cu(e);
+        vv = w.w00 * cv(a) + w.w01 * cv(b) + w.w10 * cv(d) + w.w11 * cv(e);
+    }
+    __device__ __forceinline__ void wa_at(int kI, float kF, int j, int i, const W4 &w, float alw, float ala,
+                                          bool wantA, float &ww, float &aa) const {
+        const size_t t = tile(kI, j, i);
+        const WA8<int16_t> *p = war + t * kTileNodes + node(j, i);
+        const WA8<int16_t> a = p[0], b = p[1], d = p[kTile + 1], e = p[kTile + 2];
+        const TileScaleWA m = twa[t];
+        const float Lt = lerpf(m.m[0], m.m[1], alw), Lb = lerpf(m.m[2], m.m[3], alw);
+        auto corner = [&](const WA8<int16_t> &q, float &wc, float &ac) {
+            const float wt = fmaf(m.s[0], lerpf(i16f(q.v[0]), i16f(q.v[1]), alw), Lt);
+            const float wb = fmaf(m.s[1], lerpf(i16f(q.v[2]), i16f(q.v[3]), alw), Lb);
+            wc = lerpf(wt, wb, kF);
+            if (wantA) {
+                const float at = m.s[2] * lerpf(i16f(q.v[4]), i16f(q.v[5]), ala);
+                const float ab = m.s[3] * lerpf(i16f(q.v[6]), i16f(q.v[7]), ala);
+                ac = lerpf(at, ab, kF);
+            } else {
+                ac = 0.f;
+            }
+        };
+        float w0, w1, w2, w3, a0, a1, a2, a3;
+        corner(a, w0, a0);
+        corner(b, w1, a1);
+        corner(d, w2, a2);
+        corner(e, w3, a3);
+        ww = w.w00 * w0 + w.w01 * w1 + w.w10 * w2 + w.w11 * w3;
+        aa = w.w00 * a0 + w.w01 * a1 + w.w10 * a2 + w.w11 * a3;
     }
 };
 
@@ -734,6 +806,134 @@ __global__ void tile_pack_kernel(RecordPair<float> F, const float *__restrict__ 
             b.v[6] = fromf<T>(F.akt0[wb]); b.v[7] = fromf<T>(F.akt1[wb]);
             if (lv == 0 && dj < kTile && di < kTile)
                 z[c] = ZH{tof<T>(fromf<T>(F.zeta0[c])), tof<T>(fromf<T>(F.zeta1[c])), hf[c], 0.0f};
+        }
+        uvr[q] = a;
+        war[q] = b;
+    }
+}
+
+// Smallest power of two >= x / cap (x >= 0), kept within [2^-100, 2^100]
+// (2^-100 for x = 0), so every scaled int16 stays a normal float.
+__device__ __forceinline__ double pow2_at_least(double x, double cap) {
+    int e = 0;
+    const double f = frexp(x / cap, &e);     // x / cap = f * 2^e, f in [0.5, 1)
+    if (f == 0.0) return ldexp(1.0, -100);
+    if (e < -100) e = -100;
+    if (e > 100) e = 100;
+    return ldexp(1.0, e);
+}
+
+// Block scales of the interval's record pair, one work-item per (level,
+// tile, group), the six groups being u, v (rho level k), w at W levels k and
+// k-1, and AKt at the same two levels.  For the four velocity groups: the
+// mean (in FP64) of each record's non-zero values among the tile's
+// (kTile+1)^2 nodes, a power-of-two scale covering the largest deviation of
+// either record from its mean (zeros included) in 32766 steps and keeping
+// |mean / scale| below 2^23, and each mean rounded to a multiple of it
+// (exact in FP32).  AKt: a scale covering its largest magnitude.
+__global__ void tile_scale_kernel(RecordPair<float> F, TileScaleUV *__restrict__ tuv,
+                                  TileScaleWA *__restrict__ twa, int s_rho, int s_w, int eta, int xi, int tilesX,
+                                  size_t tplane) {
+    const int nlev = s_w - 1;
+    const size_t plane = (size_t)eta * xi, total = tplane * nlev * 6;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t lt = q / 6;
+        const int g = (int)(q - lt * 6);
+        const size_t lv = lt / tplane, t = lt - lv * tplane;
+        const int k = (int)lv - (nlev - 1);
+        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
+        const size_t r0 = (size_t)(k + s_rho - 1) * plane, wt = (size_t)(k + s_w - 1) * plane, wb = wt - plane;
+        const int j0 = tj * kTile, i0 = ti * kTile;
+        const int j1 = min(j0 + kTile, eta - 1), i1 = min(i0 + kTile, xi - 1);
+        const float *a, *b;                     // the group's two records
+        switch (g) {
+            case 0: a = F.u0 + r0; b = F.u1 + r0; break;
+            case 1: a = F.v0 + r0; b = F.v1 + r0; break;
+            case 2: a = F.w0 + wt; b = F.w1 + wt; break;
+            case 3: a = F.w0 + wb; b = F.w1 + wb; break;
+            case 4: a = F.akt0 + wt; b = F.akt1 + wt; break;
+            default: a = F.akt0 + wb; b = F.akt1 + wb; break;
+        }
+        if (g >= 4) {                           // AKt: no mean
+            double m = 0.0;
+            for (int j = j0; j <= j1; ++j)
+                for (int i = i0; i <= i1; ++i) {
+                    const size_t c = (size_t)j * xi + i;
+                    m = fmax(m, fmax(fabs((double)a[c]), fabs((double)b[c])));
+                }
+            twa[lt].s[g - 2] = (float)pow2_at_least(m, 32767.0);
+            continue;
+        }
+        double sa = 0.0, sb = 0.0;
+        int na = 0, nb = 0;
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                const size_t c = (size_t)j * xi + i;
+                const float va = a[c], vb = b[c];
+                if (va != 0.0f) { sa += va; ++na; }
+                if (vb != 0.0f) { sb += vb; ++nb; }
+            }
+        const double ma = na ? sa / na : 0.0, mb = nb ? sb / nb : 0.0;
+        double dev = 0.0;
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                const size_t c = (size_t)j * xi + i;
+                dev = fmax(dev, fmax(fabs((double)a[c] - ma), fabs((double)b[c] - mb)));
+            }
+        const double sc = fmax(pow2_at_least(dev, 32766.0), pow2_at_least(fmax(fabs(ma), fabs(mb)), 8388608.0));
+        const float qa = (float)(rint(ma / sc) * sc), qb = (float)(rint(mb / sc) * sc);
+        if (g < 2) {
+            tuv[lt].m[2 * g] = qa;
+            tuv[lt].m[2 * g + 1] = qb;
+            tuv[lt].s[g] = (float)sc;
+            if (g == 0) tuv[lt].pad[0] = tuv[lt].pad[1] = 0.f;
+        } else {
+            twa[lt].m[2 * (g - 2)] = qa;
+            twa[lt].m[2 * (g - 2) + 1] = qb;
+            twa[lt].s[g - 2] = (float)sc;
+        }
+    }
+}
+
+// (v - m) / s to the nearest integer; |v - m| <= 32766.5 s by construction
+// of the scale, the clamp only guards the int16 range.
+__device__ __forceinline__ int16_t quant16(float v, float m, float s) {
+    const double q = rint(((double)v - (double)m) / (double)s);
+    return (int16_t)fmin(fmax(q, -32767.0), 32767.0);
+}
+
+// int16 residuals in the tile layout, one work-item per (level, tile, node);
+// nodes beyond the grid hold zeros.  Level 0 also writes zeta and h (FP32).
+__global__ void tile_pack_i16_kernel(RecordPair<float> F, const float *__restrict__ hf,
+                                     const TileScaleUV *__restrict__ tuv, const TileScaleWA *__restrict__ twa,
+                                     UV4<int16_t> *__restrict__ uvr, WA8<int16_t> *__restrict__ war,
+                                     ZH *__restrict__ z, int s_rho, int s_w, int eta, int xi, int tilesX,
+                                     size_t tplane) {
+    const int nlev = s_w - 1;
+    const size_t plane = (size_t)eta * xi, total = tplane * nlev * kTileNodes;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t lt = q / kTileNodes;
+        const int nd = (int)(q - lt * kTileNodes);
+        const size_t lv = lt / tplane, t = lt - lv * tplane;
+        const int k = (int)lv - (nlev - 1);
+        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
+        const int dj = nd / (kTile + 1), di = nd - dj * (kTile + 1);
+        const int j = tj * kTile + dj, i = ti * kTile + di;
+        UV4<int16_t> a{};
+        WA8<int16_t> b{};
+        if (j < eta && i < xi) {
+            const size_t c = (size_t)j * xi + i;
+            const size_t r = (size_t)(k + s_rho - 1) * plane + c;
+            const size_t wt = (size_t)(k + s_w - 1) * plane + c, wb = wt - plane;
+            const TileScaleUV mu = tuv[lt];
+            const TileScaleWA mw = twa[lt];
+            a.v[0] = quant16(F.u0[r], mu.m[0], mu.s[0]); a.v[1] = quant16(F.u1[r], mu.m[1], mu.s[0]);
+            a.v[2] = quant16(F.v0[r], mu.m[2], mu.s[1]); a.v[3] = quant16(F.v1[r], mu.m[3], mu.s[1]);
+            b.v[0] = quant16(F.w0[wt], mw.m[0], mw.s[0]); b.v[1] = quant16(F.w1[wt], mw.m[1], mw.s[0]);
+            b.v[2] = quant16(F.w0[wb], mw.m[2], mw.s[1]); b.v[3] = quant16(F.w1[wb], mw.m[3], mw.s[1]);
+            b.v[4] = quant16(F.akt0[wt], 0.f, mw.s[2]); b.v[5] = quant16(F.akt1[wt], 0.f, mw.s[2]);
+            b.v[6] = quant16(F.akt0[wb], 0.f, mw.s[3]); b.v[7] = quant16(F.akt1[wb], 0.f, mw.s[3]);
+            if (lv == 0 && dj < kTile && di < kTile) z[c] = ZH{F.zeta0[c], F.zeta1[c], hf[c], 0.0f};
         }
         uvr[q] = a;
         war[q] = b;
@@ -1222,6 +1422,60 @@ TileLayout tile_layout(size_t tbytes, int s_w, int eta, int xi) {
     return L;
 }
 
+// Byte offsets of the block-scaled layout: residuals as the tile-mean layout
+// with T = int16, then the two scale arrays in place of the means.
+struct TileLayoutI16 {
+    size_t uvr, war, tuv, twa, z, total;
+    int tilesX;
+    size_t tplane;
+};
+TileLayoutI16 tile_layout_i16(int s_w, int eta, int xi) {
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const TileLayout B = tile_layout(2, s_w, eta, xi);
+    const size_t blocks = (size_t)(s_w - 1) * B.tplane;
+    TileLayoutI16 L;
+    L.tilesX = B.tilesX;
+    L.tplane = B.tplane;
+    L.uvr = B.uvr;
+    L.war = B.war;
+    L.tuv = B.tm;
+    L.twa = up(L.tuv + blocks * sizeof(TileScaleUV));
+    L.z = up(L.twa + blocks * sizeof(TileScaleWA));
+    L.total = up(L.z + (size_t)eta * xi * sizeof(ZH));
+    return L;
+}
+
+hipError_t launch_tiled_i16(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G,
+                            const SoA &s, int n, unsigned long long *ctr, bool events, const uint32_t *perm,
+                            int stages, hipStream_t stream) {
+    const TileLayoutI16 L = tile_layout_i16(P.s_w, P.eta, P.xi);
+    char *base = (char *)packBuf;
+    TiledI16 S;
+    S.uvr = (const UV4<int16_t> *)(base + L.uvr);
+    S.war = (const WA8<int16_t> *)(base + L.war);
+    S.tuv = (const TileScaleUV *)(base + L.tuv);
+    S.twa = (const TileScaleWA *)(base + L.twa);
+    S.z = (const ZH *)(base + L.z);
+    S.md = (const float2 *)G.idxf;
+    S.depthf = G.depthf;
+    S.xi = P.xi;
+    S.nlev = P.s_w - 1;
+    S.tilesX = L.tilesX;
+    S.tplane = L.tplane;
+    if (stages & kStagePack) {
+        const size_t blocks = L.tplane * (size_t)S.nlev;
+        tile_scale_kernel<<<grid_blocks(blocks * 6), 256, 0, stream>>>(F, (TileScaleUV *)S.tuv, (TileScaleWA *)S.twa,
+                                                                   P.s_rho, P.s_w, P.eta, P.xi, L.tilesX, L.tplane);
+        tile_pack_i16_kernel<<<grid_blocks(blocks * kTileNodes), 256, 0, stream>>>(
+            F, G.hf, S.tuv, S.twa, (UV4<int16_t> *)S.uvr, (WA8<int16_t> *)S.war, (ZH *)S.z, P.s_rho, P.s_w, P.eta,
+            P.xi, L.tilesX, L.tplane);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (!(stages & kStageMove) || n <= 0) return hipSuccess;
+    return launch_move<int16_t>(P, S, s, n, ctr, events, perm, stream);
+}
+
 template <class T>
 hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G,
                           const SoA &s, int n, unsigned long long *ctr, bool events, const uint32_t *perm,
@@ -1384,6 +1638,7 @@ hipError_t launch_ens_step(int store, const EnsLaunch &L, hipStream_t stream) {
 
 size_t fast_pack_bytes(int store, int s_w, int eta, int xi) {
     if (store == WC_STORE_BF16_TILE || store == WC_STORE_F16_TILE) return tile_layout(2, s_w, eta, xi).total;
+    if (store == WC_STORE_I16_TILE) return tile_layout_i16(s_w, eta, xi).total;
     const size_t nlev = (size_t)(s_w - 1), plane = (size_t)eta * xi;
     const size_t t = store == WC_STORE_F32 ? 4 : 2;
     return plane * nlev * (4 * t + 8 * t) + plane * sizeof(ZH);
@@ -1398,6 +1653,7 @@ hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair
         return launch_tiled_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
     if (store == WC_STORE_BF16_TILE)
         return launch_tiled_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
+    if (store == WC_STORE_I16_TILE) return launch_tiled_i16(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
     return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
 }
 

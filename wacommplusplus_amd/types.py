@@ -15,7 +15,7 @@ CONSTRAINT, KILL, REFLECTION = 1, 2, 3     # Config.hpp:201-203
 EULER, MIDPOINT = 1, 2                     # Config.hpp:206-207
 FORWARD, BACKWARD = 1, -1                  # Config.hpp:204-205
 
-STORE = {"f32": 0, "f16": 1, "bf16": 2, "bf16_tile": 3, "f16_tile": 4}
+STORE = {"f32": 0, "f16": 1, "bf16": 2, "bf16_tile": 3, "f16_tile": 4, "i16_tile": 5}
 # closure-event bits of wc_get_events
 EVENT = {"shore": 1, "domain": 2, "fallback": 4, "vertical": 8, "shallow": 16, "coast": 32}
 PATH = {"parity": 0, "fast": 1}
