@@ -374,9 +374,10 @@ int wc_particle_count(const wc_ctx *ctx, size_t *n);
 int wc_set_event_tracking(wc_ctx *ctx, int enabled);
 
 /* Cell-tile order of the moves (no counterpart in the reference): with
- * tile > 0, wc_move_interval and wc_grid visit the particles tile by tile of
- * tile x tile cells, so the lanes of a wavefront gather forcing from
- * neighbouring cells.  The order is a permutation kept beside the population,
+ * tile > 0, wc_move_interval visits the particles tile by tile of tile x
+ * tile cells, so the lanes of a wavefront gather forcing from neighbouring
+ * cells (wc_grid keeps the population order: its per-block LDS hash already
+ * aggregates equal cells, and the permuted reads measured slower).  The order is a permutation kept beside the population,
  * which itself stays in the reference's order (every result, and
  * wc_get_particles, is unchanged); it is recomputed every period-th move and
  * follows compaction, release and appends.  tile 0 (the default) turns it

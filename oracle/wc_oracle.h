@@ -35,7 +35,12 @@
  *     trajectory of tests/ApplicationEndToEndTest.cpp:252-257, the diffusion
  *     variance sigma_k^2 per step of docs/model.md:204-210, and closure
  *     invariants.  Tests also plant mistakes in this oracle (WCO_PLANT) to
- *     show that each pin catches a plausible error.
+ *     show that each pin catches a plausible error.  What these pins leave
+ *     open: the float casts and rounding points of Particle.cpp:186-202,
+ *     271-290 and 356-474 are followed by reading, and no reference-held
+ *     vector or closed form fixes them at 1e-9 (the pins use exactly
+ *     representable fields), so at that level wco_move's rounding points are
+ *     parity unpinned against the reference.
  *   - wco_grid_count (Concentration::evaluate): tests/ConcentrationTest.cpp
  *     and brute force on tiny inputs.
  *   - wco_kji2deplatlon / wco_particle_records: the inverse deplatlon2kji

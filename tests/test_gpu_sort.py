@@ -40,7 +40,8 @@ def run(ctx, W, cfg, p, n, path, store, tile, period, extra=None):
     return out, frames
 
 
-@pytest.mark.parametrize("path,store", [("parity", "f32"), ("fast", "bf16_tile"), ("fast", "bf16")])
+@pytest.mark.parametrize("path,store", [("parity", "f32"), ("fast", "bf16_tile"), ("fast", "bf16"),
+                                        ("fast", "i16_tile")])
 def test_sorted_moves_equal_unsorted(full, path, store):
     p = release(full, 4_000_000, 50, spread=2.0)[::20].copy()
     extra = release(full, 4_000_000, 50, spread=2.0)[7::400].copy()
