@@ -213,6 +213,22 @@ hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, 
                            long long P,
                            float *dd, float *dwh, float *loss, hipStream_t s);
 hipError_t launch_flip(const uint16_t *w, int cout, int cin, uint16_t *wf, hipStream_t s);
+// Where each conv layer's weights sit: FP32 parameters [w_off, w_off +
+// cout*9*cin) as [cout][9][cin]; their BF16 copy at wb_off of wb, and the
+// same weights flipped and transposed for the data gradient,
+// w'[ci][8 - tap][co], at wb_off of wf.
+constexpr int kMaxConvLayers = 16;
+struct WeightMap {
+    int layers;
+    int cout[kMaxConvLayers], cin[kMaxConvLayers];
+    long long w_off[kMaxConvLayers], wb_off[kMaxConvLayers];
+};
+// Adam on every parameter, then the BF16 and flipped BF16 copies of the conv
+// weights from the updated values (one pass).
+hipError_t launch_adam_bf16(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,
+                            float eps, int t, const WeightMap &map, uint16_t *wb, uint16_t *wf, hipStream_t s);
+// The BF16 and flipped BF16 copies of the conv weights alone.
+hipError_t launch_weights_bf16(const float *p, const WeightMap &map, uint16_t *wb, uint16_t *wf, hipStream_t s);
 hipError_t launch_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,
                        float eps, int t, hipStream_t s);
 hipError_t launch_to_bf16(const float *x, long long n, uint16_t *y, hipStream_t s);

@@ -757,6 +757,46 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
     }
 }
 
+// The BF16 copy of a conv weight and its flipped, transposed copy
+// (w'[ci][8 - tap][co] = w[co][tap][ci], as flip_kernel), if parameter e is a
+// conv weight.
+__device__ __forceinline__ void weight_copies(long long e, float x, const WeightMap &map, uint16_t *__restrict__ wb,
+                                              uint16_t *__restrict__ wf) {
+    for (int l = 0; l < map.layers; ++l) {
+        const long long idx = e - map.w_off[l];
+        const int cout = map.cout[l], cin = map.cin[l];
+        if (idx < 0 || idx >= (long long)cout * 9 * cin) continue;
+        const int ci = (int)(idx % cin), t = (int)(idx / cin);
+        const int tap = t % 9, co = t / 9;
+        const uint16_t h = to_bf16(x);
+        wb[map.wb_off[l] + idx] = h;
+        wf[map.wb_off[l] + ((long long)ci * 9 + (8 - tap)) * cout + co] = h;
+        return;
+    }
+}
+
+// Adam as adam_kernel, then the BF16 copies of the updated conv weights.
+__global__ void adam_bf16_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                                 float *__restrict__ v, long long n, float lr, float b1, float b2, float eps, float c1,
+                                 float c2, WeightMap map, uint16_t *__restrict__ wb, uint16_t *__restrict__ wf) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const float gi = g[e];
+        const float mi = b1 * m[e] + (1.f - b1) * gi;
+        const float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+        m[e] = mi;
+        v[e] = vi;
+        const float pi = p[e] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[e] = pi;
+        weight_copies(e, pi, map, wb, wf);
+    }
+}
+
+__global__ void weights_bf16_kernel(const float *__restrict__ p, long long n, WeightMap map, uint16_t *__restrict__ wb,
+                                    uint16_t *__restrict__ wf) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        weight_copies(e, p[e], map, wb, wf);
+}
+
 __global__ void to_bf16_kernel(const float *__restrict__ x, long long n, uint16_t *__restrict__ y) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
         y[e] = to_bf16(x[e]);
@@ -838,7 +878,8 @@ hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
 hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *db,
                             float *part, size_t partCap, hipStream_t s) {
     if (cout < 4 || cout > 1024 || cout % 4 || 1024 % cout || E % cout) return hipErrorInvalidValue;
-    const int b = blocks_for(E / 4);
+    // 1024 workgroups keep the copy at full rate and the bias partials few
+    const int b = blocks_for(E / 4) < 1024 ? blocks_for(E / 4) : 1024;
     if ((size_t)b * cout > partCap) return hipErrorInvalidValue;
     mask_bf16_kernel<<<b, 256, 0, s>>>(dy, y, E / 4, cout, out, part);
     hipError_t e = hipGetLastError();
@@ -911,6 +952,20 @@ hipError_t launch_adam(float *p, const float *g, float *m, float *v, long long n
                        float eps, int t, hipStream_t s) {
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
     adam_kernel<<<blocks_for(n), 256, 0, s>>>(p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+hipError_t launch_adam_bf16(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,
+                            float eps, int t, const WeightMap &map, uint16_t *wb, uint16_t *wf, hipStream_t s) {
+    if (map.layers < 0 || map.layers > kMaxConvLayers) return hipErrorInvalidValue;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    adam_bf16_kernel<<<blocks_for(n), 256, 0, s>>>(p, g, m, v, n, lr, b1, b2, eps, c1, c2, map, wb, wf);
+    return hipGetLastError();
+}
+hipError_t launch_weights_bf16(const float *p, const WeightMap &map, uint16_t *wb, uint16_t *wf, hipStream_t s) {
+    if (map.layers < 1 || map.layers > kMaxConvLayers) return hipErrorInvalidValue;
+    const int L = map.layers - 1;
+    const long long n = map.w_off[L] + (long long)map.cout[L] * 9 * map.cin[L];
+    weights_bf16_kernel<<<blocks_for(n), 256, 0, s>>>(p, n, map, wb, wf);
     return hipGetLastError();
 }
 hipError_t launch_to_bf16(const float *x, long long n, uint16_t *y, hipStream_t s) {

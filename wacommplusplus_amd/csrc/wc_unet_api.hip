@@ -46,7 +46,8 @@ struct wc_unet {
     std::vector<Layer> layers;
     size_t nparam = 0, nwb = 0, head_off = 0;
     float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
-    uint16_t *d_wb = nullptr, *d_wf = nullptr;     // BF16 weights, flipped scratch
+    uint16_t *d_wb = nullptr, *d_wf = nullptr;     // BF16 weights, and flipped for the data gradient
+    WeightMap wmap{};
     float *d_loss = nullptr;
     float *d_part = nullptr;                     // weight-gradient partial sums (kWgradPartFloats)
     int adam_t = 0;
@@ -174,11 +175,9 @@ int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, con
     g.dyb = u->gmask;
     UCHK(u, launch_wgrad(g, u->stream));
     if (!gx1) return WC_OK;
-    const int cin = L.c1 + L.c2;
-    UCHK(u, launch_flip(u->d_wb + L.wb_off, L.cout, cin, u->d_wf, u->stream));
     ConvArgs a{};
     a.x1 = u->gmask; a.x2 = nullptr; a.m1 = nullptr; a.c1 = L.cout; a.c2 = 0;
-    a.w_ = u->d_wf; a.bias = nullptr;
+    a.w_ = u->d_wf + L.wb_off; a.bias = nullptr;
     a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
     a.n = u->n; a.h = h; a.w = w;
     UCHK(u, launch_conv3x3(a, false, u->stream));
@@ -280,8 +279,7 @@ int upload(wc_unet *u, const float *x, const float *t, const float *wgt, int n, 
 }
 
 int refresh_bf16(wc_unet *u) {
-    for (const Layer &L : u->layers)
-        UCHK(u, launch_to_bf16(u->d_p + L.w_off, (long long)L.cout * 9 * (L.c1 + L.c2), u->d_wb + L.wb_off, u->stream));
+    UCHK(u, launch_weights_bf16(u->d_p, u->wmap, u->d_wb, u->d_wf, u->stream));
     return WC_OK;
 }
 
@@ -305,7 +303,7 @@ int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
                              {4 * C, 2 * C, 0},  {4 * C, 4 * C, 0}, {8 * C, 4 * C, 0}, {8 * C, 8 * C, 0},
                              {4 * C, 4 * C, 8 * C}, {4 * C, 4 * C, 0}, {2 * C, 2 * C, 4 * C}, {2 * C, 2 * C, 0},
                              {C, C, 2 * C},      {C, C, 0}};
-    size_t off = 0, woff = 0, maxw = 0;
+    size_t off = 0, woff = 0;
     for (auto &s : spec) {
         Layer L{s[0], s[1], s[2], 0, 0, 0};
         const size_t nw = (size_t)L.cout * 9 * (L.c1 + L.c2);
@@ -315,8 +313,15 @@ int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
         off += L.cout;
         L.wb_off = woff;
         woff += nw;
-        if (nw > maxw) maxw = nw;
         u->layers.push_back(L);
+    }
+    u->wmap.layers = (int)u->layers.size();
+    for (int l = 0; l < u->wmap.layers; ++l) {
+        const Layer &L = u->layers[l];
+        u->wmap.cout[l] = L.cout;
+        u->wmap.cin[l] = L.c1 + L.c2;
+        u->wmap.w_off[l] = (long long)L.w_off;
+        u->wmap.wb_off[l] = (long long)L.wb_off;
     }
     u->head_off = off;
     off += C + 1;
@@ -344,7 +349,7 @@ int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&u->stream) != hipSuccess) return bad(WC_ERR_HIP);
     if (hipMalloc(&u->d_p, u->nparam * 4) || hipMalloc(&u->d_g, u->nparam * 4) || hipMalloc(&u->d_m, u->nparam * 4) ||
         hipMalloc(&u->d_v, u->nparam * 4) || hipMalloc(&u->d_wb, u->nwb * 2 + 256) ||
-        hipMalloc(&u->d_wf, maxw * 2 + 256) || hipMalloc(&u->d_loss, 256) ||
+        hipMalloc(&u->d_wf, u->nwb * 2 + 256) || hipMalloc(&u->d_loss, 256) ||
         hipMalloc(&u->d_part, kWgradPartFloats * sizeof(float)))
         return bad(WC_ERR_NOMEM);
     if (hipMemcpy(u->d_p, p.data(), u->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return bad(WC_ERR_HIP);
@@ -450,10 +455,8 @@ int wc_unet_train_step_weighted(wc_unet *u, const float *x, const float *t, cons
     if (!rc) rc = backward(u);
     if (rc) return rc;
     u->adam_t += 1;
-    UCHK(u, launch_adam(u->d_p, u->d_g, u->d_m, u->d_v, (long long)u->nparam, lr, 0.9f, 0.999f, 1e-8f, u->adam_t,
-                        u->stream));
-    rc = refresh_bf16(u);
-    if (rc) return rc;
+    UCHK(u, launch_adam_bf16(u->d_p, u->d_g, u->d_m, u->d_v, (long long)u->nparam, lr, 0.9f, 0.999f, 1e-8f,
+                             u->adam_t, u->wmap, u->d_wb, u->d_wf, u->stream));
     float l = 0;
     UCHK(u, hipMemcpyAsync(&l, u->d_loss, 4, hipMemcpyDeviceToHost, u->stream));
     UCHK(u, hipStreamSynchronize(u->stream));
