@@ -163,17 +163,25 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
 
 // LDS-tiled 3x3 convolution (BF16 input, no input mask).  A workgroup owns R
 // whole rows of one image (R * w <= 256 pixels) and 32 output channels.  Per
-// 32-channel input chunk each wave loads its B operands of the 9 taps (the
-// chunk's weights of its 32 output channels) into registers, and the
-// workgroup stages in LDS
+// 32-channel input chunk the workgroup stages in LDS
 //   sx[(rr, xx)][40]   input rows y0-1 .. y0+R, columns -1 .. w (zero outside
 //                      the image), 32 channels padded to 40 (80 B per pixel:
 //                      16-byte aligned, banks spread);
-// then each wave computes 64 pixels (4 MFMA row tiles) x 32 channels: per
-// tap 4 A reads from LDS feed 8 MFMAs.  The epilogue is the one of
+//   sw[tap][co][40]    the chunk's weights of the 32 output channels (WL;
+//                      otherwise each wave keeps them in registers, for
+//                      the wide rows whose stage leaves no room);
+// every work-item issues up to kStageBatch 16-byte loads before it stores
+// any, so a stage costs about one memory latency, not one per load.  Then
+// each wave computes 16 NI pixels (NI MFMA row tiles) x 32 channels: per tap
+// NI A and 2 B reads from LDS feed 2 NI MFMAs.  NI (4, 2 or 1) is the
+// smallest that gives the four waves all R * w pixels, so the small images
+// of the deep levels keep every wave busy.  The epilogue is the one of
 // conv3x3_kernel.
 constexpr int kCvPS = 40;
+constexpr int kStageBatch = 8;
+constexpr int kCvWRows = 9 * 32;          // staged weight rows per chunk
 
+template <int NI, bool WL>
 __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, int rowsPerImg) {
     extern __shared__ __align__(16) uint16_t smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -182,63 +190,88 @@ __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, in
     const int rv = min(R, H - y0), RW = R * W;
     const int co0 = blockIdx.y * 32;
     const int nout = a.cy1 + a.cy2, cin = a.c1 + a.c2;
-    uint16_t *sx = smem;                                  // [(R+2) * (W+2)][kCvPS]
+    const int npos = (R + 2) * Wp;
+    const uint16_t *sx = smem;                            // [(R+2) * (W+2)][kCvPS]
+    const uint16_t *sw = smem + npos * kCvPS;             // [9][32][kCvPS]
     const int row = lane & 15, kq = lane >> 4;
-    // this lane's A rows: stage pixel 64 wave + 16 i + row (its centre-tap
+    // this lane's A rows: stage pixel 16 NI wave + 16 i + row (its centre-tap
     // LDS offset; pixels past the stage read pixel 0 and are not stored)
-    int base[4];
-    for (int i = 0; i < 4; ++i) {
-        const int q = 64 * wave + 16 * i + row;
+    int base[NI];
+    for (int i = 0; i < NI; ++i) {
+        const int q = 16 * NI * wave + 16 * i + row;
         const int r = q < RW ? q / W : 0, x = q < RW ? q - r * W : 0;
         base[i] = ((r + 1) * Wp + (x + 1)) * kCvPS + 8 * kq;
     }
-    const bool active = 64 * wave < RW;
-    f32x4 acc[4][2];
-    for (int i = 0; i < 4; ++i)
+    const bool active = 16 * NI * wave < RW;
+    f32x4 acc[NI][2];
+    for (int i = 0; i < NI; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int npos = (R + 2) * Wp;
+    const int nx = npos * 4, nt = nx + (WL ? kCvWRows * 4 : 0);     // 16-byte groups: input, then weights
     for (int c0 = 0; c0 < cin; c0 += 32) {
         const bool first = c0 < a.c1;
         const uint16_t *xs = (const uint16_t *)(first ? a.x1 : a.x2);
         const int cc = first ? a.c1 : a.c2, cl0 = first ? c0 : c0 - a.c1;
-        // B of the 9 taps: channels c0 + 8 kq .. +7 of output channels co0 + 16 j + row
-        Pack8 B[9][2];
-        if (active)
-            for (int tap = 0; tap < 9; ++tap)
+        __syncthreads();                                  // the previous chunk's reads are done
+        constexpr int SB = WL ? kStageBatch : kStageBatch / 2;   // the register-weight kernel keeps fewer
+        for (int e0 = tid; e0 < nt; e0 += 256 * SB) {
+            uint4 v[SB];
+            int dst[SB];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int e = e0 + 256 * u;
+                v[u] = make_uint4(0, 0, 0, 0);
+                dst[u] = -1;
+                if (e < nx) {
+                    const int pos = e >> 2, g = e & 3;
+                    const int rr = pos / Wp, xx = pos - rr * Wp;
+                    const int y = y0 - 1 + rr, x = xx - 1;
+                    dst[u] = pos * kCvPS + 8 * g;
+                    if (y >= 0 && y < H && x >= 0 && x < W)
+                        v[u] = *(const uint4 *)(xs + ((size_t)img * hw + (size_t)y * W + x) * cc + cl0 + 8 * g);
+                } else if (e < nt) {
+                    const int f = e - nx, g = f & 3, tc = f >> 2;          // tc = tap * 32 + local channel
+                    const int co = co0 + (tc & 31), tap = tc >> 5;
+                    dst[u] = (npos + tc) * kCvPS + 8 * g;
+                    if (co < nout) v[u] = *(const uint4 *)(a.w_ + ((size_t)co * 9 + tap) * cin + c0 + 8 * g);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SB; ++u)
+                if (dst[u] >= 0) *(uint4 *)(smem + dst[u]) = v[u];
+        }
+        // !WL: B of the 9 taps in registers (loaded after the stage, so the
+        // two do not hold registers at once), channels c0 + 8 kq .. +7 of
+        // output channels co0 + 16 j + row
+        Pack8 Br[WL ? 1 : 9][2];
+        if (!WL && active)
+            for (int tap = 0; tap < (WL ? 1 : 9); ++tap)
                 for (int j = 0; j < 2; ++j) {
                     const int co = co0 + 16 * j + row;
-                    B[tap][j].u = co < nout ? *(const uint4 *)(a.w_ + ((size_t)co * 9 + tap) * cin + c0 + 8 * kq)
-                                            : make_uint4(0, 0, 0, 0);
+                    Br[tap][j].u = co < nout ? *(const uint4 *)(a.w_ + ((size_t)co * 9 + tap) * cin + c0 + 8 * kq)
+                                             : make_uint4(0, 0, 0, 0);
                 }
-        __syncthreads();                                  // the previous chunk's reads are done
-        int rr = (tid >> 2) / Wp, xx = (tid >> 2) - rr * Wp;          // position of e >> 2, by counting
-        for (int e = tid; e < npos * 4; e += 256) {
-            const int pos = e >> 2, g = e & 3;
-            const int y = y0 - 1 + rr, x = xx - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (y >= 0 && y < H && x >= 0 && x < W)
-                v = *(const uint4 *)(xs + ((size_t)img * hw + (size_t)y * W + x) * cc + cl0 + 8 * g);
-            *(uint4 *)(sx + pos * kCvPS + 8 * g) = v;
-            for (xx += 64; xx >= Wp; xx -= Wp) ++rr;
-        }
         __syncthreads();
         if (active)
             for (int tap = 0; tap < 9; ++tap) {
                 const int off = ((tap / 3 - 1) * Wp + (tap % 3 - 1)) * kCvPS;
-                for (int i = 0; i < 4; ++i) {
+                Pack8 B[2];
+                for (int j = 0; j < 2; ++j)
+                    B[j].u = WL ? *(const uint4 *)(sw + (tap * 32 + 16 * j + row) * kCvPS + 8 * kq)
+                                : Br[WL ? 0 : tap][j].u;
+                for (int i = 0; i < NI; ++i) {
                     Pack8 A;
                     A.u = *(const uint4 *)(sx + base[i] + off);
                     for (int j = 0; j < 2; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, B[tap][j].v, acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, B[j].v, acc[i][j], 0, 0, 0);
                 }
             }
     }
     if (!active) return;
-    // D[4 kq + r][row] of tile (i, j): stage pixel 64 wave + 16 i + 4 kq + r,
+    // D[4 kq + r][row] of tile (i, j): stage pixel 16 NI wave + 16 i + 4 kq + r,
     // channel co0 + 16 j + row; rows of the stage pixels by counting
-    int er[4][4];
-    for (int i = 0; i < 4; ++i) {
-        const int q0 = 64 * wave + 16 * i + 4 * kq;
+    int er[NI][4];
+    for (int i = 0; i < NI; ++i) {
+        const int q0 = 16 * NI * wave + 16 * i + 4 * kq;
         int rr = q0 / W, xx = q0 - rr * W;
         for (int r = 0; r < 4; ++r) {
             er[i][r] = rr;
@@ -251,9 +284,9 @@ __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, in
         const bool first = co < a.cy1;
         const int cl = first ? co : co - a.cy1;
         const int cc = first ? a.cy1 : a.cy2;
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < NI; ++i)
             for (int r = 0; r < 4; ++r) {
-                const int q = 64 * wave + 16 * i + 4 * kq + r;
+                const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
                 if (q >= RW || er[i][r] >= rv) continue;
                 // stage pixel q of image row y0 + er is image pixel y0 * W + q
                 const long long p = (long long)img * hw + (long long)y0 * W + q;
@@ -825,13 +858,24 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
         // grid has fewer than 512 workgroups, as long as the stage fits in LDS
         int R = 256 / a.w < a.h ? 256 / a.w : a.h;
         if (R < 1) R = 1;
-        auto lds = [&](int r) { return (size_t)(r + 2) * (a.w + 2) * kCvPS * sizeof(uint16_t); };
-        while (R > 1 && lds(R) > 64 * 1024) --R;
+        // stage bytes with the weights in LDS (wl) or in registers
+        auto lds = [&](int r, bool wl) {
+            return ((size_t)(r + 2) * (a.w + 2) + (wl ? kCvWRows : 0)) * kCvPS * sizeof(uint16_t);
+        };
+        while (R > 1 && lds(R, false) > 64 * 1024) --R;
         while (R > 1 && (long long)a.n * ((a.h + R - 1) / R) * (nout / 32) < 512) R = (R + 1) / 2;
-        if (lds(R) <= 64 * 1024) {
+        if (lds(R, false) <= 64 * 1024 && R * a.w <= 256) {
             const int rowsPerImg = (a.h + R - 1) / R;
             const dim3 grid((unsigned)(a.n * rowsPerImg), (unsigned)(nout / 32));
-            conv3x3_tile_kernel<<<grid, 256, lds(R), s>>>(a, R, rowsPerImg);
+            const int RW = R * a.w;
+            if (lds(R, true) > 64 * 1024)
+                conv3x3_tile_kernel<4, false><<<grid, 256, lds(R, false), s>>>(a, R, rowsPerImg);
+            else if (RW <= 64)
+                conv3x3_tile_kernel<1, true><<<grid, 256, lds(R, true), s>>>(a, R, rowsPerImg);
+            else if (RW <= 128)
+                conv3x3_tile_kernel<2, true><<<grid, 256, lds(R, true), s>>>(a, R, rowsPerImg);
+            else
+                conv3x3_tile_kernel<4, true><<<grid, 256, lds(R, true), s>>>(a, R, rowsPerImg);
             return hipGetLastError();
         }
     }
