@@ -318,7 +318,7 @@ struct TiledPacked : TileBase {
 // Block-scaled storage (WC_STORE_I16_TILE): the tile layout above, each
 // residual an int16 multiple q of a power-of-two scale s of its tile (one
 // per field and W level, shared by both records) and each mean an integer
-// multiple of s (tile_scale_kernel).  A corner's value is
+// multiple of s (tile_group_scale).  A corner's value is
 //   s * lerp(q(t0), q(t1), al) + lerp(m(t0), m(t1), al),
 // off by at most s/2 per record: about 2^-16 of the tile's largest deviation
 // from its mean.  Scaling by a power of two commutes with rounding, so with
@@ -823,121 +823,190 @@ __device__ __forceinline__ double pow2_at_least(double x, double cap) {
     return ldexp(1.0, e);
 }
 
-// Block scales of the interval's record pair, one work-item per (level,
-// tile, group), the six groups being u, v (rho level k), w at W levels k and
-// k-1, and AKt at the same two levels.  For the four velocity groups: the
-// mean (in FP64) of each record's non-zero values among the tile's
-// (kTile+1)^2 nodes, a power-of-two scale covering the largest deviation of
-// either record from its mean (zeros included) in 32766 steps and keeping
-// |mean / scale| below 2^23, and each mean rounded to a multiple of it
-// (exact in FP32).  AKt: a scale covering its largest magnitude.
-__global__ void tile_scale_kernel(RecordPair<float> F, TileScaleUV *__restrict__ tuv,
-                                  TileScaleWA *__restrict__ twa, int s_rho, int s_w, int eta, int xi, int tilesX,
-                                  size_t tplane) {
-    const int nlev = s_w - 1;
-    const size_t plane = (size_t)eta * xi, total = tplane * nlev * 6;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const size_t lt = q / 6;
-        const int g = (int)(q - lt * 6);
-        const size_t lv = lt / tplane, t = lt - lv * tplane;
-        const int k = (int)lv - (nlev - 1);
-        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
-        const size_t r0 = (size_t)(k + s_rho - 1) * plane, wt = (size_t)(k + s_w - 1) * plane, wb = wt - plane;
-        const int j0 = tj * kTile, i0 = ti * kTile;
-        const int j1 = min(j0 + kTile, eta - 1), i1 = min(i0 + kTile, xi - 1);
-        const float *a, *b;                     // the group's two records
-        switch (g) {
-            case 0: a = F.u0 + r0; b = F.u1 + r0; break;
-            case 1: a = F.v0 + r0; b = F.v1 + r0; break;
-            case 2: a = F.w0 + wt; b = F.w1 + wt; break;
-            case 3: a = F.w0 + wb; b = F.w1 + wb; break;
-            case 4: a = F.akt0 + wt; b = F.akt1 + wt; break;
-            default: a = F.akt0 + wb; b = F.akt1 + wb; break;
-        }
-        if (g >= 4) {                           // AKt: no mean
-            double m = 0.0;
-            for (int j = j0; j <= j1; ++j)
-                for (int i = i0; i <= i1; ++i) {
-                    const size_t c = (size_t)j * xi + i;
-                    m = fmax(m, fmax(fabs((double)a[c]), fabs((double)b[c])));
-                }
-            twa[lt].s[g - 2] = (float)pow2_at_least(m, 32767.0);
-            continue;
-        }
-        double sa = 0.0, sb = 0.0;
-        int na = 0, nb = 0;
-        for (int j = j0; j <= j1; ++j)
-            for (int i = i0; i <= i1; ++i) {
-                const size_t c = (size_t)j * xi + i;
-                const float va = a[c], vb = b[c];
-                if (va != 0.0f) { sa += va; ++na; }
-                if (vb != 0.0f) { sb += vb; ++nb; }
-            }
-        const double ma = na ? sa / na : 0.0, mb = nb ? sb / nb : 0.0;
-        double dev = 0.0;
-        for (int j = j0; j <= j1; ++j)
-            for (int i = i0; i <= i1; ++i) {
-                const size_t c = (size_t)j * xi + i;
-                dev = fmax(dev, fmax(fabs((double)a[c] - ma), fabs((double)b[c] - mb)));
-            }
-        const double sc = fmax(pow2_at_least(dev, 32766.0), pow2_at_least(fmax(fabs(ma), fabs(mb)), 8388608.0));
-        const float qa = (float)(rint(ma / sc) * sc), qb = (float)(rint(mb / sc) * sc);
-        if (g < 2) {
-            tuv[lt].m[2 * g] = qa;
-            tuv[lt].m[2 * g + 1] = qb;
-            tuv[lt].s[g] = (float)sc;
-            if (g == 0) tuv[lt].pad[0] = tuv[lt].pad[1] = 0.f;
-        } else {
-            twa[lt].m[2 * (g - 2)] = qa;
-            twa[lt].m[2 * (g - 2) + 1] = qb;
-            twa[lt].s[g - 2] = (float)sc;
-        }
-    }
-}
-
-// (v - m) / s to the nearest integer; |v - m| <= 32766.5 s by construction
-// of the scale, the clamp only guards the int16 range.
-__device__ __forceinline__ int16_t quant16(float v, float m, float s) {
-    const double q = rint(((double)v - (double)m) / (double)s);
+// (v - m) / s to the nearest integer, given inv = 1 / s (exact: s is a power
+// of two, and so is the product in FP64); |v - m| <= 32766.5 s by
+// construction of the scale, the clamp only guards the int16 range.
+__device__ __forceinline__ int16_t quant16(float v, float m, double inv) {
+    const double q = rint(((double)v - (double)m) * inv);
     return (int16_t)fmin(fmax(q, -32767.0), 32767.0);
 }
 
-// int16 residuals in the tile layout, one work-item per (level, tile, node);
-// nodes beyond the grid hold zeros.  Level 0 also writes zeta and h (FP32).
-__global__ void tile_pack_i16_kernel(RecordPair<float> F, const float *__restrict__ hf,
-                                     const TileScaleUV *__restrict__ tuv, const TileScaleWA *__restrict__ twa,
-                                     UV4<int16_t> *__restrict__ uvr, WA8<int16_t> *__restrict__ war,
-                                     ZH *__restrict__ z, int s_rho, int s_w, int eta, int xi, int tilesX,
-                                     size_t tplane) {
-    const int nlev = s_w - 1;
-    const size_t plane = (size_t)eta * xi, total = tplane * nlev * kTileNodes;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const size_t lt = q / kTileNodes;
-        const int nd = (int)(q - lt * kTileNodes);
-        const size_t lv = lt / tplane, t = lt - lv * tplane;
-        const int k = (int)lv - (nlev - 1);
-        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
-        const int dj = nd / (kTile + 1), di = nd - dj * (kTile + 1);
-        const int j = tj * kTile + dj, i = ti * kTile + di;
-        UV4<int16_t> a{};
-        WA8<int16_t> b{};
-        if (j < eta && i < xi) {
-            const size_t c = (size_t)j * xi + i;
-            const size_t r = (size_t)(k + s_rho - 1) * plane + c;
-            const size_t wt = (size_t)(k + s_w - 1) * plane + c, wb = wt - plane;
-            const TileScaleUV mu = tuv[lt];
-            const TileScaleWA mw = twa[lt];
-            a.v[0] = quant16(F.u0[r], mu.m[0], mu.s[0]); a.v[1] = quant16(F.u1[r], mu.m[1], mu.s[0]);
-            a.v[2] = quant16(F.v0[r], mu.m[2], mu.s[1]); a.v[3] = quant16(F.v1[r], mu.m[3], mu.s[1]);
-            b.v[0] = quant16(F.w0[wt], mw.m[0], mw.s[0]); b.v[1] = quant16(F.w1[wt], mw.m[1], mw.s[0]);
-            b.v[2] = quant16(F.w0[wb], mw.m[2], mw.s[1]); b.v[3] = quant16(F.w1[wb], mw.m[3], mw.s[1]);
-            b.v[4] = quant16(F.akt0[wt], 0.f, mw.s[2]); b.v[5] = quant16(F.akt1[wt], 0.f, mw.s[2]);
-            b.v[6] = quant16(F.akt0[wb], 0.f, mw.s[3]); b.v[7] = quant16(F.akt1[wb], 0.f, mw.s[3]);
-            if (lv == 0 && dj < kTile && di < kTile) z[c] = ZH{F.zeta0[c], F.zeta1[c], hf[c], 0.0f};
-        }
-        uvr[q] = a;
-        war[q] = b;
+// Scale and mean of one group over a tile's nodes (rows [0, nj), columns
+// [c0, c0 + ni) of two staged records a, b of row pitch kI16Pitch): the mean
+// (FP64, rows in order) of each record's non-zero values, a power-of-two
+// scale covering the largest deviation of either record from its mean
+// (zeros included) in 32766 steps and keeping |mean / scale| below 2^23, and
+// the means rounded to multiples of it (exact in FP32).  amax: AKt, no mean:
+// a scale covering the largest magnitude in 32767 steps.
+constexpr int kI16Tiles = 64;                            // tiles per workgroup (a strip of one tile row)
+constexpr int kI16Pitch = kI16Tiles * kTile + 4;         // staged nodes per row (257 used)
+
+__device__ void tile_group_scale(const float *a, const float *b, int nj, int c0, int ni, bool amax, float &ma,
+                                 float &mb, float &sc) {
+    if (amax) {
+        double m = 0.0;
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i)
+                m = fmax(m, fmax(fabs((double)a[j * kI16Pitch + c0 + i]), fabs((double)b[j * kI16Pitch + c0 + i])));
+        ma = mb = 0.f;
+        sc = (float)pow2_at_least(m, 32767.0);
+        return;
     }
+    double sa = 0.0, sb = 0.0;
+    int na = 0, nb = 0;
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const float va = a[j * kI16Pitch + c0 + i], vb = b[j * kI16Pitch + c0 + i];
+            if (va != 0.0f) { sa += va; ++na; }
+            if (vb != 0.0f) { sb += vb; ++nb; }
+        }
+    const double mda = na ? sa / na : 0.0, mdb = nb ? sb / nb : 0.0;
+    double dev = 0.0;
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i)
+            dev = fmax(dev, fmax(fabs((double)a[j * kI16Pitch + c0 + i] - mda),
+                                 fabs((double)b[j * kI16Pitch + c0 + i] - mdb)));
+    const double s = fmax(pow2_at_least(dev, 32766.0), pow2_at_least(fmax(fabs(mda), fabs(mdb)), 8388608.0));
+    ma = (float)(rint(mda / s) * s);
+    mb = (float)(rint(mdb / s) * s);
+    sc = (float)s;
+}
+
+// The whole block-scaled pack of one interval in one pass.  A workgroup
+// takes a strip of up to kI16Tiles tiles of one tile row at one level:
+// it stages the strip's (kTile+1) node rows of u, v (both records) in LDS
+// with coalesced loads, computes each tile's u and v scales and means
+// (tile_group_scale), writes them and the int16 residuals of every node of
+// every tile; then the same for w at W levels k and k-1 and AKt.  Nodes
+// beyond the grid hold zeros (no valid stencil reads them).  Level 0 also
+// writes zeta and h (FP32) for the strip's cells.
+__global__ __launch_bounds__(256) void tile_pack_i16_kernel(RecordPair<float> F, const float *__restrict__ hf,
+                                                            TileScaleUV *__restrict__ tuv,
+                                                            TileScaleWA *__restrict__ twa,
+                                                            UV4<int16_t> *__restrict__ uvr,
+                                                            WA8<int16_t> *__restrict__ war, ZH *__restrict__ z,
+                                                            int s_rho, int s_w, int eta, int xi, int tilesX,
+                                                            size_t tplane) {
+    extern __shared__ float st[];                         // [8][kTile + 1][kI16Pitch]
+    __shared__ float scl[kI16Tiles][8];                   // per tile: 4 means, 2 or 4 scales
+    __shared__ float sinv[kI16Tiles][4];                  // and the scales' reciprocals (exact)
+    const int nlev = s_w - 1;
+    const int strips = (tilesX + kI16Tiles - 1) / kI16Tiles;
+    const int tilesY = (int)(tplane / (size_t)tilesX);
+    const int b = blockIdx.x, lv = b / (tilesY * strips), rem = b - lv * tilesY * strips;
+    const int tj = rem / strips, ti0 = (rem - tj * strips) * kI16Tiles;
+    const int nt = min(kI16Tiles, tilesX - ti0);
+    const int k = lv - (nlev - 1);
+    const size_t plane = (size_t)eta * xi;
+    const size_t r0 = (size_t)(k + s_rho - 1) * plane, wt = (size_t)(k + s_w - 1) * plane, wb = wt - plane;
+    const int j0 = tj * kTile, i0 = ti0 * kTile;
+    const int nj = min(kTile + 1, eta - j0), ni = min(nt * kTile + 1, xi - i0);    // staged node rows, columns
+    const size_t lt0 = (size_t)lv * tplane + (size_t)tj * tilesX + ti0;          // first tile of the strip
+    const int tid = threadIdx.x;
+    constexpr int kRows = kTile + 1, kArr = kRows * kI16Pitch;
+    for (int phase = 0; phase < 2; ++phase) {
+        const int na = phase == 0 ? 4 : 8;
+        const float *src[8];
+        if (phase == 0) {
+            src[0] = F.u0 + r0; src[1] = F.u1 + r0; src[2] = F.v0 + r0; src[3] = F.v1 + r0;
+        } else {
+            src[0] = F.w0 + wt; src[1] = F.w1 + wt; src[2] = F.w0 + wb; src[3] = F.w1 + wb;
+            src[4] = F.akt0 + wt; src[5] = F.akt1 + wt; src[6] = F.akt0 + wb; src[7] = F.akt1 + wb;
+        }
+        __syncthreads();                                  // the previous phase's reads are done
+        // column tid of every staged row: all loads issued before any store
+        // (one memory latency per phase, not one per row)
+        float vals[8][kRows];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int j = 0; j < kRows; ++j)
+                vals[a][j] = (a < na && j < nj && tid < ni) ? src[a][(size_t)(j0 + j) * xi + i0 + tid] : 0.f;
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int j = 0; j < kRows; ++j)
+                if (a < na && tid < ni) st[a * kArr + j * kI16Pitch + tid] = vals[a][j];
+        for (int e = tid; ni > 256 && e < na * kRows; e += 256) {      // the strip's last node column
+            const int a = e / kRows, j = e - a * kRows;
+            st[a * kArr + j * kI16Pitch + 256] = j < nj ? src[a][(size_t)(j0 + j) * xi + i0 + 256] : 0.f;
+        }
+        __syncthreads();
+        // scales: one work-item per (tile, group)
+        const int groups = phase == 0 ? 2 : 4;
+        if (tid < nt * groups) {
+            const int t = tid / groups, g = tid - t * groups;
+            const int c0 = t * kTile, nic = min(kTile + 1, ni - c0);
+            float ma, mb, sc;
+            tile_group_scale(st + 2 * g * kArr, st + (2 * g + 1) * kArr, nj, c0, nic, phase == 1 && g >= 2, ma,
+                             mb, sc);
+            if (phase == 0 || g < 2) {
+                scl[t][2 * g] = ma;
+                scl[t][2 * g + 1] = mb;
+            }
+            scl[t][4 + g] = sc;
+            sinv[t][g] = 1.0f / sc;
+        }
+        __syncthreads();
+        if (tid < nt) {
+            if (phase == 0) {
+                TileScaleUV o;
+                for (int f = 0; f < 4; ++f) o.m[f] = scl[tid][f];
+                o.s[0] = scl[tid][4];
+                o.s[1] = scl[tid][5];
+                o.pad[0] = o.pad[1] = 0.f;
+                tuv[lt0 + tid] = o;
+            } else {
+                TileScaleWA o;
+                for (int f = 0; f < 4; ++f) o.m[f] = scl[tid][f];
+                for (int f = 0; f < 4; ++f) o.s[f] = scl[tid][4 + f];
+                twa[lt0 + tid] = o;
+            }
+        }
+        // residuals: one work-item per (tile, node), nodes of a tile consecutive
+        for (int e = tid; e < nt * kTileNodes; e += 256) {
+            const int t = e / kTileNodes, nd = e - t * kTileNodes;
+            const int dj = nd / (kTile + 1), di = nd - dj * (kTile + 1);
+            const int c = t * kTile + di;                          // staged column
+            const bool in = dj < nj && c < ni;
+            const int o = dj * kI16Pitch + c;
+            const double i0s = sinv[t][0], i1s = sinv[t][1];
+            if (phase == 0) {
+                UV4<int16_t> q{};
+                if (in) {
+                    q.v[0] = quant16(st[0 * kArr + o], scl[t][0], i0s);
+                    q.v[1] = quant16(st[1 * kArr + o], scl[t][1], i0s);
+                    q.v[2] = quant16(st[2 * kArr + o], scl[t][2], i1s);
+                    q.v[3] = quant16(st[3 * kArr + o], scl[t][3], i1s);
+                }
+                uvr[(lt0 + t) * kTileNodes + nd] = q;
+            } else {
+                WA8<int16_t> q{};
+                if (in) {
+                    const double i2s = sinv[t][2], i3s = sinv[t][3];
+                    q.v[0] = quant16(st[0 * kArr + o], scl[t][0], i0s);
+                    q.v[1] = quant16(st[1 * kArr + o], scl[t][1], i0s);
+                    q.v[2] = quant16(st[2 * kArr + o], scl[t][2], i1s);
+                    q.v[3] = quant16(st[3 * kArr + o], scl[t][3], i1s);
+                    q.v[4] = quant16(st[4 * kArr + o], 0.f, i2s);
+                    q.v[5] = quant16(st[5 * kArr + o], 0.f, i2s);
+                    q.v[6] = quant16(st[6 * kArr + o], 0.f, i3s);
+                    q.v[7] = quant16(st[7 * kArr + o], 0.f, i3s);
+                }
+                war[(lt0 + t) * kTileNodes + nd] = q;
+            }
+        }
+        __syncthreads();                                  // scl is rewritten by the next phase
+    }
+    if (lv == 0)                                          // zeta and h of the strip's cells
+        for (int e = tid; e < kTile * nt * kTile; e += 256) {
+            const int dj = e / (nt * kTile), di = e - dj * nt * kTile;
+            const int j = j0 + dj, i = i0 + di;
+            if (j < eta && i < xi) {
+                const size_t c = (size_t)j * xi + i;
+                z[c] = ZH{F.zeta0[c], F.zeta1[c], hf[c], 0.0f};
+            }
+        }
 }
 
 // ---- batched ensemble ------------------------------------------------------
@@ -1463,12 +1532,12 @@ hipError_t launch_tiled_i16(const MoveParams &P, const RecordPair<float> &F, voi
     S.tilesX = L.tilesX;
     S.tplane = L.tplane;
     if (stages & kStagePack) {
-        const size_t blocks = L.tplane * (size_t)S.nlev;
-        tile_scale_kernel<<<grid_blocks(blocks * 6), 256, 0, stream>>>(F, (TileScaleUV *)S.tuv, (TileScaleWA *)S.twa,
-                                                                   P.s_rho, P.s_w, P.eta, P.xi, L.tilesX, L.tplane);
-        tile_pack_i16_kernel<<<grid_blocks(blocks * kTileNodes), 256, 0, stream>>>(
-            F, G.hf, S.tuv, S.twa, (UV4<int16_t> *)S.uvr, (WA8<int16_t> *)S.war, (ZH *)S.z, P.s_rho, P.s_w, P.eta,
-            P.xi, L.tilesX, L.tplane);
+        const int strips = (L.tilesX + kI16Tiles - 1) / kI16Tiles;
+        const size_t blocks = (size_t)S.nlev * (L.tplane / (size_t)L.tilesX) * strips;
+        const size_t lds = 8 * (kTile + 1) * kI16Pitch * sizeof(float);
+        tile_pack_i16_kernel<<<(unsigned)blocks, 256, lds, stream>>>(
+            F, G.hf, (TileScaleUV *)S.tuv, (TileScaleWA *)S.twa, (UV4<int16_t> *)S.uvr, (WA8<int16_t> *)S.war,
+            (ZH *)S.z, P.s_rho, P.s_w, P.eta, P.xi, L.tilesX, L.tplane);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
