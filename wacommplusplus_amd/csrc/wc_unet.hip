@@ -207,12 +207,18 @@ __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, in
     for (int i = 0; i < NI; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int nx = npos * 4, nt = nx + (WL ? kCvWRows * 4 : 0);     // 16-byte groups: input, then weights
+    // the staging position of group tid (row rr0, column xx0 of the stage);
+    // later groups advance by 64 positions, followed by counting
+    const int rr0 = (tid >> 2) / Wp, xx0 = (tid >> 2) - rr0 * Wp, g0 = tid & 3;
     for (int c0 = 0; c0 < cin; c0 += 32) {
         const bool first = c0 < a.c1;
-        const uint16_t *xs = (const uint16_t *)(first ? a.x1 : a.x2);
         const int cc = first ? a.c1 : a.c2, cl0 = first ? c0 : c0 - a.c1;
+        // this image's rows (offsets below stay within one image: h w c < 2^31)
+        const uint16_t *xs = (const uint16_t *)(first ? a.x1 : a.x2) + (size_t)img * hw * cc + cl0 + 8 * g0;
+        const uint16_t *ws = a.w_ + (size_t)co0 * 9 * cin + c0 + 8 * g0;
         __syncthreads();                                  // the previous chunk's reads are done
         constexpr int SB = WL ? kStageBatch : kStageBatch / 2;   // the register-weight kernel keeps fewer
+        int rr = rr0, xx = xx0;
         for (int e0 = tid; e0 < nt; e0 += 256 * SB) {
             uint4 v[SB];
             int dst[SB];
@@ -222,18 +228,16 @@ __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, in
                 v[u] = make_uint4(0, 0, 0, 0);
                 dst[u] = -1;
                 if (e < nx) {
-                    const int pos = e >> 2, g = e & 3;
-                    const int rr = pos / Wp, xx = pos - rr * Wp;
                     const int y = y0 - 1 + rr, x = xx - 1;
-                    dst[u] = pos * kCvPS + 8 * g;
-                    if (y >= 0 && y < H && x >= 0 && x < W)
-                        v[u] = *(const uint4 *)(xs + ((size_t)img * hw + (size_t)y * W + x) * cc + cl0 + 8 * g);
+                    dst[u] = (e >> 2) * kCvPS + 8 * g0;
+                    if (y >= 0 && y < H && x >= 0 && x < W) v[u] = *(const uint4 *)(xs + (y * W + x) * cc);
                 } else if (e < nt) {
-                    const int f = e - nx, g = f & 3, tc = f >> 2;          // tc = tap * 32 + local channel
-                    const int co = co0 + (tc & 31), tap = tc >> 5;
-                    dst[u] = (npos + tc) * kCvPS + 8 * g;
-                    if (co < nout) v[u] = *(const uint4 *)(a.w_ + ((size_t)co * 9 + tap) * cin + c0 + 8 * g);
+                    const int tc = (e - nx) >> 2;                           // tap * 32 + local channel
+                    const int col = tc & 31, tap = tc >> 5;
+                    dst[u] = (npos + tc) * kCvPS + 8 * g0;
+                    if (co0 + col < nout) v[u] = *(const uint4 *)(ws + (col * 9 + tap) * cin);
                 }
+                for (xx += 64; xx >= Wp; xx -= Wp) ++rr;
             }
 #pragma unroll
             for (int u = 0; u < SB; ++u)
@@ -267,39 +271,36 @@ __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, in
             }
     }
     if (!active) return;
-    // D[4 kq + r][row] of tile (i, j): stage pixel 16 NI wave + 16 i + 4 kq + r,
-    // channel co0 + 16 j + row; rows of the stage pixels by counting
-    int er[NI][4];
-    for (int i = 0; i < NI; ++i) {
-        const int q0 = 16 * NI * wave + 16 * i + 4 * kq;
-        int rr = q0 / W, xx = q0 - rr * W;
-        for (int r = 0; r < 4; ++r) {
-            er[i][r] = rr;
-            if (++xx == W) { xx = 0; ++rr; }
-        }
-    }
+    // D[4 kq + r][row] of tile (i, j): stage pixel q = 16 NI wave + 16 i +
+    // 4 kq + r, channel co0 + 16 j + row.  The stage holds whole rows, so
+    // stage pixel q is image pixel y0 w + q: stored while q < rv w (rows past
+    // the image's last are not).
+    const int qmax = rv * W;
+    const long long p0 = (long long)img * hw + (long long)y0 * W;
     for (int j = 0; j < 2; ++j) {
         const int co = co0 + j * 16 + row;
         if (co >= nout) continue;
         const bool first = co < a.cy1;
         const int cl = first ? co : co - a.cy1;
         const int cc = first ? a.cy1 : a.cy2;
-        for (int i = 0; i < NI; ++i)
-            for (int r = 0; r < 4; ++r) {
-                const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
-                if (q >= RW || er[i][r] >= rv) continue;
-                // stage pixel q of image row y0 + er is image pixel y0 * W + q
-                const long long p = (long long)img * hw + (long long)y0 * W + q;
-                float v = acc[i][j][r];
-                if (a.mode == 0) {                                  // forward: bias + ReLU, BF16
-                    v += a.bias[co];
-                    ((uint16_t *)a.y1)[p * a.cy1 + co] = to_bf16(v > 0.f ? v : 0.f);
-                } else {
-                    float *y = (float *)(first ? a.y1 : a.y2) + p * cc + cl;
-                    const bool accum = first ? a.acc1 : a.acc2;
-                    *y = accum ? *y + v : v;
+        if (a.mode == 0) {                                  // forward: bias + ReLU, BF16
+            uint16_t *y = (uint16_t *)a.y1 + p0 * a.cy1 + co;
+            const float b = a.bias[co];
+            for (int i = 0; i < NI; ++i)
+                for (int r = 0; r < 4; ++r) {
+                    const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
+                    const float v = acc[i][j][r] + b;
+                    if (q < qmax) y[q * a.cy1] = to_bf16(v > 0.f ? v : 0.f);
                 }
-            }
+        } else {
+            float *y = (float *)(first ? a.y1 : a.y2) + p0 * cc + cl;
+            const bool accum = first ? a.acc1 : a.acc2;
+            for (int i = 0; i < NI; ++i)
+                for (int r = 0; r < 4; ++r) {
+                    const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
+                    if (q < qmax) y[q * cc] = accum ? y[q * cc] + acc[i][j][r] : acc[i][j][r];
+                }
+        }
     }
 }
 
