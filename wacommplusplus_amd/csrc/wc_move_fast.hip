@@ -28,6 +28,8 @@
 // reference's SplitMix64 draws, not draw-for-draw identical.
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #include "wc_launch.h"
 #include "wc_philox.h"
 
@@ -222,7 +224,7 @@ struct TilePacked {
 
 // Tile-mean storage (WC_STORE_BF16_TILE, WC_STORE_F16_TILE): u, v and w are
 // stored as 16-bit residuals from a mean per tile of kTile x kTile cells (per
-// level, field and record; see tile_mean_kernel).  The cells [kTile*tj, kTile*tj + kTile) of a
+// level, field and record; see tile_pack_strip_kernel).  The cells [kTile*tj, kTile*tj + kTile) of a
 // tile read nodes up to kTile*tj + kTile, so each tile keeps its own copy of
 // (kTile+1)^2 nodes and a stencil reads one tile and one mean:
 //   UVR[lev][tile][kTile+1][kTile+1] = residuals {u(t0), u(t1), v(t0), v(t1)}
@@ -735,83 +737,6 @@ __global__ void pack_kernel(RecordPair<float> F, const float *__restrict__ hf, U
     pack_body<T>(F, hf, uv, wa, z, s_rho, s_w, plane);
 }
 
-// Tile means of the interval's record pair, one work-item per (level, tile):
-// for u, v at rho level k and w at W levels k and k-1, both records, the mean
-// (summed in FP64) of the non-zero values among the (kTile+1)^2 nodes the
-// tile stores, rounded to T (0 when all are zero).  With a mean that T
-// represents, a zero node's residual -mean is exact, so zeros decode to 0.
-template <class T>
-__global__ void tile_mean_kernel(RecordPair<float> F, WA8<T> *__restrict__ tm, int s_rho, int s_w, int eta, int xi,
-                                 int tilesX, size_t tplane) {
-    const int nlev = s_w - 1;
-    const size_t plane = (size_t)eta * xi, total = tplane * nlev;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const size_t lv = q / tplane, t = q - lv * tplane;
-        const int k = (int)lv - (nlev - 1);
-        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
-        const size_t r0 = (size_t)(k + s_rho - 1) * plane, wt = (size_t)(k + s_w - 1) * plane, wb = wt - plane;
-        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int dj = 0; dj <= kTile; ++dj) {
-            const int j = tj * kTile + dj;
-            if (j >= eta) break;
-            for (int di = 0; di <= kTile; ++di) {
-                const int i = ti * kTile + di;
-                if (i >= xi) break;
-                const size_t c = (size_t)j * xi + i;
-                const float v[8] = {F.u0[r0 + c], F.u1[r0 + c], F.v0[r0 + c], F.v1[r0 + c],
-                                    F.w0[wt + c], F.w1[wt + c], F.w0[wb + c], F.w1[wb + c]};
-                for (int f = 0; f < 8; ++f)
-                    if (v[f] != 0.0f) {
-                        a[f] += v[f];
-                        ++cnt[f];
-                    }
-            }
-        }
-        WA8<T> m;
-        for (int f = 0; f < 8; ++f) m.v[f] = fromf<T>(cnt[f] ? (float)(a[f] / cnt[f]) : 0.0f);
-        tm[q] = m;
-    }
-}
-
-// Residuals from the tile means, one work-item per (level, tile, node of the
-// tile's (kTile+1)^2 block); nodes beyond the grid hold zeros (no valid
-// stencil reads them).  Level 0 also writes zeta (rounded to T) and h.
-template <class T>
-__global__ void tile_pack_kernel(RecordPair<float> F, const float *__restrict__ hf, const WA8<T> *__restrict__ tm,
-                                 UV4<T> *__restrict__ uvr, WA8<T> *__restrict__ war,
-                                 ZH *__restrict__ z, int s_rho, int s_w, int eta, int xi, int tilesX, size_t tplane) {
-    const int nlev = s_w - 1;
-    const size_t plane = (size_t)eta * xi, total = tplane * nlev * kTileNodes;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const size_t lt = q / kTileNodes;
-        const int nd = (int)(q - lt * kTileNodes);
-        const size_t lv = lt / tplane, t = lt - lv * tplane;
-        const int k = (int)lv - (nlev - 1);
-        const int tj = (int)(t / (size_t)tilesX), ti = (int)(t - (size_t)tj * tilesX);
-        const int dj = nd / (kTile + 1), di = nd - dj * (kTile + 1);
-        const int j = tj * kTile + dj, i = ti * kTile + di;
-        UV4<T> a{};
-        WA8<T> b{};
-        if (j < eta && i < xi) {
-            const size_t c = (size_t)j * xi + i;
-            const size_t r = (size_t)(k + s_rho - 1) * plane + c;
-            const size_t wt = (size_t)(k + s_w - 1) * plane + c, wb = wt - plane;
-            const WA8<T> m = tm[lt];
-            a.v[0] = fromf<T>(F.u0[r] - tof(m.v[0])); a.v[1] = fromf<T>(F.u1[r] - tof(m.v[1]));
-            a.v[2] = fromf<T>(F.v0[r] - tof(m.v[2])); a.v[3] = fromf<T>(F.v1[r] - tof(m.v[3]));
-            b.v[0] = fromf<T>(F.w0[wt] - tof(m.v[4])); b.v[1] = fromf<T>(F.w1[wt] - tof(m.v[5]));
-            b.v[2] = fromf<T>(F.w0[wb] - tof(m.v[6])); b.v[3] = fromf<T>(F.w1[wb] - tof(m.v[7]));
-            b.v[4] = fromf<T>(F.akt0[wt]); b.v[5] = fromf<T>(F.akt1[wt]);
-            b.v[6] = fromf<T>(F.akt0[wb]); b.v[7] = fromf<T>(F.akt1[wb]);
-            if (lv == 0 && dj < kTile && di < kTile)
-                z[c] = ZH{tof<T>(fromf<T>(F.zeta0[c])), tof<T>(fromf<T>(F.zeta1[c])), hf[c], 0.0f};
-        }
-        uvr[q] = a;
-        war[q] = b;
-    }
-}
-
 // Smallest power of two >= x / cap (x >= 0), kept within [2^-100, 2^100]
 // (2^-100 for x = 0), so every scaled int16 stays a normal float.
 __device__ __forceinline__ double pow2_at_least(double x, double cap) {
@@ -872,24 +797,47 @@ __device__ void tile_group_scale(const float *a, const float *b, int nj, int c0,
     sc = (float)s;
 }
 
-// The whole block-scaled pack of one interval in one pass.  A workgroup
-// takes a strip of up to kI16Tiles tiles of one tile row at one level:
-// it stages the strip's (kTile+1) node rows of u, v (both records) in LDS
-// with coalesced loads, computes each tile's u and v scales and means
-// (tile_group_scale), writes them and the int16 residuals of every node of
-// every tile; then the same for w at W levels k and k-1 and AKt.  Nodes
-// beyond the grid hold zeros (no valid stencil reads them).  Level 0 also
-// writes zeta and h (FP32) for the strip's cells.
-__global__ __launch_bounds__(256) void tile_pack_i16_kernel(RecordPair<float> F, const float *__restrict__ hf,
-                                                            TileScaleUV *__restrict__ tuv,
-                                                            TileScaleWA *__restrict__ twa,
-                                                            UV4<int16_t> *__restrict__ uvr,
-                                                            WA8<int16_t> *__restrict__ war, ZH *__restrict__ z,
-                                                            int s_rho, int s_w, int eta, int xi, int tilesX,
-                                                            size_t tplane) {
+// Mean of each of two staged records' non-zero values over a tile's nodes
+// (FP64, rows in order; 0 when all are zero), as tile_group_scale takes it.
+__device__ void tile_group_mean(const float *a, const float *b, int nj, int c0, int ni, double &ma, double &mb) {
+    double sa = 0.0, sb = 0.0;
+    int na = 0, nb = 0;
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const float va = a[j * kI16Pitch + c0 + i], vb = b[j * kI16Pitch + c0 + i];
+            if (va != 0.0f) { sa += va; ++na; }
+            if (vb != 0.0f) { sb += vb; ++nb; }
+        }
+    ma = na ? sa / na : 0.0;
+    mb = nb ? sb / nb : 0.0;
+}
+
+// The whole tiled pack of one interval in one pass, for the block-scaled
+// store (T = int16_t) and the tile-mean stores (T = BF16 bits or _Float16).
+// A workgroup takes a strip of up to kI16Tiles tiles of one tile row at one
+// level: it stages the strip's (kTile+1) node rows of u, v (both records) in
+// LDS with coalesced loads (all of a work-item's loads before its stores),
+// computes each tile's means (and, block-scaled, scales: tile_group_scale),
+// and writes them and the residuals of every node of every tile; then the
+// same for w at W levels k and k-1 and AKt.  Tile means: the mean of the
+// non-zero values rounded to T (with a mean that T represents, a zero
+// node's residual -mean is exact, so zeros decode to 0), each residual
+// v - mean rounded to T, AKt rounded to T.  Nodes beyond the grid hold zeros
+// (no valid stencil reads them).
+// Level 0 also writes zeta (FP32 block-scaled, rounded to T otherwise) and h
+// for the strip's cells.
+template <class T>
+__global__ __launch_bounds__(256) void tile_pack_strip_kernel(RecordPair<float> F, const float *__restrict__ hf,
+                                                              TileScaleUV *__restrict__ tuv,
+                                                              TileScaleWA *__restrict__ twa, WA8<T> *__restrict__ tm,
+                                                              UV4<T> *__restrict__ uvr, WA8<T> *__restrict__ war,
+                                                              ZH *__restrict__ z, int s_rho, int s_w, int eta, int xi,
+                                                              int tilesX, size_t tplane) {
+    constexpr bool kScaled = std::is_same<T, int16_t>::value;
     extern __shared__ float st[];                         // [8][kTile + 1][kI16Pitch]
     __shared__ float scl[kI16Tiles][8];                   // per tile: 4 means, 2 or 4 scales
     __shared__ float sinv[kI16Tiles][4];                  // and the scales' reciprocals (exact)
+    __shared__ float smean[kI16Tiles][4];                 // tile means: the u, v means (phase 0)
     const int nlev = s_w - 1;
     const int strips = (tilesX + kI16Tiles - 1) / kI16Tiles;
     const int tilesY = (int)(tplane / (size_t)tilesX);
@@ -909,6 +857,7 @@ __global__ __launch_bounds__(256) void tile_pack_i16_kernel(RecordPair<float> F,
         const float *src[8];
         if (phase == 0) {
             src[0] = F.u0 + r0; src[1] = F.u1 + r0; src[2] = F.v0 + r0; src[3] = F.v1 + r0;
+            src[4] = src[5] = src[6] = src[7] = nullptr;
         } else {
             src[0] = F.w0 + wt; src[1] = F.w1 + wt; src[2] = F.w0 + wb; src[3] = F.w1 + wb;
             src[4] = F.akt0 + wt; src[5] = F.akt1 + wt; src[6] = F.akt0 + wb; src[7] = F.akt1 + wb;
@@ -932,35 +881,53 @@ __global__ __launch_bounds__(256) void tile_pack_i16_kernel(RecordPair<float> F,
             st[a * kArr + j * kI16Pitch + 256] = j < nj ? src[a][(size_t)(j0 + j) * xi + i0 + 256] : 0.f;
         }
         __syncthreads();
-        // scales: one work-item per (tile, group)
-        const int groups = phase == 0 ? 2 : 4;
+        // means (and scales): one work-item per (tile, group)
+        const int groups = phase == 0 ? 2 : (kScaled ? 4 : 2);
         if (tid < nt * groups) {
             const int t = tid / groups, g = tid - t * groups;
             const int c0 = t * kTile, nic = min(kTile + 1, ni - c0);
-            float ma, mb, sc;
-            tile_group_scale(st + 2 * g * kArr, st + (2 * g + 1) * kArr, nj, c0, nic, phase == 1 && g >= 2, ma,
-                             mb, sc);
-            if (phase == 0 || g < 2) {
-                scl[t][2 * g] = ma;
-                scl[t][2 * g + 1] = mb;
+            if constexpr (kScaled) {
+                float ma, mb, sc;
+                tile_group_scale(st + 2 * g * kArr, st + (2 * g + 1) * kArr, nj, c0, nic, phase == 1 && g >= 2, ma,
+                                 mb, sc);
+                if (phase == 0 || g < 2) {
+                    scl[t][2 * g] = ma;
+                    scl[t][2 * g + 1] = mb;
+                }
+                scl[t][4 + g] = sc;
+                sinv[t][g] = 1.0f / sc;
+            } else {
+                double ma, mb;
+                tile_group_mean(st + 2 * g * kArr, st + (2 * g + 1) * kArr, nj, c0, nic, ma, mb);
+                scl[t][2 * g] = tof<T>(fromf<T>((float)ma));
+                scl[t][2 * g + 1] = tof<T>(fromf<T>((float)mb));
+                if (phase == 0) {
+                    smean[t][2 * g] = scl[t][2 * g];
+                    smean[t][2 * g + 1] = scl[t][2 * g + 1];
+                }
             }
-            scl[t][4 + g] = sc;
-            sinv[t][g] = 1.0f / sc;
         }
         __syncthreads();
         if (tid < nt) {
-            if (phase == 0) {
-                TileScaleUV o;
-                for (int f = 0; f < 4; ++f) o.m[f] = scl[tid][f];
-                o.s[0] = scl[tid][4];
-                o.s[1] = scl[tid][5];
-                o.pad[0] = o.pad[1] = 0.f;
-                tuv[lt0 + tid] = o;
-            } else {
-                TileScaleWA o;
-                for (int f = 0; f < 4; ++f) o.m[f] = scl[tid][f];
-                for (int f = 0; f < 4; ++f) o.s[f] = scl[tid][4 + f];
-                twa[lt0 + tid] = o;
+            if constexpr (kScaled) {
+                if (phase == 0) {
+                    TileScaleUV o;
+                    for (int f = 0; f < 4; ++f) o.m[f] = scl[tid][f];
+                    o.s[0] = scl[tid][4];
+                    o.s[1] = scl[tid][5];
+                    o.pad[0] = o.pad[1] = 0.f;
+                    tuv[lt0 + tid] = o;
+                } else {
+                    TileScaleWA o;
+                    for (int f = 0; f < 4; ++f) o.m[f] = scl[tid][f];
+                    for (int f = 0; f < 4; ++f) o.s[f] = scl[tid][4 + f];
+                    twa[lt0 + tid] = o;
+                }
+            } else if (phase == 1) {
+                WA8<T> o;
+                for (int f = 0; f < 4; ++f) o.v[f] = fromf<T>(smean[tid][f]);
+                for (int f = 0; f < 4; ++f) o.v[4 + f] = fromf<T>(scl[tid][f]);
+                tm[lt0 + tid] = o;
             }
         }
         // residuals: one work-item per (tile, node), nodes of a tile consecutive
@@ -970,28 +937,34 @@ __global__ __launch_bounds__(256) void tile_pack_i16_kernel(RecordPair<float> F,
             const int c = t * kTile + di;                          // staged column
             const bool in = dj < nj && c < ni;
             const int o = dj * kI16Pitch + c;
-            const double i0s = sinv[t][0], i1s = sinv[t][1];
+            auto res = [&](int a, int m, int sg) -> T {       // array a, mean m, scale group sg
+                if constexpr (kScaled) return quant16(st[a * kArr + o], scl[t][m], sinv[t][sg]);
+                else return fromf<T>(st[a * kArr + o] - scl[t][m]);
+            };
+            auto raw = [&](int a, int sg) -> T {               // AKt: no mean
+                if constexpr (kScaled) return quant16(st[a * kArr + o], 0.f, sinv[t][sg]);
+                else return fromf<T>(st[a * kArr + o]);
+            };
             if (phase == 0) {
-                UV4<int16_t> q{};
+                UV4<T> q{};
                 if (in) {
-                    q.v[0] = quant16(st[0 * kArr + o], scl[t][0], i0s);
-                    q.v[1] = quant16(st[1 * kArr + o], scl[t][1], i0s);
-                    q.v[2] = quant16(st[2 * kArr + o], scl[t][2], i1s);
-                    q.v[3] = quant16(st[3 * kArr + o], scl[t][3], i1s);
+                    q.v[0] = res(0, 0, 0);
+                    q.v[1] = res(1, 1, 0);
+                    q.v[2] = res(2, 2, 1);
+                    q.v[3] = res(3, 3, 1);
                 }
                 uvr[(lt0 + t) * kTileNodes + nd] = q;
             } else {
-                WA8<int16_t> q{};
+                WA8<T> q{};
                 if (in) {
-                    const double i2s = sinv[t][2], i3s = sinv[t][3];
-                    q.v[0] = quant16(st[0 * kArr + o], scl[t][0], i0s);
-                    q.v[1] = quant16(st[1 * kArr + o], scl[t][1], i0s);
-                    q.v[2] = quant16(st[2 * kArr + o], scl[t][2], i1s);
-                    q.v[3] = quant16(st[3 * kArr + o], scl[t][3], i1s);
-                    q.v[4] = quant16(st[4 * kArr + o], 0.f, i2s);
-                    q.v[5] = quant16(st[5 * kArr + o], 0.f, i2s);
-                    q.v[6] = quant16(st[6 * kArr + o], 0.f, i3s);
-                    q.v[7] = quant16(st[7 * kArr + o], 0.f, i3s);
+                    q.v[0] = res(0, 0, 0);
+                    q.v[1] = res(1, 1, 0);
+                    q.v[2] = res(2, 2, 1);
+                    q.v[3] = res(3, 3, 1);
+                    q.v[4] = raw(4, 2);
+                    q.v[5] = raw(5, 2);
+                    q.v[6] = raw(6, 3);
+                    q.v[7] = raw(7, 3);
                 }
                 war[(lt0 + t) * kTileNodes + nd] = q;
             }
@@ -1004,7 +977,8 @@ __global__ __launch_bounds__(256) void tile_pack_i16_kernel(RecordPair<float> F,
             const int j = j0 + dj, i = i0 + di;
             if (j < eta && i < xi) {
                 const size_t c = (size_t)j * xi + i;
-                z[c] = ZH{F.zeta0[c], F.zeta1[c], hf[c], 0.0f};
+                if constexpr (kScaled) z[c] = ZH{F.zeta0[c], F.zeta1[c], hf[c], 0.0f};
+                else z[c] = ZH{tof<T>(fromf<T>(F.zeta0[c])), tof<T>(fromf<T>(F.zeta1[c])), hf[c], 0.0f};
             }
         }
 }
@@ -1535,9 +1509,9 @@ hipError_t launch_tiled_i16(const MoveParams &P, const RecordPair<float> &F, voi
         const int strips = (L.tilesX + kI16Tiles - 1) / kI16Tiles;
         const size_t blocks = (size_t)S.nlev * (L.tplane / (size_t)L.tilesX) * strips;
         const size_t lds = 8 * (kTile + 1) * kI16Pitch * sizeof(float);
-        tile_pack_i16_kernel<<<(unsigned)blocks, 256, lds, stream>>>(
-            F, G.hf, (TileScaleUV *)S.tuv, (TileScaleWA *)S.twa, (UV4<int16_t> *)S.uvr, (WA8<int16_t> *)S.war,
-            (ZH *)S.z, P.s_rho, P.s_w, P.eta, P.xi, L.tilesX, L.tplane);
+        tile_pack_strip_kernel<int16_t><<<(unsigned)blocks, 256, lds, stream>>>(
+            F, G.hf, (TileScaleUV *)S.tuv, (TileScaleWA *)S.twa, nullptr, (UV4<int16_t> *)S.uvr,
+            (WA8<int16_t> *)S.war, (ZH *)S.z, P.s_rho, P.s_w, P.eta, P.xi, L.tilesX, L.tplane);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1563,12 +1537,12 @@ hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void 
     S.tilesX = L.tilesX;
     S.tplane = L.tplane;
     if (stages & kStagePack) {
-        const size_t blocks = L.tplane * (size_t)S.nlev;
-        tile_mean_kernel<T><<<grid_blocks(blocks), 256, 0, stream>>>(F, (WA8<T> *)S.tm, P.s_rho, P.s_w, P.eta,
-                                                                     P.xi, L.tilesX, L.tplane);
-        tile_pack_kernel<T><<<grid_blocks(blocks * kTileNodes), 256, 0, stream>>>(
-            F, G.hf, S.tm, (UV4<T> *)S.uvr, (WA8<T> *)S.war, (ZH *)S.z, P.s_rho, P.s_w, P.eta, P.xi, L.tilesX,
-            L.tplane);
+        const int strips = (L.tilesX + kI16Tiles - 1) / kI16Tiles;
+        const size_t blocks = (size_t)S.nlev * (L.tplane / (size_t)L.tilesX) * strips;
+        const size_t lds = 8 * (kTile + 1) * kI16Pitch * sizeof(float);
+        tile_pack_strip_kernel<T><<<(unsigned)blocks, 256, lds, stream>>>(
+            F, G.hf, nullptr, nullptr, (WA8<T> *)S.tm, (UV4<T> *)S.uvr, (WA8<T> *)S.war, (ZH *)S.z, P.s_rho, P.s_w,
+            P.eta, P.xi, L.tilesX, L.tplane);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
