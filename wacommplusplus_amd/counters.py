@@ -269,7 +269,8 @@ def bench_trace(a) -> None:
     steps, warmup = line["steps"], line["warmup"]
     v = line["variants"]
     runs = []
-    for key in ["f64"] + [k for k in v if "_" not in k.replace("bf16_tile", "bt").replace("f16_tile", "ft")]:
+    bases = [k for k in v if not k.endswith(("_unsorted", "_euler"))]       # the variants, in run order
+    for key in ["f64"] + bases:
         tile = line["config"]["cell_tile_sort"] if key == "f64" else v[key]["sort_tile"]
         runs.append(dict(key=key + "_unsorted", variant=key, warmup_launches=warmup, timed_launches=steps))
         if line["config"].get("sort_autotuned", True):
