@@ -178,6 +178,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
 // of the deep levels keep every wave busy.  The epilogue is the one of
 // conv3x3_kernel.
 constexpr int kCvPS = 40;
+constexpr int kCvOutF = 36;               // FP32 epilogue row: 32 channels padded to 144 B
 constexpr int kStageBatch = 8;
 constexpr int kCvWRows = 9 * 32;          // staged weight rows per chunk
 
@@ -270,36 +271,58 @@ __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, in
                 }
             }
     }
-    if (!active) return;
-    // D[4 kq + r][row] of tile (i, j): stage pixel q = 16 NI wave + 16 i +
-    // 4 kq + r, channel co0 + 16 j + row.  The stage holds whole rows, so
-    // stage pixel q is image pixel y0 w + q: stored while q < rv w (rows past
-    // the image's last are not).
+    // Epilogue through LDS: D[4 kq + r][row] of tile (i, j) is stage pixel
+    // q = 16 NI wave + 16 i + 4 kq + r, channel co0 + 16 j + row.  The tile
+    // [q][32 channels] is written to LDS (the stage is no longer read), then
+    // stored with 16-byte accesses, 8 BF16 or 4 FP32 channels of one pixel
+    // per access.  The stage holds whole rows, so stage pixel q is image pixel
+    // y0 w + q: stored while q < rv w (rows past the image's last are not).
     const int qmax = rv * W;
     const long long p0 = (long long)img * hw + (long long)y0 * W;
-    for (int j = 0; j < 2; ++j) {
-        const int co = co0 + j * 16 + row;
-        if (co >= nout) continue;
-        const bool first = co < a.cy1;
-        const int cl = first ? co : co - a.cy1;
-        const int cc = first ? a.cy1 : a.cy2;
-        if (a.mode == 0) {                                  // forward: bias + ReLU, BF16
-            uint16_t *y = (uint16_t *)a.y1 + p0 * a.cy1 + co;
-            const float b = a.bias[co];
-            for (int i = 0; i < NI; ++i)
-                for (int r = 0; r < 4; ++r) {
-                    const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
-                    const float v = acc[i][j][r] + b;
-                    if (q < qmax) y[q * a.cy1] = to_bf16(v > 0.f ? v : 0.f);
-                }
-        } else {
-            float *y = (float *)(first ? a.y1 : a.y2) + p0 * cc + cl;
-            const bool accum = first ? a.acc1 : a.acc2;
-            for (int i = 0; i < NI; ++i)
-                for (int r = 0; r < 4; ++r) {
-                    const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
-                    if (q < qmax) y[q * cc] = accum ? y[q * cc] + acc[i][j][r] : acc[i][j][r];
-                }
+    __syncthreads();                                      // every wave is done with the stage
+    if (a.mode == 0) {                                    // forward: bias + ReLU, BF16
+        uint16_t *so = smem;                              // [RW][kCvPS]
+        if (active)
+            for (int j = 0; j < 2; ++j) {
+                const int col = 16 * j + row;
+                const float b = co0 + col < nout ? a.bias[co0 + col] : 0.f;
+                for (int i = 0; i < NI; ++i)
+                    for (int r = 0; r < 4; ++r) {
+                        const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
+                        const float v = acc[i][j][r] + b;
+                        if (q < RW) so[q * kCvPS + col] = to_bf16(v > 0.f ? v : 0.f);
+                    }
+            }
+        __syncthreads();
+        uint16_t *y = (uint16_t *)a.y1 + p0 * a.cy1 + co0;
+        for (int c = tid; c < qmax * 4; c += 256) {
+            const int q = c >> 2, part = c & 3;
+            if (co0 + 8 * part < nout) *(uint4 *)(y + (size_t)q * a.cy1 + 8 * part) = *(const uint4 *)(so + q * kCvPS + 8 * part);
+        }
+    } else {                                              // input gradient: FP32, stored or accumulated
+        float *so = (float *)smem;                        // [RW][kCvOutF]
+        if (active)
+            for (int j = 0; j < 2; ++j)
+                for (int i = 0; i < NI; ++i)
+                    for (int r = 0; r < 4; ++r) {
+                        const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
+                        if (q < RW) so[q * kCvOutF + 16 * j + row] = acc[i][j][r];
+                    }
+        __syncthreads();
+        const bool first = co0 < a.cy1;
+        const int cc = first ? a.cy1 : a.cy2, cl0 = first ? co0 : co0 - a.cy1;
+        const bool accum = first ? a.acc1 : a.acc2;
+        float *y = (float *)(first ? a.y1 : a.y2) + p0 * cc + cl0;
+        for (int c = tid; c < qmax * 8; c += 256) {
+            const int q = c >> 3, part = c & 7;
+            if (co0 + 4 * part >= nout) continue;
+            float4 v = *(const float4 *)(so + q * kCvOutF + 4 * part);
+            float4 *dst = (float4 *)(y + (size_t)q * cc + 4 * part);
+            if (accum) {
+                const float4 o = *dst;
+                v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+            }
+            *dst = v;
         }
     }
 }
@@ -859,9 +882,12 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
         // grid has fewer than 512 workgroups, as long as the stage fits in LDS
         int R = 256 / a.w < a.h ? 256 / a.w : a.h;
         if (R < 1) R = 1;
-        // stage bytes with the weights in LDS (wl) or in registers
+        // stage bytes with the weights in LDS (wl) or in registers, at least
+        // the epilogue's tile of r w pixels x 32 channels
         auto lds = [&](int r, bool wl) {
-            return ((size_t)(r + 2) * (a.w + 2) + (wl ? kCvWRows : 0)) * kCvPS * sizeof(uint16_t);
+            const size_t stage = ((size_t)(r + 2) * (a.w + 2) + (wl ? kCvWRows : 0)) * kCvPS * sizeof(uint16_t);
+            const size_t out = (size_t)r * a.w * (a.mode == 0 ? kCvPS * sizeof(uint16_t) : kCvOutF * sizeof(float));
+            return stage > out ? stage : out;
         };
         while (R > 1 && lds(R, false) > 64 * 1024) --R;
         while (R > 1 && (long long)a.n * ((a.h + R - 1) / R) * (nout / 32) < 512) R = (R + 1) / 2;
