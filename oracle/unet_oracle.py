@@ -194,7 +194,7 @@ class UNetOracle:
         gq = self._r(gm)
         x = x1 if x2 is None else np.concatenate([x1, x2], axis=3)
         dW = self._wgrad(gq, x)
-        db = gm.sum(axis=(0, 1, 2))
+        db = gq.sum(axis=(0, 1, 2))      # the device sums the rounded gradient it feeds the products
         if not need_dx:
             return dW, db, None, None
         gx = self._dgrad(gq, W)

@@ -151,3 +151,38 @@ def test_training_reduces_loss(net):
 def test_rejects_bad_shapes(net):
     with pytest.raises(wc.WcError):
         net.forward(np.zeros((1, 12, 16), np.float32))
+
+
+def test_graph_replay_equals_direct_launches():
+    """Training steps replayed from the captured graph equal the same steps
+    launched one by one, bit for bit, on every replay: eight steps that
+    switch batch shape and loss kind (each switch captures anew), with the
+    parameters and the loss read back between replays.  Every sum over
+    pixels runs in a fixed order where each level's width is a multiple of
+    8 (the tiled weight gradient), so the two runs can be compared exactly."""
+    rng = np.random.default_rng(21)
+    p0 = uo.he_params(seed=5).astype(np.float32)
+    batches = []
+    for shape, weighted in (((4, 16, 64), False), ((4, 16, 64), False), ((2, 32, 128), False),
+                            ((4, 16, 64), True), ((4, 16, 64), True), ((4, 16, 64), False),
+                            ((2, 32, 128), True), ((4, 16, 64), False)):
+        x = rng.uniform(0.0, 3.0, shape).astype(np.float32)
+        t = (x + rng.normal(0.0, 0.3, shape)).astype(np.float32)
+        w = rng.uniform(0.5, 2.0, shape).astype(np.float32) if weighted else None
+        batches.append((x, t, w))
+    runs = {}
+    for graph in (True, False):
+        n = wc.UNet(0, seed=3)
+        n.set_graph(graph)
+        n.set_params(p0)
+        out = []
+        for x, t, w in batches:
+            loss = n.train_step(x, t, 1e-3, weight=w)
+            out.append((loss, n.get_params()))
+        lg, g = n.gradients(*batches[0][:2])
+        out.append((lg, g))
+        n.close()
+        runs[graph] = out
+    for q, ((la, pa), (lb, pb)) in enumerate(zip(runs[True], runs[False])):
+        assert la == lb, (q, la, lb)
+        assert np.array_equal(pa, pb), q

@@ -30,7 +30,7 @@ EXPORTS = (
     "wc_ens_set", "wc_ens_set_launch", "wc_ens_get_launch", "wc_ens_autotune", "wc_ens_run", "wc_ens_surface",
     "wc_ens_get", "wc_ens_release", "wc_ens_set_sort", "wc_ens_get_sort", "wc_ens_set_lds", "wc_ens_get_lds", "wc_unet_create", "wc_unet_destroy", "wc_unet_last_error", "wc_unet_param_count",
     "wc_unet_get_params", "wc_unet_set_params", "wc_unet_forward", "wc_unet_gradients", "wc_unet_train_step",
-    "wc_unet_create_ex", "wc_unet_gradients_weighted", "wc_unet_train_step_weighted",
+    "wc_unet_create_ex", "wc_unet_gradients_weighted", "wc_unet_train_step_weighted", "wc_unet_set_graph",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
     "wc_particle_records", "wc_set_event_tracking", "wc_clear_events", "wc_get_events", "wc_grid_noalloc",
     "wc_set_sort", "wc_get_sort", "wc_autotune_sort",
@@ -197,6 +197,7 @@ def load_library(path: str = LIB_PATH):
         "wc_unet_train_step": [vp, vp, vp, i, i, i, C.c_float, C.POINTER(C.c_float)],
         "wc_unet_gradients_weighted": [vp, vp, vp, vp, i, i, i, vp, C.POINTER(C.c_float)],
         "wc_unet_train_step_weighted": [vp, vp, vp, vp, i, i, i, C.c_float, C.POINTER(C.c_float)],
+        "wc_unet_set_graph": [vp, i],
     }
     for name, args in unet.items():
         f = getattr(L, name)
@@ -611,6 +612,12 @@ class UNet:
             self.close()
         except Exception:
             pass
+
+    def set_graph(self, on: bool):
+        """Replay the forward and backward passes of gradients() and
+        train_step() from a captured graph (the default) or launch them one
+        by one; the results are identical."""
+        self._chk(self.L.wc_unet_set_graph(self.h, 1 if on else 0))
 
     def get_params(self):
         out = np.zeros(self.n_params, np.float32)

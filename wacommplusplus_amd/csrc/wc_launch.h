@@ -189,16 +189,40 @@ struct WgradArgs {
     size_t partCap;
     // the masked output gradient already in BF16 (or nullptr: dy and y)
     const uint16_t *dyb;
+    // the bias gradient (optional, with dyb): db[co] += sum of dyb over the
+    // pixels.  The tiled path sums it beside dW and needs db == dw + cout 9
+    // cin (the parameter layout); the split-K path adds the mask launch's
+    // partials bpart[bnb][cout].
+    float *db;
+    const float *bpart;
+    int bnb;
 };
 // floats of workspace that launch_wgrad's tiled path may use
 constexpr size_t kWgradPartFloats = (size_t)1024 * 9 * 1024;
+// A sum over workgroups that an earlier launch left as partials, finished by
+// one workgroup of a later launch in a fixed order (so the result does not
+// depend on scheduling): out[c] += scale * sum_{b < nb} part[b * stride + c]
+// for c < m (m <= 256).  part null: no job.
+struct PartialSums {
+    const float *part;
+    int nb, m, stride;
+    float *out;
+    float scale;
+};
+struct SideSums {
+    PartialSums job[2];
+};
 hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s);
 hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s);
+// whether launch_wgrad takes the tiled path (which sums the bias itself)
+bool wgrad_is_tiled(const WgradArgs &a);
 // out[e] = bf16(y[e] > 0 ? dy[e] : 0), a layer's ReLU-masked output gradient
-// (E = pixels x cout), and its bias gradient db[co] += sum of the masked dy
-// (partial sums in part, partCap floats)
-hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *db,
-                            float *part, size_t partCap, hipStream_t s);
+// (E = pixels x cout, cout <= 256), and, when part is set, per workgroup the
+// sums of that BF16 gradient per channel in part[b][cout] (partCap floats;
+// *nblocks receives the number of workgroups).  The last workgroups also
+// finish the jobs in `side`.
+hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *part,
+                            size_t partCap, int *nblocks, const SideSums &side, hipStream_t s);
 hipError_t launch_pool_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
 hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, hipStream_t s);
 hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
@@ -209,9 +233,13 @@ hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c
                              hipStream_t s);
 hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, long long P, float *out,
                            hipStream_t s);
+// The head's backward for loss = mean wgt (out - t)^2: dd (the gradient of
+// the head's input), and per workgroup the partials part[b][kHead + 2] of
+// the head weights' and bias's gradients and of the loss (*nblocks
+// workgroups), left for a later PartialSums job.
+constexpr int kHeadPartials = 34;
 hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wgt, const float *wh,
-                           long long P,
-                           float *dd, float *dwh, float *loss, hipStream_t s);
+                           long long P, float *dd, float *part, size_t partCap, int *nblocks, hipStream_t s);
 hipError_t launch_flip(const uint16_t *w, int cout, int cin, uint16_t *wf, hipStream_t s);
 // Where each conv layer's weights sit: FP32 parameters [w_off, w_off +
 // cout*9*cin) as [cout][9][cin]; their BF16 copy at wb_off of wb, and the

@@ -47,6 +47,37 @@ union Pack8 {
     bf16x8 v;
 };
 
+// A PartialSums job by one workgroup of 256, in a fixed order: thread
+// c + m s sums the partials of workgroups s, s + S, s + 2S, ... (S = 256 / m)
+// of value c into four running sums (every fourth term each, so four loads
+// are in flight), then thread c adds the S segment sums in order.  The
+// partials come from an earlier launch on the stream.
+__device__ void fixed_order_sum(const PartialSums &j) {
+    __shared__ float seg[256];
+    const int m = j.m, S = 256 / m, t = threadIdx.x;
+    float s = 0.f;
+    if (t < m * S) {
+        const float *p = j.part + t % m;
+        const size_t st = (size_t)S * j.stride;
+        int b = t / m;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        for (; b + 3 * S < j.nb; b += 4 * S) {
+            const float *q = p + (size_t)b * j.stride;
+            s0 += q[0]; s1 += q[st]; s2 += q[2 * st]; s3 += q[3 * st];
+        }
+        for (; b < j.nb; b += S) s0 += p[(size_t)b * j.stride];
+        s = (s0 + s1) + (s2 + s3);
+    }
+    seg[t] = s;
+    __syncthreads();
+    if (t < m) {
+        float tot = 0.f;
+        for (int q = 0; q < S; ++q) tot += seg[q * m + t];
+        j.out[t] += j.scale * tot;
+    }
+}
+
+
 // 8 consecutive channels of one pixel as BF16 (zero when `valid` is false),
 // optionally masked by the BF16 ReLU output m (value kept where m > 0).
 template <class In>
@@ -397,8 +428,11 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
 // For a chunk of 32 pixels a lane's MFMA k-group is 8 consecutive pixels of
 // one row: A = 8 pixels of dY (one aligned 16-byte read); for each row
 // offset dy, three aligned 16-byte reads of the X row give the windows for
-// dx = -1, 0, +1 by byte alignment.  9 MFMAs per chunk.  Partial sums go to
-// part[z][co][tap][ci] (no atomics); wgrad_reduce_kernel adds them into dW.
+// dx = -1, 0, +1 by byte alignment.  9 MFMAs per chunk.  The workgroups of
+// the first input-channel block also sum the staged dY per output channel
+// (the bias gradient).  Partial sums go to part[z] = {dW [co][tap][ci], db
+// [co]} (no atomics); the reduction adds them into dW and db, which follow
+// each other in the parameters.
 constexpr int kWgPix = 256;
 
 __device__ __forceinline__ bf16x8 window(const uint4 &lo, const uint4 &mid, const uint4 &hi, int dx) {
@@ -435,6 +469,10 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
     const int ti = wave >> 1, tj = wave & 1, row = lane & 15, kq = lane >> 4;
     f32x4 acc[9];
     for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // bias gradient (workgroups of the first input-channel block): channel
+    // tid / 8 of the block, 16-byte groups tid % 8, tid % 8 + 8, ... of its row
+    const bool doBias = a.db && blockIdx.y == 0;
+    float bsum = 0.f;
     for (int st = blockIdx.z; st < stages; st += gridDim.z) {
         const int img = st / rowsPerImg, y0 = (st - img * rowsPerImg) * R;
         const int rv = min(R, H - y0);                 // rows of this stage inside the image
@@ -478,6 +516,14 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
             for (xp += 64; xp >= rowX; xp -= rowX) ++rr;
         }
         __syncthreads();
+        if (doBias) {
+            const uint16_t *bRow = sdy + (tid >> 3) * planeD;
+            for (int g8 = 8 * (tid & 7); g8 < RW; g8 += 64) {
+                Pack8 v;
+                v.u = *(const uint4 *)(bRow + g8);
+                for (int q = 0; q < 8; ++q) bsum += from_bf16(v.h[q]);
+            }
+        }
         const uint16_t *dRow = sdy + (ti * 16 + row) * planeD;
         const uint16_t *xPlane = sx + (tj * 16 + row) * planeX;
         int gr = (8 * kq) / W, gx = 8 * kq - gr * W;       // row and column of pixel pb + 8 kq
@@ -499,13 +545,19 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
             }
         }
     }
-    // D[4 kq + r][row]: co = co0 + 16 ti + 4 kq + r, ci = ci0 + 16 tj + row
-    float *dst = a.part + (size_t)blockIdx.z * a.cout * 9 * cin;
+    // D[4 kq + r][row]: co = co0 + 16 ti + 4 kq + r, ci = ci0 + 16 tj + row.
+    // Slice z holds dW [cout][9][cin] then the bias gradient [cout].
+    const size_t E = (size_t)a.cout * 9 * cin;
+    float *dst = a.part + (size_t)blockIdx.z * (E + (a.db ? a.cout : 0));
     for (int t = 0; t < 9; ++t)
         for (int r = 0; r < 4; ++r) {
             const int co = co0 + ti * 16 + kq * 4 + r, ci = ci0 + tj * 16 + row;
             dst[((size_t)co * 9 + t) * cin + ci] = acc[t][r];
         }
+    if (doBias) {                    // the 8 work-items of a channel are lanes 8c .. 8c + 7 of a wave
+        for (int off = 1; off < 8; off <<= 1) bsum += __shfl_xor(bsum, off, 64);
+        if ((tid & 7) == 0) dst[E + co0 + (tid >> 3)] = bsum;
+    }
 }
 
 // Deterministic reduction of part[zn][E] into dW in two passes: pass 1 sums
@@ -522,50 +574,19 @@ __global__ void wgrad_reduce_chunks_kernel(float *__restrict__ part, int zn, siz
     part[(size_t)z0 * E + e] = s;
 }
 
-__global__ void wgrad_reduce_kernel(const float *__restrict__ part, int zn, size_t E, float *__restrict__ dw) {
+// dW[e] += sum over slices z = 0, step, 2 step, ... < zn (step kWgChunk after
+// the chunk pass, 1 without it)
+__global__ void wgrad_reduce_kernel(const float *__restrict__ part, int zn, int step, size_t E,
+                                    float *__restrict__ dw) {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < E; e += (size_t)gridDim.x * blockDim.x) {
         float s = 0.f;
-        for (int z = 0; z < zn; z += kWgChunk) s += part[(size_t)z * E + e];
+        for (int z = 0; z < zn; z += step) s += part[(size_t)z * E + e];
         dw[e] += s;
     }
 }
 
-// The ReLU-masked output gradient of a layer in BF16, and each workgroup's
-// sums of the masked dY per channel (FP32) in part[block][cout], which
-// bias_reduce_kernel adds into the bias gradient.  A work-item takes 4
-// channels; cout divides 1024, so its channels stay the same over the
-// grid-stride loop.
-__global__ __launch_bounds__(256) void mask_bf16_kernel(const float *__restrict__ dy, const uint16_t *__restrict__ y,
-                                                        long long E4, int cout, uint16_t *__restrict__ out,
-                                                        float *__restrict__ part) {
-    const int tid = threadIdx.x;
-    float sb[4] = {0.f, 0.f, 0.f, 0.f};
-    for (long long q = blockIdx.x * 256LL + tid; q < E4; q += (long long)gridDim.x * 256) {
-        const float4 d = ((const float4 *)dy)[q];
-        const uint2 m = ((const uint2 *)y)[q];
-        const float g0 = from_bf16((uint16_t)m.x) > 0.f ? d.x : 0.f;
-        const float g1 = from_bf16((uint16_t)(m.x >> 16)) > 0.f ? d.y : 0.f;
-        const float g2 = from_bf16((uint16_t)m.y) > 0.f ? d.z : 0.f;
-        const float g3 = from_bf16((uint16_t)(m.y >> 16)) > 0.f ? d.w : 0.f;
-        sb[0] += g0; sb[1] += g1; sb[2] += g2; sb[3] += g3;
-        uint2 r;
-        r.x = (uint32_t)to_bf16(g0) | (uint32_t)to_bf16(g1) << 16;
-        r.y = (uint32_t)to_bf16(g2) | (uint32_t)to_bf16(g3) << 16;
-        ((uint2 *)out)[q] = r;
-    }
-    __shared__ float red[256][4];
-    for (int k = 0; k < 4; ++k) red[tid][k] = sb[k];
-    __syncthreads();
-    const int groups = cout / 4;               // work-items of one channel set: tid, tid + groups, ...
-    if (tid < groups)
-        for (int k = 0; k < 4; ++k) {
-            float s = 0.f;
-            for (int j = tid; j < 256; j += groups) s += red[j][k];
-            part[(size_t)blockIdx.x * cout + 4 * tid + k] = s;
-        }
-}
-
 // db[c] += sum_b part[b][c]: one workgroup per channel, a fixed-order tree
+// (the bias gradient beside the split-K weight gradient)
 __global__ __launch_bounds__(256) void bias_reduce_kernel(const float *__restrict__ part, int nb, int cout,
                                                           float *__restrict__ db) {
     const int c = blockIdx.x, tid = threadIdx.x;
@@ -579,6 +600,53 @@ __global__ __launch_bounds__(256) void bias_reduce_kernel(const float *__restric
         __syncthreads();
     }
     if (tid == 0) db[c] += red[0];
+}
+
+// The ReLU-masked output gradient of a layer in BF16 and, when part is set,
+// each workgroup's sums of the rounded masked dY per channel in
+// part[block][cout] (the bias gradient's partials for the split-K weight
+// gradient; the tiled one sums the bias itself).  A work-item takes 4
+// channels; cout divides 1024, so its channels stay the same over the
+// grid-stride loop.  The last workgroups also finish up to two pending sums.
+__global__ __launch_bounds__(256) void mask_bf16_kernel(const float *__restrict__ dy, const uint16_t *__restrict__ y,
+                                                        long long E4, int cout, uint16_t *__restrict__ out,
+                                                        float *__restrict__ part, SideSums side) {
+    const int tid = threadIdx.x;
+    float sb[4] = {0.f, 0.f, 0.f, 0.f};
+    for (long long q = blockIdx.x * 256LL + tid; q < E4; q += (long long)gridDim.x * 256) {
+        const float4 d = ((const float4 *)dy)[q];
+        const uint2 m = ((const uint2 *)y)[q];
+        const float g0 = from_bf16((uint16_t)m.x) > 0.f ? d.x : 0.f;
+        const float g1 = from_bf16((uint16_t)(m.x >> 16)) > 0.f ? d.y : 0.f;
+        const float g2 = from_bf16((uint16_t)m.y) > 0.f ? d.z : 0.f;
+        const float g3 = from_bf16((uint16_t)(m.y >> 16)) > 0.f ? d.w : 0.f;
+        const uint16_t h0 = to_bf16(g0), h1 = to_bf16(g1), h2 = to_bf16(g2), h3 = to_bf16(g3);
+        if (part) {
+            sb[0] += from_bf16(h0); sb[1] += from_bf16(h1); sb[2] += from_bf16(h2); sb[3] += from_bf16(h3);
+        }
+        uint2 r;
+        r.x = (uint32_t)h0 | (uint32_t)h1 << 16;
+        r.y = (uint32_t)h2 | (uint32_t)h3 << 16;
+        ((uint2 *)out)[q] = r;
+    }
+    __shared__ float red[256][4];
+    if (part) {
+        for (int k = 0; k < 4; ++k) red[tid][k] = sb[k];
+        __syncthreads();
+    }
+    const int groups = cout / 4;               // work-items of one channel set: tid, tid + groups, ...
+    if (part && tid < groups)
+        for (int k = 0; k < 4; ++k) {
+            float s = 0.f;
+            for (int j = tid; j < 256; j += groups) s += red[j][k];
+            part[(size_t)blockIdx.x * cout + 4 * tid + k] = s;
+        }
+    // the last workgroups also finish up to two jobs of earlier launches
+    for (int q = 0; q < 2; ++q)
+        if (side.job[q].part && (int)blockIdx.x == (int)gridDim.x - 1 - min(q, (int)gridDim.x - 1)) {
+            __syncthreads();
+            fixed_order_sum(side.job[q]);
+        }
 }
 
 // 2x2 average pooling, BF16 NHWC (h, w even)
@@ -744,8 +812,7 @@ __global__ void head_fwd_kernel(const uint16_t *__restrict__ d, const float *__r
 __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *__restrict__ d, const float *__restrict__ out,
                                                        const float *__restrict__ t, const float *__restrict__ wgt,
                                                        const float *__restrict__ wh, long long P,
-                                                       float *__restrict__ dd, float *__restrict__ dwh,
-                                                       float *__restrict__ loss) {
+                                                       float *__restrict__ dd, float *__restrict__ part) {
     // four work-items per pixel, 8 of its kHead channels each (16-byte loads)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cg = tid & 3;
     const float inv = 1.0f / (float)P;
@@ -775,15 +842,18 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *__restric
     if (lane < 4)
         for (int q = 0; q < 10; ++q) red[wave][lane][q] = v[q];
     __syncthreads();
+    // this workgroup's partials: the kHead weight gradients, the bias
+    // gradient, the weighted squared error over P (summed by a later launch)
+    constexpr int m = kHead + 2;
+    static_assert(m == kHeadPartials, "head partials layout");
     if (tid < 40) {
         const int c = tid / 10, q = tid - c * 10;
         const float s = red[0][c][q] + red[1][c][q] + red[2][c][q] + red[3][c][q];
-        if (q < 8) {
-            if (s != 0.f) atomicAdd(&dwh[8 * c + q], s);
-        } else if (c == 0) {
-            if (q == 8 && s != 0.f) atomicAdd(&dwh[kHead], s);
-            if (q == 9) atomicAdd(loss, s * inv);
-        }
+        float *pb = part + (size_t)blockIdx.x * m;
+        if (q < 8)
+            pb[8 * c + q] = s;
+        else if (c == 0)
+            pb[kHead + q - 8] = q == 9 ? s * inv : s;
     }
 }
 
@@ -864,6 +934,8 @@ __global__ void fill_f32_kernel(float *__restrict__ a, long long n, float v) {
         a[e] = v;
 }
 
+bool job_ok(const PartialSums &j) { return !j.part || (j.m >= 1 && j.m <= 256 && j.nb >= 1 && j.stride >= j.m); }
+
 int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     if (b > 16384) b = 16384;
@@ -926,36 +998,65 @@ hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
         int zn = 512 / blocks;
         if (zn > stages) zn = stages;
         if (zn < 1) zn = 1;
-        const size_t E = (size_t)a.cout * 9 * cin;
+        // a slice of partials: dW, then the bias gradient (a.db must be a.dw + E)
+        const size_t E = (size_t)a.cout * 9 * cin + (a.db ? a.cout : 0);
+        if (a.db && a.db != a.dw + (E - a.cout)) return hipErrorInvalidValue;
         if (lds <= 64 * 1024 && E * zn <= a.partCap) {
             const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)zn);
             wgrad_tile_kernel<<<grid, 256, lds, s>>>(a, R, rowsPerImg, stages);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
-            const dim3 g1((unsigned)((E + 255) / 256), (unsigned)((zn + kWgChunk - 1) / kWgChunk));
-            wgrad_reduce_chunks_kernel<<<g1, 256, 0, s>>>(a.part, zn, E);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-            wgrad_reduce_kernel<<<blocks_for((long long)E), 256, 0, s>>>(a.part, zn, E, a.dw);
+            int step = 1;
+            if (zn > kWgChunk) {        // sum chunks of kWgChunk slices first
+                const dim3 g1((unsigned)((E + 255) / 256), (unsigned)((zn + kWgChunk - 1) / kWgChunk));
+                wgrad_reduce_chunks_kernel<<<g1, 256, 0, s>>>(a.part, zn, E);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+                step = kWgChunk;
+            }
+            wgrad_reduce_kernel<<<blocks_for((long long)E), 256, 0, s>>>(a.part, zn, step, E, a.dw);
             return hipGetLastError();
         }
     }
     const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)(9 * a.splits));
     wgrad_kernel<<<grid, 256, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !a.db) return e;
+    if (!a.dyb || !a.bpart || a.bnb < 1) return hipErrorInvalidValue;
+    bias_reduce_kernel<<<a.cout, 256, 0, s>>>(a.bpart, a.bnb, a.cout, a.db);
     return hipGetLastError();
+}
+
+bool wgrad_is_tiled(const WgradArgs &a) {
+    const int cin = a.c1 + a.c2;
+    if (!a.part || a.w % 8 || a.w > kWgPix) return false;
+    const int R = kWgPix / a.w < a.h ? kWgPix / a.w : a.h;
+    const size_t lds = (size_t)(32 * (R * a.w + 8) + 32 * ((R + 2) * (a.w + 16) + 8)) * sizeof(uint16_t);
+    const int rowsPerImg = (a.h + R - 1) / R, stages = a.n * rowsPerImg;
+    const int blocks = (a.cout / 32) * (cin / 32);
+    int zn = 512 / blocks;
+    if (zn > stages) zn = stages;
+    if (zn < 1) zn = 1;
+    const size_t E = (size_t)a.cout * 9 * cin + (a.db ? a.cout : 0);
+    return lds <= 64 * 1024 && E * zn <= a.partCap;
 }
 
 // E = pixels x cout; cout a multiple of 4 that divides 1024; part holds
 // blocks x cout floats (at most 16384 x cout)
-hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *db,
-                            float *part, size_t partCap, hipStream_t s) {
-    if (cout < 4 || cout > 1024 || cout % 4 || 1024 % cout || E % cout) return hipErrorInvalidValue;
-    // 1024 workgroups keep the copy at full rate and the bias partials few
-    const int b = blocks_for(E / 4) < 1024 ? blocks_for(E / 4) : 1024;
+int mask_bf16_blocks(long long E) {
+    // at least 8 groups of 4 elements per work-item, at most 1024 workgroups:
+    // the copy runs at full rate and the bias partials stay few
+    long long b = (E / 4 + 2047) / 2048;
+    return (int)(b > 1024 ? 1024 : (b < 1 ? 1 : b));
+}
+
+hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *part,
+                            size_t partCap, int *nblocks, const SideSums &side, hipStream_t s) {
+    if (cout < 4 || cout > 256 || cout % 4 || 1024 % cout || E % cout) return hipErrorInvalidValue;
+    if (!job_ok(side.job[0]) || !job_ok(side.job[1])) return hipErrorInvalidValue;
+    const int b = mask_bf16_blocks(E);
     if ((size_t)b * cout > partCap) return hipErrorInvalidValue;
-    mask_bf16_kernel<<<b, 256, 0, s>>>(dy, y, E / 4, cout, out, part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    bias_reduce_kernel<<<cout, 256, 0, s>>>(part, b, cout, db);
+    mask_bf16_kernel<<<b, 256, 0, s>>>(dy, y, E / 4, cout, out, part, side);
+    *nblocks = b;
     return hipGetLastError();
 }
 
@@ -1008,11 +1109,12 @@ hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, l
     return hipGetLastError();
 }
 hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wgt, const float *wh,
-                           long long P,
-                           float *dd, float *dwh, float *loss, hipStream_t s) {
+                           long long P, float *dd, float *part, size_t partCap, int *nblocks, hipStream_t s) {
     int b = blocks_for(P);
-    if (b > 1024) b = 1024;
-    head_bwd_kernel<<<b, 256, 0, s>>>(d, out, t, wgt, wh, P, dd, dwh, loss);
+    if (b > 256) b = 256;
+    if ((size_t)b * (kHead + 2) > partCap) return hipErrorInvalidValue;
+    head_bwd_kernel<<<b, 256, 0, s>>>(d, out, t, wgt, wh, P, dd, part);
+    *nblocks = b;
     return hipGetLastError();
 }
 hipError_t launch_flip(const uint16_t *w, int cout, int cin, uint16_t *wf, hipStream_t s) {

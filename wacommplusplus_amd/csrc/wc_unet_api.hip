@@ -36,6 +36,8 @@ struct Layer {
 };
 
 constexpr int kBase = 32;
+constexpr size_t kBiasPartFloats = (size_t)1024 * 256;       // mask workgroups x channels
+constexpr size_t kHeadPartFloats = (size_t)256 * kHeadPartials;
 
 }  // namespace
 
@@ -50,8 +52,17 @@ struct wc_unet {
     WeightMap wmap{};
     float *d_loss = nullptr;
     float *d_part = nullptr;                     // weight-gradient partial sums (kWgradPartFloats)
+    float *d_bpart = nullptr;                    // a layer's bias-gradient partials (kBiasPartFloats)
+    float *d_hpart = nullptr;                    // the head's partials (kHeadPartFloats)
+    SideSums pending{};                          // sums the next mask launch finishes
     int adam_t = 0;
     double fwd_ms = 0;
+    // forward + backward of the gradient and training calls, captured once
+    // per batch shape (and loss kind) and replayed: ~110 launches per step
+    bool useGraph = true;
+    hipGraphExec_t gexec = nullptr;
+    int gN = 0, gH = 0, gW = 0;
+    bool gWgt = false;
     // activations / gradients for the current shape
     int n = 0, h = 0, w = 0;
     std::vector<void *> bufs;
@@ -93,7 +104,13 @@ int ufail(wc_unet *u, int code, const char *fmt, ...) {
                          hipGetErrorString(e_), __FILE__, __LINE__);                                 \
     } while (0)
 
+void drop_graph(wc_unet *u) {
+    if (u->gexec) hipGraphExecDestroy(u->gexec);
+    u->gexec = nullptr;
+}
+
 void free_shape(wc_unet *u) {
+    drop_graph(u);           // its kernels point into the buffers freed below
     for (void *b : u->bufs) hipFree(b);
     u->bufs.clear();
     u->n = u->h = u->w = 0;
@@ -169,10 +186,19 @@ int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, con
     g.part = u->d_part;
     g.partCap = kWgradPartFloats;
     // the masked gradient in BF16, once for the weight gradient and the data
-    // gradient, with the bias gradient
-    UCHK(u, launch_mask_bf16(gy, y, P * L.cout, L.cout, u->gmask, u->d_g + L.b_off, u->d_part, kWgradPartFloats,
-                             u->stream));
+    // gradient, and the bias gradient: summed by the tiled weight gradient,
+    // or from the mask launch's partials; the mask launch also finishes sums
+    // left pending
     g.dyb = u->gmask;
+    g.db = u->d_g + L.b_off;
+    const bool tiled = wgrad_is_tiled(g);
+    int nb = 0;
+    const SideSums side = u->pending;
+    u->pending = SideSums{};
+    UCHK(u, launch_mask_bf16(gy, y, P * L.cout, L.cout, u->gmask, tiled ? nullptr : u->d_bpart, kBiasPartFloats, &nb,
+                             side, u->stream));
+    g.bpart = u->d_bpart;
+    g.bnb = nb;
     UCHK(u, launch_wgrad(g, u->stream));
     if (!gx1) return WC_OK;
     ConvArgs a{};
@@ -224,11 +250,16 @@ int backward(wc_unet *u) {
     const int H = u->h, W = u->w, C = kBase, n = u->n;
     const auto &Ls = u->layers;
     int rc;
+    u->pending = SideSums{};
     UCHK(u, launch_fill_f32(u->d_g, (long long)u->nparam, 0.f, u->stream));
     UCHK(u, launch_fill_f32(u->d_loss, 1, 0.f, u->stream));
+    int hb = 0;
     UCHK(u, launch_head_bwd(u->d1, u->out, u->tgt, u->useWgt ? u->wgt : nullptr, u->d_p + u->head_off,
-                            (long long)n * H * W, u->g_d1,
-                            u->d_g + u->head_off, u->d_loss, u->stream));
+                            (long long)n * H * W, u->g_d1, u->d_hpart, kHeadPartFloats, &hb, u->stream));
+    // the head's weight and bias gradients and the loss: finished by the
+    // next mask launch
+    u->pending.job[0] = PartialSums{u->d_hpart, hb, kHeadPartials - 1, kHeadPartials, u->d_g + u->head_off, 1.0f};
+    u->pending.job[1] = PartialSums{u->d_hpart + kHeadPartials - 1, hb, 1, kHeadPartials, u->d_loss, 1.0f};
     // decoder, level 0
     if ((rc = conv_bwd(u, Ls[13], u->g_d1, u->d1, u->d1a, nullptr, u->g_d1a, false, nullptr, false, H, W))) return rc;
     if ((rc = conv_bwd(u, Ls[12], u->g_d1a, u->d1a, u->s1, u->u1, u->g_s1, false, u->g_u1, false, H, W))) return rc;
@@ -264,6 +295,37 @@ int backward(wc_unet *u) {
     UCHK(u, launch_pool_bwd(u->g_p1, n, H, W, C, u->g_s1, true, u->stream));
     if ((rc = conv_bwd(u, Ls[1], u->g_s1, u->s1, u->a1, nullptr, u->g_a1, false, nullptr, false, H, W))) return rc;
     if ((rc = conv_bwd(u, Ls[0], u->g_a1, u->a1, u->in0, nullptr, nullptr, false, nullptr, false, H, W))) return rc;
+    return WC_OK;
+}
+
+// forward() then backward() on the current batch.  With useGraph the pair is
+// captured into a graph the first time a batch shape (and loss kind) is seen
+// and replayed afterwards: the same kernels with the same arguments, so the
+// results are those of direct launches.  The batch buffers are fixed per
+// shape and the uploads write into them before the launch.
+int forward_backward(wc_unet *u) {
+    if (!u->useGraph) {
+        const int rc = forward(u);
+        return rc ? rc : backward(u);
+    }
+    if (!u->gexec || u->gN != u->n || u->gH != u->h || u->gW != u->w || u->gWgt != u->useWgt) {
+        drop_graph(u);
+        UCHK(u, hipStreamBeginCapture(u->stream, hipStreamCaptureModeThreadLocal));
+        int rc = forward(u);
+        if (!rc) rc = backward(u);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(u->stream, &g);    // ends the capture on every path
+        if (!rc && e != hipSuccess)
+            rc = ufail(u, WC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        if (!rc && hipGraphInstantiate(&u->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+            u->gexec = nullptr;
+            rc = ufail(u, WC_ERR_HIP, "hipGraphInstantiate of the training step failed");
+        }
+        if (g) hipGraphDestroy(g);
+        if (rc) return rc;
+        u->gN = u->n; u->gH = u->h; u->gW = u->w; u->gWgt = u->useWgt;
+    }
+    UCHK(u, hipGraphLaunch(u->gexec, u->stream));
     return WC_OK;
 }
 
@@ -350,7 +412,8 @@ int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
     if (hipMalloc(&u->d_p, u->nparam * 4) || hipMalloc(&u->d_g, u->nparam * 4) || hipMalloc(&u->d_m, u->nparam * 4) ||
         hipMalloc(&u->d_v, u->nparam * 4) || hipMalloc(&u->d_wb, u->nwb * 2 + 256) ||
         hipMalloc(&u->d_wf, u->nwb * 2 + 256) || hipMalloc(&u->d_loss, 256) ||
-        hipMalloc(&u->d_part, kWgradPartFloats * sizeof(float)))
+        hipMalloc(&u->d_part, kWgradPartFloats * sizeof(float)) ||
+        hipMalloc(&u->d_bpart, kBiasPartFloats * sizeof(float)) || hipMalloc(&u->d_hpart, kHeadPartFloats * sizeof(float)))
         return bad(WC_ERR_NOMEM);
     if (hipMemcpy(u->d_p, p.data(), u->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return bad(WC_ERR_HIP);
     if (launch_fill_f32(u->d_m, (long long)u->nparam, 0.f, u->stream) != hipSuccess ||
@@ -367,13 +430,22 @@ int wc_unet_destroy(wc_unet *u) {
     if (u->stream) hipStreamSynchronize(u->stream);
     free_shape(u);
     hipFree(u->d_p); hipFree(u->d_g); hipFree(u->d_m); hipFree(u->d_v); hipFree(u->d_wb); hipFree(u->d_wf);
-    hipFree(u->d_loss); hipFree(u->d_part);
+    hipFree(u->d_loss); hipFree(u->d_part); hipFree(u->d_bpart); hipFree(u->d_hpart);
     if (u->stream) hipStreamDestroy(u->stream);
     delete u;
     return WC_OK;
 }
 
 const char *wc_unet_last_error(const wc_unet *u) { return u ? u->err.c_str() : "null network"; }
+
+int wc_unet_set_graph(wc_unet *u, int on) {
+    if (!u) return WC_ERR_INVALID;
+    UCHK(u, hipSetDevice(u->device));
+    UCHK(u, hipStreamSynchronize(u->stream));
+    drop_graph(u);
+    u->useGraph = on != 0;
+    return WC_OK;
+}
 
 size_t wc_unet_param_count(const wc_unet *u) { return u ? u->nparam : 0; }
 
@@ -431,8 +503,7 @@ int wc_unet_gradients_weighted(wc_unet *u, const float *x, const float *t, const
     if (!u || !x || !t) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
     int rc = upload(u, x, t, weight, n, h, w);
-    if (!rc) rc = forward(u);
-    if (!rc) rc = backward(u);
+    if (!rc) rc = forward_backward(u);
     if (rc) return rc;
     if (grad) UCHK(u, hipMemcpyAsync(grad, u->d_g, u->nparam * 4, hipMemcpyDeviceToHost, u->stream));
     float l = 0;
@@ -451,8 +522,7 @@ int wc_unet_train_step_weighted(wc_unet *u, const float *x, const float *t, cons
     if (!u || !x || !t || !(lr > 0)) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
     int rc = upload(u, x, t, weight, n, h, w);
-    if (!rc) rc = forward(u);
-    if (!rc) rc = backward(u);
+    if (!rc) rc = forward_backward(u);
     if (rc) return rc;
     u->adam_t += 1;
     UCHK(u, launch_adam_bf16(u->d_p, u->d_g, u->d_m, u->d_v, (long long)u->nparam, lr, 0.9f, 0.999f, 1e-8f,
