@@ -44,6 +44,11 @@ constexpr size_t kHeadPartFloats = (size_t)256 * kHeadPartials;
 struct wc_unet {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the backward pass runs each layer's weight gradient (and its
+    // reductions) on `side`, beside the data gradient on `stream`:
+    // evMask[l] orders it after layer l's mask, evWgrad[l] marks it done
+    hipStream_t side = nullptr;
+    hipEvent_t evMask[kMaxConvLayers] = {}, evWgrad[kMaxConvLayers] = {};
     std::string err;
     std::vector<Layer> layers;
     size_t nparam = 0, nwb = 0, head_off = 0;
@@ -52,7 +57,7 @@ struct wc_unet {
     WeightMap wmap{};
     float *d_loss = nullptr;
     float *d_part = nullptr;                     // weight-gradient partial sums (kWgradPartFloats)
-    float *d_bpart = nullptr;                    // a layer's bias-gradient partials (kBiasPartFloats)
+    float *d_bpart = nullptr;                    // bias-gradient partials, two buffers of kBiasPartFloats
     float *d_hpart = nullptr;                    // the head's partials (kHeadPartFloats)
     SideSums pending{};                          // sums the next mask launch finishes
     int adam_t = 0;
@@ -70,7 +75,10 @@ struct wc_unet {
              *p2 = nullptr, *a3 = nullptr, *s3 = nullptr, *p3 = nullptr, *a4 = nullptr, *b4 = nullptr,
              *u3 = nullptr, *d3a = nullptr, *d3 = nullptr, *u2 = nullptr, *d2a = nullptr, *d2 = nullptr,
              *u1 = nullptr, *d1a = nullptr, *d1 = nullptr;
-    uint16_t *gmask = nullptr;      // a layer's ReLU-masked output gradient in BF16 (backward scratch)
+    // a layer's ReLU-masked output gradient in BF16 (backward scratch): layer
+    // l uses gmask[l % 2], so the weight gradient of layer l may still read
+    // it while the mask of layer l - 1 is written
+    uint16_t *gmask[2] = {nullptr, nullptr};
     float *x = nullptr, *out = nullptr, *tgt = nullptr;
     float *wgt = nullptr;     // per-pixel loss weights of the current batch
     bool useWgt = false;
@@ -131,6 +139,7 @@ int ensure_shape(wc_unet *u, int n, int h, int w) {
         return ufail(u, WC_ERR_INVALID, "images must be at least 8x8 with sides divisible by 8 (got %dx%d)", h, w);
     if ((long long)n * h * w * 384 > (1LL << 40)) return ufail(u, WC_ERR_INVALID, "batch too large");
     UCHK(u, hipStreamSynchronize(u->stream));
+    UCHK(u, hipStreamSynchronize(u->side));
     free_shape(u);
     const size_t P0 = (size_t)n * h * w, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
     const int C = kBase;
@@ -141,7 +150,7 @@ int ensure_shape(wc_unet *u, int n, int h, int w) {
     rc |= alloc(u, u->a4, P3 * 8 * C) | alloc(u, u->b4, P3 * 8 * C) | alloc(u, u->u3, P2 * 8 * C);
     rc |= alloc(u, u->d3a, P2 * 4 * C) | alloc(u, u->d3, P2 * 4 * C) | alloc(u, u->u2, P1 * 4 * C);
     rc |= alloc(u, u->d2a, P1 * 2 * C) | alloc(u, u->d2, P1 * 2 * C) | alloc(u, u->u1, P0 * 2 * C);
-    rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C) | alloc(u, u->gmask, P0 * C);
+    rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C) | alloc(u, u->gmask[0], P0 * C) | alloc(u, u->gmask[1], P0 * C);
     rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0) | alloc(u, u->wgt, P0);
     if (u->cin > 1) rc |= alloc(u, u->xc, P0 * u->cin);
     rc |= alloc(u, u->g_d1, P0 * C) | alloc(u, u->g_d1a, P0 * C) | alloc(u, u->g_s1, P0 * C) |
@@ -178,6 +187,11 @@ int conv_fwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2,
 int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, const uint16_t *x1, const uint16_t *x2,
              float *gx1, bool acc1, float *gx2, bool acc2, int h, int w) {
     const long long P = (long long)u->n * h * w;
+    const int li = (int)(&L - u->layers.data()), buf = li & 1;
+    uint16_t *gm = u->gmask[buf];
+    float *bpart = u->d_bpart + buf * kBiasPartFloats;
+    // the buffers of parity `buf` were last read by the weight gradient of layer li + 2
+    if (li + 2 < (int)u->layers.size()) UCHK(u, hipStreamWaitEvent(u->stream, u->evWgrad[li + 2], 0));
     WgradArgs g{};
     g.dy = gy; g.y = y; g.x1 = x1; g.x2 = x2; g.c1 = L.c1; g.c2 = L.c2; g.cout = L.cout;
     g.dw = u->d_g + L.w_off; g.n = u->n; g.h = h; g.w = w;
@@ -189,20 +203,23 @@ int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, con
     // gradient, and the bias gradient: summed by the tiled weight gradient,
     // or from the mask launch's partials; the mask launch also finishes sums
     // left pending
-    g.dyb = u->gmask;
+    g.dyb = gm;
     g.db = u->d_g + L.b_off;
     const bool tiled = wgrad_is_tiled(g);
     int nb = 0;
     const SideSums side = u->pending;
     u->pending = SideSums{};
-    UCHK(u, launch_mask_bf16(gy, y, P * L.cout, L.cout, u->gmask, tiled ? nullptr : u->d_bpart, kBiasPartFloats, &nb,
-                             side, u->stream));
-    g.bpart = u->d_bpart;
+    UCHK(u, launch_mask_bf16(gy, y, P * L.cout, L.cout, gm, tiled ? nullptr : bpart, kBiasPartFloats, &nb, side,
+                             u->stream));
+    UCHK(u, hipEventRecord(u->evMask[li], u->stream));
+    g.bpart = bpart;
     g.bnb = nb;
-    UCHK(u, launch_wgrad(g, u->stream));
+    UCHK(u, hipStreamWaitEvent(u->side, u->evMask[li], 0));
+    UCHK(u, launch_wgrad(g, u->side));
+    UCHK(u, hipEventRecord(u->evWgrad[li], u->side));
     if (!gx1) return WC_OK;
     ConvArgs a{};
-    a.x1 = u->gmask; a.x2 = nullptr; a.m1 = nullptr; a.c1 = L.cout; a.c2 = 0;
+    a.x1 = gm; a.x2 = nullptr; a.m1 = nullptr; a.c1 = L.cout; a.c2 = 0;
     a.w_ = u->d_wf + L.wb_off; a.bias = nullptr;
     a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
     a.n = u->n; a.h = h; a.w = w;
@@ -295,6 +312,8 @@ int backward(wc_unet *u) {
     UCHK(u, launch_pool_bwd(u->g_p1, n, H, W, C, u->g_s1, true, u->stream));
     if ((rc = conv_bwd(u, Ls[1], u->g_s1, u->s1, u->a1, nullptr, u->g_a1, false, nullptr, false, H, W))) return rc;
     if ((rc = conv_bwd(u, Ls[0], u->g_a1, u->a1, u->in0, nullptr, nullptr, false, nullptr, false, H, W))) return rc;
+    // the weight gradients ran in layer order on the side stream: the last one closes it
+    UCHK(u, hipStreamWaitEvent(u->stream, u->evWgrad[0], 0));
     return WC_OK;
 }
 
@@ -408,12 +427,18 @@ int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
         wc_unet_destroy(u);
         return code;
     };
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&u->stream) != hipSuccess) return bad(WC_ERR_HIP);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&u->stream) != hipSuccess ||
+        hipStreamCreate(&u->side) != hipSuccess)
+        return bad(WC_ERR_HIP);
+    for (int l = 0; l < kMaxConvLayers; ++l)
+        if (hipEventCreateWithFlags(&u->evMask[l], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&u->evWgrad[l], hipEventDisableTiming) != hipSuccess)
+            return bad(WC_ERR_HIP);
     if (hipMalloc(&u->d_p, u->nparam * 4) || hipMalloc(&u->d_g, u->nparam * 4) || hipMalloc(&u->d_m, u->nparam * 4) ||
         hipMalloc(&u->d_v, u->nparam * 4) || hipMalloc(&u->d_wb, u->nwb * 2 + 256) ||
         hipMalloc(&u->d_wf, u->nwb * 2 + 256) || hipMalloc(&u->d_loss, 256) ||
         hipMalloc(&u->d_part, kWgradPartFloats * sizeof(float)) ||
-        hipMalloc(&u->d_bpart, kBiasPartFloats * sizeof(float)) || hipMalloc(&u->d_hpart, kHeadPartFloats * sizeof(float)))
+        hipMalloc(&u->d_bpart, 2 * kBiasPartFloats * sizeof(float)) || hipMalloc(&u->d_hpart, kHeadPartFloats * sizeof(float)))
         return bad(WC_ERR_NOMEM);
     if (hipMemcpy(u->d_p, p.data(), u->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return bad(WC_ERR_HIP);
     if (launch_fill_f32(u->d_m, (long long)u->nparam, 0.f, u->stream) != hipSuccess ||
@@ -428,9 +453,15 @@ int wc_unet_destroy(wc_unet *u) {
     if (!u) return WC_OK;
     hipSetDevice(u->device);
     if (u->stream) hipStreamSynchronize(u->stream);
+    if (u->side) hipStreamSynchronize(u->side);
     free_shape(u);
     hipFree(u->d_p); hipFree(u->d_g); hipFree(u->d_m); hipFree(u->d_v); hipFree(u->d_wb); hipFree(u->d_wf);
     hipFree(u->d_loss); hipFree(u->d_part); hipFree(u->d_bpart); hipFree(u->d_hpart);
+    for (int l = 0; l < kMaxConvLayers; ++l) {
+        if (u->evMask[l]) hipEventDestroy(u->evMask[l]);
+        if (u->evWgrad[l]) hipEventDestroy(u->evWgrad[l]);
+    }
+    if (u->side) hipStreamDestroy(u->side);
     if (u->stream) hipStreamDestroy(u->stream);
     delete u;
     return WC_OK;
@@ -442,6 +473,7 @@ int wc_unet_set_graph(wc_unet *u, int on) {
     if (!u) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
     UCHK(u, hipStreamSynchronize(u->stream));
+    UCHK(u, hipStreamSynchronize(u->side));
     drop_graph(u);
     u->useGraph = on != 0;
     return WC_OK;
