@@ -527,7 +527,7 @@ def main():
     for key, r in results.items():
         if key == "f64":
             continue
-        store = key.replace("_euler", "").replace("_unsorted", "")
+        store = key.replace("_euler", "").replace("_unsorted", "").replace("_lds", "")
         euler = key.endswith("_euler")
         ckey = f"{key.replace('_unsorted', '')}@t{r['sort_tile']}"
         if store == "f64":
@@ -542,6 +542,10 @@ def main():
                                  integrator="euler" if euler else "midpoint",
                                  path="parity (FP64)" if store == "f64" else f"fast ({store} storage)",
                                  roofline=b)
+        if key.endswith("_lds"):
+            variants_out[key]["lds"] = dict(chunk=LDS_TILING[0], halo=LDS_TILING[1],
+                                            what="forcing boxes staged in LDS per chunk of substeps (wc_set_lds); "
+                                                 "results identical to the plain move")
     line = {
         "metric": "particle-steps/s at 4M (BF16 vs FP64 forcing); conc. L2 err at equal GPU-ms",
         "value": value,

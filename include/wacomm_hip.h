@@ -389,6 +389,18 @@ int wc_set_sort(wc_ctx *ctx, int tile, int period);
 int wc_get_sort(wc_ctx *ctx, int *tile, int *period);
 int wc_autotune_sort(wc_ctx *ctx, const wc_config_data *config, double start, double end, int slot0, int slot1,
                      int path, int *tile, double *best_ms);
+/* LDS-staged forcing for the mixed-precision move (no counterpart in the
+ * reference): with chunk > 0, WC_PATH_FAST moves of the plain packed storage
+ * types (WC_STORE_BF16, WC_STORE_F16, WC_STORE_F32) run the interval in
+ * chunks of `chunk` substeps, and before each chunk every workgroup copies
+ * the interval pack of the cells its particles occupy, widened by `halo`
+ * cells (shrunk until the box fits in 47 KiB), into LDS; stencils inside the
+ * box read LDS, any other reads global memory.  The values read, and so
+ * every result, are those of the plain move.  Other storage types, the
+ * drift-side jibe and event tracking use the plain move.  chunk 0 (the
+ * default) turns it off; chunk <= 1024, halo <= 16. */
+int wc_set_lds(wc_ctx *ctx, int chunk, int halo);
+int wc_get_lds(wc_ctx *ctx, int *chunk, int *halo);
 int wc_clear_events(wc_ctx *ctx);
 int wc_get_events(wc_ctx *ctx, uint8_t *events, size_t n);
 

@@ -33,7 +33,7 @@ EXPORTS = (
     "wc_unet_create_ex", "wc_unet_gradients_weighted", "wc_unet_train_step_weighted", "wc_unet_set_graph",
     "wc_write_concentration_nc", "wc_write_restart_nc", "wc_read_restart_nc", "wc_set_geo",
     "wc_particle_records", "wc_set_event_tracking", "wc_clear_events", "wc_get_events", "wc_grid_noalloc",
-    "wc_set_sort", "wc_get_sort", "wc_autotune_sort",
+    "wc_set_sort", "wc_get_sort", "wc_autotune_sort", "wc_set_lds", "wc_get_lds",
 )
 
 
@@ -141,6 +141,8 @@ def load_library(path: str = LIB_PATH):
         "wc_set_event_tracking": [vp, i],
         "wc_set_sort": [vp, i, i],
         "wc_get_sort": [vp, C.POINTER(i), C.POINTER(i)],
+        "wc_set_lds": [vp, i, i],
+        "wc_get_lds": [vp, C.POINTER(i), C.POINTER(i)],
         "wc_autotune_sort": [vp, C.POINTER(ConfigData), d, d, i, i, i, C.POINTER(i), C.POINTER(d)],
         "wc_clear_events": [vp],
         "wc_get_events": [vp, vp, sz],
@@ -368,6 +370,16 @@ class Context:
         return out
 
     # -- cell-tile order of the moves ----------------------------------------
+    def set_lds(self, chunk: int, halo: int = 2):
+        """LDS-staged forcing for the mixed-precision move (wc_set_lds);
+        chunk 0 turns it off.  Results are those of the plain move."""
+        self._chk(self.L.wc_set_lds(self.h, int(chunk), int(halo)))
+
+    def get_lds(self):
+        c, h = C.c_int(), C.c_int()
+        self._chk(self.L.wc_get_lds(self.h, C.byref(c), C.byref(h)))
+        return c.value, h.value
+
     def set_sort(self, tile: int, period: int = 1):
         self._chk(self.L.wc_set_sort(self.h, int(tile), int(period)))
 

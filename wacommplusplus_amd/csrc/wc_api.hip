@@ -67,6 +67,8 @@ struct wc_ctx {
     uint32_t *d_scanSums = nullptr;       // kScanSumsCap entries: scratch of the multi-block scans
     int sortTile = 0, sortPeriod = 1;
     long sortCount = 0;
+    // LDS staging of the forcing for the mixed-precision move (wc_set_lds)
+    LdsTiling lds{0, 0};
     bool permValid = false;
     SoA s{}, s2{};
     uint32_t *d_blockCounts = nullptr;
@@ -1116,6 +1118,21 @@ int wc_set_sort(wc_ctx *c, int tile, int period) {
     return WC_OK;
 }
 
+int wc_set_lds(wc_ctx *c, int chunk, int halo) {
+    if (!c) return WC_ERR_INVALID;
+    if (chunk < 0 || chunk > 1024 || halo < 0 || halo > 16)
+        return fail(c, WC_ERR_INVALID, "LDS chunk %d (0..1024), halo %d (0..16)", chunk, halo);
+    c->lds = LdsTiling{chunk, halo};
+    return WC_OK;
+}
+
+int wc_get_lds(wc_ctx *c, int *chunk, int *halo) {
+    if (!c) return WC_ERR_INVALID;
+    if (chunk) *chunk = c->lds.chunk;
+    if (halo) *halo = c->lds.halo;
+    return WC_OK;
+}
+
 int wc_get_sort(wc_ctx *c, int *tile, int *period) {
     if (!c) return WC_ERR_INVALID;
     if (tile) *tile = c->sortTile;
@@ -1228,7 +1245,7 @@ int wc_move_interval(wc_ctx *c, const wc_config_data *cfg, double start, double 
                                              c->trackEvents, perm, stage, c->stream));
         else
             HIPCHK(c, launch_move_fast_soa(c->store, P, F, c->d_pack, G, c->s, (int)c->n, c->d_steps,
-                                           c->trackEvents, perm, stage, c->stream));
+                                           c->trackEvents, perm, stage, c->lds, c->stream));
         if (stage == kStageMove) {
             // an upload into either slot now waits for this launch
             HIPCHK(c, hipEventRecord(c->slots[slot0].released, c->stream));

@@ -31,9 +31,16 @@ hipError_t launch_grid_hash(const double *lon, const double *lat, size_t n, unsi
 // into packBuf (fast_pack_bytes) and moves the particles.  G.idxf must point to
 // the float2 {1/dx, 1/dy} table.
 size_t fast_pack_bytes(int store, int s_w, int eta, int xi);
+// LDS staging of the forcing for the move (plain packed storage, no jibe, no
+// event tracking): chunk > 0 substeps per staged box, halo cells around it
+struct LdsTiling {
+    int chunk, halo;
+};
+constexpr int kMoveLdsBytes = 47 * 1024;     // per workgroup: three workgroups per CU
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
                                 const GridPtrs &G, const SoA &s, int n, unsigned long long *stepCounter,
-                                bool events, const uint32_t *perm, int stages, hipStream_t stream);
+                                bool events, const uint32_t *perm, int stages, const LdsTiling &lt,
+                                hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *a, size_t n, uint32_t v, hipStream_t stream);
 // perm (optional): visit the particles in that order

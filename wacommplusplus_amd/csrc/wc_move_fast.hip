@@ -1083,6 +1083,80 @@ void move_ens_kernel(const EnsStep *__restrict__ steps, const int *__restrict__ 
     if (stepCounter) wave_add_u64(stepCounter, total);
 }
 
+// The LDS box of one chunk of substeps, shared by the LDS-tiled moves.  Each
+// work-item offers its cell and level when `act` (it steps in the chunk);
+// the workgroup reduces the box of those cells widened by `halo` cells, one
+// level each way and the stencil's next cell, shrinks the halo until the
+// box fits in `ldsBytes`, and copies the box's packed u/v, w/AKt, zeta/h and
+// metric into LDS with coalesced loads (no box when nothing steps: a 1 x 1
+// box at (-2, -2) with no levels, which no cell or stencil lies in).  The
+// first barrier also ends every read of the previous chunk's box.  The
+// returned sampler reads LDS inside the box and global memory outside, so
+// the values are those of S wherever a particle goes.
+template <class T>
+__device__ TilePacked<T> stage_box(const Packed<T> &S, bool act, int jc, int ic, int kI, int eta, int xi, int halo,
+                                   int ldsBytes, unsigned char *lds, int (*part)[6], int *box) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nlev = S.nlev;
+    int v[6] = {act ? jc : INT_MAX, act ? -jc : INT_MAX, act ? ic : INT_MAX,
+                act ? -ic : INT_MAX, act ? kI : INT_MAX, act ? -kI : INT_MAX};
+    for (int off = 32; off > 0; off >>= 1)
+        for (int r = 0; r < 6; ++r) v[r] = min(v[r], __shfl_xor(v[r], off, 64));
+    if (lane == 0)
+        for (int r = 0; r < 6; ++r) part[wave][r] = v[r];
+    __syncthreads();
+    if (tid == 0) {
+        int m[6];
+        for (int r = 0; r < 6; ++r) m[r] = min(min(part[0][r], part[1][r]), min(part[2][r], part[3][r]));
+        int b[6] = {-2, 1, -2, 1, 0, 0};                  // j0, bh, i0, bw, l0, nl
+        const int lLo = max(0, m[4] - 2 + nlev - 1), lHi = min(nlev - 1, -m[5] + 1 + nlev - 1);
+        const int nl = lHi - lLo + 1;
+        if (m[0] != INT_MAX && nl >= 1) {
+            for (int h = halo; h >= 0; --h) {
+                const int j0 = max(0, m[0] - h), j1 = min(eta - 1, -m[1] + 1 + h);
+                const int i0 = max(0, m[2] - h), i1 = min(xi - 1, -m[3] + 1 + h);
+                const int bh = j1 - j0 + 1, bw = i1 - i0 + 1, cells = bh * bw;
+                const long long need = (long long)nl * cells * (long long)(sizeof(WA8<T>) + sizeof(UV4<T>)) +
+                                       (long long)cells * (long long)(sizeof(ZH) + sizeof(float2));
+                if (need <= ldsBytes) {
+                    b[0] = j0; b[1] = bh; b[2] = i0; b[3] = bw; b[4] = lLo; b[5] = nl;
+                    break;
+                }
+            }
+        }
+        for (int r = 0; r < 6; ++r) box[r] = b[r];
+    }
+    __syncthreads();
+    TilePacked<T> TS;
+    TS.g = S;
+    TS.j0 = box[0]; TS.bh = box[1]; TS.i0 = box[2]; TS.bw = box[3]; TS.l0 = box[4]; TS.nl = box[5];
+    const int cells = TS.bh * TS.bw;
+    WA8<T> *wL = reinterpret_cast<WA8<T> *>(lds);
+    ZH *zL = reinterpret_cast<ZH *>(lds + (size_t)TS.nl * cells * sizeof(WA8<T>));
+    UV4<T> *uL = reinterpret_cast<UV4<T> *>(reinterpret_cast<unsigned char *>(zL) + (size_t)cells * sizeof(ZH));
+    float2 *mL = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(uL) +
+                                            (size_t)TS.nl * cells * sizeof(UV4<T>));
+    TS.wL = wL; TS.zL = zL; TS.uL = uL; TS.mL = mL;
+    if (TS.nl > 0) {
+        for (int c = tid; c < cells; c += 256) {
+            const int jj = c / TS.bw, ii = c - jj * TS.bw;
+            const size_t g = (size_t)(TS.j0 + jj) * S.xi + (TS.i0 + ii);
+            zL[c] = S.z[g];
+            mL[c] = S.md[g];
+        }
+        const int items = TS.nl * cells;
+        for (int c = tid; c < items; c += 256) {
+            const int l = c / cells, r = c - l * cells;
+            const int jj = r / TS.bw, ii = r - jj * TS.bw;
+            const size_t g = (size_t)(TS.l0 + l) * S.plane + (size_t)(TS.j0 + jj) * S.xi + (TS.i0 + ii);
+            wL[c] = S.wa[g];
+            uL[c] = S.uv[g];
+        }
+    }
+    __syncthreads();
+    return TS;
+}
+
 // The ensemble move with the forcing staged in LDS.  One particle per work-
 // item, 256 per workgroup.  The interval runs in chunks of `chunk` substeps;
 // before each chunk the workgroup reduces the cells and levels of its
@@ -1102,8 +1176,7 @@ void move_ens_lds_kernel(const EnsStep *__restrict__ steps, const int *__restric
     __shared__ int part[4][6];
     __shared__ int box[6];
     const MoveParams P = steps[*stepIdx].P;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int idx = blockIdx.x * 256 + tid;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
     const bool have = idx < n;
     Reg q{};
     float k0 = 0.f, h0 = 0.f, kStart = 0.f, sigma = 0.f;
@@ -1148,72 +1221,14 @@ void move_ens_lds_kernel(const EnsStep *__restrict__ steps, const int *__restric
     }
     const double L = P.intervalLength, span = (double)chunk * P.dti;
     const int nChunks = L > P.elapsed0 ? (int)ceil((L - P.elapsed0) / span) : 0;
-    const int nlev = S.nlev;
     unsigned total = 0;
     for (int ch = 0; ch < nChunks; ++ch) {
         const double tEnd = ch + 1 == nChunks ? L : fmin(L, P.elapsed0 + (double)(ch + 1) * span);
         // box of the particles that step in this chunk (forward: emitted before its end)
         const bool act = !q.done && q.inside && q.elapsed < tEnd && q.k >= -1e6f && q.k <= 1e6f &&
                          (P.dirInt < 0 || ptime - P.intervalStart < tEnd);
-        const int kI = act ? (int)ceilf(q.k) : 0;
-        int v[6] = {act ? q.jc : INT_MAX, act ? -q.jc : INT_MAX, act ? q.ic : INT_MAX,
-                    act ? -q.ic : INT_MAX, act ? kI : INT_MAX, act ? -kI : INT_MAX};
-        for (int off = 32; off > 0; off >>= 1)
-            for (int r = 0; r < 6; ++r) v[r] = min(v[r], __shfl_xor(v[r], off, 64));
-        if (lane == 0)
-            for (int r = 0; r < 6; ++r) part[wave][r] = v[r];
-        __syncthreads();                 // also: every read of the previous chunk's tile is done
-        if (tid == 0) {
-            int m[6];
-            for (int r = 0; r < 6; ++r) m[r] = min(min(part[0][r], part[1][r]), min(part[2][r], part[3][r]));
-            // j0, bh, i0, bw, l0, nl; no box: a 1 x 1 box at (-2, -2) with no
-            // levels, which no cell or stencil lies in
-            int b[6] = {-2, 1, -2, 1, 0, 0};
-            const int lLo = max(0, m[4] - 2 + nlev - 1), lHi = min(nlev - 1, -m[5] + 1 + nlev - 1);
-            const int nl = lHi - lLo + 1;
-            if (m[0] != INT_MAX && nl >= 1) {
-                for (int h = halo; h >= 0; --h) {
-                    const int j0 = max(0, m[0] - h), j1 = min(P.eta - 1, -m[1] + 1 + h);
-                    const int i0 = max(0, m[2] - h), i1 = min(P.xi - 1, -m[3] + 1 + h);
-                    const int bh = j1 - j0 + 1, bw = i1 - i0 + 1, cells = bh * bw;
-                    const long long need = (long long)nl * cells * (long long)(sizeof(WA8<T>) + sizeof(UV4<T>)) +
-                                           (long long)cells * (long long)(sizeof(ZH) + sizeof(float2));
-                    if (need <= ldsBytes) {
-                        b[0] = j0; b[1] = bh; b[2] = i0; b[3] = bw; b[4] = lLo; b[5] = nl;
-                        break;
-                    }
-                }
-            }
-            for (int r = 0; r < 6; ++r) box[r] = b[r];
-        }
-        __syncthreads();
-        TilePacked<T> TS;
-        TS.g = S;
-        TS.j0 = box[0]; TS.bh = box[1]; TS.i0 = box[2]; TS.bw = box[3]; TS.l0 = box[4]; TS.nl = box[5];
-        const int cells = TS.bh * TS.bw;
-        WA8<T> *wL = reinterpret_cast<WA8<T> *>(lds);
-        ZH *zL = reinterpret_cast<ZH *>(lds + (size_t)TS.nl * cells * sizeof(WA8<T>));
-        UV4<T> *uL = reinterpret_cast<UV4<T> *>(reinterpret_cast<unsigned char *>(zL) + (size_t)cells * sizeof(ZH));
-        float2 *mL = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(uL) +
-                                                (size_t)TS.nl * cells * sizeof(UV4<T>));
-        TS.wL = wL; TS.zL = zL; TS.uL = uL; TS.mL = mL;
-        if (TS.nl > 0) {
-            for (int c = tid; c < cells; c += 256) {
-                const int jj = c / TS.bw, ii = c - jj * TS.bw;
-                const size_t g = (size_t)(TS.j0 + jj) * S.xi + (TS.i0 + ii);
-                zL[c] = S.z[g];
-                mL[c] = S.md[g];
-            }
-            const int items = TS.nl * cells;
-            for (int c = tid; c < items; c += 256) {
-                const int l = c / cells, r = c - l * cells;
-                const int jj = r / TS.bw, ii = r - jj * TS.bw;
-                const size_t g = (size_t)(TS.l0 + l) * S.plane + (size_t)(TS.j0 + jj) * S.xi + (TS.i0 + ii);
-                wL[c] = S.wa[g];
-                uL[c] = S.uv[g];
-            }
-        }
-        __syncthreads();
+        const TilePacked<T> TS = stage_box<T>(S, act, q.jc, q.ic, act ? (int)ceilf(q.k) : 0, P.eta, P.xi, halo,
+                                              ldsBytes, lds, part, box);
         if (have) total += run_interval<T>(P, TS, sigma, id, ptime, q, tEnd);
     }
     if (have) {
@@ -1230,6 +1245,81 @@ void move_ens_lds_kernel(const EnsStep *__restrict__ steps, const int *__restric
         } else if (kStart != k0 || (float)q.health != h0) {
             st.k[idx] = kStart;
             st.health[idx] = (float)q.health;
+        }
+    }
+    if (stepCounter) wave_add_u64(stepCounter, total);
+}
+
+// The context's mixed-precision move (move_fast_kernel, plain packed
+// storage, no jibe, no event tracking) with the forcing staged in LDS as in
+// move_ens_lds_kernel: the interval runs in chunks of `chunk` substeps, and
+// before each chunk the workgroup stages the box of its particles that step
+// in it (stage_box).  Work-item t moves particle perm[t] (a cell-tile
+// order), so with the population sorted the box stays small.  Every value a
+// stencil reads is the one the plain kernel reads, and run_interval resumes
+// a particle exactly at a chunk boundary, so the results are identical.
+// Three waves per SIMD: three workgroups of kMoveLdsBytes share a CU.
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void move_fast_lds_kernel(MoveParams P, Packed<T> S, SoA st, int n, int chunk,
+                                                            int halo, int ldsBytes, unsigned long long *stepCounter,
+                                                            const uint32_t *__restrict__ perm) {
+    extern __shared__ __align__(32) unsigned char lds[];
+    __shared__ int part[4][6];
+    __shared__ int box[6];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const bool have = t < n;
+    const int idx = have ? (perm ? (int)perm[t] : t) : 0;
+    Reg q{};
+    q.done = true;
+    uint64_t id = 0;
+    double ptime = 0, kd = 0;
+    if (have) {
+        id = st.id[idx];
+        kd = st.k[idx];
+        double health = st.health[idx];
+        ptime = st.time[idx];
+        if (kd > 0) {                                              // Particle.cpp:134-176
+            if (P.upper == WC_CLOSURE_CONSTRAINT) kd = 0;
+            else if (P.upper == WC_CLOSURE_KILL) health = -1;
+            else if (P.upper == WC_CLOSURE_REFLECTION) kd = -kd;
+        }
+        if (kd < P.kLowerLimit) {
+            if (P.lower == WC_CLOSURE_CONSTRAINT) kd = P.kLowerLimit;
+            else if (P.lower == WC_CLOSURE_KILL) health = -1;
+            else if (P.lower == WC_CLOSURE_REFLECTION) kd = 2.0 * P.kLowerLimit - kd;
+        }
+        to_cell(st.j[idx], st.i[idx], kd, P.eta, P.xi, q);
+        q.k = (float)kd;
+        q.health = health;
+        q.age = st.age[idx];
+        q.healthFresh = true;
+        q.changed = false;
+        q.elapsed = P.elapsed0;
+        q.done = false;
+    }
+    const double L = P.intervalLength, span = (double)chunk * P.dti;
+    const int nChunks = L > P.elapsed0 ? (int)ceil((L - P.elapsed0) / span) : 0;
+    unsigned total = 0;
+    for (int ch = 0; ch < nChunks; ++ch) {
+        const double tEnd = ch + 1 == nChunks ? L : fmin(L, P.elapsed0 + (double)(ch + 1) * span);
+        const bool act = !q.done && q.inside && q.elapsed < tEnd && q.k >= -1e6f && q.k <= 1e6f &&
+                         (P.dirInt < 0 || ptime - P.intervalStart < tEnd);
+        const TilePacked<T> TS = stage_box<T>(S, act, q.jc, q.ic, act ? (int)ceilf(q.k) : 0, P.eta, P.xi, halo,
+                                              ldsBytes, lds, part, box);
+        if (have) total += run_interval<T>(P, TS, (float)P.sigma, id, ptime, q, tEnd);
+    }
+    if (have) {
+        if (!q.healthFresh) q.health = exp(-q.age / P.tau0);
+        if (q.changed) {
+            st.k[idx] = (double)q.k;
+            st.j[idx] = (double)q.jc + (double)q.fj;
+            st.i[idx] = (double)q.ic + (double)q.fi;
+            st.health[idx] = q.health;
+            st.age[idx] = q.age;
+        } else if (kd != st.k[idx] || q.health != st.health[idx]) {  // pre-step closures only
+            st.k[idx] = kd;
+            st.health[idx] = q.health;
         }
     }
     if (stepCounter) wave_add_u64(stepCounter, total);
@@ -1552,7 +1642,7 @@ hipError_t launch_tiled_t(const MoveParams &P, const RecordPair<float> &F, void 
 
 template <class T>
 hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packBuf, const GridPtrs &G, const SoA &s,
-                    int n, unsigned long long *ctr, bool events, const uint32_t *perm, int stages,
+                    int n, unsigned long long *ctr, bool events, const uint32_t *perm, int stages, const LdsTiling &lt,
                     hipStream_t stream) {
     const size_t plane = (size_t)P.eta * P.xi;
     const int nlev = P.s_w - 1;
@@ -1575,6 +1665,11 @@ hipError_t launch_t(const MoveParams &P, const RecordPair<float> &F, void *packB
         if (e != hipSuccess) return e;
     }
     if (!(stages & kStageMove) || n <= 0) return hipSuccess;
+    if (lt.chunk > 0 && !P.jibe && !events) {
+        move_fast_lds_kernel<T><<<(n + 255) / 256, 256, kMoveLdsBytes, stream>>>(P, S, s, n, lt.chunk, lt.halo,
+                                                                                 kMoveLdsBytes, ctr, perm);
+        return hipGetLastError();
+    }
     return launch_move<T>(P, S, s, n, ctr, events, perm, stream);
 }
 
@@ -1689,15 +1784,17 @@ size_t fast_pack_bytes(int store, int s_w, int eta, int xi) {
 
 hipError_t launch_move_fast_soa(int store, const MoveParams &P, const RecordPair<float> &F, void *packBuf,
                                 const GridPtrs &G, const SoA &s, int n, unsigned long long *ctr, bool events,
-                                const uint32_t *perm, int stages, hipStream_t stream) {
-    if (store == WC_STORE_F16) return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
-    if (store == WC_STORE_BF16) return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
+                                const uint32_t *perm, int stages, const LdsTiling &lt, hipStream_t stream) {
+    if (store == WC_STORE_F16)
+        return launch_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stages, lt, stream);
+    if (store == WC_STORE_BF16)
+        return launch_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stages, lt, stream);
     if (store == WC_STORE_F16_TILE)
         return launch_tiled_t<_Float16>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
     if (store == WC_STORE_BF16_TILE)
         return launch_tiled_t<uint16_t>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
     if (store == WC_STORE_I16_TILE) return launch_tiled_i16(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
-    return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, perm, stages, stream);
+    return launch_t<float>(P, F, packBuf, G, s, n, ctr, events, perm, stages, lt, stream);
 }
 
 }  // namespace wc

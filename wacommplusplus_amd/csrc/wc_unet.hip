@@ -213,6 +213,68 @@ constexpr int kCvOutF = 36;               // FP32 epilogue row: 32 channels padd
 constexpr int kStageBatch = 8;
 constexpr int kCvWRows = 9 * 32;          // staged weight rows per chunk
 
+// Epilogue through LDS: D[4 kq + r][row] of tile (i, j) is stage pixel
+// q = 16 NI wave + 16 i + 4 kq + r, channel co0 + 16 j + row.  The tile
+// [q][32 channels] is written to LDS (the caller has synchronised after the
+// last read of the stage), then stored with 16-byte accesses, 8 BF16 or 4
+// FP32 channels of one pixel per access.  The stage holds whole rows, so
+// stage pixel q is image pixel y0 w + q: stored while q < rv w (rows past the
+// image's last are not).  The tile stays within the input stage's part of
+// LDS.  Returns after the last LDS read.
+template <int NI>
+__device__ __forceinline__ void conv_epilogue(const ConvArgs &a, const f32x4 (&acc)[NI][2], uint16_t *smem, int rv,
+                                              int img, int y0, int co0, bool active) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, row = lane & 15, kq = lane >> 4;
+    const int W = a.w, nout = a.cy1 + a.cy2;
+    const int qmax = rv * W;                              // stage pixels inside the image
+    const long long p0 = (long long)img * a.h * W + (long long)y0 * W;
+    if (a.mode == 0) {                                    // forward: bias + ReLU, BF16
+        uint16_t *so = smem;                              // [q][kCvPS]
+        if (active)
+            for (int j = 0; j < 2; ++j) {
+                const int col = 16 * j + row;
+                const float b = co0 + col < nout ? a.bias[co0 + col] : 0.f;
+                for (int i = 0; i < NI; ++i)
+                    for (int r = 0; r < 4; ++r) {
+                        const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
+                        const float v = acc[i][j][r] + b;
+                        if (q < qmax) so[q * kCvPS + col] = to_bf16(v > 0.f ? v : 0.f);
+                    }
+            }
+        __syncthreads();
+        uint16_t *y = (uint16_t *)a.y1 + p0 * a.cy1 + co0;
+        for (int c = tid; c < qmax * 4; c += 256) {
+            const int q = c >> 2, part = c & 3;
+            if (co0 + 8 * part < nout) *(uint4 *)(y + (size_t)q * a.cy1 + 8 * part) = *(const uint4 *)(so + q * kCvPS + 8 * part);
+        }
+    } else {                                              // input gradient: FP32, stored or accumulated
+        float *so = (float *)smem;                        // [q][kCvOutF]
+        if (active)
+            for (int j = 0; j < 2; ++j)
+                for (int i = 0; i < NI; ++i)
+                    for (int r = 0; r < 4; ++r) {
+                        const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
+                        if (q < qmax) so[q * kCvOutF + 16 * j + row] = acc[i][j][r];
+                    }
+        __syncthreads();
+        const bool first = co0 < a.cy1;
+        const int cc = first ? a.cy1 : a.cy2, cl0 = first ? co0 : co0 - a.cy1;
+        const bool accum = first ? a.acc1 : a.acc2;
+        float *y = (float *)(first ? a.y1 : a.y2) + p0 * cc + cl0;
+        for (int c = tid; c < qmax * 8; c += 256) {
+            const int q = c >> 3, part = c & 7;
+            if (co0 + 4 * part >= nout) continue;
+            float4 v = *(const float4 *)(so + q * kCvOutF + 4 * part);
+            float4 *dst = (float4 *)(y + (size_t)q * cc + 4 * part);
+            if (accum) {
+                const float4 o = *dst;
+                v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+            }
+            *dst = v;
+        }
+    }
+}
+
 template <int NI, bool WL>
 __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, int rowsPerImg) {
     extern __shared__ __align__(16) uint16_t smem[];
@@ -302,60 +364,8 @@ __global__ __launch_bounds__(256) void conv3x3_tile_kernel(ConvArgs a, int R, in
                 }
             }
     }
-    // Epilogue through LDS: D[4 kq + r][row] of tile (i, j) is stage pixel
-    // q = 16 NI wave + 16 i + 4 kq + r, channel co0 + 16 j + row.  The tile
-    // [q][32 channels] is written to LDS (the stage is no longer read), then
-    // stored with 16-byte accesses, 8 BF16 or 4 FP32 channels of one pixel
-    // per access.  The stage holds whole rows, so stage pixel q is image pixel
-    // y0 w + q: stored while q < rv w (rows past the image's last are not).
-    const int qmax = rv * W;
-    const long long p0 = (long long)img * hw + (long long)y0 * W;
     __syncthreads();                                      // every wave is done with the stage
-    if (a.mode == 0) {                                    // forward: bias + ReLU, BF16
-        uint16_t *so = smem;                              // [RW][kCvPS]
-        if (active)
-            for (int j = 0; j < 2; ++j) {
-                const int col = 16 * j + row;
-                const float b = co0 + col < nout ? a.bias[co0 + col] : 0.f;
-                for (int i = 0; i < NI; ++i)
-                    for (int r = 0; r < 4; ++r) {
-                        const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
-                        const float v = acc[i][j][r] + b;
-                        if (q < RW) so[q * kCvPS + col] = to_bf16(v > 0.f ? v : 0.f);
-                    }
-            }
-        __syncthreads();
-        uint16_t *y = (uint16_t *)a.y1 + p0 * a.cy1 + co0;
-        for (int c = tid; c < qmax * 4; c += 256) {
-            const int q = c >> 2, part = c & 3;
-            if (co0 + 8 * part < nout) *(uint4 *)(y + (size_t)q * a.cy1 + 8 * part) = *(const uint4 *)(so + q * kCvPS + 8 * part);
-        }
-    } else {                                              // input gradient: FP32, stored or accumulated
-        float *so = (float *)smem;                        // [RW][kCvOutF]
-        if (active)
-            for (int j = 0; j < 2; ++j)
-                for (int i = 0; i < NI; ++i)
-                    for (int r = 0; r < 4; ++r) {
-                        const int q = 16 * NI * wave + 16 * i + 4 * kq + r;
-                        if (q < RW) so[q * kCvOutF + 16 * j + row] = acc[i][j][r];
-                    }
-        __syncthreads();
-        const bool first = co0 < a.cy1;
-        const int cc = first ? a.cy1 : a.cy2, cl0 = first ? co0 : co0 - a.cy1;
-        const bool accum = first ? a.acc1 : a.acc2;
-        float *y = (float *)(first ? a.y1 : a.y2) + p0 * cc + cl0;
-        for (int c = tid; c < qmax * 8; c += 256) {
-            const int q = c >> 3, part = c & 7;
-            if (co0 + 4 * part >= nout) continue;
-            float4 v = *(const float4 *)(so + q * kCvOutF + 4 * part);
-            float4 *dst = (float4 *)(y + (size_t)q * cc + 4 * part);
-            if (accum) {
-                const float4 o = *dst;
-                v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
-            }
-            *dst = v;
-        }
-    }
+    conv_epilogue<NI>(a, acc, smem, rv, img, y0, co0, active);
 }
 
 // dW[co][tap][ci] += sum_p dY[p][co] * (Y[p][co] > 0) * X[p + tap][ci];
