@@ -15,7 +15,8 @@ NAMES = {"f64": "void wc::(anonymous namespace)::move_parity_kernel<wc::(anonymo
          "i16_tile": "void wc::(anonymous namespace)::move_fast_kernel<short, wc::(anonymous namespace)::TiledI16, "
                      "false, false>(wc::MoveParams)",
          "bf16": "void wc::(anonymous namespace)::move_fast_kernel<unsigned short, wc::(anonymous namespace)::"
-                 "Packed<unsigned short>, false, false>(wc::MoveParams)"}
+                 "Packed<unsigned short>, false, false>(wc::MoveParams)",
+         "bf16_lds": "void wc::(anonymous namespace)::move_fast_lds_kernel<unsigned short>(wc::MoveParams)"}
 
 
 def write_trace(path, seq):
@@ -40,9 +41,10 @@ def test_bench_trace_assigns_runs(tmp_path):
     plan = [("f64", "f64_unsorted", True), ("f64", "tune", False), ("f64", "f64", True), ("f64", "f64_euler", True),
             ("i16_tile", "i16_tile_unsorted", True), ("i16_tile", "tune", False),
             ("i16_tile", "i16_tile_euler", True),
-            ("bf16", "bf16_unsorted", True), ("bf16", "tune", False), ("bf16", "bf16", True)]
+            ("bf16", "bf16_unsorted", True), ("bf16", "tune", False), ("bf16", "bf16", True),
+            ("bf16_lds", "bf16_lds", True)]
     ms = {"f64_unsorted": 25.0, "f64": 24.0, "f64_euler": 19.0, "i16_tile_unsorted": 13.0, "i16_tile_euler": 8.0,
-          "bf16_unsorted": 11.0, "bf16": 9.0}
+          "bf16_unsorted": 11.0, "bf16": 9.0, "bf16_lds": 16.0}
     seq = []
     for v, key, timed in plan:
         if key == "tune":
@@ -56,7 +58,8 @@ def test_bench_trace_assigns_runs(tmp_path):
         "i16_tile_unsorted": dict(move_kernel_ms=13.0, sort_tile=0),
         "i16_tile": dict(move_kernel_ms=13.0, sort_tile=0),
         "i16_tile_euler": dict(move_kernel_ms=8.0, sort_tile=0),
-        "bf16_unsorted": dict(move_kernel_ms=11.0, sort_tile=0), "bf16": dict(move_kernel_ms=9.0, sort_tile=1)}
+        "bf16_unsorted": dict(move_kernel_ms=11.0, sort_tile=0), "bf16": dict(move_kernel_ms=9.0, sort_tile=1),
+        "bf16_lds": dict(move_kernel_ms=16.0, sort_tile=1)}
     line = {"steps": steps, "warmup": warmup, "value": 1.0, "variants": variants,
             "config": {"cell_tile_sort": 1, "sort_autotuned": True},
             "roofline": {"avg_launch_ms": 24.0, "frac": 0.5, "counters": {"key": "f64@t1"}}}

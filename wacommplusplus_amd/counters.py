@@ -33,7 +33,7 @@ import os
 import sys
 from collections import OrderedDict, defaultdict
 
-MOVE_KERNELS = ("move_parity_kernel", "move_fast_kernel")
+MOVE_KERNELS = ("move_parity_kernel", "move_fast_kernel", "move_fast_lds_kernel")
 
 
 def copy_probe(mib: int, reps: int) -> None:
@@ -269,7 +269,7 @@ def bench_trace(a) -> None:
     steps, warmup = line["steps"], line["warmup"]
     v = line["variants"]
     runs = []
-    bases = [k for k in v if not k.endswith(("_unsorted", "_euler"))]       # the variants, in run order
+    bases = [k for k in v if not k.endswith(("_unsorted", "_euler", "_lds"))]       # the variants, in run order
     for key in ["f64"] + bases:
         tile = line["config"]["cell_tile_sort"] if key == "f64" else v[key]["sort_tile"]
         runs.append(dict(key=key + "_unsorted", variant=key, warmup_launches=warmup, timed_launches=steps))
@@ -280,6 +280,8 @@ def bench_trace(a) -> None:
             runs.append(dict(key=key, variant=key, warmup_launches=warmup, timed_launches=steps))
         if key + "_euler" in v:
             runs.append(dict(key=key + "_euler", variant=key, warmup_launches=warmup, timed_launches=steps))
+        if key + "_lds" in v:            # the LDS-staged move: a kernel symbol of its own
+            runs.append(dict(key=key + "_lds", variant=key + "_lds", warmup_launches=warmup, timed_launches=steps))
     disp = OrderedDict()
     with open(a.moves) as f:
         rd = csv.reader(f)
