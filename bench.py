@@ -389,6 +389,7 @@ def precision_vs_f64(ctx, W, p, steps, cfg):
 
 
 EULER_VARIANTS = ("f64", "i16_tile", "bf16")
+LDS_TILING = (30, 2)      # chunk of substeps per staged box, halo cells
 
 
 def profile_runs(ctx, W, cfg, p0, variants, tiles, args):
@@ -425,6 +426,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-euler", action="store_true", help="skip the Euler-integrator timings")
     ap.add_argument("--no-sort", action="store_true", help="no cell-tile sort (skip its autotuning)")
+    ap.add_argument("--no-lds", action="store_true", help="skip the LDS-staged BF16 move")
     ap.add_argument("--no-pcie", action="store_true", help="skip the upload-inclusive rate")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-reconstruction workload")
@@ -478,6 +480,13 @@ def main():
                           progress=key + " euler", tile=tile)
             results[key + "_euler"] = e
             log(f"[{key} euler] {e}")
+        if key == "bf16" and not args.no_lds:
+            # the same move with the forcing staged in LDS (wc_set_lds), at the autotuned cell-tile order
+            ctx.set_lds(*LDS_TILING)
+            e = run_timed(ctx, W, cfg, p0, args.steps, args.warmup, path, progress=key + " lds", tile=tile)
+            ctx.set_lds(0)
+            results[key + "_lds"] = e
+            log(f"[{key} lds {LDS_TILING}] {e}")
 
     precision = None
     if not args.no_variants and not args.no_precision:
