@@ -13,12 +13,14 @@
 //   - the checks the reference makes before it normalises (:152-166, 290-298):
 //     a wet u or v face holding a non-finite or missing value, or a rotated
 //     velocity outside float range, sets a bit of *flag.
-// One work-item per (level, rho point): s_w x plane work-items keep enough
-// loads in flight to stream HBM (a column-per-work-item loop over the levels
-// ran at ~30 % of peak), each issuing all its loads before the first use
-// (one memory round trip instead of three: 58 us per 1024x768x11 record,
-// against 70 us).  An XCD-aware mapping that sweeps the levels of a tile in
-// one L2 measured slower (65 us): cache reuse is not what bounds it.  The three masks are folded at set-up into one byte
+// One work-item per rho point and 4 of its levels: it reads the point's
+// mask byte and rotation once for the 4 levels, and issues all its loads
+// before the first use (one memory round trip).  Per 1024x768x11 record:
+// 55 us, against 63 us with one level per work-item (70 us with its loads
+// not hoisted) and 58 us with all 12 levels per work-item (a loop over the
+// column without hoisted loads ran at ~30 % of peak).  An XCD-aware mapping
+// that swept the levels of a tile in one L2 measured slower (65 us).  The
+// three masks are folded at set-up into one byte
 // per rho point (kFace* bits: the point is wet, and which of its four faces
 // are wet), so a work-item reads 1 B of mask state.  Consecutive work-items
 // take consecutive i, so every load and store coalesces.  The u/v grids are
@@ -38,6 +40,7 @@ namespace wc {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kLevelsPerItem = 4;     // levels of one rho point per work-item
 
 __device__ __forceinline__ bool valid_value(float value, const double *missing, int n) {
     const double x = value;
@@ -97,35 +100,44 @@ __global__ __launch_bounds__(kBlock) void roms_normalize_kernel(RomsDims d, Roms
     const size_t plane = (size_t)d.eta * d.xi;
     const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= plane) return;
-    const int k = blockIdx.y;                   // storage level, 0 = bottom
+    const int k0 = blockIdx.y * kLevelsPerItem;    // storage levels k0 .. k0 + kLevelsPerItem - 1, 0 = bottom
     const int j = (int)(c / d.xi), i = (int)(c - (size_t)j * d.xi);
-    const size_t o = (size_t)k * plane + c;
-    const bool rho_level = k < d.s_rho;
 
     // Every load is issued before the first use, guarded only by index
     // bounds (no memory dependence), so a work-item waits for one memory
     // round trip; the face bits select what the arithmetic uses.
-    const bool in_u = rho_level && j < d.eta_u && i < d.xi_u;
-    const bool in_uw = rho_level && j < d.eta_u && i > 0 && i - 1 < d.xi_u;
-    const bool in_v = rho_level && j < d.eta_v && i < d.xi_v;
-    const bool in_vs = rho_level && j > 0 && j - 1 < d.eta_v && i < d.xi_v;
-    const float *uk = u + ((size_t)k * d.eta_u + j) * d.xi_u + i;     // face (j, i) of level k
-    const float *vk = v + ((size_t)k * d.eta_v + j) * d.xi_v + i;
+    const bool in_u = j < d.eta_u && i < d.xi_u;
+    const bool in_uw = j < d.eta_u && i > 0 && i - 1 < d.xi_u;
+    const bool in_v = j < d.eta_v && i < d.xi_v;
+    const bool in_vs = j > 0 && j - 1 < d.eta_v && i < d.xi_v;
     const uint32_t f = face[c];
-    const double2 cs = (rho_level && !d.earth) ? rot[c] : make_double2(1.0, 0.0);   // {cos, sin}; 1, 0 on land
-    const float u0 = in_u ? uk[0] : 0.0f;
-    const float uw = in_uw ? uk[-1] : 0.0f;
-    const float v0 = in_v ? vk[0] : 0.0f;
-    const float vs = in_vs ? vk[-(ptrdiff_t)d.xi_v] : 0.0f;
-    const float z = k == 0 ? zeta[c] : 0.0f;
-    const bool wet = f & kFaceWet;
-
-    if (!wet) {
-        W[o] = 0.0f;
-        AKT[o] = 0.0f;
+    const double2 cs = (k0 < d.s_rho && !d.earth) ? rot[c] : make_double2(1.0, 0.0);   // {cos, sin}; 1, 0 on land
+    float u0[kLevelsPerItem], uw[kLevelsPerItem], v0[kLevelsPerItem], vs[kLevelsPerItem];
+#pragma unroll
+    for (int q = 0; q < kLevelsPerItem; ++q) {
+        const int k = k0 + q;
+        const bool rho_level = k < d.s_rho;
+        const float *uk = u + ((size_t)k * d.eta_u + j) * d.xi_u + i;     // face (j, i) of level k
+        const float *vk = v + ((size_t)k * d.eta_v + j) * d.xi_v + i;
+        u0[q] = rho_level && in_u ? uk[0] : 0.0f;
+        uw[q] = rho_level && in_uw ? uk[-1] : 0.0f;
+        v0[q] = rho_level && in_v ? vk[0] : 0.0f;
+        vs[q] = rho_level && in_vs ? vk[-(ptrdiff_t)d.xi_v] : 0.0f;
     }
-    if (rho_level) normalize_uv(d, miss, f, cs, u0, uw, v0, vs, U + o, V + o, flag);
-    if (k == 0 && (!wet || z >= 1e+36f)) zeta[c] = 0.0f;
+    const float z = k0 == 0 ? zeta[c] : 0.0f;
+    const bool wet = f & kFaceWet;
+#pragma unroll
+    for (int q = 0; q < kLevelsPerItem; ++q) {
+        const int k = k0 + q;
+        if (k > d.s_rho) break;                 // W and AKT have s_rho + 1 levels
+        const size_t o = (size_t)k * plane + c;
+        if (!wet) {
+            W[o] = 0.0f;
+            AKT[o] = 0.0f;
+        }
+        if (k < d.s_rho) normalize_uv(d, miss, f, cs, u0[q], uw[q], v0[q], vs[q], U + o, V + o, flag);
+    }
+    if (k0 == 0 && (!wet || z >= 1e+36f)) zeta[c] = 0.0f;
 }
 
 // The duplicate-boundary check of OceanModelAdapter::appendBoundaryRecord
@@ -176,7 +188,8 @@ hipError_t launch_roms_normalize(const RomsDims &d, const RomsMissing &miss, con
                                  float *AKT, float *zeta, uint32_t *flag, hipStream_t stream) {
     const size_t plane = (size_t)d.eta * d.xi;
     if (plane == 0) return hipSuccess;
-    const dim3 grid((unsigned)((plane + kBlock - 1) / kBlock), (unsigned)(d.s_rho + 1));
+    const dim3 grid((unsigned)((plane + kBlock - 1) / kBlock),
+                    (unsigned)((d.s_rho + 1 + kLevelsPerItem - 1) / kLevelsPerItem));
     roms_normalize_kernel<<<grid, kBlock, 0, stream>>>(d, miss, face, rot, u, v, U, V, W, AKT, zeta, flag);
     return hipGetLastError();
 }
