@@ -172,6 +172,65 @@ def roms_records(W, t, mask_u, mask_v):
     return zeta, uf, vf, w, akt
 
 
+def dropin_bench(wc, W, cfg, p0, steps):
+    """The literal cudaMoveParticle replacement, wc_hip_move_particles
+    (kernel.cu:370-382): particle_data AoS (64 B) and the whole window in the
+    reference's device layout (zeta [T][eta][xi], u/v [T][s_rho][eta][xi],
+    w/akt [T][s_w][eta][xi], FP32), one call per interval as Wacomm::run makes
+    it; the FP64 parity kernel over the unpacked layout.  Particle-steps are
+    recovered from the ages (every particle is released at t0 and a live
+    substep ages it by dti; the substep that kills a particle does not, so
+    the count is a few in 1e5 below the move's own count)."""
+    import ctypes as C
+    L = wc.load_library()
+    hip = C.CDLL("libamdhip64.so.7")               # the runtime the library already loaded
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    bufs = []
+
+    def dev(a):
+        a = np.ascontiguousarray(a)
+        ptr = C.c_void_p()
+        assert hip.hipMalloc(C.byref(ptr), max(1, a.nbytes)) == 0
+        assert hip.hipMemcpy(ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 1) == 0
+        bufs.append(ptr)
+        return ptr
+
+    try:
+        zeta, u, v, w, akt = W.stacked()
+        f64 = lambda a: np.asarray(a, np.float64)
+        ptrs = [dev(f64(W.times)), dev(f64(W.mask)), dev(f64(W.lonRad)), dev(f64(W.latRad)),
+                dev(f64(W.depthIntervals)), dev(f64(W.h))] + [dev(a.astype(np.float32)) for a in (zeta, u, v, w, akt)]
+        del zeta, u, v, w, akt
+        p = np.ascontiguousarray(p0.astype(wc.PARTICLE_DTYPE))
+        pd, cd = dev(p), dev(np.frombuffer(bytes(cfg), np.uint8))
+
+        def call(t):
+            rc = L.wc_hip_move_particles(cd, pd, t, W.T, W.s_w, W.s_rho, W.eta, W.xi, *ptrs, None, None, None, None,
+                                         len(p), 1, 1, None)
+            assert rc == wc.WC_OK, rc
+
+        call(0)                                        # warm-up: first call builds the metric table
+        assert hip.hipMemcpy(pd, p.ctypes.data_as(C.c_void_p), p.nbytes, 1) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        t0 = time.perf_counter()
+        for t in range(steps):
+            call(t)
+        assert hip.hipDeviceSynchronize() == 0
+        wall = time.perf_counter() - t0
+        out = np.empty_like(p)
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), pd, out.nbytes, 2) == 0
+        n_steps = int(np.rint(np.sum(out["age"] - p["age"]) / cfg.dti))
+        return dict(value=n_steps / wall, unit="particle-steps/s", ms_per_interval=1e3 * wall / steps,
+                    particle_steps=n_steps, intervals=steps, particles=len(p),
+                    what="wc_hip_move_particles (cudaMoveParticle's argument list): AoS particle_data and the "
+                         "window in the reference's device layout, one call per interval, FP64 parity kernel")
+    finally:
+        for b in bufs:
+            hip.hipFree(b)
+
+
 def roms_normalize_bench(wc, ctx, W, n_records):
     """ROMS normalisation of full-size records (1024x768, 11 levels) into
     forcing slots: device kernel time per record and the algorithmic bytes
@@ -427,6 +486,7 @@ def main():
     ap.add_argument("--no-euler", action="store_true", help="skip the Euler-integrator timings")
     ap.add_argument("--no-sort", action="store_true", help="no cell-tile sort (skip its autotuning)")
     ap.add_argument("--no-lds", action="store_true", help="skip the LDS-staged BF16 move")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in entry point's rate")
     ap.add_argument("--no-pcie", action="store_true", help="skip the upload-inclusive rate")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-reconstruction workload")
@@ -515,6 +575,11 @@ def main():
         log(f"[pcie] {pcie:.4g} particle-steps/s")
     roms = roms_normalize_bench(wc, ctx, W, args.steps)
     log(f"[roms] {roms}")
+    dropin = None
+    if not args.no_dropin:
+        ctx.set_particles(wc.make_particles(0))            # release the context's copy first
+        dropin = dropin_bench(wc, W, cfg, p0, args.steps)
+        log(f"[drop-in] {dropin}")
     ens = None if args.no_ensemble else ensemble_bench(wc, ctx, W, cfg, args.steps)
     learned = None
     if not args.no_learned:
@@ -585,6 +650,7 @@ def main():
                              "for throughput (variants.i16_tile, variants.bf16)",
         "precision_vs_f64": precision,
         "pcie_inclusive_value": pcie,
+        "dropin": dropin,
         "roms_normalize": roms,
         "ensemble": ens,
         "learned_reconstruction": learned,
