@@ -177,7 +177,8 @@ def dropin_bench(wc, W, cfg, p0, steps):
     (kernel.cu:370-382): particle_data AoS (64 B) and the whole window in the
     reference's device layout (zeta [T][eta][xi], u/v [T][s_rho][eta][xi],
     w/akt [T][s_w][eta][xi], FP32), one call per interval as Wacomm::run makes
-    it; the FP64 parity kernel over the unpacked layout.  Particle-steps are
+    it (each call packs the interval's record pair, then runs the FP64 parity
+    kernel over the pack).  Particle-steps are
     recovered from the ages (every particle is released at t0 and a live
     substep ages it by dti; the substep that kills a particle does not, so
     the count is a few in 1e5 below the move's own count)."""
@@ -225,7 +226,8 @@ def dropin_bench(wc, W, cfg, p0, steps):
         return dict(value=n_steps / wall, unit="particle-steps/s", ms_per_interval=1e3 * wall / steps,
                     particle_steps=n_steps, intervals=steps, particles=len(p),
                     what="wc_hip_move_particles (cudaMoveParticle's argument list): AoS particle_data and the "
-                         "window in the reference's device layout, one call per interval, FP64 parity kernel")
+                         "window in the reference's device layout, one call per interval (pack of the record "
+                         "pair, then the FP64 parity kernel)")
     finally:
         for b in bufs:
             hip.hipFree(b)

@@ -422,10 +422,38 @@ struct DropinMetric {
     int eta = 0, xi = 0;
     unsigned long long hash = 0;
     bool valid = false;
+    // the interval pack of the drop-in move, rewritten by every call: a call
+    // on any stream waits (on the device) for the previous call's move
+    void *pack = nullptr;
+    size_t packBytes = 0;
+    hipEvent_t packFree = nullptr;
 };
 constexpr int kDropinDevices = 64;
 std::mutex dropinLock;
 DropinMetric dropinMetric[kDropinDevices];
+
+// The current device's drop-in pack buffer, at least parity_pack_bytes(s_w,
+// plane) bytes, after the previous user's move on any stream (the caller
+// records packFree after its own launches).  Caller holds dropinLock.
+int dropin_pack(int s_w, size_t plane, hipStream_t st, void **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return WC_ERR_HIP;
+    if (dev < 0 || dev >= kDropinDevices) return WC_ERR_INVALID;
+    DropinMetric &M = dropinMetric[dev];
+    if (!M.packFree && hipEventCreateWithFlags(&M.packFree, hipEventDisableTiming) != hipSuccess) return WC_ERR_HIP;
+    const size_t need = parity_pack_bytes(s_w, plane);
+    if (M.packBytes < need) {
+        if (hipEventSynchronize(M.packFree) != hipSuccess) return WC_ERR_HIP;   // no launch reads the old buffer
+        hipFree(M.pack);
+        M.pack = nullptr;
+        M.packBytes = 0;
+        if (hipMalloc(&M.pack, need) != hipSuccess) return WC_ERR_NOMEM;
+        M.packBytes = need;
+    }
+    if (hipStreamWaitEvent(st, M.packFree, 0) != hipSuccess) return WC_ERR_HIP;
+    *out = M.pack;
+    return WC_OK;
+}
 
 // Caller holds dropinLock.
 int dropin_metric(int eta, int xi, const double *lon, const double *lat, hipStream_t st, const double2 **out) {
@@ -546,8 +574,15 @@ int wc_hip_move_particles(wc_config_data *config, wc_particle_data *particles, i
                         w + plane * sW * timeIndex, w + plane * sW * next,
                         akt + plane * sW * timeIndex, akt + plane * sW * next};
     GridPtrs G{h, depthIntervals, nullptr, nullptr, lonRad, latRad, nullptr, nullptr, nullptr, nullptr};
-    return launch_move_parity_aos(P, F, G, metric, particles, particleCount, nullptr, st) == hipSuccess ? WC_OK
-                                                                                                        : WC_ERR_HIP;
+    void *pack = nullptr;
+    rc = dropin_pack(sW, plane, st, &pack);
+    if (rc) return rc;
+    if (launch_move_parity_aos(P, F, G, metric, pack, particles, particleCount, nullptr, st) != hipSuccess)
+        return WC_ERR_HIP;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipEventRecord(dropinMetric[dev].packFree, st) != hipSuccess)
+        return WC_ERR_HIP;
+    return WC_OK;
 }
 
 int wc_hip_concentration(const wc_particle_data *particles, int particleCount, int s_rho, int eta, int xi,

@@ -20,9 +20,11 @@ hipError_t launch_move_parity_soa(const MoveParams &P, const RecordPair<float> &
                                   const double2 *metric, const SoA &s, int n, unsigned long long *stepCounter,
                                   bool events, const uint32_t *perm, int stages, hipStream_t stream);
 // Drop-in path: the reference's array layout; metric = {dx, dy} table.
+// With packBuf (parity_pack_bytes) the record pair is packed first and the
+// move gathers from the pack; without, it gathers from the raw records.
 hipError_t launch_move_parity_aos(const MoveParams &P, const RecordPair<float> &F, const GridPtrs &G,
-                                  const double2 *metric, wc_particle_data *p, int n, unsigned long long *stepCounter,
-                                  hipStream_t stream);
+                                  const double2 *metric, void *packBuf, wc_particle_data *p, int n,
+                                  unsigned long long *stepCounter, hipStream_t stream);
 // *hash += a content hash of lon[0..n) and lat[0..n) (zero it first).
 hipError_t launch_grid_hash(const double *lon, const double *lat, size_t n, unsigned long long *hash,
                             hipStream_t stream);
