@@ -996,36 +996,53 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
     return hipGetLastError();
 }
 
+// The tiled weight gradient's shape for these arguments: stage rows R,
+// stages, partial slices zn and slice length E (dW, then the bias gradient);
+// false when the tiled path does not apply (the split-K path runs).
+struct WgradTiling {
+    int R, rowsPerImg, stages, zn;
+    size_t lds, E;
+};
+static bool wgrad_tiling(const WgradArgs &a, WgradTiling &t) {
+    const int cin = a.c1 + a.c2;
+    if (!a.part || a.w % 8 || a.w > kWgPix) return false;           // widths that are a multiple of 8
+    t.R = kWgPix / a.w < a.h ? kWgPix / a.w : a.h;
+    t.lds = (size_t)(32 * (t.R * a.w + 8) + 32 * ((t.R + 2) * (a.w + 16) + 8)) * sizeof(uint16_t);
+    t.rowsPerImg = (a.h + t.R - 1) / t.R;
+    t.stages = a.n * t.rowsPerImg;
+    const int blocks = (a.cout / 32) * (cin / 32);
+    t.zn = 512 / blocks;
+    if (t.zn > t.stages) t.zn = t.stages;
+    if (t.zn < 1) t.zn = 1;
+    t.E = (size_t)a.cout * 9 * cin + (a.db ? a.cout : 0);
+    return t.lds <= 64 * 1024 && t.E * t.zn <= a.partCap;             // the stage fits, and the partials
+}
+
+bool wgrad_is_tiled(const WgradArgs &a) {
+    WgradTiling t;
+    return wgrad_tiling(a, t);
+}
+
 hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
     const int cin = a.c1 + a.c2;
     if ((cin % 32) || (a.cout % 32) || a.splits < 1) return hipErrorInvalidValue;
-    // tiled path: widths that are a multiple of 8, whose stage fits in LDS
-    if (a.part && a.w % 8 == 0 && a.w <= kWgPix) {
-        const int R = kWgPix / a.w < a.h ? kWgPix / a.w : a.h;
-        const size_t lds = (size_t)(32 * (R * a.w + 8) + 32 * ((R + 2) * (a.w + 16) + 8)) * sizeof(uint16_t);
-        const int rowsPerImg = (a.h + R - 1) / R, stages = a.n * rowsPerImg;
-        const int blocks = (a.cout / 32) * (cin / 32);
-        int zn = 512 / blocks;
-        if (zn > stages) zn = stages;
-        if (zn < 1) zn = 1;
-        // a slice of partials: dW, then the bias gradient (a.db must be a.dw + E)
-        const size_t E = (size_t)a.cout * 9 * cin + (a.db ? a.cout : 0);
-        if (a.db && a.db != a.dw + (E - a.cout)) return hipErrorInvalidValue;
-        if (lds <= 64 * 1024 && E * zn <= a.partCap) {
-            const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)zn);
-            wgrad_tile_kernel<<<grid, 256, lds, s>>>(a, R, rowsPerImg, stages);
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) return e;
-            int step = 1;
-            if (zn > kWgChunk) {        // sum chunks of kWgChunk slices first
-                const dim3 g1((unsigned)((E + 255) / 256), (unsigned)((zn + kWgChunk - 1) / kWgChunk));
-                wgrad_reduce_chunks_kernel<<<g1, 256, 0, s>>>(a.part, zn, E);
-                if ((e = hipGetLastError()) != hipSuccess) return e;
-                step = kWgChunk;
-            }
-            wgrad_reduce_kernel<<<blocks_for((long long)E), 256, 0, s>>>(a.part, zn, step, E, a.dw);
-            return hipGetLastError();
+    WgradTiling t;
+    if (wgrad_tiling(a, t)) {
+        // a slice of partials: dW, then the bias gradient (a.db must be a.dw + cout 9 cin)
+        if (a.db && a.db != a.dw + (t.E - a.cout)) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)t.zn);
+        wgrad_tile_kernel<<<grid, 256, t.lds, s>>>(a, t.R, t.rowsPerImg, t.stages);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        int step = 1;
+        if (t.zn > kWgChunk) {        // sum chunks of kWgChunk slices first
+            const dim3 g1((unsigned)((t.E + 255) / 256), (unsigned)((t.zn + kWgChunk - 1) / kWgChunk));
+            wgrad_reduce_chunks_kernel<<<g1, 256, 0, s>>>(a.part, t.zn, t.E);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            step = kWgChunk;
         }
+        wgrad_reduce_kernel<<<blocks_for((long long)t.E), 256, 0, s>>>(a.part, t.zn, step, t.E, a.dw);
+        return hipGetLastError();
     }
     const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)(9 * a.splits));
     wgrad_kernel<<<grid, 256, 0, s>>>(a);
@@ -1034,20 +1051,6 @@ hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
     if (!a.dyb || !a.bpart || a.bnb < 1) return hipErrorInvalidValue;
     bias_reduce_kernel<<<a.cout, 256, 0, s>>>(a.bpart, a.bnb, a.cout, a.db);
     return hipGetLastError();
-}
-
-bool wgrad_is_tiled(const WgradArgs &a) {
-    const int cin = a.c1 + a.c2;
-    if (!a.part || a.w % 8 || a.w > kWgPix) return false;
-    const int R = kWgPix / a.w < a.h ? kWgPix / a.w : a.h;
-    const size_t lds = (size_t)(32 * (R * a.w + 8) + 32 * ((R + 2) * (a.w + 16) + 8)) * sizeof(uint16_t);
-    const int rowsPerImg = (a.h + R - 1) / R, stages = a.n * rowsPerImg;
-    const int blocks = (a.cout / 32) * (cin / 32);
-    int zn = 512 / blocks;
-    if (zn > stages) zn = stages;
-    if (zn < 1) zn = 1;
-    const size_t E = (size_t)a.cout * 9 * cin + (a.db ? a.cout : 0);
-    return lds <= 64 * 1024 && E * zn <= a.partCap;
 }
 
 // E = pixels x cout; cout a multiple of 4 that divides 1024; part holds
