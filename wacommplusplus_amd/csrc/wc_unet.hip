@@ -463,6 +463,26 @@ __device__ __forceinline__ bf16x8 window(const uint4 &lo, const uint4 &mid, cons
     return r.v;
 }
 
+// Transpose of an 8 x 8 block of 16-bit values: in[j] holds 8 channels of
+// pixel j; dst + q plane receives channel q's 8 pixels as one 16-byte write.
+__device__ __forceinline__ void transpose8_store(const uint4 (&in)[8], uint16_t *dst, int plane) {
+    uint32_t w[8][4];
+    for (int j = 0; j < 8; ++j) {
+        w[j][0] = in[j].x;
+        w[j][1] = in[j].y;
+        w[j][2] = in[j].z;
+        w[j][3] = in[j].w;
+    }
+    for (int q = 0; q < 8; ++q) {
+        uint32_t o[4];
+        for (int k = 0; k < 4; ++k) {          // pixels 2k, 2k + 1 of channel q
+            const uint32_t lo = w[2 * k][q >> 1], hi = w[2 * k + 1][q >> 1];
+            o[k] = (q & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+        }
+        *(uint4 *)(dst + (size_t)q * plane) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+}
+
 __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int rowsPerImg, int stages) {
     extern __shared__ __align__(16) uint16_t smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -487,43 +507,36 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
         const int img = st / rowsPerImg, y0 = (st - img * rowsPerImg) * R;
         const int rv = min(R, H - y0);                 // rows of this stage inside the image
         __syncthreads();                               // the previous stage's reads are done
-        // masked dY in BF16: 8 channels (16 B) of one pixel per work-item;
-        // pixel pix = (e >> 2) advances by 64 per step, its row r and column
-        // x by counting (no division in the loop)
-        int r = (tid >> 2) / W, x0 = (tid >> 2) - r * W;
-        for (int e = tid; e < RW * 4; e += 256) {
-            const int pix = e >> 2, c8 = (e & 3) * 8;
-            Pack8 v;
-            v.u = make_uint4(0, 0, 0, 0);
+        // 8 pixels x 8 channels per work-item: eight 16-byte loads (one per
+        // pixel), a register transpose, eight 16-byte LDS writes (one per
+        // channel).  W and the padded row rowX are multiples of 8, so an
+        // octet never crosses a row and lies wholly inside or outside the
+        // image.  dY: RW / 8 octets x 4 channel groups.
+        for (int e = tid; e < RW / 2; e += 256) {
+            const int c8 = (e & 3) * 8, p0 = (e >> 2) * 8;
+            const int r = p0 / W, x = p0 - r * W;
+            uint4 in[8];
             if (r < rv) {
-                const size_t o = ((size_t)img * hw + (size_t)(y0 + r) * W + x0) * a.cout + co0 + c8;
-                if (a.dyb) {
-                    v.u = *(const uint4 *)(a.dyb + o);
-                } else {
-                    const float4 d0 = *(const float4 *)(a.dy + o), d1 = *(const float4 *)(a.dy + o + 4);
-                    const uint4 m = *(const uint4 *)(a.y + o);
-                    const float d[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-                    const uint32_t mw[4] = {m.x, m.y, m.z, m.w};
-                    for (int q = 0; q < 8; ++q)
-                        v.h[q] = to_bf16(from_bf16((uint16_t)(mw[q >> 1] >> (16 * (q & 1)))) > 0.f ? d[q] : 0.f);
-                }
+                const uint16_t *src = a.dyb + ((size_t)img * hw + (size_t)(y0 + r) * W + x) * a.cout + co0 + c8;
+                for (int j = 0; j < 8; ++j) in[j] = *(const uint4 *)(src + (size_t)j * a.cout);
+            } else {
+                for (int j = 0; j < 8; ++j) in[j] = make_uint4(0, 0, 0, 0);   // rows beyond the image
             }
-            for (int q = 0; q < 8; ++q) sdy[(c8 + q) * planeD + pix] = v.h[q];
-            for (x0 += 64; x0 >= W; x0 -= W) ++r;
+            transpose8_store(in, sdy + c8 * planeD + p0, planeD);
         }
-        // X rows y0-1 .. y0+R, padded: 8 channels (16 B) of one position per work-item
-        const int positions = (R + 2) * rowX;
-        int rr = (tid >> 2) / rowX, xp = (tid >> 2) - rr * rowX;     // position (rr, xp) of e >> 2
-        for (int e = tid; e < positions * 4; e += 256) {
-            const int pos = e >> 2, c8 = (e & 3) * 8;
-            const int xx = xp - 8;
-            const int y = y0 - 1 + rr;
-            Pack8 v;
-            v.u = make_uint4(0, 0, 0, 0);
-            if (y >= 0 && y < H && xx >= 0 && xx < W)
-                v.u = *(const uint4 *)(xs + ((size_t)img * hw + (size_t)y * W + xx) * cc + cl0 + c8);
-            for (int q = 0; q < 8; ++q) sx[(c8 + q) * planeX + pos] = v.h[q];
-            for (xp += 64; xp >= rowX; xp -= rowX) ++rr;
+        // X rows y0-1 .. y0+R with 8 zero columns either side: (R + 2) rowX / 8 octets
+        const int octX = (R + 2) * rowX / 8;
+        for (int e = tid; e < octX * 4; e += 256) {
+            const int c8 = (e & 3) * 8, q0 = (e >> 2) * 8;
+            const int rr = q0 / rowX, xx = q0 - rr * rowX - 8, y = y0 - 1 + rr;
+            uint4 in[8];
+            if (y >= 0 && y < H && xx >= 0 && xx < W) {
+                const uint16_t *src = xs + ((size_t)img * hw + (size_t)y * W + xx) * cc + cl0 + c8;
+                for (int j = 0; j < 8; ++j) in[j] = *(const uint4 *)(src + (size_t)j * cc);
+            } else {
+                for (int j = 0; j < 8; ++j) in[j] = make_uint4(0, 0, 0, 0);
+            }
+            transpose8_store(in, sx + c8 * planeX + q0, planeX);
         }
         __syncthreads();
         if (doBias) {
@@ -570,28 +583,56 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
     }
 }
 
-// Deterministic reduction of part[zn][E] into dW in two passes: pass 1 sums
-// each chunk of kWgChunk slices into the chunk's first slice (grid: E blocks
-// x chunks); pass 2 adds the chunk sums into dW.
-constexpr int kWgChunk = 32;
-
-__global__ void wgrad_reduce_chunks_kernel(float *__restrict__ part, int zn, size_t E) {
-    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    const int z0 = blockIdx.y * kWgChunk, z1 = min(zn, z0 + kWgChunk);
-    if (e >= E) return;
-    float s = 0.f;
-    for (int z = z0; z < z1; ++z) s += part[(size_t)z * E + e];
-    part[(size_t)z0 * E + e] = s;
-}
-
-// dW[e] += sum over slices z = 0, step, 2 step, ... < zn (step kWgChunk after
-// the chunk pass, 1 without it)
-__global__ void wgrad_reduce_kernel(const float *__restrict__ part, int zn, int step, size_t E,
-                                    float *__restrict__ dw) {
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < E; e += (size_t)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int z = 0; z < zn; z += step) s += part[(size_t)z * E + e];
-        dw[e] += s;
+// Deterministic reduction of part[zn][E] into dW in one launch: dW[e] +=
+// sum_z part[z][e].  A workgroup takes `cols` float4 columns (4 cols
+// consecutive elements) and zg = 256 / cols groups of slices; work-item
+// (col, g) sums slices g, g + zg, g + 2 zg, ... in that order (eight loads
+// in flight), then the zg group sums of a column are added in group order.
+// The order depends only on zn and zg, so the result is reproducible.
+__global__ __launch_bounds__(256) void wgrad_sum_slices_kernel(const float4 *__restrict__ part, int zn, int zg,
+                                                               size_t E4, float4 *__restrict__ dw) {
+    __shared__ float4 sums[256];
+    const int tid = threadIdx.x, cols = 256 / zg, col = tid % cols, g = tid / cols;
+    const size_t e4 = (size_t)blockIdx.x * cols + col;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (e4 < E4) {
+        const float4 *p = part + e4;
+        int z = g;
+        for (; z + 7 * zg < zn; z += 8 * zg) {
+            float4 v[8];
+            for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(z + u * zg) * E4];
+            for (int u = 0; u < 8; ++u) {
+                s.x += v[u].x;
+                s.y += v[u].y;
+                s.z += v[u].z;
+                s.w += v[u].w;
+            }
+        }
+        for (; z < zn; z += zg) {
+            const float4 v = p[(size_t)z * E4];
+            s.x += v.x;
+            s.y += v.y;
+            s.z += v.z;
+            s.w += v.w;
+        }
+    }
+    sums[tid] = s;
+    __syncthreads();
+    if (g == 0 && e4 < E4) {
+        float4 t = sums[col];
+        for (int k = 1; k < zg; ++k) {
+            const float4 v = sums[k * cols + col];
+            t.x += v.x;
+            t.y += v.y;
+            t.z += v.z;
+            t.w += v.w;
+        }
+        float4 o = dw[e4];
+        o.x += t.x;
+        o.y += t.y;
+        o.z += t.z;
+        o.w += t.w;
+        dw[e4] = o;
     }
 }
 
@@ -1005,7 +1046,7 @@ struct WgradTiling {
 };
 static bool wgrad_tiling(const WgradArgs &a, WgradTiling &t) {
     const int cin = a.c1 + a.c2;
-    if (!a.part || a.w % 8 || a.w > kWgPix) return false;           // widths that are a multiple of 8
+    if (!a.part || !a.dyb || a.w % 8 || a.w > kWgPix) return false;  // BF16 dY, widths a multiple of 8
     t.R = kWgPix / a.w < a.h ? kWgPix / a.w : a.h;
     t.lds = (size_t)(32 * (t.R * a.w + 8) + 32 * ((t.R + 2) * (a.w + 16) + 8)) * sizeof(uint16_t);
     t.rowsPerImg = (a.h + t.R - 1) / t.R;
@@ -1030,18 +1071,18 @@ hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
     if (wgrad_tiling(a, t)) {
         // a slice of partials: dW, then the bias gradient (a.db must be a.dw + cout 9 cin)
         if (a.db && a.db != a.dw + (t.E - a.cout)) return hipErrorInvalidValue;
+        // float4 sums: E is a multiple of 4 (cout is), dW and the partials 16-byte aligned
+        if (((uintptr_t)a.dw | (uintptr_t)a.part) & 15) return hipErrorInvalidValue;
         const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)t.zn);
         wgrad_tile_kernel<<<grid, 256, t.lds, s>>>(a, t.R, t.rowsPerImg, t.stages);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        int step = 1;
-        if (t.zn > kWgChunk) {        // sum chunks of kWgChunk slices first
-            const dim3 g1((unsigned)((t.E + 255) / 256), (unsigned)((t.zn + kWgChunk - 1) / kWgChunk));
-            wgrad_reduce_chunks_kernel<<<g1, 256, 0, s>>>(a.part, t.zn, t.E);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-            step = kWgChunk;
-        }
-        wgrad_reduce_kernel<<<blocks_for((long long)t.E), 256, 0, s>>>(a.part, t.zn, step, t.E, a.dw);
+        // slice groups: the largest power of two <= min(16, zn)
+        int zg = 1;
+        while (zg * 2 <= 16 && zg * 2 <= t.zn) zg *= 2;
+        const size_t E4 = t.E / 4, cols = 256 / zg;
+        wgrad_sum_slices_kernel<<<(unsigned)((E4 + cols - 1) / cols), 256, 0, s>>>(
+            reinterpret_cast<const float4 *>(a.part), t.zn, zg, E4, reinterpret_cast<float4 *>(a.dw));
         return hipGetLastError();
     }
     const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)(9 * a.splits));

@@ -57,7 +57,7 @@ struct wc_unet {
     WeightMap wmap{};
     float *d_loss = nullptr;
     float *d_part = nullptr;                     // weight-gradient partial sums (kWgradPartFloats)
-    float *d_bpart = nullptr;                    // bias-gradient partials, two buffers of kBiasPartFloats
+    float *d_bpart = nullptr;                    // bias-gradient partials, kBiasPartFloats per layer
     float *d_hpart = nullptr;                    // the head's partials (kHeadPartFloats)
     SideSums pending{};                          // sums the next mask launch finishes
     int adam_t = 0;
@@ -75,10 +75,11 @@ struct wc_unet {
              *p2 = nullptr, *a3 = nullptr, *s3 = nullptr, *p3 = nullptr, *a4 = nullptr, *b4 = nullptr,
              *u3 = nullptr, *d3a = nullptr, *d3 = nullptr, *u2 = nullptr, *d2a = nullptr, *d2 = nullptr,
              *u1 = nullptr, *d1a = nullptr, *d1 = nullptr;
-    // a layer's ReLU-masked output gradient in BF16 (backward scratch): layer
-    // l uses gmask[l % 2], so the weight gradient of layer l may still read
-    // it while the mask of layer l - 1 is written
-    uint16_t *gmask[2] = {nullptr, nullptr};
+    // each layer's ReLU-masked output gradient in BF16 (backward scratch),
+    // gmaskCap[l] elements: one buffer per layer, so the data-gradient chain
+    // never waits for a weight gradient that still reads an earlier mask
+    uint16_t *gmask[kMaxConvLayers] = {};
+    size_t gmaskCap[kMaxConvLayers] = {};
     float *x = nullptr, *out = nullptr, *tgt = nullptr;
     float *wgt = nullptr;     // per-pixel loss weights of the current batch
     bool useWgt = false;
@@ -150,7 +151,13 @@ int ensure_shape(wc_unet *u, int n, int h, int w) {
     rc |= alloc(u, u->a4, P3 * 8 * C) | alloc(u, u->b4, P3 * 8 * C) | alloc(u, u->u3, P2 * 8 * C);
     rc |= alloc(u, u->d3a, P2 * 4 * C) | alloc(u, u->d3, P2 * 4 * C) | alloc(u, u->u2, P1 * 4 * C);
     rc |= alloc(u, u->d2a, P1 * 2 * C) | alloc(u, u->d2, P1 * 2 * C) | alloc(u, u->u1, P0 * 2 * C);
-    rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C) | alloc(u, u->gmask[0], P0 * C) | alloc(u, u->gmask[1], P0 * C);
+    rc |= alloc(u, u->d1a, P0 * C) | alloc(u, u->d1, P0 * C);
+    for (size_t l = 0; l < u->layers.size(); ++l) {     // a layer of cout = C 2^k runs at level k: P0 / 4^k pixels
+        int k = 0;
+        while ((C << k) < u->layers[l].cout) ++k;
+        u->gmaskCap[l] = (P0 >> (2 * k)) * (size_t)u->layers[l].cout;
+        rc |= alloc(u, u->gmask[l], u->gmaskCap[l]);
+    }
     rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0) | alloc(u, u->wgt, P0);
     if (u->cin > 1) rc |= alloc(u, u->xc, P0 * u->cin);
     rc |= alloc(u, u->g_d1, P0 * C) | alloc(u, u->g_d1a, P0 * C) | alloc(u, u->g_s1, P0 * C) |
@@ -187,11 +194,10 @@ int conv_fwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2,
 int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, const uint16_t *x1, const uint16_t *x2,
              float *gx1, bool acc1, float *gx2, bool acc2, int h, int w) {
     const long long P = (long long)u->n * h * w;
-    const int li = (int)(&L - u->layers.data()), buf = li & 1;
-    uint16_t *gm = u->gmask[buf];
-    float *bpart = u->d_bpart + buf * kBiasPartFloats;
-    // the buffers of parity `buf` were last read by the weight gradient of layer li + 2
-    if (li + 2 < (int)u->layers.size()) UCHK(u, hipStreamWaitEvent(u->stream, u->evWgrad[li + 2], 0));
+    const int li = (int)(&L - u->layers.data());
+    if ((size_t)P * L.cout > u->gmaskCap[li]) return ufail(u, WC_ERR_INVALID, "layer %d: mask buffer too small", li);
+    uint16_t *gm = u->gmask[li];
+    float *bpart = u->d_bpart + li * kBiasPartFloats;
     WgradArgs g{};
     g.dy = gy; g.y = y; g.x1 = x1; g.x2 = x2; g.c1 = L.c1; g.c2 = L.c2; g.cout = L.cout;
     g.dw = u->d_g + L.w_off; g.n = u->n; g.h = h; g.w = w;
@@ -212,18 +218,21 @@ int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, con
     UCHK(u, launch_mask_bf16(gy, y, P * L.cout, L.cout, gm, tiled ? nullptr : bpart, kBiasPartFloats, &nb, side,
                              u->stream));
     UCHK(u, hipEventRecord(u->evMask[li], u->stream));
+    // the data gradient first: it is recorded as the mask's first dependent,
+    // so a replayed graph keeps the data-gradient chain on one queue
+    if (gx1) {
+        ConvArgs a{};
+        a.x1 = gm; a.x2 = nullptr; a.m1 = nullptr; a.c1 = L.cout; a.c2 = 0;
+        a.w_ = u->d_wf + L.wb_off; a.bias = nullptr;
+        a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
+        a.n = u->n; a.h = h; a.w = w;
+        UCHK(u, launch_conv3x3(a, false, u->stream));
+    }
     g.bpart = bpart;
     g.bnb = nb;
     UCHK(u, hipStreamWaitEvent(u->side, u->evMask[li], 0));
     UCHK(u, launch_wgrad(g, u->side));
     UCHK(u, hipEventRecord(u->evWgrad[li], u->side));
-    if (!gx1) return WC_OK;
-    ConvArgs a{};
-    a.x1 = gm; a.x2 = nullptr; a.m1 = nullptr; a.c1 = L.cout; a.c2 = 0;
-    a.w_ = u->d_wf + L.wb_off; a.bias = nullptr;
-    a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
-    a.n = u->n; a.h = h; a.w = w;
-    UCHK(u, launch_conv3x3(a, false, u->stream));
     return WC_OK;
 }
 
@@ -438,7 +447,7 @@ int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
         hipMalloc(&u->d_v, u->nparam * 4) || hipMalloc(&u->d_wb, u->nwb * 2 + 256) ||
         hipMalloc(&u->d_wf, u->nwb * 2 + 256) || hipMalloc(&u->d_loss, 256) ||
         hipMalloc(&u->d_part, kWgradPartFloats * sizeof(float)) ||
-        hipMalloc(&u->d_bpart, 2 * kBiasPartFloats * sizeof(float)) || hipMalloc(&u->d_hpart, kHeadPartFloats * sizeof(float)))
+        hipMalloc(&u->d_bpart, kMaxConvLayers * kBiasPartFloats * sizeof(float)) || hipMalloc(&u->d_hpart, kHeadPartFloats * sizeof(float)))
         return bad(WC_ERR_NOMEM);
     if (hipMemcpy(u->d_p, p.data(), u->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return bad(WC_ERR_HIP);
     if (launch_fill_f32(u->d_m, (long long)u->nparam, 0.f, u->stream) != hipSuccess ||
