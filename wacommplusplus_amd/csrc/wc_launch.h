@@ -181,6 +181,19 @@ struct ConvArgs {
     bool acc1, acc2;
     int mode;
     int n, h, w;
+    // input gradient (mode 1) of a single-input layer, stored: with mo set
+    // the result goes out as the previous layer's ReLU-masked gradient
+    // mo[e] = bf16(my[e] > 0 ? g[e] : 0) (my: that layer's output); y1 is
+    // still needed as FP32 scratch where the general kernel runs
+    uint16_t *mo;
+    const uint16_t *my;
+};
+// A layer's ReLU-masked output gradient in BF16, written by the kernel that
+// produces the gradient: gm[e] = bf16(y[e] > 0 ? g[e] : 0) in place of the
+// FP32 gradient (gm null: the FP32 gradient is written)
+struct MaskOut {
+    uint16_t *gm;
+    const uint16_t *y;
 };
 // dW[co][tap][ci] += sum_p dY[p][co] * (Y[p][co] > 0) * X[p + tap][ci]
 struct WgradArgs {
@@ -226,16 +239,17 @@ hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s);
 // whether launch_wgrad takes the tiled path (which sums the bias itself)
 bool wgrad_is_tiled(const WgradArgs &a);
 // out[e] = bf16(y[e] > 0 ? dy[e] : 0), a layer's ReLU-masked output gradient
-// (E = pixels x cout, cout <= 256), and, when part is set, per workgroup the
-// sums of that BF16 gradient per channel in part[b][cout] (partCap floats;
-// *nblocks receives the number of workgroups).  The last workgroups also
-// finish the jobs in `side`.
+// (E = pixels x cout, cout <= 256; dy null: out already holds it), and, when
+// part is set, per workgroup the sums of that BF16 gradient per channel in
+// part[b][cout] (partCap floats; *nblocks receives the number of
+// workgroups).  The last workgroups also finish the jobs in `side`.
 hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int cout, uint16_t *out, float *part,
                             size_t partCap, int *nblocks, const SideSums &side, hipStream_t s);
 hipError_t launch_pool_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
-hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, hipStream_t s);
+hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, const MaskOut &m,
+                           hipStream_t s);
 hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s);
-hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx, hipStream_t s);
+hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx, const MaskOut &m, hipStream_t s);
 // x [n][cin][h][w] -> the first layer's NHWC BF16 input with c channels
 // (P = n h w, hw = h w); x0 (optional) receives channel 0 as [n][h][w]
 hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c, uint16_t *y, float *x0,
@@ -243,12 +257,17 @@ hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c
 hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, long long P, float *out,
                            hipStream_t s);
 // The head's backward for loss = mean wgt (out - t)^2: dd (the gradient of
-// the head's input), and per workgroup the partials part[b][kHead + 2] of
+// the head's input d), or with gm set that gradient ReLU-masked by d in BF16
+// (the last layer's masked gradient) in place of dd, and per workgroup the
+// partials part[b][kHead + 2] of
 // the head weights' and bias's gradients and of the loss (*nblocks
 // workgroups), left for a later PartialSums job.
 constexpr int kHeadPartials = 34;
 hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wgt, const float *wh,
-                           long long P, float *dd, float *part, size_t partCap, int *nblocks, hipStream_t s);
+                           long long P, float *dd, uint16_t *gm, float *part, size_t partCap, int *nblocks,
+                           hipStream_t s);
+// the jobs of `side` (up to two), one workgroup each
+hipError_t launch_finish_sums(const SideSums &side, hipStream_t s);
 hipError_t launch_flip(const uint16_t *w, int cout, int cin, uint16_t *wf, hipStream_t s);
 // Where each conv layer's weights sit: FP32 parameters [w_off, w_off +
 // cout*9*cin) as [cout][9][cin]; their BF16 copy at wb_off of wb, and the

@@ -52,6 +52,15 @@ union Pack8 {
 // of value c into four running sums (every fourth term each, so four loads
 // are in flight), then thread c adds the S segment sums in order.  The
 // partials come from an earlier launch on the stream.
+// bf16(y > 0 ? g : 0) of four channels (y: four BF16 values in a uint2)
+__device__ __forceinline__ uint2 mask4(const float4 &g, const uint2 &y) {
+    const uint16_t h0 = to_bf16(from_bf16((uint16_t)y.x) > 0.f ? g.x : 0.f);
+    const uint16_t h1 = to_bf16(from_bf16((uint16_t)(y.x >> 16)) > 0.f ? g.y : 0.f);
+    const uint16_t h2 = to_bf16(from_bf16((uint16_t)y.y) > 0.f ? g.z : 0.f);
+    const uint16_t h3 = to_bf16(from_bf16((uint16_t)(y.y >> 16)) > 0.f ? g.w : 0.f);
+    return make_uint2((uint32_t)h0 | (uint32_t)h1 << 16, (uint32_t)h2 | (uint32_t)h3 << 16);
+}
+
 __device__ void fixed_order_sum(const PartialSums &j) {
     __shared__ float seg[256];
     const int m = j.m, S = 256 / m, t = threadIdx.x;
@@ -259,6 +268,17 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs &a, const f32x4 (&a
         __syncthreads();
         const bool first = co0 < a.cy1;
         const int cc = first ? a.cy1 : a.cy2, cl0 = first ? co0 : co0 - a.cy1;
+        if (a.mo) {                                       // the next layer's masked gradient (one output, stored)
+            const uint16_t *my = a.my + p0 * cc + cl0;
+            uint16_t *mo = a.mo + p0 * cc + cl0;
+            for (int c = tid; c < qmax * 8; c += 256) {
+                const int q = c >> 3, part = c & 7;
+                if (co0 + 4 * part >= nout) continue;
+                const float4 v = *(const float4 *)(so + q * kCvOutF + 4 * part);
+                *(uint2 *)(mo + (size_t)q * cc + 4 * part) = mask4(v, *(const uint2 *)(my + (size_t)q * cc + 4 * part));
+            }
+            return;
+        }
         const bool accum = first ? a.acc1 : a.acc2;
         float *y = (float *)(first ? a.y1 : a.y2) + p0 * cc + cl0;
         for (int c = tid; c < qmax * 8; c += 256) {
@@ -390,13 +410,18 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
         for (int e = tid; e < 64 * 32; e += 256) {
             const int pl = e >> 5, c = e & 31;
             const long long p = pb + pl;
-            float g = 0.f;
+            uint16_t gv = 0;
             if (p < pend && co0 + c < a.cout) {
                 const size_t o = (size_t)p * a.cout + co0 + c;
-                g = a.dy[o];
-                if (!(from_bf16(a.y[o]) > 0.f)) g = 0.f;
+                if (a.dyb) {
+                    gv = a.dyb[o];
+                } else {
+                    float g = a.dy[o];
+                    if (!(from_bf16(a.y[o]) > 0.f)) g = 0.f;
+                    gv = to_bf16(g);
+                }
             }
-            sdy[c][pl] = to_bf16(g);
+            sdy[c][pl] = gv;
             uint16_t xv = 0;
             if (p < pend && ci0 + c < cin) {
                 const int nimg = (int)(p / hw), rr = (int)(p - (long long)nimg * hw);
@@ -653,32 +678,32 @@ __global__ __launch_bounds__(256) void bias_reduce_kernel(const float *__restric
     if (tid == 0) db[c] += red[0];
 }
 
-// The ReLU-masked output gradient of a layer in BF16 and, when part is set,
-// each workgroup's sums of the rounded masked dY per channel in
-// part[block][cout] (the bias gradient's partials for the split-K weight
-// gradient; the tiled one sums the bias itself).  A work-item takes 4
-// channels; cout divides 1024, so its channels stay the same over the
-// grid-stride loop.  The last workgroups also finish up to two pending sums.
+// The ReLU-masked output gradient of a layer in BF16 (dy null: read from
+// out, already masked) and, when part is set, each workgroup's sums of the
+// rounded masked dY per channel in part[block][cout] (the bias gradient's
+// partials for the split-K weight gradient; the tiled one sums the bias
+// itself).  A work-item takes 4 channels; cout divides 1024, so its channels
+// stay the same over the grid-stride loop.  The last workgroups also finish
+// up to two pending sums.
 __global__ __launch_bounds__(256) void mask_bf16_kernel(const float *__restrict__ dy, const uint16_t *__restrict__ y,
                                                         long long E4, int cout, uint16_t *__restrict__ out,
                                                         float *__restrict__ part, SideSums side) {
     const int tid = threadIdx.x;
     float sb[4] = {0.f, 0.f, 0.f, 0.f};
     for (long long q = blockIdx.x * 256LL + tid; q < E4; q += (long long)gridDim.x * 256) {
-        const float4 d = ((const float4 *)dy)[q];
-        const uint2 m = ((const uint2 *)y)[q];
-        const float g0 = from_bf16((uint16_t)m.x) > 0.f ? d.x : 0.f;
-        const float g1 = from_bf16((uint16_t)(m.x >> 16)) > 0.f ? d.y : 0.f;
-        const float g2 = from_bf16((uint16_t)m.y) > 0.f ? d.z : 0.f;
-        const float g3 = from_bf16((uint16_t)(m.y >> 16)) > 0.f ? d.w : 0.f;
-        const uint16_t h0 = to_bf16(g0), h1 = to_bf16(g1), h2 = to_bf16(g2), h3 = to_bf16(g3);
-        if (part) {
-            sb[0] += from_bf16(h0); sb[1] += from_bf16(h1); sb[2] += from_bf16(h2); sb[3] += from_bf16(h3);
-        }
         uint2 r;
-        r.x = (uint32_t)h0 | (uint32_t)h1 << 16;
-        r.y = (uint32_t)h2 | (uint32_t)h3 << 16;
-        ((uint2 *)out)[q] = r;
+        if (dy) {
+            r = mask4(((const float4 *)dy)[q], ((const uint2 *)y)[q]);
+            ((uint2 *)out)[q] = r;
+        } else {
+            r = ((const uint2 *)out)[q];
+        }
+        if (part) {
+            sb[0] += from_bf16((uint16_t)r.x);
+            sb[1] += from_bf16((uint16_t)(r.x >> 16));
+            sb[2] += from_bf16((uint16_t)r.y);
+            sb[3] += from_bf16((uint16_t)(r.y >> 16));
+        }
     }
     __shared__ float red[256][4];
     if (part) {
@@ -698,6 +723,11 @@ __global__ __launch_bounds__(256) void mask_bf16_kernel(const float *__restrict_
             __syncthreads();
             fixed_order_sum(side.job[q]);
         }
+}
+
+// up to two sums left as partials by earlier launches, one workgroup each
+__global__ __launch_bounds__(256) void finish_sums_kernel(SideSums side) {
+    if (side.job[blockIdx.x].part) fixed_order_sum(side.job[blockIdx.x]);
 }
 
 // 2x2 average pooling, BF16 NHWC (h, w even)
@@ -760,19 +790,22 @@ __global__ void pool_fwd_vec_kernel(const uint4 *__restrict__ x, unsigned n, uns
     }
 }
 
+// m.gm set: the sum goes out ReLU-masked in BF16 to m.gm instead of dx
 __global__ void pool_bwd_vec_kernel(const float4 *__restrict__ dy, unsigned n, unsigned h, unsigned w,
-                                    unsigned groups, float4 *__restrict__ dx, int accumulate) {
+                                    unsigned groups, float4 *__restrict__ dx, int accumulate, MaskOut m) {
     const unsigned ho = h / 2, wo = w / 2, total = n * h * w * groups;
     for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
         const PixIdx q = pix_idx(e, groups, w, h);
         const float4 d = dy[(((size_t)q.ni * ho + q.y / 2) * wo + q.x / 2) * groups + q.g];
-        const float4 g = make_float4(0.25f * d.x, 0.25f * d.y, 0.25f * d.z, 0.25f * d.w);
+        float4 g = make_float4(0.25f * d.x, 0.25f * d.y, 0.25f * d.z, 0.25f * d.w);
         if (accumulate) {
             const float4 a = dx[e];
-            dx[e] = make_float4(a.x + g.x, a.y + g.y, a.z + g.z, a.w + g.w);
-        } else {
-            dx[e] = g;
+            g = make_float4(a.x + g.x, a.y + g.y, a.z + g.z, a.w + g.w);
         }
+        if (m.gm)
+            ((uint2 *)m.gm)[e] = mask4(g, ((const uint2 *)m.y)[e]);
+        else
+            dx[e] = g;
     }
 }
 
@@ -785,16 +818,21 @@ __global__ void up_fwd_vec_kernel(const uint4 *__restrict__ x, unsigned n, unsig
     }
 }
 
+// m.gm set: the sum goes out ReLU-masked in BF16 to m.gm instead of dx
 __global__ void up_bwd_vec_kernel(const float4 *__restrict__ dy, unsigned n, unsigned h, unsigned w, unsigned groups,
-                                  float4 *__restrict__ dx) {
+                                  float4 *__restrict__ dx, MaskOut m) {
     const unsigned wo = 2 * w, total = n * h * w * groups;
     for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
         const PixIdx q = pix_idx(e, groups, w, h);
         const size_t b = (((size_t)q.ni * 2 * h + 2 * q.y) * wo + 2 * q.x) * groups + q.g;
         const float4 a = dy[b], bb = dy[b + groups], c = dy[b + (size_t)wo * groups],
                      d = dy[b + (size_t)wo * groups + groups];
-        dx[e] = make_float4(a.x + bb.x + c.x + d.x, a.y + bb.y + c.y + d.y, a.z + bb.z + c.z + d.z,
-                            a.w + bb.w + c.w + d.w);
+        const float4 g = make_float4(a.x + bb.x + c.x + d.x, a.y + bb.y + c.y + d.y, a.z + bb.z + c.z + d.z,
+                                     a.w + bb.w + c.w + d.w);
+        if (m.gm)
+            ((uint2 *)m.gm)[e] = mask4(g, ((const uint2 *)m.y)[e]);
+        else
+            dx[e] = g;
     }
 }
 
@@ -859,11 +897,14 @@ __global__ void head_fwd_kernel(const uint16_t *__restrict__ d, const float *__r
 }
 
 // loss += mean wgt (out - t)^2 (wgt = 1 when null); g = d loss / d out;
-// head gradients dwh; dd[p][c] = g[p] wh[c]
+// head gradients dwh; dd[p][c] = g[p] wh[c], or with gm set the last
+// layer's masked gradient gm[p][c] = bf16(d[p][c] > 0 ? g[p] wh[c] : 0)
+// (d is that layer's ReLU output) in place of dd
 __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *__restrict__ d, const float *__restrict__ out,
                                                        const float *__restrict__ t, const float *__restrict__ wgt,
                                                        const float *__restrict__ wh, long long P,
-                                                       float *__restrict__ dd, float *__restrict__ part) {
+                                                       float *__restrict__ dd, uint16_t *__restrict__ gm,
+                                                       float *__restrict__ part) {
     // four work-items per pixel, 8 of its kHead channels each (16-byte loads)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cg = tid & 3;
     const float inv = 1.0f / (float)P;
@@ -882,9 +923,18 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *__restric
         Pack8 dv;
         dv.u = *(const uint4 *)(d + p * kHead + 8 * cg);
         for (int q = 0; q < 8; ++q) v[q] += g * from_bf16(dv.h[q]);
-        float *o = dd + p * kHead + 8 * cg;
-        *(float4 *)o = make_float4(g * w8[0], g * w8[1], g * w8[2], g * w8[3]);
-        *(float4 *)(o + 4) = make_float4(g * w8[4], g * w8[5], g * w8[6], g * w8[7]);
+        const float4 g0 = make_float4(g * w8[0], g * w8[1], g * w8[2], g * w8[3]);
+        const float4 g1 = make_float4(g * w8[4], g * w8[5], g * w8[6], g * w8[7]);
+        if (gm) {
+            uint4 r;
+            const uint2 lo = mask4(g0, make_uint2(dv.u.x, dv.u.y)), hi = mask4(g1, make_uint2(dv.u.z, dv.u.w));
+            r.x = lo.x; r.y = lo.y; r.z = hi.x; r.w = hi.y;
+            *(uint4 *)(gm + p * kHead + 8 * cg) = r;
+        } else {
+            float *o = dd + p * kHead + 8 * cg;
+            *(float4 *)o = g0;
+            *(float4 *)(o + 4) = g1;
+        }
     }
     // sum over the lanes of a wave with the same cg, then over the waves
     for (int off = 4; off < 64; off <<= 1)
@@ -995,11 +1045,21 @@ int blocks_for(long long n) {
 
 }  // namespace
 
+// where no kernel writes it directly, a masked output is the FP32 result masked by a second launch
+static hipError_t mask_after(const float *dx, long long E, int c, const MaskOut &m, hipStream_t s) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !m.gm) return e;
+    int nb = 0;
+    return launch_mask_bf16(dx, m.y, E, c, m.gm, nullptr, 0, &nb, SideSums{}, s);
+}
+
 hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
     const long long P = (long long)a.n * a.h * a.w;
     const int nout = a.cy1 + a.cy2;
     if (P <= 0 || nout <= 0) return hipSuccess;
     if ((a.c1 % 32) || (a.c2 % 32) || (nout % 32) || ((a.cy1 % 32) && a.cy2)) return hipErrorInvalidValue;
+    // a masked BF16 output: the input gradient of a single-input layer, stored
+    if (a.mo && (a.mode != 1 || a.cy2 || a.acc1 || !a.my || !a.y1)) return hipErrorInvalidValue;
     if (!fp32_input && !a.m1) {
         // tiled path: R rows per workgroup (R * w <= 256), halved while the
         // grid has fewer than 512 workgroups, as long as the stage fits in LDS
@@ -1029,12 +1089,13 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
             return hipGetLastError();
         }
     }
+    // the general kernel writes the FP32 gradient to y1; a second launch masks it
     const dim3 grid((unsigned)((P + 127) / 128), (unsigned)(nout / 32));
     if (fp32_input)
         conv3x3_kernel<float><<<grid, 256, 0, s>>>(a);
     else
         conv3x3_kernel<uint16_t><<<grid, 256, 0, s>>>(a);
-    return hipGetLastError();
+    return mask_after((const float *)a.y1, P * a.cy1, a.cy1, MaskOut{a.mo, a.my}, s);
 }
 
 // The tiled weight gradient's shape for these arguments: stage rows R,
@@ -1114,6 +1175,13 @@ hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int
     return hipGetLastError();
 }
 
+hipError_t launch_finish_sums(const SideSums &side, hipStream_t s) {
+    if (!job_ok(side.job[0]) || !job_ok(side.job[1])) return hipErrorInvalidValue;
+    if (!side.job[0].part && !side.job[1].part) return hipSuccess;
+    finish_sums_kernel<<<2, 256, 0, s>>>(side);
+    return hipGetLastError();
+}
+
 // the vector forms apply when the channels split into 16-byte groups and the
 // group count of the larger tensor stays below 2^31
 static bool vec_ok(long long groups_total, int c, int per) { return c % per == 0 && groups_total < (1LL << 31); }
@@ -1127,13 +1195,15 @@ hipError_t launch_pool_fwd(const uint16_t *x, int n, int h, int w, int c, uint16
     }
     return hipGetLastError();
 }
-hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, hipStream_t s) {
-    if (vec_ok((long long)n * h * w * (c / 4), c, 4))
+hipError_t launch_pool_bwd(const float *dy, int n, int h, int w, int c, float *dx, bool accumulate, const MaskOut &m,
+                           hipStream_t s) {
+    if (vec_ok((long long)n * h * w * (c / 4), c, 4)) {
         pool_bwd_vec_kernel<<<blocks_for((long long)n * h * w * (c / 4)), 256, 0, s>>>(
-            (const float4 *)dy, n, h, w, c / 4, (float4 *)dx, accumulate ? 1 : 0);
-    else
-        pool_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx, accumulate ? 1 : 0);
-    return hipGetLastError();
+            (const float4 *)dy, n, h, w, c / 4, (float4 *)dx, accumulate ? 1 : 0, m);
+        return hipGetLastError();
+    }
+    pool_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx, accumulate ? 1 : 0);
+    return mask_after(dx, (long long)n * h * w * c, c, m, s);
 }
 hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t *y, hipStream_t s) {
     if (vec_ok((long long)n * 4 * h * w * (c / 8), c, 8))
@@ -1143,13 +1213,14 @@ hipError_t launch_up_fwd(const uint16_t *x, int n, int h, int w, int c, uint16_t
         up_fwd_kernel<<<blocks_for((long long)n * 4 * h * w * c), 256, 0, s>>>(x, n, h, w, c, y);
     return hipGetLastError();
 }
-hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx, hipStream_t s) {
-    if (vec_ok((long long)n * 4 * h * w * (c / 4), c, 4))
+hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx, const MaskOut &m, hipStream_t s) {
+    if (vec_ok((long long)n * 4 * h * w * (c / 4), c, 4)) {
         up_bwd_vec_kernel<<<blocks_for((long long)n * h * w * (c / 4)), 256, 0, s>>>((const float4 *)dy, n, h, w,
-                                                                                 c / 4, (float4 *)dx);
-    else
-        up_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx);
-    return hipGetLastError();
+                                                                                 c / 4, (float4 *)dx, m);
+        return hipGetLastError();
+    }
+    up_bwd_kernel<<<blocks_for((long long)n * h * w * c), 256, 0, s>>>(dy, n, h, w, c, dx);
+    return mask_after(dx, (long long)n * h * w * c, c, m, s);
 }
 hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c, uint16_t *y, float *x0,
                              hipStream_t s) {
@@ -1163,11 +1234,12 @@ hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, l
     return hipGetLastError();
 }
 hipError_t launch_head_bwd(const uint16_t *d, const float *out, const float *t, const float *wgt, const float *wh,
-                           long long P, float *dd, float *part, size_t partCap, int *nblocks, hipStream_t s) {
+                           long long P, float *dd, uint16_t *gm, float *part, size_t partCap, int *nblocks,
+                           hipStream_t s) {
     int b = blocks_for(P);
     if (b > 256) b = 256;
     if ((size_t)b * (kHead + 2) > partCap) return hipErrorInvalidValue;
-    head_bwd_kernel<<<b, 256, 0, s>>>(d, out, t, wgt, wh, P, dd, part);
+    head_bwd_kernel<<<b, 256, 0, s>>>(d, out, t, wgt, wh, P, dd, gm, part);
     *nblocks = b;
     return hipGetLastError();
 }

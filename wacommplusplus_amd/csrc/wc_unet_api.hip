@@ -188,35 +188,42 @@ int conv_fwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2,
     return WC_OK;
 }
 
-// backward of layer L, whose output y = relu(...) received gradient gy:
-// weight and bias gradients; when gx1 != null, the input gradients
-// (channels [0, c1) to gx1, [c1, c1+c2) to gx2), stored or accumulated.
-int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, const uint16_t *x1, const uint16_t *x2,
-             float *gx1, bool acc1, float *gx2, bool acc2, int h, int w) {
+// The masked output gradient of layer li (P pixels, output y): the buffer
+// that the kernel producing the gradient writes it to.
+int mask_of(wc_unet *u, int li, const uint16_t *y, long long P, MaskOut &m) {
+    if ((size_t)P * u->layers[li].cout > u->gmaskCap[li])
+        return ufail(u, WC_ERR_INVALID, "layer %d: mask buffer too small", li);
+    m = MaskOut{u->gmask[li], y};
+    return WC_OK;
+}
+
+// backward of layer L, whose ReLU-masked output gradient u->gmask[li] its
+// producer has written: the weight and bias gradients (side stream); when
+// gx1 != null the input gradients (channels [0, c1) to gx1, [c1, c1+c2) to
+// gx2, stored or accumulated), or, with next.gm set, the previous layer's
+// masked gradient in place of gx1 (a single input, stored).
+int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2, float *gx1, bool acc1, float *gx2,
+             bool acc2, int h, int w, const MaskOut &next = MaskOut{}) {
     const long long P = (long long)u->n * h * w;
     const int li = (int)(&L - u->layers.data());
     if ((size_t)P * L.cout > u->gmaskCap[li]) return ufail(u, WC_ERR_INVALID, "layer %d: mask buffer too small", li);
+    if (next.gm && (!gx1 || li == 0 || L.c2 || acc1 || u->layers[li - 1].cout != L.c1))
+        return ufail(u, WC_ERR_INVALID, "layer %d: a masked input gradient needs one stored input", li);
     uint16_t *gm = u->gmask[li];
     float *bpart = u->d_bpart + li * kBiasPartFloats;
     WgradArgs g{};
-    g.dy = gy; g.y = y; g.x1 = x1; g.x2 = x2; g.c1 = L.c1; g.c2 = L.c2; g.cout = L.cout;
+    g.x1 = x1; g.x2 = x2; g.c1 = L.c1; g.c2 = L.c2; g.cout = L.cout;
     g.dw = u->d_g + L.w_off; g.n = u->n; g.h = h; g.w = w;
     long long sp = P / 2048;
     g.splits = (int)(sp < 1 ? 1 : (sp > 1024 ? 1024 : sp));
     g.part = u->d_part;
     g.partCap = kWgradPartFloats;
-    // the masked gradient in BF16, once for the weight gradient and the data
-    // gradient, and the bias gradient: summed by the tiled weight gradient,
-    // or from the mask launch's partials; the mask launch also finishes sums
-    // left pending
+    // the masked gradient in BF16 feeds the weight gradient and the data
+    // gradient; the bias gradient is summed by the tiled weight gradient, or
+    // from per-workgroup partials of the masked gradient (split-K path)
     g.dyb = gm;
     g.db = u->d_g + L.b_off;
     const bool tiled = wgrad_is_tiled(g);
-    int nb = 0;
-    const SideSums side = u->pending;
-    u->pending = SideSums{};
-    UCHK(u, launch_mask_bf16(gy, y, P * L.cout, L.cout, gm, tiled ? nullptr : bpart, kBiasPartFloats, &nb, side,
-                             u->stream));
     UCHK(u, hipEventRecord(u->evMask[li], u->stream));
     // the data gradient first: it is recorded as the mask's first dependent,
     // so a replayed graph keeps the data-gradient chain on one queue
@@ -226,11 +233,19 @@ int conv_bwd(wc_unet *u, const Layer &L, const float *gy, const uint16_t *y, con
         a.w_ = u->d_wf + L.wb_off; a.bias = nullptr;
         a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
         a.n = u->n; a.h = h; a.w = w;
+        a.mo = next.gm; a.my = next.y;
         UCHK(u, launch_conv3x3(a, false, u->stream));
     }
+    UCHK(u, hipStreamWaitEvent(u->side, u->evMask[li], 0));
+    // sums an earlier launch left as partials (the head's)
+    UCHK(u, launch_finish_sums(u->pending, u->side));
+    u->pending = SideSums{};
+    int nb = 0;
+    if (!tiled)
+        UCHK(u, launch_mask_bf16(nullptr, nullptr, P * L.cout, L.cout, gm, bpart, kBiasPartFloats, &nb, SideSums{},
+                                 u->side));
     g.bpart = bpart;
     g.bnb = nb;
-    UCHK(u, hipStreamWaitEvent(u->side, u->evMask[li], 0));
     UCHK(u, launch_wgrad(g, u->side));
     UCHK(u, hipEventRecord(u->evWgrad[li], u->side));
     return WC_OK;
@@ -279,48 +294,64 @@ int backward(wc_unet *u) {
     u->pending = SideSums{};
     UCHK(u, launch_fill_f32(u->d_g, (long long)u->nparam, 0.f, u->stream));
     UCHK(u, launch_fill_f32(u->d_loss, 1, 0.f, u->stream));
+    const long long P0 = (long long)n * H * W, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
+    // each layer's masked output gradient is written by the kernel that
+    // produces the gradient: the head, a data-gradient convolution, an
+    // upsampling's or a pooling's backward
+    MaskOut m{};
     int hb = 0;
-    UCHK(u, launch_head_bwd(u->d1, u->out, u->tgt, u->useWgt ? u->wgt : nullptr, u->d_p + u->head_off,
-                            (long long)n * H * W, u->g_d1, u->d_hpart, kHeadPartFloats, &hb, u->stream));
-    // the head's weight and bias gradients and the loss: finished by the
-    // next mask launch
+    if ((rc = mask_of(u, 13, u->d1, P0, m))) return rc;
+    UCHK(u, launch_head_bwd(u->d1, u->out, u->tgt, u->useWgt ? u->wgt : nullptr, u->d_p + u->head_off, P0, u->g_d1,
+                            m.gm, u->d_hpart, kHeadPartFloats, &hb, u->stream));
+    // the head's weight and bias gradients and the loss: finished on the
+    // side stream before the first weight gradient
     u->pending.job[0] = PartialSums{u->d_hpart, hb, kHeadPartials - 1, kHeadPartials, u->d_g + u->head_off, 1.0f};
     u->pending.job[1] = PartialSums{u->d_hpart + kHeadPartials - 1, hb, 1, kHeadPartials, u->d_loss, 1.0f};
     // decoder, level 0
-    if ((rc = conv_bwd(u, Ls[13], u->g_d1, u->d1, u->d1a, nullptr, u->g_d1a, false, nullptr, false, H, W))) return rc;
-    if ((rc = conv_bwd(u, Ls[12], u->g_d1a, u->d1a, u->s1, u->u1, u->g_s1, false, u->g_u1, false, H, W))) return rc;
-    UCHK(u, launch_up_bwd(u->g_u1, n, H / 2, W / 2, 2 * C, u->g_d2, u->stream));
+    if ((rc = mask_of(u, 12, u->d1a, P0, m)) ||
+        (rc = conv_bwd(u, Ls[13], u->d1a, nullptr, u->g_d1a, false, nullptr, false, H, W, m)) ||
+        (rc = conv_bwd(u, Ls[12], u->s1, u->u1, u->g_s1, false, u->g_u1, false, H, W)) ||
+        (rc = mask_of(u, 11, u->d2, P1, m)))
+        return rc;
+    UCHK(u, launch_up_bwd(u->g_u1, n, H / 2, W / 2, 2 * C, u->g_d2, m, u->stream));
     // level 1
-    if ((rc = conv_bwd(u, Ls[11], u->g_d2, u->d2, u->d2a, nullptr, u->g_d2a, false, nullptr, false, H / 2, W / 2)))
+    if ((rc = mask_of(u, 10, u->d2a, P1, m)) ||
+        (rc = conv_bwd(u, Ls[11], u->d2a, nullptr, u->g_d2a, false, nullptr, false, H / 2, W / 2, m)) ||
+        (rc = conv_bwd(u, Ls[10], u->s2, u->u2, u->g_s2, false, u->g_u2, false, H / 2, W / 2)) ||
+        (rc = mask_of(u, 9, u->d3, P2, m)))
         return rc;
-    if ((rc = conv_bwd(u, Ls[10], u->g_d2a, u->d2a, u->s2, u->u2, u->g_s2, false, u->g_u2, false, H / 2, W / 2)))
-        return rc;
-    UCHK(u, launch_up_bwd(u->g_u2, n, H / 4, W / 4, 4 * C, u->g_d3, u->stream));
+    UCHK(u, launch_up_bwd(u->g_u2, n, H / 4, W / 4, 4 * C, u->g_d3, m, u->stream));
     // level 2
-    if ((rc = conv_bwd(u, Ls[9], u->g_d3, u->d3, u->d3a, nullptr, u->g_d3a, false, nullptr, false, H / 4, W / 4)))
+    if ((rc = mask_of(u, 8, u->d3a, P2, m)) ||
+        (rc = conv_bwd(u, Ls[9], u->d3a, nullptr, u->g_d3a, false, nullptr, false, H / 4, W / 4, m)) ||
+        (rc = conv_bwd(u, Ls[8], u->s3, u->u3, u->g_s3, false, u->g_u3, false, H / 4, W / 4)) ||
+        (rc = mask_of(u, 7, u->b4, P3, m)))
         return rc;
-    if ((rc = conv_bwd(u, Ls[8], u->g_d3a, u->d3a, u->s3, u->u3, u->g_s3, false, u->g_u3, false, H / 4, W / 4)))
-        return rc;
-    UCHK(u, launch_up_bwd(u->g_u3, n, H / 8, W / 8, 8 * C, u->g_b4, u->stream));
+    UCHK(u, launch_up_bwd(u->g_u3, n, H / 8, W / 8, 8 * C, u->g_b4, m, u->stream));
     // bottleneck
-    if ((rc = conv_bwd(u, Ls[7], u->g_b4, u->b4, u->a4, nullptr, u->g_a4, false, nullptr, false, H / 8, W / 8)))
+    if ((rc = mask_of(u, 6, u->a4, P3, m)) ||
+        (rc = conv_bwd(u, Ls[7], u->a4, nullptr, u->g_a4, false, nullptr, false, H / 8, W / 8, m)) ||
+        (rc = conv_bwd(u, Ls[6], u->p3, nullptr, u->g_p3, false, nullptr, false, H / 8, W / 8)) ||
+        (rc = mask_of(u, 5, u->s3, P2, m)))
         return rc;
-    if ((rc = conv_bwd(u, Ls[6], u->g_a4, u->a4, u->p3, nullptr, u->g_p3, false, nullptr, false, H / 8, W / 8)))
-        return rc;
-    UCHK(u, launch_pool_bwd(u->g_p3, n, H / 4, W / 4, 4 * C, u->g_s3, true, u->stream));
+    UCHK(u, launch_pool_bwd(u->g_p3, n, H / 4, W / 4, 4 * C, u->g_s3, true, m, u->stream));
     // encoder
-    if ((rc = conv_bwd(u, Ls[5], u->g_s3, u->s3, u->a3, nullptr, u->g_a3, false, nullptr, false, H / 4, W / 4)))
+    if ((rc = mask_of(u, 4, u->a3, P2, m)) ||
+        (rc = conv_bwd(u, Ls[5], u->a3, nullptr, u->g_a3, false, nullptr, false, H / 4, W / 4, m)) ||
+        (rc = conv_bwd(u, Ls[4], u->p2, nullptr, u->g_p2, false, nullptr, false, H / 4, W / 4)) ||
+        (rc = mask_of(u, 3, u->s2, P1, m)))
         return rc;
-    if ((rc = conv_bwd(u, Ls[4], u->g_a3, u->a3, u->p2, nullptr, u->g_p2, false, nullptr, false, H / 4, W / 4)))
+    UCHK(u, launch_pool_bwd(u->g_p2, n, H / 2, W / 2, 2 * C, u->g_s2, true, m, u->stream));
+    if ((rc = mask_of(u, 2, u->a2, P1, m)) ||
+        (rc = conv_bwd(u, Ls[3], u->a2, nullptr, u->g_a2, false, nullptr, false, H / 2, W / 2, m)) ||
+        (rc = conv_bwd(u, Ls[2], u->p1, nullptr, u->g_p1, false, nullptr, false, H / 2, W / 2)) ||
+        (rc = mask_of(u, 1, u->s1, P0, m)))
         return rc;
-    UCHK(u, launch_pool_bwd(u->g_p2, n, H / 2, W / 2, 2 * C, u->g_s2, true, u->stream));
-    if ((rc = conv_bwd(u, Ls[3], u->g_s2, u->s2, u->a2, nullptr, u->g_a2, false, nullptr, false, H / 2, W / 2)))
+    UCHK(u, launch_pool_bwd(u->g_p1, n, H, W, C, u->g_s1, true, m, u->stream));
+    if ((rc = mask_of(u, 0, u->a1, P0, m)) ||
+        (rc = conv_bwd(u, Ls[1], u->a1, nullptr, u->g_a1, false, nullptr, false, H, W, m)) ||
+        (rc = conv_bwd(u, Ls[0], u->in0, nullptr, nullptr, false, nullptr, false, H, W)))
         return rc;
-    if ((rc = conv_bwd(u, Ls[2], u->g_a2, u->a2, u->p1, nullptr, u->g_p1, false, nullptr, false, H / 2, W / 2)))
-        return rc;
-    UCHK(u, launch_pool_bwd(u->g_p1, n, H, W, C, u->g_s1, true, u->stream));
-    if ((rc = conv_bwd(u, Ls[1], u->g_s1, u->s1, u->a1, nullptr, u->g_a1, false, nullptr, false, H, W))) return rc;
-    if ((rc = conv_bwd(u, Ls[0], u->g_a1, u->a1, u->in0, nullptr, nullptr, false, nullptr, false, H, W))) return rc;
     // the weight gradients ran in layer order on the side stream: the last one closes it
     UCHK(u, hipStreamWaitEvent(u->stream, u->evWgrad[0], 0));
     return WC_OK;
