@@ -195,7 +195,8 @@ struct MaskOut {
     uint16_t *gm;
     const uint16_t *y;
 };
-// dW[co][tap][ci] += sum_p dY[p][co] * (Y[p][co] > 0) * X[p + tap][ci]
+// dW[co][tap][ci] = sum_p dY[p][co] * (Y[p][co] > 0) * X[p + tap][ci] (dW and db
+// are written, not accumulated)
 struct WgradArgs {
     const float *dy;
     const uint16_t *y;
@@ -211,19 +212,48 @@ struct WgradArgs {
     size_t partCap;
     // the masked output gradient already in BF16 (or nullptr: dy and y)
     const uint16_t *dyb;
-    // the bias gradient (optional, with dyb): db[co] += sum of dyb over the
+    // the bias gradient (optional, with dyb): db[co] = sum of dyb over the
     // pixels.  The tiled path sums it beside dW and needs db == dw + cout 9
-    // cin (the parameter layout); the split-K path adds the mask launch's
-    // partials bpart[bnb][cout].
+    // cin (the parameter layout); the split-K path sums the partials
+    // bpart[bnb][cout] of the masked gradient.
     float *db;
     const float *bpart;
     int bnb;
 };
+// Tiled weight gradients of several layers in one launch, then their slice
+// sums in one more: each job has its own partial slices (a.part) and its
+// workgroup range [first, next job's first); wgrad_plan fills a job.
+constexpr int kWgradBatchMax = 6;
+struct WgradJob {
+    WgradArgs a;
+    int R, rowsPerImg, stages, zn, gx, gy;
+    int first;
+};
+struct WgradJobs {
+    int nj;
+    WgradJob job[kWgradBatchMax];
+};
+struct SliceSumJob {
+    const float4 *part;
+    float4 *dw;
+    size_t E4;
+    int zn, zg, first;
+};
+struct SliceSumJobs {
+    int n;
+    SliceSumJob s[kWgradBatchMax];
+};
+// The tiling of layer a (whose a.part is not yet set): false when the tiled
+// path does not apply; *partFloats receives the partial floats it needs.
+bool wgrad_plan(const WgradArgs &a, WgradJob &j, size_t *partFloats);
+// b.job[0 .. b.nj) planned, each a.part set: the weight and bias gradients
+// into each job's dW and db (written, not accumulated)
+hipError_t launch_wgrad_batch(WgradJobs &b, hipStream_t s);
 // floats of workspace that launch_wgrad's tiled path may use
 constexpr size_t kWgradPartFloats = (size_t)1024 * 9 * 1024;
 // A sum over workgroups that an earlier launch left as partials, finished by
 // one workgroup of a later launch in a fixed order (so the result does not
-// depend on scheduling): out[c] += scale * sum_{b < nb} part[b * stride + c]
+// depend on scheduling): out[c] = scale * sum_{b < nb} part[b * stride + c]
 // for c < m (m <= 256).  part null: no job.
 struct PartialSums {
     const float *part;

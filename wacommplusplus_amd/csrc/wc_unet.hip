@@ -47,11 +47,6 @@ union Pack8 {
     bf16x8 v;
 };
 
-// A PartialSums job by one workgroup of 256, in a fixed order: thread
-// c + m s sums the partials of workgroups s, s + S, s + 2S, ... (S = 256 / m)
-// of value c into four running sums (every fourth term each, so four loads
-// are in flight), then thread c adds the S segment sums in order.  The
-// partials come from an earlier launch on the stream.
 // bf16(y > 0 ? g : 0) of four channels (y: four BF16 values in a uint2)
 __device__ __forceinline__ uint2 mask4(const float4 &g, const uint2 &y) {
     const uint16_t h0 = to_bf16(from_bf16((uint16_t)y.x) > 0.f ? g.x : 0.f);
@@ -61,6 +56,11 @@ __device__ __forceinline__ uint2 mask4(const float4 &g, const uint2 &y) {
     return make_uint2((uint32_t)h0 | (uint32_t)h1 << 16, (uint32_t)h2 | (uint32_t)h3 << 16);
 }
 
+// A PartialSums job by one workgroup of 256, in a fixed order: thread
+// c + m s sums the partials of workgroups s, s + S, s + 2S, ... (S = 256 / m)
+// of value c into four running sums (every fourth term each, so four loads
+// are in flight), then thread c adds the S segment sums in order.  The
+// partials come from an earlier launch on the stream.
 __device__ void fixed_order_sum(const PartialSums &j) {
     __shared__ float seg[256];
     const int m = j.m, S = 256 / m, t = threadIdx.x;
@@ -82,7 +82,7 @@ __device__ void fixed_order_sum(const PartialSums &j) {
     if (t < m) {
         float tot = 0.f;
         for (int q = 0; q < S; ++q) tot += seg[q * m + t];
-        j.out[t] += j.scale * tot;
+        j.out[t] = j.scale * tot;
     }
 }
 
@@ -508,10 +508,11 @@ __device__ __forceinline__ void transpose8_store(const uint4 (&in)[8], uint16_t 
     }
 }
 
-__global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int rowsPerImg, int stages) {
-    extern __shared__ __align__(16) uint16_t smem[];
+// Workgroup (bx, by, bz) of zn slices; smem: the dynamic LDS of the launch.
+__device__ __forceinline__ void wgrad_tile_body(const WgradArgs &a, int R, int rowsPerImg, int stages, int bx, int by,
+                                                int bz, int zn, uint16_t *smem) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int co0 = blockIdx.x * 32, ci0 = blockIdx.y * 32;
+    const int co0 = bx * 32, ci0 = by * 32;
     const int W = a.w, H = a.h, hw = H * W;
     const int cin = a.c1 + a.c2;
     const bool first = ci0 < a.c1;
@@ -526,9 +527,9 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
     for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     // bias gradient (workgroups of the first input-channel block): channel
     // tid / 8 of the block, 16-byte groups tid % 8, tid % 8 + 8, ... of its row
-    const bool doBias = a.db && blockIdx.y == 0;
+    const bool doBias = a.db && by == 0;
     float bsum = 0.f;
-    for (int st = blockIdx.z; st < stages; st += gridDim.z) {
+    for (int st = bz; st < stages; st += zn) {
         const int img = st / rowsPerImg, y0 = (st - img * rowsPerImg) * R;
         const int rv = min(R, H - y0);                 // rows of this stage inside the image
         __syncthreads();                               // the previous stage's reads are done
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
     // D[4 kq + r][row]: co = co0 + 16 ti + 4 kq + r, ci = ci0 + 16 tj + row.
     // Slice z holds dW [cout][9][cin] then the bias gradient [cout].
     const size_t E = (size_t)a.cout * 9 * cin;
-    float *dst = a.part + (size_t)blockIdx.z * (E + (a.db ? a.cout : 0));
+    float *dst = a.part + (size_t)bz * (E + (a.db ? a.cout : 0));
     for (int t = 0; t < 9; ++t)
         for (int r = 0; r < 4; ++r) {
             const int co = co0 + ti * 16 + kq * 4 + r, ci = ci0 + tj * 16 + row;
@@ -608,17 +609,33 @@ __global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int
     }
 }
 
-// Deterministic reduction of part[zn][E] into dW in one launch: dW[e] +=
-// sum_z part[z][e].  A workgroup takes `cols` float4 columns (4 cols
-// consecutive elements) and zg = 256 / cols groups of slices; work-item
+__global__ __launch_bounds__(256) void wgrad_tile_kernel(WgradArgs a, int R, int rowsPerImg, int stages) {
+    extern __shared__ __align__(16) uint16_t smem[];
+    wgrad_tile_body(a, R, rowsPerImg, stages, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, smem);
+}
+
+// Several layers' weight gradients in one launch: job j takes workgroups
+// [job[j].first, job[j + 1].first), laid out as its own (gx, gy, zn) grid.
+__global__ __launch_bounds__(256) void wgrad_batch_kernel(WgradJobs b) {
+    extern __shared__ __align__(16) uint16_t smem[];
+    int j = 0;
+    while (j + 1 < b.nj && (int)blockIdx.x >= b.job[j + 1].first) ++j;
+    const WgradJob &J = b.job[j];
+    const int w = (int)blockIdx.x - J.first, bx = w % J.gx, byz = w / J.gx;
+    wgrad_tile_body(J.a, J.R, J.rowsPerImg, J.stages, bx, byz % J.gy, byz / J.gy, J.zn, smem);
+}
+
+// Deterministic reduction of part[zn][E] into dW: dW[e] = sum_z part[z][e]
+// (stored: dW needs no zeroing).  A workgroup takes `cols` float4 columns (4
+// cols consecutive elements) and zg = 256 / cols groups of slices; work-item
 // (col, g) sums slices g, g + zg, g + 2 zg, ... in that order (eight loads
 // in flight), then the zg group sums of a column are added in group order.
 // The order depends only on zn and zg, so the result is reproducible.
-__global__ __launch_bounds__(256) void wgrad_sum_slices_kernel(const float4 *__restrict__ part, int zn, int zg,
-                                                               size_t E4, float4 *__restrict__ dw) {
+__device__ __forceinline__ void slice_sum_body(const float4 *__restrict__ part, int zn, int zg, size_t E4,
+                                               float4 *__restrict__ dw, unsigned blk) {
     __shared__ float4 sums[256];
     const int tid = threadIdx.x, cols = 256 / zg, col = tid % cols, g = tid / cols;
-    const size_t e4 = (size_t)blockIdx.x * cols + col;
+    const size_t e4 = (size_t)blk * cols + col;
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     if (e4 < E4) {
         const float4 *p = part + e4;
@@ -652,16 +669,24 @@ __global__ __launch_bounds__(256) void wgrad_sum_slices_kernel(const float4 *__r
             t.z += v.z;
             t.w += v.w;
         }
-        float4 o = dw[e4];
-        o.x += t.x;
-        o.y += t.y;
-        o.z += t.z;
-        o.w += t.w;
-        dw[e4] = o;
+        dw[e4] = t;
     }
 }
 
-// db[c] += sum_b part[b][c]: one workgroup per channel, a fixed-order tree
+__global__ __launch_bounds__(256) void wgrad_sum_slices_kernel(const float4 *__restrict__ part, int zn, int zg,
+                                                               size_t E4, float4 *__restrict__ dw) {
+    slice_sum_body(part, zn, zg, E4, dw, blockIdx.x);
+}
+
+// the slice sums of a batch: entry j takes workgroups [s[j].first, s[j + 1].first)
+__global__ __launch_bounds__(256) void wgrad_sum_batch_kernel(SliceSumJobs b) {
+    int j = 0;
+    while (j + 1 < b.n && (int)blockIdx.x >= b.s[j + 1].first) ++j;
+    const SliceSumJob &S = b.s[j];
+    slice_sum_body(S.part, S.zn, S.zg, S.E4, S.dw, blockIdx.x - S.first);
+}
+
+// db[c] = sum_b part[b][c]: one workgroup per channel, a fixed-order tree
 // (the bias gradient beside the split-K weight gradient)
 __global__ __launch_bounds__(256) void bias_reduce_kernel(const float *__restrict__ part, int nb, int cout,
                                                           float *__restrict__ db) {
@@ -675,7 +700,7 @@ __global__ __launch_bounds__(256) void bias_reduce_kernel(const float *__restric
         if (tid < h) red[tid] += red[tid + h];
         __syncthreads();
     }
-    if (tid == 0) db[c] += red[0];
+    if (tid == 0) db[c] = red[0];
 }
 
 // The ReLU-masked output gradient of a layer in BF16 (dy null: read from
@@ -1125,6 +1150,52 @@ bool wgrad_is_tiled(const WgradArgs &a) {
     return wgrad_tiling(a, t);
 }
 
+bool wgrad_plan(const WgradArgs &a, WgradJob &j, size_t *partFloats) {
+    WgradArgs probe = a;
+    float dummy;
+    if (!probe.part) probe.part = &dummy;              // the tiling only asks that a workspace exist
+    probe.partCap = (size_t)-1;
+    WgradTiling t;
+    if (!wgrad_tiling(probe, t)) return false;
+    j.a = a;
+    j.R = t.R; j.rowsPerImg = t.rowsPerImg; j.stages = t.stages; j.zn = t.zn;
+    j.gx = a.cout / 32; j.gy = (a.c1 + a.c2) / 32;
+    j.first = 0;
+    *partFloats = t.E * t.zn;
+    return true;
+}
+
+hipError_t launch_wgrad_batch(WgradJobs &b, hipStream_t s) {
+    if (b.nj < 1 || b.nj > kWgradBatchMax) return hipErrorInvalidValue;
+    SliceSumJobs sums{};
+    sums.n = b.nj;
+    int wgs = 0, sumBlocks = 0;
+    size_t lds = 0;
+    for (int q = 0; q < b.nj; ++q) {
+        WgradJob &j = b.job[q];
+        const WgradArgs &a = j.a;
+        const int cin = a.c1 + a.c2;
+        WgradTiling t;
+        if ((cin % 32) || (a.cout % 32) || !wgrad_tiling(a, t) || t.zn != j.zn) return hipErrorInvalidValue;
+        if (a.db && a.db != a.dw + (t.E - a.cout)) return hipErrorInvalidValue;
+        if (((uintptr_t)a.dw | (uintptr_t)a.part) & 15) return hipErrorInvalidValue;
+        j.first = wgs;
+        wgs += j.gx * j.gy * j.zn;
+        lds = t.lds > lds ? t.lds : lds;
+        int zg = 1;                                    // slice groups: the largest power of two <= min(16, zn)
+        while (zg * 2 <= 16 && zg * 2 <= t.zn) zg *= 2;
+        const size_t E4 = t.E / 4, cols = 256 / zg;
+        sums.s[q] = SliceSumJob{reinterpret_cast<const float4 *>(a.part), reinterpret_cast<float4 *>(a.dw), E4, t.zn, zg,
+                                sumBlocks};
+        sumBlocks += (int)((E4 + cols - 1) / cols);
+    }
+    wgrad_batch_kernel<<<wgs, 256, lds, s>>>(b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    wgrad_sum_batch_kernel<<<sumBlocks, 256, 0, s>>>(sums);
+    return hipGetLastError();
+}
+
 hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
     const int cin = a.c1 + a.c2;
     if ((cin % 32) || (a.cout % 32) || a.splits < 1) return hipErrorInvalidValue;
@@ -1146,9 +1217,12 @@ hipError_t launch_wgrad(const WgradArgs &a, hipStream_t s) {
             reinterpret_cast<const float4 *>(a.part), t.zn, zg, E4, reinterpret_cast<float4 *>(a.dw));
         return hipGetLastError();
     }
+    // the split-K kernel adds with atomics: dW starts from zero
+    hipError_t e = launch_fill_f32(a.dw, (long long)a.cout * 9 * cin, 0.f, s);
+    if (e != hipSuccess) return e;
     const dim3 grid((unsigned)(a.cout / 32), (unsigned)(cin / 32), (unsigned)(9 * a.splits));
     wgrad_kernel<<<grid, 256, 0, s>>>(a);
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess || !a.db) return e;
     if (!a.dyb || !a.bpart || a.bnb < 1) return hipErrorInvalidValue;
     bias_reduce_kernel<<<a.cout, 256, 0, s>>>(a.bpart, a.bnb, a.cout, a.db);

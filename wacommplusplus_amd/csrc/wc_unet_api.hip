@@ -38,6 +38,13 @@ struct Layer {
 constexpr int kBase = 32;
 constexpr size_t kBiasPartFloats = (size_t)1024 * 256;       // mask workgroups x channels
 constexpr size_t kHeadPartFloats = (size_t)256 * kHeadPartials;
+// weight-gradient partial slices of one batch of layers (the batches run in
+// order on one stream, so each reuses the space)
+constexpr size_t kUnetPartFloats = 3 * kWgradPartFloats;
+// The tiled weight gradients go out in batches, each launched once the
+// masked gradient of its last layer is ready: layers 13-10, 9-5 and 4-0
+// (backward order).  A batch also closes when it is full or out of space.
+bool batch_ends_at(int li) { return li == 10 || li == 5 || li == 0; }
 
 }  // namespace
 
@@ -56,16 +63,23 @@ struct wc_unet {
     uint16_t *d_wb = nullptr, *d_wf = nullptr;     // BF16 weights, and flipped for the data gradient
     WeightMap wmap{};
     float *d_loss = nullptr;
-    float *d_part = nullptr;                     // weight-gradient partial sums (kWgradPartFloats)
+    float *d_part = nullptr;                     // weight-gradient partial sums (kUnetPartFloats)
+    // tiled weight gradients waiting for their batch launch on `side`: the
+    // jobs, the partial floats they take of d_part, the last one's layer
+    WgradJobs batch{};
+    size_t batchFloats = 0;
+    int batchLast = -1;
     float *d_bpart = nullptr;                    // bias-gradient partials, kBiasPartFloats per layer
     float *d_hpart = nullptr;                    // the head's partials (kHeadPartFloats)
     SideSums pending{};                          // sums the next mask launch finishes
     int adam_t = 0;
     double fwd_ms = 0;
-    // forward + backward of the gradient and training calls, captured once
-    // per batch shape (and loss kind) and replayed: ~110 launches per step
+    // forward and backward of the gradient and training calls, each captured
+    // once per batch shape (and loss kind) and replayed; between the two the
+    // targets are copied in on `side` (evUp), beside the forward pass
     bool useGraph = true;
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipEvent_t evUp = nullptr;
     int gN = 0, gH = 0, gW = 0;
     bool gWgt = false;
     // activations / gradients for the current shape
@@ -81,6 +95,11 @@ struct wc_unet {
     uint16_t *gmask[kMaxConvLayers] = {};
     size_t gmaskCap[kMaxConvLayers] = {};
     float *x = nullptr, *out = nullptr, *tgt = nullptr;
+    // pinned host staging of a batch, [x (P cin) | t (P) | wgt (P)]: the
+    // caller's arrays are copied in on the host, then to the device
+    // asynchronously (a copy from pageable memory would hold the host until
+    // the device is idle)
+    float *h_stage = nullptr;
     float *wgt = nullptr;     // per-pixel loss weights of the current batch
     bool useWgt = false;
     int cin = 1;                // input channels (x holds channel 0, xc all of them when cin > 1)
@@ -114,14 +133,18 @@ int ufail(wc_unet *u, int code, const char *fmt, ...) {
     } while (0)
 
 void drop_graph(wc_unet *u) {
-    if (u->gexec) hipGraphExecDestroy(u->gexec);
-    u->gexec = nullptr;
+    for (hipGraphExec_t &g : u->gexec) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 void free_shape(wc_unet *u) {
     drop_graph(u);           // its kernels point into the buffers freed below
     for (void *b : u->bufs) hipFree(b);
     u->bufs.clear();
+    if (u->h_stage) hipHostFree(u->h_stage);
+    u->h_stage = nullptr;
     u->n = u->h = u->w = 0;
 }
 
@@ -160,6 +183,10 @@ int ensure_shape(wc_unet *u, int n, int h, int w) {
     }
     rc |= alloc(u, u->x, P0) | alloc(u, u->out, P0) | alloc(u, u->tgt, P0) | alloc(u, u->wgt, P0);
     if (u->cin > 1) rc |= alloc(u, u->xc, P0 * u->cin);
+    if (!rc && hipHostMalloc((void **)&u->h_stage, P0 * (u->cin + 2) * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+        u->h_stage = nullptr;
+        rc = ufail(u, WC_ERR_NOMEM, "pinned staging of %zu bytes", P0 * (u->cin + 2) * sizeof(float));
+    }
     rc |= alloc(u, u->g_d1, P0 * C) | alloc(u, u->g_d1a, P0 * C) | alloc(u, u->g_s1, P0 * C) |
           alloc(u, u->g_u1, P0 * 2 * C);
     rc |= alloc(u, u->g_d2, P1 * 2 * C) | alloc(u, u->g_d2a, P1 * 2 * C) | alloc(u, u->g_s2, P1 * 2 * C) |
@@ -197,11 +224,28 @@ int mask_of(wc_unet *u, int li, const uint16_t *y, long long P, MaskOut &m) {
     return WC_OK;
 }
 
+// The batched weight gradients on the side stream, after the mask of the
+// batch's last layer (and every earlier one: the main stream is in order);
+// sums left pending (the head's) are finished first.
+int flush_wgrad(wc_unet *u) {
+    if (u->batch.nj == 0) return WC_OK;
+    UCHK(u, hipStreamWaitEvent(u->side, u->evMask[u->batchLast], 0));
+    UCHK(u, launch_finish_sums(u->pending, u->side));
+    u->pending = SideSums{};
+    UCHK(u, launch_wgrad_batch(u->batch, u->side));
+    UCHK(u, hipEventRecord(u->evWgrad[u->batchLast], u->side));
+    u->batch = WgradJobs{};
+    u->batchFloats = 0;
+    u->batchLast = -1;
+    return WC_OK;
+}
+
 // backward of layer L, whose ReLU-masked output gradient u->gmask[li] its
-// producer has written: the weight and bias gradients (side stream); when
-// gx1 != null the input gradients (channels [0, c1) to gx1, [c1, c1+c2) to
-// gx2, stored or accumulated), or, with next.gm set, the previous layer's
-// masked gradient in place of gx1 (a single input, stored).
+// producer has written: the weight and bias gradients (side stream, batched
+// with neighbouring layers where the tiled path applies); when gx1 != null
+// the input gradients (channels [0, c1) to gx1, [c1, c1+c2) to gx2, stored
+// or accumulated), or, with next.gm set, the previous layer's masked
+// gradient in place of gx1 (a single input, stored).
 int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2, float *gx1, bool acc1, float *gx2,
              bool acc2, int h, int w, const MaskOut &next = MaskOut{}) {
     const long long P = (long long)u->n * h * w;
@@ -210,20 +254,19 @@ int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2,
     if (next.gm && (!gx1 || li == 0 || L.c2 || acc1 || u->layers[li - 1].cout != L.c1))
         return ufail(u, WC_ERR_INVALID, "layer %d: a masked input gradient needs one stored input", li);
     uint16_t *gm = u->gmask[li];
-    float *bpart = u->d_bpart + li * kBiasPartFloats;
     WgradArgs g{};
     g.x1 = x1; g.x2 = x2; g.c1 = L.c1; g.c2 = L.c2; g.cout = L.cout;
     g.dw = u->d_g + L.w_off; g.n = u->n; g.h = h; g.w = w;
     long long sp = P / 2048;
     g.splits = (int)(sp < 1 ? 1 : (sp > 1024 ? 1024 : sp));
-    g.part = u->d_part;
-    g.partCap = kWgradPartFloats;
     // the masked gradient in BF16 feeds the weight gradient and the data
     // gradient; the bias gradient is summed by the tiled weight gradient, or
     // from per-workgroup partials of the masked gradient (split-K path)
     g.dyb = gm;
     g.db = u->d_g + L.b_off;
-    const bool tiled = wgrad_is_tiled(g);
+    WgradJob job{};
+    size_t need = 0;
+    const bool tiled = wgrad_plan(g, job, &need) && need <= kUnetPartFloats;
     UCHK(u, hipEventRecord(u->evMask[li], u->stream));
     // the data gradient first: it is recorded as the mask's first dependent,
     // so a replayed graph keeps the data-gradient chain on one queue
@@ -236,14 +279,28 @@ int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2,
         a.mo = next.gm; a.my = next.y;
         UCHK(u, launch_conv3x3(a, false, u->stream));
     }
+    int rc;
+    if (tiled) {
+        if (u->batch.nj == kWgradBatchMax || u->batchFloats + need > kUnetPartFloats) {
+            if ((rc = flush_wgrad(u))) return rc;
+        }
+        job.a.part = u->d_part + u->batchFloats;
+        job.a.partCap = kUnetPartFloats - u->batchFloats;
+        u->batch.job[u->batch.nj++] = job;
+        u->batchFloats += need;
+        u->batchLast = li;
+        return batch_ends_at(li) ? flush_wgrad(u) : WC_OK;
+    }
+    // split-K path, in layer order after the batch so far
+    if ((rc = flush_wgrad(u))) return rc;
+    float *bpart = u->d_bpart + li * kBiasPartFloats;
     UCHK(u, hipStreamWaitEvent(u->side, u->evMask[li], 0));
-    // sums an earlier launch left as partials (the head's)
     UCHK(u, launch_finish_sums(u->pending, u->side));
     u->pending = SideSums{};
     int nb = 0;
-    if (!tiled)
-        UCHK(u, launch_mask_bf16(nullptr, nullptr, P * L.cout, L.cout, gm, bpart, kBiasPartFloats, &nb, SideSums{},
-                                 u->side));
+    UCHK(u, launch_mask_bf16(nullptr, nullptr, P * L.cout, L.cout, gm, bpart, kBiasPartFloats, &nb, SideSums{},
+                             u->side));
+    g.part = nullptr;               // the atomic path
     g.bpart = bpart;
     g.bnb = nb;
     UCHK(u, launch_wgrad(g, u->side));
@@ -286,14 +343,17 @@ int forward(wc_unet *u) {
     return WC_OK;
 }
 
-// gradients of the mean squared error into d_g (zeroed first); loss in d_loss
+// gradients of the mean squared error into d_g; loss in d_loss
 int backward(wc_unet *u) {
     const int H = u->h, W = u->w, C = kBase, n = u->n;
     const auto &Ls = u->layers;
     int rc;
     u->pending = SideSums{};
-    UCHK(u, launch_fill_f32(u->d_g, (long long)u->nparam, 0.f, u->stream));
-    UCHK(u, launch_fill_f32(u->d_loss, 1, 0.f, u->stream));
+    u->batch = WgradJobs{};
+    u->batchFloats = 0;
+    u->batchLast = -1;
+    // every gradient and the loss are written (not accumulated) by the
+    // launches below: no zeroing first
     const long long P0 = (long long)n * H * W, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
     // each layer's masked output gradient is written by the kernel that
     // produces the gradient: the head, a data-gradient convolution, an
@@ -353,49 +413,80 @@ int backward(wc_unet *u) {
         (rc = conv_bwd(u, Ls[0], u->in0, nullptr, nullptr, false, nullptr, false, H, W)))
         return rc;
     // the weight gradients ran in layer order on the side stream: the last one closes it
+    if ((rc = flush_wgrad(u))) return rc;
     UCHK(u, hipStreamWaitEvent(u->stream, u->evWgrad[0], 0));
     return WC_OK;
 }
 
-// forward() then backward() on the current batch.  With useGraph the pair is
-// captured into a graph the first time a batch shape (and loss kind) is seen
-// and replayed afterwards: the same kernels with the same arguments, so the
-// results are those of direct launches.  The batch buffers are fixed per
-// shape and the uploads write into them before the launch.
-int forward_backward(wc_unet *u) {
-    if (!u->useGraph) {
-        const int rc = forward(u);
-        return rc ? rc : backward(u);
+// Captures body(u) on the main stream into a graph (`what` names it in errors).
+int capture(wc_unet *u, int (*body)(wc_unet *), hipGraphExec_t &out, const char *what) {
+    UCHK(u, hipStreamBeginCapture(u->stream, hipStreamCaptureModeThreadLocal));
+    int rc = body(u);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(u->stream, &g);        // ends the capture on every path
+    if (!rc && e != hipSuccess) rc = ufail(u, WC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (!rc && hipGraphInstantiate(&out, g, nullptr, nullptr, 0) != hipSuccess) {
+        out = nullptr;
+        rc = ufail(u, WC_ERR_HIP, "hipGraphInstantiate of the %s failed", what);
     }
-    if (!u->gexec || u->gN != u->n || u->gH != u->h || u->gW != u->w || u->gWgt != u->useWgt) {
-        drop_graph(u);
-        UCHK(u, hipStreamBeginCapture(u->stream, hipStreamCaptureModeThreadLocal));
-        int rc = forward(u);
-        if (!rc) rc = backward(u);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(u->stream, &g);    // ends the capture on every path
-        if (!rc && e != hipSuccess)
-            rc = ufail(u, WC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-        if (!rc && hipGraphInstantiate(&u->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
-            u->gexec = nullptr;
-            rc = ufail(u, WC_ERR_HIP, "hipGraphInstantiate of the training step failed");
-        }
-        if (g) hipGraphDestroy(g);
-        if (rc) return rc;
-        u->gN = u->n; u->gH = u->h; u->gW = u->w; u->gWgt = u->useWgt;
-    }
-    UCHK(u, hipGraphLaunch(u->gexec, u->stream));
-    return WC_OK;
+    if (g) hipGraphDestroy(g);
+    if (!rc) UCHK(u, hipGraphUpload(out, u->stream));
+    return rc;
 }
 
-int upload(wc_unet *u, const float *x, const float *t, const float *wgt, int n, int h, int w) {
+// The batch's inputs (and the loss kind): into the per-shape buffers on the
+// main stream, before the forward pass.
+int upload_inputs(wc_unet *u, const float *x, const float *wgt, int n, int h, int w) {
     int rc = ensure_shape(u, n, h, w);
     if (rc) return rc;
     const size_t P = (size_t)n * h * w;
-    UCHK(u, hipMemcpyAsync(u->cin > 1 ? u->xc : u->x, x, P * u->cin * 4, hipMemcpyHostToDevice, u->stream));
-    if (t) UCHK(u, hipMemcpyAsync(u->tgt, t, P * 4, hipMemcpyHostToDevice, u->stream));
+    // the previous call's copies out of the staging are complete: every
+    // call ends with the main stream synchronized, after the side stream's copies
+    std::memcpy(u->h_stage, x, P * u->cin * sizeof(float));
+    UCHK(u, hipMemcpyAsync(u->cin > 1 ? u->xc : u->x, u->h_stage, P * u->cin * 4, hipMemcpyHostToDevice, u->stream));
     u->useWgt = wgt != nullptr;
-    if (wgt) UCHK(u, hipMemcpyAsync(u->wgt, wgt, P * 4, hipMemcpyHostToDevice, u->stream));
+    return WC_OK;
+}
+
+// The targets (and loss weights), needed from the head's backward on: copied
+// on the side stream while the forward pass runs; the main stream waits.
+int upload_targets(wc_unet *u, const float *t, const float *wgt) {
+    const size_t P = (size_t)u->n * u->h * u->w;
+    float *st = u->h_stage + P * u->cin, *sw = st + P;
+    std::memcpy(st, t, P * sizeof(float));
+    UCHK(u, hipMemcpyAsync(u->tgt, st, P * 4, hipMemcpyHostToDevice, u->side));
+    if (u->useWgt) {
+        std::memcpy(sw, wgt, P * sizeof(float));
+        UCHK(u, hipMemcpyAsync(u->wgt, sw, P * 4, hipMemcpyHostToDevice, u->side));
+    }
+    UCHK(u, hipEventRecord(u->evUp, u->side));
+    UCHK(u, hipStreamWaitEvent(u->stream, u->evUp, 0));
+    return WC_OK;
+}
+
+// forward(), the target upload, then backward() on the current batch.  With
+// useGraph the two passes are captured into graphs the first time a batch
+// shape (and loss kind) is seen and replayed afterwards: the same kernels
+// with the same arguments, so the results are those of direct launches.  The
+// batch buffers are fixed per shape and the uploads write into them.
+int forward_backward(wc_unet *u, const float *t, const float *wgt) {
+    int rc;
+    if (!u->useGraph) {
+        if ((rc = forward(u)) || (rc = upload_targets(u, t, wgt))) return rc;
+        return backward(u);
+    }
+    if (!u->gexec[0] || u->gN != u->n || u->gH != u->h || u->gW != u->w || u->gWgt != u->useWgt) {
+        drop_graph(u);
+        if ((rc = capture(u, forward, u->gexec[0], "forward pass")) ||
+            (rc = capture(u, backward, u->gexec[1], "backward pass"))) {
+            drop_graph(u);
+            return rc;
+        }
+        u->gN = u->n; u->gH = u->h; u->gW = u->w; u->gWgt = u->useWgt;
+    }
+    UCHK(u, hipGraphLaunch(u->gexec[0], u->stream));
+    if ((rc = upload_targets(u, t, wgt))) return rc;
+    UCHK(u, hipGraphLaunch(u->gexec[1], u->stream));
     return WC_OK;
 }
 
@@ -474,10 +565,11 @@ int wc_unet_create_ex(int device, uint64_t seed, int channels, wc_unet **out) {
         if (hipEventCreateWithFlags(&u->evMask[l], hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&u->evWgrad[l], hipEventDisableTiming) != hipSuccess)
             return bad(WC_ERR_HIP);
+    if (hipEventCreateWithFlags(&u->evUp, hipEventDisableTiming) != hipSuccess) return bad(WC_ERR_HIP);
     if (hipMalloc(&u->d_p, u->nparam * 4) || hipMalloc(&u->d_g, u->nparam * 4) || hipMalloc(&u->d_m, u->nparam * 4) ||
         hipMalloc(&u->d_v, u->nparam * 4) || hipMalloc(&u->d_wb, u->nwb * 2 + 256) ||
         hipMalloc(&u->d_wf, u->nwb * 2 + 256) || hipMalloc(&u->d_loss, 256) ||
-        hipMalloc(&u->d_part, kWgradPartFloats * sizeof(float)) ||
+        hipMalloc(&u->d_part, kUnetPartFloats * sizeof(float)) ||
         hipMalloc(&u->d_bpart, kMaxConvLayers * kBiasPartFloats * sizeof(float)) || hipMalloc(&u->d_hpart, kHeadPartFloats * sizeof(float)))
         return bad(WC_ERR_NOMEM);
     if (hipMemcpy(u->d_p, p.data(), u->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return bad(WC_ERR_HIP);
@@ -501,6 +593,7 @@ int wc_unet_destroy(wc_unet *u) {
         if (u->evMask[l]) hipEventDestroy(u->evMask[l]);
         if (u->evWgrad[l]) hipEventDestroy(u->evWgrad[l]);
     }
+    if (u->evUp) hipEventDestroy(u->evUp);
     if (u->side) hipStreamDestroy(u->side);
     if (u->stream) hipStreamDestroy(u->stream);
     delete u;
@@ -545,7 +638,7 @@ int wc_unet_set_params(wc_unet *u, const float *p) {
 int wc_unet_forward(wc_unet *u, const float *x, int n, int h, int w, float *y, double *device_ms) {
     if (!u || !x || !y) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
-    int rc = upload(u, x, nullptr, nullptr, n, h, w);
+    int rc = upload_inputs(u, x, nullptr, n, h, w);
     if (rc) return rc;
     hipEvent_t e0, e1;
     UCHK(u, hipEventCreate(&e0));
@@ -574,8 +667,8 @@ int wc_unet_gradients_weighted(wc_unet *u, const float *x, const float *t, const
                                float *grad, float *loss) {
     if (!u || !x || !t) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
-    int rc = upload(u, x, t, weight, n, h, w);
-    if (!rc) rc = forward_backward(u);
+    int rc = upload_inputs(u, x, weight, n, h, w);
+    if (!rc) rc = forward_backward(u, t, weight);
     if (rc) return rc;
     if (grad) UCHK(u, hipMemcpyAsync(grad, u->d_g, u->nparam * 4, hipMemcpyDeviceToHost, u->stream));
     float l = 0;
@@ -593,8 +686,8 @@ int wc_unet_train_step_weighted(wc_unet *u, const float *x, const float *t, cons
                                 float lr, float *loss) {
     if (!u || !x || !t || !(lr > 0)) return WC_ERR_INVALID;
     UCHK(u, hipSetDevice(u->device));
-    int rc = upload(u, x, t, weight, n, h, w);
-    if (!rc) rc = forward_backward(u);
+    int rc = upload_inputs(u, x, weight, n, h, w);
+    if (!rc) rc = forward_backward(u, t, weight);
     if (rc) return rc;
     u->adam_t += 1;
     UCHK(u, launch_adam_bf16(u->d_p, u->d_g, u->d_m, u->d_v, (long long)u->nparam, lr, 0.9f, 0.999f, 1e-8f,
