@@ -187,6 +187,12 @@ struct ConvArgs {
     // still needed as FP32 scratch where the general kernel runs
     uint16_t *mo;
     const uint16_t *my;
+    // with mo: this layer's input was the 2x2 average pool of the previous
+    // layer's output, whose gradient is pg (FP32, twice the height and width)
+    // plus a quarter of this one in each pooled pixel; mo, my and pg are at
+    // that resolution (pg is read and, by the general kernel's second launch,
+    // also updated)
+    const float *pg;
 };
 // A layer's ReLU-masked output gradient in BF16, written by the kernel that
 // produces the gradient: gm[e] = bf16(y[e] > 0 ? g[e] : 0) in place of the

@@ -11,10 +11,13 @@
 // on v_mfma_f32_16x16x32_bf16.  The same products serve
 //   - forward: two inputs concatenated along channels (decoder skip + up),
 //     bias + ReLU, BF16 out;
-//   - input gradient: the ReLU-masked output gradient (BF16, made once per
-//     layer by mask_bf16_kernel) convolved with the flipped, transposed
-//     weights, FP32 out split over the concat's two inputs, stored or
-//     accumulated;
+//   - input gradient: the ReLU-masked output gradient (BF16, written once
+//     per layer by the kernel that produces it: the head's backward, the
+//     next layer's input-gradient epilogue, or an upsampling's backward)
+//     convolved with the flipped, transposed weights, FP32 out split over
+//     the concat's two inputs, stored or accumulated, or the previous
+//     layer's masked gradient directly (through a pooling's backward where
+//     the layer's input was pooled);
 //   - weight gradient: dW[co][tap][ci] = sum_p dYm[p][co] X[p + tap][ci], a
 //     GEMM whose reduction runs over pixels.
 // conv3x3_tile_kernel (forward, input gradient) and wgrad_tile_kernel
@@ -268,6 +271,27 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs &a, const f32x4 (&a
         __syncthreads();
         const bool first = co0 < a.cy1;
         const int cc = first ? a.cy1 : a.cy2, cl0 = first ? co0 : co0 - a.cy1;
+        if (a.mo && a.pg) {
+            // the input was the 2x2 average pool of the previous layer's
+            // output: each of its four pixels gets pg + g / 4, masked into mo
+            // (pg, mo, my at twice the height and width)
+            const int W2 = 2 * W;
+            for (int c = tid; c < qmax * 8; c += 256) {
+                const int q = c >> 3, part = c & 7;
+                if (co0 + 4 * part >= nout) continue;
+                const float4 v = *(const float4 *)(so + q * kCvOutF + 4 * part);
+                const float4 g = make_float4(0.25f * v.x, 0.25f * v.y, 0.25f * v.z, 0.25f * v.w);
+                const int r = q / W, x = q - r * W;
+                const size_t pix00 = ((size_t)img * 2 * a.h + 2 * (y0 + r)) * W2 + 2 * x;
+                for (int k = 0; k < 4; ++k) {
+                    const size_t o = (pix00 + (size_t)(k >> 1) * W2 + (k & 1)) * cc + cl0 + 4 * part;
+                    const float4 s = *(const float4 *)(a.pg + o);
+                    *(uint2 *)(a.mo + o) =
+                        mask4(make_float4(s.x + g.x, s.y + g.y, s.z + g.z, s.w + g.w), *(const uint2 *)(a.my + o));
+                }
+            }
+            return;
+        }
         if (a.mo) {                                       // the next layer's masked gradient (one output, stored)
             const uint16_t *my = a.my + p0 * cc + cl0;
             uint16_t *mo = a.mo + p0 * cc + cl0;
@@ -1085,6 +1109,7 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
     if ((a.c1 % 32) || (a.c2 % 32) || (nout % 32) || ((a.cy1 % 32) && a.cy2)) return hipErrorInvalidValue;
     // a masked BF16 output: the input gradient of a single-input layer, stored
     if (a.mo && (a.mode != 1 || a.cy2 || a.acc1 || !a.my || !a.y1)) return hipErrorInvalidValue;
+    if (a.pg && !a.mo) return hipErrorInvalidValue;
     if (!fp32_input && !a.m1) {
         // tiled path: R rows per workgroup (R * w <= 256), halved while the
         // grid has fewer than 512 workgroups, as long as the stage fits in LDS
@@ -1120,6 +1145,12 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
         conv3x3_kernel<float><<<grid, 256, 0, s>>>(a);
     else
         conv3x3_kernel<uint16_t><<<grid, 256, 0, s>>>(a);
+    if (a.pg) {                         // the pooling's backward, masked, by a second launch
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return launch_pool_bwd((const float *)a.y1, a.n, 2 * a.h, 2 * a.w, a.cy1, const_cast<float *>(a.pg), true,
+                               MaskOut{a.mo, a.my}, s);
+    }
     return mask_after((const float *)a.y1, P * a.cy1, a.cy1, MaskOut{a.mo, a.my}, s);
 }
 

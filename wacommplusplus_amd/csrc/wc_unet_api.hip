@@ -245,9 +245,12 @@ int flush_wgrad(wc_unet *u) {
 // with neighbouring layers where the tiled path applies); when gx1 != null
 // the input gradients (channels [0, c1) to gx1, [c1, c1+c2) to gx2, stored
 // or accumulated), or, with next.gm set, the previous layer's masked
-// gradient in place of gx1 (a single input, stored).
+// gradient in place of gx1 (a single input, stored).  pooledFrom: the input
+// was the 2x2 average pool of the previous layer's output, whose gradient
+// so far is pooledFrom (FP32); next.gm then receives that gradient plus the
+// pooling's backward of this one, masked (gx1: FP32 scratch).
 int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2, float *gx1, bool acc1, float *gx2,
-             bool acc2, int h, int w, const MaskOut &next = MaskOut{}) {
+             bool acc2, int h, int w, const MaskOut &next = MaskOut{}, const float *pooledFrom = nullptr) {
     const long long P = (long long)u->n * h * w;
     const int li = (int)(&L - u->layers.data());
     if ((size_t)P * L.cout > u->gmaskCap[li]) return ufail(u, WC_ERR_INVALID, "layer %d: mask buffer too small", li);
@@ -276,7 +279,7 @@ int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2,
         a.w_ = u->d_wf + L.wb_off; a.bias = nullptr;
         a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
         a.n = u->n; a.h = h; a.w = w;
-        a.mo = next.gm; a.my = next.y;
+        a.mo = next.gm; a.my = next.y; a.pg = pooledFrom;
         UCHK(u, launch_conv3x3(a, false, u->stream));
     }
     int rc;
@@ -391,23 +394,20 @@ int backward(wc_unet *u) {
     // bottleneck
     if ((rc = mask_of(u, 6, u->a4, P3, m)) ||
         (rc = conv_bwd(u, Ls[7], u->a4, nullptr, u->g_a4, false, nullptr, false, H / 8, W / 8, m)) ||
-        (rc = conv_bwd(u, Ls[6], u->p3, nullptr, u->g_p3, false, nullptr, false, H / 8, W / 8)) ||
-        (rc = mask_of(u, 5, u->s3, P2, m)))
+        (rc = mask_of(u, 5, u->s3, P2, m)) ||
+        (rc = conv_bwd(u, Ls[6], u->p3, nullptr, u->g_p3, false, nullptr, false, H / 8, W / 8, m, u->g_s3)))
         return rc;
-    UCHK(u, launch_pool_bwd(u->g_p3, n, H / 4, W / 4, 4 * C, u->g_s3, true, m, u->stream));
     // encoder
     if ((rc = mask_of(u, 4, u->a3, P2, m)) ||
         (rc = conv_bwd(u, Ls[5], u->a3, nullptr, u->g_a3, false, nullptr, false, H / 4, W / 4, m)) ||
-        (rc = conv_bwd(u, Ls[4], u->p2, nullptr, u->g_p2, false, nullptr, false, H / 4, W / 4)) ||
-        (rc = mask_of(u, 3, u->s2, P1, m)))
+        (rc = mask_of(u, 3, u->s2, P1, m)) ||
+        (rc = conv_bwd(u, Ls[4], u->p2, nullptr, u->g_p2, false, nullptr, false, H / 4, W / 4, m, u->g_s2)))
         return rc;
-    UCHK(u, launch_pool_bwd(u->g_p2, n, H / 2, W / 2, 2 * C, u->g_s2, true, m, u->stream));
     if ((rc = mask_of(u, 2, u->a2, P1, m)) ||
         (rc = conv_bwd(u, Ls[3], u->a2, nullptr, u->g_a2, false, nullptr, false, H / 2, W / 2, m)) ||
-        (rc = conv_bwd(u, Ls[2], u->p1, nullptr, u->g_p1, false, nullptr, false, H / 2, W / 2)) ||
-        (rc = mask_of(u, 1, u->s1, P0, m)))
+        (rc = mask_of(u, 1, u->s1, P0, m)) ||
+        (rc = conv_bwd(u, Ls[2], u->p1, nullptr, u->g_p1, false, nullptr, false, H / 2, W / 2, m, u->g_s1)))
         return rc;
-    UCHK(u, launch_pool_bwd(u->g_p1, n, H, W, C, u->g_s1, true, m, u->stream));
     if ((rc = mask_of(u, 0, u->a1, P0, m)) ||
         (rc = conv_bwd(u, Ls[1], u->a1, nullptr, u->g_a1, false, nullptr, false, H, W, m)) ||
         (rc = conv_bwd(u, Ls[0], u->in0, nullptr, nullptr, false, nullptr, false, H, W)))
