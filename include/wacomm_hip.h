@@ -645,11 +645,13 @@ int wc_unet_gradients_weighted(wc_unet *net, const float *x, const float *t, con
                                int w, float *grad, float *loss);
 int wc_unet_train_step_weighted(wc_unet *net, const float *x, const float *t, const float *weight, int n, int h,
                                 int w, float lr, float *loss);
-/* The gradient and training calls launch the forward and backward passes
- * (about 110 kernels) as one HIP graph, captured the first time a batch
+/* The gradient and training calls launch the forward and the backward pass
+ * (about 55 kernels) as two HIP graphs, captured the first time a batch
  * shape (and loss kind) is seen and replayed afterwards (on = 1, the
  * default), or launch them one by one (on = 0).  Both run the same kernels
- * with the same arguments and give identical results. */
+ * with the same arguments and give identical results.  The batch is staged
+ * through pinned host memory the network owns; the calls return after the
+ * device has finished. */
 int wc_unet_set_graph(wc_unet *net, int on);
 
 /* ---------------------------------------------------------------------------
