@@ -1279,7 +1279,7 @@ hipError_t launch_mask_bf16(const float *dy, const uint16_t *y, long long E, int
     if (cout < 4 || cout > 256 || cout % 4 || 1024 % cout || E % cout) return hipErrorInvalidValue;
     if (!job_ok(side.job[0]) || !job_ok(side.job[1])) return hipErrorInvalidValue;
     const int b = mask_bf16_blocks(E);
-    if ((size_t)b * cout > partCap) return hipErrorInvalidValue;
+    if (part && (size_t)b * cout > partCap) return hipErrorInvalidValue;
     mask_bf16_kernel<<<b, 256, 0, s>>>(dy, y, E / 4, cout, out, part, side);
     *nblocks = b;
     return hipGetLastError();
