@@ -433,7 +433,6 @@ int capture(wc_unet *u, int (*body)(wc_unet *), hipGraphExec_t &out, const char 
         rc = ufail(u, WC_ERR_HIP, "hipGraphInstantiate of the %s failed", what);
     }
     if (g) hipGraphDestroy(g);
-    if (!rc) UCHK(u, hipGraphUpload(out, u->stream));
     return rc;
 }
 
