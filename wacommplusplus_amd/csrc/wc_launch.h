@@ -193,6 +193,15 @@ struct ConvArgs {
     // that resolution (pg is read and, by the general kernel's second launch,
     // also updated)
     const float *pg;
+    // input gradient (mode 1) of a two-input layer whose second input was the
+    // 2x nearest upsampling of the previous layer's output: with uo set, that
+    // part goes out as the previous layer's ReLU-masked gradient uo = bf16(uy >
+    // 0 ? sum of the four pixels it fed : 0), at half the height and width; y2
+    // (stored) and ud (FP32, half size) are scratch where the general kernel
+    // or an odd row count runs
+    uint16_t *uo;
+    const uint16_t *uy;
+    float *ud;
 };
 // A layer's ReLU-masked output gradient in BF16, written by the kernel that
 // produces the gradient: gm[e] = bf16(y[e] > 0 ? g[e] : 0) in place of the

@@ -271,6 +271,28 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs &a, const f32x4 (&a
         __syncthreads();
         const bool first = co0 < a.cy1;
         const int cc = first ? a.cy1 : a.cy2, cl0 = first ? co0 : co0 - a.cy1;
+        if (a.uo && !first) {
+            // the second input was the 2x nearest upsampling of the previous
+            // layer's output (uy, half the height and width): each of its
+            // pixels gets the sum of the four it fed, masked into uo (the
+            // stage's rows pair up: R, h and w are even)
+            const int Wl = W >> 1, ql = (rv >> 1) * Wl;
+            const size_t lp0 = ((size_t)img * (a.h >> 1) + (y0 >> 1)) * Wl;
+            for (int c = tid; c < ql * 8; c += 256) {
+                const int q = c >> 3, part = c & 7;
+                if (co0 + 4 * part >= nout) continue;
+                const int lr = q / Wl, lx = q - lr * Wl, q00 = 2 * lr * W + 2 * lx;
+                const float4 v0 = *(const float4 *)(so + q00 * kCvOutF + 4 * part);
+                const float4 v1 = *(const float4 *)(so + (q00 + 1) * kCvOutF + 4 * part);
+                const float4 v2 = *(const float4 *)(so + (q00 + W) * kCvOutF + 4 * part);
+                const float4 v3 = *(const float4 *)(so + (q00 + W + 1) * kCvOutF + 4 * part);
+                const float4 g = make_float4(v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y,
+                                             v0.z + v1.z + v2.z + v3.z, v0.w + v1.w + v2.w + v3.w);
+                const size_t o = (lp0 + q) * cc + cl0 + 4 * part;
+                *(uint2 *)(a.uo + o) = mask4(g, *(const uint2 *)(a.uy + o));
+            }
+            return;
+        }
         if (a.mo && a.pg) {
             // the input was the 2x2 average pool of the previous layer's
             // output: each of its four pixels gets pg + g / 4, masked into mo
@@ -1114,6 +1136,8 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
     // a masked BF16 output: the input gradient of a single-input layer, stored
     if (a.mo && (a.mode != 1 || a.cy2 || a.acc1 || !a.my || !a.y1)) return hipErrorInvalidValue;
     if (a.pg && !a.mo) return hipErrorInvalidValue;
+    if (a.uo && (a.mode != 1 || !a.cy2 || a.acc2 || !a.uy || !a.y2 || !a.ud || (a.h | a.w) & 1))
+        return hipErrorInvalidValue;
     if (!fp32_input && !a.m1) {
         // tiled path: R rows per workgroup (R * w <= 256), halved while the
         // grid has fewer than 512 workgroups, as long as the stage fits in LDS
@@ -1131,25 +1155,41 @@ hipError_t launch_conv3x3(const ConvArgs &a, bool fp32_input, hipStream_t s) {
         while (R > 1 && (long long)a.n * ((a.h + R - 1) / R) * (nout / 32) < 512) R = (R + 1) / 2;
         if (lds(R, false) <= 64 * 1024 && R * a.w <= 256) {
             const int rowsPerImg = (a.h + R - 1) / R;
+            // the upsampling's backward needs row pairs in a stage; with odd
+            // R it runs as a second launch on the FP32 gradient
+            ConvArgs b = a;
+            const bool upLater = a.uo && (R & 1);
+            if (upLater) b.uo = nullptr;
             const dim3 grid((unsigned)(a.n * rowsPerImg), (unsigned)(nout / 32));
             const int RW = R * a.w;
             if (lds(R, true) > 64 * 1024)
-                conv3x3_tile_kernel<4, false><<<grid, 256, lds(R, false), s>>>(a, R, rowsPerImg);
+                conv3x3_tile_kernel<4, false><<<grid, 256, lds(R, false), s>>>(b, R, rowsPerImg);
             else if (RW <= 64)
-                conv3x3_tile_kernel<1, true><<<grid, 256, lds(R, true), s>>>(a, R, rowsPerImg);
+                conv3x3_tile_kernel<1, true><<<grid, 256, lds(R, true), s>>>(b, R, rowsPerImg);
             else if (RW <= 128)
-                conv3x3_tile_kernel<2, true><<<grid, 256, lds(R, true), s>>>(a, R, rowsPerImg);
+                conv3x3_tile_kernel<2, true><<<grid, 256, lds(R, true), s>>>(b, R, rowsPerImg);
             else
-                conv3x3_tile_kernel<4, true><<<grid, 256, lds(R, true), s>>>(a, R, rowsPerImg);
-            return hipGetLastError();
+                conv3x3_tile_kernel<4, true><<<grid, 256, lds(R, true), s>>>(b, R, rowsPerImg);
+            if (!upLater) return hipGetLastError();
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            return launch_up_bwd((const float *)a.y2, a.n, a.h / 2, a.w / 2, a.cy2, a.ud, MaskOut{a.uo, a.uy}, s);
         }
     }
-    // the general kernel writes the FP32 gradient to y1; a second launch masks it
+    // the general kernel writes the FP32 gradients to y1 and y2; second
+    // launches mask them (through the pooling's or the upsampling's backward)
     const dim3 grid((unsigned)((P + 127) / 128), (unsigned)(nout / 32));
     if (fp32_input)
         conv3x3_kernel<float><<<grid, 256, 0, s>>>(a);
     else
         conv3x3_kernel<uint16_t><<<grid, 256, 0, s>>>(a);
+    if (a.uo) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const hipError_t e2 = launch_up_bwd((const float *)a.y2, a.n, a.h / 2, a.w / 2, a.cy2, a.ud,
+                                            MaskOut{a.uo, a.uy}, s);
+        if (e2 != hipSuccess || !a.mo) return e2;
+    }
     if (a.pg) {                         // the pooling's backward, masked, by a second launch
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

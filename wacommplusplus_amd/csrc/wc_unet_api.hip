@@ -251,14 +251,20 @@ int flush_wgrad(wc_unet *u) {
 // gradient in place of gx1 (a single input, stored).  pooledFrom: the input
 // was the 2x2 average pool of the previous layer's output, whose gradient
 // so far is pooledFrom (FP32); next.gm then receives that gradient plus the
-// pooling's backward of this one, masked (gx1: FP32 scratch).
+// pooling's backward of this one, masked (gx1: FP32 scratch).  up.gm: the
+// second input was the 2x upsampling of an earlier layer's output up.y, whose
+// masked gradient up.gm receives the upsampling's backward of the gx2 part
+// (gx2 and upScratch, FP32 at half size, are scratch).
 int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2, float *gx1, bool acc1, float *gx2,
-             bool acc2, int h, int w, const MaskOut &next = MaskOut{}, const float *pooledFrom = nullptr) {
+             bool acc2, int h, int w, const MaskOut &next = MaskOut{}, const float *pooledFrom = nullptr,
+             const MaskOut &up = MaskOut{}, float *upScratch = nullptr) {
     const long long P = (long long)u->n * h * w;
     const int li = (int)(&L - u->layers.data());
     if ((size_t)P * L.cout > u->gmaskCap[li]) return ufail(u, WC_ERR_INVALID, "layer %d: mask buffer too small", li);
     if (next.gm && (!gx1 || li == 0 || L.c2 || acc1 || u->layers[li - 1].cout != L.c1))
         return ufail(u, WC_ERR_INVALID, "layer %d: a masked input gradient needs one stored input", li);
+    if (up.gm && (!gx1 || !gx2 || !L.c2 || acc2 || !upScratch))
+        return ufail(u, WC_ERR_INVALID, "layer %d: the upsampling's backward needs a second, stored input", li);
     uint16_t *gm = u->gmask[li];
     WgradArgs g{};
     g.x1 = x1; g.x2 = x2; g.c1 = L.c1; g.c2 = L.c2; g.cout = L.cout;
@@ -283,6 +289,7 @@ int conv_bwd(wc_unet *u, const Layer &L, const uint16_t *x1, const uint16_t *x2,
         a.y1 = gx1; a.y2 = gx2; a.cy1 = L.c1; a.cy2 = L.c2; a.acc1 = acc1; a.acc2 = acc2; a.mode = 1;
         a.n = u->n; a.h = h; a.w = w;
         a.mo = next.gm; a.my = next.y; a.pg = pooledFrom;
+        a.uo = up.gm; a.uy = up.y; a.ud = upScratch;
         UCHK(u, launch_conv3x3(a, false, u->stream));
     }
     int rc;
@@ -376,24 +383,23 @@ int backward(wc_unet *u) {
     // decoder, level 0
     if ((rc = mask_of(u, 12, u->d1a, P0, m)) ||
         (rc = conv_bwd(u, Ls[13], u->d1a, nullptr, u->g_d1a, false, nullptr, false, H, W, m)) ||
-        (rc = conv_bwd(u, Ls[12], u->s1, u->u1, u->g_s1, false, u->g_u1, false, H, W)) ||
-        (rc = mask_of(u, 11, u->d2, P1, m)))
+        (rc = mask_of(u, 11, u->d2, P1, m)) ||
+        (rc = conv_bwd(u, Ls[12], u->s1, u->u1, u->g_s1, false, u->g_u1, false, H, W, MaskOut{}, nullptr, m, u->g_d2)))
         return rc;
-    UCHK(u, launch_up_bwd(u->g_u1, n, H / 2, W / 2, 2 * C, u->g_d2, m, u->stream));
     // level 1
     if ((rc = mask_of(u, 10, u->d2a, P1, m)) ||
         (rc = conv_bwd(u, Ls[11], u->d2a, nullptr, u->g_d2a, false, nullptr, false, H / 2, W / 2, m)) ||
-        (rc = conv_bwd(u, Ls[10], u->s2, u->u2, u->g_s2, false, u->g_u2, false, H / 2, W / 2)) ||
-        (rc = mask_of(u, 9, u->d3, P2, m)))
+        (rc = mask_of(u, 9, u->d3, P2, m)) ||
+        (rc = conv_bwd(u, Ls[10], u->s2, u->u2, u->g_s2, false, u->g_u2, false, H / 2, W / 2, MaskOut{}, nullptr, m,
+                       u->g_d3)))
         return rc;
-    UCHK(u, launch_up_bwd(u->g_u2, n, H / 4, W / 4, 4 * C, u->g_d3, m, u->stream));
     // level 2
     if ((rc = mask_of(u, 8, u->d3a, P2, m)) ||
         (rc = conv_bwd(u, Ls[9], u->d3a, nullptr, u->g_d3a, false, nullptr, false, H / 4, W / 4, m)) ||
-        (rc = conv_bwd(u, Ls[8], u->s3, u->u3, u->g_s3, false, u->g_u3, false, H / 4, W / 4)) ||
-        (rc = mask_of(u, 7, u->b4, P3, m)))
+        (rc = mask_of(u, 7, u->b4, P3, m)) ||
+        (rc = conv_bwd(u, Ls[8], u->s3, u->u3, u->g_s3, false, u->g_u3, false, H / 4, W / 4, MaskOut{}, nullptr, m,
+                       u->g_b4)))
         return rc;
-    UCHK(u, launch_up_bwd(u->g_u3, n, H / 8, W / 8, 8 * C, u->g_b4, m, u->stream));
     // bottleneck
     if ((rc = mask_of(u, 6, u->a4, P3, m)) ||
         (rc = conv_bwd(u, Ls[7], u->a4, nullptr, u->g_a4, false, nullptr, false, H / 8, W / 8, m)) ||
