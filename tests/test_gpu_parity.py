@@ -241,6 +241,51 @@ def test_random_windows_all_closures(ctx, seed):
                 assert np.all((ev[~ok] & CLOSURE_EVENTS) != 0), (case, np.where(~ok & (ev == 0))[0][:5])
 
 
+@pytest.mark.parametrize("seed", range(2))
+def test_random_windows_decay_threshold(ctx, seed):
+    """Decay across the survival threshold inside an interval
+    (Particle.cpp:218-223, 797-804): tau0 = 15000 s and survprob = 0.3 put
+    the kill age at 18060 s, and the particles start at ages 0..12000 s, so
+    in the 3 h window health is evaluated on both sides of it, particles die
+    by decay at substeps throughout the window, and some are killed by a
+    closure first.
+    dti = 70 s leaves a shortened last step in every interval (a whole-step
+    and a partial-step diffusion scale, a substep index that is not
+    elapsed / dti of a whole step)."""
+    rng = np.random.default_rng(300 + seed)
+    W = random_window(seed)
+    n = 400
+    p = wc.make_particles(n)
+    p["k"] = rng.uniform(-3.2, 0.2, n)
+    p["j"] = rng.uniform(-0.5, W.eta - 0.5, n)
+    p["i"] = rng.uniform(-0.5, W.xi - 0.5, n)
+    p["time"] = rng.choice([0.0, 0.0, 900.0, 2000.0], n)
+    p["age"] = rng.uniform(0.0, 12000.0, n)
+    kill_age = -15000.0 * np.log(0.3)
+    p["id"] = rng.integers(0, 2**63, n, dtype=np.uint64)
+    for integ in (1, 2):
+        for random in (False, True):
+            case = dict(integrator=integ, random=random, seed=seed)
+            cfg = wc.default_config(integrator=integ, upperClosure=3, lowerClosure=2, horizontalClosure=2,
+                                    random=random, sigma=30.0, dti=70.0, survprob=0.3, tau0=15000.0,
+                                    randomSeed=seed)
+            ref = run_oracle(W, cfg, p, 3)
+            ctx.set_event_tracking(True)
+            dev = run_device(ctx, W, cfg, p, 3)
+            ev = ctx.get_events()
+            ctx.set_event_tracking(False)
+            dead = ref["health"] < 0
+            assert np.count_nonzero(dead & (ref["age"] > kill_age)) >= 20, case     # died by decay
+            if not random:
+                assert_same(dev, ref, tol=0.0), case
+                continue
+            ok = np.all(np.stack([close(dev[f], ref[f], 1e-9) for f in ("k", "j", "i")]), axis=0)
+            ok &= close(dev["health"], ref["health"], 1e-15)
+            print(f"{case}: {ok.mean():.4f} within 1e-9, {dead.mean():.3f} dead", flush=True)
+            assert ok.mean() >= 0.97, (case, float(ok.mean()), np.where(~ok)[0][:5])
+            assert np.all((ev[~ok] & CLOSURE_EVENTS) != 0), (case, np.where(~ok & (ev == 0))[0][:5])
+
+
 @pytest.mark.parametrize("seed", range(3))
 def test_random_windows_backward_and_restart(ctx, seed):
     """Backward tracking from the last record (Particle.cpp:81-125: reversed

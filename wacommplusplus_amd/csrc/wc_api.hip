@@ -215,6 +215,19 @@ bool make_params(const wc_config_data *cfg, double start, double end, int s_w, i
     P.kLowerLimit = -s_w + 2;
     P.direction = cfg->trackingDirection;
     P.ageKill = cfg->survprob > 0 ? -cfg->tau0 * std::log(cfg->survprob) : INFINITY;
+    // The FP64 move may defer exp(-age/tau0) while age < ageLazy: there
+    // -age/tau0 > -L (1 - 1e-6) with L = min(-log(survprob), 700), so the
+    // correctly rounded quotient and a faithful exp give a value above
+    // survprob by a relative 1e-6 L and far above 0; neither survival test
+    // (health < survprob, health > 0) can then depend on the exact value.
+    // Anything else (tau0 <= 0 or not finite, survprob >= 1) never defers.
+    {
+        double L = cfg->survprob > 0 ? -std::log(cfg->survprob) : 700.0;
+        if (L > 700.0) L = 700.0;
+        const bool ok = std::isfinite(cfg->tau0) && cfg->tau0 > 0 && L > 0;
+        P.ageLazy = ok ? cfg->tau0 * L * (1.0 - 1e-6) : -INFINITY;
+    }
+    P.invDti = 1.0 / cfg->dti;
     P.seed = cfg->randomSeed;
     P.intervalKey = (int64_t)std::llround(std_min(start, end));
     P.jibeHourly = cfg->leewayJibeProbabilityHourly;

@@ -22,6 +22,8 @@ struct MoveParams {
     double kLowerLimit;     // -s_w + 2
     double direction;       // trackingDirection as double
     double ageKill;         // exp(-age/tau0) < survprob  <=>  age > ageKill (mixed-precision path)
+    double ageLazy;         // FP64 path: below it exp(-age/tau0) is surely >= survprob and > 0
+    double invDti;          // 1 / dti (an estimate only: checked exactly before use)
     uint64_t seed;
     int64_t intervalKey;    // llround(min(start, end))
     double jibeHourly;      // leewayJibeProbabilityHourly

@@ -212,6 +212,9 @@ __device__ __forceinline__ void velocity(const S &smp, int kI, double kF, int jI
     ww = tril(wt, wb, iF, jF, kF);
 }
 
+// Stand-in for a health value exp(-age/tau0) not yet computed (see ageLazy).
+constexpr double kHealthPending = 2.0;
+
 // Particle::move for one particle (Particle.cpp:71-824); returns the number of
 // live substeps taken.  With Ev, the closures that act are OR-ed into ev
 // (WC_EVENT_* bits); they change nothing else.
@@ -236,6 +239,7 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p, uint32
         else if (P.lower == WC_CLOSURE_REFLECTION) p.k = 2.0 * kLL - p.k;
     }
     const double direction = P.direction;
+    bool pending = false;                                   // p.health holds kHealthPending
     // random-key prefix, constant over the interval
     const uint64_t keyPrefix = mix64(mix64(mix64(P.seed) ^ p.id) ^ (uint64_t)P.intervalKey);
 
@@ -288,8 +292,13 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p, uint32
             S.akt4(kI, jI, iI, alpha, at, ab);
             const float aa = tril(at, ab, iF, jF, kF);
             const double sigmaDepth = P.sigma * (1 - p.k / kLL);
-            const uint64_t substep = (uint64_t)floor(elapsed / P.dti);
-            const double stochasticScale = sqrt(stepDt / P.dti);
+            // floor(elapsed / dti): where elapsed is exactly q dti (the FMA
+            // residual is 0), the quotient is exactly q; otherwise divide
+            double q = rint(elapsed * P.invDti);
+            if (fma(q, P.dti, -elapsed) != 0.0) q = floor(elapsed / P.dti);
+            const uint64_t substep = (uint64_t)q;
+            // sqrt(stepDt / dti) is exactly 1 for a whole step
+            const double stochasticScale = stepDt == P.dti ? 1.0 : sqrt(stepDt / P.dti);
             const uint64_t sk = mix64(keyPrefix ^ substep);
             rxleap = sigmaDepth * keyed_normal(sk, 0) * stochasticScale;
             ryleap = sigmaDepth * keyed_normal(sk, 1) * stochasticScale;
@@ -406,12 +415,24 @@ __device__ unsigned move_one(const MoveParams &P, const Smp &S, State &p, uint32
         }
         if (p.health > 0) {                                 // :797-804
             p.age = p.age + stepDt;
-            p.health = 1.0 * exp(-p.age / P.tau0);
+            // health = exp(-age / tau0).  Below ageLazy the value is surely
+            // >= survprob and > 0, so the stand-in kHealthPending (2, above
+            // any survprob that permits deferring) answers both survival
+            // tests alike, and the value is computed once, after the loop
+            if (p.age < P.ageLazy) {
+                p.health = kHealthPending;
+                pending = true;
+            } else {
+                p.health = 1.0 * exp(-p.age / P.tau0);
+                pending = false;
+            }
             if (P.jibe && tag_type(p.tag) != 0 && jibe_flips(P, keyPrefix, elapsed, stepDt))   // :805-813
                 p.tag = flip_side(p.tag);
         }
         elapsed += stepDt;                                  // :821
     }
+    // a kill after the last deferral overwrote the stand-in with -1
+    if (pending && p.health == kHealthPending) p.health = 1.0 * exp(-p.age / P.tau0);
     return steps;
 }
 
