@@ -958,6 +958,24 @@ __global__ void input_kernel(const float *__restrict__ x, long long P, int hw, i
     }
 }
 
+// The same conversion, one work-item per pixel (c a multiple of 8): its cin
+// values are read (coalesced across the wave), and its c channels written
+// with 16-byte stores.  Bit-identical to input_kernel.
+__global__ void input_pix_kernel(const float *__restrict__ x, long long P, int hw, int cin, int c,
+                                 uint16_t *__restrict__ y, float *__restrict__ x0) {
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < P; p += (long long)gridDim.x * blockDim.x) {
+        const long long ni = p / hw, pix = p - ni * hw;
+        const float *xp = x + ni * cin * hw + pix;
+        uint4 *yp = (uint4 *)(y + p * c);
+        for (int g = 0; g < c; g += 8) {
+            Pack8 o;
+            for (int q = 0; q < 8; ++q) o.h[q] = g + q < cin ? to_bf16(xp[(long long)(g + q) * hw]) : (uint16_t)0;
+            yp[g / 8] = o.u;
+        }
+        if (x0) x0[p] = xp[0];
+    }
+}
+
 // head: out[p] = x[p] + b + sum_c wh[c] * d[p][c]   (FP32; d BF16 with kHead
 // channels, wh = kHead weights then the bias)
 constexpr int kHead = 32;
@@ -1375,7 +1393,10 @@ hipError_t launch_up_bwd(const float *dy, int n, int h, int w, int c, float *dx,
 hipError_t launch_unet_input(const float *x, long long P, int hw, int cin, int c, uint16_t *y, float *x0,
                              hipStream_t s) {
     if (cin < 1 || cin > c) return hipErrorInvalidValue;
-    input_kernel<<<blocks_for(P * c), 256, 0, s>>>(x, P, hw, cin, c, y, x0);
+    if (c % 8 == 0)
+        input_pix_kernel<<<blocks_for(P), 256, 0, s>>>(x, P, hw, cin, c, y, x0);
+    else
+        input_kernel<<<blocks_for(P * c), 256, 0, s>>>(x, P, hw, cin, c, y, x0);
     return hipGetLastError();
 }
 hipError_t launch_head_fwd(const uint16_t *d, const float *x, const float *wh, long long P, float *out,
