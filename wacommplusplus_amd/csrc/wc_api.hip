@@ -283,38 +283,54 @@ int reserve(wc_ctx *c, size_t need) {
     size_t cap = c->cap ? c->cap : 1024;
     while (cap < need) cap *= 2;
     if (cap > (size_t)kMaxParticles) cap = kMaxParticles;
+    // every new buffer is allocated before anything is committed, so a
+    // failure leaves the context as it was
     SoA a{}, b{};
+    uint32_t *blockCounts = nullptr, *perm = nullptr, *perm2 = nullptr, *remap = nullptr;
+    auto drop = [&]() {
+        free_soa(a);
+        free_soa(b);
+        hipFree(blockCounts); hipFree(perm); hipFree(perm2); hipFree(remap);
+    };
     int rc = alloc_soa(c, a, cap);
-    if (rc) { free_soa(a); return rc; }
-    rc = alloc_soa(c, b, cap);
-    if (rc) { free_soa(a); free_soa(b); return rc; }
+    if (!rc) rc = alloc_soa(c, b, cap);
+    if (!rc && (hipMalloc(&blockCounts, ((cap + 255) / 256) * sizeof(uint32_t)) != hipSuccess ||
+                hipMalloc(&perm, cap * sizeof(uint32_t)) != hipSuccess ||
+                hipMalloc(&perm2, cap * sizeof(uint32_t)) != hipSuccess ||
+                hipMalloc(&remap, cap * sizeof(uint32_t)) != hipSuccess))
+        rc = fail(c, WC_ERR_NOMEM, "particle buffers for %zu particles", cap);
+    if (rc) {
+        drop();
+        return rc;
+    }
     if (c->n) {
         const size_t n = c->n;
-        HIPCHK(c, hipMemcpyAsync(a.id, c->s.id, n * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.k, c->s.k, n * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.j, c->s.j, n * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.i, c->s.i, n * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.health, c->s.health, n * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.age, c->s.age, n * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.time, c->s.time, n * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.tag, c->s.tag, n * 4, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(a.ev, c->s.ev, n, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double *src[7] = {c->s.k, c->s.j, c->s.i, c->s.health, c->s.age, c->s.time, nullptr};
+        double *dst[7] = {a.k, a.j, a.i, a.health, a.age, a.time, nullptr};
+        hipError_t e = hipMemcpyAsync(a.id, c->s.id, n * 8, hipMemcpyDeviceToDevice, c->stream);
+        for (int q = 0; q < 6 && e == hipSuccess; ++q)
+            e = hipMemcpyAsync(dst[q], src[q], n * 8, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(a.tag, c->s.tag, n * 4, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(a.ev, c->s.ev, n, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            hipStreamSynchronize(c->stream);
+            drop();
+            return fail(c, WC_ERR_HIP, "copying the population into larger buffers: %s", hipGetErrorString(e));
+        }
     }
     free_soa(c->s);
     free_soa(c->s2);
     hipFree(c->d_blockCounts);
     hipFree(c->d_perm); hipFree(c->d_perm2); hipFree(c->d_remap);
-    c->d_blockCounts = nullptr;
-    c->d_perm = c->d_perm2 = c->d_remap = nullptr;
-    c->permValid = false;
     c->s = a;
     c->s2 = b;
     c->cap = cap;
-    HIPCHK(c, hipMalloc(&c->d_blockCounts, ((cap + 255) / 256) * sizeof(uint32_t)));
-    HIPCHK(c, hipMalloc(&c->d_perm, cap * sizeof(uint32_t)));
-    HIPCHK(c, hipMalloc(&c->d_perm2, cap * sizeof(uint32_t)));
-    HIPCHK(c, hipMalloc(&c->d_remap, cap * sizeof(uint32_t)));
+    c->d_blockCounts = blockCounts;
+    c->d_perm = perm;
+    c->d_perm2 = perm2;
+    c->d_remap = remap;
+    c->permValid = false;
     return WC_OK;
 }
 
@@ -1207,17 +1223,24 @@ int wc_autotune_sort(wc_ctx *c, const wc_config_data *cfg, double start, double 
     };
     unsigned long long steps0 = 0;
     HIPCHK(c, hipMemcpy(&steps0, c->d_steps, sizeof steps0, hipMemcpyDeviceToHost));
+    // the events first: after the population is saved, every path restores it
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        return fail(c, WC_ERR_HIP, "hipEventCreate");
+    }
     int rc = copy(c->s2, c->s);
-    if (rc) return rc;
+    if (rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+    }
     const int tiles[] = {0, 1, 2, 4, 8};
     const int origTile = c->sortTile, origPeriod = c->sortPeriod;
     const bool origTiming = c->timing;
     c->timing = false;
     int keep = origTile;
     float best = 1e30f;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
     for (int trial = -1; trial < 5 && !rc; ++trial) {       // trial -1 warms up
         const int tile = tiles[trial < 0 ? 1 : trial];
         if ((rc = copy(c->s, c->s2))) break;
@@ -1232,7 +1255,10 @@ int wc_autotune_sort(wc_ctx *c, const wc_config_data *cfg, double start, double 
             break;
         }
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(c, WC_ERR_HIP, "hipEventElapsedTime");
+            break;
+        }
         if (trial >= 0 && ms < best) {
             best = ms;
             keep = tile;
@@ -1240,7 +1266,9 @@ int wc_autotune_sort(wc_ctx *c, const wc_config_data *cfg, double start, double 
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (!rc) rc = copy(c->s, c->s2);
+    // the saved population goes back after a failed trial too
+    const int rcRestore = copy(c->s, c->s2);
+    if (!rc) rc = rcRestore;
     c->timing = origTiming;
     c->sortTile = rc ? origTile : keep;
     c->sortPeriod = rc ? origPeriod : 1;
@@ -1771,11 +1799,17 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     auto &E = c->ens;
     const double times[2] = {start, end};
     const int slots[2] = {slot0, slot1};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        return fail(c, WC_ERR_HIP, "hipEventCreate");
+    }
     rc = copy_ens_soa(c, E.save, E.s, E.n);                    // the population to restore
-    if (rc) return rc;
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    if (rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+    }
     // candidates: the plain move over block x particles per work-item x
     // occupancy hint, and the LDS-tiled move (256 x 1) over occupancy hint x
     // chunk; each with every sort tile
@@ -1809,7 +1843,10 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
             break;
         }
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(c, WC_ERR_HIP, "hipEventElapsedTime");
+            break;
+        }
         if (trial >= 0 && ms < best) {
             best = ms;
             keep = t;
@@ -1820,7 +1857,9 @@ int wc_ens_autotune(wc_ctx *c, const wc_config_data *cfg, double start, double e
     for (int q = 0; q < 2 && !rc; ++q)
         if (hipEventRecord(c->slots[slots[q]].released, c->stream) != hipSuccess)
             rc = fail(c, WC_ERR_HIP, "hipEventRecord failed");
-    if (!rc) rc = copy_ens_soa(c, E.s, E.save, E.n);
+    // the saved population goes back after a failed trial too
+    const int rcRestore = copy_ens_soa(c, E.s, E.save, E.n);
+    if (!rc) rc = rcRestore;
     if (rc) {                                   // leave the caller's launch as it was
         E.block = orig.block; E.ppt = orig.ppt; E.occ = orig.occ; E.sortTile = orig.tile;
         E.ldsChunk = orig.chunk; E.sortPeriod = origPeriod;
@@ -1863,14 +1902,21 @@ int wc_ens_run(wc_ctx *c, const wc_config_data *cfg, const double *times, const 
                 return fail(c, WC_ERR_HIP, "capturing the ensemble step failed: %s / %s", hipGetErrorString(le),
                             hipGetErrorString(ce));
             }
+            const hipError_t ie = hipGraphInstantiate(&E.exec, g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                E.exec = nullptr;
+                hipGraphDestroy(g);
+                return fail(c, WC_ERR_HIP, "instantiating the ensemble step failed: %s", hipGetErrorString(ie));
+            }
             E.graph = g;
-            HIPCHK(c, hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0));
             E.graphSurface = wantSurface;
         }
     }
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        return fail(c, WC_ERR_HIP, "hipEventCreate");
+    }
     const auto t0 = std::chrono::steady_clock::now();
     hipError_t he = hipEventRecord(e0, c->stream);
     for (int q = 0; q < n_intervals && he == hipSuccess; ++q)
